@@ -1,0 +1,175 @@
+/*
+ * vardyn.h -- C ABI of libvardyn_hip.so, the MI355X (gfx950) implementation of
+ * the QG 4D-Var hot path of datlas-ocean/VarDyn: the 1.5-layer QG model step,
+ * its tangent-linear and adjoint, the L3 observation operator and the
+ * reductions of the 4D-Var cost.
+ *
+ * Conventions
+ *   - Every entry point returns an int status: VD_OK (0) or a negative
+ *     VD_ERR_* code; vd_last_error() gives a thread-local message.
+ *   - Fields are row-major (ny, nx) arrays in device memory, axis 0 = y,
+ *     contiguous in x, of the plan precision (float for VD_F32, double for
+ *     VD_F64).  Observation values, weights, misfits and cost accumulators
+ *     are double.
+ *   - The caller owns every buffer passed in.  A plan owns its constants
+ *     (sine matrices, 1/Lambda, masks) and a scratch set, allocated by
+ *     vd_qg_plan_create; calls that share a plan must not overlap (the
+ *     scratch is shared), calls on distinct plans may run concurrently.
+ *   - `stream` is a hipStream_t (NULL = default stream).  Work is enqueued
+ *     asynchronously; nothing in the step/obs calls allocates or
+ *     synchronises, so a sequence of calls can be captured in a HIP graph.
+ *
+ * The reference is Python/JAX with no FFI; each entry point names the
+ * reference function whose contract it takes over (file:line in
+ * datlas-ocean/VarDyn).
+ */
+#ifndef VARDYN_H
+#define VARDYN_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VD_VERSION_MAJOR 0
+#define VD_VERSION_MINOR 1
+
+enum {
+  VD_OK = 0,
+  VD_ERR_INVALID_ARG = -1,
+  VD_ERR_SHAPE = -2,
+  VD_ERR_HIP = -3,
+  VD_ERR_UNSUPPORTED = -4,
+  VD_ERR_ALLOC = -5
+};
+
+enum { VD_F32 = 0, VD_F64 = 1 };
+enum { VD_EULER = 0, VD_RK2 = 1, VD_RK3 = 2 };
+
+/* Model parameters.  Mirrors the arguments of Qgm.__init__
+ * (mapping/models/model_qg1l/jqgm.py:113) after its reductions:
+ * dx, dy are the centre-pixel spacings (jqgm.py:126-128), f0 and c the
+ * nanmeans of f and c (jqgm.py:140-151).  Host pointers; copied at create. */
+typedef struct vd_qg_params {
+  int ny, nx;            /* grid shape, >= 8 each */
+  int precision;         /* VD_F32 or VD_F64 */
+  int time_scheme;       /* VD_EULER, VD_RK2 or VD_RK3 (jqgm.py:398-477) */
+  double dx, dy;         /* metres */
+  double dt;             /* seconds */
+  double g, f0, c;       /* gravity, Coriolis, gravity-wave speed */
+  int has_kdiffus;       /* PV diffusion (jqgm.py:362-369) */
+  double kdiffus;
+  const int8_t* mask;    /* (ny, nx) mask 0..3 (jqgm.py:179-217) or NULL for
+                            the land-free ring mask */
+  const double* beta;    /* (ny-2, nx) beta = (f[2:]-f[:-2])/(2 dy)
+                            (jqgm.py:144) or NULL (f plane) */
+  const double* fbathy;  /* (ny, nx) f * bathymetry_PV_term (jqgm.py:353-354)
+                            or NULL */
+  const double* mdt;     /* (ny, nx) mean dynamic topography (jqgm.py:532-555)
+                            or NULL */
+} vd_qg_params;
+
+typedef struct vd_plan vd_plan;
+
+/* Library version. */
+int vd_version(int* major, int* minor);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* vd_last_error(void);
+
+/* Build a plan: Helmholtz eigenvalues, sine matrices, masks, scratch.
+ * Replaces Qgm.__init__ (jqgm.py:113-236). */
+int vd_qg_plan_create(const vd_qg_params* params, vd_plan** plan);
+int vd_qg_plan_destroy(vd_plan* plan);
+
+/* Bytes of trajectory tape for a segment of `nstep` steps (stage states of
+ * every step plus the final state). */
+int vd_qg_tape_bytes(const vd_plan* plan, int nstep, size_t* bytes);
+
+/* Bytes of plan-owned device memory (constants + scratch). */
+int vd_qg_plan_bytes(const vd_plan* plan, size_t* bytes);
+
+/* Forward propagation over `nstep` steps: h1 = M(h0; hb).  Replaces
+ * Qgm.step (jqgm.py:517-557): q0 = h2pv(h0), q is prognostic, h re-diagnosed
+ * by the Helmholtz inversion, NaN written on land in h1.  `tape` may be
+ * NULL; otherwise it receives the stage states (vd_qg_tape_bytes). */
+int vd_qg_step(vd_plan* plan, const void* h0, const void* hb, void* h1, int nstep,
+               void* tape, void* stream);
+
+/* Tangent-linear propagation dh1 = M'(h0; hb) dh0.  Replaces
+ * Qgm.step_tgl (jqgm.py:559-563, jax.jvp of step).  The trajectory is
+ * recomputed alongside; `h1` (may be NULL) receives the forward result. */
+int vd_qg_step_tgl(vd_plan* plan, const void* dh0, const void* h0, const void* hb, void* dh1,
+                   void* h1, int nstep, void* stream);
+
+/* Adjoint propagation adh0 = M'(h0; hb)^T adh1.  Replaces Qgm.step_adj
+ * (jqgm.py:565-569, jax.vjp of step).  If `tape_valid` is 0 the segment is
+ * first recomputed into `tape` (required, vd_qg_tape_bytes); if 1, `tape`
+ * must hold the tape written by vd_qg_step for the same (h0, hb, nstep). */
+int vd_qg_step_adj(vd_plan* plan, const void* adh1, const void* h0, const void* hb, void* adh0,
+                   int nstep, void* tape, int tape_valid, void* stream);
+
+/* Elliptic inversion h = hb + S(S (q - qb)_int S ./ Lambda)S on the
+ * interior, hb on the ring.  Replaces Qgm.pv2h (jqgm.py:295-319) and
+ * inverse_elliptic_dst (jqgm.py:83-91).  qb may be NULL (zero). */
+int vd_qg_pv2h(vd_plan* plan, const void* q, const void* hb, const void* qb, void* h,
+               void* stream);
+
+/* q = h2pv(h, hb) (jqgm.py:264-293). */
+int vd_qg_h2pv(vd_plan* plan, const void* h, const void* hb, void* q, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * L3 observation operator (Obsop_interp_l3_jax, mapping/src/obsop.py:375-678).
+ * One table per observation time:
+ *   row_ptr[n_obs+1], col[nnz] (int32 flat grid index), w[nnz] (double)
+ *   y[n_obs], err[n_obs] (double);
+ * and its transpose for the deterministic adjoint:
+ *   tcol[n_cols] (int32 grid index), tptr[n_cols+1], trow[nnz], tw[nnz].
+ * ------------------------------------------------------------------------- */
+#define VD_REDUCE_SCRATCH 1024 /* doubles of scratch a reduction needs */
+
+typedef struct vd_obs_table {
+  int n_obs, nnz, n_cols;
+  const int32_t* row_ptr;
+  const int32_t* col;
+  const double* w;
+  const double* y;
+  const double* err;
+  const int32_t* tcol;
+  const int32_t* tptr;
+  const int32_t* trow;
+  const double* tw;
+  double* partial;       /* device scratch of VD_REDUCE_SCRATCH doubles */
+} vd_obs_table;
+
+/* misfit = where(isnan(Hx - y), 0, Hx - y) / err for x of the plan precision
+ * (obsop.py:610-628); valid[o] = 1 where Hx - y is not NaN.  If `jo` is not
+ * NULL, sum(misfit^2) * inv_sigma_r2 is ADDED to *jo (device double) with a
+ * deterministic reduction (tools_4Dvar.py:287-292). */
+int vd_obs_misfit(const vd_obs_table* tab, int precision, const void* x, double* misfit,
+                  uint8_t* valid, double* jo, double inv_sigma_r2, void* stream);
+
+/* adx += H^T (valid * cot / err), the vjp of the misfit (obsop.py:630-678),
+ * gathered per grid point in a fixed order (bitwise reproducible). */
+int vd_obs_adjoint(const vd_obs_table* tab, int precision, const double* cot,
+                   const uint8_t* valid, void* adx, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Small device helpers for the 4D-Var driver (tools_4Dvar.py:252-400).
+ * ------------------------------------------------------------------------- */
+/* *out += alpha * sum_i x[i] * y[i] (double, deterministic); `scratch`
+ * holds VD_REDUCE_SCRATCH doubles of device memory. */
+int vd_dot_f64(int64_t n, const double* x, const double* y, double alpha, double* out,
+               double* scratch, void* stream);
+/* y[i] = a * x[i] + b * y[i] in the given precision (x, y same precision). */
+int vd_axpby(int precision, int64_t n, double a, const void* x, double b, void* y, void* stream);
+/* Precision conversion: dst[i] = (dst type) src[i]; NaN -> 0 if nan_to_zero. */
+int vd_convert(int src_precision, const void* src, int dst_precision, void* dst, int64_t n,
+               int nan_to_zero, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VARDYN_H */

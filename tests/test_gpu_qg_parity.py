@@ -1,0 +1,150 @@
+"""HIP QG kernels against the FP64 oracle (through the C ABI).
+
+Bars: FP64 plans agree with the oracle to 1e-10 relative (the DST is a GEMM
+here and an FFT in the oracle, so only rounding differs); FP32 plans to the
+tolerances written in each test (FP32 rounding over a few steps).
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle.qg import QgmOracle
+from vardyn_amd.qg import QgPlan, build_mask
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    m = ~(np.isnan(a) | np.isnan(b))
+    assert np.array_equal(np.isnan(a), np.isnan(b)), "NaN pattern differs"
+    return np.linalg.norm(a[m] - b[m]) / max(np.linalg.norm(b[m]), 1e-300)
+
+
+def make_pair(ny, nx, scheme, precision, land=False, beta=False, kdiff=False, mdt=False,
+              bathy=False, seed=0):
+    rng = np.random.default_rng(seed)
+    dx, dy, dt, c = 10e3, 12e3, 1200.0, 2.7
+    ssh = None
+    if land:
+        ssh = np.zeros((ny, nx))
+        ssh[ny // 2, nx // 3] = np.nan
+        ssh[ny // 2 + 1, nx // 3] = np.nan
+        ssh[3, nx - 6] = np.nan
+    f = 1e-4
+    if beta:
+        f = 1e-4 + 2e-11 * dy * (np.arange(ny)[:, None] - ny / 2) * np.ones((1, nx))
+    K = 40.0 if kdiff else None
+    m = 0.05 * rng.standard_normal((ny, nx)) if mdt else None
+    b = 0.1 * rng.standard_normal((ny, nx)) if bathy else None
+    orc = QgmOracle(dx=dx, dy=dy, dt=dt, c=c, f=f, time_scheme=scheme, shape=(ny, nx), SSH=ssh,
+                    Kdiffus=K, mdt=m, bathymetry_PV_term=b)
+    fb = None if b is None else float(orc.f) * b
+    plan = QgPlan(ny, nx, dx, dy, dt, c, float(orc.f), time_scheme=scheme, precision=precision,
+                  kdiffus=K, mask=build_mask(ny, nx, ssh), beta=orc.beta, fbathy=fb, mdt=m)
+    h0 = 0.1 * rng.standard_normal((ny, nx))
+    if land:
+        h0[np.isnan(ssh)] = np.nan
+    hb = 0.01 * rng.standard_normal((ny, nx))
+    return orc, plan, h0, hb, rng
+
+
+def test_mask_matches_oracle():
+    from oracle.qg import build_mask as omask
+    ssh = np.zeros((20, 17))
+    ssh[5, 6] = np.nan
+    ssh[12, 2] = np.nan
+    assert np.array_equal(build_mask(20, 17, ssh), omask(20, 17, ssh))
+
+
+@pytest.mark.parametrize("precision,tol", [("float64", 1e-12), ("float32", 2e-6)])
+@pytest.mark.parametrize("shape", [(40, 48), (66, 34), (128, 128)])
+def test_pv2h_h2pv(precision, tol, shape):
+    ny, nx = shape
+    orc, plan, h0, hb, rng = make_pair(ny, nx, "rk2", precision)
+    q = rng.standard_normal((ny, nx)) * 1e-5
+    qb = orc.h2pv(hb, hb)
+    h_ref = orc.pv2h(q, hb, qb)
+    h = plan.pv2h(plan.field(q), plan.field(hb), plan.field(qb))
+    torch.cuda.synchronize()
+    assert rel(h.cpu().numpy(), h_ref) < tol
+    q_ref = orc.h2pv(h0, hb)
+    qq = plan.h2pv(plan.field(h0), plan.field(hb))
+    assert rel(qq.cpu().numpy(), q_ref) < tol
+
+
+@pytest.mark.parametrize("scheme", ["Euler", "rk2", "rk3"])
+@pytest.mark.parametrize("precision,tol", [("float64", 1e-10), ("float32", 2e-5)])
+def test_step(scheme, precision, tol):
+    orc, plan, h0, hb, _ = make_pair(40, 48, scheme, precision)
+    ref = orc.step(h0, hb, 6)
+    h1 = plan.step(plan.field(h0), plan.field(hb), 6)
+    assert rel(h1.cpu().numpy(), ref) < tol
+
+
+@pytest.mark.parametrize("opts", [dict(land=True), dict(beta=True), dict(kdiff=True),
+                                  dict(mdt=True), dict(bathy=True),
+                                  dict(land=True, beta=True, kdiff=True, mdt=True, bathy=True)])
+@pytest.mark.parametrize("scheme", ["Euler", "rk2", "rk3"])
+def test_step_options_f64(scheme, opts):
+    orc, plan, h0, hb, _ = make_pair(36, 44, scheme, "float64", **opts)
+    ref = orc.step(h0, hb, 4)
+    h1 = plan.step(plan.field(h0), plan.field(hb), 4)
+    assert rel(h1.cpu().numpy(), ref) < 1e-10
+
+
+@pytest.mark.parametrize("opts", [dict(), dict(land=True, beta=True, kdiff=True, mdt=True,
+                                               bathy=True)])
+@pytest.mark.parametrize("scheme", ["Euler", "rk2", "rk3"])
+def test_tangent_adjoint_f64(scheme, opts):
+    orc, plan, h0, hb, rng = make_pair(36, 44, scheme, "float64", **opts)
+    dh = 0.1 * rng.standard_normal(h0.shape)
+    ay = rng.standard_normal(h0.shape)
+    n = 5
+    tl_ref, _ = orc.step_tgl(dh, h0, hb, n)
+    ad_ref = orc.step_adj(ay, h0, hb, n)
+    tl = plan.step_tgl(plan.field(dh), plan.field(h0), plan.field(hb), n).cpu().numpy()
+    ad = plan.step_adj(plan.field(ay), plan.field(h0), plan.field(hb), n).cpu().numpy()
+    assert rel(tl, tl_ref) < 1e-10
+    assert rel(ad, ad_ref) < 1e-10
+    # dot-product identity on the GPU results themselves
+    h1 = orc.step(h0, hb, n)
+    m = ~np.isnan(h1)
+    p1 = np.sum(tl[m] * ay[m])
+    p2 = np.nansum(dh * ad)
+    assert abs(p1 / p2 - 1) < 1e-11
+
+
+@pytest.mark.parametrize("scheme", ["Euler", "rk2", "rk3"])
+def test_tangent_adjoint_f32(scheme):
+    orc, plan, h0, hb, rng = make_pair(64, 64, scheme, "float32")
+    dh = 0.1 * rng.standard_normal(h0.shape)
+    ay = rng.standard_normal(h0.shape)
+    n = 4
+    tl_ref, _ = orc.step_tgl(dh, h0, hb, n)
+    ad_ref = orc.step_adj(ay, h0, hb, n)
+    tl = plan.step_tgl(plan.field(dh), plan.field(h0), plan.field(hb), n).cpu().numpy()
+    ad = plan.step_adj(plan.field(ay), plan.field(h0), plan.field(hb), n).cpu().numpy()
+    assert rel(tl, tl_ref) < 1e-4
+    assert rel(ad, ad_ref) < 1e-4
+
+
+def test_adjoint_tape_reuse_bitwise():
+    """Adjoint from a tape written by step() equals the recomputing adjoint bit for bit."""
+    orc, plan, h0, hb, rng = make_pair(48, 40, "rk2", "float64")
+    n = 7
+    H0, HB = plan.field(h0), plan.field(hb)
+    ay = plan.field(rng.standard_normal(h0.shape))
+    tape = plan.new_tape(n)
+    plan.step(H0, HB, n, tape=tape)
+    a1 = plan.step_adj(ay, H0, HB, n, tape=tape, tape_valid=True).clone()
+    a2 = plan.step_adj(ay, H0, HB, n)
+    assert torch.equal(a1, a2)
+
+
+def test_nstep_zero():
+    orc, plan, h0, hb, _ = make_pair(32, 32, "rk2", "float64", land=True)
+    ref = orc.step(h0, hb, 0)
+    h1 = plan.step(plan.field(h0), plan.field(hb), 0).cpu().numpy()
+    assert rel(h1, ref) < 1e-14

@@ -1,0 +1,95 @@
+"""ctypes binding of libvardyn_hip.so (the C ABI declared in include/vardyn.h).
+
+The library is loaded from the package directory.  There is no fallback: if
+the library is missing or a call fails, ``VardynError`` is raised.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libvardyn_hip.so"
+
+VD_OK = 0
+VD_F32, VD_F64 = 0, 1
+VD_EULER, VD_RK2, VD_RK3 = 0, 1, 2
+VD_REDUCE_SCRATCH = 1024
+
+SCHEMES = {"Euler": VD_EULER, "rk2": VD_RK2, "rk3": VD_RK3}
+
+
+class VardynError(RuntimeError):
+    """A call into libvardyn_hip failed (status code and message)."""
+
+
+class QgParams(C.Structure):
+    _fields_ = [
+        ("ny", C.c_int), ("nx", C.c_int), ("precision", C.c_int), ("time_scheme", C.c_int),
+        ("dx", C.c_double), ("dy", C.c_double), ("dt", C.c_double),
+        ("g", C.c_double), ("f0", C.c_double), ("c", C.c_double),
+        ("has_kdiffus", C.c_int), ("kdiffus", C.c_double),
+        ("mask", C.c_void_p), ("beta", C.c_void_p), ("fbathy", C.c_void_p), ("mdt", C.c_void_p),
+    ]
+
+
+class ObsTable(C.Structure):
+    _fields_ = [
+        ("n_obs", C.c_int), ("nnz", C.c_int), ("n_cols", C.c_int),
+        ("row_ptr", C.c_void_p), ("col", C.c_void_p), ("w", C.c_void_p),
+        ("y", C.c_void_p), ("err", C.c_void_p),
+        ("tcol", C.c_void_p), ("tptr", C.c_void_p), ("trow", C.c_void_p), ("tw", C.c_void_p),
+        ("partial", C.c_void_p),
+    ]
+
+
+# name -> (restype, argtypes)
+_VP, _I, _D, _SZ, _I64 = C.c_void_p, C.c_int, C.c_double, C.c_size_t, C.c_int64
+SIGNATURES = {
+    "vd_version": (_I, [C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "vd_last_error": (C.c_char_p, []),
+    "vd_qg_plan_create": (_I, [C.POINTER(QgParams), C.POINTER(_VP)]),
+    "vd_qg_plan_destroy": (_I, [_VP]),
+    "vd_qg_tape_bytes": (_I, [_VP, _I, C.POINTER(_SZ)]),
+    "vd_qg_plan_bytes": (_I, [_VP, C.POINTER(_SZ)]),
+    "vd_qg_step": (_I, [_VP, _VP, _VP, _VP, _I, _VP, _VP]),
+    "vd_qg_step_tgl": (_I, [_VP, _VP, _VP, _VP, _VP, _VP, _I, _VP]),
+    "vd_qg_step_adj": (_I, [_VP, _VP, _VP, _VP, _VP, _I, _VP, _I, _VP]),
+    "vd_qg_pv2h": (_I, [_VP, _VP, _VP, _VP, _VP, _VP]),
+    "vd_qg_h2pv": (_I, [_VP, _VP, _VP, _VP, _VP]),
+    "vd_obs_misfit": (_I, [C.POINTER(ObsTable), _I, _VP, _VP, _VP, _VP, _D, _VP]),
+    "vd_obs_adjoint": (_I, [C.POINTER(ObsTable), _I, _VP, _VP, _VP, _VP]),
+    "vd_dot_f64": (_I, [_I64, _VP, _VP, _D, _VP, _VP, _VP]),
+    "vd_axpby": (_I, [_I, _I64, _D, _VP, _D, _VP, _VP]),
+    "vd_convert": (_I, [_I, _VP, _I, _VP, _I64, _I, _VP]),
+}
+
+_lib = None
+
+
+def load():
+    """Load libvardyn_hip.so once; raise VardynError if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise VardynError(f"{LIB_PATH} is not built (run `python -m vardyn_amd.build`)")
+    lib = C.CDLL(str(LIB_PATH))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def last_error() -> str:
+    return load().vd_last_error().decode()
+
+
+def check(rc: int, what: str = "") -> None:
+    if rc != VD_OK:
+        raise VardynError(f"{what} failed with status {rc}: {last_error()}")
+
+
+def call(name: str, *args) -> None:
+    check(getattr(load(), name)(*args), name)

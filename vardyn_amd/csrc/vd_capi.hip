@@ -1,0 +1,448 @@
+// C ABI of libvardyn_hip: plan construction and the step / tangent-linear /
+// adjoint drivers of the QG model (mapping/models/model_qg1l/jqgm.py,
+// datlas-ocean/VarDyn).
+#include "vd_internal.h"
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace vd {
+
+static thread_local std::string g_last_error;
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+// Ring mask of jqgm.py:179-193 (no land).
+static void ring_mask(int ny, int nx, std::vector<uint8_t>& m) {
+  m.assign((size_t)ny * nx, 3);
+  auto at = [&](int i, int j) -> uint8_t& { return m[(size_t)i * nx + j]; };
+  for (int j = 0; j < nx; ++j) { at(0, j) = 1; at(ny - 1, j) = 1; }
+  for (int i = 0; i < ny; ++i) { at(i, 0) = 1; at(i, nx - 1) = 1; }
+  for (int j = 1; j < nx - 1; ++j) { at(1, j) = 2; at(ny - 2, j) = 2; at(ny - 3, j) = 2; }
+  for (int i = 1; i < ny - 1; ++i) { at(i, 1) = 2; at(i, nx - 2) = 2; at(i, nx - 3) = 2; }
+}
+
+template <typename T>
+static int upload(const std::vector<double>& h, void** dptr) {
+  std::vector<T> t(h.size());
+  for (size_t i = 0; i < h.size(); ++i) t[i] = static_cast<T>(h[i]);
+  VD_HIP_TRY(hipMalloc(dptr, t.size() * sizeof(T)));
+  VD_HIP_TRY(hipMemcpy(*dptr, t.data(), t.size() * sizeof(T), hipMemcpyHostToDevice));
+  return VD_OK;
+}
+
+static void release_plan_buffers(Plan& p) {
+  hipFree(p.Sy); hipFree(p.Sx); hipFree(p.lam_inv); hipFree(p.mask);
+  hipFree(p.beta); hipFree(p.fbathy); hipFree(p.mdt);
+  for (auto& g : p.gT) hipFree(g);
+  for (auto& f : p.fld) hipFree(f);
+}
+
+template <typename T>
+static int build_constants(Plan& p, const vd_qg_params& prm) {
+  // Orthonormal DST-I matrices S_m[j][k] = sqrt(2/(m+1)) sin(pi (j+1)(k+1)/(m+1)),
+  // zero-padded to a multiple of the GEMM tile (jqgm.py:75-81).
+  auto sine = [](int m, int M) {
+    std::vector<double> S((size_t)M * M, 0.0);
+    const double a = std::sqrt(2.0 / (m + 1));
+    for (int j = 0; j < m; ++j)
+      for (int k = 0; k < m; ++k) {
+        // reduce (j+1)(k+1) mod 2(m+1) to keep the sine argument small
+        const long r = ((long)(j + 1) * (k + 1)) % (2L * (m + 1));
+        S[(size_t)j * M + k] = a * std::sin(M_PI * (double)r / (m + 1));
+      }
+    return S;
+  };
+  int rc;
+  if ((rc = upload<T>(sine(p.my, p.MY), &p.Sy)) != VD_OK) return rc;
+  if ((rc = upload<T>(sine(p.mx, p.MX), &p.Sx)) != VD_OK) return rc;
+  // 1/Lambda with Lambda = g/f [2(cos(pi(j+1)/(nx-1))-1)/dx^2 + 2(cos(pi(i+1)/(ny-1))-1)/dy^2] - g f/c^2
+  // (jqgm.py:165-172); zero in the padding.
+  std::vector<double> li((size_t)p.MY * p.MX, 0.0);
+  for (int i = 0; i < p.my; ++i)
+    for (int j = 0; j < p.mx; ++j) {
+      const double lap = 2 * (std::cos(M_PI / (p.nx - 1) * (j + 1)) - 1) / (p.dx * p.dx) +
+                         2 * (std::cos(M_PI / (p.ny - 1) * (i + 1)) - 1) / (p.dy * p.dy);
+      const double lam = p.g / p.f0 * lap - p.g * p.f0 / (p.c * p.c);
+      li[(size_t)i * p.MX + j] = 1.0 / lam;
+    }
+  if ((rc = upload<T>(li, &p.lam_inv)) != VD_OK) return rc;
+
+  std::vector<uint8_t> mask;
+  if (prm.mask) {
+    mask.resize(p.nf);
+    for (size_t i = 0; i < p.nf; ++i) {
+      VD_REQUIRE(prm.mask[i] >= 0 && prm.mask[i] <= 3, VD_ERR_INVALID_ARG, "mask values must be 0..3");
+      mask[i] = (uint8_t)prm.mask[i];
+    }
+  } else {
+    ring_mask(p.ny, p.nx, mask);
+  }
+  // The stencils read up to two cells around an active (mask 3) point; the
+  // reference's ring layout keeps active points in rows/columns 2..n-4.
+  for (int i = 0; i < p.ny; ++i)
+    for (int j = 0; j < p.nx; ++j)
+      if (mask[(size_t)i * p.nx + j] == 3)
+        VD_REQUIRE(i >= 2 && i <= p.ny - 4 && j >= 2 && j <= p.nx - 4, VD_ERR_INVALID_ARG,
+                   "mask value 3 (active) is only allowed in rows/columns 2..n-4");
+  VD_HIP_TRY(hipMalloc((void**)&p.mask, p.nf));
+  VD_HIP_TRY(hipMemcpy(p.mask, mask.data(), p.nf, hipMemcpyHostToDevice));
+
+  if (p.has_beta) {
+    std::vector<double> b(p.nf, 0.0);
+    for (int i = 1; i < p.ny - 1; ++i)
+      for (int j = 0; j < p.nx; ++j) b[(size_t)i * p.nx + j] = prm.beta[(size_t)(i - 1) * p.nx + j];
+    if ((rc = upload<T>(b, &p.beta)) != VD_OK) return rc;
+  }
+  if (p.has_bathy) {
+    std::vector<double> b(prm.fbathy, prm.fbathy + p.nf);
+    if ((rc = upload<T>(b, &p.fbathy)) != VD_OK) return rc;
+  }
+  if (p.has_mdt) {
+    std::vector<double> b(prm.mdt, prm.mdt + p.nf);
+    if ((rc = upload<T>(b, &p.mdt)) != VD_OK) return rc;
+  }
+  for (auto& g : p.gT) {
+    VD_HIP_TRY(hipMalloc(&g, p.ng * sizeof(T)));
+    VD_HIP_TRY(hipMemset(g, 0, p.ng * sizeof(T)));
+  }
+  for (int f = 0; f < F_COUNT; ++f) {
+    VD_HIP_TRY(hipMalloc(&p.fld[f], p.nf * sizeof(T)));
+    VD_HIP_TRY(hipMemset(p.fld[f], 0, p.nf * sizeof(T)));
+  }
+  p.nfld = F_COUNT;
+  VD_HIP_TRY(hipDeviceSynchronize());
+  return VD_OK;
+}
+
+template <typename T>
+static inline T* fld(const Plan& p, int id) { return static_cast<T*>(p.fld[id]); }
+
+// Tape slot (step n, stage st): two fields (h, q).
+template <typename T>
+struct Tape {
+  T* base;
+  size_t nf;
+  int S;
+  T* h(int n, int st) const { return base + ((size_t)(n * S + st) * 2) * nf; }
+  T* q(int n, int st) const { return base + ((size_t)(n * S + st) * 2 + 1) * nf; }
+};
+
+// ---------------------------------------------------------------------------
+// Forward: Qgm.step (jqgm.py:517-557) with one_step/euler/rk2/rk3 (:398-503).
+// ---------------------------------------------------------------------------
+template <typename T>
+static int qg_forward(Plan& p, const T* h0, const T* hb, T* h1, int nstep, T* tape_ptr,
+                      hipStream_t s) {
+  const T dt = T(p.dt);
+  Tape<T> tp{tape_ptr, p.nf, p.nstage};
+  T* hbm = fld<T>(p, F_HBM);
+  T* qb = fld<T>(p, F_QB);
+  T* hc = tape_ptr ? tp.h(0, 0) : fld<T>(p, F_HA);
+  T* qc = tape_ptr ? tp.q(0, 0) : fld<T>(p, F_QA);
+  launch_prep<T>(p, h0, hb, hc, hbm, qc, qb, s);
+  for (int n = 0; n < nstep; ++n) {
+    T *hn, *qn, *hs1, *qs1, *hs2, *qs2;
+    if (tape_ptr) {
+      hn = tp.h(n + 1, 0); qn = tp.q(n + 1, 0);
+      hs1 = p.nstage > 1 ? tp.h(n, 1) : nullptr; qs1 = p.nstage > 1 ? tp.q(n, 1) : nullptr;
+      hs2 = p.nstage > 2 ? tp.h(n, 2) : nullptr; qs2 = p.nstage > 2 ? tp.q(n, 2) : nullptr;
+    } else {
+      const bool even = (n % 2) == 0;
+      hn = fld<T>(p, even ? F_HB : F_HA); qn = fld<T>(p, even ? F_QB2 : F_QA);
+      hs1 = fld<T>(p, F_HS1); qs1 = fld<T>(p, F_QS1);
+      hs2 = fld<T>(p, F_HS2); qs2 = fld<T>(p, F_QS2);
+    }
+    if (p.scheme == VD_EULER) {
+      launch_stage<T>(p, hc, qc, qc, nullptr, nullptr, 0, 0, dt, qn, nullptr, s);
+    } else if (p.scheme == VD_RK2) {
+      launch_stage<T>(p, hc, qc, qc, nullptr, nullptr, 0, 0, T(0.5) * dt, qs1, nullptr, s);
+      launch_inversion<T>(p, qs1, qb, hbm, hs1, INV_FWD, s);
+      launch_stage<T>(p, hs1, qs1, qc, nullptr, nullptr, 0, 0, dt, qn, nullptr, s);
+    } else {
+      T* k0 = fld<T>(p, F_K0);
+      T* k1 = fld<T>(p, F_K1);
+      launch_stage<T>(p, hc, qc, qc, nullptr, nullptr, 0, 0, T(0.5) * dt, qs1, k0, s);
+      launch_inversion<T>(p, qs1, qb, hbm, hs1, INV_FWD, s);
+      launch_stage<T>(p, hs1, qs1, qc, k0, nullptr, -dt, 0, T(2) * dt, qs2, k1, s);
+      launch_inversion<T>(p, qs2, qb, hbm, hs2, INV_FWD, s);
+      launch_stage<T>(p, hs2, qs2, qc, k0, k1, dt / T(6), T(4) * dt / T(6), dt / T(6), qn,
+                      nullptr, s);
+    }
+    launch_inversion<T>(p, qn, qb, hbm, hn, INV_FWD, s);
+    hc = hn;
+    qc = qn;
+  }
+  launch_finish<T>(p, hc, h1, s);
+  VD_HIP_TRY(hipGetLastError());
+  return VD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Tangent linear (jvp of step, jqgm.py:559-563): primal and tangent advance
+// together, stage by stage.
+// ---------------------------------------------------------------------------
+template <typename T>
+static int qg_tangent(Plan& p, const T* dh0, const T* h0, const T* hb, T* dh1, T* h1, int nstep,
+                      hipStream_t s) {
+  const T dt = T(p.dt);
+  T* hbm = fld<T>(p, F_HBM);
+  T* qb = fld<T>(p, F_QB);
+  T* hc = fld<T>(p, F_HA);
+  T* qc = fld<T>(p, F_QA);
+  T* dhc = fld<T>(p, F_DH);
+  T* dqc = fld<T>(p, F_DQ);
+  launch_prep<T>(p, h0, hb, hc, hbm, qc, qb, s);
+  launch_prep_tl<T>(p, dh0, hc, dhc, dqc, s);
+  T *hs1 = fld<T>(p, F_HS1), *qs1 = fld<T>(p, F_QS1), *hs2 = fld<T>(p, F_HS2),
+    *qs2 = fld<T>(p, F_QS2);
+  T *dhs1 = fld<T>(p, F_DHS1), *dqs1 = fld<T>(p, F_DQS1), *dhs2 = fld<T>(p, F_DHS2),
+    *dqs2 = fld<T>(p, F_DQS2);
+  for (int n = 0; n < nstep; ++n) {
+    const bool even = (n % 2) == 0;
+    T* hn = fld<T>(p, even ? F_HB : F_HA);
+    T* qn = fld<T>(p, even ? F_QB2 : F_QA);
+    T* dhn = fld<T>(p, even ? F_DH2 : F_DH);
+    T* dqn = fld<T>(p, even ? F_DQ2 : F_DQ);
+    if (p.scheme == VD_EULER) {
+      launch_stage<T>(p, hc, qc, qc, nullptr, nullptr, 0, 0, dt, qn, nullptr, s);
+      launch_stage_tl<T>(p, hc, qc, dhc, dqc, dqc, nullptr, nullptr, 0, 0, dt, dqn, nullptr, s);
+    } else if (p.scheme == VD_RK2) {
+      const T h2 = T(0.5) * dt;
+      launch_stage<T>(p, hc, qc, qc, nullptr, nullptr, 0, 0, h2, qs1, nullptr, s);
+      launch_stage_tl<T>(p, hc, qc, dhc, dqc, dqc, nullptr, nullptr, 0, 0, h2, dqs1, nullptr, s);
+      launch_inversion<T>(p, qs1, qb, hbm, hs1, INV_FWD, s);
+      launch_inversion<T>(p, dqs1, nullptr, nullptr, dhs1, INV_TL, s);
+      launch_stage<T>(p, hs1, qs1, qc, nullptr, nullptr, 0, 0, dt, qn, nullptr, s);
+      launch_stage_tl<T>(p, hs1, qs1, dhs1, dqs1, dqc, nullptr, nullptr, 0, 0, dt, dqn, nullptr,
+                         s);
+    } else {
+      T *k0 = fld<T>(p, F_K0), *k1 = fld<T>(p, F_K1);
+      T *dk0 = fld<T>(p, F_DK0), *dk1 = fld<T>(p, F_DK1);
+      const T h2 = T(0.5) * dt;
+      launch_stage<T>(p, hc, qc, qc, nullptr, nullptr, 0, 0, h2, qs1, k0, s);
+      launch_stage_tl<T>(p, hc, qc, dhc, dqc, dqc, nullptr, nullptr, 0, 0, h2, dqs1, dk0, s);
+      launch_inversion<T>(p, qs1, qb, hbm, hs1, INV_FWD, s);
+      launch_inversion<T>(p, dqs1, nullptr, nullptr, dhs1, INV_TL, s);
+      launch_stage<T>(p, hs1, qs1, qc, k0, nullptr, -dt, 0, T(2) * dt, qs2, k1, s);
+      launch_stage_tl<T>(p, hs1, qs1, dhs1, dqs1, dqc, dk0, nullptr, -dt, 0, T(2) * dt, dqs2, dk1,
+                         s);
+      launch_inversion<T>(p, qs2, qb, hbm, hs2, INV_FWD, s);
+      launch_inversion<T>(p, dqs2, nullptr, nullptr, dhs2, INV_TL, s);
+      const T c1 = dt / T(6), c2 = T(4) * dt / T(6);
+      launch_stage<T>(p, hs2, qs2, qc, k0, k1, c1, c2, c1, qn, nullptr, s);
+      launch_stage_tl<T>(p, hs2, qs2, dhs2, dqs2, dqc, dk0, dk1, c1, c2, c1, dqn, nullptr, s);
+    }
+    launch_inversion<T>(p, qn, qb, hbm, hn, INV_FWD, s);
+    launch_inversion<T>(p, dqn, nullptr, nullptr, dhn, INV_TL, s);
+    hc = hn; qc = qn; dhc = dhn; dqc = dqn;
+  }
+  launch_finish_tl<T>(p, dhc, dh1, s);
+  if (h1) launch_finish<T>(p, hc, h1, s);
+  VD_HIP_TRY(hipGetLastError());
+  return VD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint (vjp of step, jqgm.py:565-569): reverse sweep over the tape.
+// ---------------------------------------------------------------------------
+template <typename T>
+static int qg_adjoint(Plan& p, const T* adh1, const T* h0, const T* hb, T* adh0, int nstep,
+                      T* tape_ptr, int tape_valid, hipStream_t s) {
+  int rc;
+  if (!tape_valid) {
+    if ((rc = qg_forward<T>(p, h0, hb, fld<T>(p, F_HOUT), nstep, tape_ptr, s)) != VD_OK) return rc;
+  }
+  const T dt = T(p.dt);
+  Tape<T> tp{tape_ptr, p.nf, p.nstage};
+  T* lh = fld<T>(p, F_LH);
+  T* lq = fld<T>(p, F_LQ);
+  T* lq1 = fld<T>(p, F_LQ1);
+  T* lq2 = fld<T>(p, F_LQ2);
+  T* lhs = fld<T>(p, F_LHS);
+  T* W[7];
+  for (int k = 0; k < 7; ++k) W[k] = fld<T>(p, F_W0 + k);
+  launch_adj_init<T>(p, adh1, lh, lq, s);
+  for (int n = nstep - 1; n >= 0; --n) {
+    // h_{n+1} = pv2h(q_{n+1}): lq += P lh
+    launch_inversion<T>(p, lh, nullptr, nullptr, lq, INV_ACC, s);
+    if (p.scheme == VD_EULER) {
+      launch_stage_adj<T>(p, tp.h(n, 0), tp.q(n, 0), lq, nullptr, nullptr, dt, 0, 0, lq, nullptr,
+                          nullptr, lq, lh, W, s);
+    } else if (p.scheme == VD_RK2) {
+      launch_stage_adj<T>(p, tp.h(n, 1), tp.q(n, 1), lq, nullptr, nullptr, dt, 0, 0, nullptr,
+                          nullptr, nullptr, lq1, lhs, W, s);
+      launch_inversion<T>(p, lhs, nullptr, nullptr, lq1, INV_ACC, s);
+      launch_stage_adj<T>(p, tp.h(n, 0), tp.q(n, 0), lq1, nullptr, nullptr, T(0.5) * dt, 0, 0, lq,
+                          lq1, nullptr, lq, lh, W, s);
+    } else {
+      launch_stage_adj<T>(p, tp.h(n, 2), tp.q(n, 2), lq, nullptr, nullptr, dt / T(6), 0, 0,
+                          nullptr, nullptr, nullptr, lq2, lhs, W, s);
+      launch_inversion<T>(p, lhs, nullptr, nullptr, lq2, INV_ACC, s);
+      launch_stage_adj<T>(p, tp.h(n, 1), tp.q(n, 1), lq, lq2, nullptr, T(4) * dt / T(6),
+                          T(2) * dt, 0, nullptr, nullptr, nullptr, lq1, lhs, W, s);
+      launch_inversion<T>(p, lhs, nullptr, nullptr, lq1, INV_ACC, s);
+      launch_stage_adj<T>(p, tp.h(n, 0), tp.q(n, 0), lq, lq2, lq1, dt / T(6), -dt, T(0.5) * dt,
+                          lq, lq2, lq1, lq, lh, W, s);
+    }
+  }
+  launch_adj_final<T>(p, tp.h(0, 0), lh, lq, adh0, s);
+  VD_HIP_TRY(hipGetLastError());
+  return VD_OK;
+}
+
+}  // namespace vd
+
+using namespace vd;
+
+struct vd_plan {
+  Plan p;
+};
+
+extern "C" {
+
+int vd_version(int* major, int* minor) {
+  if (major) *major = VD_VERSION_MAJOR;
+  if (minor) *minor = VD_VERSION_MINOR;
+  return VD_OK;
+}
+
+const char* vd_last_error(void) { return g_last_error.c_str(); }
+
+int vd_qg_plan_create(const vd_qg_params* prm, vd_plan** out) {
+  VD_REQUIRE(prm && out, VD_ERR_INVALID_ARG, "null argument");
+  VD_REQUIRE(prm->ny >= 8 && prm->nx >= 8, VD_ERR_SHAPE, "grid must be at least 8x8");
+  VD_REQUIRE(prm->precision == VD_F32 || prm->precision == VD_F64, VD_ERR_INVALID_ARG,
+             "precision must be VD_F32 or VD_F64");
+  VD_REQUIRE(prm->time_scheme >= VD_EULER && prm->time_scheme <= VD_RK3, VD_ERR_INVALID_ARG,
+             "time_scheme must be VD_EULER, VD_RK2 or VD_RK3");
+  VD_REQUIRE(prm->dx > 0 && prm->dy > 0 && prm->f0 != 0 && prm->c > 0 && prm->g > 0,
+             VD_ERR_INVALID_ARG, "dx, dy, c, g must be positive and f0 non-zero");
+  vd_plan* h = new vd_plan();
+  Plan& p = h->p;
+  p.ny = prm->ny; p.nx = prm->nx;
+  p.my = p.ny - 2; p.mx = p.nx - 2;
+  p.MY = round_up(p.my, kTile); p.MX = round_up(p.mx, kTile);
+  p.prec = prm->precision;
+  p.scheme = prm->time_scheme;
+  p.nstage = p.scheme == VD_EULER ? 1 : (p.scheme == VD_RK2 ? 2 : 3);
+  p.dx = prm->dx; p.dy = prm->dy; p.dt = prm->dt;
+  p.g = prm->g; p.f0 = prm->f0; p.c = prm->c;
+  p.has_kdiff = prm->has_kdiffus; p.kdiffus = prm->kdiffus;
+  p.has_beta = prm->beta != nullptr;
+  p.has_bathy = prm->fbathy != nullptr;
+  p.has_mdt = prm->mdt != nullptr;
+  p.es = p.prec == VD_F64 ? 8 : 4;
+  p.nf = (size_t)p.ny * p.nx;
+  p.ng = (size_t)p.MY * p.MX;
+  if (hipGetDevice(&p.device) != hipSuccess) {
+    set_error("hipGetDevice failed");
+    delete h;
+    return VD_ERR_HIP;
+  }
+  int rc = p.prec == VD_F64 ? build_constants<double>(p, *prm) : build_constants<float>(p, *prm);
+  if (rc != VD_OK) {
+    const std::string msg = g_last_error;
+    release_plan_buffers(p);
+    delete h;
+    set_error(msg);
+    return rc;
+  }
+  *out = h;
+  return VD_OK;
+}
+
+int vd_qg_plan_destroy(vd_plan* h) {
+  if (!h) return VD_OK;
+  release_plan_buffers(h->p);
+  delete h;
+  return VD_OK;
+}
+
+int vd_qg_tape_bytes(const vd_plan* h, int nstep, size_t* bytes) {
+  VD_REQUIRE(h && bytes, VD_ERR_INVALID_ARG, "null argument");
+  VD_REQUIRE(nstep >= 0, VD_ERR_INVALID_ARG, "nstep must be >= 0");
+  const Plan& p = h->p;
+  *bytes = ((size_t)nstep * p.nstage + 1) * 2 * p.nf * p.es;
+  return VD_OK;
+}
+
+int vd_qg_plan_bytes(const vd_plan* h, size_t* bytes) {
+  VD_REQUIRE(h && bytes, VD_ERR_INVALID_ARG, "null argument");
+  const Plan& p = h->p;
+  size_t b = ((size_t)p.MY * p.MY + (size_t)p.MX * p.MX + p.ng * 4) * p.es + p.nf +
+             (size_t)F_COUNT * p.nf * p.es;
+  if (p.has_beta) b += p.nf * p.es;
+  if (p.has_bathy) b += p.nf * p.es;
+  if (p.has_mdt) b += p.nf * p.es;
+  *bytes = b;
+  return VD_OK;
+}
+
+int vd_qg_step(vd_plan* h, const void* h0, const void* hb, void* h1, int nstep, void* tape,
+               void* stream) {
+  VD_REQUIRE(h && h0 && hb && h1, VD_ERR_INVALID_ARG, "null argument");
+  VD_REQUIRE(nstep >= 0, VD_ERR_INVALID_ARG, "nstep must be >= 0");
+  Plan& p = h->p;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (p.prec == VD_F64)
+    return qg_forward<double>(p, (const double*)h0, (const double*)hb, (double*)h1, nstep,
+                              (double*)tape, s);
+  return qg_forward<float>(p, (const float*)h0, (const float*)hb, (float*)h1, nstep, (float*)tape,
+                           s);
+}
+
+int vd_qg_step_tgl(vd_plan* h, const void* dh0, const void* h0, const void* hb, void* dh1,
+                   void* h1, int nstep, void* stream) {
+  VD_REQUIRE(h && dh0 && h0 && hb && dh1, VD_ERR_INVALID_ARG, "null argument");
+  VD_REQUIRE(nstep >= 0, VD_ERR_INVALID_ARG, "nstep must be >= 0");
+  Plan& p = h->p;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (p.prec == VD_F64)
+    return qg_tangent<double>(p, (const double*)dh0, (const double*)h0, (const double*)hb,
+                              (double*)dh1, (double*)h1, nstep, s);
+  return qg_tangent<float>(p, (const float*)dh0, (const float*)h0, (const float*)hb, (float*)dh1,
+                           (float*)h1, nstep, s);
+}
+
+int vd_qg_step_adj(vd_plan* h, const void* adh1, const void* h0, const void* hb, void* adh0,
+                   int nstep, void* tape, int tape_valid, void* stream) {
+  VD_REQUIRE(h && adh1 && h0 && hb && adh0 && tape, VD_ERR_INVALID_ARG, "null argument");
+  VD_REQUIRE(nstep >= 0, VD_ERR_INVALID_ARG, "nstep must be >= 0");
+  Plan& p = h->p;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (p.prec == VD_F64)
+    return qg_adjoint<double>(p, (const double*)adh1, (const double*)h0, (const double*)hb,
+                              (double*)adh0, nstep, (double*)tape, tape_valid, s);
+  return qg_adjoint<float>(p, (const float*)adh1, (const float*)h0, (const float*)hb,
+                           (float*)adh0, nstep, (float*)tape, tape_valid, s);
+}
+
+int vd_qg_pv2h(vd_plan* h, const void* q, const void* hb, const void* qb, void* hout,
+               void* stream) {
+  VD_REQUIRE(h && q && hout, VD_ERR_INVALID_ARG, "null argument");
+  Plan& p = h->p;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (p.prec == VD_F64)
+    launch_inversion<double>(p, (const double*)q, (const double*)qb, (const double*)hb,
+                             (double*)hout, INV_FWD, s);
+  else
+    launch_inversion<float>(p, (const float*)q, (const float*)qb, (const float*)hb, (float*)hout,
+                            INV_FWD, s);
+  VD_HIP_TRY(hipGetLastError());
+  return VD_OK;
+}
+
+int vd_qg_h2pv(vd_plan* h, const void* hh, const void* hb, void* q, void* stream) {
+  VD_REQUIRE(h && hh && hb && q, VD_ERR_INVALID_ARG, "null argument");
+  Plan& p = h->p;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (p.prec == VD_F64)
+    launch_h2pv<double>(p, (const double*)hh, (const double*)hb, (double*)q, s);
+  else
+    launch_h2pv<float>(p, (const float*)hh, (const float*)hb, (float*)q, s);
+  VD_HIP_TRY(hipGetLastError());
+  return VD_OK;
+}
+
+}  // extern "C"

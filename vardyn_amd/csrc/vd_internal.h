@@ -1,0 +1,142 @@
+// Internal declarations shared by the HIP translation units of libvardyn_hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "../../include/vardyn.h"
+
+namespace vd {
+
+void set_error(const std::string& msg);
+
+#define VD_HIP_TRY(expr)                                                        \
+  do {                                                                          \
+    hipError_t e_ = (expr);                                                     \
+    if (e_ != hipSuccess) {                                                     \
+      ::vd::set_error(std::string(#expr " failed: ") + hipGetErrorString(e_));  \
+      return VD_ERR_HIP;                                                        \
+    }                                                                           \
+  } while (0)
+
+#define VD_REQUIRE(cond, code, msg)  \
+  do {                               \
+    if (!(cond)) {                   \
+      ::vd::set_error(msg);          \
+      return (code);                 \
+    }                                \
+  } while (0)
+
+constexpr int kTile = 32;  // GEMM output tile (rows and columns) and K chunk
+
+inline int round_up(int v, int m) { return (v + m - 1) / m * m; }
+
+// Physical constants of one QG problem, passed by value to every stencil kernel.
+template <typename T>
+struct Phys {
+  int ny, nx;
+  T cu;       // -g/f/(4 dy): u = cu * (4-point difference of h)
+  T cv;       //  g/f/(4 dx)
+  T inv6dx, inv6dy;
+  T kx, ky;   // Kdiffus/dx^2, Kdiffus/dy^2 (0 when absent)
+  T gfc2;     // g f / c^2
+  T gf_dx2, gf_dy2;  // g/f/dx^2, g/f/dy^2
+  const uint8_t* mask;  // 0 land, 1 ring/coast, 2 next to ring/coast, 3 active
+  const T* beta;        // (ny, nx) beta at T points, or nullptr (f plane)
+  const T* fbathy;      // (ny, nx) f * bathymetry PV term, or nullptr
+  int has_kdiff;
+};
+
+// Device-side state of one plan.  Buffers are typed by the plan precision.
+struct Plan {
+  int ny, nx, my, mx, MY, MX;
+  int prec;     // VD_F32 / VD_F64
+  int scheme;   // VD_EULER / VD_RK2 / VD_RK3
+  int nstage;
+  double dx, dy, dt, g, f0, c, kdiffus;
+  int has_beta, has_kdiff, has_bathy, has_mdt;
+  int device;
+  size_t es;    // element size
+  size_t nf;    // ny*nx
+  size_t ng;    // MY*MX
+
+  // constants
+  void* Sy = nullptr;
+  void* Sx = nullptr;
+  void* lam_inv = nullptr;
+  uint8_t* mask = nullptr;
+  void* beta = nullptr;
+  void* fbathy = nullptr;
+  void* mdt = nullptr;
+
+  // scratch (one call at a time per plan)
+  void* gT[3] = {nullptr, nullptr, nullptr};
+  void* fld[40] = {};
+  int nfld = 0;
+};
+
+// Named scratch fields inside Plan::fld.
+enum Field {
+  F_HBM = 0, F_QB,
+  F_HA, F_QA, F_HB, F_QB2, F_HS1, F_QS1, F_HS2, F_QS2, F_K0, F_K1,
+  F_DH, F_DQ, F_DH2, F_DQ2, F_DHS1, F_DQS1, F_DHS2, F_DQS2, F_DK0, F_DK1,
+  F_LH, F_LQ, F_LQ1, F_LQ2, F_LHS,
+  F_W0, F_W1, F_W2, F_W3, F_W4, F_W5, F_W6,
+  F_HOUT,
+  F_COUNT
+};
+
+// Launchers implemented in vd_kernels.hip -----------------------------------
+template <typename T> Phys<T> make_phys(const Plan& p);
+
+// Helmholtz inversion: out = hb + S_y (S_y (F - Fsub) S_x ./ Lambda) S_x, with
+// mode selecting the epilogue (see InvMode).
+enum InvMode { INV_FWD = 0, INV_TL = 1, INV_ACC = 2 };
+template <typename T>
+void launch_inversion(const Plan& p, const T* F, const T* Fsub, const T* hb, T* out, int mode,
+                      hipStream_t s);
+
+// Prepare a segment: hcur = h0 + mdt, hbm = hb + mdt, q0 = h2pv(hcur, hbm), qb = h2pv(hbm, hbm).
+template <typename T>
+void launch_prep(const Plan& p, const T* h0, const T* hb, T* hcur, T* hbm, T* q0, T* qb,
+                 hipStream_t s);
+template <typename T>
+void launch_h2pv(const Plan& p, const T* h, const T* hb, T* q, hipStream_t s);
+// h1 = NaN on land, else h - mdt.
+template <typename T>
+void launch_finish(const Plan& p, const T* h, T* h1, hipStream_t s);
+
+// Forward stage: k = rhs(h2uv(h), q); out = qbase + sum_i c_i k_i where the
+// k_i are (kprev0, kprev1, k) with coefficients (c0, c1, c2).  Optionally stores k.
+template <typename T>
+void launch_stage(const Plan& p, const T* h, const T* q, const T* qbase, const T* kprev0,
+                  const T* kprev1, T c0, T c1, T c2, T* qout, T* kstore, hipStream_t s);
+
+// Tangent-linear stage around (h, q).
+template <typename T>
+void launch_stage_tl(const Plan& p, const T* h, const T* q, const T* dh, const T* dq,
+                     const T* dqbase, const T* dkprev0, const T* dkprev1, T c0, T c1, T c2,
+                     T* dqout, T* dkstore, hipStream_t s);
+template <typename T>
+void launch_prep_tl(const Plan& p, const T* dh0, const T* hcur, T* dhcur, T* dq0, hipStream_t s);
+template <typename T>
+void launch_finish_tl(const Plan& p, const T* dh, T* dh1, hipStream_t s);
+
+// Adjoint stage around (h, q): ak = mask * (a0 s0 + a1 s1 + a2 s2);
+// lq_out = acc0 + acc1 + acc2 + adv^T ak (+ diffusion^T); lh_out = h2uv^T(...).
+template <typename T>
+void launch_stage_adj(const Plan& p, const T* h, const T* q, const T* s0, const T* s1,
+                      const T* s2, T a0, T a1, T a2, const T* acc0, const T* acc1,
+                      const T* acc2, T* lq_out, T* lh_out, T* W[7], hipStream_t s);
+// lh = adh1 with zeros on land; lq = 0.
+template <typename T>
+void launch_adj_init(const Plan& p, const T* adh1, T* lh, T* lq, hipStream_t s);
+// adh0 = lh + h2pv^T(lq) at the NaN pattern of hcur.
+template <typename T>
+void launch_adj_final(const Plan& p, const T* hcur, const T* lh, const T* lq, T* adh0,
+                      hipStream_t s);
+
+}  // namespace vd

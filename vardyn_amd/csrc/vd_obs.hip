@@ -1,0 +1,229 @@
+// L3 observation operator and FP64 reductions for the 4D-Var cost.
+//
+// Reference: Obsop_interp_l3_jax (mapping/src/obsop.py:375-678) and
+// Variational.cost_and_grad (mapping/src/tools_4Dvar.py:252-400),
+// datlas-ocean/VarDyn.
+#include "vd_internal.h"
+
+namespace vd {
+
+constexpr int kRedThreads = 256;
+constexpr int kRedBlocks = VD_REDUCE_SCRATCH;
+
+// Deterministic block sum: every block writes its partial to scratch[block];
+// a single-block pass adds them in index order.
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  const int tid = threadIdx.x;
+  sh[tid] = v;
+  __syncthreads();
+  for (int s = kRedThreads / 2; s > 0; s >>= 1) {
+    if (tid < s) sh[tid] += sh[tid + s];
+    __syncthreads();
+  }
+  return sh[0];
+}
+
+__global__ __launch_bounds__(kRedThreads) void k_sum_partials(const double* __restrict__ part,
+                                                              int n, double alpha,
+                                                              double* __restrict__ out) {
+  __shared__ double sh[kRedThreads];
+  double v = 0;
+  for (int i = threadIdx.x; i < n; i += kRedThreads) v += part[i];
+  const double s = block_sum(v, sh);
+  if (threadIdx.x == 0) *out += alpha * s;
+}
+
+static int red_blocks(int64_t n) {
+  int64_t b = (n + kRedThreads - 1) / kRedThreads;
+  if (b < 1) b = 1;
+  if (b > kRedBlocks) b = kRedBlocks;
+  return (int)b;
+}
+
+// misfit[o] = (sum_e w[e] x[col[e]] - y[o]) / err[o], NaN -> 0  (obsop.py:610-628)
+template <typename TX>
+__global__ __launch_bounds__(kRedThreads) void k_obs_misfit(
+    int n_obs, const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
+    const double* __restrict__ w, const double* __restrict__ y, const double* __restrict__ err,
+    const TX* __restrict__ x, double* __restrict__ misfit, uint8_t* __restrict__ valid,
+    double* __restrict__ part) {
+  __shared__ double sh[kRedThreads];
+  double m2 = 0;
+  for (int o = blockIdx.x * kRedThreads + threadIdx.x; o < n_obs; o += gridDim.x * kRedThreads) {
+    double hx = 0;
+    const int e1 = row_ptr[o + 1];
+    for (int e = row_ptr[o]; e < e1; ++e) hx += w[e] * (double)x[col[e]];
+    const double d = hx - y[o];
+    const bool ok = !isnan(d);
+    double inverr = 1.0 / err[o];
+    if (isnan(inverr)) inverr = 0.0;
+    const double m = inverr * (ok ? d : 0.0);
+    misfit[o] = m;
+    valid[o] = ok ? 1 : 0;
+    m2 += m * m;
+  }
+  if (part) {
+    const double s = block_sum(m2, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+  }
+}
+
+// adx[tcol[c]] += sum_{e in c} tw[e] * valid[trow[e]] * cot[trow[e]] / err[trow[e]]
+// (the vjp of the misfit, obsop.py:630-654), one thread per grid point.
+template <typename TX>
+__global__ void k_obs_adjoint(int n_cols, const int32_t* __restrict__ tcol,
+                              const int32_t* __restrict__ tptr, const int32_t* __restrict__ trow,
+                              const double* __restrict__ tw, const double* __restrict__ err,
+                              const double* __restrict__ cot, const uint8_t* __restrict__ valid,
+                              TX* __restrict__ adx) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n_cols) return;
+  double s = 0;
+  const int e1 = tptr[c + 1];
+  for (int e = tptr[c]; e < e1; ++e) {
+    const int o = trow[e];
+    if (!valid[o]) continue;
+    double inverr = 1.0 / err[o];
+    if (isnan(inverr)) inverr = 0.0;
+    s += tw[e] * (inverr * cot[o]);
+  }
+  const int g = tcol[c];
+  adx[g] = adx[g] + (TX)s;
+}
+
+__global__ __launch_bounds__(kRedThreads) void k_dot(int64_t n, const double* __restrict__ x,
+                                                     const double* __restrict__ y,
+                                                     double* __restrict__ part) {
+  __shared__ double sh[kRedThreads];
+  double v = 0;
+  for (int64_t i = (int64_t)blockIdx.x * kRedThreads + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kRedThreads)
+    v += x[i] * y[i];
+  const double s = block_sum(v, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+template <typename T>
+__global__ void k_axpby(int64_t n, T a, const T* __restrict__ x, T b, T* __restrict__ y) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = a * x[i] + b * y[i];
+}
+
+template <typename TS, typename TD>
+__global__ void k_convert(int64_t n, const TS* __restrict__ src, TD* __restrict__ dst, int nz) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    TS v = src[i];
+    if (nz && isnan(v)) v = TS(0);
+    dst[i] = (TD)v;
+  }
+}
+
+static int grid1d(int64_t n) {
+  int64_t b = (n + 255) / 256;
+  if (b < 1) b = 1;
+  if (b > 65535) b = 65535;
+  return (int)b;
+}
+
+}  // namespace vd
+
+using namespace vd;
+
+extern "C" {
+
+int vd_obs_misfit(const vd_obs_table* t, int precision, const void* x, double* misfit,
+                  uint8_t* valid, double* jo, double inv_sigma_r2, void* stream) {
+  VD_REQUIRE(t && x && misfit && valid, VD_ERR_INVALID_ARG, "null argument");
+  VD_REQUIRE(precision == VD_F32 || precision == VD_F64, VD_ERR_INVALID_ARG, "bad precision");
+  VD_REQUIRE(!jo || t->partial, VD_ERR_INVALID_ARG, "jo requires table scratch (partial)");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (t->n_obs <= 0) return VD_OK;
+  const int nb = red_blocks(t->n_obs);
+  double* part = jo ? t->partial : nullptr;
+  if (precision == VD_F64)
+    hipLaunchKernelGGL(k_obs_misfit<double>, dim3(nb), dim3(kRedThreads), 0, s, t->n_obs,
+                       t->row_ptr, t->col, t->w, t->y, t->err, (const double*)x, misfit, valid,
+                       part);
+  else
+    hipLaunchKernelGGL(k_obs_misfit<float>, dim3(nb), dim3(kRedThreads), 0, s, t->n_obs,
+                       t->row_ptr, t->col, t->w, t->y, t->err, (const float*)x, misfit, valid,
+                       part);
+  if (jo)
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kRedThreads), 0, s, t->partial, nb,
+                       inv_sigma_r2, jo);
+  VD_HIP_TRY(hipGetLastError());
+  return VD_OK;
+}
+
+int vd_obs_adjoint(const vd_obs_table* t, int precision, const double* cot, const uint8_t* valid,
+                   void* adx, void* stream) {
+  VD_REQUIRE(t && cot && valid && adx, VD_ERR_INVALID_ARG, "null argument");
+  VD_REQUIRE(precision == VD_F32 || precision == VD_F64, VD_ERR_INVALID_ARG, "bad precision");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (t->n_cols <= 0) return VD_OK;
+  const int nb = (t->n_cols + 255) / 256;
+  if (precision == VD_F64)
+    hipLaunchKernelGGL(k_obs_adjoint<double>, dim3(nb), dim3(256), 0, s, t->n_cols, t->tcol,
+                       t->tptr, t->trow, t->tw, t->err, cot, valid, (double*)adx);
+  else
+    hipLaunchKernelGGL(k_obs_adjoint<float>, dim3(nb), dim3(256), 0, s, t->n_cols, t->tcol,
+                       t->tptr, t->trow, t->tw, t->err, cot, valid, (float*)adx);
+  VD_HIP_TRY(hipGetLastError());
+  return VD_OK;
+}
+
+int vd_dot_f64(int64_t n, const double* x, const double* y, double alpha, double* out,
+               double* scratch, void* stream) {
+  VD_REQUIRE(x && y && out && scratch, VD_ERR_INVALID_ARG, "null argument");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (n <= 0) return VD_OK;
+  const int nb = red_blocks(n);
+  hipLaunchKernelGGL(k_dot, dim3(nb), dim3(kRedThreads), 0, s, n, x, y, scratch);
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kRedThreads), 0, s, scratch, nb, alpha, out);
+  VD_HIP_TRY(hipGetLastError());
+  return VD_OK;
+}
+
+int vd_axpby(int precision, int64_t n, double a, const void* x, double b, void* y, void* stream) {
+  VD_REQUIRE(x && y, VD_ERR_INVALID_ARG, "null argument");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (n <= 0) return VD_OK;
+  if (precision == VD_F64)
+    hipLaunchKernelGGL(k_axpby<double>, dim3(grid1d(n)), dim3(256), 0, s, n, a, (const double*)x,
+                       b, (double*)y);
+  else if (precision == VD_F32)
+    hipLaunchKernelGGL(k_axpby<float>, dim3(grid1d(n)), dim3(256), 0, s, n, (float)a,
+                       (const float*)x, (float)b, (float*)y);
+  else
+    VD_REQUIRE(false, VD_ERR_INVALID_ARG, "bad precision");
+  VD_HIP_TRY(hipGetLastError());
+  return VD_OK;
+}
+
+int vd_convert(int sp, const void* src, int dp, void* dst, int64_t n, int nan_to_zero,
+               void* stream) {
+  VD_REQUIRE(src && dst, VD_ERR_INVALID_ARG, "null argument");
+  VD_REQUIRE((sp == VD_F32 || sp == VD_F64) && (dp == VD_F32 || dp == VD_F64),
+             VD_ERR_INVALID_ARG, "bad precision");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (n <= 0) return VD_OK;
+  const dim3 g(grid1d(n)), b(256);
+  if (sp == VD_F64 && dp == VD_F64)
+    hipLaunchKernelGGL((k_convert<double, double>), g, b, 0, s, n, (const double*)src,
+                       (double*)dst, nan_to_zero);
+  else if (sp == VD_F64 && dp == VD_F32)
+    hipLaunchKernelGGL((k_convert<double, float>), g, b, 0, s, n, (const double*)src, (float*)dst,
+                       nan_to_zero);
+  else if (sp == VD_F32 && dp == VD_F64)
+    hipLaunchKernelGGL((k_convert<float, double>), g, b, 0, s, n, (const float*)src, (double*)dst,
+                       nan_to_zero);
+  else
+    hipLaunchKernelGGL((k_convert<float, float>), g, b, 0, s, n, (const float*)src, (float*)dst,
+                       nan_to_zero);
+  VD_HIP_TRY(hipGetLastError());
+  return VD_OK;
+}
+
+}  // extern "C"
