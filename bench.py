@@ -1,0 +1,180 @@
+"""Benchmark: forward+adjoint 4D-Var gradient evaluations per second.
+
+Workload (default): 1.5-layer QG in a closed 512x512 basin, dx = 5 km,
+Ld = 20 km (c = 2 m/s), dt = 900 s, 30-day window (2880 RK2 steps); 60
+seeded nadir tracks every 6 km plus one 120 km swath per day at 2 km pixels
+(sigma = 0.02 m); initial-state control with x_b = 0, sigma_b = 0.1 m;
+checkpoints every 54 steps plus every observation time.  One "step" of
+this benchmark is one full ``cost_and_grad`` evaluation (forward sweep with
+misfits, reverse sweep with the hand-written adjoint) with the control
+vector already resident in HBM.
+
+Prints one JSON line.  ``--gpus N`` with N > 1 exits: this build measures a
+single GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "fwd+adjoint 4D-Var gradients/sec, QG 512² 30-day window, grad rel.err≤1e-5"
+HBM_PEAK_GBS = 8000.0
+MFMA_PEAK = {"f64": 78.6, "f32": 157.3}  # dense TFLOP/s, MI355X
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--precision", default="float64", choices=["float64", "float32"])
+    ap.add_argument("--tape", default="full", choices=["full", "checkpoint"])
+    ap.add_argument("--window-steps", type=int, default=None,
+                    help="shorten the window (profiling only; the default is the full 2880)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample-steps", type=int, default=24)
+    ap.add_argument("--check-f32", action="store_true",
+                    help="also evaluate the FP32 path and report its gradient error vs FP64")
+    return ap.parse_args()
+
+
+def gemm_roofline(plan, precision, n_inv=200):
+    """Average duration of the inversion GEMMs, timed with HIP events on the
+    stream they run on; algorithmic flops 4 m_y m_x (m_y + m_x) per inversion
+    (4 GEMM launches)."""
+    import torch
+    q = torch.randn(plan.shape, dtype=plan.dtype, device=plan.device) * 1e-6
+    hb = torch.zeros_like(q)
+    out = torch.empty_like(q)
+    for _ in range(10):
+        plan.pv2h(q, hb, None, out)
+    torch.cuda.synchronize()
+    s = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    for _ in range(n_inv):
+        plan.pv2h(q, hb, None, out)
+    e1.record(s)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    my, mx = plan.ny - 2, plan.nx - 2
+    flops_inv = 4.0 * my * mx * (my + mx)
+    per_launch_s = ms * 1e-3 / (4 * n_inv)
+    achieved = flops_inv / 4 / per_launch_s / 1e12
+    peak = MFMA_PEAK["f64" if precision == "float64" else "f32"]
+    return {"bound": "mfma", "kernel": "k_gemm32 (sine-matrix GEMM of the Helmholtz inversion)",
+            "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
+            "frac": round(achieved / peak, 4), "traffic": None,
+            "avg_launch_us": round(per_launch_s * 1e6, 3),
+            "flops_per_launch": flops_inv / 4}
+
+
+def cpu_baseline(case, sample_steps):
+    """FP64 oracle forward+adjoint over a bounded sample of the same workload,
+    on one host thread; scaled to gradients/s of the full window."""
+    from threadpoolctl import threadpool_limits
+    from oracle.qg import QgmOracle
+    from vardyn_amd import synthetic as syn
+    h0 = syn.gaussian_random_field(case.ny, case.nx, case.dx, case.wavelength, case.rms, case.seed)
+    qg = QgmOracle(dx=case.dx, dy=case.dx, dt=case.dt, c=case.c, f=case.f0,
+                   time_scheme=case.time_scheme, shape=(case.ny, case.nx))
+    hb = np.zeros_like(h0)
+    with threadpool_limits(limits=1):
+        t0 = time.perf_counter()
+        qg.step(h0, hb, sample_steps)                   # forward sweep
+        qg.step_adj(np.ones_like(h0), h0, hb, sample_steps)  # recompute + reverse sweep
+        t = time.perf_counter() - t0
+    per_grad = t * case.n_steps / sample_steps
+    return {"value": round(1.0 / per_grad, 6), "unit": "gradients/s", "cores": 1, "kind": "port",
+            "sample": f"FP64 NumPy/SciPy oracle, forward + adjoint of {sample_steps} RK2 steps at "
+                      f"{case.ny}x{case.nx} ({t:.1f} s), scaled by {case.n_steps}/{sample_steps}",
+            "host": platform.processor() or platform.machine(), "host_cpus": os.cpu_count()}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+              f"(independent problems shard across GPUs with no communication)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from vardyn_amd import synthetic as syn
+    from vardyn_amd.problem import build_problem
+
+    case = syn.case_basin512()
+    n_steps = args.window_steps or case.n_steps
+    t0 = time.time()
+    log(f"building problem {case.name}: {case.ny}x{case.nx}, {n_steps} steps, {args.precision}")
+    pb = build_problem(case, precision=args.precision, n_steps=n_steps, control="ic",
+                       tape=args.tape)
+    eng = pb.engine
+    log(f"setup {time.time() - t0:.1f}s: {pb.H.n_total} obs at {len(pb.H.t_obs)} times, "
+        f"{len(pb.checkpoints) - 1} segments")
+    X0 = torch.zeros(eng.nctrl, dtype=torch.float64, device=pb.plan.device)
+    for w in range(args.warmup):
+        J, G = eng.run(X0)
+        torch.cuda.synchronize()
+        log(f"warmup {w}: J={J.item():.6e}")
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        J, G = eng.run(X0)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t1
+    ms = dt * 1e3 / args.steps
+    value = args.steps / dt
+    Jv, gnorm = J.item(), float(torch.linalg.vector_norm(G).item())
+    log(f"timed {args.steps} gradients: {ms:.1f} ms each; J={Jv:.10e} |G|={gnorm:.10e}")
+
+    roof = gemm_roofline(pb.plan, args.precision)
+    log(f"roofline: {roof}")
+    out = {
+        "metric": METRIC, "value": round(value, 4), "unit": "gradients/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if args.precision == "float64" else "f32",
+        "data": "synthetic (seeded random initial SSH, truth run, nadir tracks + swaths)",
+        "config": {"workload": case.name, "grid": [case.ny, case.nx], "dx_m": case.dx,
+                   "dt_s": case.dt, "window_steps": n_steps, "time_scheme": case.time_scheme,
+                   "obs": pb.H.n_total, "segments": len(pb.checkpoints) - 1,
+                   "control": "initial SSH", "tape": args.tape},
+        "grid_point_steps_per_s": round(case.ny * case.nx * n_steps * value, 1),
+        "J": Jv, "grad_norm": gnorm,
+        "roofline": roof,
+    }
+    if args.check_f32 and args.precision == "float64":
+        pb32 = build_problem(case, precision="float32", n_steps=n_steps, control="ic",
+                             tape=args.tape, truth_plan=pb.plan)
+        G64 = G.clone()
+        J32, G32 = pb32.engine.run(X0)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        pb32.engine.run(X0)
+        torch.cuda.synchronize()
+        out["f32_path"] = {
+            "gradients_per_s": round(1.0 / (time.perf_counter() - t2), 4),
+            "grad_rel_err_vs_f64": float((torch.linalg.vector_norm(G32 - G64) /
+                                          torch.linalg.vector_norm(G64)).item()),
+            "J_rel_err_vs_f64": abs(J32.item() - Jv) / abs(Jv)}
+        log(f"f32 path: {out['f32_path']}")
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(case, args.cpu_sample_steps)
+        log(f"cpu baseline: {out['cpu_baseline']}")
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

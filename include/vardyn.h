@@ -120,6 +120,10 @@ int vd_qg_pv2h(vd_plan* plan, const void* q, const void* hb, const void* qb, voi
 /* q = h2pv(h, hb) (jqgm.py:264-293). */
 int vd_qg_h2pv(vd_plan* plan, const void* h, const void* hb, void* q, void* stream);
 
+/* field[i] = value where the plan mask is 0 (land); the reference fills the
+ * initial state with NaN there (mapping/src/mod.py:697-700). */
+int vd_qg_fill_land(vd_plan* plan, void* field, double value, void* stream);
+
 /* ---------------------------------------------------------------------------
  * L3 observation operator (Obsop_interp_l3_jax, mapping/src/obsop.py:375-678).
  * One table per observation time:

@@ -56,6 +56,7 @@ SIGNATURES = {
     "vd_qg_step_adj": (_I, [_VP, _VP, _VP, _VP, _VP, _I, _VP, _I, _VP]),
     "vd_qg_pv2h": (_I, [_VP, _VP, _VP, _VP, _VP, _VP]),
     "vd_qg_h2pv": (_I, [_VP, _VP, _VP, _VP, _VP]),
+    "vd_qg_fill_land": (_I, [_VP, _VP, _D, _VP]),
     "vd_obs_misfit": (_I, [C.POINTER(ObsTable), _I, _VP, _VP, _VP, _VP, _D, _VP]),
     "vd_obs_adjoint": (_I, [C.POINTER(ObsTable), _I, _VP, _VP, _VP, _VP]),
     "vd_dot_f64": (_I, [_I64, _VP, _VP, _D, _VP, _VP, _VP]),

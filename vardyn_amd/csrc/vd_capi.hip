@@ -445,4 +445,16 @@ int vd_qg_h2pv(vd_plan* h, const void* hh, const void* hb, void* q, void* stream
   return VD_OK;
 }
 
+int vd_qg_fill_land(vd_plan* h, void* field, double value, void* stream) {
+  VD_REQUIRE(h && field, VD_ERR_INVALID_ARG, "null argument");
+  Plan& p = h->p;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (p.prec == VD_F64)
+    launch_fill_land<double>(p, (double*)field, value, s);
+  else
+    launch_fill_land<float>(p, (float*)field, (float)value, s);
+  VD_HIP_TRY(hipGetLastError());
+  return VD_OK;
+}
+
 }  // extern "C"

@@ -105,6 +105,8 @@ void launch_prep(const Plan& p, const T* h0, const T* hb, T* hcur, T* hbm, T* q0
                  hipStream_t s);
 template <typename T>
 void launch_h2pv(const Plan& p, const T* h, const T* hb, T* q, hipStream_t s);
+template <typename T>
+void launch_fill_land(const Plan& p, T* f, T value, hipStream_t s);
 // h1 = NaN on land, else h - mdt.
 template <typename T>
 void launch_finish(const Plan& p, const T* h, T* h1, hipStream_t s);

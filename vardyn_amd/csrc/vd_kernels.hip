@@ -91,9 +91,13 @@ __global__ __launch_bounds__(256) void k_gemm32(GemmArgs<T> g) {
     }
   };
 
-  acc_t acc;
+  // four independent accumulators (K slices interleaved) keep the MFMA
+  // pipeline busy without a serial dependency on one accumulator
+  acc_t acc4[4];
 #pragma unroll
-  for (int r = 0; r < 4; ++r) acc[r] = T(0);
+  for (int c = 0; c < 4; ++c)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) acc4[c][r] = T(0);
 
   load(0);
   for (int k0 = 0; k0 < g.K; k0 += kTile) {
@@ -109,9 +113,12 @@ __global__ __launch_bounds__(256) void k_gemm32(GemmArgs<T> g) {
     for (int kk = 0; kk < kTile / 4; ++kk) {
       const T a = As[wm * 16 + (lane & 15)][kk * 4 + (lane >> 4)];
       const T b = Bs[kk * 4 + (lane >> 4)][wn * 16 + (lane & 15)];
-      acc = Mfma<T>::run(a, b, acc);
+      acc4[kk & 3] = Mfma<T>::run(a, b, acc4[kk & 3]);
     }
   }
+  acc_t acc;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) acc[r] = (acc4[0][r] + acc4[1][r]) + (acc4[2][r] + acc4[3][r]);
 
   const int col = n0 + wn * 16 + (lane & 15);
 #pragma unroll
@@ -330,6 +337,12 @@ __global__ void k_h2pv(Phys<T> P, const T* __restrict__ h, const T* __restrict__
                        T* __restrict__ q) {
   VD_POINT_2D
   q[o] = h2pv_at(h, hb, i, j, P.mask[o], P);
+}
+
+template <typename T>
+__global__ void k_fill_land(Phys<T> P, T* __restrict__ f, T value) {
+  VD_POINT_2D
+  if (P.mask[o] == 0) f[o] = value;
 }
 
 template <typename T>
@@ -596,6 +609,12 @@ void launch_h2pv(const Plan& p, const T* h, const T* hb, T* q, hipStream_t s) {
 }
 
 template <typename T>
+void launch_fill_land(const Plan& p, T* f, T value, hipStream_t s) {
+  Phys<T> P = make_phys<T>(p);
+  hipLaunchKernelGGL(k_fill_land<T>, grid2d(p.ny, p.nx), kBlock2d, 0, s, P, f, value);
+}
+
+template <typename T>
 void launch_finish(const Plan& p, const T* h, T* h1, hipStream_t s) {
   Phys<T> P = make_phys<T>(p);
   const T* mdt = p.has_mdt ? static_cast<const T*>(p.mdt) : nullptr;
@@ -662,6 +681,7 @@ void launch_adj_final(const Plan& p, const T* hcur, const T* lh, const T* lq, T*
   template void launch_prep<T>(const Plan&, const T*, const T*, T*, T*, T*, T*, hipStream_t); \
   template void launch_h2pv<T>(const Plan&, const T*, const T*, T*, hipStream_t);             \
   template void launch_finish<T>(const Plan&, const T*, T*, hipStream_t);                     \
+  template void launch_fill_land<T>(const Plan&, T*, T, hipStream_t);                         \
   template void launch_stage<T>(const Plan&, const T*, const T*, const T*, const T*,          \
                                 const T*, T, T, T, T*, T*, hipStream_t);                      \
   template void launch_stage_tl<T>(const Plan&, const T*, const T*, const T*, const T*,       \

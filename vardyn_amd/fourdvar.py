@@ -1,0 +1,231 @@
+"""GPU-resident 4D-Var cost and gradient for the QG model.
+
+Reference: ``Variational.cost_and_grad`` (mapping/src/tools_4Dvar.py:252-400)
+driving ``Model_qg1l_jax.step/step_adj`` (mapping/src/mod.py:867-1076) over
+the checkpoint schedule of ``Inv_4Dvar`` (mapping/src/inv.py:144-163),
+datlas-ocean/VarDyn.
+
+One evaluation follows the reference step by step:
+
+forward, for each segment [c_i, c_{i+1}):
+  misfit at t_i if it is an observation time (Jo += m.m / sigma_R^2),
+  control -> flux where c_i % dtbasis == 0 (flux controls),
+  segment start state kept (the trajectory checkpoint),
+  QG step over the segment, then ``+ nstep*dt/86400 * Fssh``;
+misfit at the final time; J = (Jo + Jb) / 2;
+reverse, from the last segment to the first:
+  ``adFssh += nstep*dt/86400 * adX1`` then the segment adjoint,
+  basis transpose where c_i % dtbasis == 0, observation adjoint at t_i
+  (fed with the raw misfit, as the reference does, tools_4Dvar.py:349,380).
+
+Controls: ``"ic"`` -- the control vector is the initial SSH (the reference
+computes this adjoint and drops it); ``"offset"`` -- the reference's
+``Basis_offset`` flux (basis.py:3939-4000).
+
+Tape: ``"full"`` keeps every stage state of the window in HBM during the
+forward sweep (no recomputation); ``"checkpoint"`` keeps only the segment
+start states and recomputes each segment into one reusable tape during the
+reverse sweep, like the reference's ``jax.vjp`` of a segment.  Both give
+identical results.
+
+All arithmetic is on the device: cost accumulators and the control/gradient
+vectors are float64; model fields use the plan precision.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import VardynError
+from .qg import QgPlan, current_stream_ptr, ptr
+
+
+class FourDVar:
+    def __init__(self, plan: QgPlan, obsop, checkpoints, T, control="ic", sigma_b=None,
+                 Xb=None, sigma_r=None, prec=False, dtbasis=None, tape="checkpoint", hb=None,
+                 land=None):
+        if control not in ("ic", "offset"):
+            raise ValueError("control must be 'ic' or 'offset'")
+        if tape not in ("checkpoint", "full"):
+            raise ValueError("tape must be 'checkpoint' or 'full'")
+        if sigma_b is not None and not np.isscalar(sigma_b):
+            raise NotImplementedError("only a scalar background standard deviation is supported")
+        if prec and sigma_b is None:
+            raise ValueError("preconditioning needs sigma_b")
+        self.plan = plan
+        self.H = obsop
+        self.dev = plan.device
+        self.dtype = plan.dtype
+        self.control = control
+        self.sigma_b = sigma_b
+        self.sigma_r = 1.0 if sigma_r is None else float(sigma_r)
+        self.prec = bool(prec)
+        self.tape_mode = tape
+        cps = np.asarray(checkpoints, dtype=np.int64)
+        if cps[0] != 0 or np.any(np.diff(cps) <= 0):
+            raise VardynError("checkpoints must start at 0 and increase")
+        self.cps = cps
+        self.T = np.asarray(T, dtype=np.float64)
+        if len(self.T) != cps[-1] + 1:
+            raise VardynError("T must hold one time per model step (len == checkpoints[-1] + 1)")
+        self.dtbasis = int(dtbasis) if dtbasis is not None else int(np.max(np.diff(cps)))
+        self.nseg = len(cps) - 1
+        self.nsteps = np.diff(cps).astype(int)
+        self.n = plan.ny * plan.nx
+        self.nctrl = self.n if control == "ic" else 1
+
+        # observation table index at each segment start and at the final time
+        def obs_index(t):
+            return self.H.index(t) if self.H.is_obs_time(t) else -1
+        self.seg_obs = [obs_index(self.T[c]) for c in cps[:-1]]
+        self.final_obs = obs_index(self.T[cps[-1]])
+
+        shape = (plan.ny, plan.nx)
+        dd = dict(device=self.dev)
+        self.hb = torch.zeros(shape, dtype=self.dtype, **dd) if hb is None else plan.field(hb)
+        self.states = torch.empty((self.nseg + 1,) + shape, dtype=self.dtype, **dd)
+        if tape == "full":
+            self.tape_elems = [plan.tape_bytes(int(k)) // plan.empty().element_size()
+                               for k in self.nsteps]
+            self.tape_off = np.concatenate([[0], np.cumsum(self.tape_elems)]).astype(np.int64)
+            self.tape = torch.empty(int(self.tape_off[-1]), dtype=self.dtype, **dd)
+        else:
+            self.tape = plan.new_tape(int(self.nsteps.max()))
+        self.lam = torch.empty(shape, dtype=self.dtype, **dd)
+        self.lam2 = torch.empty(shape, dtype=self.dtype, **dd)
+        self.params = torch.zeros(shape, dtype=self.dtype, **dd)
+        self.adparams = torch.zeros(shape, dtype=self.dtype, **dd)
+        self.ones64 = torch.ones(self.n, dtype=torch.float64, **dd)
+        self.X = torch.empty(self.nctrl, dtype=torch.float64, **dd)
+        self.G = torch.empty(self.nctrl, dtype=torch.float64, **dd)
+        self.J = torch.zeros(1, dtype=torch.float64, **dd)
+        self.scratch = torch.empty(_lib.VD_REDUCE_SCRATCH, dtype=torch.float64, **dd)
+        self.Xb = torch.zeros(self.nctrl, dtype=torch.float64, **dd) if Xb is None else \
+            torch.as_tensor(np.asarray(Xb, dtype=np.float64), device=self.dev).reshape(-1).clone()
+        self.var0 = torch.zeros(shape, dtype=self.dtype, **dd)
+        self.land = land
+
+    # ------------------------------------------------------------ helpers
+    def _s(self):
+        return current_stream_ptr(self.dev)
+
+    def _tape(self, i):
+        if self.tape_mode == "full":
+            return self.tape[int(self.tape_off[i]):int(self.tape_off[i + 1])]
+        return self.tape
+
+    def _axpby(self, a, x, b, y):
+        prec = _lib.VD_F64 if y.dtype == torch.float64 else _lib.VD_F32
+        _lib.call("vd_axpby", prec, y.numel(), float(a), ptr(x), float(b), ptr(y), self._s())
+
+    def _dot(self, x, y, alpha, out):
+        _lib.call("vd_dot_f64", x.numel(), ptr(x), ptr(y), float(alpha), ptr(out), ptr(self.scratch),
+                  self._s())
+
+    def _convert(self, src, dst, nan_to_zero=False):
+        sp = _lib.VD_F64 if src.dtype == torch.float64 else _lib.VD_F32
+        dp = _lib.VD_F64 if dst.dtype == torch.float64 else _lib.VD_F32
+        _lib.call("vd_convert", sp, ptr(src), dp, ptr(dst), src.numel(), 1 if nan_to_zero else 0,
+                  self._s())
+
+    def _flux_scale(self, i):
+        return self.nsteps[i] * self.plan.dt / (3600 * 24)
+
+    # --------------------------------------------------------- evaluation
+    def run(self, X0):
+        """Enqueue one cost+gradient evaluation; X0 is a float64 device vector.
+
+        Results are left in ``self.J`` (1-element float64) and ``self.G``.
+        """
+        plan, H = self.plan, self.H
+        if X0.dtype != torch.float64 or X0.device != self.dev or X0.numel() != self.nctrl:
+            raise VardynError(f"control must be a float64 device vector of size {self.nctrl}")
+        X0 = X0.contiguous()
+        J, X = self.J, self.X
+        J.zero_()
+        inv_sb2 = 0.0 if self.sigma_b is None else 1.0 / float(self.sigma_b) ** 2
+        # background term (tools_4Dvar.py:262-271); J accumulates (Jo + Jb) / 2
+        if self.sigma_b is not None:
+            if self.prec:
+                self._convert(self.Xb, X)
+                self._axpby(self.sigma_b, X0, 1.0, X)
+                self._dot(X0, X0, 0.5, J)
+            else:
+                self._convert(self.Xb, X)
+                self._axpby(1.0, X0, 1.0, X)
+                self._dot(X0, X0, 0.5 * inv_sb2, J)
+        else:
+            self._convert(X0, X)
+            self._axpby(-1.0, self.Xb, 1.0, X)
+        inv_r2 = 0.5 / self.sigma_r ** 2
+        s0 = self.states[0]
+        if self.control == "ic":
+            self._convert(X, s0)
+            if self.land is not None:
+                _lib.call("vd_qg_fill_land", plan._h, ptr(s0), float("nan"), self._s())
+        else:
+            s0.copy_(self.var0)
+        params = self.params
+        params.zero_()
+        # forward sweep
+        for i in range(self.nseg):
+            st, nxt = self.states[i], self.states[i + 1]
+            if self.seg_obs[i] >= 0:
+                H.misfit_dev(self.seg_obs[i], st, jo=J, inv_sigma_r2=inv_r2)
+            if self.control == "offset" and self.cps[i] % self.dtbasis == 0:
+                # Basis_offset.operg: params = X[0] everywhere (basis.py:3968-3984)
+                params.copy_(X[0].to(params.dtype).expand_as(params))
+            tape = self._tape(i) if self.tape_mode == "full" else None
+            plan.step(st, self.hb, int(self.nsteps[i]), h1=nxt, tape=tape)
+            if self.control == "offset":
+                self._axpby(self._flux_scale(i), params, 1.0, nxt)
+        if self.final_obs >= 0:
+            H.misfit_dev(self.final_obs, self.states[-1], jo=J, inv_sigma_r2=inv_r2)
+
+        # reverse sweep
+        lam, lam2 = self.lam, self.lam2
+        lam.zero_()
+        adpar = self.adparams
+        adpar.zero_()
+        G = self.G
+        G.zero_()
+        if self.final_obs >= 0:
+            H.adj_dev(self.final_obs, H.tables[self.final_obs].misfit, lam)
+        for i in reversed(range(self.nseg)):
+            if self.control == "offset":
+                self._axpby(self._flux_scale(i), lam, 1.0, adpar)
+            plan.step_adj(lam, self.states[i], self.hb, int(self.nsteps[i]), adh0=lam2,
+                          tape=self._tape(i), tape_valid=self.tape_mode == "full")
+            lam, lam2 = lam2, lam
+            if self.control == "offset" and self.cps[i] % self.dtbasis == 0:
+                tmp = self._adpar64()
+                self._dot(tmp, self.ones64, 1.0, G)
+                adpar.zero_()
+            if self.seg_obs[i] >= 0:
+                H.adj_dev(self.seg_obs[i], H.tables[self.seg_obs[i]].misfit, lam)
+        # gradient (tools_4Dvar.py:388-398)
+        if self.control == "ic":
+            self._convert(lam, G, nan_to_zero=True)
+        if self.prec:
+            self._axpby(0.0, G, float(self.sigma_b), G)
+            self._axpby(1.0, X0, 1.0, G)
+        elif self.sigma_b is not None:
+            self._axpby(inv_sb2, X0, 1.0, G)
+        return J, G
+
+    def _params64(self):
+        if not hasattr(self, "_p64"):
+            self._p64 = torch.empty(self.n, dtype=torch.float64, device=self.dev)
+        return self._p64
+
+    def _adpar64(self):
+        t = self._params64()
+        self._convert(self.adparams, t)
+        return t
+
+    def cost_and_grad(self, X0):
+        """Host interface for scipy: numpy in, (float J, numpy G) out."""
+        x = torch.as_tensor(np.asarray(X0, dtype=np.float64), device=self.dev)
+        J, G = self.run(x)
+        return float(J.item()), G.cpu().numpy()

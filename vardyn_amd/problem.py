@@ -1,0 +1,74 @@
+"""Assemble a complete synthetic QG 4D-Var problem on the GPU.
+
+Grid, truth run, observations (sampled from the truth on the model itself),
+observation operator tables, checkpoint schedule and the cost/gradient
+engine -- what ``Inv_4Dvar`` sets up before minimising (mapping/src/inv.py:105-263).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import synthetic as syn
+from .fourdvar import FourDVar
+from .inv import checkpoint_schedule
+from .obsop import Obsop_interp_l3
+from .qg import QgPlan
+
+
+@dataclass
+class Problem:
+    case: syn.QgCase
+    grid: syn.Grid
+    plan: QgPlan
+    H: Obsop_interp_l3
+    T: np.ndarray
+    checkpoints: np.ndarray
+    obs: syn.ObsSet
+    dict_obs: dict
+    truth0: np.ndarray
+    engine: FourDVar | None = None
+
+
+def truth_run(plan: QgPlan, h0, steps, dt):
+    """SSH of a free run from ``h0`` at the requested steps (host arrays)."""
+    out = {}
+    h = plan.field(h0)
+    hb = torch.zeros_like(h)
+    nxt = torch.empty_like(h)
+    s0 = 0
+    for s in sorted(steps):
+        if s > s0:
+            plan.step(h, hb, s - s0, h1=nxt)
+            h, nxt = nxt, h
+        out[s] = h.double().cpu().numpy()
+        s0 = s
+    return out
+
+
+def build_problem(case: syn.QgCase, precision="float64", n_steps=None, control="ic",
+                  tape="checkpoint", device=None, truth_plan: QgPlan | None = None,
+                  with_engine=True) -> Problem:
+    n_steps = case.n_steps if n_steps is None else int(n_steps)
+    grid = syn.make_grid(case.ny, case.nx, case.dx)
+    plan = QgPlan(case.ny, case.nx, case.dx, case.dx, case.dt, case.c, case.f0, g=case.g,
+                  time_scheme=case.time_scheme, precision=precision, device=device)
+    obs = syn.make_observations(case, grid, n_steps)
+    truth0 = syn.gaussian_random_field(case.ny, case.nx, case.dx, case.wavelength, case.rms,
+                                       case.seed)
+    tp = truth_plan if truth_plan is not None else plan
+    syn.sample_truth(obs, lambda steps: truth_run(tp, truth0, steps, case.dt), case.sigma_obs,
+                     case.seed + 3)
+    T = np.arange(n_steps + 1) * case.dt
+    dict_obs = {float(T[s]): (e["lon"], e["lat"], e["y"], e["err"]) for s, e in obs.steps.items()}
+    H = Obsop_interp_l3(grid.lon, grid.lat, grid.DX, grid.DY, T, case.dt, dict_obs, Npix=4,
+                        device=plan.device).process_obs()
+    cps = checkpoint_schedule(n_steps + 1, case.checkpoint_steps, lambda i: H.is_obs(T[i]))
+    pb = Problem(case, grid, plan, H, T, cps, obs, dict_obs, truth0)
+    if with_engine:
+        sb = case.sigma_b if control == "ic" else 0.01
+        pb.engine = FourDVar(plan, H, cps, T, control=control, sigma_b=sb, tape=tape,
+                             dtbasis=case.checkpoint_steps)
+    return pb
