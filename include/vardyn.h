@@ -69,6 +69,9 @@ typedef struct vd_qg_params {
                             or NULL */
   const double* mdt;     /* (ny, nx) mean dynamic topography (jqgm.py:532-555)
                             or NULL */
+  int dst_dense;         /* 0: fold the sine transform by its symmetry when
+                            ny-2 and nx-2 are even (half the GEMM work);
+                            1: always the full (n-2)^2 sine matrices */
 } vd_qg_params;
 
 typedef struct vd_plan vd_plan;
@@ -90,6 +93,13 @@ int vd_qg_tape_bytes(const vd_plan* plan, int nstep, size_t* bytes);
 
 /* Bytes of plan-owned device memory (constants + scratch). */
 int vd_qg_plan_bytes(const vd_plan* plan, size_t* bytes);
+
+/* Diagnostics: device address and size of the plan's GEMM intermediates
+ * (which = 0: the first/third GEMM output, 1: the scaled second GEMM
+ * output), each (MY x MX) row-major with MY, MX the interior sizes rounded
+ * up to 32.  Valid until the next call on the plan.  No reference
+ * counterpart (the reference's inversion has no exposed intermediates). */
+int vd_qg_debug_scratch(const vd_plan* plan, int which, void** ptr, int* rows, int* cols);
 
 /* Forward propagation over `nstep` steps: h1 = M(h0; hb).  Replaces
  * Qgm.step (jqgm.py:517-557): q0 = h2pv(h0), q is prognostic, h re-diagnosed

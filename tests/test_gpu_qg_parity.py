@@ -23,7 +23,7 @@ def rel(a, b):
 
 
 def make_pair(ny, nx, scheme, precision, land=False, beta=False, kdiff=False, mdt=False,
-              bathy=False, seed=0):
+              bathy=False, seed=0, dst_dense=False):
     rng = np.random.default_rng(seed)
     dx, dy, dt, c = 10e3, 12e3, 1200.0, 2.7
     ssh = None
@@ -42,7 +42,8 @@ def make_pair(ny, nx, scheme, precision, land=False, beta=False, kdiff=False, md
                     Kdiffus=K, mdt=m, bathymetry_PV_term=b)
     fb = None if b is None else float(orc.f) * b
     plan = QgPlan(ny, nx, dx, dy, dt, c, float(orc.f), time_scheme=scheme, precision=precision,
-                  kdiffus=K, mask=build_mask(ny, nx, ssh), beta=orc.beta, fbathy=fb, mdt=m)
+                  kdiffus=K, mask=build_mask(ny, nx, ssh), beta=orc.beta, fbathy=fb, mdt=m,
+                  dst_dense=dst_dense)
     h0 = 0.1 * rng.standard_normal((ny, nx))
     if land:
         h0[np.isnan(ssh)] = np.nan
@@ -58,11 +59,12 @@ def test_mask_matches_oracle():
     assert np.array_equal(build_mask(20, 17, ssh), omask(20, 17, ssh))
 
 
+@pytest.mark.parametrize("dst_dense", [False, True])
 @pytest.mark.parametrize("precision,tol", [("float64", 1e-12), ("float32", 2e-6)])
-@pytest.mark.parametrize("shape", [(40, 48), (66, 34), (128, 128)])
-def test_pv2h_h2pv(precision, tol, shape):
+@pytest.mark.parametrize("shape", [(40, 48), (66, 34), (128, 128), (41, 47), (40, 45)])
+def test_pv2h_h2pv(precision, tol, shape, dst_dense):
     ny, nx = shape
-    orc, plan, h0, hb, rng = make_pair(ny, nx, "rk2", precision)
+    orc, plan, h0, hb, rng = make_pair(ny, nx, "rk2", precision, dst_dense=dst_dense)
     q = rng.standard_normal((ny, nx)) * 1e-5
     qb = orc.h2pv(hb, hb)
     h_ref = orc.pv2h(q, hb, qb)
@@ -74,10 +76,11 @@ def test_pv2h_h2pv(precision, tol, shape):
     assert rel(qq.cpu().numpy(), q_ref) < tol
 
 
+@pytest.mark.parametrize("dst_dense", [False, True])
 @pytest.mark.parametrize("scheme", ["Euler", "rk2", "rk3"])
 @pytest.mark.parametrize("precision,tol", [("float64", 1e-10), ("float32", 2e-5)])
-def test_step(scheme, precision, tol):
-    orc, plan, h0, hb, _ = make_pair(40, 48, scheme, precision)
+def test_step(scheme, precision, tol, dst_dense):
+    orc, plan, h0, hb, _ = make_pair(40, 48, scheme, precision, dst_dense=dst_dense)
     ref = orc.step(h0, hb, 6)
     h1 = plan.step(plan.field(h0), plan.field(hb), 6)
     assert rel(h1.cpu().numpy(), ref) < tol

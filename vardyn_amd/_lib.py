@@ -29,6 +29,7 @@ class QgParams(C.Structure):
         ("g", C.c_double), ("f0", C.c_double), ("c", C.c_double),
         ("has_kdiffus", C.c_int), ("kdiffus", C.c_double),
         ("mask", C.c_void_p), ("beta", C.c_void_p), ("fbathy", C.c_void_p), ("mdt", C.c_void_p),
+        ("dst_dense", C.c_int),
     ]
 
 
@@ -51,6 +52,7 @@ SIGNATURES = {
     "vd_qg_plan_destroy": (_I, [_VP]),
     "vd_qg_tape_bytes": (_I, [_VP, _I, C.POINTER(_SZ)]),
     "vd_qg_plan_bytes": (_I, [_VP, C.POINTER(_SZ)]),
+    "vd_qg_debug_scratch": (_I, [_VP, _I, C.POINTER(_VP), C.POINTER(_I), C.POINTER(_I)]),
     "vd_qg_step": (_I, [_VP, _VP, _VP, _VP, _I, _VP, _VP]),
     "vd_qg_step_tgl": (_I, [_VP, _VP, _VP, _VP, _VP, _VP, _I, _VP]),
     "vd_qg_step_adj": (_I, [_VP, _VP, _VP, _VP, _VP, _I, _VP, _I, _VP]),

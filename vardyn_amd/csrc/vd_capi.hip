@@ -32,6 +32,7 @@ static int upload(const std::vector<double>& h, void** dptr) {
 }
 
 static void release_plan_buffers(Plan& p) {
+  hipFree(p.SyF); hipFree(p.SxF);
   hipFree(p.Sy); hipFree(p.Sx); hipFree(p.lam_inv); hipFree(p.mask);
   hipFree(p.beta); hipFree(p.fbathy); hipFree(p.mdt);
   for (auto& g : p.gT) hipFree(g);
@@ -56,6 +57,21 @@ static int build_constants(Plan& p, const vd_qg_params& prm) {
   int rc;
   if ((rc = upload<T>(sine(p.my, p.MY), &p.Sy)) != VD_OK) return rc;
   if ((rc = upload<T>(sine(p.mx, p.MX), &p.Sx)) != VD_OK) return rc;
+  if (p.fold) {
+    // SyF[p][r][k] = S_y[2r+p][k], SxF[p][l][c] = S_x[l][2c+p] for r, k, l, c < m/2
+    const std::vector<double> Sy = sine(p.my, p.my), Sx = sine(p.mx, p.mx);
+    std::vector<double> fy(2 * (size_t)p.HY * p.HY, 0.0), fx(2 * (size_t)p.HX * p.HX, 0.0);
+    for (int par = 0; par < 2; ++par) {
+      for (int r = 0; r < p.hy; ++r)
+        for (int k = 0; k < p.hy; ++k)
+          fy[((size_t)par * p.HY + r) * p.HY + k] = Sy[(size_t)(2 * r + par) * p.my + k];
+      for (int l = 0; l < p.hx; ++l)
+        for (int c = 0; c < p.hx; ++c)
+          fx[((size_t)par * p.HX + l) * p.HX + c] = Sx[(size_t)l * p.mx + 2 * c + par];
+    }
+    if ((rc = upload<T>(fy, &p.SyF)) != VD_OK) return rc;
+    if ((rc = upload<T>(fx, &p.SxF)) != VD_OK) return rc;
+  }
   // 1/Lambda with Lambda = g/f [2(cos(pi(j+1)/(nx-1))-1)/dx^2 + 2(cos(pi(i+1)/(ny-1))-1)/dy^2] - g f/c^2
   // (jqgm.py:165-172); zero in the padding.
   std::vector<double> li((size_t)p.MY * p.MX, 0.0);
@@ -323,6 +339,9 @@ int vd_qg_plan_create(const vd_qg_params* prm, vd_plan** out) {
   p.ny = prm->ny; p.nx = prm->nx;
   p.my = p.ny - 2; p.mx = p.nx - 2;
   p.MY = round_up(p.my, kTile); p.MX = round_up(p.mx, kTile);
+  p.fold = (!prm->dst_dense && p.my % 2 == 0 && p.mx % 2 == 0) ? 1 : 0;
+  p.hy = p.my / 2; p.hx = p.mx / 2;
+  p.HY = round_up(p.hy, kTile); p.HX = round_up(p.hx, kTile);
   p.prec = prm->precision;
   p.scheme = prm->time_scheme;
   p.nstage = p.scheme == VD_EULER ? 1 : (p.scheme == VD_RK2 ? 2 : 3);
@@ -376,6 +395,15 @@ int vd_qg_plan_bytes(const vd_plan* h, size_t* bytes) {
   if (p.has_bathy) b += p.nf * p.es;
   if (p.has_mdt) b += p.nf * p.es;
   *bytes = b;
+  return VD_OK;
+}
+
+int vd_qg_debug_scratch(const vd_plan* h, int which, void** ptr, int* rows, int* cols) {
+  VD_REQUIRE(h && ptr && rows && cols, VD_ERR_INVALID_ARG, "null argument");
+  VD_REQUIRE(which == 0 || which == 1, VD_ERR_INVALID_ARG, "which must be 0 or 1");
+  *ptr = h->p.gT[which];
+  *rows = h->p.MY;
+  *cols = h->p.MX;
   return VD_OK;
 }
 

@@ -63,6 +63,13 @@ struct Plan {
   size_t nf;    // ny*nx
   size_t ng;    // MY*MX
 
+  // DST by symmetric folding (even interior sizes): two half-size GEMMs per
+  // sine-matrix product instead of one full one.
+  int fold = 0;
+  int hy = 0, hx = 0, HY = 0, HX = 0;
+  void* SyF = nullptr;  // [2][HY][HY]: rows 2r+p of S_y, columns 0..hy-1
+  void* SxF = nullptr;  // [2][HX][HX]: SxF[p][l][c] = S_x[l][2c+p]
+
   // constants
   void* Sy = nullptr;
   void* Sx = nullptr;

@@ -13,18 +13,24 @@ namespace vd {
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef double doublex4 __attribute__((ext_vector_type(4)));
 
+// 16x16x4 MFMA tiles.  Inputs, both precisions: lane l supplies A[l%16][l/16]
+// and B[l/16][l%16].  Result register r of lane l holds column l%16 and row
+// out_row(l, r), which differs between the two instructions (measured on
+// gfx950: the f64 form interleaves the four registers across row groups).
 template <typename T> struct Mfma;
 template <> struct Mfma<float> {
   typedef floatx4 acc_t;
   static __device__ __forceinline__ acc_t run(float a, float b, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int out_row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 template <> struct Mfma<double> {
   typedef doublex4 acc_t;
   static __device__ __forceinline__ acc_t run(double a, double b, acc_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int out_row(int lane, int r) { return 4 * r + (lane >> 4); }
 };
 
 // ---------------------------------------------------------------------------
@@ -123,7 +129,7 @@ __global__ __launch_bounds__(256) void k_gemm32(GemmArgs<T> g) {
   const int col = n0 + wn * 16 + (lane & 15);
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    const int row = m0 + wm * 16 + 4 * (lane >> 4) + r;
+    const int row = m0 + wm * 16 + Mfma<T>::out_row(lane, r);
     const T val = acc[r];
     if (EPI == EPI_STORE) {
       g.C[(size_t)row * g.ldc + col] = val;
@@ -163,9 +169,195 @@ static void gemm(const GemmArgs<T>& a, int M, int N, hipStream_t s) {
   hipLaunchKernelGGL((k_gemm32<T, BSRC, EPI>), grid, dim3(256), 0, s, a);
 }
 
+// ---------------------------------------------------------------------------
+// Folded DST GEMMs.  For an even interior size m, S[j][m-1-k] = (-1)^j S[j][k]
+// (and S is symmetric), so with h = m/2
+//   (S X)[2r+p][:] = sum_{k<h} S[2r+p][k] (X[k][:] + (-1)^p X[m-1-k][:])
+//   (X S)[:][2c+p] = sum_{l<h} (X[:][l] + (-1)^p X[:][m-1-l]) S[l][2c+p]
+// i.e. each m x m sine product becomes two h x h products (p = blockIdx.z):
+// the loaders form the sums/differences, the epilogues interleave the
+// even/odd outputs back into natural order.
+// ---------------------------------------------------------------------------
+enum { AOP_DENSE = 0, AOP_FOLDCOL = 1 };
+enum { BOP_DENSE = 0, BOP_FOLDROW_FIELD = 1, BOP_FOLDROW_DENSE = 2 };
+enum { EPO_ROWFOLD = 0, EPO_SCALE_COLFOLD = 1, EPO_FIELD_SET_COLFOLD = 2, EPO_FIELD_ACC_COLFOLD = 3 };
+
+template <typename T>
+struct FoldArgs {
+  const T* A;
+  int lda;
+  size_t strideA;  // A block of parity p at A + p * strideA (AOP_DENSE)
+  const T* B;
+  int ldb;
+  size_t strideB;  // B block of parity p at B + p * strideB (BOP_DENSE)
+  const T* F;      // BOP_FOLDROW_FIELD: field and optional subtrahend
+  const T* Fsub;
+  int ny, nx, my, mx, hy, hx;
+  int K;
+  T* C;
+  int ldc;
+  const T* scale;  // EPO_SCALE_COLFOLD, ld = ldc
+  const T* hb;
+  T* out;
+};
+
+template <typename T, int AOP, int BOP, int EPO>
+__global__ __launch_bounds__(256) void k_gemm_fold(FoldArgs<T> g) {
+  typedef typename Mfma<T>::acc_t acc_t;
+  __shared__ T As[kTile][kTile + 1];
+  __shared__ T Bs[kTile][kTile + 1];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int w = tid >> 6;
+  const int wm = w >> 1, wn = w & 1;
+  const int par = blockIdx.z;
+  const T sgn = par ? T(-1) : T(1);
+  const int m0 = blockIdx.y * kTile, n0 = blockIdx.x * kTile;
+  const int lr = tid >> 3, lc = (tid & 7) * 4;
+  const T* A = g.A + (AOP == AOP_DENSE ? par * g.strideA : 0);
+  const T* B = g.B + (BOP == BOP_DENSE ? par * g.strideB : 0);
+
+  T ra[4], rb[4];
+  auto load = [&](int k0) {
+    const int i = m0 + lr;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int l = k0 + lc + u;
+      if (AOP == AOP_DENSE) {
+        ra[u] = A[(size_t)i * g.lda + l];
+      } else {
+        ra[u] = l < g.hx ? A[(size_t)i * g.lda + l] + sgn * A[(size_t)i * g.lda + (g.mx - 1 - l)]
+                         : T(0);
+      }
+    }
+    const int k = k0 + lr;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int n = n0 + lc + u;
+      if (BOP == BOP_DENSE) {
+        rb[u] = B[(size_t)k * g.ldb + n];
+      } else if (BOP == BOP_FOLDROW_DENSE) {
+        rb[u] = k < g.hy ? B[(size_t)k * g.ldb + n] + sgn * B[(size_t)(g.my - 1 - k) * g.ldb + n]
+                         : T(0);
+      } else {
+        T v = T(0);
+        if (k < g.hy && n < g.mx) {
+          const size_t i1 = (size_t)(k + 1) * g.nx + (n + 1);
+          const size_t i2 = (size_t)(g.my - k) * g.nx + (n + 1);
+          T a = g.F[i1], b = g.F[i2];
+          if (g.Fsub) { a -= g.Fsub[i1]; b -= g.Fsub[i2]; }
+          v = a + sgn * b;
+        }
+        rb[u] = v;
+      }
+    }
+  };
+
+  acc_t acc4[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) acc4[c][r] = T(0);
+
+  load(0);
+  for (int k0 = 0; k0 < g.K; k0 += kTile) {
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      As[lr][lc + u] = ra[u];
+      Bs[lr][lc + u] = rb[u];
+    }
+    __syncthreads();
+    if (k0 + kTile < g.K) load(k0 + kTile);
+#pragma unroll
+    for (int kk = 0; kk < kTile / 4; ++kk) {
+      const T a = As[wm * 16 + (lane & 15)][kk * 4 + (lane >> 4)];
+      const T b = Bs[kk * 4 + (lane >> 4)][wn * 16 + (lane & 15)];
+      acc4[kk & 3] = Mfma<T>::run(a, b, acc4[kk & 3]);
+    }
+  }
+
+  const int col = n0 + wn * 16 + (lane & 15);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int row = m0 + wm * 16 + Mfma<T>::out_row(lane, r);
+    const T val = (acc4[0][r] + acc4[1][r]) + (acc4[2][r] + acc4[3][r]);
+    if (EPO == EPO_ROWFOLD) {
+      if (row < g.hy) g.C[(size_t)(2 * row + par) * g.ldc + col] = val;
+    } else if (EPO == EPO_SCALE_COLFOLD) {
+      if (col < g.hx) {
+        const size_t o = (size_t)row * g.ldc + 2 * col + par;
+        g.C[o] = val * g.scale[o];
+      }
+    } else {
+      if (row < g.my && col < g.hx) {
+        const int nx = g.nx, ny = g.ny;
+        const size_t i = row + 1, j = 2 * col + par + 1;
+        if (EPO == EPO_FIELD_ACC_COLFOLD) {
+          g.out[i * nx + j] += val;
+        } else {
+          const T* hb = g.hb;
+          g.out[i * nx + j] = (hb ? hb[i * nx + j] : T(0)) + val;
+          auto ring = [&](size_t ii, size_t jj) {
+            g.out[ii * nx + jj] = hb ? hb[ii * nx + jj] : T(0);
+          };
+          const int jc = 2 * col + par;
+          const bool top = row == 0, bot = row == g.my - 1, lft = jc == 0, rgt = jc == g.mx - 1;
+          if (top) ring(0, j);
+          if (bot) ring(ny - 1, j);
+          if (lft) ring(i, 0);
+          if (rgt) ring(i, nx - 1);
+          if (top && lft) ring(0, 0);
+          if (top && rgt) ring(0, nx - 1);
+          if (bot && lft) ring(ny - 1, 0);
+          if (bot && rgt) ring(ny - 1, nx - 1);
+        }
+      }
+    }
+  }
+}
+
+template <typename T, int AOP, int BOP, int EPO>
+static void gemm_fold(const FoldArgs<T>& a, int M, int N, hipStream_t s) {
+  dim3 grid(N / kTile, M / kTile, 2);
+  hipLaunchKernelGGL((k_gemm_fold<T, AOP, BOP, EPO>), grid, dim3(256), 0, s, a);
+}
+
+template <typename T>
+static void launch_inversion_fold(const Plan& p, const T* F, const T* Fsub, const T* hb, T* out,
+                                  int mode, hipStream_t s) {
+  T* T1 = static_cast<T*>(p.gT[0]);
+  T* T2 = static_cast<T*>(p.gT[1]);
+  FoldArgs<T> a{};
+  a.ny = p.ny; a.nx = p.nx; a.my = p.my; a.mx = p.mx; a.hy = p.hy; a.hx = p.hx;
+  const T* SyF = static_cast<const T*>(p.SyF);
+  const T* SxF = static_cast<const T*>(p.SxF);
+  // T1 = S_y (F - Fsub)
+  a.A = SyF; a.lda = p.HY; a.strideA = (size_t)p.HY * p.HY;
+  a.F = F; a.Fsub = Fsub; a.K = p.HY; a.C = T1; a.ldc = p.MX;
+  gemm_fold<T, AOP_DENSE, BOP_FOLDROW_FIELD, EPO_ROWFOLD>(a, p.HY, p.MX, s);
+  // T2 = (T1 S_x) ./ Lambda
+  a.A = T1; a.lda = p.MX; a.B = SxF; a.ldb = p.HX; a.strideB = (size_t)p.HX * p.HX;
+  a.K = p.HX; a.C = T2; a.ldc = p.MX; a.scale = static_cast<const T*>(p.lam_inv);
+  gemm_fold<T, AOP_FOLDCOL, BOP_DENSE, EPO_SCALE_COLFOLD>(a, p.MY, p.HX, s);
+  // T1 = S_y T2
+  a.A = SyF; a.lda = p.HY; a.B = T2; a.ldb = p.MX; a.K = p.HY; a.C = T1; a.ldc = p.MX;
+  gemm_fold<T, AOP_DENSE, BOP_FOLDROW_DENSE, EPO_ROWFOLD>(a, p.HY, p.MX, s);
+  // out = epilogue(T1 S_x)
+  a.A = T1; a.lda = p.MX; a.B = SxF; a.ldb = p.HX; a.K = p.HX; a.out = out; a.hb = hb;
+  if (mode == INV_ACC)
+    gemm_fold<T, AOP_FOLDCOL, BOP_DENSE, EPO_FIELD_ACC_COLFOLD>(a, p.MY, p.HX, s);
+  else
+    gemm_fold<T, AOP_FOLDCOL, BOP_DENSE, EPO_FIELD_SET_COLFOLD>(a, p.MY, p.HX, s);
+}
+
 template <typename T>
 void launch_inversion(const Plan& p, const T* F, const T* Fsub, const T* hb, T* out, int mode,
                       hipStream_t s) {
+  if (p.fold) {
+    launch_inversion_fold<T>(p, F, Fsub, hb, out, mode, s);
+    return;
+  }
   const T* Sy = static_cast<const T*>(p.Sy);
   const T* Sx = static_cast<const T*>(p.Sx);
   T* T1 = static_cast<T*>(p.gT[0]);

@@ -7,6 +7,7 @@ datlas-ocean/VarDyn); see include/vardyn.h for the contract of each call.
 from __future__ import annotations
 
 import ctypes as C
+import os
 
 import numpy as np
 import torch
@@ -64,7 +65,7 @@ class QgPlan:
 
     def __init__(self, ny, nx, dx, dy, dt, c, f0, g=9.81, time_scheme="rk2",
                  precision="float64", kdiffus=None, mask=None, beta=None, fbathy=None,
-                 mdt=None, device=None):
+                 mdt=None, device=None, dst_dense=None):
         if not torch.cuda.is_available():
             raise VardynError("no HIP device available: the QG plan runs on the GPU only")
         self.device = torch.device("cuda", torch.cuda.current_device() if device is None
@@ -100,6 +101,10 @@ class QgPlan:
         p.beta = host(beta, (self.ny - 2, self.nx), np.float64)
         p.fbathy = host(fbathy, (self.ny, self.nx), np.float64)
         p.mdt = host(mdt, (self.ny, self.nx), np.float64)
+        if dst_dense is None:
+            # VARDYN_DST_DENSE=1 selects the full sine matrices for A/B comparisons
+            dst_dense = os.environ.get("VARDYN_DST_DENSE", "0") == "1"
+        p.dst_dense = 1 if dst_dense else 0
         self.params = p
         handle = C.c_void_p()
         with torch.cuda.device(self.device):
