@@ -46,40 +46,54 @@ def parse():
                     help="shorten the window (profiling only; the default is the full 2880)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-steps", type=int, default=24)
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch kernels eagerly instead of replaying a captured HIP graph")
     ap.add_argument("--check-f32", action="store_true",
                     help="also evaluate the FP32 path and report its gradient error vs FP64")
     return ap.parse_args()
 
 
-def gemm_roofline(plan, precision, n_inv=200):
-    """Average duration of the inversion GEMMs, timed with HIP events on the
-    stream they run on; algorithmic flops 4 m_y m_x (m_y + m_x) per inversion
-    (4 GEMM launches)."""
+def gemm_roofline(plan, precision, n_inv=100, reps=5):
+    """Average duration of the inversion GEMMs on the GPU: n_inv inversions are
+    captured in one HIP graph (so host launch cost is out of the picture) and
+    the replays are timed with HIP events on the stream they run on.
+    Algorithmic flops: 4 m_y m_x (m_y + m_x) per inversion with the full sine
+    matrices; the folded path executes half of them, and ``frac`` is taken on
+    the flops actually executed."""
     import torch
     q = torch.randn(plan.shape, dtype=plan.dtype, device=plan.device) * 1e-6
     hb = torch.zeros_like(q)
     out = torch.empty_like(q)
-    for _ in range(10):
+    for _ in range(3):
         plan.pv2h(q, hb, None, out)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        for _ in range(n_inv):
+            plan.pv2h(q, hb, None, out)
+    g.replay()
     torch.cuda.synchronize()
     s = torch.cuda.current_stream()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record(s)
-    for _ in range(n_inv):
-        plan.pv2h(q, hb, None, out)
+    for _ in range(reps):
+        g.replay()
     e1.record(s)
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1)
     my, mx = plan.ny - 2, plan.nx - 2
-    flops_inv = 4.0 * my * mx * (my + mx)
-    per_launch_s = ms * 1e-3 / (4 * n_inv)
-    achieved = flops_inv / 4 / per_launch_s / 1e12
+    dense_flops_inv = 4.0 * my * mx * (my + mx)
+    folded = plan.folded
+    exec_flops_inv = dense_flops_inv / 2 if folded else dense_flops_inv
+    per_launch_s = ms * 1e-3 / (4 * n_inv * reps)
+    achieved = exec_flops_inv / 4 / per_launch_s / 1e12
     peak = MFMA_PEAK["f64" if precision == "float64" else "f32"]
-    return {"bound": "mfma", "kernel": "k_gemm32 (sine-matrix GEMM of the Helmholtz inversion)",
+    return {"bound": "mfma", "kernel": "k_gemm (sine-matrix GEMMs of the Helmholtz inversion)",
             "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
             "frac": round(achieved / peak, 4), "traffic": None,
             "avg_launch_us": round(per_launch_s * 1e6, 3),
-            "flops_per_launch": flops_inv / 4}
+            "flops_per_launch": exec_flops_inv / 4, "folded_dst": folded,
+            "dense_equivalent_tflops": round(dense_flops_inv / 4 / per_launch_s / 1e12, 3)}
 
 
 def cpu_baseline(case, sample_steps):
@@ -124,14 +138,18 @@ def main():
     log(f"setup {time.time() - t0:.1f}s: {pb.H.n_total} obs at {len(pb.H.t_obs)} times, "
         f"{len(pb.checkpoints) - 1} segments")
     X0 = torch.zeros(eng.nctrl, dtype=torch.float64, device=pb.plan.device)
+    if not args.no_graph:
+        t0 = time.time()
+        eng.capture()
+        log(f"captured one gradient evaluation in a HIP graph ({time.time() - t0:.1f}s)")
     for w in range(args.warmup):
-        J, G = eng.run(X0)
+        J, G = eng.evaluate(X0)
         torch.cuda.synchronize()
         log(f"warmup {w}: J={J.item():.6e}")
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     for _ in range(args.steps):
-        J, G = eng.run(X0)
+        J, G = eng.evaluate(X0)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t1
     ms = dt * 1e3 / args.steps
@@ -150,7 +168,8 @@ def main():
         "config": {"workload": case.name, "grid": [case.ny, case.nx], "dx_m": case.dx,
                    "dt_s": case.dt, "window_steps": n_steps, "time_scheme": case.time_scheme,
                    "obs": pb.H.n_total, "segments": len(pb.checkpoints) - 1,
-                   "control": "initial SSH", "tape": args.tape},
+                   "control": "initial SSH", "tape": args.tape,
+                   "hip_graph": not args.no_graph, "folded_dst": pb.plan.folded},
         "grid_point_steps_per_s": round(case.ny * case.nx * n_steps * value, 1),
         "J": Jv, "grad_norm": gnorm,
         "roofline": roof,

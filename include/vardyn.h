@@ -72,6 +72,7 @@ typedef struct vd_qg_params {
   int dst_dense;         /* 0: fold the sine transform by its symmetry when
                             ny-2 and nx-2 are even (half the GEMM work);
                             1: always the full (n-2)^2 sine matrices */
+  int gemm_ksplit;       /* K groups per GEMM workgroup: 1, 2 or 4 (0 = 4) */
 } vd_qg_params;
 
 typedef struct vd_plan vd_plan;

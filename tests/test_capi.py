@@ -46,3 +46,34 @@ def test_version_and_error_without_gpu():
     p.ny, p.nx, p.precision, p.time_scheme = 4, 4, 0, 1
     h = C.c_void_p()
     assert lib.vd_qg_plan_create(C.byref(p), C.byref(h)) == -2
+
+
+def test_error_paths_without_device():
+    """Null/invalid arguments are rejected before any device work, with a message."""
+    import ctypes as C
+    lib = _lib.load()
+    null = C.c_void_p()
+    assert lib.vd_qg_step(None, null, null, null, 1, None, None) == -1
+    assert lib.vd_qg_step_adj(None, null, null, null, null, 1, null, 0, None) == -1
+    assert lib.vd_qg_step_tgl(None, null, null, null, null, None, 1, None) == -1
+    assert lib.vd_qg_pv2h(None, null, null, null, null, None) == -1
+    assert lib.vd_obs_misfit(None, 0, null, null, null, None, 1.0, None) == -1
+    assert lib.vd_obs_adjoint(None, 0, null, null, null, None) == -1
+    assert lib.vd_dot_f64(10, None, None, 1.0, None, None, None) == -1
+    assert lib.vd_axpby(0, 10, 1.0, None, 1.0, None, None) == -1
+    assert lib.vd_convert(0, None, 1, None, 10, 0, None) == -1
+    assert lib.vd_last_error()  # a message is set
+    # parameter validation of plan creation
+    p = _lib.QgParams()
+    p.ny, p.nx, p.precision, p.time_scheme = 64, 64, 7, 1
+    p.dx = p.dy = 1e4
+    p.dt, p.g, p.f0, p.c = 900.0, 9.81, 1e-4, 2.0
+    h = C.c_void_p()
+    assert lib.vd_qg_plan_create(C.byref(p), C.byref(h)) == -1
+    p.precision, p.time_scheme = 1, 9
+    assert lib.vd_qg_plan_create(C.byref(p), C.byref(h)) == -1
+    p.time_scheme, p.c = 1, -1.0
+    assert lib.vd_qg_plan_create(C.byref(p), C.byref(h)) == -1
+    p.c, p.gemm_ksplit = 2.0, 3
+    assert lib.vd_qg_plan_create(C.byref(p), C.byref(h)) == -1
+    assert b"gemm_ksplit" in lib.vd_last_error()

@@ -29,7 +29,7 @@ class QgParams(C.Structure):
         ("g", C.c_double), ("f0", C.c_double), ("c", C.c_double),
         ("has_kdiffus", C.c_int), ("kdiffus", C.c_double),
         ("mask", C.c_void_p), ("beta", C.c_void_p), ("fbathy", C.c_void_p), ("mdt", C.c_void_p),
-        ("dst_dense", C.c_int),
+        ("dst_dense", C.c_int), ("gemm_ksplit", C.c_int),
     ]
 
 

@@ -334,6 +334,9 @@ int vd_qg_plan_create(const vd_qg_params* prm, vd_plan** out) {
              "time_scheme must be VD_EULER, VD_RK2 or VD_RK3");
   VD_REQUIRE(prm->dx > 0 && prm->dy > 0 && prm->f0 != 0 && prm->c > 0 && prm->g > 0,
              VD_ERR_INVALID_ARG, "dx, dy, c, g must be positive and f0 non-zero");
+  VD_REQUIRE(prm->gemm_ksplit == 0 || prm->gemm_ksplit == 1 || prm->gemm_ksplit == 2 ||
+                 prm->gemm_ksplit == 4,
+             VD_ERR_INVALID_ARG, "gemm_ksplit must be 0, 1, 2 or 4");
   vd_plan* h = new vd_plan();
   Plan& p = h->p;
   p.ny = prm->ny; p.nx = prm->nx;
@@ -341,6 +344,7 @@ int vd_qg_plan_create(const vd_qg_params* prm, vd_plan** out) {
   p.MY = round_up(p.my, kTile); p.MX = round_up(p.mx, kTile);
   p.fold = (!prm->dst_dense && p.my % 2 == 0 && p.mx % 2 == 0) ? 1 : 0;
   p.hy = p.my / 2; p.hx = p.mx / 2;
+  p.gemm_ksplit = prm->gemm_ksplit <= 0 ? 4 : prm->gemm_ksplit;
   p.HY = round_up(p.hy, kTile); p.HX = round_up(p.hx, kTile);
   p.prec = prm->precision;
   p.scheme = prm->time_scheme;

@@ -66,6 +66,7 @@ struct Plan {
   // DST by symmetric folding (even interior sizes): two half-size GEMMs per
   // sine-matrix product instead of one full one.
   int fold = 0;
+  int gemm_ksplit = 4;  // max K groups per GEMM workgroup (1, 2 or 4)
   int hy = 0, hx = 0, HY = 0, HX = 0;
   void* SyF = nullptr;  // [2][HY][HY]: rows 2r+p of S_y, columns 0..hy-1
   void* SxF = nullptr;  // [2][HX][HX]: SxF[p][l][c] = S_x[l][2c+p]

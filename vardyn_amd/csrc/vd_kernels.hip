@@ -34,188 +34,72 @@ template <> struct Mfma<double> {
 };
 
 // ---------------------------------------------------------------------------
-// GEMM C = A * B on 32x32 output tiles, one 16x16 MFMA quadrant per wave.
-// A is dense row-major (lda); B is dense (BSRC_DENSE) or the interior of a
-// (ny, nx) field minus an optional second field (BSRC_FIELD), zero outside
-// (my, mx).  K is a multiple of 32 and all dense operands are zero-padded,
-// so no bounds checks are needed inside the K loop.
+// Sine-matrix GEMMs of the Helmholtz inversion.
+//
+// C = A * B on 32x32 output tiles.  A workgroup holds KS groups of four
+// waves; the four waves of a group own the four 16x16 quadrants of the tile,
+// and group g accumulates the K chunks g, g+KS, g+2KS, ...  (32 wide, staged
+// through LDS with a one-chunk register prefetch).  The KS partial tiles are
+// summed through LDS in a fixed order, so results are deterministic.  More
+// groups give the CU more waves to hide L2 latency with; the product stays
+// one launch.
+//
+// Operands: A dense (AOP_DENSE) or column-folded (AOP_FOLDCOL); B dense,
+// the interior of a field minus an optional second field (BOP_FIELD), or
+// row-folded (BOP_FOLDROW_*).  All dense operands are zero-padded to
+// multiples of 32, so the K loop has no bounds checks.
+//
+// Folding: for an even interior size m, S[j][m-1-k] = (-1)^j S[j][k] and S
+// is symmetric, so with h = m/2 and p = blockIdx.z in {0, 1}
+//   (S X)[2r+p][:] = sum_{k<h} S[2r+p][k] (X[k][:] + (-1)^p X[m-1-k][:])
+//   (X S)[:][2c+p] = sum_{l<h} (X[:][l] + (-1)^p X[:][m-1-l]) S[l][2c+p]
+// and each m x m sine product becomes two h x h products; the loaders form
+// the sums/differences and the epilogues interleave outputs back into
+// natural order.
 // ---------------------------------------------------------------------------
-enum { BSRC_DENSE = 0, BSRC_FIELD = 1 };
-enum { EPI_STORE = 0, EPI_SCALE = 1, EPI_FIELD_SET = 2, EPI_FIELD_ACC = 3 };
+enum { AOP_DENSE = 0, AOP_FOLDCOL = 1 };
+enum { BOP_DENSE = 0, BOP_FIELD = 1, BOP_FOLDROW_FIELD = 2, BOP_FOLDROW_DENSE = 3 };
+enum { EK_STORE = 0, EK_SCALE = 1, EK_FIELD_SET = 2, EK_FIELD_ACC = 3 };
+enum { MAP_PLAIN = 0, MAP_ROWFOLD = 1, MAP_COLFOLD = 2 };
 
 template <typename T>
 struct GemmArgs {
   const T* A;
   int lda;
+  size_t strideA;  // block of parity p at A + p * strideA (AOP_DENSE)
   const T* B;
   int ldb;
-  const T* F;     // field source (BSRC_FIELD)
-  const T* Fsub;  // optional subtrahend field
-  int ny, nx, my, mx;
-  int K;
-  T* C;
-  int ldc;
-  const T* scale;  // EPI_SCALE: elementwise factor, ld = ldc
-  const T* hb;     // EPI_FIELD_SET: ring/offset field (nullptr = 0)
-  T* out;          // EPI_FIELD_*: (ny, nx) destination
-};
-
-template <typename T, int BSRC, int EPI>
-__global__ __launch_bounds__(256) void k_gemm32(GemmArgs<T> g) {
-  typedef typename Mfma<T>::acc_t acc_t;
-  __shared__ T As[kTile][kTile + 1];
-  __shared__ T Bs[kTile][kTile + 1];
-  const int tid = threadIdx.x;
-  const int lane = tid & 63;
-  const int w = tid >> 6;
-  const int wm = w >> 1, wn = w & 1;
-  const int m0 = blockIdx.y * kTile, n0 = blockIdx.x * kTile;
-  const int lr = tid >> 3, lc = (tid & 7) * 4;
-
-  T ra[4], rb[4];
-  auto load = [&](int k0) {
-    const T* pa = g.A + (size_t)(m0 + lr) * g.lda + k0 + lc;
-#pragma unroll
-    for (int u = 0; u < 4; ++u) ra[u] = pa[u];
-    if (BSRC == BSRC_DENSE) {
-      const T* pb = g.B + (size_t)(k0 + lr) * g.ldb + n0 + lc;
-#pragma unroll
-      for (int u = 0; u < 4; ++u) rb[u] = pb[u];
-    } else {
-      const int k = k0 + lr;
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int n = n0 + lc + u;
-        T v = T(0);
-        if (k < g.my && n < g.mx) {
-          const size_t idx = (size_t)(k + 1) * g.nx + (n + 1);
-          v = g.F[idx];
-          if (g.Fsub) v -= g.Fsub[idx];
-        }
-        rb[u] = v;
-      }
-    }
-  };
-
-  // four independent accumulators (K slices interleaved) keep the MFMA
-  // pipeline busy without a serial dependency on one accumulator
-  acc_t acc4[4];
-#pragma unroll
-  for (int c = 0; c < 4; ++c)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) acc4[c][r] = T(0);
-
-  load(0);
-  for (int k0 = 0; k0 < g.K; k0 += kTile) {
-    __syncthreads();
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      As[lr][lc + u] = ra[u];
-      Bs[lr][lc + u] = rb[u];
-    }
-    __syncthreads();
-    if (k0 + kTile < g.K) load(k0 + kTile);
-#pragma unroll
-    for (int kk = 0; kk < kTile / 4; ++kk) {
-      const T a = As[wm * 16 + (lane & 15)][kk * 4 + (lane >> 4)];
-      const T b = Bs[kk * 4 + (lane >> 4)][wn * 16 + (lane & 15)];
-      acc4[kk & 3] = Mfma<T>::run(a, b, acc4[kk & 3]);
-    }
-  }
-  acc_t acc;
-#pragma unroll
-  for (int r = 0; r < 4; ++r) acc[r] = (acc4[0][r] + acc4[1][r]) + (acc4[2][r] + acc4[3][r]);
-
-  const int col = n0 + wn * 16 + (lane & 15);
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int row = m0 + wm * 16 + Mfma<T>::out_row(lane, r);
-    const T val = acc[r];
-    if (EPI == EPI_STORE) {
-      g.C[(size_t)row * g.ldc + col] = val;
-    } else if (EPI == EPI_SCALE) {
-      g.C[(size_t)row * g.ldc + col] = val * g.scale[(size_t)row * g.ldc + col];
-    } else {
-      if (row < g.my && col < g.mx) {
-        const int nx = g.nx, ny = g.ny;
-        const size_t i = row + 1, j = col + 1;
-        if (EPI == EPI_FIELD_ACC) {
-          g.out[i * nx + j] += val;
-        } else {
-          const T* hb = g.hb;
-          g.out[i * nx + j] = (hb ? hb[i * nx + j] : T(0)) + val;
-          // ring: h = hb on rows 0, ny-1 and columns 0, nx-1 (jqgm.py:316-317)
-          auto ring = [&](size_t ii, size_t jj) {
-            g.out[ii * nx + jj] = hb ? hb[ii * nx + jj] : T(0);
-          };
-          const bool top = row == 0, bot = row == g.my - 1, lft = col == 0, rgt = col == g.mx - 1;
-          if (top) ring(0, j);
-          if (bot) ring(ny - 1, j);
-          if (lft) ring(i, 0);
-          if (rgt) ring(i, nx - 1);
-          if (top && lft) ring(0, 0);
-          if (top && rgt) ring(0, nx - 1);
-          if (bot && lft) ring(ny - 1, 0);
-          if (bot && rgt) ring(ny - 1, nx - 1);
-        }
-      }
-    }
-  }
-}
-
-template <typename T, int BSRC, int EPI>
-static void gemm(const GemmArgs<T>& a, int M, int N, hipStream_t s) {
-  dim3 grid(N / kTile, M / kTile);
-  hipLaunchKernelGGL((k_gemm32<T, BSRC, EPI>), grid, dim3(256), 0, s, a);
-}
-
-// ---------------------------------------------------------------------------
-// Folded DST GEMMs.  For an even interior size m, S[j][m-1-k] = (-1)^j S[j][k]
-// (and S is symmetric), so with h = m/2
-//   (S X)[2r+p][:] = sum_{k<h} S[2r+p][k] (X[k][:] + (-1)^p X[m-1-k][:])
-//   (X S)[:][2c+p] = sum_{l<h} (X[:][l] + (-1)^p X[:][m-1-l]) S[l][2c+p]
-// i.e. each m x m sine product becomes two h x h products (p = blockIdx.z):
-// the loaders form the sums/differences, the epilogues interleave the
-// even/odd outputs back into natural order.
-// ---------------------------------------------------------------------------
-enum { AOP_DENSE = 0, AOP_FOLDCOL = 1 };
-enum { BOP_DENSE = 0, BOP_FOLDROW_FIELD = 1, BOP_FOLDROW_DENSE = 2 };
-enum { EPO_ROWFOLD = 0, EPO_SCALE_COLFOLD = 1, EPO_FIELD_SET_COLFOLD = 2, EPO_FIELD_ACC_COLFOLD = 3 };
-
-template <typename T>
-struct FoldArgs {
-  const T* A;
-  int lda;
-  size_t strideA;  // A block of parity p at A + p * strideA (AOP_DENSE)
-  const T* B;
-  int ldb;
-  size_t strideB;  // B block of parity p at B + p * strideB (BOP_DENSE)
-  const T* F;      // BOP_FOLDROW_FIELD: field and optional subtrahend
-  const T* Fsub;
+  size_t strideB;  // block of parity p at B + p * strideB (BOP_DENSE)
+  const T* F;      // field operand (BOP_FIELD, BOP_FOLDROW_FIELD)
+  const T* Fsub;   // optional subtrahend field
   int ny, nx, my, mx, hy, hx;
   int K;
   T* C;
   int ldc;
-  const T* scale;  // EPO_SCALE_COLFOLD, ld = ldc
-  const T* hb;
-  T* out;
+  const T* scale;  // EK_SCALE factor, ld = ldc
+  const T* hb;     // EK_FIELD_SET offset / ring field (nullptr = 0)
+  T* out;          // EK_FIELD_*: (ny, nx) destination
 };
 
-template <typename T, int AOP, int BOP, int EPO>
-__global__ __launch_bounds__(256) void k_gemm_fold(FoldArgs<T> g) {
+template <typename T, int AOP, int BOP, int KIND, int MAP, int KS>
+__global__ __launch_bounds__(256 * KS) void k_gemm(GemmArgs<T> g) {
   typedef typename Mfma<T>::acc_t acc_t;
-  __shared__ T As[kTile][kTile + 1];
-  __shared__ T Bs[kTile][kTile + 1];
+  constexpr int LDT = kTile + 1;
+  __shared__ T sm[KS][2][kTile][LDT];
   const int tid = threadIdx.x;
+  const int grp = tid >> 8;          // K group
+  const int gt = tid & 255;          // thread within the group
   const int lane = tid & 63;
-  const int w = tid >> 6;
-  const int wm = w >> 1, wn = w & 1;
+  const int wq = (tid >> 6) & 3;     // quadrant of the 32x32 tile
+  const int wm = wq >> 1, wn = wq & 1;
   const int par = blockIdx.z;
   const T sgn = par ? T(-1) : T(1);
   const int m0 = blockIdx.y * kTile, n0 = blockIdx.x * kTile;
-  const int lr = tid >> 3, lc = (tid & 7) * 4;
+  const int lr = gt >> 3, lc = (gt & 7) * 4;
   const T* A = g.A + (AOP == AOP_DENSE ? par * g.strideA : 0);
   const T* B = g.B + (BOP == BOP_DENSE ? par * g.strideB : 0);
+  T(*As)[LDT] = sm[grp][0];
+  T(*Bs)[LDT] = sm[grp][1];
 
   T ra[4], rb[4];
   auto load = [&](int k0) {
@@ -239,6 +123,14 @@ __global__ __launch_bounds__(256) void k_gemm_fold(FoldArgs<T> g) {
       } else if (BOP == BOP_FOLDROW_DENSE) {
         rb[u] = k < g.hy ? B[(size_t)k * g.ldb + n] + sgn * B[(size_t)(g.my - 1 - k) * g.ldb + n]
                          : T(0);
+      } else if (BOP == BOP_FIELD) {
+        T v = T(0);
+        if (k < g.my && n < g.mx) {
+          const size_t idx = (size_t)(k + 1) * g.nx + (n + 1);
+          v = g.F[idx];
+          if (g.Fsub) v -= g.Fsub[idx];
+        }
+        rb[u] = v;
       } else {
         T v = T(0);
         if (k < g.hy && n < g.mx) {
@@ -259,8 +151,10 @@ __global__ __launch_bounds__(256) void k_gemm_fold(FoldArgs<T> g) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) acc4[c][r] = T(0);
 
-  load(0);
-  for (int k0 = 0; k0 < g.K; k0 += kTile) {
+  // every group runs the same number of chunks (K is a multiple of 32*KS)
+  const int nchunk = g.K / (kTile * KS);
+  load(grp * kTile);
+  for (int it = 0; it < nchunk; ++it) {
     __syncthreads();
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
@@ -268,7 +162,7 @@ __global__ __launch_bounds__(256) void k_gemm_fold(FoldArgs<T> g) {
       Bs[lr][lc + u] = rb[u];
     }
     __syncthreads();
-    if (k0 + kTile < g.K) load(k0 + kTile);
+    if (it + 1 < nchunk) load(((it + 1) * KS + grp) * kTile);
 #pragma unroll
     for (int kk = 0; kk < kTile / 4; ++kk) {
       const T a = As[wm * 16 + (lane & 15)][kk * 4 + (lane >> 4)];
@@ -276,110 +170,131 @@ __global__ __launch_bounds__(256) void k_gemm_fold(FoldArgs<T> g) {
       acc4[kk & 3] = Mfma<T>::run(a, b, acc4[kk & 3]);
     }
   }
+  T val[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) val[r] = (acc4[0][r] + acc4[1][r]) + (acc4[2][r] + acc4[3][r]);
+
+  if (KS > 1) {
+    // groups 1..KS-1 park their partial tiles in their (now idle) LDS slots
+    __syncthreads();
+    T* park = &sm[grp][0][0][0];
+    if (grp > 0) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) park[(wq * 64 + lane) * 4 + r] = val[r];
+    }
+    __syncthreads();
+    if (grp != 0) return;
+#pragma unroll
+    for (int q = 1; q < KS; ++q) {
+      const T* src = &sm[q][0][0][0];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) val[r] += src[(wq * 64 + lane) * 4 + r];
+    }
+  }
 
   const int col = n0 + wn * 16 + (lane & 15);
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     const int row = m0 + wm * 16 + Mfma<T>::out_row(lane, r);
-    const T val = (acc4[0][r] + acc4[1][r]) + (acc4[2][r] + acc4[3][r]);
-    if (EPO == EPO_ROWFOLD) {
-      if (row < g.hy) g.C[(size_t)(2 * row + par) * g.ldc + col] = val;
-    } else if (EPO == EPO_SCALE_COLFOLD) {
-      if (col < g.hx) {
-        const size_t o = (size_t)row * g.ldc + 2 * col + par;
-        g.C[o] = val * g.scale[o];
-      }
+    const T v = val[r];
+    if (MAP == MAP_ROWFOLD) {
+      if (row < g.hy) g.C[(size_t)(2 * row + par) * g.ldc + col] = v;
+      continue;
+    }
+    const int oc = MAP == MAP_COLFOLD ? 2 * col + par : col;
+    if (KIND == EK_STORE || KIND == EK_SCALE) {
+      if (MAP == MAP_COLFOLD && col >= g.hx) continue;
+      const size_t o = (size_t)row * g.ldc + oc;
+      g.C[o] = KIND == EK_SCALE ? v * g.scale[o] : v;
+      continue;
+    }
+    const bool ok = row < g.my && (MAP == MAP_COLFOLD ? col < g.hx : col < g.mx);
+    if (!ok) continue;
+    const int nx = g.nx, ny = g.ny;
+    const size_t i = row + 1, j = oc + 1;
+    if (KIND == EK_FIELD_ACC) {
+      g.out[i * nx + j] += v;
     } else {
-      if (row < g.my && col < g.hx) {
-        const int nx = g.nx, ny = g.ny;
-        const size_t i = row + 1, j = 2 * col + par + 1;
-        if (EPO == EPO_FIELD_ACC_COLFOLD) {
-          g.out[i * nx + j] += val;
-        } else {
-          const T* hb = g.hb;
-          g.out[i * nx + j] = (hb ? hb[i * nx + j] : T(0)) + val;
-          auto ring = [&](size_t ii, size_t jj) {
-            g.out[ii * nx + jj] = hb ? hb[ii * nx + jj] : T(0);
-          };
-          const int jc = 2 * col + par;
-          const bool top = row == 0, bot = row == g.my - 1, lft = jc == 0, rgt = jc == g.mx - 1;
-          if (top) ring(0, j);
-          if (bot) ring(ny - 1, j);
-          if (lft) ring(i, 0);
-          if (rgt) ring(i, nx - 1);
-          if (top && lft) ring(0, 0);
-          if (top && rgt) ring(0, nx - 1);
-          if (bot && lft) ring(ny - 1, 0);
-          if (bot && rgt) ring(ny - 1, nx - 1);
-        }
-      }
+      const T* hb = g.hb;
+      g.out[i * nx + j] = (hb ? hb[i * nx + j] : T(0)) + v;
+      // ring: h = hb on rows 0, ny-1 and columns 0, nx-1 (jqgm.py:316-317)
+      auto ring = [&](size_t ii, size_t jj) { g.out[ii * nx + jj] = hb ? hb[ii * nx + jj] : T(0); };
+      const bool top = row == 0, bot = row == g.my - 1, lft = oc == 0, rgt = oc == g.mx - 1;
+      if (top) ring(0, j);
+      if (bot) ring(ny - 1, j);
+      if (lft) ring(i, 0);
+      if (rgt) ring(i, nx - 1);
+      if (top && lft) ring(0, 0);
+      if (top && rgt) ring(0, nx - 1);
+      if (bot && lft) ring(ny - 1, 0);
+      if (bot && rgt) ring(ny - 1, nx - 1);
     }
   }
 }
 
-template <typename T, int AOP, int BOP, int EPO>
-static void gemm_fold(const FoldArgs<T>& a, int M, int N, hipStream_t s) {
-  dim3 grid(N / kTile, M / kTile, 2);
-  hipLaunchKernelGGL((k_gemm_fold<T, AOP, BOP, EPO>), grid, dim3(256), 0, s, a);
-}
-
-template <typename T>
-static void launch_inversion_fold(const Plan& p, const T* F, const T* Fsub, const T* hb, T* out,
-                                  int mode, hipStream_t s) {
-  T* T1 = static_cast<T*>(p.gT[0]);
-  T* T2 = static_cast<T*>(p.gT[1]);
-  FoldArgs<T> a{};
-  a.ny = p.ny; a.nx = p.nx; a.my = p.my; a.mx = p.mx; a.hy = p.hy; a.hx = p.hx;
-  const T* SyF = static_cast<const T*>(p.SyF);
-  const T* SxF = static_cast<const T*>(p.SxF);
-  // T1 = S_y (F - Fsub)
-  a.A = SyF; a.lda = p.HY; a.strideA = (size_t)p.HY * p.HY;
-  a.F = F; a.Fsub = Fsub; a.K = p.HY; a.C = T1; a.ldc = p.MX;
-  gemm_fold<T, AOP_DENSE, BOP_FOLDROW_FIELD, EPO_ROWFOLD>(a, p.HY, p.MX, s);
-  // T2 = (T1 S_x) ./ Lambda
-  a.A = T1; a.lda = p.MX; a.B = SxF; a.ldb = p.HX; a.strideB = (size_t)p.HX * p.HX;
-  a.K = p.HX; a.C = T2; a.ldc = p.MX; a.scale = static_cast<const T*>(p.lam_inv);
-  gemm_fold<T, AOP_FOLDCOL, BOP_DENSE, EPO_SCALE_COLFOLD>(a, p.MY, p.HX, s);
-  // T1 = S_y T2
-  a.A = SyF; a.lda = p.HY; a.B = T2; a.ldb = p.MX; a.K = p.HY; a.C = T1; a.ldc = p.MX;
-  gemm_fold<T, AOP_DENSE, BOP_FOLDROW_DENSE, EPO_ROWFOLD>(a, p.HY, p.MX, s);
-  // out = epilogue(T1 S_x)
-  a.A = T1; a.lda = p.MX; a.B = SxF; a.ldb = p.HX; a.K = p.HX; a.out = out; a.hb = hb;
-  if (mode == INV_ACC)
-    gemm_fold<T, AOP_FOLDCOL, BOP_DENSE, EPO_FIELD_ACC_COLFOLD>(a, p.MY, p.HX, s);
+template <typename T, int AOP, int BOP, int KIND, int MAP>
+static void gemm(const GemmArgs<T>& a, int M, int N, int nz, int ksplit, hipStream_t s) {
+  const int nchunk = a.K / kTile;
+  int ks = 1;
+  if (ksplit >= 4 && nchunk % 4 == 0) ks = 4;
+  else if (ksplit >= 2 && nchunk % 2 == 0) ks = 2;
+  dim3 grid(N / kTile, M / kTile, nz);
+  if (ks == 4)
+    hipLaunchKernelGGL((k_gemm<T, AOP, BOP, KIND, MAP, 4>), grid, dim3(1024), 0, s, a);
+  else if (ks == 2)
+    hipLaunchKernelGGL((k_gemm<T, AOP, BOP, KIND, MAP, 2>), grid, dim3(512), 0, s, a);
   else
-    gemm_fold<T, AOP_FOLDCOL, BOP_DENSE, EPO_FIELD_SET_COLFOLD>(a, p.MY, p.HX, s);
+    hipLaunchKernelGGL((k_gemm<T, AOP, BOP, KIND, MAP, 1>), grid, dim3(256), 0, s, a);
 }
 
 template <typename T>
 void launch_inversion(const Plan& p, const T* F, const T* Fsub, const T* hb, T* out, int mode,
                       hipStream_t s) {
+  T* T1 = static_cast<T*>(p.gT[0]);
+  T* T2 = static_cast<T*>(p.gT[1]);
+  const int ks = p.gemm_ksplit;
+  GemmArgs<T> a{};
+  a.ny = p.ny; a.nx = p.nx; a.my = p.my; a.mx = p.mx; a.hy = p.hy; a.hx = p.hx;
+  a.scale = static_cast<const T*>(p.lam_inv);
   if (p.fold) {
-    launch_inversion_fold<T>(p, F, Fsub, hb, out, mode, s);
+    const T* SyF = static_cast<const T*>(p.SyF);
+    const T* SxF = static_cast<const T*>(p.SxF);
+    a.strideA = (size_t)p.HY * p.HY;
+    a.strideB = (size_t)p.HX * p.HX;
+    // T1 = S_y (F - Fsub)
+    a.A = SyF; a.lda = p.HY; a.F = F; a.Fsub = Fsub; a.K = p.HY; a.C = T1; a.ldc = p.MX;
+    gemm<T, AOP_DENSE, BOP_FOLDROW_FIELD, EK_STORE, MAP_ROWFOLD>(a, p.HY, p.MX, 2, ks, s);
+    // T2 = (T1 S_x) ./ Lambda
+    a.A = T1; a.lda = p.MX; a.B = SxF; a.ldb = p.HX; a.K = p.HX; a.C = T2; a.ldc = p.MX;
+    gemm<T, AOP_FOLDCOL, BOP_DENSE, EK_SCALE, MAP_COLFOLD>(a, p.MY, p.HX, 2, ks, s);
+    // T1 = S_y T2
+    a.A = SyF; a.lda = p.HY; a.B = T2; a.ldb = p.MX; a.K = p.HY; a.C = T1; a.ldc = p.MX;
+    gemm<T, AOP_DENSE, BOP_FOLDROW_DENSE, EK_STORE, MAP_ROWFOLD>(a, p.HY, p.MX, 2, ks, s);
+    // out = epilogue(T1 S_x)
+    a.A = T1; a.lda = p.MX; a.B = SxF; a.ldb = p.HX; a.K = p.HX; a.out = out; a.hb = hb;
+    if (mode == INV_ACC)
+      gemm<T, AOP_FOLDCOL, BOP_DENSE, EK_FIELD_ACC, MAP_COLFOLD>(a, p.MY, p.HX, 2, ks, s);
+    else
+      gemm<T, AOP_FOLDCOL, BOP_DENSE, EK_FIELD_SET, MAP_COLFOLD>(a, p.MY, p.HX, 2, ks, s);
     return;
   }
   const T* Sy = static_cast<const T*>(p.Sy);
   const T* Sx = static_cast<const T*>(p.Sx);
-  T* T1 = static_cast<T*>(p.gT[0]);
-  T* T2 = static_cast<T*>(p.gT[1]);
-  GemmArgs<T> a{};
-  a.ny = p.ny; a.nx = p.nx; a.my = p.my; a.mx = p.mx;
   // T1 = S_y (F - Fsub)_interior
   a.A = Sy; a.lda = p.MY; a.F = F; a.Fsub = Fsub; a.K = p.MY; a.C = T1; a.ldc = p.MX;
-  gemm<T, BSRC_FIELD, EPI_STORE>(a, p.MY, p.MX, s);
+  gemm<T, AOP_DENSE, BOP_FIELD, EK_STORE, MAP_PLAIN>(a, p.MY, p.MX, 1, ks, s);
   // T2 = (T1 S_x) ./ Lambda
   a.A = T1; a.lda = p.MX; a.B = Sx; a.ldb = p.MX; a.K = p.MX; a.C = T2; a.ldc = p.MX;
-  a.scale = static_cast<const T*>(p.lam_inv);
-  gemm<T, BSRC_DENSE, EPI_SCALE>(a, p.MY, p.MX, s);
+  gemm<T, AOP_DENSE, BOP_DENSE, EK_SCALE, MAP_PLAIN>(a, p.MY, p.MX, 1, ks, s);
   // T1 = S_y T2
   a.A = Sy; a.lda = p.MY; a.B = T2; a.ldb = p.MX; a.K = p.MY; a.C = T1; a.ldc = p.MX;
-  gemm<T, BSRC_DENSE, EPI_STORE>(a, p.MY, p.MX, s);
+  gemm<T, AOP_DENSE, BOP_DENSE, EK_STORE, MAP_PLAIN>(a, p.MY, p.MX, 1, ks, s);
   // out = epilogue(T1 S_x)
   a.A = T1; a.lda = p.MX; a.B = Sx; a.ldb = p.MX; a.K = p.MX; a.out = out; a.hb = hb;
   if (mode == INV_ACC)
-    gemm<T, BSRC_DENSE, EPI_FIELD_ACC>(a, p.MY, p.MX, s);
+    gemm<T, AOP_DENSE, BOP_DENSE, EK_FIELD_ACC, MAP_PLAIN>(a, p.MY, p.MX, 1, ks, s);
   else
-    gemm<T, BSRC_DENSE, EPI_FIELD_SET>(a, p.MY, p.MX, s);
+    gemm<T, AOP_DENSE, BOP_DENSE, EK_FIELD_SET, MAP_PLAIN>(a, p.MY, p.MX, 1, ks, s);
 }
 
 // ---------------------------------------------------------------------------

@@ -65,7 +65,7 @@ class QgPlan:
 
     def __init__(self, ny, nx, dx, dy, dt, c, f0, g=9.81, time_scheme="rk2",
                  precision="float64", kdiffus=None, mask=None, beta=None, fbathy=None,
-                 mdt=None, device=None, dst_dense=None):
+                 mdt=None, device=None, dst_dense=None, gemm_ksplit=None):
         if not torch.cuda.is_available():
             raise VardynError("no HIP device available: the QG plan runs on the GPU only")
         self.device = torch.device("cuda", torch.cuda.current_device() if device is None
@@ -105,6 +105,10 @@ class QgPlan:
             # VARDYN_DST_DENSE=1 selects the full sine matrices for A/B comparisons
             dst_dense = os.environ.get("VARDYN_DST_DENSE", "0") == "1"
         p.dst_dense = 1 if dst_dense else 0
+        self.folded = (not dst_dense) and (self.ny - 2) % 2 == 0 and (self.nx - 2) % 2 == 0
+        if gemm_ksplit is None:
+            gemm_ksplit = int(os.environ.get("VARDYN_GEMM_KSPLIT", "0"))
+        p.gemm_ksplit = int(gemm_ksplit)
         self.params = p
         handle = C.c_void_p()
         with torch.cuda.device(self.device):
