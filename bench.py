@@ -48,6 +48,10 @@ def parse():
     ap.add_argument("--cpu-sample-steps", type=int, default=24)
     ap.add_argument("--no-graph", action="store_true",
                     help="launch kernels eagerly instead of replaying a captured HIP graph")
+    ap.add_argument("--batch", type=int, default=1,
+                    help="independent 512x512 problems advanced in the same launches")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the shape-based GEMM tile choice instead of timing the candidates")
     ap.add_argument("--check-f32", action="store_true",
                     help="also evaluate the FP32 path and report its gradient error vs FP64")
     return ap.parse_args()
@@ -57,9 +61,9 @@ def gemm_roofline(plan, precision, n_inv=100, reps=5):
     """Average duration of the inversion GEMMs on the GPU: n_inv inversions are
     captured in one HIP graph (so host launch cost is out of the picture) and
     the replays are timed with HIP events on the stream they run on.
-    Algorithmic flops: 4 m_y m_x (m_y + m_x) per inversion with the full sine
-    matrices; the folded path executes half of them, and ``frac`` is taken on
-    the flops actually executed."""
+    Algorithmic flops: 4 m_y m_x (m_y + m_x) per inversion and batch member
+    with the full sine matrices; the folded path executes half of them, and
+    ``frac`` is taken on the flops actually executed."""
     import torch
     q = torch.randn(plan.shape, dtype=plan.dtype, device=plan.device) * 1e-6
     hb = torch.zeros_like(q)
@@ -82,7 +86,7 @@ def gemm_roofline(plan, precision, n_inv=100, reps=5):
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1)
     my, mx = plan.ny - 2, plan.nx - 2
-    dense_flops_inv = 4.0 * my * mx * (my + mx)
+    dense_flops_inv = 4.0 * my * mx * (my + mx) * plan.nb
     folded = plan.folded
     exec_flops_inv = dense_flops_inv / 2 if folded else dense_flops_inv
     per_launch_s = ms * 1e-3 / (4 * n_inv * reps)
@@ -126,18 +130,26 @@ def main():
         sys.exit(2)
     import torch
     from vardyn_amd import synthetic as syn
-    from vardyn_amd.problem import build_problem
+    from vardyn_amd.problem import build_batch, build_problem
 
     case = syn.case_basin512()
     n_steps = args.window_steps or case.n_steps
     t0 = time.time()
     log(f"building problem {case.name}: {case.ny}x{case.nx}, {n_steps} steps, {args.precision}")
-    pb = build_problem(case, precision=args.precision, n_steps=n_steps, control="ic",
-                       tape=args.tape)
+    if args.batch > 1:
+        pb = build_batch(case, args.batch, precision=args.precision, n_steps=n_steps,
+                         tape=args.tape)
+    else:
+        pb = build_problem(case, precision=args.precision, n_steps=n_steps, control="ic",
+                           tape=args.tape)
     eng = pb.engine
+    nb = args.batch
+    t0 = time.time()
+    tiles = pb.plan.autotune(reps=5) if not args.no_autotune else None
+    log(f"GEMM tile configurations {tiles} ({time.time() - t0:.2f}s)")
     log(f"setup {time.time() - t0:.1f}s: {pb.H.n_total} obs at {len(pb.H.t_obs)} times, "
         f"{len(pb.checkpoints) - 1} segments")
-    X0 = torch.zeros(eng.nctrl, dtype=torch.float64, device=pb.plan.device)
+    X0 = torch.zeros(nb * eng.nctrl, dtype=torch.float64, device=pb.plan.device)
     if not args.no_graph:
         t0 = time.time()
         eng.capture()
@@ -145,7 +157,7 @@ def main():
     for w in range(args.warmup):
         J, G = eng.evaluate(X0)
         torch.cuda.synchronize()
-        log(f"warmup {w}: J={J.item():.6e}")
+        log(f"warmup {w}: J={J.sum().item():.6e}")
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     for _ in range(args.steps):
@@ -153,9 +165,10 @@ def main():
     torch.cuda.synchronize()
     dt = time.perf_counter() - t1
     ms = dt * 1e3 / args.steps
-    value = args.steps / dt
-    Jv, gnorm = J.item(), float(torch.linalg.vector_norm(G).item())
-    log(f"timed {args.steps} gradients: {ms:.1f} ms each; J={Jv:.10e} |G|={gnorm:.10e}")
+    value = nb * args.steps / dt
+    Jv, gnorm = J[0].item(), float(torch.linalg.vector_norm(G[0]).item())
+    log(f"timed {args.steps} evaluations of {nb} problem(s): {ms:.1f} ms each; "
+        f"J[0]={Jv:.10e} |G[0]|={gnorm:.10e}")
 
     roof = gemm_roofline(pb.plan, args.precision)
     log(f"roofline: {roof}")
@@ -169,12 +182,14 @@ def main():
                    "dt_s": case.dt, "window_steps": n_steps, "time_scheme": case.time_scheme,
                    "obs": pb.H.n_total, "segments": len(pb.checkpoints) - 1,
                    "control": "initial SSH", "tape": args.tape,
-                   "hip_graph": not args.no_graph, "folded_dst": pb.plan.folded},
+                   "hip_graph": not args.no_graph, "folded_dst": pb.plan.folded,
+                   "batch": nb, "gemm_tiles": tiles},
         "grid_point_steps_per_s": round(case.ny * case.nx * n_steps * value, 1),
+        "ms_per_gradient": round(ms / nb, 2),
         "J": Jv, "grad_norm": gnorm,
         "roofline": roof,
     }
-    if args.check_f32 and args.precision == "float64":
+    if args.check_f32 and args.precision == "float64" and nb == 1:
         pb32 = build_problem(case, precision="float32", n_steps=n_steps, control="ic",
                              tape=args.tape, truth_plan=pb.plan)
         G64 = G.clone()

@@ -9,7 +9,8 @@
  *     VD_ERR_* code; vd_last_error() gives a thread-local message.
  *   - Fields are row-major (ny, nx) arrays in device memory, axis 0 = y,
  *     contiguous in x, of the plan precision (float for VD_F32, double for
- *     VD_F64).  Observation values, weights, misfits and cost accumulators
+ *     VD_F64).  A plan made for nbatch > 1 problems takes (nbatch, ny, nx)
+ *     arrays instead and advances every member in the same launches.  Observation values, weights, misfits and cost accumulators
  *     are double.
  *   - The caller owns every buffer passed in.  A plan owns its constants
  *     (sine matrices, 1/Lambda, masks) and a scratch set, allocated by
@@ -72,7 +73,14 @@ typedef struct vd_qg_params {
   int dst_dense;         /* 0: fold the sine transform by its symmetry when
                             ny-2 and nx-2 are even (half the GEMM work);
                             1: always the full (n-2)^2 sine matrices */
-  int gemm_ksplit;       /* K groups per GEMM workgroup: 1, 2 or 4 (0 = 4) */
+  int gemm_config;       /* GEMM tile configuration: 0 = chosen from the
+                            problem shape; 1..4 forces configuration k-1 of
+                            32x32 (split-K), 64x64, 64x64 (split-K 2), 128x64 */
+  int nbatch;            /* independent problems on this grid (0 or 1 = one);
+                            with nbatch > 1 every field argument of the step
+                            and inversion calls is an (nbatch, ny, nx) array */
+  const double* c_batch; /* (nbatch) gravity-wave speed per member, or NULL
+                            for c everywhere */
 } vd_qg_params;
 
 typedef struct vd_plan vd_plan;
@@ -92,6 +100,13 @@ int vd_qg_plan_destroy(vd_plan* plan);
  * every step plus the final state). */
 int vd_qg_tape_bytes(const vd_plan* plan, int nstep, size_t* bytes);
 
+/* Time every tile configuration of each of the four inversion GEMMs on the
+ * device (`reps` launches each, on `stream`) and keep the fastest in the
+ * plan; `chosen` (may be NULL) receives the four configuration indices.
+ * Synchronises the stream.  No reference counterpart: the reference leaves
+ * kernel choices to XLA. */
+int vd_qg_autotune(vd_plan* plan, int reps, int* chosen, void* stream);
+
 /* Bytes of plan-owned device memory (constants + scratch). */
 int vd_qg_plan_bytes(const vd_plan* plan, size_t* bytes);
 
@@ -105,7 +120,8 @@ int vd_qg_debug_scratch(const vd_plan* plan, int which, void** ptr, int* rows, i
 /* Forward propagation over `nstep` steps: h1 = M(h0; hb).  Replaces
  * Qgm.step (jqgm.py:517-557): q0 = h2pv(h0), q is prognostic, h re-diagnosed
  * by the Helmholtz inversion, NaN written on land in h1.  `tape` may be
- * NULL; otherwise it receives the stage states (vd_qg_tape_bytes). */
+ * NULL; otherwise it receives the stage states (vd_qg_tape_bytes).  `hb`
+ * NULL means a zero boundary field (step_tgl and step_adj take the same). */
 int vd_qg_step(vd_plan* plan, const void* h0, const void* hb, void* h1, int nstep,
                void* tape, void* stream);
 

@@ -151,3 +151,36 @@ def test_nstep_zero():
     ref = orc.step(h0, hb, 0)
     h1 = plan.step(plan.field(h0), plan.field(hb), 0).cpu().numpy()
     assert rel(h1, ref) < 1e-14
+
+
+@pytest.mark.parametrize("scheme", ["Euler", "rk2", "rk3"])
+def test_null_boundary_equals_zero_field(scheme):
+    """hb = NULL (zero boundary) skips h_b/q_b and gives the same fields."""
+    orc, plan, h0, hb, rng = make_pair(40, 48, scheme, "float64")
+    H0, Z = plan.field(h0), plan.field(np.zeros_like(h0))
+    ay = plan.field(rng.standard_normal(h0.shape))
+    dh = plan.field(rng.standard_normal(h0.shape))
+    assert torch.equal(plan.step(H0, None, 5), plan.step(H0, Z, 5))
+    assert torch.equal(plan.step_tgl(dh, H0, None, 5), plan.step_tgl(dh, H0, Z, 5))
+    assert torch.equal(plan.step_adj(ay, H0, None, 5), plan.step_adj(ay, H0, Z, 5))
+    ref = orc.step(h0, np.zeros_like(h0), 5)
+    assert rel(plan.step(H0, None, 5).cpu().numpy(), ref) < 1e-10
+
+
+@pytest.mark.parametrize("cfg", [1, 2, 3, 4])
+def test_gemm_tile_configs_agree(cfg):
+    """Every GEMM tile configuration gives the oracle's inversion (FP64)."""
+    ny, nx = 130, 258
+    rng = np.random.default_rng(3)
+    orc = QgmOracle(dx=5e3, dy=6e3, dt=900.0, c=2.0, f=1e-4, time_scheme="rk2", shape=(ny, nx))
+    plan = QgPlan(ny, nx, 5e3, 6e3, 900.0, 2.0, 1e-4, time_scheme="rk2", precision="float64",
+                  gemm_config=cfg)
+    q = 1e-5 * rng.standard_normal((ny, nx))
+    hb = 0.01 * rng.standard_normal((ny, nx))
+    qb = orc.h2pv(hb, hb)
+    h = plan.pv2h(plan.field(q), plan.field(hb), plan.field(qb)).cpu().numpy()
+    assert rel(h, orc.pv2h(q, hb, qb)) < 1e-12
+    chosen = plan.autotune(reps=2)
+    assert all(0 <= c < 4 for c in chosen)
+    h2 = plan.pv2h(plan.field(q), plan.field(hb), plan.field(qb)).cpu().numpy()
+    assert rel(h2, orc.pv2h(q, hb, qb)) < 1e-12

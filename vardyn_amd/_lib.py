@@ -29,7 +29,8 @@ class QgParams(C.Structure):
         ("g", C.c_double), ("f0", C.c_double), ("c", C.c_double),
         ("has_kdiffus", C.c_int), ("kdiffus", C.c_double),
         ("mask", C.c_void_p), ("beta", C.c_void_p), ("fbathy", C.c_void_p), ("mdt", C.c_void_p),
-        ("dst_dense", C.c_int), ("gemm_ksplit", C.c_int),
+        ("dst_dense", C.c_int), ("gemm_config", C.c_int),
+        ("nbatch", C.c_int), ("c_batch", C.c_void_p),
     ]
 
 
@@ -52,6 +53,7 @@ SIGNATURES = {
     "vd_qg_plan_destroy": (_I, [_VP]),
     "vd_qg_tape_bytes": (_I, [_VP, _I, C.POINTER(_SZ)]),
     "vd_qg_plan_bytes": (_I, [_VP, C.POINTER(_SZ)]),
+    "vd_qg_autotune": (_I, [_VP, _I, C.POINTER(C.c_int), _VP]),
     "vd_qg_debug_scratch": (_I, [_VP, _I, C.POINTER(_VP), C.POINTER(_I), C.POINTER(_I)]),
     "vd_qg_step": (_I, [_VP, _VP, _VP, _VP, _I, _VP, _VP]),
     "vd_qg_step_tgl": (_I, [_VP, _VP, _VP, _VP, _VP, _VP, _I, _VP]),

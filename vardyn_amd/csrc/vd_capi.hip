@@ -32,7 +32,7 @@ static int upload(const std::vector<double>& h, void** dptr) {
 }
 
 static void release_plan_buffers(Plan& p) {
-  hipFree(p.SyF); hipFree(p.SxF);
+  hipFree(p.SyF); hipFree(p.SxF); hipFree(p.gfc2_b);
   hipFree(p.Sy); hipFree(p.Sx); hipFree(p.lam_inv); hipFree(p.mask);
   hipFree(p.beta); hipFree(p.fbathy); hipFree(p.mdt);
   for (auto& g : p.gT) hipFree(g);
@@ -73,16 +73,22 @@ static int build_constants(Plan& p, const vd_qg_params& prm) {
     if ((rc = upload<T>(fx, &p.SxF)) != VD_OK) return rc;
   }
   // 1/Lambda with Lambda = g/f [2(cos(pi(j+1)/(nx-1))-1)/dx^2 + 2(cos(pi(i+1)/(ny-1))-1)/dy^2] - g f/c^2
-  // (jqgm.py:165-172); zero in the padding.
-  std::vector<double> li((size_t)p.MY * p.MX, 0.0);
-  for (int i = 0; i < p.my; ++i)
-    for (int j = 0; j < p.mx; ++j) {
-      const double lap = 2 * (std::cos(M_PI / (p.nx - 1) * (j + 1)) - 1) / (p.dx * p.dx) +
-                         2 * (std::cos(M_PI / (p.ny - 1) * (i + 1)) - 1) / (p.dy * p.dy);
-      const double lam = p.g / p.f0 * lap - p.g * p.f0 / (p.c * p.c);
-      li[(size_t)i * p.MX + j] = 1.0 / lam;
-    }
+  // (jqgm.py:165-172), one table per batch member (c may differ); zero in the padding.
+  std::vector<double> li((size_t)p.nb * p.MY * p.MX, 0.0), gfc2(p.nb);
+  for (int b = 0; b < p.nb; ++b) {
+    const double cb = prm.c_batch ? prm.c_batch[b] : p.c;
+    VD_REQUIRE(cb > 0, VD_ERR_INVALID_ARG, "c_batch values must be positive");
+    gfc2[b] = p.g * p.f0 / (cb * cb);
+    for (int i = 0; i < p.my; ++i)
+      for (int j = 0; j < p.mx; ++j) {
+        const double lap = 2 * (std::cos(M_PI / (p.nx - 1) * (j + 1)) - 1) / (p.dx * p.dx) +
+                           2 * (std::cos(M_PI / (p.ny - 1) * (i + 1)) - 1) / (p.dy * p.dy);
+        const double lam = p.g / p.f0 * lap - gfc2[b];
+        li[((size_t)b * p.MY + i) * p.MX + j] = 1.0 / lam;
+      }
+  }
   if ((rc = upload<T>(li, &p.lam_inv)) != VD_OK) return rc;
+  if ((rc = upload<T>(gfc2, &p.gfc2_b)) != VD_OK) return rc;
 
   std::vector<uint8_t> mask;
   if (prm.mask) {
@@ -119,12 +125,12 @@ static int build_constants(Plan& p, const vd_qg_params& prm) {
     if ((rc = upload<T>(b, &p.mdt)) != VD_OK) return rc;
   }
   for (auto& g : p.gT) {
-    VD_HIP_TRY(hipMalloc(&g, p.ng * sizeof(T)));
-    VD_HIP_TRY(hipMemset(g, 0, p.ng * sizeof(T)));
+    VD_HIP_TRY(hipMalloc(&g, p.nb * p.ng * sizeof(T)));
+    VD_HIP_TRY(hipMemset(g, 0, p.nb * p.ng * sizeof(T)));
   }
   for (int f = 0; f < F_COUNT; ++f) {
-    VD_HIP_TRY(hipMalloc(&p.fld[f], p.nf * sizeof(T)));
-    VD_HIP_TRY(hipMemset(p.fld[f], 0, p.nf * sizeof(T)));
+    VD_HIP_TRY(hipMalloc(&p.fld[f], p.nb * p.nf * sizeof(T)));
+    VD_HIP_TRY(hipMemset(p.fld[f], 0, p.nb * p.nf * sizeof(T)));
   }
   p.nfld = F_COUNT;
   VD_HIP_TRY(hipDeviceSynchronize());
@@ -142,6 +148,7 @@ struct Tape {
   int S;
   T* h(int n, int st) const { return base + ((size_t)(n * S + st) * 2) * nf; }
   T* q(int n, int st) const { return base + ((size_t)(n * S + st) * 2 + 1) * nf; }
+  // nf here is the size of one batched field: nb * ny * nx
 };
 
 // ---------------------------------------------------------------------------
@@ -151,11 +158,17 @@ template <typename T>
 static int qg_forward(Plan& p, const T* h0, const T* hb, T* h1, int nstep, T* tape_ptr,
                       hipStream_t s) {
   const T dt = T(p.dt);
-  Tape<T> tp{tape_ptr, p.nf, p.nstage};
+  Tape<T> tp{tape_ptr, p.nb * p.nf, p.nstage};
   T* hbm = fld<T>(p, F_HBM);
   T* qb = fld<T>(p, F_QB);
   T* hc = tape_ptr ? tp.h(0, 0) : fld<T>(p, F_HA);
   T* qc = tape_ptr ? tp.q(0, 0) : fld<T>(p, F_QA);
+  // A NULL boundary field means zero: then (without an MDT) h_b and q_b vanish
+  // and the inversion skips them.
+  const bool zero_b = hb == nullptr && !p.has_mdt;
+  if (!hb) hb = fld<T>(p, F_ZERO);
+  const T* qbi = zero_b ? nullptr : qb;
+  const T* hbi = zero_b ? nullptr : hbm;
   launch_prep<T>(p, h0, hb, hc, hbm, qc, qb, s);
   for (int n = 0; n < nstep; ++n) {
     T *hn, *qn, *hs1, *qs1, *hs2, *qs2;
@@ -173,19 +186,19 @@ static int qg_forward(Plan& p, const T* h0, const T* hb, T* h1, int nstep, T* ta
       launch_stage<T>(p, hc, qc, qc, nullptr, nullptr, 0, 0, dt, qn, nullptr, s);
     } else if (p.scheme == VD_RK2) {
       launch_stage<T>(p, hc, qc, qc, nullptr, nullptr, 0, 0, T(0.5) * dt, qs1, nullptr, s);
-      launch_inversion<T>(p, qs1, qb, hbm, hs1, INV_FWD, s);
+      launch_inversion<T>(p, qs1, qbi, hbi, hs1, INV_FWD, s);
       launch_stage<T>(p, hs1, qs1, qc, nullptr, nullptr, 0, 0, dt, qn, nullptr, s);
     } else {
       T* k0 = fld<T>(p, F_K0);
       T* k1 = fld<T>(p, F_K1);
       launch_stage<T>(p, hc, qc, qc, nullptr, nullptr, 0, 0, T(0.5) * dt, qs1, k0, s);
-      launch_inversion<T>(p, qs1, qb, hbm, hs1, INV_FWD, s);
+      launch_inversion<T>(p, qs1, qbi, hbi, hs1, INV_FWD, s);
       launch_stage<T>(p, hs1, qs1, qc, k0, nullptr, -dt, 0, T(2) * dt, qs2, k1, s);
-      launch_inversion<T>(p, qs2, qb, hbm, hs2, INV_FWD, s);
+      launch_inversion<T>(p, qs2, qbi, hbi, hs2, INV_FWD, s);
       launch_stage<T>(p, hs2, qs2, qc, k0, k1, dt / T(6), T(4) * dt / T(6), dt / T(6), qn,
                       nullptr, s);
     }
-    launch_inversion<T>(p, qn, qb, hbm, hn, INV_FWD, s);
+    launch_inversion<T>(p, qn, qbi, hbi, hn, INV_FWD, s);
     hc = hn;
     qc = qn;
   }
@@ -208,6 +221,12 @@ static int qg_tangent(Plan& p, const T* dh0, const T* h0, const T* hb, T* dh1, T
   T* qc = fld<T>(p, F_QA);
   T* dhc = fld<T>(p, F_DH);
   T* dqc = fld<T>(p, F_DQ);
+  // A NULL boundary field means zero: then (without an MDT) h_b and q_b vanish
+  // and the inversion skips them.
+  const bool zero_b = hb == nullptr && !p.has_mdt;
+  if (!hb) hb = fld<T>(p, F_ZERO);
+  const T* qbi = zero_b ? nullptr : qb;
+  const T* hbi = zero_b ? nullptr : hbm;
   launch_prep<T>(p, h0, hb, hc, hbm, qc, qb, s);
   launch_prep_tl<T>(p, dh0, hc, dhc, dqc, s);
   T *hs1 = fld<T>(p, F_HS1), *qs1 = fld<T>(p, F_QS1), *hs2 = fld<T>(p, F_HS2),
@@ -227,7 +246,7 @@ static int qg_tangent(Plan& p, const T* dh0, const T* h0, const T* hb, T* dh1, T
       const T h2 = T(0.5) * dt;
       launch_stage<T>(p, hc, qc, qc, nullptr, nullptr, 0, 0, h2, qs1, nullptr, s);
       launch_stage_tl<T>(p, hc, qc, dhc, dqc, dqc, nullptr, nullptr, 0, 0, h2, dqs1, nullptr, s);
-      launch_inversion<T>(p, qs1, qb, hbm, hs1, INV_FWD, s);
+      launch_inversion<T>(p, qs1, qbi, hbi, hs1, INV_FWD, s);
       launch_inversion<T>(p, dqs1, nullptr, nullptr, dhs1, INV_TL, s);
       launch_stage<T>(p, hs1, qs1, qc, nullptr, nullptr, 0, 0, dt, qn, nullptr, s);
       launch_stage_tl<T>(p, hs1, qs1, dhs1, dqs1, dqc, nullptr, nullptr, 0, 0, dt, dqn, nullptr,
@@ -238,18 +257,18 @@ static int qg_tangent(Plan& p, const T* dh0, const T* h0, const T* hb, T* dh1, T
       const T h2 = T(0.5) * dt;
       launch_stage<T>(p, hc, qc, qc, nullptr, nullptr, 0, 0, h2, qs1, k0, s);
       launch_stage_tl<T>(p, hc, qc, dhc, dqc, dqc, nullptr, nullptr, 0, 0, h2, dqs1, dk0, s);
-      launch_inversion<T>(p, qs1, qb, hbm, hs1, INV_FWD, s);
+      launch_inversion<T>(p, qs1, qbi, hbi, hs1, INV_FWD, s);
       launch_inversion<T>(p, dqs1, nullptr, nullptr, dhs1, INV_TL, s);
       launch_stage<T>(p, hs1, qs1, qc, k0, nullptr, -dt, 0, T(2) * dt, qs2, k1, s);
       launch_stage_tl<T>(p, hs1, qs1, dhs1, dqs1, dqc, dk0, nullptr, -dt, 0, T(2) * dt, dqs2, dk1,
                          s);
-      launch_inversion<T>(p, qs2, qb, hbm, hs2, INV_FWD, s);
+      launch_inversion<T>(p, qs2, qbi, hbi, hs2, INV_FWD, s);
       launch_inversion<T>(p, dqs2, nullptr, nullptr, dhs2, INV_TL, s);
       const T c1 = dt / T(6), c2 = T(4) * dt / T(6);
       launch_stage<T>(p, hs2, qs2, qc, k0, k1, c1, c2, c1, qn, nullptr, s);
       launch_stage_tl<T>(p, hs2, qs2, dhs2, dqs2, dqc, dk0, dk1, c1, c2, c1, dqn, nullptr, s);
     }
-    launch_inversion<T>(p, qn, qb, hbm, hn, INV_FWD, s);
+    launch_inversion<T>(p, qn, qbi, hbi, hn, INV_FWD, s);
     launch_inversion<T>(p, dqn, nullptr, nullptr, dhn, INV_TL, s);
     hc = hn; qc = qn; dhc = dhn; dqc = dqn;
   }
@@ -270,7 +289,7 @@ static int qg_adjoint(Plan& p, const T* adh1, const T* h0, const T* hb, T* adh0,
     if ((rc = qg_forward<T>(p, h0, hb, fld<T>(p, F_HOUT), nstep, tape_ptr, s)) != VD_OK) return rc;
   }
   const T dt = T(p.dt);
-  Tape<T> tp{tape_ptr, p.nf, p.nstage};
+  Tape<T> tp{tape_ptr, p.nb * p.nf, p.nstage};
   T* lh = fld<T>(p, F_LH);
   T* lq = fld<T>(p, F_LQ);
   T* lq1 = fld<T>(p, F_LQ1);
@@ -307,6 +326,69 @@ static int qg_adjoint(Plan& p, const T* adh1, const T* h0, const T* hb, T* adh0,
   return VD_OK;
 }
 
+void gemm_stage_shape(const Plan& p, int stage, int* M, int* N, int* K, int* nz) {
+  if (p.fold) {
+    const bool left = stage == 0 || stage == 2;  // S_y from the left
+    *M = left ? p.HY : p.MY;
+    *N = left ? p.MX : p.HX;
+    *K = left ? p.HY : p.HX;
+    *nz = 2 * p.nb;
+  } else {
+    *M = p.MY;
+    *N = p.MX;
+    *K = (stage == 0 || stage == 2) ? p.MY : p.MX;
+    *nz = p.nb;
+  }
+}
+
+// Largest tile that still gives at least one workgroup per CU (256 CUs).
+static int default_tile_cfg(const Plan& p, int stage) {
+  int M, N, K, nz;
+  gemm_stage_shape(p, stage, &M, &N, &K, &nz);
+  const int cand[3] = {3, 1, 0};  // 128x64, 64x64, 32x32
+  const int bm[4] = {32, 64, 64, 128}, bn[4] = {32, 64, 64, 64};
+  for (int c : cand) {
+    if (M % bm[c] || N % bn[c]) continue;
+    if (c == 0 || (long)nz * (M / bm[c]) * (N / bn[c]) >= 256) return c;
+  }
+  return 0;
+}
+
+template <typename T>
+static int autotune(Plan& p, int reps, hipStream_t s) {
+  hipEvent_t e0, e1;
+  VD_HIP_TRY(hipEventCreate(&e0));
+  VD_HIP_TRY(hipEventCreate(&e1));
+  T* F = fld<T>(p, F_HA);
+  T* out = fld<T>(p, F_HB);
+  for (int st = 0; st < 4; ++st) {
+    int M, N, K, nz;
+    gemm_stage_shape(p, st, &M, &N, &K, &nz);
+    const int bm[4] = {32, 64, 64, 128}, bn[4] = {32, 64, 64, 64}, ks[4] = {1, 1, 2, 1};
+    float best = 1e30f;
+    int best_cfg = 0;
+    for (int c = 0; c < kNumTileCfg; ++c) {
+      if (M % bm[c] || N % bn[c] || (K / kTile) % ks[c]) continue;
+      launch_gemm_stage<T>(p, st, F, nullptr, nullptr, out, INV_FWD, c, s);  // warm
+      VD_HIP_TRY(hipEventRecord(e0, s));
+      for (int r = 0; r < reps; ++r) launch_gemm_stage<T>(p, st, F, nullptr, nullptr, out, INV_FWD, c, s);
+      VD_HIP_TRY(hipEventRecord(e1, s));
+      VD_HIP_TRY(hipEventSynchronize(e1));
+      float ms = 0;
+      VD_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best) {
+        best = ms;
+        best_cfg = c;
+      }
+    }
+    p.gemm_cfg[st] = best_cfg;
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  VD_HIP_TRY(hipGetLastError());
+  return VD_OK;
+}
+
 }  // namespace vd
 
 using namespace vd;
@@ -334,9 +416,10 @@ int vd_qg_plan_create(const vd_qg_params* prm, vd_plan** out) {
              "time_scheme must be VD_EULER, VD_RK2 or VD_RK3");
   VD_REQUIRE(prm->dx > 0 && prm->dy > 0 && prm->f0 != 0 && prm->c > 0 && prm->g > 0,
              VD_ERR_INVALID_ARG, "dx, dy, c, g must be positive and f0 non-zero");
-  VD_REQUIRE(prm->gemm_ksplit == 0 || prm->gemm_ksplit == 1 || prm->gemm_ksplit == 2 ||
-                 prm->gemm_ksplit == 4,
-             VD_ERR_INVALID_ARG, "gemm_ksplit must be 0, 1, 2 or 4");
+  VD_REQUIRE(prm->nbatch >= 0 && prm->nbatch <= 4096, VD_ERR_INVALID_ARG,
+             "nbatch must be in 0..4096");
+  VD_REQUIRE(prm->gemm_config >= 0 && prm->gemm_config <= kNumTileCfg, VD_ERR_INVALID_ARG,
+             "gemm_config must be in 0..4");
   vd_plan* h = new vd_plan();
   Plan& p = h->p;
   p.ny = prm->ny; p.nx = prm->nx;
@@ -344,7 +427,8 @@ int vd_qg_plan_create(const vd_qg_params* prm, vd_plan** out) {
   p.MY = round_up(p.my, kTile); p.MX = round_up(p.mx, kTile);
   p.fold = (!prm->dst_dense && p.my % 2 == 0 && p.mx % 2 == 0) ? 1 : 0;
   p.hy = p.my / 2; p.hx = p.mx / 2;
-  p.gemm_ksplit = prm->gemm_ksplit <= 0 ? 4 : prm->gemm_ksplit;
+  const int forced_cfg = prm->gemm_config - 1;
+  p.nb = prm->nbatch <= 0 ? 1 : prm->nbatch;
   p.HY = round_up(p.hy, kTile); p.HX = round_up(p.hx, kTile);
   p.prec = prm->precision;
   p.scheme = prm->time_scheme;
@@ -363,6 +447,8 @@ int vd_qg_plan_create(const vd_qg_params* prm, vd_plan** out) {
     delete h;
     return VD_ERR_HIP;
   }
+  for (int st = 0; st < 4; ++st)
+    p.gemm_cfg[st] = forced_cfg >= 0 ? forced_cfg : default_tile_cfg(p, st);
   int rc = p.prec == VD_F64 ? build_constants<double>(p, *prm) : build_constants<float>(p, *prm);
   if (rc != VD_OK) {
     const std::string msg = g_last_error;
@@ -386,19 +472,31 @@ int vd_qg_tape_bytes(const vd_plan* h, int nstep, size_t* bytes) {
   VD_REQUIRE(h && bytes, VD_ERR_INVALID_ARG, "null argument");
   VD_REQUIRE(nstep >= 0, VD_ERR_INVALID_ARG, "nstep must be >= 0");
   const Plan& p = h->p;
-  *bytes = ((size_t)nstep * p.nstage + 1) * 2 * p.nf * p.es;
+  *bytes = ((size_t)nstep * p.nstage + 1) * 2 * p.nb * p.nf * p.es;
   return VD_OK;
 }
 
 int vd_qg_plan_bytes(const vd_plan* h, size_t* bytes) {
   VD_REQUIRE(h && bytes, VD_ERR_INVALID_ARG, "null argument");
   const Plan& p = h->p;
-  size_t b = ((size_t)p.MY * p.MY + (size_t)p.MX * p.MX + p.ng * 4) * p.es + p.nf +
-             (size_t)F_COUNT * p.nf * p.es;
+  size_t b = ((size_t)p.MY * p.MY + (size_t)p.MX * p.MX + p.nb * p.ng * 4) * p.es + p.nf +
+             (size_t)F_COUNT * p.nb * p.nf * p.es;
   if (p.has_beta) b += p.nf * p.es;
   if (p.has_bathy) b += p.nf * p.es;
   if (p.has_mdt) b += p.nf * p.es;
   *bytes = b;
+  return VD_OK;
+}
+
+int vd_qg_autotune(vd_plan* h, int reps, int* chosen, void* stream) {
+  VD_REQUIRE(h, VD_ERR_INVALID_ARG, "null argument");
+  VD_REQUIRE(reps >= 1 && reps <= 1000, VD_ERR_INVALID_ARG, "reps must be in 1..1000");
+  Plan& p = h->p;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int rc = p.prec == VD_F64 ? autotune<double>(p, reps, s) : autotune<float>(p, reps, s);
+  if (rc != VD_OK) return rc;
+  if (chosen)
+    for (int st = 0; st < 4; ++st) chosen[st] = p.gemm_cfg[st];
   return VD_OK;
 }
 
@@ -413,7 +511,7 @@ int vd_qg_debug_scratch(const vd_plan* h, int which, void** ptr, int* rows, int*
 
 int vd_qg_step(vd_plan* h, const void* h0, const void* hb, void* h1, int nstep, void* tape,
                void* stream) {
-  VD_REQUIRE(h && h0 && hb && h1, VD_ERR_INVALID_ARG, "null argument");
+  VD_REQUIRE(h && h0 && h1, VD_ERR_INVALID_ARG, "null argument");
   VD_REQUIRE(nstep >= 0, VD_ERR_INVALID_ARG, "nstep must be >= 0");
   Plan& p = h->p;
   hipStream_t s = static_cast<hipStream_t>(stream);
@@ -426,7 +524,7 @@ int vd_qg_step(vd_plan* h, const void* h0, const void* hb, void* h1, int nstep, 
 
 int vd_qg_step_tgl(vd_plan* h, const void* dh0, const void* h0, const void* hb, void* dh1,
                    void* h1, int nstep, void* stream) {
-  VD_REQUIRE(h && dh0 && h0 && hb && dh1, VD_ERR_INVALID_ARG, "null argument");
+  VD_REQUIRE(h && dh0 && h0 && dh1, VD_ERR_INVALID_ARG, "null argument");
   VD_REQUIRE(nstep >= 0, VD_ERR_INVALID_ARG, "nstep must be >= 0");
   Plan& p = h->p;
   hipStream_t s = static_cast<hipStream_t>(stream);
@@ -439,7 +537,7 @@ int vd_qg_step_tgl(vd_plan* h, const void* dh0, const void* h0, const void* hb, 
 
 int vd_qg_step_adj(vd_plan* h, const void* adh1, const void* h0, const void* hb, void* adh0,
                    int nstep, void* tape, int tape_valid, void* stream) {
-  VD_REQUIRE(h && adh1 && h0 && hb && adh0 && tape, VD_ERR_INVALID_ARG, "null argument");
+  VD_REQUIRE(h && adh1 && h0 && adh0 && tape, VD_ERR_INVALID_ARG, "null argument");
   VD_REQUIRE(nstep >= 0, VD_ERR_INVALID_ARG, "nstep must be >= 0");
   Plan& p = h->p;
   hipStream_t s = static_cast<hipStream_t>(stream);

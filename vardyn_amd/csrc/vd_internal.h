@@ -30,7 +30,8 @@ void set_error(const std::string& msg);
     }                                \
   } while (0)
 
-constexpr int kTile = 32;  // GEMM output tile (rows and columns) and K chunk
+constexpr int kTile = 32;      // smallest GEMM output tile and the K chunk
+constexpr int kNumTileCfg = 4;  // GEMM tile configurations (see kTileCfgs)
 
 inline int round_up(int v, int m) { return (v + m - 1) / m * m; }
 
@@ -42,7 +43,8 @@ struct Phys {
   T cv;       //  g/f/(4 dx)
   T inv6dx, inv6dy;
   T kx, ky;   // Kdiffus/dx^2, Kdiffus/dy^2 (0 when absent)
-  T gfc2;     // g f / c^2
+  const T* gfc2_b;  // per batch member: g f / c^2
+  size_t nf;        // ny * nx: stride between batch members of a field
   T gf_dx2, gf_dy2;  // g/f/dx^2, g/f/dy^2
   const uint8_t* mask;  // 0 land, 1 ring/coast, 2 next to ring/coast, 3 active
   const T* beta;        // (ny, nx) beta at T points, or nullptr (f plane)
@@ -53,6 +55,7 @@ struct Phys {
 // Device-side state of one plan.  Buffers are typed by the plan precision.
 struct Plan {
   int ny, nx, my, mx, MY, MX;
+  int nb = 1;   // batch members (independent problems on the same grid)
   int prec;     // VD_F32 / VD_F64
   int scheme;   // VD_EULER / VD_RK2 / VD_RK3
   int nstage;
@@ -66,7 +69,7 @@ struct Plan {
   // DST by symmetric folding (even interior sizes): two half-size GEMMs per
   // sine-matrix product instead of one full one.
   int fold = 0;
-  int gemm_ksplit = 4;  // max K groups per GEMM workgroup (1, 2 or 4)
+  int gemm_cfg[4] = {0, 0, 0, 0};  // tile configuration of the four inversion GEMMs
   int hy = 0, hx = 0, HY = 0, HX = 0;
   void* SyF = nullptr;  // [2][HY][HY]: rows 2r+p of S_y, columns 0..hy-1
   void* SxF = nullptr;  // [2][HX][HX]: SxF[p][l][c] = S_x[l][2c+p]
@@ -74,7 +77,8 @@ struct Plan {
   // constants
   void* Sy = nullptr;
   void* Sx = nullptr;
-  void* lam_inv = nullptr;
+  void* lam_inv = nullptr;  // [nb][MY][MX]
+  void* gfc2_b = nullptr;   // [nb]
   uint8_t* mask = nullptr;
   void* beta = nullptr;
   void* fbathy = nullptr;
@@ -94,6 +98,7 @@ enum Field {
   F_LH, F_LQ, F_LQ1, F_LQ2, F_LHS,
   F_W0, F_W1, F_W2, F_W3, F_W4, F_W5, F_W6,
   F_HOUT,
+  F_ZERO,  // never written: the zero boundary field
   F_COUNT
 };
 
@@ -106,6 +111,12 @@ enum InvMode { INV_FWD = 0, INV_TL = 1, INV_ACC = 2 };
 template <typename T>
 void launch_inversion(const Plan& p, const T* F, const T* Fsub, const T* hb, T* out, int mode,
                       hipStream_t s);
+// One of the four GEMMs of the inversion with tile configuration cfg.
+template <typename T>
+void launch_gemm_stage(const Plan& p, int stage, const T* F, const T* Fsub, const T* hb, T* out,
+                       int mode, int cfg, hipStream_t s);
+// (M, N, K, products) of GEMM stage `stage` of a plan.
+void gemm_stage_shape(const Plan& p, int stage, int* M, int* N, int* K, int* nz);
 
 // Prepare a segment: hcur = h0 + mdt, hbm = hb + mdt, q0 = h2pv(hcur, hbm), qb = h2pv(hbm, hbm).
 template <typename T>

@@ -79,77 +79,131 @@ struct GemmArgs {
   const T* scale;  // EK_SCALE factor, ld = ldc
   const T* hb;     // EK_FIELD_SET offset / ring field (nullptr = 0)
   T* out;          // EK_FIELD_*: (ny, nx) destination
+  // batch strides (elements) of each operand; 0 for operands shared by all members
+  size_t bsA, bsB, bsF, bsC, bsScale, bsOut;
+  // launch shape: nz products (parities x members), tm x tn tiles each, nb members
+  int nz, tm, tn, nb;
 };
 
-template <typename T, int AOP, int BOP, int KIND, int MAP, int KS>
+template <typename T, int AOP, int BOP, int KIND, int MAP, int BM, int BN, int KS>
 __global__ __launch_bounds__(256 * KS) void k_gemm(GemmArgs<T> g) {
   typedef typename Mfma<T>::acc_t acc_t;
   constexpr int LDT = kTile + 1;
-  __shared__ T sm[KS][2][kTile][LDT];
+  constexpr int MT = BM / 32, NT = BN / 32;          // 16x16 MFMA blocks per wave
+  constexpr int ASZ = BM * LDT, BSZ = kTile * (BN + 1);
+  static_assert(KS == 1 || BM * BN <= ASZ + BSZ, "split-K partials must fit the staging area");
+  __shared__ T sm[KS][ASZ + BSZ];
   const int tid = threadIdx.x;
   const int grp = tid >> 8;          // K group
   const int gt = tid & 255;          // thread within the group
   const int lane = tid & 63;
-  const int wq = (tid >> 6) & 3;     // quadrant of the 32x32 tile
+  const int wq = (tid >> 6) & 3;     // quadrant of the tile owned by this wave
   const int wm = wq >> 1, wn = wq & 1;
-  const int par = blockIdx.z;
+  // Work item of this workgroup.  Workgroups are dispatched round-robin over
+  // the 8 XCDs (linear id % 8), each with its own L2; the mapping gives every
+  // XCD whole products (or a contiguous run of row-panels of one product) so
+  // that its L2 holds one member's operands.
+  int z, tile;
+  {
+    const int L = blockIdx.x, T_ = g.tm * g.tn;
+    const int x = L & 7, slot = L >> 3;
+    if (g.nz % 8 == 0) {
+      z = x + 8 * (slot / T_);
+      tile = slot % T_;
+    } else if (8 % g.nz == 0 && T_ % (8 / g.nz) == 0) {
+      const int per = 8 / g.nz;
+      z = x % g.nz;
+      tile = (x / g.nz) * (T_ / per) + slot;
+    } else {
+      z = L / T_;
+      tile = L % T_;
+    }
+  }
+  // z = parity * nb + member for folded products, member otherwise
+  constexpr bool FOLD = MAP != MAP_PLAIN;
+  const int par = FOLD ? z / g.nb : 0;
+  const size_t mb = FOLD ? z % g.nb : z;
   const T sgn = par ? T(-1) : T(1);
-  const int m0 = blockIdx.y * kTile, n0 = blockIdx.x * kTile;
-  const int lr = gt >> 3, lc = (gt & 7) * 4;
-  const T* A = g.A + (AOP == AOP_DENSE ? par * g.strideA : 0);
-  const T* B = g.B + (BOP == BOP_DENSE ? par * g.strideB : 0);
-  T(*As)[LDT] = sm[grp][0];
-  T(*Bs)[LDT] = sm[grp][1];
+  const int m0 = (tile / g.tn) * BM, n0 = (tile % g.tn) * BN;
+  const T* A = g.A + mb * g.bsA + (AOP == AOP_DENSE ? par * g.strideA : 0);
+  const T* B = g.B + mb * g.bsB + (BOP == BOP_DENSE ? par * g.strideB : 0);
+  const T* Fld = g.F + mb * g.bsF;
+  const T* Fsub = g.Fsub ? g.Fsub + mb * g.bsF : nullptr;
+  T* Cout = g.C + mb * g.bsC;
+  const T* scale = g.scale + mb * g.bsScale;
+  T* fout = g.out + mb * g.bsOut;
+  const T* hbm = g.hb ? g.hb + mb * g.bsOut : nullptr;
+  T* As = &sm[grp][0];               // [BM][LDT]
+  T* Bs = &sm[grp][ASZ];             // [kTile][BN + 1]
 
-  T ra[4], rb[4];
+  // A chunk: BM x 32, four consecutive k per thread, 32 rows per pass.
+  // B chunk: 32 x BN, four consecutive n per thread, 1024/BN rows per pass.
+  constexpr int APASS = BM / 32, BPASS = BN / 32, BROWS = 1024 / BN;
+  const int ar = gt >> 3, ac = (gt & 7) * 4;
+  const int br = gt / (BN / 4), bc = (gt % (BN / 4)) * 4;
+  T ra[APASS][4], rb[BPASS][4];
   auto load = [&](int k0) {
-    const int i = m0 + lr;
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int l = k0 + lc + u;
-      if (AOP == AOP_DENSE) {
-        ra[u] = A[(size_t)i * g.lda + l];
-      } else {
-        ra[u] = l < g.hx ? A[(size_t)i * g.lda + l] + sgn * A[(size_t)i * g.lda + (g.mx - 1 - l)]
-                         : T(0);
+    for (int ps = 0; ps < APASS; ++ps) {
+      const int i = m0 + ar + 32 * ps;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int l = k0 + ac + u;
+        if (AOP == AOP_DENSE) {
+          ra[ps][u] = A[(size_t)i * g.lda + l];
+        } else {
+          ra[ps][u] = l < g.hx ? A[(size_t)i * g.lda + l] +
+                                     sgn * A[(size_t)i * g.lda + (g.mx - 1 - l)]
+                               : T(0);
+        }
       }
     }
-    const int k = k0 + lr;
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int n = n0 + lc + u;
-      if (BOP == BOP_DENSE) {
-        rb[u] = B[(size_t)k * g.ldb + n];
-      } else if (BOP == BOP_FOLDROW_DENSE) {
-        rb[u] = k < g.hy ? B[(size_t)k * g.ldb + n] + sgn * B[(size_t)(g.my - 1 - k) * g.ldb + n]
-                         : T(0);
-      } else if (BOP == BOP_FIELD) {
-        T v = T(0);
-        if (k < g.my && n < g.mx) {
-          const size_t idx = (size_t)(k + 1) * g.nx + (n + 1);
-          v = g.F[idx];
-          if (g.Fsub) v -= g.Fsub[idx];
+    for (int ps = 0; ps < BPASS; ++ps) {
+      const int k = k0 + br + BROWS * ps;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int n = n0 + bc + u;
+        T v;
+        if (BOP == BOP_DENSE) {
+          v = B[(size_t)k * g.ldb + n];
+        } else if (BOP == BOP_FOLDROW_DENSE) {
+          v = k < g.hy ? B[(size_t)k * g.ldb + n] + sgn * B[(size_t)(g.my - 1 - k) * g.ldb + n]
+                       : T(0);
+        } else if (BOP == BOP_FIELD) {
+          v = T(0);
+          if (k < g.my && n < g.mx) {
+            const size_t idx = (size_t)(k + 1) * g.nx + (n + 1);
+            v = Fld[idx];
+            if (Fsub) v -= Fsub[idx];
+          }
+        } else {
+          v = T(0);
+          if (k < g.hy && n < g.mx) {
+            const size_t i1 = (size_t)(k + 1) * g.nx + (n + 1);
+            const size_t i2 = (size_t)(g.my - k) * g.nx + (n + 1);
+            T a = Fld[i1], b = Fld[i2];
+            if (Fsub) { a -= Fsub[i1]; b -= Fsub[i2]; }
+            v = a + sgn * b;
+          }
         }
-        rb[u] = v;
-      } else {
-        T v = T(0);
-        if (k < g.hy && n < g.mx) {
-          const size_t i1 = (size_t)(k + 1) * g.nx + (n + 1);
-          const size_t i2 = (size_t)(g.my - k) * g.nx + (n + 1);
-          T a = g.F[i1], b = g.F[i2];
-          if (g.Fsub) { a -= g.Fsub[i1]; b -= g.Fsub[i2]; }
-          v = a + sgn * b;
-        }
-        rb[u] = v;
+        rb[ps][u] = v;
       }
     }
   };
 
-  acc_t acc4[4];
+  // accumulators: MT x NT blocks; a single block keeps four K-interleaved
+  // accumulators so that consecutive MFMAs do not depend on each other
+  constexpr int NACC = (MT * NT == 1) ? 4 : 1;
+  acc_t acc[MT][NT][NACC];
 #pragma unroll
-  for (int c = 0; c < 4; ++c)
+  for (int a_ = 0; a_ < MT; ++a_)
 #pragma unroll
-    for (int r = 0; r < 4; ++r) acc4[c][r] = T(0);
+    for (int b_ = 0; b_ < NT; ++b_)
+#pragma unroll
+      for (int c_ = 0; c_ < NACC; ++c_)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[a_][b_][c_][r] = T(0);
 
   // every group runs the same number of chunks (K is a multiple of 32*KS)
   const int nchunk = g.K / (kTile * KS);
@@ -157,144 +211,238 @@ __global__ __launch_bounds__(256 * KS) void k_gemm(GemmArgs<T> g) {
   for (int it = 0; it < nchunk; ++it) {
     __syncthreads();
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      As[lr][lc + u] = ra[u];
-      Bs[lr][lc + u] = rb[u];
-    }
+    for (int ps = 0; ps < APASS; ++ps)
+#pragma unroll
+      for (int u = 0; u < 4; ++u) As[(ar + 32 * ps) * LDT + ac + u] = ra[ps][u];
+#pragma unroll
+    for (int ps = 0; ps < BPASS; ++ps)
+#pragma unroll
+      for (int u = 0; u < 4; ++u) Bs[(br + BROWS * ps) * (BN + 1) + bc + u] = rb[ps][u];
     __syncthreads();
     if (it + 1 < nchunk) load(((it + 1) * KS + grp) * kTile);
 #pragma unroll
     for (int kk = 0; kk < kTile / 4; ++kk) {
-      const T a = As[wm * 16 + (lane & 15)][kk * 4 + (lane >> 4)];
-      const T b = Bs[kk * 4 + (lane >> 4)][wn * 16 + (lane & 15)];
-      acc4[kk & 3] = Mfma<T>::run(a, b, acc4[kk & 3]);
+      T av[MT], bv[NT];
+#pragma unroll
+      for (int a_ = 0; a_ < MT; ++a_)
+        av[a_] = As[(wm * (BM / 2) + a_ * 16 + (lane & 15)) * LDT + kk * 4 + (lane >> 4)];
+#pragma unroll
+      for (int b_ = 0; b_ < NT; ++b_)
+        bv[b_] = Bs[(kk * 4 + (lane >> 4)) * (BN + 1) + wn * (BN / 2) + b_ * 16 + (lane & 15)];
+#pragma unroll
+      for (int a_ = 0; a_ < MT; ++a_)
+#pragma unroll
+        for (int b_ = 0; b_ < NT; ++b_) {
+          const int c_ = NACC == 4 ? (kk & 3) : 0;
+          acc[a_][b_][c_] = Mfma<T>::run(av[a_], bv[b_], acc[a_][b_][c_]);
+        }
     }
   }
-  T val[4];
+  T val[MT][NT][4];
 #pragma unroll
-  for (int r = 0; r < 4; ++r) val[r] = (acc4[0][r] + acc4[1][r]) + (acc4[2][r] + acc4[3][r]);
+  for (int a_ = 0; a_ < MT; ++a_)
+#pragma unroll
+    for (int b_ = 0; b_ < NT; ++b_)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        T v = acc[a_][b_][0][r];
+        if (NACC == 4) v = (acc[a_][b_][0][r] + acc[a_][b_][NACC > 1 ? 1 : 0][r]) +
+                           (acc[a_][b_][NACC > 2 ? 2 : 0][r] + acc[a_][b_][NACC > 3 ? 3 : 0][r]);
+        val[a_][b_][r] = v;
+      }
 
   if (KS > 1) {
     // groups 1..KS-1 park their partial tiles in their (now idle) LDS slots
     __syncthreads();
-    T* park = &sm[grp][0][0][0];
+    T* park = &sm[grp][0];
+    constexpr int PER = MT * NT * 4;
     if (grp > 0) {
 #pragma unroll
-      for (int r = 0; r < 4; ++r) park[(wq * 64 + lane) * 4 + r] = val[r];
+      for (int a_ = 0; a_ < MT; ++a_)
+#pragma unroll
+        for (int b_ = 0; b_ < NT; ++b_)
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            park[(wq * 64 + lane) * PER + (a_ * NT + b_) * 4 + r] = val[a_][b_][r];
     }
     __syncthreads();
     if (grp != 0) return;
 #pragma unroll
     for (int q = 1; q < KS; ++q) {
-      const T* src = &sm[q][0][0][0];
+      const T* src = &sm[q][0];
 #pragma unroll
-      for (int r = 0; r < 4; ++r) val[r] += src[(wq * 64 + lane) * 4 + r];
+      for (int a_ = 0; a_ < MT; ++a_)
+#pragma unroll
+        for (int b_ = 0; b_ < NT; ++b_)
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            val[a_][b_][r] += src[(wq * 64 + lane) * PER + (a_ * NT + b_) * 4 + r];
     }
   }
 
-  const int col = n0 + wn * 16 + (lane & 15);
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int row = m0 + wm * 16 + Mfma<T>::out_row(lane, r);
-    const T v = val[r];
-    if (MAP == MAP_ROWFOLD) {
-      if (row < g.hy) g.C[(size_t)(2 * row + par) * g.ldc + col] = v;
-      continue;
-    }
-    const int oc = MAP == MAP_COLFOLD ? 2 * col + par : col;
-    if (KIND == EK_STORE || KIND == EK_SCALE) {
-      if (MAP == MAP_COLFOLD && col >= g.hx) continue;
-      const size_t o = (size_t)row * g.ldc + oc;
-      g.C[o] = KIND == EK_SCALE ? v * g.scale[o] : v;
-      continue;
-    }
-    const bool ok = row < g.my && (MAP == MAP_COLFOLD ? col < g.hx : col < g.mx);
-    if (!ok) continue;
-    const int nx = g.nx, ny = g.ny;
-    const size_t i = row + 1, j = oc + 1;
-    if (KIND == EK_FIELD_ACC) {
-      g.out[i * nx + j] += v;
-    } else {
-      const T* hb = g.hb;
-      g.out[i * nx + j] = (hb ? hb[i * nx + j] : T(0)) + v;
-      // ring: h = hb on rows 0, ny-1 and columns 0, nx-1 (jqgm.py:316-317)
-      auto ring = [&](size_t ii, size_t jj) { g.out[ii * nx + jj] = hb ? hb[ii * nx + jj] : T(0); };
-      const bool top = row == 0, bot = row == g.my - 1, lft = oc == 0, rgt = oc == g.mx - 1;
-      if (top) ring(0, j);
-      if (bot) ring(ny - 1, j);
-      if (lft) ring(i, 0);
-      if (rgt) ring(i, nx - 1);
-      if (top && lft) ring(0, 0);
-      if (top && rgt) ring(0, nx - 1);
-      if (bot && lft) ring(ny - 1, 0);
-      if (bot && rgt) ring(ny - 1, nx - 1);
-    }
-  }
+  for (int a_ = 0; a_ < MT; ++a_)
+#pragma unroll
+    for (int b_ = 0; b_ < NT; ++b_)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wm * (BM / 2) + a_ * 16 + Mfma<T>::out_row(lane, r);
+        const int col = n0 + wn * (BN / 2) + b_ * 16 + (lane & 15);
+        const T v = val[a_][b_][r];
+        if (MAP == MAP_ROWFOLD) {
+          if (row < g.hy) Cout[(size_t)(2 * row + par) * g.ldc + col] = v;
+          continue;
+        }
+        const int oc = MAP == MAP_COLFOLD ? 2 * col + par : col;
+        if (KIND == EK_STORE || KIND == EK_SCALE) {
+          if (MAP == MAP_COLFOLD && col >= g.hx) continue;
+          const size_t o = (size_t)row * g.ldc + oc;
+          Cout[o] = KIND == EK_SCALE ? v * scale[o] : v;
+          continue;
+        }
+        const bool ok = row < g.my && (MAP == MAP_COLFOLD ? col < g.hx : col < g.mx);
+        if (!ok) continue;
+        const int nx = g.nx, ny = g.ny;
+        const size_t i = row + 1, j = oc + 1;
+        if (KIND == EK_FIELD_ACC) {
+          fout[i * nx + j] += v;
+        } else {
+          const T* hb = hbm;
+          fout[i * nx + j] = (hb ? hb[i * nx + j] : T(0)) + v;
+          // ring: h = hb on rows 0, ny-1 and columns 0, nx-1 (jqgm.py:316-317)
+          auto ring = [&](size_t ii, size_t jj) {
+            fout[ii * nx + jj] = hb ? hb[ii * nx + jj] : T(0);
+          };
+          const bool top = row == 0, bot = row == g.my - 1, lft = oc == 0, rgt = oc == g.mx - 1;
+          if (top) ring(0, j);
+          if (bot) ring(ny - 1, j);
+          if (lft) ring(i, 0);
+          if (rgt) ring(i, nx - 1);
+          if (top && lft) ring(0, 0);
+          if (top && rgt) ring(0, nx - 1);
+          if (bot && lft) ring(ny - 1, 0);
+          if (bot && rgt) ring(ny - 1, nx - 1);
+        }
+      }
 }
 
+// Tile configurations (BM x BN, K groups) the dispatcher and the autotuner
+// choose from.
+struct TileCfg {
+  int bm, bn, ks;
+};
+constexpr TileCfg kTileCfgs[kNumTileCfg] = {{32, 32, 4}, {64, 64, 1}, {64, 64, 2}, {128, 64, 1}};
+
 template <typename T, int AOP, int BOP, int KIND, int MAP>
-static void gemm(const GemmArgs<T>& a, int M, int N, int nz, int ksplit, hipStream_t s) {
-  const int nchunk = a.K / kTile;
-  int ks = 1;
-  if (ksplit >= 4 && nchunk % 4 == 0) ks = 4;
-  else if (ksplit >= 2 && nchunk % 2 == 0) ks = 2;
-  dim3 grid(N / kTile, M / kTile, nz);
-  if (ks == 4)
-    hipLaunchKernelGGL((k_gemm<T, AOP, BOP, KIND, MAP, 4>), grid, dim3(1024), 0, s, a);
-  else if (ks == 2)
-    hipLaunchKernelGGL((k_gemm<T, AOP, BOP, KIND, MAP, 2>), grid, dim3(512), 0, s, a);
-  else
-    hipLaunchKernelGGL((k_gemm<T, AOP, BOP, KIND, MAP, 1>), grid, dim3(256), 0, s, a);
+static void gemm(GemmArgs<T> a, int M, int N, int nz, int cfg, hipStream_t s) {
+  // nz = members x parities.  Fall back to 32x32 tiles when the shape does
+  // not tile with the requested configuration.
+  auto fits = [&](const TileCfg& c) {
+    return M % c.bm == 0 && N % c.bn == 0 && (a.K / kTile) % c.ks == 0;
+  };
+  if (cfg < 0 || cfg >= kNumTileCfg || !fits(kTileCfgs[cfg])) cfg = 0;
+  int ks = kTileCfgs[cfg].ks;
+  if (cfg == 0) {
+    const int nchunk = a.K / kTile;
+    ks = nchunk % 4 == 0 ? 4 : (nchunk % 2 == 0 ? 2 : 1);
+  }
+  const int bm = kTileCfgs[cfg].bm, bn = kTileCfgs[cfg].bn;
+  a.nz = nz;
+  a.tm = M / bm;
+  a.tn = N / bn;
+  dim3 grid(nz * a.tm * a.tn);
+  if (cfg == 0) {
+    if (ks == 4)
+      hipLaunchKernelGGL((k_gemm<T, AOP, BOP, KIND, MAP, 32, 32, 4>), grid, dim3(1024), 0, s, a);
+    else if (ks == 2)
+      hipLaunchKernelGGL((k_gemm<T, AOP, BOP, KIND, MAP, 32, 32, 2>), grid, dim3(512), 0, s, a);
+    else
+      hipLaunchKernelGGL((k_gemm<T, AOP, BOP, KIND, MAP, 32, 32, 1>), grid, dim3(256), 0, s, a);
+  } else if (cfg == 1) {
+    hipLaunchKernelGGL((k_gemm<T, AOP, BOP, KIND, MAP, 64, 64, 1>), grid, dim3(256), 0, s, a);
+  } else if (cfg == 2) {
+    hipLaunchKernelGGL((k_gemm<T, AOP, BOP, KIND, MAP, 64, 64, 2>), grid, dim3(512), 0, s, a);
+  } else {
+    hipLaunchKernelGGL((k_gemm<T, AOP, BOP, KIND, MAP, 128, 64, 1>), grid, dim3(256), 0, s, a);
+  }
 }
 
 template <typename T>
-void launch_inversion(const Plan& p, const T* F, const T* Fsub, const T* hb, T* out, int mode,
-                      hipStream_t s) {
+void launch_gemm_stage(const Plan& p, int stage, const T* F, const T* Fsub, const T* hb, T* out,
+                       int mode, int cfg, hipStream_t s) {
   T* T1 = static_cast<T*>(p.gT[0]);
   T* T2 = static_cast<T*>(p.gT[1]);
-  const int ks = p.gemm_ksplit;
+  int cf[4] = {cfg, cfg, cfg, cfg};
   GemmArgs<T> a{};
   a.ny = p.ny; a.nx = p.nx; a.my = p.my; a.mx = p.mx; a.hy = p.hy; a.hx = p.hx;
   a.scale = static_cast<const T*>(p.lam_inv);
+  const size_t NF = p.nf, NG = p.ng;
+  const int nb = p.nb;
+  a.nb = nb;
+  a.bsF = NF; a.bsOut = NF; a.bsScale = NG;
   if (p.fold) {
     const T* SyF = static_cast<const T*>(p.SyF);
     const T* SxF = static_cast<const T*>(p.SxF);
     a.strideA = (size_t)p.HY * p.HY;
     a.strideB = (size_t)p.HX * p.HX;
+    if (stage == 0) {
     // T1 = S_y (F - Fsub)
     a.A = SyF; a.lda = p.HY; a.F = F; a.Fsub = Fsub; a.K = p.HY; a.C = T1; a.ldc = p.MX;
-    gemm<T, AOP_DENSE, BOP_FOLDROW_FIELD, EK_STORE, MAP_ROWFOLD>(a, p.HY, p.MX, 2, ks, s);
+    a.bsA = 0; a.bsB = 0; a.bsC = NG;
+    gemm<T, AOP_DENSE, BOP_FOLDROW_FIELD, EK_STORE, MAP_ROWFOLD>(a, p.HY, p.MX, 2 * nb, cf[0], s);
+    } else if (stage == 1) {
     // T2 = (T1 S_x) ./ Lambda
     a.A = T1; a.lda = p.MX; a.B = SxF; a.ldb = p.HX; a.K = p.HX; a.C = T2; a.ldc = p.MX;
-    gemm<T, AOP_FOLDCOL, BOP_DENSE, EK_SCALE, MAP_COLFOLD>(a, p.MY, p.HX, 2, ks, s);
+    a.bsA = NG; a.bsB = 0; a.bsC = NG;
+    gemm<T, AOP_FOLDCOL, BOP_DENSE, EK_SCALE, MAP_COLFOLD>(a, p.MY, p.HX, 2 * nb, cf[1], s);
+    } else if (stage == 2) {
     // T1 = S_y T2
     a.A = SyF; a.lda = p.HY; a.B = T2; a.ldb = p.MX; a.K = p.HY; a.C = T1; a.ldc = p.MX;
-    gemm<T, AOP_DENSE, BOP_FOLDROW_DENSE, EK_STORE, MAP_ROWFOLD>(a, p.HY, p.MX, 2, ks, s);
+    a.bsA = 0; a.bsB = NG; a.bsC = NG;
+    gemm<T, AOP_DENSE, BOP_FOLDROW_DENSE, EK_STORE, MAP_ROWFOLD>(a, p.HY, p.MX, 2 * nb, cf[2], s);
+    } else {
     // out = epilogue(T1 S_x)
     a.A = T1; a.lda = p.MX; a.B = SxF; a.ldb = p.HX; a.K = p.HX; a.out = out; a.hb = hb;
+    a.bsA = NG; a.bsB = 0; a.bsC = 0;
     if (mode == INV_ACC)
-      gemm<T, AOP_FOLDCOL, BOP_DENSE, EK_FIELD_ACC, MAP_COLFOLD>(a, p.MY, p.HX, 2, ks, s);
+      gemm<T, AOP_FOLDCOL, BOP_DENSE, EK_FIELD_ACC, MAP_COLFOLD>(a, p.MY, p.HX, 2 * nb, cf[3], s);
     else
-      gemm<T, AOP_FOLDCOL, BOP_DENSE, EK_FIELD_SET, MAP_COLFOLD>(a, p.MY, p.HX, 2, ks, s);
+      gemm<T, AOP_FOLDCOL, BOP_DENSE, EK_FIELD_SET, MAP_COLFOLD>(a, p.MY, p.HX, 2 * nb, cf[3], s);
+    }
     return;
   }
   const T* Sy = static_cast<const T*>(p.Sy);
   const T* Sx = static_cast<const T*>(p.Sx);
+  if (stage == 0) {
   // T1 = S_y (F - Fsub)_interior
   a.A = Sy; a.lda = p.MY; a.F = F; a.Fsub = Fsub; a.K = p.MY; a.C = T1; a.ldc = p.MX;
-  gemm<T, AOP_DENSE, BOP_FIELD, EK_STORE, MAP_PLAIN>(a, p.MY, p.MX, 1, ks, s);
+  a.bsA = 0; a.bsB = 0; a.bsC = NG;
+  gemm<T, AOP_DENSE, BOP_FIELD, EK_STORE, MAP_PLAIN>(a, p.MY, p.MX, nb, cf[0], s);
+  } else if (stage == 1) {
   // T2 = (T1 S_x) ./ Lambda
   a.A = T1; a.lda = p.MX; a.B = Sx; a.ldb = p.MX; a.K = p.MX; a.C = T2; a.ldc = p.MX;
-  gemm<T, AOP_DENSE, BOP_DENSE, EK_SCALE, MAP_PLAIN>(a, p.MY, p.MX, 1, ks, s);
+  a.bsA = NG; a.bsB = 0; a.bsC = NG;
+  gemm<T, AOP_DENSE, BOP_DENSE, EK_SCALE, MAP_PLAIN>(a, p.MY, p.MX, nb, cf[1], s);
+  } else if (stage == 2) {
   // T1 = S_y T2
   a.A = Sy; a.lda = p.MY; a.B = T2; a.ldb = p.MX; a.K = p.MY; a.C = T1; a.ldc = p.MX;
-  gemm<T, AOP_DENSE, BOP_DENSE, EK_STORE, MAP_PLAIN>(a, p.MY, p.MX, 1, ks, s);
+  a.bsA = 0; a.bsB = NG; a.bsC = NG;
+  gemm<T, AOP_DENSE, BOP_DENSE, EK_STORE, MAP_PLAIN>(a, p.MY, p.MX, nb, cf[2], s);
+  } else {
   // out = epilogue(T1 S_x)
   a.A = T1; a.lda = p.MX; a.B = Sx; a.ldb = p.MX; a.K = p.MX; a.out = out; a.hb = hb;
+  a.bsA = NG; a.bsB = 0; a.bsC = 0;
   if (mode == INV_ACC)
-    gemm<T, AOP_DENSE, BOP_DENSE, EK_FIELD_ACC, MAP_PLAIN>(a, p.MY, p.MX, 1, ks, s);
+    gemm<T, AOP_DENSE, BOP_DENSE, EK_FIELD_ACC, MAP_PLAIN>(a, p.MY, p.MX, nb, cf[3], s);
   else
-    gemm<T, AOP_DENSE, BOP_DENSE, EK_FIELD_SET, MAP_PLAIN>(a, p.MY, p.MX, 1, ks, s);
+    gemm<T, AOP_DENSE, BOP_DENSE, EK_FIELD_SET, MAP_PLAIN>(a, p.MY, p.MX, nb, cf[3], s);
+  }
+}
+
+template <typename T>
+void launch_inversion(const Plan& p, const T* F, const T* Fsub, const T* hb, T* out, int mode,
+                      hipStream_t s) {
+  for (int st = 0; st < 4; ++st) launch_gemm_stage<T>(p, st, F, Fsub, hb, out, mode, p.gemm_cfg[st], s);
 }
 
 // ---------------------------------------------------------------------------
@@ -380,14 +528,22 @@ __device__ __forceinline__ T rhs_at(const T* h, const T* q, int i, int j, const 
   return nz(k);
 }
 
+// One thread per point; blockIdx.z is the batch member and zo the offset of
+// its (ny, nx) field inside every batched array.
 #define VD_POINT_2D                                    \
   const int j = blockIdx.x * blockDim.x + threadIdx.x; \
   const int i = blockIdx.y * blockDim.y + threadIdx.y; \
   if (i >= P.ny || j >= P.nx) return;                  \
   const int nx = P.nx;                                 \
-  const size_t o = (size_t)i * nx + j;
+  const size_t o = (size_t)i * nx + j;                 \
+  const size_t zo = (size_t)blockIdx.z * P.nf;         \
+  (void)zo;
 
-static inline dim3 grid2d(int ny, int nx) { return dim3((nx + 63) / 64, (ny + 3) / 4); }
+template <typename Q>
+__device__ __forceinline__ Q* bofs(Q* p, size_t zo) { return p ? p + zo : p; }
+#define VD_B(p) p = bofs(p, zo)
+
+static inline dim3 grid2d(int ny, int nx, int nb) { return dim3((nx + 63) / 64, (ny + 3) / 4, nb); }
 static const dim3 kBlock2d(64, 4);
 
 // ---------------------------------------------------------------------------
@@ -399,11 +555,11 @@ __device__ __forceinline__ T h2pv_at(const T* h, const T* hb, int i, int j, uint
                                      const Phys<T>& P) {
   const int nx = P.nx;
   if (m == 0) return T(0);
-  if (m != 3) return -P.gfc2 * hb[i * nx + j];
+  if (m != 3) return -P.gfc2_b[blockIdx.z] * hb[i * nx + j];
   const T c = h[i * nx + j];
   const T lap = P.gf_dy2 * (h[(i + 1) * nx + j] + h[(i - 1) * nx + j] - 2 * c) +
                 P.gf_dx2 * (h[i * nx + j + 1] + h[i * nx + j - 1] - 2 * c);
-  return nz(lap - P.gfc2 * c);
+  return nz(lap - P.gfc2_b[blockIdx.z] * c);
 }
 
 template <typename T>
@@ -411,6 +567,7 @@ __global__ void k_prep(Phys<T> P, const T* __restrict__ h0, const T* __restrict_
                        const T* __restrict__ mdt, T* __restrict__ hcur, T* __restrict__ hbm,
                        T* __restrict__ q0, T* __restrict__ qb) {
   VD_POINT_2D
+  VD_B(h0); VD_B(hb); VD_B(hcur); VD_B(hbm); VD_B(q0); VD_B(qb);
   const uint8_t m = P.mask[o];
   if (mdt) {
     // q needs h + mdt at neighbours: evaluate the stencil on the fly.
@@ -422,11 +579,11 @@ __global__ void k_prep(Phys<T> P, const T* __restrict__ h0, const T* __restrict_
     if (m == 3) {
       const T c = H(i, j), cb = HB(i, j);
       qv = nz(P.gf_dy2 * (H(i + 1, j) + H(i - 1, j) - 2 * c) +
-              P.gf_dx2 * (H(i, j + 1) + H(i, j - 1) - 2 * c) - P.gfc2 * c);
+              P.gf_dx2 * (H(i, j + 1) + H(i, j - 1) - 2 * c) - P.gfc2_b[blockIdx.z] * c);
       qbv = nz(P.gf_dy2 * (HB(i + 1, j) + HB(i - 1, j) - 2 * cb) +
-               P.gf_dx2 * (HB(i, j + 1) + HB(i, j - 1) - 2 * cb) - P.gfc2 * cb);
+               P.gf_dx2 * (HB(i, j + 1) + HB(i, j - 1) - 2 * cb) - P.gfc2_b[blockIdx.z] * cb);
     } else if (m != 0) {
-      qv = -P.gfc2 * HB(i, j);
+      qv = -P.gfc2_b[blockIdx.z] * HB(i, j);
       qbv = qv;
     }
     q0[o] = qv;
@@ -443,12 +600,14 @@ template <typename T>
 __global__ void k_h2pv(Phys<T> P, const T* __restrict__ h, const T* __restrict__ hb,
                        T* __restrict__ q) {
   VD_POINT_2D
+  VD_B(h); VD_B(hb); VD_B(q);
   q[o] = h2pv_at(h, hb, i, j, P.mask[o], P);
 }
 
 template <typename T>
 __global__ void k_fill_land(Phys<T> P, T* __restrict__ f, T value) {
   VD_POINT_2D
+  VD_B(f);
   if (P.mask[o] == 0) f[o] = value;
 }
 
@@ -456,6 +615,7 @@ template <typename T>
 __global__ void k_finish(Phys<T> P, const T* __restrict__ h, const T* __restrict__ mdt,
                          T* __restrict__ h1) {
   VD_POINT_2D
+  VD_B(h); VD_B(h1);
   h1[o] = P.mask[o] == 0 ? T(NAN) : (mdt ? h[o] - mdt[o] : h[o]);
 }
 
@@ -465,6 +625,7 @@ __global__ void k_stage(Phys<T> P, const T* __restrict__ h, const T* __restrict_
                         const T* __restrict__ k1, T c0, T c1, T c2, T* __restrict__ qout,
                         T* __restrict__ kstore) {
   VD_POINT_2D
+  VD_B(h); VD_B(q); VD_B(qbase); VD_B(k0); VD_B(k1); VD_B(qout); VD_B(kstore);
   T k = T(0);
   if (P.mask[o] == 3) k = rhs_at(h, q, i, j, P);
   if (kstore) kstore[o] = k;
@@ -525,6 +686,7 @@ __global__ void k_stage_tl(Phys<T> P, const T* __restrict__ h, const T* __restri
                            const T* __restrict__ dk1, T c0, T c1, T c2, T* __restrict__ dqout,
                            T* __restrict__ dkstore) {
   VD_POINT_2D
+  VD_B(h); VD_B(q); VD_B(dh); VD_B(dq); VD_B(dqbase); VD_B(dk0); VD_B(dk1); VD_B(dqout); VD_B(dkstore);
   T k = T(0);
   if (P.mask[o] == 3) k = rhs_tl_at(h, q, dh, dq, i, j, P);
   if (dkstore) dkstore[o] = k;
@@ -544,12 +706,13 @@ template <typename T>
 __global__ void k_prep_tl(Phys<T> P, const T* __restrict__ dh0, const T* __restrict__ hcur,
                           T* __restrict__ dhcur, T* __restrict__ dq0) {
   VD_POINT_2D
+  VD_B(dh0); VD_B(hcur); VD_B(dhcur); VD_B(dq0);
   dhcur[o] = dh0[o];
   T r = T(0);
   if (P.mask[o] == 3 && !q_isnan(hcur, nx, i, j)) {
     const T c = dh0[o];
     r = P.gf_dy2 * (dh0[o + nx] + dh0[o - nx] - 2 * c) +
-        P.gf_dx2 * (dh0[o + 1] + dh0[o - 1] - 2 * c) - P.gfc2 * c;
+        P.gf_dx2 * (dh0[o + 1] + dh0[o - 1] - 2 * c) - P.gfc2_b[blockIdx.z] * c;
   }
   dq0[o] = r;
 }
@@ -557,104 +720,121 @@ __global__ void k_prep_tl(Phys<T> P, const T* __restrict__ dh0, const T* __restr
 template <typename T>
 __global__ void k_finish_tl(Phys<T> P, const T* __restrict__ dh, T* __restrict__ dh1) {
   VD_POINT_2D
+  VD_B(dh); VD_B(dh1);
   dh1[o] = P.mask[o] == 0 ? T(0) : dh[o];
 }
 
 // ---------------------------------------------------------------------------
-// Adjoint kernels (deterministic two-pass gather, no atomics).
-// Pass A, per T point t (active only where mask == 3):
-//   ak = sum_s a_s src_s[t]
-//   W0 = d rhs/d uT contribution (lambda_uT), W1 = lambda_vT (incl. beta),
-//   W2..W5 = up ak/(6dx), um ak/(6dx), vp ak/(6dy), vm ak/(6dy), W6 = ak.
-// Pass B, per point p: lambda_q(p) from W2..W6 of the T points whose stencil
-//   reads q(p); lambda_h(p) from lambda_u / lambda_v at the velocity points
-//   that read h(p).
+// Adjoint kernels (deterministic gathers, no atomics).
 // ---------------------------------------------------------------------------
+// Adjoint terms at an active T point t = (i, j) (mask == 3), for the
+// cotangent ak = a0 s0 + a1 s1 + a2 s2 of rhs at t:
+//   w[0] = lambda_uT, w[1] = lambda_vT (with the beta term),
+//   w[2..5] = up ak/(6dx), um ak/(6dx), vp ak/(6dy), vm ak/(6dy), w[6] = ak.
 template <typename T>
-__global__ void k_adj_a(Phys<T> P, const T* __restrict__ h, const T* __restrict__ q,
-                        const T* __restrict__ s0, const T* __restrict__ s1,
-                        const T* __restrict__ s2, T a0, T a1, T a2, T* __restrict__ W0,
-                        T* __restrict__ W1, T* __restrict__ W2, T* __restrict__ W3,
-                        T* __restrict__ W4, T* __restrict__ W5, T* __restrict__ W6) {
-  VD_POINT_2D
-  T w0 = 0, w1 = 0, w2 = 0, w3 = 0, w4 = 0, w5 = 0, w6 = 0;
-  if (P.mask[o] == 3) {
-    T ak = a0 * s0[o];
-    if (s1) ak += a1 * s1[o];
-    if (s2) ak += a2 * s2[o];
-    const Vel<T> V = vel_T(h, nx, i, j, P);
-    const Diffs<T> D = diffs(q, nx, i, j);
-    T dxp = D.xp, dxm = D.xm, dyp = D.yp, dym = D.ym;
-    if (P.fbathy) {
-      const Diffs<T> B = diffs(P.fbathy, nx, i, j);
-      dxp += B.xp; dxm += B.xm; dyp += B.yp; dym += B.ym;
-    }
-    const T aux = ak * P.inv6dx, auy = ak * P.inv6dy;
-    w0 = (V.uT < 0 ? T(0) : -aux * dxp) + (V.uT > 0 ? T(0) : aux * dxm);
-    w1 = (V.vT < 0 ? T(0) : -auy * dyp) + (V.vT > 0 ? T(0) : auy * dym);
-    if (P.beta) w1 -= P.beta[o] * ak;
-    w2 = (V.uT < 0 ? T(0) : V.uT) * aux;
-    w3 = (V.uT > 0 ? T(0) : V.uT) * aux;
-    w4 = (V.vT < 0 ? T(0) : V.vT) * auy;
-    w5 = (V.vT > 0 ? T(0) : V.vT) * auy;
-    w6 = ak;
+__device__ __forceinline__ void adj_terms(const T* h, const T* q, const T* s0, const T* s1,
+                                          const T* s2, T a0, T a1, T a2, int i, int j,
+                                          const Phys<T>& P, T w[7]) {
+  const int nx = P.nx;
+  const size_t o = (size_t)i * nx + j;
+  T ak = a0 * s0[o];
+  if (s1) ak += a1 * s1[o];
+  if (s2) ak += a2 * s2[o];
+  const Vel<T> V = vel_T(h, nx, i, j, P);
+  const Diffs<T> D = diffs(q, nx, i, j);
+  T dxp = D.xp, dxm = D.xm, dyp = D.yp, dym = D.ym;
+  if (P.fbathy) {
+    const Diffs<T> B = diffs(P.fbathy, nx, i, j);
+    dxp += B.xp; dxm += B.xm; dyp += B.yp; dym += B.ym;
   }
-  W0[o] = w0; W1[o] = w1; W2[o] = w2; W3[o] = w3; W4[o] = w4; W5[o] = w5; W6[o] = w6;
+  const T aux = ak * P.inv6dx, auy = ak * P.inv6dy;
+  w[0] = (V.uT < 0 ? T(0) : -aux * dxp) + (V.uT > 0 ? T(0) : aux * dxm);
+  w[1] = (V.vT < 0 ? T(0) : -auy * dyp) + (V.vT > 0 ? T(0) : auy * dym);
+  if (P.beta) w[1] -= P.beta[o] * ak;
+  w[2] = (V.uT < 0 ? T(0) : V.uT) * aux;
+  w[3] = (V.uT > 0 ? T(0) : V.uT) * aux;
+  w[4] = (V.vT < 0 ? T(0) : V.vT) * auy;
+  w[5] = (V.vT > 0 ? T(0) : V.vT) * auy;
+  w[6] = ak;
 }
 
+// Fused adjoint stage (one launch, deterministic, no atomics).  A block owns
+// an AX x AY tile of output points: it first evaluates adj_terms on the tile
+// plus a halo of 2 into LDS (zero where the point is inactive or outside the
+// grid), then gathers, for each output point p,
+//   lambda_q(p) from the T points whose upwind / diffusion stencils read q(p),
+//   lambda_h(p) from lambda_u, lambda_v at the velocity points that read h(p).
+constexpr int kAX = 64, kAY = 8, kAH = 2;
+constexpr int kRW = kAX + 2 * kAH, kRH = kAY + 2 * kAH;
+
 template <typename T>
-__global__ void k_adj_b(Phys<T> P, const T* __restrict__ h, const T* __restrict__ W0,
-                        const T* __restrict__ W1, const T* __restrict__ W2,
-                        const T* __restrict__ W3, const T* __restrict__ W4,
-                        const T* __restrict__ W5, const T* __restrict__ W6,
-                        const T* acc0, const T* acc1, const T* acc2, T* lq_out,
-                        T* __restrict__ lh_out) {
+__global__ __launch_bounds__(kAX * kAY) void k_adj_fused(
+    Phys<T> P, const T* __restrict__ h, const T* __restrict__ q, const T* __restrict__ s0,
+    const T* __restrict__ s1, const T* __restrict__ s2, T a0, T a1, T a2, const T* acc0,
+    const T* acc1, const T* acc2, T* lq_out, T* __restrict__ lh_out) {
   // acc* may alias lq_out (in-place accumulation); each thread reads and
   // writes only its own point of those fields.
-  VD_POINT_2D
-  const int ny = P.ny;
-  // W fields vanish outside 2..n-4, so only array bounds need checking.
-  auto W = [&](const T* f, int ii, int jj) -> T {
-    return (ii >= 0 && ii < ny && jj >= 0 && jj < nx) ? f[ii * nx + jj] : T(0);
-  };
+  __shared__ T W[7][kRH][kRW];
+  const int ny = P.ny, nx = P.nx;
+  const int tid = threadIdx.y * kAX + threadIdx.x;
+  const int j0 = blockIdx.x * kAX, i0 = blockIdx.y * kAY;
+  const size_t zo = (size_t)blockIdx.z * P.nf;
+  VD_B(h); VD_B(q); VD_B(s0); VD_B(s1); VD_B(s2); VD_B(acc0); VD_B(acc1); VD_B(acc2);
+  VD_B(lq_out); VD_B(lh_out);
+  for (int r = tid; r < kRH * kRW; r += kAX * kAY) {
+    const int li = r / kRW, lj = r - li * kRW;
+    const int i = i0 - kAH + li, j = j0 - kAH + lj;
+    T w[7] = {0, 0, 0, 0, 0, 0, 0};
+    if (i >= 0 && i < ny && j >= 0 && j < nx && P.mask[(size_t)i * nx + j] == 3)
+      adj_terms(h, q, s0, s1, s2, a0, a1, a2, i, j, P, w);
+#pragma unroll
+    for (int k = 0; k < 7; ++k) W[k][li][lj] = w[k];
+  }
+  __syncthreads();
+  const int i = i0 + threadIdx.y, j = j0 + threadIdx.x;
+  if (i >= ny || j >= nx) return;
+  const size_t o = (size_t)i * nx + j;
+  const int li = threadIdx.y + kAH, lj = threadIdx.x + kAH;
   T lq = 0;
   if (acc0) lq += acc0[o];
   if (acc1) lq += acc1[o];
   if (acc2) lq += acc2[o];
-  // transpose of the x upwind stencils: T point t = p + d reads q(p)
-  lq += (-2) * W(W2, i, j - 1) + (-6) * W(W3, i, j - 1) + (-3) * W(W2, i, j) +
-        3 * W(W3, i, j) + 6 * W(W2, i, j + 1) + 2 * W(W3, i, j + 1) - W(W2, i, j + 2) +
-        W(W3, i, j - 2);
-  lq += (-2) * W(W4, i - 1, j) + (-6) * W(W5, i - 1, j) + (-3) * W(W4, i, j) +
-        3 * W(W5, i, j) + 6 * W(W4, i + 1, j) + 2 * W(W5, i + 1, j) - W(W4, i + 2, j) +
-        W(W5, i - 2, j);
+  // transpose of the x / y upwind stencils: T point t = p + d reads q(p)
+  lq += (-2) * W[2][li][lj - 1] + (-6) * W[3][li][lj - 1] + (-3) * W[2][li][lj] +
+        3 * W[3][li][lj] + 6 * W[2][li][lj + 1] + 2 * W[3][li][lj + 1] - W[2][li][lj + 2] +
+        W[3][li][lj - 2];
+  lq += (-2) * W[4][li - 1][lj] + (-6) * W[5][li - 1][lj] + (-3) * W[4][li][lj] +
+        3 * W[5][li][lj] + 6 * W[4][li + 1][lj] + 2 * W[5][li + 1][lj] - W[4][li + 2][lj] +
+        W[5][li - 2][lj];
   if (P.has_kdiff) {
-    const T c = W(W6, i, j);
-    lq += P.kx * (W(W6, i, j + 1) + W(W6, i, j - 1) - 2 * c) +
-          P.ky * (W(W6, i + 1, j) + W(W6, i - 1, j) - 2 * c);
+    const T c = W[6][li][lj];
+    lq += P.kx * (W[6][li][lj + 1] + W[6][li][lj - 1] - 2 * c) +
+          P.ky * (W[6][li + 1][lj] + W[6][li - 1][lj] - 2 * c);
   }
   lq_out[o] = lq;
-
   // lambda_u[a][b] = 0.5 (W0[a][b] + W0[a][b-1]) on the u domain, zero where u was NaN
   auto lu = [&](int a, int b) -> T {
     if (a < 1 || a > ny - 2 || b < 1 || b > nx - 1) return T(0);
     if (u_isnan(h, nx, a, b)) return T(0);
-    return T(0.5) * (W0[a * nx + b] + W0[a * nx + b - 1]);
+    const int la = a - i0 + kAH, lb = b - j0 + kAH;
+    return T(0.5) * (W[0][la][lb] + W[0][la][lb - 1]);
   };
   auto lv = [&](int a, int b) -> T {
     if (a < 1 || a > ny - 1 || b < 1 || b > nx - 2) return T(0);
     if (v_isnan(h, nx, a, b)) return T(0);
-    return T(0.5) * (W1[a * nx + b] + W1[(a - 1) * nx + b]);
+    const int la = a - i0 + kAH, lb = b - j0 + kAH;
+    return T(0.5) * (W[1][la][lb] + W[1][la - 1][lb]);
   };
-  const T lh = P.cu * (lu(i - 1, j + 1) + lu(i - 1, j) - lu(i + 1, j) - lu(i + 1, j + 1)) +
-               P.cv * (lv(i, j - 1) + lv(i + 1, j - 1) - lv(i + 1, j + 1) - lv(i, j + 1));
-  lh_out[o] = lh;
+  lh_out[o] = P.cu * (lu(i - 1, j + 1) + lu(i - 1, j) - lu(i + 1, j) - lu(i + 1, j + 1)) +
+              P.cv * (lv(i, j - 1) + lv(i + 1, j - 1) - lv(i + 1, j + 1) - lv(i, j + 1));
 }
+
 
 template <typename T>
 __global__ void k_adj_init(Phys<T> P, const T* __restrict__ adh1, T* __restrict__ lh,
                            T* __restrict__ lq) {
   VD_POINT_2D
+  VD_B(adh1); VD_B(lh); VD_B(lq);
   lh[o] = P.mask[o] == 0 ? T(0) : adh1[o];
   lq[o] = T(0);
 }
@@ -663,6 +843,7 @@ template <typename T>
 __global__ void k_adj_final(Phys<T> P, const T* __restrict__ hcur, const T* __restrict__ lh,
                             const T* __restrict__ lq, T* __restrict__ adh0) {
   VD_POINT_2D
+  VD_B(hcur); VD_B(lh); VD_B(lq); VD_B(adh0);
   const int ny = P.ny;
   // a(t) = lq(t) where h2pv used its Laplacian formula (mask 3, q not NaN)
   auto A = [&](int a, int b) -> T {
@@ -672,7 +853,7 @@ __global__ void k_adj_final(Phys<T> P, const T* __restrict__ hcur, const T* __re
     return lq[t];
   };
   const T r = P.gf_dy2 * (A(i + 1, j) + A(i - 1, j)) + P.gf_dx2 * (A(i, j + 1) + A(i, j - 1)) +
-              (-2 * P.gf_dy2 - 2 * P.gf_dx2 - P.gfc2) * A(i, j);
+              (-2 * P.gf_dy2 - 2 * P.gf_dx2 - P.gfc2_b[blockIdx.z]) * A(i, j);
   adh0[o] = lh[o] + r;
 }
 
@@ -690,7 +871,8 @@ Phys<T> make_phys(const Plan& p) {
   P.inv6dy = T(1.0 / (6 * p.dy));
   P.kx = T(p.has_kdiff ? p.kdiffus / (p.dx * p.dx) : 0.0);
   P.ky = T(p.has_kdiff ? p.kdiffus / (p.dy * p.dy) : 0.0);
-  P.gfc2 = T(p.g * p.f0 / (p.c * p.c));
+  P.gfc2_b = static_cast<const T*>(p.gfc2_b);
+  P.nf = p.nf;
   P.gf_dx2 = T(p.g / p.f0 / (p.dx * p.dx));
   P.gf_dy2 = T(p.g / p.f0 / (p.dy * p.dy));
   P.mask = p.mask;
@@ -705,34 +887,34 @@ void launch_prep(const Plan& p, const T* h0, const T* hb, T* hcur, T* hbm, T* q0
                  hipStream_t s) {
   Phys<T> P = make_phys<T>(p);
   const T* mdt = p.has_mdt ? static_cast<const T*>(p.mdt) : nullptr;
-  hipLaunchKernelGGL(k_prep<T>, grid2d(p.ny, p.nx), kBlock2d, 0, s, P, h0, hb, mdt, hcur, hbm,
+  hipLaunchKernelGGL(k_prep<T>, grid2d(p.ny, p.nx, p.nb), kBlock2d, 0, s, P, h0, hb, mdt, hcur, hbm,
                      q0, qb);
 }
 
 template <typename T>
 void launch_h2pv(const Plan& p, const T* h, const T* hb, T* q, hipStream_t s) {
   Phys<T> P = make_phys<T>(p);
-  hipLaunchKernelGGL(k_h2pv<T>, grid2d(p.ny, p.nx), kBlock2d, 0, s, P, h, hb, q);
+  hipLaunchKernelGGL(k_h2pv<T>, grid2d(p.ny, p.nx, p.nb), kBlock2d, 0, s, P, h, hb, q);
 }
 
 template <typename T>
 void launch_fill_land(const Plan& p, T* f, T value, hipStream_t s) {
   Phys<T> P = make_phys<T>(p);
-  hipLaunchKernelGGL(k_fill_land<T>, grid2d(p.ny, p.nx), kBlock2d, 0, s, P, f, value);
+  hipLaunchKernelGGL(k_fill_land<T>, grid2d(p.ny, p.nx, p.nb), kBlock2d, 0, s, P, f, value);
 }
 
 template <typename T>
 void launch_finish(const Plan& p, const T* h, T* h1, hipStream_t s) {
   Phys<T> P = make_phys<T>(p);
   const T* mdt = p.has_mdt ? static_cast<const T*>(p.mdt) : nullptr;
-  hipLaunchKernelGGL(k_finish<T>, grid2d(p.ny, p.nx), kBlock2d, 0, s, P, h, mdt, h1);
+  hipLaunchKernelGGL(k_finish<T>, grid2d(p.ny, p.nx, p.nb), kBlock2d, 0, s, P, h, mdt, h1);
 }
 
 template <typename T>
 void launch_stage(const Plan& p, const T* h, const T* q, const T* qbase, const T* kprev0,
                   const T* kprev1, T c0, T c1, T c2, T* qout, T* kstore, hipStream_t s) {
   Phys<T> P = make_phys<T>(p);
-  hipLaunchKernelGGL(k_stage<T>, grid2d(p.ny, p.nx), kBlock2d, 0, s, P, h, q, qbase, kprev0,
+  hipLaunchKernelGGL(k_stage<T>, grid2d(p.ny, p.nx, p.nb), kBlock2d, 0, s, P, h, q, qbase, kprev0,
                      kprev1, c0, c1, c2, qout, kstore);
 }
 
@@ -741,50 +923,52 @@ void launch_stage_tl(const Plan& p, const T* h, const T* q, const T* dh, const T
                      const T* dqbase, const T* dkprev0, const T* dkprev1, T c0, T c1, T c2,
                      T* dqout, T* dkstore, hipStream_t s) {
   Phys<T> P = make_phys<T>(p);
-  hipLaunchKernelGGL(k_stage_tl<T>, grid2d(p.ny, p.nx), kBlock2d, 0, s, P, h, q, dh, dq, dqbase,
+  hipLaunchKernelGGL(k_stage_tl<T>, grid2d(p.ny, p.nx, p.nb), kBlock2d, 0, s, P, h, q, dh, dq, dqbase,
                      dkprev0, dkprev1, c0, c1, c2, dqout, dkstore);
 }
 
 template <typename T>
 void launch_prep_tl(const Plan& p, const T* dh0, const T* hcur, T* dhcur, T* dq0, hipStream_t s) {
   Phys<T> P = make_phys<T>(p);
-  hipLaunchKernelGGL(k_prep_tl<T>, grid2d(p.ny, p.nx), kBlock2d, 0, s, P, dh0, hcur, dhcur, dq0);
+  hipLaunchKernelGGL(k_prep_tl<T>, grid2d(p.ny, p.nx, p.nb), kBlock2d, 0, s, P, dh0, hcur, dhcur, dq0);
 }
 
 template <typename T>
 void launch_finish_tl(const Plan& p, const T* dh, T* dh1, hipStream_t s) {
   Phys<T> P = make_phys<T>(p);
-  hipLaunchKernelGGL(k_finish_tl<T>, grid2d(p.ny, p.nx), kBlock2d, 0, s, P, dh, dh1);
+  hipLaunchKernelGGL(k_finish_tl<T>, grid2d(p.ny, p.nx, p.nb), kBlock2d, 0, s, P, dh, dh1);
 }
 
 template <typename T>
 void launch_stage_adj(const Plan& p, const T* h, const T* q, const T* s0, const T* s1,
                       const T* s2, T a0, T a1, T a2, const T* acc0, const T* acc1,
                       const T* acc2, T* lq_out, T* lh_out, T* W[7], hipStream_t s) {
+  (void)W;
   Phys<T> P = make_phys<T>(p);
-  hipLaunchKernelGGL(k_adj_a<T>, grid2d(p.ny, p.nx), kBlock2d, 0, s, P, h, q, s0, s1, s2, a0, a1,
-                     a2, W[0], W[1], W[2], W[3], W[4], W[5], W[6]);
-  hipLaunchKernelGGL(k_adj_b<T>, grid2d(p.ny, p.nx), kBlock2d, 0, s, P, h, W[0], W[1], W[2], W[3],
-                     W[4], W[5], W[6], acc0, acc1, acc2, lq_out, lh_out);
+  dim3 grid((p.nx + kAX - 1) / kAX, (p.ny + kAY - 1) / kAY, p.nb);
+  hipLaunchKernelGGL(k_adj_fused<T>, grid, dim3(kAX, kAY), 0, s, P, h, q, s0, s1, s2, a0, a1, a2,
+                     acc0, acc1, acc2, lq_out, lh_out);
 }
 
 template <typename T>
 void launch_adj_init(const Plan& p, const T* adh1, T* lh, T* lq, hipStream_t s) {
   Phys<T> P = make_phys<T>(p);
-  hipLaunchKernelGGL(k_adj_init<T>, grid2d(p.ny, p.nx), kBlock2d, 0, s, P, adh1, lh, lq);
+  hipLaunchKernelGGL(k_adj_init<T>, grid2d(p.ny, p.nx, p.nb), kBlock2d, 0, s, P, adh1, lh, lq);
 }
 
 template <typename T>
 void launch_adj_final(const Plan& p, const T* hcur, const T* lh, const T* lq, T* adh0,
                       hipStream_t s) {
   Phys<T> P = make_phys<T>(p);
-  hipLaunchKernelGGL(k_adj_final<T>, grid2d(p.ny, p.nx), kBlock2d, 0, s, P, hcur, lh, lq, adh0);
+  hipLaunchKernelGGL(k_adj_final<T>, grid2d(p.ny, p.nx, p.nb), kBlock2d, 0, s, P, hcur, lh, lq, adh0);
 }
 
 #define VD_INSTANTIATE(T)                                                                     \
   template Phys<T> make_phys<T>(const Plan&);                                                \
   template void launch_inversion<T>(const Plan&, const T*, const T*, const T*, T*, int,       \
                                     hipStream_t);                                             \
+  template void launch_gemm_stage<T>(const Plan&, int, const T*, const T*, const T*, T*, int,  \
+                                     int, hipStream_t);                                       \
   template void launch_prep<T>(const Plan&, const T*, const T*, T*, T*, T*, T*, hipStream_t); \
   template void launch_h2pv<T>(const Plan&, const T*, const T*, T*, hipStream_t);             \
   template void launch_finish<T>(const Plan&, const T*, T*, hipStream_t);                     \

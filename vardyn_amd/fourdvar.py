@@ -1,4 +1,5 @@
-"""GPU-resident 4D-Var cost and gradient for the QG model.
+"""GPU-resident 4D-Var cost and gradient for the QG model, for one problem or a
+batch of problems on the same grid.
 
 Reference: ``Variational.cost_and_grad`` (mapping/src/tools_4Dvar.py:252-400)
 driving ``Model_qg1l_jax.step/step_adj`` (mapping/src/mod.py:867-1076) over
@@ -28,6 +29,12 @@ start states and recomputes each segment into one reusable tape during the
 reverse sweep, like the reference's ``jax.vjp`` of a segment.  Both give
 identical results.
 
+Batches: a plan built with ``nbatch = B`` advances B independent problems in
+the same kernel launches.  Each member has its own observation operator,
+control vector, J and G; the members share the grid, the time step and the
+checkpoint schedule (their observations fall on the same model steps, as for
+regions seen by the same satellite passes).
+
 All arithmetic is on the device: cost accumulators and the control/gradient
 vectors are float64; model fields use the plan precision.
 """
@@ -54,7 +61,11 @@ class FourDVar:
         if prec and sigma_b is None:
             raise ValueError("preconditioning needs sigma_b")
         self.plan = plan
-        self.H = obsop
+        self.B = plan.nb
+        self.Hs = list(obsop) if isinstance(obsop, (list, tuple)) else [obsop]
+        if len(self.Hs) != self.B:
+            raise VardynError(f"{len(self.Hs)} observation operators for {self.B} batch members")
+        self.H = self.Hs[0]
         self.dev = plan.device
         self.dtype = plan.dtype
         self.control = control
@@ -75,36 +86,46 @@ class FourDVar:
         self.n = plan.ny * plan.nx
         self.nctrl = self.n if control == "ic" else 1
 
-        # observation table index at each segment start and at the final time
-        def obs_index(t):
-            return self.H.index(t) if self.H.is_obs_time(t) else -1
-        self.seg_obs = [obs_index(self.T[c]) for c in cps[:-1]]
-        self.final_obs = obs_index(self.T[cps[-1]])
+        # observation table index of every member at each segment start / final time
+        def obs_index(H, t):
+            return H.index(t) if H.is_obs_time(t) else -1
+        self.seg_obs = [[obs_index(H, self.T[c]) for H in self.Hs] for c in cps[:-1]]
+        self.final_obs = [obs_index(H, self.T[cps[-1]]) for H in self.Hs]
+        for row in self.seg_obs + [self.final_obs]:
+            if len({r >= 0 for r in row}) > 1:
+                raise VardynError("batch members must observe on the same model steps")
 
-        shape = (plan.ny, plan.nx)
+        fshape = (plan.ny, plan.nx) if self.B == 1 else (self.B, plan.ny, plan.nx)
         dd = dict(device=self.dev)
-        self.hb = torch.zeros(shape, dtype=self.dtype, **dd) if hb is None else plan.field(hb)
-        self.states = torch.empty((self.nseg + 1,) + shape, dtype=self.dtype, **dd)
+        self.fshape = fshape
+        # None: zero boundary field (closed basin, psi = 0); the plan then skips h_b, q_b
+        self.hb = None if hb is None else plan.field(hb)
+        self.states = torch.empty((self.nseg + 1,) + fshape, dtype=self.dtype, **dd)
         if tape == "full":
-            self.tape_elems = [plan.tape_bytes(int(k)) // plan.empty().element_size()
-                               for k in self.nsteps]
+            es = plan.empty().element_size()
+            self.tape_elems = [plan.tape_bytes(int(k)) // es for k in self.nsteps]
             self.tape_off = np.concatenate([[0], np.cumsum(self.tape_elems)]).astype(np.int64)
             self.tape = torch.empty(int(self.tape_off[-1]), dtype=self.dtype, **dd)
         else:
             self.tape = plan.new_tape(int(self.nsteps.max()))
-        self.lam = torch.empty(shape, dtype=self.dtype, **dd)
-        self.lam2 = torch.empty(shape, dtype=self.dtype, **dd)
-        self.params = torch.zeros(shape, dtype=self.dtype, **dd)
-        self.adparams = torch.zeros(shape, dtype=self.dtype, **dd)
+        self.lam = torch.empty(fshape, dtype=self.dtype, **dd)
+        self.lam2 = torch.empty(fshape, dtype=self.dtype, **dd)
+        self.params = torch.zeros(fshape, dtype=self.dtype, **dd)
+        self.adparams = torch.zeros(fshape, dtype=self.dtype, **dd)
         self.ones64 = torch.ones(self.n, dtype=torch.float64, **dd)
-        self.X = torch.empty(self.nctrl, dtype=torch.float64, **dd)
-        self.G = torch.empty(self.nctrl, dtype=torch.float64, **dd)
-        self.J = torch.zeros(1, dtype=torch.float64, **dd)
+        self.p64 = torch.empty((self.B, self.n), dtype=torch.float64, **dd)
+        self.X = torch.empty((self.B, self.nctrl), dtype=torch.float64, **dd)
+        self.G = torch.empty((self.B, self.nctrl), dtype=torch.float64, **dd)
+        self.J = torch.zeros(self.B, dtype=torch.float64, **dd)
         self.scratch = torch.empty(_lib.VD_REDUCE_SCRATCH, dtype=torch.float64, **dd)
-        self.Xb = torch.zeros(self.nctrl, dtype=torch.float64, **dd) if Xb is None else \
-            torch.as_tensor(np.asarray(Xb, dtype=np.float64), device=self.dev).reshape(-1).clone()
-        self.var0 = torch.zeros(shape, dtype=self.dtype, **dd)
+        if Xb is None:
+            self.Xb = torch.zeros((self.B, self.nctrl), dtype=torch.float64, **dd)
+        else:
+            self.Xb = torch.as_tensor(np.asarray(Xb, dtype=np.float64), device=self.dev) \
+                .reshape(self.B, self.nctrl).clone()
+        self.var0 = torch.zeros(fshape, dtype=self.dtype, **dd)
         self.land = land
+        self._graph = None
 
     # ------------------------------------------------------------ helpers
     def _s(self):
@@ -120,8 +141,8 @@ class FourDVar:
         _lib.call("vd_axpby", prec, y.numel(), float(a), ptr(x), float(b), ptr(y), self._s())
 
     def _dot(self, x, y, alpha, out):
-        _lib.call("vd_dot_f64", x.numel(), ptr(x), ptr(y), float(alpha), ptr(out), ptr(self.scratch),
-                  self._s())
+        _lib.call("vd_dot_f64", x.numel(), ptr(x), ptr(y), float(alpha), ptr(out),
+                  ptr(self.scratch), self._s())
 
     def _convert(self, src, dst, nan_to_zero=False):
         sp = _lib.VD_F64 if src.dtype == torch.float64 else _lib.VD_F32
@@ -132,32 +153,40 @@ class FourDVar:
     def _flux_scale(self, i):
         return self.nsteps[i] * self.plan.dt / (3600 * 24)
 
+    def _member(self, field, b):
+        return field if self.B == 1 else field[b]
+
+    def _misfits(self, idx, state, J, inv_r2):
+        for b, H in enumerate(self.Hs):
+            H.misfit_dev(idx[b], self._member(state, b), jo=J[b:b + 1], inv_sigma_r2=inv_r2)
+
+    def _obs_adj(self, idx, lam):
+        for b, H in enumerate(self.Hs):
+            H.adj_dev(idx[b], H.tables[idx[b]].misfit, self._member(lam, b))
+
     # --------------------------------------------------------- evaluation
     def run(self, X0):
-        """Enqueue one cost+gradient evaluation; X0 is a float64 device vector.
+        """Enqueue one cost+gradient evaluation of every member.
 
-        Results are left in ``self.J`` (1-element float64) and ``self.G``.
+        ``X0``: float64 device tensor with ``B * nctrl`` elements.  Results are
+        left in ``self.J`` (B values) and ``self.G`` (B x nctrl).
         """
-        plan, H = self.plan, self.H
-        if X0.dtype != torch.float64 or X0.device != self.dev or X0.numel() != self.nctrl:
-            raise VardynError(f"control must be a float64 device vector of size {self.nctrl}")
-        X0 = X0.contiguous()
+        plan = self.plan
+        if X0.dtype != torch.float64 or X0.device != self.dev or X0.numel() != self.B * self.nctrl:
+            raise VardynError(f"control must be a float64 device tensor of "
+                              f"{self.B * self.nctrl} elements")
+        X0 = X0.reshape(self.B, self.nctrl).contiguous()
         J, X = self.J, self.X
         J.zero_()
         inv_sb2 = 0.0 if self.sigma_b is None else 1.0 / float(self.sigma_b) ** 2
         # background term (tools_4Dvar.py:262-271); J accumulates (Jo + Jb) / 2
+        self._convert(self.Xb, X)
         if self.sigma_b is not None:
-            if self.prec:
-                self._convert(self.Xb, X)
-                self._axpby(self.sigma_b, X0, 1.0, X)
-                self._dot(X0, X0, 0.5, J)
-            else:
-                self._convert(self.Xb, X)
-                self._axpby(1.0, X0, 1.0, X)
-                self._dot(X0, X0, 0.5 * inv_sb2, J)
+            self._axpby(self.sigma_b if self.prec else 1.0, X0, 1.0, X)
+            for b in range(self.B):
+                self._dot(X0[b], X0[b], 0.5 if self.prec else 0.5 * inv_sb2, J[b:b + 1])
         else:
-            self._convert(X0, X)
-            self._axpby(-1.0, self.Xb, 1.0, X)
+            self._axpby(1.0, X0, -1.0, X)
         inv_r2 = 0.5 / self.sigma_r ** 2
         s0 = self.states[0]
         if self.control == "ic":
@@ -171,17 +200,18 @@ class FourDVar:
         # forward sweep
         for i in range(self.nseg):
             st, nxt = self.states[i], self.states[i + 1]
-            if self.seg_obs[i] >= 0:
-                H.misfit_dev(self.seg_obs[i], st, jo=J, inv_sigma_r2=inv_r2)
+            if self.seg_obs[i][0] >= 0:
+                self._misfits(self.seg_obs[i], st, J, inv_r2)
             if self.control == "offset" and self.cps[i] % self.dtbasis == 0:
-                # Basis_offset.operg: params = X[0] everywhere (basis.py:3968-3984)
-                params.copy_(X[0].to(params.dtype).expand_as(params))
+                # Basis_offset.operg: params = X everywhere (basis.py:3968-3984)
+                params.copy_(X[:, 0].to(params.dtype).reshape(self.B, 1, 1)
+                             .expand(self.B, plan.ny, plan.nx).reshape(self.fshape))
             tape = self._tape(i) if self.tape_mode == "full" else None
             plan.step(st, self.hb, int(self.nsteps[i]), h1=nxt, tape=tape)
             if self.control == "offset":
                 self._axpby(self._flux_scale(i), params, 1.0, nxt)
-        if self.final_obs >= 0:
-            H.misfit_dev(self.final_obs, self.states[-1], jo=J, inv_sigma_r2=inv_r2)
+        if self.final_obs[0] >= 0:
+            self._misfits(self.final_obs, self.states[-1], J, inv_r2)
 
         # reverse sweep
         lam, lam2 = self.lam, self.lam2
@@ -190,8 +220,8 @@ class FourDVar:
         adpar.zero_()
         G = self.G
         G.zero_()
-        if self.final_obs >= 0:
-            H.adj_dev(self.final_obs, H.tables[self.final_obs].misfit, lam)
+        if self.final_obs[0] >= 0:
+            self._obs_adj(self.final_obs, lam)
         for i in reversed(range(self.nseg)):
             if self.control == "offset":
                 self._axpby(self._flux_scale(i), lam, 1.0, adpar)
@@ -199,11 +229,12 @@ class FourDVar:
                           tape=self._tape(i), tape_valid=self.tape_mode == "full")
             lam, lam2 = lam2, lam
             if self.control == "offset" and self.cps[i] % self.dtbasis == 0:
-                tmp = self._adpar64()
-                self._dot(tmp, self.ones64, 1.0, G)
+                self._convert(adpar, self.p64)
+                for b in range(self.B):
+                    self._dot(self.p64[b], self.ones64, 1.0, G[b])
                 adpar.zero_()
-            if self.seg_obs[i] >= 0:
-                H.adj_dev(self.seg_obs[i], H.tables[self.seg_obs[i]].misfit, lam)
+            if self.seg_obs[i][0] >= 0:
+                self._obs_adj(self.seg_obs[i], lam)
         # gradient (tools_4Dvar.py:388-398)
         if self.control == "ic":
             self._convert(lam, G, nan_to_zero=True)
@@ -214,16 +245,6 @@ class FourDVar:
             self._axpby(inv_sb2, X0, 1.0, G)
         return J, G
 
-    def _params64(self):
-        if not hasattr(self, "_p64"):
-            self._p64 = torch.empty(self.n, dtype=torch.float64, device=self.dev)
-        return self._p64
-
-    def _adpar64(self):
-        t = self._params64()
-        self._convert(self.adparams, t)
-        return t
-
     # ------------------------------------------------------------ graphs
     def capture(self):
         """Capture one whole evaluation in a HIP graph.
@@ -233,7 +254,7 @@ class FourDVar:
         individual launches per gradient.  An eager evaluation runs first so
         that lazily created buffers exist before capture.
         """
-        self._x_static = torch.zeros(self.nctrl, dtype=torch.float64, device=self.dev)
+        self._x_static = torch.zeros(self.B * self.nctrl, dtype=torch.float64, device=self.dev)
         self.run(self._x_static)
         torch.cuda.synchronize(self.dev)
         g = torch.cuda.CUDAGraph()
@@ -245,10 +266,10 @@ class FourDVar:
 
     @property
     def captured(self):
-        return getattr(self, "_graph", None) is not None
+        return self._graph is not None
 
     def replay(self, X0):
-        """Run the captured evaluation on control ``X0`` (float64 device vector)."""
+        """Run the captured evaluation on control ``X0`` (float64 device tensor)."""
         if not self.captured:
             raise VardynError("capture() first")
         self._x_static.copy_(X0.reshape(-1))
@@ -260,7 +281,13 @@ class FourDVar:
         return self.replay(X0) if self.captured else self.run(X0)
 
     def cost_and_grad(self, X0):
-        """Host interface for scipy: numpy in, (float J, numpy G) out."""
+        """Host interface for scipy: numpy in, (J, G) out.
+
+        For one problem J is a float and G has ``nctrl`` values; for a batch
+        J has B values and G is (B, nctrl).
+        """
         x = torch.as_tensor(np.asarray(X0, dtype=np.float64), device=self.dev)
         J, G = self.evaluate(x)
-        return float(J.item()), G.cpu().numpy()
+        if self.B == 1:
+            return float(J[0].item()), G[0].cpu().numpy()
+        return J.cpu().numpy(), G.cpu().numpy()

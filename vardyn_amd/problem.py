@@ -48,6 +48,42 @@ def truth_run(plan: QgPlan, h0, steps, dt):
     return out
 
 
+def build_batch(case: syn.QgCase, nbatch, precision="float64", n_steps=None, tape="full",
+                device=None, c=None):
+    """``nbatch`` independent problems of ``case`` in one batched plan.
+
+    Member b has its own seeded initial field, track/swath geometry and
+    noise; all members share the observation steps, so they share the
+    checkpoint schedule.  ``c`` may give one gravity-wave speed per member.
+    """
+    n_steps = case.n_steps if n_steps is None else int(n_steps)
+    grid = syn.make_grid(case.ny, case.nx, case.dx)
+    plan = QgPlan(case.ny, case.nx, case.dx, case.dx, case.dt, case.c if c is None else c,
+                  case.f0, g=case.g, time_scheme=case.time_scheme, precision=precision,
+                  device=device, nbatch=nbatch)
+    truth0 = np.stack([syn.gaussian_random_field(case.ny, case.nx, case.dx, case.wavelength,
+                                                 case.rms, case.seed + 7919 * b)
+                       for b in range(nbatch)])
+    obs = [syn.make_observations(case, grid, n_steps, member=b) for b in range(nbatch)]
+    steps = sorted(obs[0].steps)
+    fields = truth_run(plan, truth0 if nbatch > 1 else truth0[0], steps, case.dt)
+    T = np.arange(n_steps + 1) * case.dt
+    Hs, dicts = [], []
+    for b in range(nbatch):
+        sl = {s: (fields[s][b] if nbatch > 1 else fields[s]) for s in steps}
+        syn.sample_truth(obs[b], lambda st, sl=sl: sl, case.sigma_obs, case.seed + 3 + 1009 * b)
+        d = {float(T[s]): (e["lon"], e["lat"], e["y"], e["err"]) for s, e in obs[b].steps.items()}
+        dicts.append(d)
+        Hs.append(Obsop_interp_l3(grid.lon, grid.lat, grid.DX, grid.DY, T, case.dt, d, Npix=4,
+                                  device=plan.device).process_obs())
+    cps = checkpoint_schedule(n_steps + 1, case.checkpoint_steps, lambda i: Hs[0].is_obs(T[i]))
+    pb = Problem(case, grid, plan, Hs[0], T, cps, obs[0], dicts[0], truth0[0])
+    pb.Hs, pb.dicts, pb.truth = Hs, dicts, truth0
+    pb.engine = FourDVar(plan, Hs if nbatch > 1 else Hs[0], cps, T, control="ic",
+                         sigma_b=case.sigma_b, tape=tape, dtbasis=case.checkpoint_steps)
+    return pb
+
+
 def build_problem(case: syn.QgCase, precision="float64", n_steps=None, control="ic",
                   tape="checkpoint", device=None, truth_plan: QgPlan | None = None,
                   with_engine=True) -> Problem:

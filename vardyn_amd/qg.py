@@ -65,12 +65,15 @@ class QgPlan:
 
     def __init__(self, ny, nx, dx, dy, dt, c, f0, g=9.81, time_scheme="rk2",
                  precision="float64", kdiffus=None, mask=None, beta=None, fbathy=None,
-                 mdt=None, device=None, dst_dense=None, gemm_ksplit=None):
+                 mdt=None, device=None, dst_dense=None, gemm_config=None, nbatch=1):
         if not torch.cuda.is_available():
             raise VardynError("no HIP device available: the QG plan runs on the GPU only")
         self.device = torch.device("cuda", torch.cuda.current_device() if device is None
                                    else torch.device(device).index or 0)
         self.ny, self.nx = int(ny), int(nx)
+        self.nb = int(nbatch)
+        if self.nb < 1:
+            raise VardynError("nbatch must be >= 1")
         self.dtype = _torch_dtype(precision)
         self.prec = _lib.VD_F64 if self.dtype == torch.float64 else _lib.VD_F32
         if time_scheme not in _lib.SCHEMES:
@@ -94,7 +97,13 @@ class QgPlan:
         p.precision = self.prec
         p.time_scheme = _lib.SCHEMES[time_scheme]
         p.dx, p.dy, p.dt = float(dx), float(dy), float(dt)
-        p.g, p.f0, p.c = float(g), float(f0), float(c)
+        c_arr = np.atleast_1d(np.asarray(c, dtype=np.float64))
+        if c_arr.size not in (1, self.nb):
+            raise VardynError("c must be a scalar or hold one value per batch member")
+        p.g, p.f0, p.c = float(g), float(f0), float(c_arr[0])
+        self.c = c_arr if c_arr.size == self.nb else np.full(self.nb, c_arr[0])
+        p.nbatch = self.nb
+        p.c_batch = host(self.c, (self.nb,), np.float64)
         p.has_kdiffus = 0 if kdiffus is None else 1
         p.kdiffus = 0.0 if kdiffus is None else float(kdiffus)
         p.mask = host(mask, (self.ny, self.nx), np.int8)
@@ -106,9 +115,10 @@ class QgPlan:
             dst_dense = os.environ.get("VARDYN_DST_DENSE", "0") == "1"
         p.dst_dense = 1 if dst_dense else 0
         self.folded = (not dst_dense) and (self.ny - 2) % 2 == 0 and (self.nx - 2) % 2 == 0
-        if gemm_ksplit is None:
-            gemm_ksplit = int(os.environ.get("VARDYN_GEMM_KSPLIT", "0"))
-        p.gemm_ksplit = int(gemm_ksplit)
+        if gemm_config is None:
+            # VARDYN_GEMM_CONFIG=k forces tile configuration k-1 (A/B comparisons)
+            gemm_config = int(os.environ.get("VARDYN_GEMM_CONFIG", "0"))
+        p.gemm_config = int(gemm_config)
         self.params = p
         handle = C.c_void_p()
         with torch.cuda.device(self.device):
@@ -128,12 +138,20 @@ class QgPlan:
     # ---------------------------------------------------------------- helpers
     @property
     def shape(self):
-        return (self.ny, self.nx)
+        """Shape of one field argument: (ny, nx), or (nbatch, ny, nx) for batches."""
+        return (self.ny, self.nx) if self.nb == 1 else (self.nb, self.ny, self.nx)
 
     def tape_bytes(self, nstep):
         b = C.c_size_t()
         _lib.call("vd_qg_tape_bytes", self._h, int(nstep), C.byref(b))
         return b.value
+
+    def autotune(self, reps=10):
+        """Time the GEMM tile configurations on the device and keep the fastest;
+        returns the chosen configuration index of each inversion GEMM."""
+        chosen = (C.c_int * 4)()
+        _lib.call("vd_qg_autotune", self._h, int(reps), chosen, current_stream_ptr(self.device))
+        return list(chosen)
 
     def plan_bytes(self):
         b = C.c_size_t()

@@ -134,14 +134,17 @@ def _line_points(grid: Grid, angle, offset_frac, spacing_m, margin=2.0):
     return iy[keep], ix[keep], d, nrm
 
 
-def nadir_tracks(grid: Grid, n_tracks, spacing_km, n_steps, sigma, seed, obs: ObsSet | None = None):
-    """``n_tracks`` straight tracks at seeded angles/offsets/steps (steps in [1, n_steps])."""
+def nadir_tracks(grid: Grid, n_tracks, spacing_km, n_steps, sigma, seed, obs: ObsSet | None = None,
+                 step_seed=None):
+    """``n_tracks`` straight tracks at seeded angles/offsets (``seed``) observed at
+    seeded steps in [1, n_steps] (``step_seed``, default ``seed``)."""
     rng = np.random.default_rng(seed)
+    rng_step = np.random.default_rng(seed if step_seed is None else step_seed)
     obs = ObsSet() if obs is None else obs
     for _ in range(n_tracks):
         angle = rng.uniform(0, np.pi)
         off = rng.uniform(0.2, 0.8)
-        step = int(rng.integers(1, n_steps + 1))
+        step = int(rng_step.integers(1, n_steps + 1))
         iy, ix, _, _ = _line_points(grid, angle, off, spacing_km * 1e3)
         if iy.size == 0:
             continue
@@ -150,16 +153,18 @@ def nadir_tracks(grid: Grid, n_tracks, spacing_km, n_steps, sigma, seed, obs: Ob
     return obs
 
 
-def swaths(grid: Grid, n_days, steps_per_day, width_km, pixel_km, sigma, seed, obs: ObsSet | None = None):
+def swaths(grid: Grid, n_days, steps_per_day, width_km, pixel_km, sigma, seed, obs: ObsSet | None = None,
+           step_seed=None):
     """One straight swath per day, ``width_km`` wide with ``pixel_km`` pixels."""
     rng = np.random.default_rng(seed)
+    rng_step = np.random.default_rng(seed if step_seed is None else step_seed)
     obs = ObsSet() if obs is None else obs
     n_across = int(round(width_km / pixel_km))
     across = (np.arange(n_across) - 0.5 * (n_across - 1)) * pixel_km * 1e3
     for day in range(n_days):
         angle = rng.uniform(0, np.pi)
         off = rng.uniform(0.3, 0.7)
-        step = int(day * steps_per_day + rng.integers(1, steps_per_day + 1))
+        step = int(day * steps_per_day + rng_step.integers(1, steps_per_day + 1))
         iy0, ix0, d, nrm = _line_points(grid, angle, off, pixel_km * 1e3, margin=-1e9)
         y = iy0[:, None] * grid.dy + across[None, :] * nrm[0]
         x = ix0[:, None] * grid.dx + across[None, :] * nrm[1]
@@ -249,12 +254,16 @@ def case_basin512():
                   swath_per_day=True, checkpoint_steps=54)
 
 
-def make_observations(case: QgCase, grid: Grid, n_steps=None):
+def make_observations(case: QgCase, grid: Grid, n_steps=None, member=0):
+    """Tracks (and swaths) of ``case``.  Batch members share the observation
+    steps (the pass times) and differ in track geometry."""
     n_steps = case.n_steps if n_steps is None else n_steps
-    obs = nadir_tracks(grid, case.n_tracks, case.track_km, n_steps, case.sigma_obs, case.seed + 1)
+    gseed = case.seed + 1 + 1009 * member
+    obs = nadir_tracks(grid, case.n_tracks, case.track_km, n_steps, case.sigma_obs, gseed,
+                       step_seed=case.seed + 1)
     if case.swath_per_day:
         spd = int(round(86400 / case.dt))
         n_days = max(1, int(n_steps // spd))
         swaths(grid, n_days, spd, case.swath_width_km, case.swath_pixel_km, case.sigma_obs,
-               case.seed + 2, obs)
+               case.seed + 2 + 1009 * member, obs, step_seed=case.seed + 2)
     return obs.finalize()
