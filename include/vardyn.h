@@ -74,8 +74,9 @@ typedef struct vd_qg_params {
                             ny-2 and nx-2 are even (half the GEMM work);
                             1: always the full (n-2)^2 sine matrices */
   int gemm_config;       /* GEMM tile configuration: 0 = chosen from the
-                            problem shape; 1..4 forces configuration k-1 of
-                            32x32 (split-K), 64x64, 64x64 (split-K 2), 128x64 */
+                            problem shape; 1..5 forces configuration k-1 of
+                            32x32 (split-K), 64x64, 64x64 (split-K 2),
+                            128x64, 128x64 (split-K 2) */
   int nbatch;            /* independent problems on this grid (0 or 1 = one);
                             with nbatch > 1 every field argument of the step
                             and inversion calls is an (nbatch, ny, nx) array */

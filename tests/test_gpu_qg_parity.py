@@ -167,7 +167,7 @@ def test_null_boundary_equals_zero_field(scheme):
     assert rel(plan.step(H0, None, 5).cpu().numpy(), ref) < 1e-10
 
 
-@pytest.mark.parametrize("cfg", [1, 2, 3, 4])
+@pytest.mark.parametrize("cfg", [1, 2, 3, 4, 5])
 def test_gemm_tile_configs_agree(cfg):
     """Every GEMM tile configuration gives the oracle's inversion (FP64)."""
     ny, nx = 130, 258
@@ -181,6 +181,6 @@ def test_gemm_tile_configs_agree(cfg):
     h = plan.pv2h(plan.field(q), plan.field(hb), plan.field(qb)).cpu().numpy()
     assert rel(h, orc.pv2h(q, hb, qb)) < 1e-12
     chosen = plan.autotune(reps=2)
-    assert all(0 <= c < 4 for c in chosen)
+    assert all(0 <= c < 5 for c in chosen)
     h2 = plan.pv2h(plan.field(q), plan.field(hb), plan.field(qb)).cpu().numpy()
     assert rel(h2, orc.pv2h(q, hb, qb)) < 1e-12

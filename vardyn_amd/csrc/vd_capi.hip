@@ -346,7 +346,7 @@ static int default_tile_cfg(const Plan& p, int stage) {
   int M, N, K, nz;
   gemm_stage_shape(p, stage, &M, &N, &K, &nz);
   const int cand[3] = {3, 1, 0};  // 128x64, 64x64, 32x32
-  const int bm[4] = {32, 64, 64, 128}, bn[4] = {32, 64, 64, 64};
+  const int bm[5] = {32, 64, 64, 128, 128}, bn[5] = {32, 64, 64, 64, 64};
   for (int c : cand) {
     if (M % bm[c] || N % bn[c]) continue;
     if (c == 0 || (long)nz * (M / bm[c]) * (N / bn[c]) >= 256) return c;
@@ -364,7 +364,7 @@ static int autotune(Plan& p, int reps, hipStream_t s) {
   for (int st = 0; st < 4; ++st) {
     int M, N, K, nz;
     gemm_stage_shape(p, st, &M, &N, &K, &nz);
-    const int bm[4] = {32, 64, 64, 128}, bn[4] = {32, 64, 64, 64}, ks[4] = {1, 1, 2, 1};
+    const int bm[5] = {32, 64, 64, 128, 128}, bn[5] = {32, 64, 64, 64, 64}, ks[5] = {1, 1, 2, 1, 2};
     float best = 1e30f;
     int best_cfg = 0;
     for (int c = 0; c < kNumTileCfg; ++c) {
@@ -419,7 +419,7 @@ int vd_qg_plan_create(const vd_qg_params* prm, vd_plan** out) {
   VD_REQUIRE(prm->nbatch >= 0 && prm->nbatch <= 4096, VD_ERR_INVALID_ARG,
              "nbatch must be in 0..4096");
   VD_REQUIRE(prm->gemm_config >= 0 && prm->gemm_config <= kNumTileCfg, VD_ERR_INVALID_ARG,
-             "gemm_config must be in 0..4");
+             "gemm_config must be in 0..5");
   vd_plan* h = new vd_plan();
   Plan& p = h->p;
   p.ny = prm->ny; p.nx = prm->nx;

@@ -31,7 +31,7 @@ void set_error(const std::string& msg);
   } while (0)
 
 constexpr int kTile = 32;      // smallest GEMM output tile and the K chunk
-constexpr int kNumTileCfg = 4;  // GEMM tile configurations (see kTileCfgs)
+constexpr int kNumTileCfg = 5;  // GEMM tile configurations (see kTileCfgs)
 
 inline int round_up(int v, int m) { return (v + m - 1) / m * m; }
 

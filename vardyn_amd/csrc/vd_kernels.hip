@@ -91,7 +91,9 @@ __global__ __launch_bounds__(256 * KS) void k_gemm(GemmArgs<T> g) {
   constexpr int LDT = kTile + 1;
   constexpr int MT = BM / 32, NT = BN / 32;          // 16x16 MFMA blocks per wave
   constexpr int ASZ = BM * LDT, BSZ = kTile * (BN + 1);
-  static_assert(KS == 1 || BM * BN <= ASZ + BSZ, "split-K partials must fit the staging area");
+  // split-K partials of groups 1..KS-1 are parked over the whole staging block
+  static_assert(KS == 1 || (KS - 1) * BM * BN <= KS * (ASZ + BSZ),
+                "split-K partials must fit the staging area");
   __shared__ T sm[KS][ASZ + BSZ];
   const int tid = threadIdx.x;
   const int grp = tid >> 8;          // K group
@@ -252,11 +254,13 @@ __global__ __launch_bounds__(256 * KS) void k_gemm(GemmArgs<T> g) {
       }
 
   if (KS > 1) {
-    // groups 1..KS-1 park their partial tiles in their (now idle) LDS slots
+    // groups 1..KS-1 park their partial tiles in the (now idle) staging block;
+    // group 0 adds them in group order
     __syncthreads();
-    T* park = &sm[grp][0];
+    T* flat = &sm[0][0];
     constexpr int PER = MT * NT * 4;
     if (grp > 0) {
+      T* park = flat + (size_t)(grp - 1) * BM * BN;
 #pragma unroll
       for (int a_ = 0; a_ < MT; ++a_)
 #pragma unroll
@@ -269,7 +273,7 @@ __global__ __launch_bounds__(256 * KS) void k_gemm(GemmArgs<T> g) {
     if (grp != 0) return;
 #pragma unroll
     for (int q = 1; q < KS; ++q) {
-      const T* src = &sm[q][0];
+      const T* src = flat + (size_t)(q - 1) * BM * BN;
 #pragma unroll
       for (int a_ = 0; a_ < MT; ++a_)
 #pragma unroll
@@ -331,7 +335,8 @@ __global__ __launch_bounds__(256 * KS) void k_gemm(GemmArgs<T> g) {
 struct TileCfg {
   int bm, bn, ks;
 };
-constexpr TileCfg kTileCfgs[kNumTileCfg] = {{32, 32, 4}, {64, 64, 1}, {64, 64, 2}, {128, 64, 1}};
+constexpr TileCfg kTileCfgs[kNumTileCfg] = {
+    {32, 32, 4}, {64, 64, 1}, {64, 64, 2}, {128, 64, 1}, {128, 64, 2}};
 
 template <typename T, int AOP, int BOP, int KIND, int MAP>
 static void gemm(GemmArgs<T> a, int M, int N, int nz, int cfg, hipStream_t s) {
@@ -362,8 +367,10 @@ static void gemm(GemmArgs<T> a, int M, int N, int nz, int cfg, hipStream_t s) {
     hipLaunchKernelGGL((k_gemm<T, AOP, BOP, KIND, MAP, 64, 64, 1>), grid, dim3(256), 0, s, a);
   } else if (cfg == 2) {
     hipLaunchKernelGGL((k_gemm<T, AOP, BOP, KIND, MAP, 64, 64, 2>), grid, dim3(512), 0, s, a);
-  } else {
+  } else if (cfg == 3) {
     hipLaunchKernelGGL((k_gemm<T, AOP, BOP, KIND, MAP, 128, 64, 1>), grid, dim3(256), 0, s, a);
+  } else {
+    hipLaunchKernelGGL((k_gemm<T, AOP, BOP, KIND, MAP, 128, 64, 2>), grid, dim3(512), 0, s, a);
   }
 }
 
