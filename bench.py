@@ -52,6 +52,8 @@ def parse():
                     help="independent 512x512 problems advanced in the same launches")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the shape-based GEMM tile choice instead of timing the candidates")
+    ap.add_argument("--no-stencil", action="store_true",
+                    help="skip the batched stencil HBM measurement")
     ap.add_argument("--check-f32", action="store_true",
                     help="also evaluate the FP32 path and report its gradient error vs FP64")
     return ap.parse_args()
@@ -61,9 +63,12 @@ def gemm_roofline(plan, precision, n_inv=100, reps=5):
     """Average duration of the inversion GEMMs on the GPU: n_inv inversions are
     captured in one HIP graph (so host launch cost is out of the picture) and
     the replays are timed with HIP events on the stream they run on.
-    Algorithmic flops: 4 m_y m_x (m_y + m_x) per inversion and batch member
-    with the full sine matrices; the folded path executes half of them, and
-    ``frac`` is taken on the flops actually executed."""
+
+    Algorithmic flops per inversion and batch member: 4 m_y m_x (m_y + m_x)
+    (two 2-D sine transforms as dense products), a quarter of it per GEMM
+    launch; ``achieved`` uses that count.  With folded sine matrices the
+    kernels execute half of it: ``executed_tflops`` / ``executed_frac`` give
+    the MFMA utilisation on the flops actually issued."""
     import torch
     q = torch.randn(plan.shape, dtype=plan.dtype, device=plan.device) * 1e-6
     hb = torch.zeros_like(q)
@@ -86,18 +91,63 @@ def gemm_roofline(plan, precision, n_inv=100, reps=5):
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1)
     my, mx = plan.ny - 2, plan.nx - 2
-    dense_flops_inv = 4.0 * my * mx * (my + mx) * plan.nb
-    folded = plan.folded
-    exec_flops_inv = dense_flops_inv / 2 if folded else dense_flops_inv
+    alg_flops_launch = my * mx * (my + mx) * plan.nb
+    exec_flops_launch = alg_flops_launch / 2 if plan.folded else alg_flops_launch
     per_launch_s = ms * 1e-3 / (4 * n_inv * reps)
-    achieved = exec_flops_inv / 4 / per_launch_s / 1e12
     peak = MFMA_PEAK["f64" if precision == "float64" else "f32"]
-    return {"bound": "mfma", "kernel": "k_gemm (sine-matrix GEMMs of the Helmholtz inversion)",
+    achieved = alg_flops_launch / per_launch_s / 1e12
+    executed = exec_flops_launch / per_launch_s / 1e12
+    return {"bound": "mfma", "kernel": "k_gemm / k_gemm_rd (sine-matrix GEMMs of the Helmholtz "
+                                       "inversion, 4 launches per inversion)",
             "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
             "frac": round(achieved / peak, 4), "traffic": None,
             "avg_launch_us": round(per_launch_s * 1e6, 3),
-            "flops_per_launch": exec_flops_inv / 4, "folded_dst": folded,
-            "dense_equivalent_tflops": round(dense_flops_inv / 4 / per_launch_s / 1e12, 3)}
+            "algorithmic_flops_per_launch": alg_flops_launch,
+            "executed_flops_per_launch": exec_flops_launch, "folded_dst": plan.folded,
+            "executed_tflops": round(executed, 3), "executed_frac": round(executed / peak, 4)}
+
+
+def stencil_roofline(case, precision, nbatch=64, n_launch=20, reps=5):
+    """HBM roofline of the fused RK stage stencil (``k_stage`` behind
+    ``vd_qg_rhs``: out = base + dt * rhs(h, q)) on ``nbatch`` problems of the
+    workload's grid at once, so that its working set (three input fields and
+    one output per problem) is far larger than the 256 MB Infinity Cache.
+    Algorithmic bytes per point: read h, q, base and write out (4 words) plus
+    the shared int8 mask once per launch."""
+    import torch
+    from vardyn_amd.qg import QgPlan
+    plan = QgPlan(case.ny, case.nx, case.dx, case.dx, case.dt, case.c, case.f0,
+                  time_scheme=case.time_scheme, precision=precision, nbatch=nbatch)
+    g = torch.Generator(device=plan.device).manual_seed(5)
+    h = 0.1 * torch.randn(plan.shape, dtype=plan.dtype, device=plan.device, generator=g)
+    q = plan.h2pv(h, torch.zeros_like(h))
+    out = torch.empty_like(q)
+    for _ in range(3):
+        plan.rhs(h, q, q, case.dt, out)
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        for _ in range(n_launch):
+            plan.rhs(h, q, q, case.dt, out)
+    graph.replay()
+    torch.cuda.synchronize()
+    s = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    for _ in range(reps):
+        graph.replay()
+    e1.record(s)
+    torch.cuda.synchronize()
+    per_launch_s = e0.elapsed_time(e1) * 1e-3 / (n_launch * reps)
+    npts = case.ny * case.nx
+    nbytes = nbatch * npts * 4 * q.element_size() + npts
+    gbs = nbytes / per_launch_s / 1e9
+    del plan, h, q, out, graph
+    torch.cuda.empty_cache()
+    return {"bound": "hbm", "kernel": "k_stage (fused h2uv + upwind advection + RK stage update)",
+            "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "batch": nbatch,
+            "bytes_per_launch": nbytes, "avg_launch_us": round(per_launch_s * 1e6, 2)}
 
 
 def cpu_baseline(case, sample_steps):
@@ -172,6 +222,10 @@ def main():
 
     roof = gemm_roofline(pb.plan, args.precision)
     log(f"roofline: {roof}")
+    stencil = None
+    if not args.no_stencil:
+        stencil = stencil_roofline(case, args.precision)
+        log(f"stencil roofline: {stencil}")
     out = {
         "metric": METRIC, "value": round(value, 4), "unit": "gradients/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 2),
@@ -189,6 +243,8 @@ def main():
         "J": Jv, "grad_norm": gnorm,
         "roofline": roof,
     }
+    if stencil is not None:
+        out["stencil_roofline"] = stencil
     if args.check_f32 and args.precision == "float64" and nb == 1:
         pb32 = build_problem(case, precision="float32", n_steps=n_steps, control="ic",
                              tape=args.tape, truth_plan=pb.plan)

@@ -10,8 +10,9 @@
  *   - Fields are row-major (ny, nx) arrays in device memory, axis 0 = y,
  *     contiguous in x, of the plan precision (float for VD_F32, double for
  *     VD_F64).  A plan made for nbatch > 1 problems takes (nbatch, ny, nx)
- *     arrays instead and advances every member in the same launches.  Observation values, weights, misfits and cost accumulators
- *     are double.
+ *     arrays instead and advances every member in the same launches.
+ *     Observation values, weights, misfits and cost accumulators are
+ *     double.
  *   - The caller owns every buffer passed in.  A plan owns its constants
  *     (sine matrices, 1/Lambda, masks) and a scratch set, allocated by
  *     vd_qg_plan_create; calls that share a plan must not overlap (the
@@ -74,9 +75,12 @@ typedef struct vd_qg_params {
                             ny-2 and nx-2 are even (half the GEMM work);
                             1: always the full (n-2)^2 sine matrices */
   int gemm_config;       /* GEMM tile configuration: 0 = chosen from the
-                            problem shape; 1..5 forces configuration k-1 of
-                            32x32 (split-K), 64x64, 64x64 (split-K 2),
-                            128x64, 128x64 (split-K 2) */
+                            problem shape; k in 1..10 forces configuration
+                            k-1 of: LDS-staged 32x32 (split-K), 64x64,
+                            64x64 (split-K 2), 128x64, 128x64 (split-K 2);
+                            register-direct 32x32 (split-K 4), 32x32
+                            (split-K 2), 64x64 (split-K 2), 64x64 (split-K
+                            2, shallower prefetch), 128x64 */
   int nbatch;            /* independent problems on this grid (0 or 1 = one);
                             with nbatch > 1 every field argument of the step
                             and inversion calls is an (nbatch, ny, nx) array */
@@ -147,6 +151,15 @@ int vd_qg_pv2h(vd_plan* plan, const void* q, const void* hb, const void* qb, voi
 
 /* q = h2pv(h, hb) (jqgm.py:264-293). */
 int vd_qg_h2pv(vd_plan* plan, const void* h, const void* hb, void* q, void* stream);
+
+/* out = base + coef * rhs(h, q): the PV tendency of Qgm.rhs (jqgm.py:321-375,
+ * with the upwind advection of Qgm.adv :377-396 and the beta, diffusion and
+ * bathymetry terms the plan was created with) fused with one explicit stage
+ * update -- Qgm.euler (jqgm.py:398-409) is base = q, coef = dt.  base NULL
+ * means zero (out = coef * rhs).  rhs is zero outside the active points
+ * (mask 3), as in the reference. */
+int vd_qg_rhs(vd_plan* plan, const void* h, const void* q, const void* base, double coef,
+              void* out, void* stream);
 
 /* field[i] = value where the plan mask is 0 (land); the reference fills the
  * initial state with NaN there (mapping/src/mod.py:697-700). */

@@ -167,10 +167,13 @@ def test_null_boundary_equals_zero_field(scheme):
     assert rel(plan.step(H0, None, 5).cpu().numpy(), ref) < 1e-10
 
 
-@pytest.mark.parametrize("cfg", [1, 2, 3, 4, 5])
-def test_gemm_tile_configs_agree(cfg):
-    """Every GEMM tile configuration gives the oracle's inversion (FP64)."""
-    ny, nx = 130, 258
+@pytest.mark.parametrize("shape", [(130, 258), (258, 258)])
+@pytest.mark.parametrize("cfg", list(range(1, 11)))
+def test_gemm_tile_configs_agree(cfg, shape):
+    """Every GEMM tile configuration gives the oracle's inversion (FP64).  At
+    130 x 258 the folded K of 64 does not split four ways, so the split-K 4
+    configurations fall back to the default; at 258 x 258 every one applies."""
+    ny, nx = shape
     rng = np.random.default_rng(3)
     orc = QgmOracle(dx=5e3, dy=6e3, dt=900.0, c=2.0, f=1e-4, time_scheme="rk2", shape=(ny, nx))
     plan = QgPlan(ny, nx, 5e3, 6e3, 900.0, 2.0, 1e-4, time_scheme="rk2", precision="float64",
@@ -181,6 +184,37 @@ def test_gemm_tile_configs_agree(cfg):
     h = plan.pv2h(plan.field(q), plan.field(hb), plan.field(qb)).cpu().numpy()
     assert rel(h, orc.pv2h(q, hb, qb)) < 1e-12
     chosen = plan.autotune(reps=2)
-    assert all(0 <= c < 5 for c in chosen)
+    assert all(0 <= c < 10 for c in chosen)
     h2 = plan.pv2h(plan.field(q), plan.field(hb), plan.field(qb)).cpu().numpy()
     assert rel(h2, orc.pv2h(q, hb, qb)) < 1e-12
+
+
+@pytest.mark.parametrize("cfg", [1, 6, 8, 10])
+def test_gemm_tile_configs_f32(cfg):
+    """LDS-staged and register-direct configurations on the FP32 MFMA path
+    (FP32 rounding over four sine products: 1e-5 relative)."""
+    ny, nx = 258, 258
+    rng = np.random.default_rng(4)
+    orc = QgmOracle(dx=5e3, dy=6e3, dt=900.0, c=2.0, f=1e-4, time_scheme="rk2", shape=(ny, nx))
+    plan = QgPlan(ny, nx, 5e3, 6e3, 900.0, 2.0, 1e-4, time_scheme="rk2", precision="float32",
+                  gemm_config=cfg)
+    q = 1e-5 * rng.standard_normal((ny, nx))
+    hb = 0.01 * rng.standard_normal((ny, nx))
+    qb = orc.h2pv(hb, hb)
+    h = plan.pv2h(plan.field(q), plan.field(hb), plan.field(qb)).cpu().numpy()
+    assert rel(h, orc.pv2h(q, hb, qb)) < 1e-5
+
+
+@pytest.mark.parametrize("opts", [dict(), dict(land=True, beta=True, kdiff=True, bathy=True)])
+def test_rhs_stage_f64(opts):
+    """vd_qg_rhs: rhs(h, q) alone and fused with an Euler update, against the
+    oracle's rhs (jqgm.py:321-375)."""
+    orc, plan, h0, hb, rng = make_pair(36, 44, "Euler", "float64", **opts)
+    q = orc.h2pv(h0, hb)
+    ref = orc.rhs(*orc.h2uv(h0), q)
+    H, Q = plan.field(h0), plan.field(q)
+    r = plan.rhs(H, Q).cpu().numpy()
+    assert rel(r, ref) < 1e-12
+    e = plan.rhs(H, Q, base=Q, coef=900.0).cpu().numpy()
+    assert rel(e, q + 900.0 * ref) < 1e-13
+

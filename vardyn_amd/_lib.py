@@ -60,6 +60,7 @@ SIGNATURES = {
     "vd_qg_step_adj": (_I, [_VP, _VP, _VP, _VP, _VP, _I, _VP, _I, _VP]),
     "vd_qg_pv2h": (_I, [_VP, _VP, _VP, _VP, _VP, _VP]),
     "vd_qg_h2pv": (_I, [_VP, _VP, _VP, _VP, _VP]),
+    "vd_qg_rhs": (_I, [_VP, _VP, _VP, _VP, C.c_double, _VP, _VP]),
     "vd_qg_fill_land": (_I, [_VP, _VP, _D, _VP]),
     "vd_obs_misfit": (_I, [C.POINTER(ObsTable), _I, _VP, _VP, _VP, _VP, _D, _VP]),
     "vd_obs_adjoint": (_I, [C.POINTER(ObsTable), _I, _VP, _VP, _VP, _VP]),

@@ -346,10 +346,9 @@ static int default_tile_cfg(const Plan& p, int stage) {
   int M, N, K, nz;
   gemm_stage_shape(p, stage, &M, &N, &K, &nz);
   const int cand[3] = {3, 1, 0};  // 128x64, 64x64, 32x32
-  const int bm[5] = {32, 64, 64, 128, 128}, bn[5] = {32, 64, 64, 64, 64};
   for (int c : cand) {
-    if (M % bm[c] || N % bn[c]) continue;
-    if (c == 0 || (long)nz * (M / bm[c]) * (N / bn[c]) >= 256) return c;
+    if (!tile_cfg_fits(c, M, N, K)) continue;
+    if (c == 0 || (long)nz * (M / kTileCfgs[c].bm) * (N / kTileCfgs[c].bn) >= 256) return c;
   }
   return 0;
 }
@@ -364,11 +363,10 @@ static int autotune(Plan& p, int reps, hipStream_t s) {
   for (int st = 0; st < 4; ++st) {
     int M, N, K, nz;
     gemm_stage_shape(p, st, &M, &N, &K, &nz);
-    const int bm[5] = {32, 64, 64, 128, 128}, bn[5] = {32, 64, 64, 64, 64}, ks[5] = {1, 1, 2, 1, 2};
     float best = 1e30f;
     int best_cfg = 0;
     for (int c = 0; c < kNumTileCfg; ++c) {
-      if (M % bm[c] || N % bn[c] || (K / kTile) % ks[c]) continue;
+      if (!tile_cfg_fits(c, M, N, K)) continue;
       launch_gemm_stage<T>(p, st, F, nullptr, nullptr, out, INV_FWD, c, s);  // warm
       VD_HIP_TRY(hipEventRecord(e0, s));
       for (int r = 0; r < reps; ++r) launch_gemm_stage<T>(p, st, F, nullptr, nullptr, out, INV_FWD, c, s);
@@ -419,7 +417,7 @@ int vd_qg_plan_create(const vd_qg_params* prm, vd_plan** out) {
   VD_REQUIRE(prm->nbatch >= 0 && prm->nbatch <= 4096, VD_ERR_INVALID_ARG,
              "nbatch must be in 0..4096");
   VD_REQUIRE(prm->gemm_config >= 0 && prm->gemm_config <= kNumTileCfg, VD_ERR_INVALID_ARG,
-             "gemm_config must be in 0..5");
+             "gemm_config must be in 0..10");
   vd_plan* h = new vd_plan();
   Plan& p = h->p;
   p.ny = prm->ny; p.nx = prm->nx;
@@ -571,6 +569,24 @@ int vd_qg_h2pv(vd_plan* h, const void* hh, const void* hb, void* q, void* stream
     launch_h2pv<double>(p, (const double*)hh, (const double*)hb, (double*)q, s);
   else
     launch_h2pv<float>(p, (const float*)hh, (const float*)hb, (float*)q, s);
+  VD_HIP_TRY(hipGetLastError());
+  return VD_OK;
+}
+
+int vd_qg_rhs(vd_plan* h, const void* hh, const void* q, const void* base, double coef, void* out,
+              void* stream) {
+  VD_REQUIRE(h && hh && q && out, VD_ERR_INVALID_ARG, "null argument");
+  Plan& p = h->p;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (p.prec == VD_F64) {
+    const double* b = base ? (const double*)base : fld<double>(p, F_ZERO);
+    launch_stage<double>(p, (const double*)hh, (const double*)q, b, nullptr, nullptr, 0.0, 0.0,
+                         coef, (double*)out, nullptr, s);
+  } else {
+    const float* b = base ? (const float*)base : fld<float>(p, F_ZERO);
+    launch_stage<float>(p, (const float*)hh, (const float*)q, b, nullptr, nullptr, 0.f, 0.f,
+                        (float)coef, (float*)out, nullptr, s);
+  }
   VD_HIP_TRY(hipGetLastError());
   return VD_OK;
 }

@@ -30,8 +30,24 @@ void set_error(const std::string& msg);
     }                                \
   } while (0)
 
-constexpr int kTile = 32;      // smallest GEMM output tile and the K chunk
-constexpr int kNumTileCfg = 5;  // GEMM tile configurations (see kTileCfgs)
+constexpr int kTile = 32;  // smallest GEMM output tile and the K chunk
+
+// GEMM tile configurations: BM x BN output tile per workgroup, ks K groups of
+// four waves, and the operand path -- staged through LDS (rd = 0) or loaded
+// straight into MFMA registers with nbuf 16-wide K chunks in flight per wave
+// (rd = 1).  Configuration 0 picks ks from K (4, 2 or 1 groups).
+struct TileCfg {
+  int bm, bn, ks, rd, nbuf;
+};
+constexpr int kNumTileCfg = 10;
+inline constexpr TileCfg kTileCfgs[kNumTileCfg] = {
+    {32, 32, 4, 0, 0},  {64, 64, 1, 0, 0},  {64, 64, 2, 0, 0}, {128, 64, 1, 0, 0},
+    {128, 64, 2, 0, 0}, {32, 32, 4, 1, 4},  {32, 32, 2, 1, 4}, {64, 64, 2, 1, 4},
+    {64, 64, 2, 1, 2},  {128, 64, 1, 1, 2}};
+inline bool tile_cfg_fits(int c, int M, int N, int K) {
+  return M % kTileCfgs[c].bm == 0 && N % kTileCfgs[c].bn == 0 &&
+         (c == 0 || (K / kTile) % kTileCfgs[c].ks == 0);
+}
 
 inline int round_up(int v, int m) { return (v + m - 1) / m * m; }
 

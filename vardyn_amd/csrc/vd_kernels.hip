@@ -45,8 +45,8 @@ template <> struct Mfma<double> {
 // one launch.
 //
 // Operands: A dense (AOP_DENSE) or column-folded (AOP_FOLDCOL); B dense,
-// the interior of a field minus an optional second field (BOP_FIELD), or
-// row-folded (BOP_FOLDROW_*).  All dense operands are zero-padded to
+// the interior of a field, optionally minus a second field (BOP_FIELD,
+// BOP_FIELD_SUB), or row-folded (BOP_FOLDROW_*).  All dense operands are zero-padded to
 // multiples of 32, so the K loop has no bounds checks.
 //
 // Folding: for an even interior size m, S[j][m-1-k] = (-1)^j S[j][k] and S
@@ -58,7 +58,14 @@ template <> struct Mfma<double> {
 // natural order.
 // ---------------------------------------------------------------------------
 enum { AOP_DENSE = 0, AOP_FOLDCOL = 1 };
-enum { BOP_DENSE = 0, BOP_FIELD = 1, BOP_FOLDROW_FIELD = 2, BOP_FOLDROW_DENSE = 3 };
+enum {
+  BOP_DENSE = 0,
+  BOP_FIELD = 1,
+  BOP_FOLDROW_FIELD = 2,
+  BOP_FOLDROW_DENSE = 3,
+  BOP_FIELD_SUB = 4,
+  BOP_FOLDROW_FIELD_SUB = 5
+};
 enum { EK_STORE = 0, EK_SCALE = 1, EK_FIELD_SET = 2, EK_FIELD_ACC = 3 };
 enum { MAP_PLAIN = 0, MAP_ROWFOLD = 1, MAP_COLFOLD = 2 };
 
@@ -85,6 +92,161 @@ struct GemmArgs {
   int nz, tm, tn, nb;
 };
 
+// Operand elements.  A(i, l): dense, or column-folded (sum/difference of
+// columns l and mx-1-l).  B(k, n): dense, the interior of field F (minus a
+// second field Fsub for the *_SUB kinds), or row-folded.  Indices past the
+// folded/interior extent read as zero.
+//
+// Each access is split into fetch (the raw words, from clamped -- always
+// valid -- addresses) and combine (fold, subtraction and range select).
+// Kernels fetch a chunk ahead and combine only where the value is consumed,
+// so no arithmetic on a prefetched word forces a wait for it early and every
+// load of a chunk is in flight while the previous chunk is multiplied.
+template <int AOP>
+constexpr int a_words() { return AOP == AOP_DENSE ? 1 : 2; }
+template <int BOP>
+constexpr int b_words() {
+  return (BOP == BOP_DENSE || BOP == BOP_FIELD) ? 1 : (BOP == BOP_FOLDROW_FIELD_SUB ? 4 : 2);
+}
+template <typename T, int AOP>
+__device__ __forceinline__ void fetch_a(const GemmArgs<T>& g, const T* A, int i, int l, T* r) {
+  if (AOP == AOP_DENSE) {
+    r[0] = A[(size_t)i * g.lda + l];
+  } else {
+    const int lc = min(l, g.hx - 1);
+    r[0] = A[(size_t)i * g.lda + lc];
+    r[1] = A[(size_t)i * g.lda + (g.mx - 1 - lc)];
+  }
+}
+template <typename T, int AOP>
+__device__ __forceinline__ T combine_a(const GemmArgs<T>& g, int l, const T* r, T sgn) {
+  if (AOP == AOP_DENSE) return r[0];
+  return l < g.hx ? r[0] + sgn * r[1] : T(0);
+}
+template <typename T, int BOP>
+__device__ __forceinline__ void fetch_b(const GemmArgs<T>& g, const T* B, const T* Fld,
+                                        const T* Fsub, int k, int n, T* r) {
+  if (BOP == BOP_DENSE) {
+    r[0] = B[(size_t)k * g.ldb + n];
+  } else if (BOP == BOP_FOLDROW_DENSE) {
+    const int kc = min(k, g.hy - 1);
+    r[0] = B[(size_t)kc * g.ldb + n];
+    r[1] = B[(size_t)(g.my - 1 - kc) * g.ldb + n];
+  } else if (BOP == BOP_FIELD || BOP == BOP_FIELD_SUB) {
+    const size_t idx = (size_t)(min(k, g.my - 1) + 1) * g.nx + (min(n, g.mx - 1) + 1);
+    r[0] = Fld[idx];
+    if (BOP == BOP_FIELD_SUB) r[1] = Fsub[idx];
+  } else {
+    const int kc = min(k, g.hy - 1), nc = min(n, g.mx - 1);
+    const size_t i1 = (size_t)(kc + 1) * g.nx + (nc + 1);
+    const size_t i2 = (size_t)(g.my - kc) * g.nx + (nc + 1);
+    r[0] = Fld[i1];
+    r[1] = Fld[i2];
+    if (BOP == BOP_FOLDROW_FIELD_SUB) {
+      r[2] = Fsub[i1];
+      r[3] = Fsub[i2];
+    }
+  }
+}
+template <typename T, int BOP>
+__device__ __forceinline__ T combine_b(const GemmArgs<T>& g, int k, int n, const T* r, T sgn) {
+  if (BOP == BOP_DENSE) return r[0];
+  if (BOP == BOP_FOLDROW_DENSE) return k < g.hy ? r[0] + sgn * r[1] : T(0);
+  if (BOP == BOP_FIELD) return (k < g.my && n < g.mx) ? r[0] : T(0);
+  if (BOP == BOP_FIELD_SUB) return (k < g.my && n < g.mx) ? r[0] - r[1] : T(0);
+  if (BOP == BOP_FOLDROW_FIELD) return (k < g.hy && n < g.mx) ? r[0] + sgn * r[1] : T(0);
+  return (k < g.hy && n < g.mx) ? (r[0] - r[2]) + sgn * (r[1] - r[3]) : T(0);
+}
+
+// Work item of a workgroup.  Workgroups are dispatched round-robin over the 8
+// XCDs (linear id % 8), each with its own L2; the mapping gives every XCD
+// whole products (or a contiguous run of row-panels of one product) so that
+// its L2 holds one member's operands.  z = parity * nb + member for folded
+// products, member otherwise.
+template <typename T>
+__device__ __forceinline__ void work_item(const GemmArgs<T>& g, int* z, int* tile) {
+  const int L = blockIdx.x, T_ = g.tm * g.tn;
+  const int x = L & 7, slot = L >> 3;
+  if (g.nz % 8 == 0) {
+    *z = x + 8 * (slot / T_);
+    *tile = slot % T_;
+  } else if (8 % g.nz == 0 && T_ % (8 / g.nz) == 0) {
+    const int per = 8 / g.nz;
+    *z = x % g.nz;
+    *tile = (x / g.nz) * (T_ / per) + slot;
+  } else {
+    *z = L / T_;
+    *tile = L % T_;
+  }
+}
+
+// Operand and output pointers of one product.
+template <typename T>
+struct GemmPtrs {
+  const T *A, *B, *Fld, *Fsub, *scale, *hbm;
+  T *Cout, *fout;
+  int par;
+  T sgn;
+};
+template <typename T, int AOP, int BOP, int MAP>
+__device__ __forceinline__ GemmPtrs<T> gemm_ptrs(const GemmArgs<T>& g, int z) {
+  constexpr bool FOLD = MAP != MAP_PLAIN;
+  GemmPtrs<T> P;
+  P.par = FOLD ? z / g.nb : 0;
+  const size_t mb = FOLD ? z % g.nb : z;
+  P.sgn = P.par ? T(-1) : T(1);
+  P.A = g.A + mb * g.bsA + (AOP == AOP_DENSE ? P.par * g.strideA : 0);
+  P.B = g.B + mb * g.bsB + (BOP == BOP_DENSE ? P.par * g.strideB : 0);
+  P.Fld = g.F + mb * g.bsF;
+  P.Fsub = g.Fsub ? g.Fsub + mb * g.bsF : nullptr;
+  P.Cout = g.C + mb * g.bsC;
+  P.scale = g.scale + mb * g.bsScale;
+  P.fout = g.out + mb * g.bsOut;
+  P.hbm = g.hb ? g.hb + mb * g.bsOut : nullptr;
+  return P;
+}
+
+// Epilogue for output element (row, col) of a product.
+template <typename T, int KIND, int MAP>
+__device__ __forceinline__ void store_out(const GemmArgs<T>& g, const GemmPtrs<T>& P, int row,
+                                          int col, T v) {
+  if (MAP == MAP_ROWFOLD) {
+    if (row < g.hy) P.Cout[(size_t)(2 * row + P.par) * g.ldc + col] = v;
+    return;
+  }
+  const int oc = MAP == MAP_COLFOLD ? 2 * col + P.par : col;
+  if (KIND == EK_STORE || KIND == EK_SCALE) {
+    if (MAP == MAP_COLFOLD && col >= g.hx) return;
+    const size_t o = (size_t)row * g.ldc + oc;
+    P.Cout[o] = KIND == EK_SCALE ? v * P.scale[o] : v;
+    return;
+  }
+  const bool ok = row < g.my && (MAP == MAP_COLFOLD ? col < g.hx : col < g.mx);
+  if (!ok) return;
+  const int nx = g.nx, ny = g.ny;
+  const size_t i = row + 1, j = oc + 1;
+  T* fout = P.fout;
+  if (KIND == EK_FIELD_ACC) {
+    fout[i * nx + j] += v;
+    return;
+  }
+  const T* hb = P.hbm;
+  fout[i * nx + j] = (hb ? hb[i * nx + j] : T(0)) + v;
+  // ring: h = hb on rows 0, ny-1 and columns 0, nx-1 (jqgm.py:316-317)
+  auto ring = [&](size_t ii, size_t jj) { fout[ii * nx + jj] = hb ? hb[ii * nx + jj] : T(0); };
+  const bool top = row == 0, bot = row == g.my - 1, lft = oc == 0, rgt = oc == g.mx - 1;
+  if (top) ring(0, j);
+  if (bot) ring(ny - 1, j);
+  if (lft) ring(i, 0);
+  if (rgt) ring(i, nx - 1);
+  if (top && lft) ring(0, 0);
+  if (top && rgt) ring(0, nx - 1);
+  if (bot && lft) ring(ny - 1, 0);
+  if (bot && rgt) ring(ny - 1, nx - 1);
+}
+
+// LDS-staged kernel: each K chunk of 32 is loaded by the whole group into
+// LDS (one chunk of register prefetch) and shared by its four waves.
 template <typename T, int AOP, int BOP, int KIND, int MAP, int BM, int BN, int KS>
 __global__ __launch_bounds__(256 * KS) void k_gemm(GemmArgs<T> g) {
   typedef typename Mfma<T>::acc_t acc_t;
@@ -101,40 +263,10 @@ __global__ __launch_bounds__(256 * KS) void k_gemm(GemmArgs<T> g) {
   const int lane = tid & 63;
   const int wq = (tid >> 6) & 3;     // quadrant of the tile owned by this wave
   const int wm = wq >> 1, wn = wq & 1;
-  // Work item of this workgroup.  Workgroups are dispatched round-robin over
-  // the 8 XCDs (linear id % 8), each with its own L2; the mapping gives every
-  // XCD whole products (or a contiguous run of row-panels of one product) so
-  // that its L2 holds one member's operands.
   int z, tile;
-  {
-    const int L = blockIdx.x, T_ = g.tm * g.tn;
-    const int x = L & 7, slot = L >> 3;
-    if (g.nz % 8 == 0) {
-      z = x + 8 * (slot / T_);
-      tile = slot % T_;
-    } else if (8 % g.nz == 0 && T_ % (8 / g.nz) == 0) {
-      const int per = 8 / g.nz;
-      z = x % g.nz;
-      tile = (x / g.nz) * (T_ / per) + slot;
-    } else {
-      z = L / T_;
-      tile = L % T_;
-    }
-  }
-  // z = parity * nb + member for folded products, member otherwise
-  constexpr bool FOLD = MAP != MAP_PLAIN;
-  const int par = FOLD ? z / g.nb : 0;
-  const size_t mb = FOLD ? z % g.nb : z;
-  const T sgn = par ? T(-1) : T(1);
+  work_item(g, &z, &tile);
+  const GemmPtrs<T> P = gemm_ptrs<T, AOP, BOP, MAP>(g, z);
   const int m0 = (tile / g.tn) * BM, n0 = (tile % g.tn) * BN;
-  const T* A = g.A + mb * g.bsA + (AOP == AOP_DENSE ? par * g.strideA : 0);
-  const T* B = g.B + mb * g.bsB + (BOP == BOP_DENSE ? par * g.strideB : 0);
-  const T* Fld = g.F + mb * g.bsF;
-  const T* Fsub = g.Fsub ? g.Fsub + mb * g.bsF : nullptr;
-  T* Cout = g.C + mb * g.bsC;
-  const T* scale = g.scale + mb * g.bsScale;
-  T* fout = g.out + mb * g.bsOut;
-  const T* hbm = g.hb ? g.hb + mb * g.bsOut : nullptr;
   T* As = &sm[grp][0];               // [BM][LDT]
   T* Bs = &sm[grp][ASZ];             // [kTile][BN + 1]
 
@@ -143,55 +275,18 @@ __global__ __launch_bounds__(256 * KS) void k_gemm(GemmArgs<T> g) {
   constexpr int APASS = BM / 32, BPASS = BN / 32, BROWS = 1024 / BN;
   const int ar = gt >> 3, ac = (gt & 7) * 4;
   const int br = gt / (BN / 4), bc = (gt % (BN / 4)) * 4;
-  T ra[APASS][4], rb[BPASS][4];
-  auto load = [&](int k0) {
+  constexpr int NA = a_words<AOP>(), NB = b_words<BOP>();
+  T ra[APASS][4][NA], rb[BPASS][4][NB];
+  auto fetch = [&](int k0) {
 #pragma unroll
-    for (int ps = 0; ps < APASS; ++ps) {
-      const int i = m0 + ar + 32 * ps;
+    for (int ps = 0; ps < APASS; ++ps)
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int l = k0 + ac + u;
-        if (AOP == AOP_DENSE) {
-          ra[ps][u] = A[(size_t)i * g.lda + l];
-        } else {
-          ra[ps][u] = l < g.hx ? A[(size_t)i * g.lda + l] +
-                                     sgn * A[(size_t)i * g.lda + (g.mx - 1 - l)]
-                               : T(0);
-        }
-      }
-    }
+      for (int u = 0; u < 4; ++u) fetch_a<T, AOP>(g, P.A, m0 + ar + 32 * ps, k0 + ac + u, ra[ps][u]);
 #pragma unroll
-    for (int ps = 0; ps < BPASS; ++ps) {
-      const int k = k0 + br + BROWS * ps;
+    for (int ps = 0; ps < BPASS; ++ps)
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int n = n0 + bc + u;
-        T v;
-        if (BOP == BOP_DENSE) {
-          v = B[(size_t)k * g.ldb + n];
-        } else if (BOP == BOP_FOLDROW_DENSE) {
-          v = k < g.hy ? B[(size_t)k * g.ldb + n] + sgn * B[(size_t)(g.my - 1 - k) * g.ldb + n]
-                       : T(0);
-        } else if (BOP == BOP_FIELD) {
-          v = T(0);
-          if (k < g.my && n < g.mx) {
-            const size_t idx = (size_t)(k + 1) * g.nx + (n + 1);
-            v = Fld[idx];
-            if (Fsub) v -= Fsub[idx];
-          }
-        } else {
-          v = T(0);
-          if (k < g.hy && n < g.mx) {
-            const size_t i1 = (size_t)(k + 1) * g.nx + (n + 1);
-            const size_t i2 = (size_t)(g.my - k) * g.nx + (n + 1);
-            T a = Fld[i1], b = Fld[i2];
-            if (Fsub) { a -= Fsub[i1]; b -= Fsub[i2]; }
-            v = a + sgn * b;
-          }
-        }
-        rb[ps][u] = v;
-      }
-    }
+      for (int u = 0; u < 4; ++u)
+        fetch_b<T, BOP>(g, P.B, P.Fld, P.Fsub, k0 + br + BROWS * ps, n0 + bc + u, rb[ps][u]);
   };
 
   // accumulators: MT x NT blocks; a single block keeps four K-interleaved
@@ -209,19 +304,24 @@ __global__ __launch_bounds__(256 * KS) void k_gemm(GemmArgs<T> g) {
 
   // every group runs the same number of chunks (K is a multiple of 32*KS)
   const int nchunk = g.K / (kTile * KS);
-  load(grp * kTile);
+  int kcur = grp * kTile;
+  fetch(kcur);
   for (int it = 0; it < nchunk; ++it) {
     __syncthreads();
 #pragma unroll
     for (int ps = 0; ps < APASS; ++ps)
 #pragma unroll
-      for (int u = 0; u < 4; ++u) As[(ar + 32 * ps) * LDT + ac + u] = ra[ps][u];
+      for (int u = 0; u < 4; ++u)
+        As[(ar + 32 * ps) * LDT + ac + u] = combine_a<T, AOP>(g, kcur + ac + u, ra[ps][u], P.sgn);
 #pragma unroll
     for (int ps = 0; ps < BPASS; ++ps)
 #pragma unroll
-      for (int u = 0; u < 4; ++u) Bs[(br + BROWS * ps) * (BN + 1) + bc + u] = rb[ps][u];
+      for (int u = 0; u < 4; ++u)
+        Bs[(br + BROWS * ps) * (BN + 1) + bc + u] =
+            combine_b<T, BOP>(g, kcur + br + BROWS * ps, n0 + bc + u, rb[ps][u], P.sgn);
     __syncthreads();
-    if (it + 1 < nchunk) load(((it + 1) * KS + grp) * kTile);
+    kcur = ((it + 1) * KS + grp) * kTile;
+    if (it + 1 < nchunk) fetch(kcur);
 #pragma unroll
     for (int kk = 0; kk < kTile / 4; ++kk) {
       T av[MT], bv[NT];
@@ -289,63 +389,135 @@ __global__ __launch_bounds__(256 * KS) void k_gemm(GemmArgs<T> g) {
 #pragma unroll
     for (int b_ = 0; b_ < NT; ++b_)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = m0 + wm * (BM / 2) + a_ * 16 + Mfma<T>::out_row(lane, r);
-        const int col = n0 + wn * (BN / 2) + b_ * 16 + (lane & 15);
-        const T v = val[a_][b_][r];
-        if (MAP == MAP_ROWFOLD) {
-          if (row < g.hy) Cout[(size_t)(2 * row + par) * g.ldc + col] = v;
-          continue;
-        }
-        const int oc = MAP == MAP_COLFOLD ? 2 * col + par : col;
-        if (KIND == EK_STORE || KIND == EK_SCALE) {
-          if (MAP == MAP_COLFOLD && col >= g.hx) continue;
-          const size_t o = (size_t)row * g.ldc + oc;
-          Cout[o] = KIND == EK_SCALE ? v * scale[o] : v;
-          continue;
-        }
-        const bool ok = row < g.my && (MAP == MAP_COLFOLD ? col < g.hx : col < g.mx);
-        if (!ok) continue;
-        const int nx = g.nx, ny = g.ny;
-        const size_t i = row + 1, j = oc + 1;
-        if (KIND == EK_FIELD_ACC) {
-          fout[i * nx + j] += v;
-        } else {
-          const T* hb = hbm;
-          fout[i * nx + j] = (hb ? hb[i * nx + j] : T(0)) + v;
-          // ring: h = hb on rows 0, ny-1 and columns 0, nx-1 (jqgm.py:316-317)
-          auto ring = [&](size_t ii, size_t jj) {
-            fout[ii * nx + jj] = hb ? hb[ii * nx + jj] : T(0);
-          };
-          const bool top = row == 0, bot = row == g.my - 1, lft = oc == 0, rgt = oc == g.mx - 1;
-          if (top) ring(0, j);
-          if (bot) ring(ny - 1, j);
-          if (lft) ring(i, 0);
-          if (rgt) ring(i, nx - 1);
-          if (top && lft) ring(0, 0);
-          if (top && rgt) ring(0, nx - 1);
-          if (bot && lft) ring(ny - 1, 0);
-          if (bot && rgt) ring(ny - 1, nx - 1);
-        }
-      }
+      for (int r = 0; r < 4; ++r)
+        store_out<T, KIND, MAP>(g, P, m0 + wm * (BM / 2) + a_ * 16 + Mfma<T>::out_row(lane, r),
+                                n0 + wn * (BN / 2) + b_ * 16 + (lane & 15), val[a_][b_][r]);
 }
 
-// Tile configurations (BM x BN, K groups) the dispatcher and the autotuner
-// choose from.
-struct TileCfg {
-  int bm, bn, ks;
-};
-constexpr TileCfg kTileCfgs[kNumTileCfg] = {
-    {32, 32, 4}, {64, 64, 1}, {64, 64, 2}, {128, 64, 1}, {128, 64, 2}};
+// Register-direct kernel: no LDS staging and no barrier in the K loop.  The
+// four waves of a group own the 2x2 quadrants (16 WM x 16 WN each) of the
+// BM x BN tile; group g covers the contiguous K range [g K/KS, (g+1) K/KS) in
+// chunks of 16.  Within a chunk lane l (r = l%16, q = l/16) loads A(row r,
+// k = 4q..4q+3) and B(k = 4q..4q+3, col r); MFMA step s then pairs
+// k = 4q + s across the four lane quarters on both operands, so the four steps
+// cover the 16 k of the chunk.  NBUF chunks are in flight per wave; short K
+// ranges are loaded completely before the first MFMA.  Partial tiles of the K
+// groups are summed through LDS in group order (deterministic).
+template <typename T, int AOP, int BOP, int KIND, int MAP, int WM, int WN, int KS, int NBUF>
+__global__ __launch_bounds__(256 * KS) void k_gemm_rd(GemmArgs<T> g) {
+  typedef typename Mfma<T>::acc_t acc_t;
+  constexpr int BM = 32 * WM, BN = 32 * WN;
+  constexpr int NACC = (WM * WN == 1) ? 4 : 1;
+  constexpr int PER = WM * WN * 4;  // partial results per thread
+  __shared__ T park[KS > 1 ? (KS - 1) * 256 * PER : 1];
+  const int tid = threadIdx.x;
+  const int grp = tid >> 8, gt = tid & 255, lane = tid & 63;
+  const int wq = (tid >> 6) & 3, wm = wq >> 1, wn = wq & 1;
+  const int r16 = lane & 15, q4 = lane >> 4;
+  int z, tile;
+  work_item(g, &z, &tile);
+  const GemmPtrs<T> P = gemm_ptrs<T, AOP, BOP, MAP>(g, z);
+  const int m0 = (tile / g.tn) * BM, n0 = (tile % g.tn) * BN;
+  const int row0 = m0 + wm * (16 * WM) + r16;
+  const int col0 = n0 + wn * (16 * WN) + r16;
+  const int nch = g.K / (16 * KS);
+  const int kbase = grp * nch * 16 + 4 * q4;
+
+  constexpr int NA = a_words<AOP>(), NB = b_words<BOP>();
+  T av[NBUF][WM][4][NA], bv[NBUF][WN][4][NB];
+  auto load = [&](T (&a)[WM][4][NA], T (&b)[WN][4][NB], int c) {
+    const int k0 = kbase + 16 * c;
+#pragma unroll
+    for (int a_ = 0; a_ < WM; ++a_)
+#pragma unroll
+      for (int s = 0; s < 4; ++s) fetch_a<T, AOP>(g, P.A, row0 + 16 * a_, k0 + s, a[a_][s]);
+#pragma unroll
+    for (int b_ = 0; b_ < WN; ++b_)
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+        fetch_b<T, BOP>(g, P.B, P.Fld, P.Fsub, k0 + s, col0 + 16 * b_, b[b_][s]);
+  };
+  acc_t acc[WM][WN][NACC];
+#pragma unroll
+  for (int a_ = 0; a_ < WM; ++a_)
+#pragma unroll
+    for (int b_ = 0; b_ < WN; ++b_)
+#pragma unroll
+      for (int c_ = 0; c_ < NACC; ++c_)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[a_][b_][c_][r] = T(0);
+
+#pragma unroll
+  for (int b = 0; b < NBUF; ++b)
+    if (b < nch) load(av[b], bv[b], b);
+  for (int c = 0; c < nch; c += NBUF) {
+#pragma unroll
+    for (int b = 0; b < NBUF; ++b) {
+      if (c + b < nch) {
+        const int k0 = kbase + 16 * (c + b);
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          T x[WM], y[WN];
+#pragma unroll
+          for (int a_ = 0; a_ < WM; ++a_) x[a_] = combine_a<T, AOP>(g, k0 + s, av[b][a_][s], P.sgn);
+#pragma unroll
+          for (int b_ = 0; b_ < WN; ++b_)
+            y[b_] = combine_b<T, BOP>(g, k0 + s, col0 + 16 * b_, bv[b][b_][s], P.sgn);
+#pragma unroll
+          for (int a_ = 0; a_ < WM; ++a_)
+#pragma unroll
+            for (int b_ = 0; b_ < WN; ++b_) {
+              const int c_ = NACC == 4 ? s : 0;
+              acc[a_][b_][c_] = Mfma<T>::run(x[a_], y[b_], acc[a_][b_][c_]);
+            }
+        }
+        if (c + b + NBUF < nch) load(av[b], bv[b], c + b + NBUF);
+      }
+    }
+  }
+
+  T val[PER];
+#pragma unroll
+  for (int a_ = 0; a_ < WM; ++a_)
+#pragma unroll
+    for (int b_ = 0; b_ < WN; ++b_)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        T v = acc[a_][b_][0][r];
+        if (NACC == 4) v = (acc[a_][b_][0][r] + acc[a_][b_][NACC > 1 ? 1 : 0][r]) +
+                           (acc[a_][b_][NACC > 2 ? 2 : 0][r] + acc[a_][b_][NACC > 3 ? 3 : 0][r]);
+        val[(a_ * WN + b_) * 4 + r] = v;
+      }
+  if (KS > 1) {
+    if (grp > 0) {
+      T* pk = park + (size_t)(grp - 1) * 256 * PER;
+#pragma unroll
+      for (int i = 0; i < PER; ++i) pk[i * 256 + gt] = val[i];
+    }
+    __syncthreads();
+    if (grp != 0) return;
+#pragma unroll
+    for (int q = 1; q < KS; ++q) {
+      const T* src = park + (size_t)(q - 1) * 256 * PER;
+#pragma unroll
+      for (int i = 0; i < PER; ++i) val[i] += src[i * 256 + gt];
+    }
+  }
+#pragma unroll
+  for (int a_ = 0; a_ < WM; ++a_)
+#pragma unroll
+    for (int b_ = 0; b_ < WN; ++b_)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        store_out<T, KIND, MAP>(g, P, m0 + wm * (16 * WM) + 16 * a_ + Mfma<T>::out_row(lane, r),
+                                n0 + wn * (16 * WN) + 16 * b_ + r16, val[(a_ * WN + b_) * 4 + r]);
+}
 
 template <typename T, int AOP, int BOP, int KIND, int MAP>
 static void gemm(GemmArgs<T> a, int M, int N, int nz, int cfg, hipStream_t s) {
-  // nz = members x parities.  Fall back to 32x32 tiles when the shape does
-  // not tile with the requested configuration.
-  auto fits = [&](const TileCfg& c) {
-    return M % c.bm == 0 && N % c.bn == 0 && (a.K / kTile) % c.ks == 0;
-  };
-  if (cfg < 0 || cfg >= kNumTileCfg || !fits(kTileCfgs[cfg])) cfg = 0;
+  // nz = members x parities.  Fall back to configuration 0 when the shape
+  // does not tile with the requested one.
+  if (cfg < 0 || cfg >= kNumTileCfg || !tile_cfg_fits(cfg, M, N, a.K)) cfg = 0;
   int ks = kTileCfgs[cfg].ks;
   if (cfg == 0) {
     const int nchunk = a.K / kTile;
@@ -355,23 +527,25 @@ static void gemm(GemmArgs<T> a, int M, int N, int nz, int cfg, hipStream_t s) {
   a.nz = nz;
   a.tm = M / bm;
   a.tn = N / bn;
-  dim3 grid(nz * a.tm * a.tn);
-  if (cfg == 0) {
-    if (ks == 4)
-      hipLaunchKernelGGL((k_gemm<T, AOP, BOP, KIND, MAP, 32, 32, 4>), grid, dim3(1024), 0, s, a);
-    else if (ks == 2)
-      hipLaunchKernelGGL((k_gemm<T, AOP, BOP, KIND, MAP, 32, 32, 2>), grid, dim3(512), 0, s, a);
-    else
-      hipLaunchKernelGGL((k_gemm<T, AOP, BOP, KIND, MAP, 32, 32, 1>), grid, dim3(256), 0, s, a);
-  } else if (cfg == 1) {
-    hipLaunchKernelGGL((k_gemm<T, AOP, BOP, KIND, MAP, 64, 64, 1>), grid, dim3(256), 0, s, a);
-  } else if (cfg == 2) {
-    hipLaunchKernelGGL((k_gemm<T, AOP, BOP, KIND, MAP, 64, 64, 2>), grid, dim3(512), 0, s, a);
-  } else if (cfg == 3) {
-    hipLaunchKernelGGL((k_gemm<T, AOP, BOP, KIND, MAP, 128, 64, 1>), grid, dim3(256), 0, s, a);
-  } else {
-    hipLaunchKernelGGL((k_gemm<T, AOP, BOP, KIND, MAP, 128, 64, 2>), grid, dim3(512), 0, s, a);
+  const dim3 grid(nz * a.tm * a.tn), block(256 * ks);
+#define VD_G(...) hipLaunchKernelGGL((__VA_ARGS__), grid, block, 0, s, a)
+  switch (cfg) {
+    case 0:
+      if (ks == 4) VD_G(k_gemm<T, AOP, BOP, KIND, MAP, 32, 32, 4>);
+      else if (ks == 2) VD_G(k_gemm<T, AOP, BOP, KIND, MAP, 32, 32, 2>);
+      else VD_G(k_gemm<T, AOP, BOP, KIND, MAP, 32, 32, 1>);
+      break;
+    case 1: VD_G(k_gemm<T, AOP, BOP, KIND, MAP, 64, 64, 1>); break;
+    case 2: VD_G(k_gemm<T, AOP, BOP, KIND, MAP, 64, 64, 2>); break;
+    case 3: VD_G(k_gemm<T, AOP, BOP, KIND, MAP, 128, 64, 1>); break;
+    case 4: VD_G(k_gemm<T, AOP, BOP, KIND, MAP, 128, 64, 2>); break;
+    case 5: VD_G(k_gemm_rd<T, AOP, BOP, KIND, MAP, 1, 1, 4, 4>); break;
+    case 6: VD_G(k_gemm_rd<T, AOP, BOP, KIND, MAP, 1, 1, 2, 4>); break;
+    case 7: VD_G(k_gemm_rd<T, AOP, BOP, KIND, MAP, 2, 2, 2, 4>); break;
+    case 8: VD_G(k_gemm_rd<T, AOP, BOP, KIND, MAP, 2, 2, 2, 2>); break;
+    default: VD_G(k_gemm_rd<T, AOP, BOP, KIND, MAP, 4, 2, 1, 2>); break;
   }
+#undef VD_G
 }
 
 template <typename T>
@@ -396,7 +570,10 @@ void launch_gemm_stage(const Plan& p, int stage, const T* F, const T* Fsub, cons
     // T1 = S_y (F - Fsub)
     a.A = SyF; a.lda = p.HY; a.F = F; a.Fsub = Fsub; a.K = p.HY; a.C = T1; a.ldc = p.MX;
     a.bsA = 0; a.bsB = 0; a.bsC = NG;
-    gemm<T, AOP_DENSE, BOP_FOLDROW_FIELD, EK_STORE, MAP_ROWFOLD>(a, p.HY, p.MX, 2 * nb, cf[0], s);
+    if (Fsub)
+      gemm<T, AOP_DENSE, BOP_FOLDROW_FIELD_SUB, EK_STORE, MAP_ROWFOLD>(a, p.HY, p.MX, 2 * nb, cf[0], s);
+    else
+      gemm<T, AOP_DENSE, BOP_FOLDROW_FIELD, EK_STORE, MAP_ROWFOLD>(a, p.HY, p.MX, 2 * nb, cf[0], s);
     } else if (stage == 1) {
     // T2 = (T1 S_x) ./ Lambda
     a.A = T1; a.lda = p.MX; a.B = SxF; a.ldb = p.HX; a.K = p.HX; a.C = T2; a.ldc = p.MX;
@@ -424,7 +601,10 @@ void launch_gemm_stage(const Plan& p, int stage, const T* F, const T* Fsub, cons
   // T1 = S_y (F - Fsub)_interior
   a.A = Sy; a.lda = p.MY; a.F = F; a.Fsub = Fsub; a.K = p.MY; a.C = T1; a.ldc = p.MX;
   a.bsA = 0; a.bsB = 0; a.bsC = NG;
-  gemm<T, AOP_DENSE, BOP_FIELD, EK_STORE, MAP_PLAIN>(a, p.MY, p.MX, nb, cf[0], s);
+  if (Fsub)
+    gemm<T, AOP_DENSE, BOP_FIELD_SUB, EK_STORE, MAP_PLAIN>(a, p.MY, p.MX, nb, cf[0], s);
+  else
+    gemm<T, AOP_DENSE, BOP_FIELD, EK_STORE, MAP_PLAIN>(a, p.MY, p.MX, nb, cf[0], s);
   } else if (stage == 1) {
   // T2 = (T1 S_x) ./ Lambda
   a.A = T1; a.lda = p.MX; a.B = Sx; a.ldb = p.MX; a.K = p.MX; a.C = T2; a.ldc = p.MX;

@@ -141,6 +141,10 @@ class QgPlan:
         """Shape of one field argument: (ny, nx), or (nbatch, ny, nx) for batches."""
         return (self.ny, self.nx) if self.nb == 1 else (self.nb, self.ny, self.nx)
 
+    @property
+    def _numel(self):
+        return self.nb * self.ny * self.nx
+
     def tape_bytes(self, nstep):
         b = C.c_size_t()
         _lib.call("vd_qg_tape_bytes", self._h, int(nstep), C.byref(b))
@@ -181,6 +185,9 @@ class QgPlan:
                 continue
             if (t.device != self.device or t.dtype != self.dtype or not t.is_contiguous()):
                 raise VardynError("fields must be contiguous device tensors of the plan dtype")
+            if t.numel() != self._numel:
+                raise VardynError(f"field has {t.numel()} elements, the plan expects "
+                                  f"{self._numel} {tuple(self.shape)}")
 
     def _check_tape(self, tape, nstep):
         need = self.tape_bytes(nstep)
@@ -230,5 +237,14 @@ class QgPlan:
         out = self.empty() if out is None else out
         self._check(h, hb, out)
         _lib.call("vd_qg_h2pv", self._h, ptr(h), ptr(hb), ptr(out),
+                  current_stream_ptr(self.device))
+        return out
+
+    def rhs(self, h, q, base=None, coef=1.0, out=None):
+        """``base + coef * rhs(h, q)`` (Qgm.rhs, jqgm.py:321-375, fused with one
+        explicit stage update; ``base=None`` means zero)."""
+        out = self.empty() if out is None else out
+        self._check(h, q, base, out)
+        _lib.call("vd_qg_rhs", self._h, ptr(h), ptr(q), ptr(base), float(coef), ptr(out),
                   current_stream_ptr(self.device))
         return out
