@@ -1,13 +1,19 @@
 """Benchmark: forward+adjoint 4D-Var gradient evaluations per second.
 
-Workload (default): 1.5-layer QG in a closed 512x512 basin, dx = 5 km,
-Ld = 20 km (c = 2 m/s), dt = 900 s, 30-day window (2880 RK2 steps); 60
-seeded nadir tracks every 6 km plus one 120 km swath per day at 2 km pixels
-(sigma = 0.02 m); initial-state control with x_b = 0, sigma_b = 0.1 m;
-checkpoints every 54 steps plus every observation time.  One "step" of
-this benchmark is one full ``cost_and_grad`` evaluation (forward sweep with
-misfits, reverse sweep with the hand-written adjoint) with the control
-vector already resident in HBM.
+Workload: 1.5-layer QG in a closed 512x512 basin, dx = 5 km, Ld = 20 km
+(c = 2 m/s), dt = 900 s, 30-day window (2880 RK2 steps); 60 seeded nadir
+tracks every 6 km plus one 120 km swath per day at 2 km pixels (sigma =
+0.02 m); initial-state control with x_b = 0, sigma_b = 0.1 m; checkpoints
+every 54 steps plus every observation time.  One "step" of this benchmark is
+one full ``cost_and_grad`` evaluation (forward sweep with misfits, reverse
+sweep with the hand-written adjoint), replayed from a HIP graph, with the
+control vectors already resident in HBM.
+
+``value`` is the batched path: ``--batch`` independent problems of this
+workload (own initial fields, tracks and noise) advanced in the same kernel
+launches.  The plain single-problem path is measured in the same run and
+reported beside it with the ratio.  ``--batch 1`` makes the plain path the
+headline.
 
 Prints one JSON line.  ``--gpus N`` with N > 1 exits: this build measures a
 single GPU.
@@ -15,6 +21,7 @@ single GPU.
 from __future__ import annotations
 
 import argparse
+import gc
 import json
 import os
 import platform
@@ -41,19 +48,24 @@ def parse():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--precision", default="float64", choices=["float64", "float32"])
-    ap.add_argument("--tape", default="full", choices=["full", "checkpoint"])
+    ap.add_argument("--batch", type=int, default=8,
+                    help="independent 512x512 problems advanced in the same launches")
+    ap.add_argument("--tape", default="auto", choices=["auto", "full", "checkpoint"],
+                    help="trajectory storage of the batched path (auto: full if it fits in HBM)")
     ap.add_argument("--window-steps", type=int, default=None,
                     help="shorten the window (profiling only; the default is the full 2880)")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="skip the single-problem measurement")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-steps", type=int, default=24)
     ap.add_argument("--no-graph", action="store_true",
                     help="launch kernels eagerly instead of replaying a captured HIP graph")
-    ap.add_argument("--batch", type=int, default=1,
-                    help="independent 512x512 problems advanced in the same launches")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the shape-based GEMM tile choice instead of timing the candidates")
     ap.add_argument("--no-stencil", action="store_true",
                     help="skip the batched stencil HBM measurement")
+    ap.add_argument("--stencil-only", action="store_true",
+                    help="only the stencil measurement and a copy of known size (for counter runs)")
     ap.add_argument("--check-f32", action="store_true",
                     help="also evaluate the FP32 path and report its gradient error vs FP64")
     return ap.parse_args()
@@ -150,26 +162,105 @@ def stencil_roofline(case, precision, nbatch=64, n_launch=20, reps=5):
             "bytes_per_launch": nbytes, "avg_launch_us": round(per_launch_s * 1e6, 2)}
 
 
-def cpu_baseline(case, sample_steps):
-    """FP64 oracle forward+adjoint over a bounded sample of the same workload,
-    on one host thread; scaled to gradients/s of the full window."""
+def copy_calibration(nbytes=256 << 20):
+    """A plain device copy of known size, for scaling the FETCH_SIZE / WRITE_SIZE
+    counters of a profiled run (bytes moved: nbytes read + nbytes written)."""
+    import torch
+    src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    gbs = 2 * nbytes / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    return {"bytes_read": nbytes, "bytes_written": nbytes, "achieved_GBs": round(gbs, 1)}
+
+
+def cpu_baseline(case, sample_steps, precision):
+    """The FP64 oracle's ``cost_and_grad`` on one host thread over a bounded
+    sample of the same workload (the first ``sample_steps`` steps of the
+    window, with their observations), scaled to the full window.  The same
+    sample is evaluated by the GPU engine, and the relative error of its
+    gradient against the oracle's is reported beside the timing."""
+    import torch
     from threadpoolctl import threadpool_limits
+    from oracle.fourdvar import BasisInitialState, CostOracle, Cov, ModelOracle
+    from oracle.obsop import ObsopOracle
     from oracle.qg import QgmOracle
-    from vardyn_amd import synthetic as syn
-    h0 = syn.gaussian_random_field(case.ny, case.nx, case.dx, case.wavelength, case.rms, case.seed)
+    from vardyn_amd.problem import build_problem
+    pb = build_problem(case, precision=precision, n_steps=sample_steps, control="ic",
+                       tape="full")
+    X = 0.5 * pb.truth0.ravel()
+    J, G = pb.engine.cost_and_grad(X)
     qg = QgmOracle(dx=case.dx, dy=case.dx, dt=case.dt, c=case.c, f=case.f0,
                    time_scheme=case.time_scheme, shape=(case.ny, case.nx))
-    hb = np.zeros_like(h0)
+    H = ObsopOracle(pb.grid.lon, pb.grid.lat, pb.grid.DX, pb.grid.DY, pb.T, case.dt, pb.dict_obs,
+                    Npix=4)
+    cost = CostOracle(ModelOracle(qg), H, BasisInitialState((case.ny, case.nx), case.sigma_b),
+                      pb.checkpoints, pb.T, dtbasis=case.checkpoint_steps, B=Cov(case.sigma_b),
+                      Xb=np.zeros(case.ny * case.nx))
     with threadpool_limits(limits=1):
         t0 = time.perf_counter()
-        qg.step(h0, hb, sample_steps)                   # forward sweep
-        qg.step_adj(np.ones_like(h0), h0, hb, sample_steps)  # recompute + reverse sweep
+        Jr, Gr = cost.cost_and_grad(X)
         t = time.perf_counter() - t0
     per_grad = t * case.n_steps / sample_steps
+    del pb
+    gc.collect()
+    torch.cuda.empty_cache()
     return {"value": round(1.0 / per_grad, 6), "unit": "gradients/s", "cores": 1, "kind": "port",
-            "sample": f"FP64 NumPy/SciPy oracle, forward + adjoint of {sample_steps} RK2 steps at "
-                      f"{case.ny}x{case.nx} ({t:.1f} s), scaled by {case.n_steps}/{sample_steps}",
-            "host": platform.processor() or platform.machine(), "host_cpus": os.cpu_count()}
+            "sample": f"FP64 NumPy/SciPy oracle cost_and_grad over the first {sample_steps} RK2 "
+                      f"steps of the {case.ny}x{case.nx} window with their observations "
+                      f"({t:.1f} s), scaled by {case.n_steps}/{sample_steps}",
+            "host": platform.processor() or platform.machine(), "host_cpus": os.cpu_count(),
+            "gpu_grad_rel_err_vs_oracle": float(np.linalg.norm(G - Gr) / np.linalg.norm(Gr)),
+            "gpu_J_rel_err_vs_oracle": float(abs(J - Jr) / abs(Jr))}
+
+
+def measure(pb, args, label):
+    """Autotune, capture and time ``args.steps`` evaluations of ``pb.engine``."""
+    import torch
+    eng = pb.engine
+    nb = pb.plan.nb
+    t0 = time.time()
+    tiles = pb.plan.autotune(reps=5) if not args.no_autotune else None
+    log(f"{label}: GEMM tile configurations {tiles} ({time.time() - t0:.2f}s); "
+        f"{pb.H.n_total} obs at {len(pb.H.t_obs)} times per member, "
+        f"{len(pb.checkpoints) - 1} segments, tape {eng.tape_mode}")
+    X0 = torch.zeros(nb * eng.nctrl, dtype=torch.float64, device=pb.plan.device)
+    if not args.no_graph:
+        t0 = time.time()
+        eng.capture()
+        log(f"{label}: captured one gradient evaluation in a HIP graph ({time.time() - t0:.1f}s)")
+    for w in range(args.warmup):
+        J, G = eng.evaluate(X0)
+        torch.cuda.synchronize()
+        log(f"{label}: warmup {w}: J[0]={J[0].item():.6e}")
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        J, G = eng.evaluate(X0)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t1
+    ms = dt * 1e3 / args.steps
+    r = {"value": nb * args.steps / dt, "ms_per_step": ms, "batch": nb, "tiles": tiles,
+         "tape": eng.tape_mode, "J0": J[0].item(),
+         "gnorm0": float(torch.linalg.vector_norm(G[0]).item())}
+    log(f"{label}: timed {args.steps} evaluations of {nb} problem(s): {ms:.1f} ms each; "
+        f"J[0]={r['J0']:.10e} |G[0]|={r['gnorm0']:.10e}")
+    return r
+
+
+def release(pb):
+    import torch
+    if pb is not None and pb.engine is not None:
+        pb.engine._graph = None
+    gc.collect()
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
 
 
 def main():
@@ -183,72 +274,63 @@ def main():
     from vardyn_amd.problem import build_batch, build_problem
 
     case = syn.case_basin512()
+    if args.stencil_only:
+        out = {"stencil_roofline": stencil_roofline(case, args.precision),
+               "copy_calibration": copy_calibration()}
+        print(json.dumps(out), flush=True)
+        return
     n_steps = args.window_steps or case.n_steps
-    t0 = time.time()
-    log(f"building problem {case.name}: {case.ny}x{case.nx}, {n_steps} steps, {args.precision}")
-    if args.batch > 1:
-        pb = build_batch(case, args.batch, precision=args.precision, n_steps=n_steps,
-                         tape=args.tape)
-    else:
-        pb = build_problem(case, precision=args.precision, n_steps=n_steps, control="ic",
-                           tape=args.tape)
-    eng = pb.engine
-    nb = args.batch
-    t0 = time.time()
-    tiles = pb.plan.autotune(reps=5) if not args.no_autotune else None
-    log(f"GEMM tile configurations {tiles} ({time.time() - t0:.2f}s)")
-    log(f"setup {time.time() - t0:.1f}s: {pb.H.n_total} obs at {len(pb.H.t_obs)} times, "
-        f"{len(pb.checkpoints) - 1} segments")
-    X0 = torch.zeros(nb * eng.nctrl, dtype=torch.float64, device=pb.plan.device)
-    if not args.no_graph:
+    nb = max(1, args.batch)
+
+    plain = None
+    if nb == 1 or not args.no_plain:
+        log(f"plain path: one {case.ny}x{case.nx} problem, {n_steps} steps, {args.precision}")
+        pb1 = build_problem(case, precision=args.precision, n_steps=n_steps, control="ic",
+                            tape="full" if args.tape == "auto" else args.tape)
+        plain = measure(pb1, args, "plain")
+        if nb == 1:
+            pb, head = pb1, plain
+        else:
+            del pb1
+            release(None)
+    if nb > 1:
         t0 = time.time()
-        eng.capture()
-        log(f"captured one gradient evaluation in a HIP graph ({time.time() - t0:.1f}s)")
-    for w in range(args.warmup):
-        J, G = eng.evaluate(X0)
-        torch.cuda.synchronize()
-        log(f"warmup {w}: J={J.sum().item():.6e}")
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    for _ in range(args.steps):
-        J, G = eng.evaluate(X0)
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t1
-    ms = dt * 1e3 / args.steps
-    value = nb * args.steps / dt
-    Jv, gnorm = J[0].item(), float(torch.linalg.vector_norm(G[0]).item())
-    log(f"timed {args.steps} evaluations of {nb} problem(s): {ms:.1f} ms each; "
-        f"J[0]={Jv:.10e} |G[0]|={gnorm:.10e}")
+        pb = build_batch(case, nb, precision=args.precision, n_steps=n_steps, tape=args.tape)
+        log(f"batched path: {nb} problems built in {time.time() - t0:.1f}s")
+        head = measure(pb, args, "batched")
 
     roof = gemm_roofline(pb.plan, args.precision)
     log(f"roofline: {roof}")
-    stencil = None
-    if not args.no_stencil:
-        stencil = stencil_roofline(case, args.precision)
-        log(f"stencil roofline: {stencil}")
+    value, ms = head["value"], head["ms_per_step"]
     out = {
         "metric": METRIC, "value": round(value, 4), "unit": "gradients/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 2),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if args.precision == "float64" else "f32",
-        "data": "synthetic (seeded random initial SSH, truth run, nadir tracks + swaths)",
-        "config": {"workload": case.name, "grid": [case.ny, case.nx], "dx_m": case.dx,
-                   "dt_s": case.dt, "window_steps": n_steps, "time_scheme": case.time_scheme,
-                   "obs": pb.H.n_total, "segments": len(pb.checkpoints) - 1,
-                   "control": "initial SSH", "tape": args.tape,
+        "data": "synthetic (seeded random initial SSH, truth run, nadir tracks + swaths per member)",
+        "config": {"workload": f"{case.name}" + (f" x{nb} batched" if nb > 1 else ""),
+                   "grid": [case.ny, case.nx], "dx_m": case.dx, "dt_s": case.dt,
+                   "window_steps": n_steps, "time_scheme": case.time_scheme,
+                   "obs_per_member": pb.H.n_total, "segments": len(pb.checkpoints) - 1,
+                   "control": "initial SSH", "tape": head["tape"], "batch": nb,
                    "hip_graph": not args.no_graph, "folded_dst": pb.plan.folded,
-                   "batch": nb, "gemm_tiles": tiles},
+                   "gemm_tiles": head["tiles"]},
         "grid_point_steps_per_s": round(case.ny * case.nx * n_steps * value, 1),
         "ms_per_gradient": round(ms / nb, 2),
-        "J": Jv, "grad_norm": gnorm,
+        "J": head["J0"], "grad_norm": head["gnorm0"],
         "roofline": roof,
     }
-    if stencil is not None:
-        out["stencil_roofline"] = stencil
+    if plain is not None and nb > 1:
+        out["plain_path"] = {"value": round(plain["value"], 4), "unit": "gradients/s",
+                             "ms_per_gradient": round(plain["ms_per_step"], 2),
+                             "tape": plain["tape"], "gemm_tiles": plain["tiles"]}
+        out["batched_over_plain"] = round(value / plain["value"], 3)
     if args.check_f32 and args.precision == "float64" and nb == 1:
+        X0 = torch.zeros(pb.engine.nctrl, dtype=torch.float64, device=pb.plan.device)
+        J64, G64 = pb.engine.run(X0)
+        G64 = G64.clone()
         pb32 = build_problem(case, precision="float32", n_steps=n_steps, control="ic",
-                             tape=args.tape, truth_plan=pb.plan)
-        G64 = G.clone()
+                             tape="full", truth_plan=pb.plan)
         J32, G32 = pb32.engine.run(X0)
         torch.cuda.synchronize()
         t2 = time.perf_counter()
@@ -258,10 +340,16 @@ def main():
             "gradients_per_s": round(1.0 / (time.perf_counter() - t2), 4),
             "grad_rel_err_vs_f64": float((torch.linalg.vector_norm(G32 - G64) /
                                           torch.linalg.vector_norm(G64)).item()),
-            "J_rel_err_vs_f64": abs(J32.item() - Jv) / abs(Jv)}
+            "J_rel_err_vs_f64": abs(J32.item() - J64.item()) / abs(J64.item())}
         log(f"f32 path: {out['f32_path']}")
+        del pb32
+    del pb
+    release(None)
+    if not args.no_stencil:
+        out["stencil_roofline"] = stencil_roofline(case, args.precision)
+        log(f"stencil roofline: {out['stencil_roofline']}")
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(case, args.cpu_sample_steps)
+        out["cpu_baseline"] = cpu_baseline(case, args.cpu_sample_steps, args.precision)
         log(f"cpu baseline: {out['cpu_baseline']}")
     print(json.dumps(out), flush=True)
 

@@ -75,12 +75,9 @@ typedef struct vd_qg_params {
                             ny-2 and nx-2 are even (half the GEMM work);
                             1: always the full (n-2)^2 sine matrices */
   int gemm_config;       /* GEMM tile configuration: 0 = chosen from the
-                            problem shape; k in 1..10 forces configuration
-                            k-1 of: LDS-staged 32x32 (split-K), 64x64,
-                            64x64 (split-K 2), 128x64, 128x64 (split-K 2);
-                            register-direct 32x32 (split-K 4), 32x32
-                            (split-K 2), 64x64 (split-K 2), 64x64 (split-K
-                            2, shallower prefetch), 128x64 */
+                            problem shape; k in 1..5 forces configuration
+                            k-1 of 32x32 (split-K), 64x64, 64x64 (split-K
+                            2), 128x64, 128x64 (split-K 2) */
   int nbatch;            /* independent problems on this grid (0 or 1 = one);
                             with nbatch > 1 every field argument of the step
                             and inversion calls is an (nbatch, ny, nx) array */
@@ -102,7 +99,7 @@ int vd_qg_plan_create(const vd_qg_params* params, vd_plan** plan);
 int vd_qg_plan_destroy(vd_plan* plan);
 
 /* Bytes of trajectory tape for a segment of `nstep` steps (stage states of
- * every step plus the final state). */
+ * every step plus the final state, for every batch member). */
 int vd_qg_tape_bytes(const vd_plan* plan, int nstep, size_t* bytes);
 
 /* Time every tile configuration of each of the four inversion GEMMs on the

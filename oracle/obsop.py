@@ -14,9 +14,12 @@ Follows ``mapping/src/obsop.py`` (class ``Obsop_interp_l3_jax``) and
   scatter ``H^T (misfit / sigma)``)
 
 Pinning: ``scipy.spatial.KDTree`` is the library the reference calls; the
-selected indices are compared with a brute-force distance sort on small
-cases and with committed golden index fixtures.  ``adj`` is pinned by the
-exact identity <H x, y> = <x, H^T y> and by finite differences of ``misfit``.
+selected indices and weights are compared with a brute-force distance sort
+over every grid point on small cases (the reference's tests hold no
+fixtures for this operator).  ``geo2cart`` is pinned by the WGS84 radii at
+the equator and the pole, the time matching by hand-worked cases.  ``adj``
+is pinned by the exact identity <H x, y> = <x, H^T y> and by finite
+differences of ``misfit``.
 """
 from __future__ import annotations
 

@@ -210,3 +210,50 @@ def test_grad_test_on_engine():
     assert r[-1] < 1e-5
     drop = r[-2:] / r[-3:-1]   # the last two decades, past the pre-asymptotic range
     assert np.all(drop < 0.2) and np.all(drop > 0.05)
+
+
+def test_autograd_backward_is_the_adjoint():
+    """QgStep.backward = the hand adjoint: h0.grad of <h1, w> equals the
+    oracle's step_adj(w) (jqgm.py:565-569)."""
+    from vardyn_amd.autograd import qg_step
+    from vardyn_amd.qg import QgPlan
+    ny, nx, n = 40, 46, 4
+    orc = QgmOracle(dx=9e3, dy=11e3, dt=1200.0, c=2.5, f=1e-4, time_scheme="rk3", shape=(ny, nx))
+    plan = QgPlan(ny, nx, 9e3, 11e3, 1200.0, 2.5, 1e-4, time_scheme="rk3", precision="float64")
+    rng = np.random.default_rng(11)
+    h0n, hbn, wn = (0.1 * rng.standard_normal((ny, nx)), 0.01 * rng.standard_normal((ny, nx)),
+                    rng.standard_normal((ny, nx)))
+    h0 = plan.field(h0n).requires_grad_(True)
+    hb, w = plan.field(hbn), plan.field(wn)
+    h1 = qg_step(h0, hb, plan, n)
+    assert rel(h1.detach().cpu().numpy(), orc.step(h0n, hbn, n)) < 1e-10
+    (h1 * w).sum().backward()
+    assert rel(h0.grad.cpu().numpy(), orc.step_adj(wn, h0n, hbn, n)) < 1e-10
+
+
+def test_autograd_jvp_is_the_tangent_linear():
+    from vardyn_amd.autograd import qg_step
+    from vardyn_amd.qg import QgPlan
+    ny, nx, n = 40, 46, 4
+    orc = QgmOracle(dx=9e3, dy=11e3, dt=1200.0, c=2.5, f=1e-4, time_scheme="rk2", shape=(ny, nx))
+    plan = QgPlan(ny, nx, 9e3, 11e3, 1200.0, 2.5, 1e-4, time_scheme="rk2", precision="float64")
+    rng = np.random.default_rng(12)
+    h0n, dhn = 0.1 * rng.standard_normal((ny, nx)), 0.1 * rng.standard_normal((ny, nx))
+    import torch.autograd.forward_ad as fwAD
+    with fwAD.dual_level():
+        y = qg_step(fwAD.make_dual(plan.field(h0n), plan.field(dhn)), None, plan, n)
+        h1, dh1 = fwAD.unpack_dual(y)
+    tl_ref, h1_ref = orc.step_tgl(dhn, h0n, np.zeros((ny, nx)), n)
+    assert rel(h1.cpu().numpy(), orc.step(h0n, np.zeros((ny, nx)), n)) < 1e-10
+    assert rel(dh1.cpu().numpy(), tl_ref) < 1e-10
+
+
+def test_autograd_rejects_boundary_gradient():
+    from vardyn_amd._lib import VardynError
+    from vardyn_amd.autograd import qg_step
+    from vardyn_amd.qg import QgPlan
+    plan = QgPlan(24, 24, 1e4, 1e4, 900.0, 2.0, 1e-4, precision="float64")
+    h0 = torch.zeros((24, 24), dtype=torch.float64, device="cuda", requires_grad=True)
+    hb = torch.zeros_like(h0, requires_grad=True)
+    with pytest.raises(VardynError):
+        qg_step(h0, hb, plan, 1)

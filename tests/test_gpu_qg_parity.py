@@ -168,11 +168,12 @@ def test_null_boundary_equals_zero_field(scheme):
 
 
 @pytest.mark.parametrize("shape", [(130, 258), (258, 258)])
-@pytest.mark.parametrize("cfg", list(range(1, 11)))
+@pytest.mark.parametrize("cfg", [1, 2, 3, 4, 5])
 def test_gemm_tile_configs_agree(cfg, shape):
     """Every GEMM tile configuration gives the oracle's inversion (FP64).  At
-    130 x 258 the folded K of 64 does not split four ways, so the split-K 4
-    configurations fall back to the default; at 258 x 258 every one applies."""
+    130 x 258 the folded K of 64 does not split four ways and 64 rows do not
+    tile by 128, so some configurations fall back to the default; at 258 x 258
+    every one applies."""
     ny, nx = shape
     rng = np.random.default_rng(3)
     orc = QgmOracle(dx=5e3, dy=6e3, dt=900.0, c=2.0, f=1e-4, time_scheme="rk2", shape=(ny, nx))
@@ -184,15 +185,15 @@ def test_gemm_tile_configs_agree(cfg, shape):
     h = plan.pv2h(plan.field(q), plan.field(hb), plan.field(qb)).cpu().numpy()
     assert rel(h, orc.pv2h(q, hb, qb)) < 1e-12
     chosen = plan.autotune(reps=2)
-    assert all(0 <= c < 10 for c in chosen)
+    assert all(0 <= c < 5 for c in chosen)
     h2 = plan.pv2h(plan.field(q), plan.field(hb), plan.field(qb)).cpu().numpy()
     assert rel(h2, orc.pv2h(q, hb, qb)) < 1e-12
 
 
-@pytest.mark.parametrize("cfg", [1, 6, 8, 10])
+@pytest.mark.parametrize("cfg", [1, 2, 3, 4, 5])
 def test_gemm_tile_configs_f32(cfg):
-    """LDS-staged and register-direct configurations on the FP32 MFMA path
-    (FP32 rounding over four sine products: 1e-5 relative)."""
+    """Every tile configuration on the FP32 MFMA path (FP32 rounding over four
+    sine products: 1e-5 relative)."""
     ny, nx = 258, 258
     rng = np.random.default_rng(4)
     orc = QgmOracle(dx=5e3, dy=6e3, dt=900.0, c=2.0, f=1e-4, time_scheme="rk2", shape=(ny, nx))

@@ -417,7 +417,7 @@ int vd_qg_plan_create(const vd_qg_params* prm, vd_plan** out) {
   VD_REQUIRE(prm->nbatch >= 0 && prm->nbatch <= 4096, VD_ERR_INVALID_ARG,
              "nbatch must be in 0..4096");
   VD_REQUIRE(prm->gemm_config >= 0 && prm->gemm_config <= kNumTileCfg, VD_ERR_INVALID_ARG,
-             "gemm_config must be in 0..10");
+             "gemm_config must be in 0..5");
   vd_plan* h = new vd_plan();
   Plan& p = h->p;
   p.ny = prm->ny; p.nx = prm->nx;

@@ -32,18 +32,14 @@ void set_error(const std::string& msg);
 
 constexpr int kTile = 32;  // smallest GEMM output tile and the K chunk
 
-// GEMM tile configurations: BM x BN output tile per workgroup, ks K groups of
-// four waves, and the operand path -- staged through LDS (rd = 0) or loaded
-// straight into MFMA registers with nbuf 16-wide K chunks in flight per wave
-// (rd = 1).  Configuration 0 picks ks from K (4, 2 or 1 groups).
+// GEMM tile configurations: BM x BN output tile per workgroup and ks K
+// groups of four waves (configuration 0 picks ks = 4, 2 or 1 from K).
 struct TileCfg {
-  int bm, bn, ks, rd, nbuf;
+  int bm, bn, ks;
 };
-constexpr int kNumTileCfg = 10;
+constexpr int kNumTileCfg = 5;
 inline constexpr TileCfg kTileCfgs[kNumTileCfg] = {
-    {32, 32, 4, 0, 0},  {64, 64, 1, 0, 0},  {64, 64, 2, 0, 0}, {128, 64, 1, 0, 0},
-    {128, 64, 2, 0, 0}, {32, 32, 4, 1, 4},  {32, 32, 2, 1, 4}, {64, 64, 2, 1, 4},
-    {64, 64, 2, 1, 2},  {128, 64, 1, 1, 2}};
+    {32, 32, 4}, {64, 64, 1}, {64, 64, 2}, {128, 64, 1}, {128, 64, 2}};
 inline bool tile_cfg_fits(int c, int M, int N, int K) {
   return M % kTileCfgs[c].bm == 0 && N % kTileCfgs[c].bn == 0 &&
          (c == 0 || (K / kTile) % kTileCfgs[c].ks == 0);

@@ -39,7 +39,7 @@ template <> struct Mfma<double> {
 // C = A * B on 32x32 output tiles.  A workgroup holds KS groups of four
 // waves; the four waves of a group own the four 16x16 quadrants of the tile,
 // and group g accumulates the K chunks g, g+KS, g+2KS, ...  (32 wide, staged
-// through LDS with a one-chunk register prefetch).  The KS partial tiles are
+// through LDS with a two-chunk register prefetch).  The KS partial tiles are
 // summed through LDS in a fixed order, so results are deterministic.  More
 // groups give the CU more waves to hide L2 latency with; the product stays
 // one launch.
@@ -98,9 +98,9 @@ struct GemmArgs {
 // folded/interior extent read as zero.
 //
 // Each access is split into fetch (the raw words, from clamped -- always
-// valid -- addresses) and combine (fold, subtraction and range select).
-// Kernels fetch a chunk ahead and combine only where the value is consumed,
-// so no arithmetic on a prefetched word forces a wait for it early and every
+// valid -- addresses) and combine (fold, subtraction and range select).  The
+// kernel fetches chunks ahead and combines only when it writes them to LDS,
+// so no arithmetic on a prefetched word forces an early wait for it and every
 // load of a chunk is in flight while the previous chunk is multiplied.
 template <int AOP>
 constexpr int a_words() { return AOP == AOP_DENSE ? 1 : 2; }
@@ -246,7 +246,7 @@ __device__ __forceinline__ void store_out(const GemmArgs<T>& g, const GemmPtrs<T
 }
 
 // LDS-staged kernel: each K chunk of 32 is loaded by the whole group into
-// LDS (one chunk of register prefetch) and shared by its four waves.
+// LDS (two chunks of register prefetch) and shared by its four waves.
 template <typename T, int AOP, int BOP, int KIND, int MAP, int BM, int BN, int KS>
 __global__ __launch_bounds__(256 * KS) void k_gemm(GemmArgs<T> g) {
   typedef typename Mfma<T>::acc_t acc_t;
@@ -276,17 +276,40 @@ __global__ __launch_bounds__(256 * KS) void k_gemm(GemmArgs<T> g) {
   const int ar = gt >> 3, ac = (gt & 7) * 4;
   const int br = gt / (BN / 4), bc = (gt % (BN / 4)) * 4;
   constexpr int NA = a_words<AOP>(), NB = b_words<BOP>();
-  T ra[APASS][4][NA], rb[BPASS][4][NB];
-  auto fetch = [&](int k0) {
+  typedef T ABuf[APASS][4][NA];
+  typedef T BBuf[BPASS][4][NB];
+  // Prefetch depth: two chunks in flight when their registers take less than
+  // half of what a wave may hold in a group of 256 KS threads (512 / KS
+  // VGPRs), otherwise one -- deeper prefetch would spill.
+  constexpr int WORDS = APASS * 4 * NA + BPASS * 4 * NB;
+  constexpr int PD = 2 * WORDS * (int)(sizeof(T) / 4) < 256 / KS ? 2 : 1;
+  ABuf ra[PD];
+  BBuf rb[PD];
+  auto fetch = [&](ABuf& a, BBuf& b, int k0) {
 #pragma unroll
     for (int ps = 0; ps < APASS; ++ps)
 #pragma unroll
-      for (int u = 0; u < 4; ++u) fetch_a<T, AOP>(g, P.A, m0 + ar + 32 * ps, k0 + ac + u, ra[ps][u]);
+      for (int u = 0; u < 4; ++u) fetch_a<T, AOP>(g, P.A, m0 + ar + 32 * ps, k0 + ac + u, a[ps][u]);
 #pragma unroll
     for (int ps = 0; ps < BPASS; ++ps)
 #pragma unroll
       for (int u = 0; u < 4; ++u)
-        fetch_b<T, BOP>(g, P.B, P.Fld, P.Fsub, k0 + br + BROWS * ps, n0 + bc + u, rb[ps][u]);
+        fetch_b<T, BOP>(g, P.B, P.Fld, P.Fsub, k0 + br + BROWS * ps, n0 + bc + u, b[ps][u]);
+  };
+  auto stage = [&](const ABuf& a, const BBuf& b, int k0) {
+    __syncthreads();
+#pragma unroll
+    for (int ps = 0; ps < APASS; ++ps)
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        As[(ar + 32 * ps) * LDT + ac + u] = combine_a<T, AOP>(g, k0 + ac + u, a[ps][u], P.sgn);
+#pragma unroll
+    for (int ps = 0; ps < BPASS; ++ps)
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        Bs[(br + BROWS * ps) * (BN + 1) + bc + u] =
+            combine_b<T, BOP>(g, k0 + br + BROWS * ps, n0 + bc + u, b[ps][u], P.sgn);
+    __syncthreads();
   };
 
   // accumulators: MT x NT blocks; a single block keeps four K-interleaved
@@ -301,27 +324,7 @@ __global__ __launch_bounds__(256 * KS) void k_gemm(GemmArgs<T> g) {
       for (int c_ = 0; c_ < NACC; ++c_)
 #pragma unroll
         for (int r = 0; r < 4; ++r) acc[a_][b_][c_][r] = T(0);
-
-  // every group runs the same number of chunks (K is a multiple of 32*KS)
-  const int nchunk = g.K / (kTile * KS);
-  int kcur = grp * kTile;
-  fetch(kcur);
-  for (int it = 0; it < nchunk; ++it) {
-    __syncthreads();
-#pragma unroll
-    for (int ps = 0; ps < APASS; ++ps)
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-        As[(ar + 32 * ps) * LDT + ac + u] = combine_a<T, AOP>(g, kcur + ac + u, ra[ps][u], P.sgn);
-#pragma unroll
-    for (int ps = 0; ps < BPASS; ++ps)
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-        Bs[(br + BROWS * ps) * (BN + 1) + bc + u] =
-            combine_b<T, BOP>(g, kcur + br + BROWS * ps, n0 + bc + u, rb[ps][u], P.sgn);
-    __syncthreads();
-    kcur = ((it + 1) * KS + grp) * kTile;
-    if (it + 1 < nchunk) fetch(kcur);
+  auto multiply = [&]() {
 #pragma unroll
     for (int kk = 0; kk < kTile / 4; ++kk) {
       T av[MT], bv[NT];
@@ -338,6 +341,35 @@ __global__ __launch_bounds__(256 * KS) void k_gemm(GemmArgs<T> g) {
           const int c_ = NACC == 4 ? (kk & 3) : 0;
           acc[a_][b_][c_] = Mfma<T>::run(av[a_], bv[b_], acc[a_][b_][c_]);
         }
+    }
+  };
+
+  // Group grp takes the K chunks grp, grp+KS, grp+2KS, ...; every group runs
+  // the same number of them (K is a multiple of 32*KS).  PD chunks are in
+  // flight in registers while the staged one is multiplied; with PD = 2, four
+  // K groups and K = 256, both of a group's chunks load before its first
+  // barrier.
+  const int nchunk = g.K / (kTile * KS);
+  auto kof = [&](int it) { return (it * KS + grp) * kTile; };
+  if constexpr (PD == 1) {
+    fetch(ra[0], rb[0], kof(0));
+    for (int it = 0; it < nchunk; ++it) {
+      stage(ra[0], rb[0], kof(it));
+      if (it + 1 < nchunk) fetch(ra[0], rb[0], kof(it + 1));
+      multiply();
+    }
+  } else {
+    fetch(ra[0], rb[0], kof(0));
+    if (nchunk > 1) fetch(ra[PD - 1], rb[PD - 1], kof(1));
+    for (int it = 0; it < nchunk; it += 2) {
+      stage(ra[0], rb[0], kof(it));
+      if (it + 2 < nchunk) fetch(ra[0], rb[0], kof(it + 2));
+      multiply();
+      if (it + 1 < nchunk) {
+        stage(ra[PD - 1], rb[PD - 1], kof(it + 1));
+        if (it + 3 < nchunk) fetch(ra[PD - 1], rb[PD - 1], kof(it + 3));
+        multiply();
+      }
     }
   }
   T val[MT][NT][4];
@@ -394,125 +426,6 @@ __global__ __launch_bounds__(256 * KS) void k_gemm(GemmArgs<T> g) {
                                 n0 + wn * (BN / 2) + b_ * 16 + (lane & 15), val[a_][b_][r]);
 }
 
-// Register-direct kernel: no LDS staging and no barrier in the K loop.  The
-// four waves of a group own the 2x2 quadrants (16 WM x 16 WN each) of the
-// BM x BN tile; group g covers the contiguous K range [g K/KS, (g+1) K/KS) in
-// chunks of 16.  Within a chunk lane l (r = l%16, q = l/16) loads A(row r,
-// k = 4q..4q+3) and B(k = 4q..4q+3, col r); MFMA step s then pairs
-// k = 4q + s across the four lane quarters on both operands, so the four steps
-// cover the 16 k of the chunk.  NBUF chunks are in flight per wave; short K
-// ranges are loaded completely before the first MFMA.  Partial tiles of the K
-// groups are summed through LDS in group order (deterministic).
-template <typename T, int AOP, int BOP, int KIND, int MAP, int WM, int WN, int KS, int NBUF>
-__global__ __launch_bounds__(256 * KS) void k_gemm_rd(GemmArgs<T> g) {
-  typedef typename Mfma<T>::acc_t acc_t;
-  constexpr int BM = 32 * WM, BN = 32 * WN;
-  constexpr int NACC = (WM * WN == 1) ? 4 : 1;
-  constexpr int PER = WM * WN * 4;  // partial results per thread
-  __shared__ T park[KS > 1 ? (KS - 1) * 256 * PER : 1];
-  const int tid = threadIdx.x;
-  const int grp = tid >> 8, gt = tid & 255, lane = tid & 63;
-  const int wq = (tid >> 6) & 3, wm = wq >> 1, wn = wq & 1;
-  const int r16 = lane & 15, q4 = lane >> 4;
-  int z, tile;
-  work_item(g, &z, &tile);
-  const GemmPtrs<T> P = gemm_ptrs<T, AOP, BOP, MAP>(g, z);
-  const int m0 = (tile / g.tn) * BM, n0 = (tile % g.tn) * BN;
-  const int row0 = m0 + wm * (16 * WM) + r16;
-  const int col0 = n0 + wn * (16 * WN) + r16;
-  const int nch = g.K / (16 * KS);
-  const int kbase = grp * nch * 16 + 4 * q4;
-
-  constexpr int NA = a_words<AOP>(), NB = b_words<BOP>();
-  T av[NBUF][WM][4][NA], bv[NBUF][WN][4][NB];
-  auto load = [&](T (&a)[WM][4][NA], T (&b)[WN][4][NB], int c) {
-    const int k0 = kbase + 16 * c;
-#pragma unroll
-    for (int a_ = 0; a_ < WM; ++a_)
-#pragma unroll
-      for (int s = 0; s < 4; ++s) fetch_a<T, AOP>(g, P.A, row0 + 16 * a_, k0 + s, a[a_][s]);
-#pragma unroll
-    for (int b_ = 0; b_ < WN; ++b_)
-#pragma unroll
-      for (int s = 0; s < 4; ++s)
-        fetch_b<T, BOP>(g, P.B, P.Fld, P.Fsub, k0 + s, col0 + 16 * b_, b[b_][s]);
-  };
-  acc_t acc[WM][WN][NACC];
-#pragma unroll
-  for (int a_ = 0; a_ < WM; ++a_)
-#pragma unroll
-    for (int b_ = 0; b_ < WN; ++b_)
-#pragma unroll
-      for (int c_ = 0; c_ < NACC; ++c_)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) acc[a_][b_][c_][r] = T(0);
-
-#pragma unroll
-  for (int b = 0; b < NBUF; ++b)
-    if (b < nch) load(av[b], bv[b], b);
-  for (int c = 0; c < nch; c += NBUF) {
-#pragma unroll
-    for (int b = 0; b < NBUF; ++b) {
-      if (c + b < nch) {
-        const int k0 = kbase + 16 * (c + b);
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-          T x[WM], y[WN];
-#pragma unroll
-          for (int a_ = 0; a_ < WM; ++a_) x[a_] = combine_a<T, AOP>(g, k0 + s, av[b][a_][s], P.sgn);
-#pragma unroll
-          for (int b_ = 0; b_ < WN; ++b_)
-            y[b_] = combine_b<T, BOP>(g, k0 + s, col0 + 16 * b_, bv[b][b_][s], P.sgn);
-#pragma unroll
-          for (int a_ = 0; a_ < WM; ++a_)
-#pragma unroll
-            for (int b_ = 0; b_ < WN; ++b_) {
-              const int c_ = NACC == 4 ? s : 0;
-              acc[a_][b_][c_] = Mfma<T>::run(x[a_], y[b_], acc[a_][b_][c_]);
-            }
-        }
-        if (c + b + NBUF < nch) load(av[b], bv[b], c + b + NBUF);
-      }
-    }
-  }
-
-  T val[PER];
-#pragma unroll
-  for (int a_ = 0; a_ < WM; ++a_)
-#pragma unroll
-    for (int b_ = 0; b_ < WN; ++b_)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        T v = acc[a_][b_][0][r];
-        if (NACC == 4) v = (acc[a_][b_][0][r] + acc[a_][b_][NACC > 1 ? 1 : 0][r]) +
-                           (acc[a_][b_][NACC > 2 ? 2 : 0][r] + acc[a_][b_][NACC > 3 ? 3 : 0][r]);
-        val[(a_ * WN + b_) * 4 + r] = v;
-      }
-  if (KS > 1) {
-    if (grp > 0) {
-      T* pk = park + (size_t)(grp - 1) * 256 * PER;
-#pragma unroll
-      for (int i = 0; i < PER; ++i) pk[i * 256 + gt] = val[i];
-    }
-    __syncthreads();
-    if (grp != 0) return;
-#pragma unroll
-    for (int q = 1; q < KS; ++q) {
-      const T* src = park + (size_t)(q - 1) * 256 * PER;
-#pragma unroll
-      for (int i = 0; i < PER; ++i) val[i] += src[i * 256 + gt];
-    }
-  }
-#pragma unroll
-  for (int a_ = 0; a_ < WM; ++a_)
-#pragma unroll
-    for (int b_ = 0; b_ < WN; ++b_)
-#pragma unroll
-      for (int r = 0; r < 4; ++r)
-        store_out<T, KIND, MAP>(g, P, m0 + wm * (16 * WM) + 16 * a_ + Mfma<T>::out_row(lane, r),
-                                n0 + wn * (16 * WN) + 16 * b_ + r16, val[(a_ * WN + b_) * 4 + r]);
-}
-
 template <typename T, int AOP, int BOP, int KIND, int MAP>
 static void gemm(GemmArgs<T> a, int M, int N, int nz, int cfg, hipStream_t s) {
   // nz = members x parities.  Fall back to configuration 0 when the shape
@@ -538,12 +451,7 @@ static void gemm(GemmArgs<T> a, int M, int N, int nz, int cfg, hipStream_t s) {
     case 1: VD_G(k_gemm<T, AOP, BOP, KIND, MAP, 64, 64, 1>); break;
     case 2: VD_G(k_gemm<T, AOP, BOP, KIND, MAP, 64, 64, 2>); break;
     case 3: VD_G(k_gemm<T, AOP, BOP, KIND, MAP, 128, 64, 1>); break;
-    case 4: VD_G(k_gemm<T, AOP, BOP, KIND, MAP, 128, 64, 2>); break;
-    case 5: VD_G(k_gemm_rd<T, AOP, BOP, KIND, MAP, 1, 1, 4, 4>); break;
-    case 6: VD_G(k_gemm_rd<T, AOP, BOP, KIND, MAP, 1, 1, 2, 4>); break;
-    case 7: VD_G(k_gemm_rd<T, AOP, BOP, KIND, MAP, 2, 2, 2, 4>); break;
-    case 8: VD_G(k_gemm_rd<T, AOP, BOP, KIND, MAP, 2, 2, 2, 2>); break;
-    default: VD_G(k_gemm_rd<T, AOP, BOP, KIND, MAP, 4, 2, 1, 2>); break;
+    default: VD_G(k_gemm<T, AOP, BOP, KIND, MAP, 128, 64, 2>); break;
   }
 #undef VD_G
 }

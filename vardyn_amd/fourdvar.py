@@ -26,7 +26,8 @@ computes this adjoint and drops it); ``"offset"`` -- the reference's
 Tape: ``"full"`` keeps every stage state of the window in HBM during the
 forward sweep (no recomputation); ``"checkpoint"`` keeps only the segment
 start states and recomputes each segment into one reusable tape during the
-reverse sweep, like the reference's ``jax.vjp`` of a segment.  Both give
+reverse sweep, like the reference's ``jax.vjp`` of a segment; ``"auto"``
+takes the full tape when it fits in 80% of the free device memory.  All give
 identical results.
 
 Batches: a plan built with ``nbatch = B`` advances B independent problems in
@@ -54,8 +55,8 @@ class FourDVar:
                  land=None):
         if control not in ("ic", "offset"):
             raise ValueError("control must be 'ic' or 'offset'")
-        if tape not in ("checkpoint", "full"):
-            raise ValueError("tape must be 'checkpoint' or 'full'")
+        if tape not in ("checkpoint", "full", "auto"):
+            raise ValueError("tape must be 'checkpoint', 'full' or 'auto'")
         if sigma_b is not None and not np.isscalar(sigma_b):
             raise NotImplementedError("only a scalar background standard deviation is supported")
         if prec and sigma_b is None:
@@ -72,7 +73,6 @@ class FourDVar:
         self.sigma_b = sigma_b
         self.sigma_r = 1.0 if sigma_r is None else float(sigma_r)
         self.prec = bool(prec)
-        self.tape_mode = tape
         cps = np.asarray(checkpoints, dtype=np.int64)
         if cps[0] != 0 or np.any(np.diff(cps) <= 0):
             raise VardynError("checkpoints must start at 0 and increase")
@@ -85,6 +85,12 @@ class FourDVar:
         self.nsteps = np.diff(cps).astype(int)
         self.n = plan.ny * plan.nx
         self.nctrl = self.n if control == "ic" else 1
+        if tape == "auto":
+            # the whole-window tape when it takes at most 80% of the free HBM
+            need = sum(plan.tape_bytes(int(k)) for k in self.nsteps)
+            free, _ = torch.cuda.mem_get_info(self.dev)
+            tape = "full" if need <= 0.8 * free else "checkpoint"
+        self.tape_mode = tape
 
         # observation table index of every member at each segment start / final time
         def obs_index(H, t):

@@ -154,8 +154,9 @@ def nadir_tracks(grid: Grid, n_tracks, spacing_km, n_steps, sigma, seed, obs: Ob
 
 
 def swaths(grid: Grid, n_days, steps_per_day, width_km, pixel_km, sigma, seed, obs: ObsSet | None = None,
-           step_seed=None):
-    """One straight swath per day, ``width_km`` wide with ``pixel_km`` pixels."""
+           step_seed=None, max_step=None):
+    """One straight swath per day, ``width_km`` wide with ``pixel_km`` pixels;
+    swaths after ``max_step`` (a window shorter than a day) are dropped."""
     rng = np.random.default_rng(seed)
     rng_step = np.random.default_rng(seed if step_seed is None else step_seed)
     obs = ObsSet() if obs is None else obs
@@ -165,6 +166,8 @@ def swaths(grid: Grid, n_days, steps_per_day, width_km, pixel_km, sigma, seed, o
         angle = rng.uniform(0, np.pi)
         off = rng.uniform(0.3, 0.7)
         step = int(day * steps_per_day + rng_step.integers(1, steps_per_day + 1))
+        if max_step is not None and step > max_step:
+            continue
         iy0, ix0, d, nrm = _line_points(grid, angle, off, pixel_km * 1e3, margin=-1e9)
         y = iy0[:, None] * grid.dy + across[None, :] * nrm[0]
         x = ix0[:, None] * grid.dx + across[None, :] * nrm[1]
@@ -265,5 +268,5 @@ def make_observations(case: QgCase, grid: Grid, n_steps=None, member=0):
         spd = int(round(86400 / case.dt))
         n_days = max(1, int(n_steps // spd))
         swaths(grid, n_days, spd, case.swath_width_km, case.swath_pixel_km, case.sigma_obs,
-               case.seed + 2 + 1009 * member, obs, step_seed=case.seed + 2)
+               case.seed + 2 + 1009 * member, obs, step_seed=case.seed + 2, max_step=n_steps)
     return obs.finalize()
