@@ -75,9 +75,11 @@ typedef struct vd_qg_params {
                             ny-2 and nx-2 are even (half the GEMM work);
                             1: always the full (n-2)^2 sine matrices */
   int gemm_config;       /* GEMM tile configuration: 0 = chosen from the
-                            problem shape; k in 1..5 forces configuration
-                            k-1 of 32x32 (split-K), 64x64, 64x64 (split-K
-                            2), 128x64, 128x64 (split-K 2) */
+                            problem shape; k in 1..10 forces configuration
+                            k-1: single-LDS-buffer 32x32 (split-K), 64x64,
+                            64x64 (split-K 2), 128x64, 128x64 (split-K 2);
+                            double-buffered 32x32 (split-K 4), 64x64,
+                            64x64 (split-K 2), 128x64, 64x32 (split-K 2) */
   int nbatch;            /* independent problems on this grid (0 or 1 = one);
                             with nbatch > 1 every field argument of the step
                             and inversion calls is an (nbatch, ny, nx) array */
@@ -154,7 +156,7 @@ int vd_qg_h2pv(vd_plan* plan, const void* h, const void* hb, void* q, void* stre
  * bathymetry terms the plan was created with) fused with one explicit stage
  * update -- Qgm.euler (jqgm.py:398-409) is base = q, coef = dt.  base NULL
  * means zero (out = coef * rhs).  rhs is zero outside the active points
- * (mask 3), as in the reference. */
+ * (mask 3), as in the reference.  out may alias base, not h or q. */
 int vd_qg_rhs(vd_plan* plan, const void* h, const void* q, const void* base, double coef,
               void* out, void* stream);
 

@@ -70,3 +70,25 @@ def test_batched_cost_grad_equals_members():
         Jb, Gb = eng.cost_and_grad(X[b])
         assert Jb == J[b]
         assert np.array_equal(Gb, G[b])
+
+
+def test_problem_set_equals_standalone_engines():
+    """Problems of different grids and windows evaluated concurrently on several
+    streams give exactly the results of their own engines run alone."""
+    from vardyn_amd.problem import build_problem
+    from vardyn_amd.problemset import ProblemSet
+    cases = [syn.QgCase("a", 40, 56, 10e3, 2.0, 1200.0, 30, n_tracks=4, track_km=7.0,
+                        checkpoint_steps=6, seed=31),
+             syn.QgCase("b", 64, 48, 10e3, 3.1, 1200.0, 22, n_tracks=6, track_km=7.0,
+                        checkpoint_steps=5, seed=32),
+             syn.QgCase("c", 36, 36, 10e3, 2.5, 1200.0, 41, n_tracks=3, track_km=7.0,
+                        checkpoint_steps=7, seed=33)]
+    ps = ProblemSet(cases, n_streams=2, autotune=False).capture()
+    rng = np.random.default_rng(3)
+    X = [torch.as_tensor(0.05 * rng.standard_normal(c.ny * c.nx), device="cuda") for c in cases]
+    out, done_ms = ps.evaluate(X)
+    assert len(done_ms) == 3 and all(t > 0 for t in done_ms)
+    for c, x, (J, G) in zip(cases, X, out):
+        eng = build_problem(c, precision="float64", control="ic", tape="full").engine
+        Jr, Gr = eng.run(x)
+        assert torch.equal(J, Jr) and torch.equal(G, Gr)

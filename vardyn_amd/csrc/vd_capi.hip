@@ -3,6 +3,8 @@
 // datlas-ocean/VarDyn).
 #include "vd_internal.h"
 
+#include <utility>
+
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -295,30 +297,34 @@ static int qg_adjoint(Plan& p, const T* adh1, const T* h0, const T* hb, T* adh0,
   T* lq1 = fld<T>(p, F_LQ1);
   T* lq2 = fld<T>(p, F_LQ2);
   T* lhs = fld<T>(p, F_LHS);
-  T* W[7];
-  for (int k = 0; k < 7; ++k) W[k] = fld<T>(p, F_W0 + k);
+  // The adjoint stage reads its cotangents (s0..s2) at neighbouring points, so
+  // it must not write into one of them: the Euler stage and the last RK3
+  // stage, whose result replaces lq, write lqx and the two are swapped.
+  T* lqx = fld<T>(p, F_LQX);
   launch_adj_init<T>(p, adh1, lh, lq, s);
   for (int n = nstep - 1; n >= 0; --n) {
     // h_{n+1} = pv2h(q_{n+1}): lq += P lh
     launch_inversion<T>(p, lh, nullptr, nullptr, lq, INV_ACC, s);
     if (p.scheme == VD_EULER) {
       launch_stage_adj<T>(p, tp.h(n, 0), tp.q(n, 0), lq, nullptr, nullptr, dt, 0, 0, lq, nullptr,
-                          nullptr, lq, lh, W, s);
+                          nullptr, lqx, lh, s);
+      std::swap(lq, lqx);
     } else if (p.scheme == VD_RK2) {
       launch_stage_adj<T>(p, tp.h(n, 1), tp.q(n, 1), lq, nullptr, nullptr, dt, 0, 0, nullptr,
-                          nullptr, nullptr, lq1, lhs, W, s);
+                          nullptr, nullptr, lq1, lhs, s);
       launch_inversion<T>(p, lhs, nullptr, nullptr, lq1, INV_ACC, s);
       launch_stage_adj<T>(p, tp.h(n, 0), tp.q(n, 0), lq1, nullptr, nullptr, T(0.5) * dt, 0, 0, lq,
-                          lq1, nullptr, lq, lh, W, s);
+                          lq1, nullptr, lq, lh, s);
     } else {
       launch_stage_adj<T>(p, tp.h(n, 2), tp.q(n, 2), lq, nullptr, nullptr, dt / T(6), 0, 0,
-                          nullptr, nullptr, nullptr, lq2, lhs, W, s);
+                          nullptr, nullptr, nullptr, lq2, lhs, s);
       launch_inversion<T>(p, lhs, nullptr, nullptr, lq2, INV_ACC, s);
       launch_stage_adj<T>(p, tp.h(n, 1), tp.q(n, 1), lq, lq2, nullptr, T(4) * dt / T(6),
-                          T(2) * dt, 0, nullptr, nullptr, nullptr, lq1, lhs, W, s);
+                          T(2) * dt, 0, nullptr, nullptr, nullptr, lq1, lhs, s);
       launch_inversion<T>(p, lhs, nullptr, nullptr, lq1, INV_ACC, s);
       launch_stage_adj<T>(p, tp.h(n, 0), tp.q(n, 0), lq, lq2, lq1, dt / T(6), -dt, T(0.5) * dt,
-                          lq, lq2, lq1, lq, lh, W, s);
+                          lq, lq2, lq1, lqx, lh, s);
+      std::swap(lq, lqx);
     }
   }
   launch_adj_final<T>(p, tp.h(0, 0), lh, lq, adh0, s);
@@ -417,7 +423,7 @@ int vd_qg_plan_create(const vd_qg_params* prm, vd_plan** out) {
   VD_REQUIRE(prm->nbatch >= 0 && prm->nbatch <= 4096, VD_ERR_INVALID_ARG,
              "nbatch must be in 0..4096");
   VD_REQUIRE(prm->gemm_config >= 0 && prm->gemm_config <= kNumTileCfg, VD_ERR_INVALID_ARG,
-             "gemm_config must be in 0..5");
+             "gemm_config must be in 0..10");
   vd_plan* h = new vd_plan();
   Plan& p = h->p;
   p.ny = prm->ny; p.nx = prm->nx;
@@ -576,6 +582,8 @@ int vd_qg_h2pv(vd_plan* h, const void* hh, const void* hb, void* q, void* stream
 int vd_qg_rhs(vd_plan* h, const void* hh, const void* q, const void* base, double coef, void* out,
               void* stream) {
   VD_REQUIRE(h && hh && q && out, VD_ERR_INVALID_ARG, "null argument");
+  VD_REQUIRE(out != hh && out != q, VD_ERR_INVALID_ARG,
+             "out must not alias h or q (the stencil reads them at neighbouring points)");
   Plan& p = h->p;
   hipStream_t s = static_cast<hipStream_t>(stream);
   if (p.prec == VD_F64) {

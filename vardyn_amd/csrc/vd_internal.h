@@ -32,14 +32,17 @@ void set_error(const std::string& msg);
 
 constexpr int kTile = 32;  // smallest GEMM output tile and the K chunk
 
-// GEMM tile configurations: BM x BN output tile per workgroup and ks K
-// groups of four waves (configuration 0 picks ks = 4, 2 or 1 from K).
+// GEMM tile configurations: BM x BN output tile per workgroup, ks K groups of
+// four waves, and the kernel: 0 = single LDS buffer (k_gemm), 1 = double
+// buffer with transposed B (k_gemm2).  Configuration 0 picks ks = 4, 2 or 1
+// from K.
 struct TileCfg {
-  int bm, bn, ks;
+  int bm, bn, ks, kern;
 };
-constexpr int kNumTileCfg = 5;
+constexpr int kNumTileCfg = 10;
 inline constexpr TileCfg kTileCfgs[kNumTileCfg] = {
-    {32, 32, 4}, {64, 64, 1}, {64, 64, 2}, {128, 64, 1}, {128, 64, 2}};
+    {32, 32, 4, 0}, {64, 64, 1, 0}, {64, 64, 2, 0}, {128, 64, 1, 0}, {128, 64, 2, 0},
+    {32, 32, 4, 1}, {64, 64, 1, 1}, {64, 64, 2, 1}, {128, 64, 1, 1}, {64, 32, 2, 1}};
 inline bool tile_cfg_fits(int c, int M, int N, int K) {
   return M % kTileCfgs[c].bm == 0 && N % kTileCfgs[c].bn == 0 &&
          (c == 0 || (K / kTile) % kTileCfgs[c].ks == 0);
@@ -107,8 +110,7 @@ enum Field {
   F_HBM = 0, F_QB,
   F_HA, F_QA, F_HB, F_QB2, F_HS1, F_QS1, F_HS2, F_QS2, F_K0, F_K1,
   F_DH, F_DQ, F_DH2, F_DQ2, F_DHS1, F_DQS1, F_DHS2, F_DQS2, F_DK0, F_DK1,
-  F_LH, F_LQ, F_LQ1, F_LQ2, F_LHS,
-  F_W0, F_W1, F_W2, F_W3, F_W4, F_W5, F_W6,
+  F_LH, F_LQ, F_LQ1, F_LQ2, F_LHS, F_LQX,
   F_HOUT,
   F_ZERO,  // never written: the zero boundary field
   F_COUNT
@@ -163,7 +165,7 @@ void launch_finish_tl(const Plan& p, const T* dh, T* dh1, hipStream_t s);
 template <typename T>
 void launch_stage_adj(const Plan& p, const T* h, const T* q, const T* s0, const T* s1,
                       const T* s2, T a0, T a1, T a2, const T* acc0, const T* acc1,
-                      const T* acc2, T* lq_out, T* lh_out, T* W[7], hipStream_t s);
+                      const T* acc2, T* lq_out, T* lh_out, hipStream_t s);
 // lh = adh1 with zeros on land; lq = 0.
 template <typename T>
 void launch_adj_init(const Plan& p, const T* adh1, T* lh, T* lq, hipStream_t s);

@@ -426,6 +426,172 @@ __global__ __launch_bounds__(256 * KS) void k_gemm(GemmArgs<T> g) {
                                 n0 + wn * (BN / 2) + b_ * 16 + (lane & 15), val[a_][b_][r]);
 }
 
+// Double-buffered kernel.  Per K chunk of 32 there is one barrier: while
+// the four waves of a group multiply the chunk in one LDS buffer, the next
+// chunk (fetched a chunk earlier) is combined into the other buffer and the
+// one after is fetched.  LDS holds A row-major (As[m][k]) and B transposed
+// (Bt[n][k]) with a row stride of 34 doubles / 36 floats, so that lane
+// (r = l%16, q = l/16) reads its four consecutive k of a 16-wide group with
+// 16-byte loads and no bank conflicts; MFMA step s pairs k = 4q + s across the
+// four lane quarters on both operands.  A is fetched four consecutive k per
+// thread along its rows, B one column per lane (four k per thread), so every
+// global load instruction reads contiguous row segments.
+template <typename T, int AOP, int BOP, int KIND, int MAP, int BM, int BN, int KS>
+__global__ __launch_bounds__(256 * KS) void k_gemm2(GemmArgs<T> g) {
+  typedef typename Mfma<T>::acc_t acc_t;
+  constexpr int LDK = sizeof(T) == 8 ? 34 : 36;
+  constexpr int MT = BM / 32, NT = BN / 32;
+  constexpr int ASZ = BM * LDK, BSZ = BN * LDK;
+  constexpr int STAGE = 2 * (ASZ + BSZ);  // two buffers of A and B per group
+  static_assert(KS == 1 || (KS - 1) * BM * BN <= KS * STAGE, "split-K partials must fit");
+  __shared__ __align__(16) T sm[KS * STAGE];
+  const int tid = threadIdx.x;
+  const int grp = tid >> 8, gt = tid & 255, lane = tid & 63;
+  const int wq = (tid >> 6) & 3, wm = wq >> 1, wn = wq & 1;
+  const int r16 = lane & 15, q4 = lane >> 4;
+  int z, tile;
+  work_item(g, &z, &tile);
+  const GemmPtrs<T> P = gemm_ptrs<T, AOP, BOP, MAP>(g, z);
+  const int m0 = (tile / g.tn) * BM, n0 = (tile % g.tn) * BN;
+  T* base = sm + grp * STAGE;
+  auto As = [&](int b) { return base + b * (ASZ + BSZ); };
+  auto Bs = [&](int b) { return base + b * (ASZ + BSZ) + ASZ; };
+
+  // fetch mapping: A row ar + 32 ps, k = ac..ac+3; B column bn, k = bk + 16 v + 0..3
+  constexpr int APASS = BM / 32, BV = BN / 32, BKG = 256 / BN;
+  const int ar = gt >> 3, ac = (gt & 7) * 4;
+  const int bn = gt % BN, bk = (gt / BN) * 4;
+  static_assert(4 * BKG * BV == kTile, "the B fetch mapping covers the 32 k of a chunk");
+  constexpr int NA = a_words<AOP>(), NB = b_words<BOP>();
+  T ra[APASS][4][NA], rb[BV][4][NB];
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int ps = 0; ps < APASS; ++ps)
+#pragma unroll
+      for (int u = 0; u < 4; ++u) fetch_a<T, AOP>(g, P.A, m0 + ar + 32 * ps, k0 + ac + u, ra[ps][u]);
+#pragma unroll
+    for (int v = 0; v < BV; ++v)
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        fetch_b<T, BOP>(g, P.B, P.Fld, P.Fsub, k0 + bk + 4 * BKG * v + u, n0 + bn, rb[v][u]);
+  };
+  auto store = [&](int b, int k0) {
+    T* a = As(b);
+    T* bt = Bs(b);
+#pragma unroll
+    for (int ps = 0; ps < APASS; ++ps)
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        a[(ar + 32 * ps) * LDK + ac + u] = combine_a<T, AOP>(g, k0 + ac + u, ra[ps][u], P.sgn);
+#pragma unroll
+    for (int v = 0; v < BV; ++v)
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        bt[bn * LDK + bk + 4 * BKG * v + u] =
+            combine_b<T, BOP>(g, k0 + bk + 4 * BKG * v + u, n0 + bn, rb[v][u], P.sgn);
+  };
+
+  constexpr int NACC = (MT * NT == 1) ? 4 : 1;
+  acc_t acc[MT][NT][NACC];
+#pragma unroll
+  for (int a_ = 0; a_ < MT; ++a_)
+#pragma unroll
+    for (int b_ = 0; b_ < NT; ++b_)
+#pragma unroll
+      for (int c_ = 0; c_ < NACC; ++c_)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[a_][b_][c_][r] = T(0);
+  auto multiply = [&](int b) {
+    const T* a = As(b);
+    const T* bt = Bs(b);
+#pragma unroll
+    for (int g2 = 0; g2 < 2; ++g2) {
+      T av[MT][4], bv[NT][4];
+#pragma unroll
+      for (int a_ = 0; a_ < MT; ++a_) {
+        const T* src = a + (wm * (BM / 2) + a_ * 16 + r16) * LDK + 16 * g2 + 4 * q4;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) av[a_][s] = src[s];
+      }
+#pragma unroll
+      for (int b_ = 0; b_ < NT; ++b_) {
+        const T* src = bt + (wn * (BN / 2) + b_ * 16 + r16) * LDK + 16 * g2 + 4 * q4;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) bv[b_][s] = src[s];
+      }
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int a_ = 0; a_ < MT; ++a_)
+#pragma unroll
+          for (int b_ = 0; b_ < NT; ++b_) {
+            const int c_ = NACC == 4 ? s : 0;
+            acc[a_][b_][c_] = Mfma<T>::run(av[a_][s], bv[b_][s], acc[a_][b_][c_]);
+          }
+    }
+  };
+
+  // group grp takes chunks grp, grp + KS, ...; all groups run the same count
+  const int nchunk = g.K / (kTile * KS);
+  auto kof = [&](int it) { return (it * KS + grp) * kTile; };
+  fetch(kof(0));
+  store(0, kof(0));
+  if (nchunk > 1) fetch(kof(1));
+  __syncthreads();
+  for (int it = 0; it < nchunk; ++it) {
+    multiply(it & 1);
+    if (it + 1 < nchunk) store((it + 1) & 1, kof(it + 1));
+    if (it + 2 < nchunk) fetch(kof(it + 2));
+    __syncthreads();
+  }
+
+  T val[MT][NT][4];
+#pragma unroll
+  for (int a_ = 0; a_ < MT; ++a_)
+#pragma unroll
+    for (int b_ = 0; b_ < NT; ++b_)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        T v = acc[a_][b_][0][r];
+        if (NACC == 4) v = (acc[a_][b_][0][r] + acc[a_][b_][NACC > 1 ? 1 : 0][r]) +
+                           (acc[a_][b_][NACC > 2 ? 2 : 0][r] + acc[a_][b_][NACC > 3 ? 3 : 0][r]);
+        val[a_][b_][r] = v;
+      }
+  if (KS > 1) {
+    // the loop ended on a barrier, so the staging area is free for the partials
+    constexpr int PER = MT * NT * 4;
+    if (grp > 0) {
+      T* park = sm + (size_t)(grp - 1) * 256 * PER;
+#pragma unroll
+      for (int a_ = 0; a_ < MT; ++a_)
+#pragma unroll
+        for (int b_ = 0; b_ < NT; ++b_)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) park[((a_ * NT + b_) * 4 + r) * 256 + gt] = val[a_][b_][r];
+    }
+    __syncthreads();
+    if (grp != 0) return;
+#pragma unroll
+    for (int qg = 1; qg < KS; ++qg) {
+      const T* src = sm + (size_t)(qg - 1) * 256 * PER;
+#pragma unroll
+      for (int a_ = 0; a_ < MT; ++a_)
+#pragma unroll
+        for (int b_ = 0; b_ < NT; ++b_)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) val[a_][b_][r] += src[((a_ * NT + b_) * 4 + r) * 256 + gt];
+    }
+  }
+#pragma unroll
+  for (int a_ = 0; a_ < MT; ++a_)
+#pragma unroll
+    for (int b_ = 0; b_ < NT; ++b_)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        store_out<T, KIND, MAP>(g, P, m0 + wm * (BM / 2) + a_ * 16 + Mfma<T>::out_row(lane, r),
+                                n0 + wn * (BN / 2) + b_ * 16 + r16, val[a_][b_][r]);
+}
+
 template <typename T, int AOP, int BOP, int KIND, int MAP>
 static void gemm(GemmArgs<T> a, int M, int N, int nz, int cfg, hipStream_t s) {
   // nz = members x parities.  Fall back to configuration 0 when the shape
@@ -451,7 +617,12 @@ static void gemm(GemmArgs<T> a, int M, int N, int nz, int cfg, hipStream_t s) {
     case 1: VD_G(k_gemm<T, AOP, BOP, KIND, MAP, 64, 64, 1>); break;
     case 2: VD_G(k_gemm<T, AOP, BOP, KIND, MAP, 64, 64, 2>); break;
     case 3: VD_G(k_gemm<T, AOP, BOP, KIND, MAP, 128, 64, 1>); break;
-    default: VD_G(k_gemm<T, AOP, BOP, KIND, MAP, 128, 64, 2>); break;
+    case 4: VD_G(k_gemm<T, AOP, BOP, KIND, MAP, 128, 64, 2>); break;
+    case 5: VD_G(k_gemm2<T, AOP, BOP, KIND, MAP, 32, 32, 4>); break;
+    case 6: VD_G(k_gemm2<T, AOP, BOP, KIND, MAP, 64, 64, 1>); break;
+    case 7: VD_G(k_gemm2<T, AOP, BOP, KIND, MAP, 64, 64, 2>); break;
+    case 8: VD_G(k_gemm2<T, AOP, BOP, KIND, MAP, 128, 64, 1>); break;
+    default: VD_G(k_gemm2<T, AOP, BOP, KIND, MAP, 64, 32, 2>); break;
   }
 #undef VD_G
 }
@@ -714,20 +885,28 @@ __global__ void k_finish(Phys<T> P, const T* __restrict__ h, const T* __restrict
   h1[o] = P.mask[o] == 0 ? T(NAN) : (mdt ? h[o] - mdt[o] : h[o]);
 }
 
+// Fused RK stage: qout = qbase + c0 k0 + c1 k1 + c2 rhs(h, q) (rhs stored in
+// kstore when given), one point per thread.  The stencil loads are taken at
+// indices clamped into the band 2..n-3 (which holds every active point, mask
+// == 3), so they are always in bounds and issued without a branch; the mask
+// then selects rhs or zero.  (Several rows per thread were measured and gave
+// no gain.)
 template <typename T>
-__global__ void k_stage(Phys<T> P, const T* __restrict__ h, const T* __restrict__ q,
-                        const T* __restrict__ qbase, const T* __restrict__ k0,
-                        const T* __restrict__ k1, T c0, T c1, T c2, T* __restrict__ qout,
-                        T* __restrict__ kstore) {
+__global__ __launch_bounds__(256) void k_stage(Phys<T> P, const T* __restrict__ h,
+                                               const T* __restrict__ q,
+                                               const T* __restrict__ qbase,
+                                               const T* __restrict__ k0,
+                                               const T* __restrict__ k1, T c0, T c1, T c2,
+                                               T* __restrict__ qout, T* __restrict__ kstore) {
   VD_POINT_2D
   VD_B(h); VD_B(q); VD_B(qbase); VD_B(k0); VD_B(k1); VD_B(qout); VD_B(kstore);
-  T k = T(0);
-  if (P.mask[o] == 3) k = rhs_at(h, q, i, j, P);
+  const T kk = rhs_at(h, q, min(max(i, 2), P.ny - 3), min(max(j, 2), nx - 3), P);
+  const T k = P.mask[o] == 3 ? kk : T(0);
   if (kstore) kstore[o] = k;
-  T r = qbase[o];
-  if (k0) r += c0 * k0[o];
-  if (k1) r += c1 * k1[o];
-  qout[o] = r + c2 * k;
+  T v = qbase[o];
+  if (k0) v += c0 * k0[o];
+  if (k1) v += c1 * k1[o];
+  qout[o] = v + c2 * k;
 }
 
 // ---------------------------------------------------------------------------
@@ -854,12 +1033,14 @@ __device__ __forceinline__ void adj_terms(const T* h, const T* q, const T* s0, c
 }
 
 // Fused adjoint stage (one launch, deterministic, no atomics).  A block owns
-// an AX x AY tile of output points: it first evaluates adj_terms on the tile
+// an AX x AY tile of output points.  Phase 1 evaluates adj_terms on the tile
 // plus a halo of 2 into LDS (zero where the point is inactive or outside the
-// grid), then gathers, for each output point p,
+// grid), together with the validity of u and v at every tile point (inside
+// their domains and not NaN, jqgm.py:238-262).  Phase 2 gathers, for each
+// output point p,
 //   lambda_q(p) from the T points whose upwind / diffusion stencils read q(p),
 //   lambda_h(p) from lambda_u, lambda_v at the velocity points that read h(p).
-constexpr int kAX = 64, kAY = 8, kAH = 2;
+constexpr int kAX = 64, kAY = 16, kAH = 2;
 constexpr int kRW = kAX + 2 * kAH, kRH = kAY + 2 * kAH;
 
 template <typename T>
@@ -867,9 +1048,12 @@ __global__ __launch_bounds__(kAX * kAY) void k_adj_fused(
     Phys<T> P, const T* __restrict__ h, const T* __restrict__ q, const T* __restrict__ s0,
     const T* __restrict__ s1, const T* __restrict__ s2, T a0, T a1, T a2, const T* acc0,
     const T* acc1, const T* acc2, T* lq_out, T* __restrict__ lh_out) {
-  // acc* may alias lq_out (in-place accumulation); each thread reads and
-  // writes only its own point of those fields.
+  // acc* may alias lq_out (in-place accumulation): each thread reads and
+  // writes only its own point of those fields.  s0..s2 are read at the halo
+  // points of the tile, which other blocks write, so they must not alias
+  // lq_out.
   __shared__ T W[7][kRH][kRW];
+  __shared__ uint8_t okU[kRH][kRW], okV[kRH][kRW];
   const int ny = P.ny, nx = P.nx;
   const int tid = threadIdx.y * kAX + threadIdx.x;
   const int j0 = blockIdx.x * kAX, i0 = blockIdx.y * kAY;
@@ -884,6 +1068,8 @@ __global__ __launch_bounds__(kAX * kAY) void k_adj_fused(
       adj_terms(h, q, s0, s1, s2, a0, a1, a2, i, j, P, w);
 #pragma unroll
     for (int k = 0; k < 7; ++k) W[k][li][lj] = w[k];
+    okU[li][lj] = i >= 1 && i <= ny - 2 && j >= 1 && j <= nx - 1 && !u_isnan(h, nx, i, j);
+    okV[li][lj] = i >= 1 && i <= ny - 1 && j >= 1 && j <= nx - 2 && !v_isnan(h, nx, i, j);
   }
   __syncthreads();
   const int i = i0 + threadIdx.y, j = j0 + threadIdx.x;
@@ -907,21 +1093,17 @@ __global__ __launch_bounds__(kAX * kAY) void k_adj_fused(
           P.ky * (W[6][li + 1][lj] + W[6][li - 1][lj] - 2 * c);
   }
   lq_out[o] = lq;
-  // lambda_u[a][b] = 0.5 (W0[a][b] + W0[a][b-1]) on the u domain, zero where u was NaN
-  auto lu = [&](int a, int b) -> T {
-    if (a < 1 || a > ny - 2 || b < 1 || b > nx - 1) return T(0);
-    if (u_isnan(h, nx, a, b)) return T(0);
-    const int la = a - i0 + kAH, lb = b - j0 + kAH;
-    return T(0.5) * (W[0][la][lb] + W[0][la][lb - 1]);
+  // lambda_u at u point (a, b) = 0.5 (W0[a][b] + W0[a][b-1]) where u is valid;
+  // the arguments are LDS coordinates (every one used below is inside the tile
+  // plus halo)
+  auto lu = [&](int la, int lb) -> T {
+    return okU[la][lb] ? T(0.5) * (W[0][la][lb] + W[0][la][lb - 1]) : T(0);
   };
-  auto lv = [&](int a, int b) -> T {
-    if (a < 1 || a > ny - 1 || b < 1 || b > nx - 2) return T(0);
-    if (v_isnan(h, nx, a, b)) return T(0);
-    const int la = a - i0 + kAH, lb = b - j0 + kAH;
-    return T(0.5) * (W[1][la][lb] + W[1][la - 1][lb]);
+  auto lv = [&](int la, int lb) -> T {
+    return okV[la][lb] ? T(0.5) * (W[1][la][lb] + W[1][la - 1][lb]) : T(0);
   };
-  lh_out[o] = P.cu * (lu(i - 1, j + 1) + lu(i - 1, j) - lu(i + 1, j) - lu(i + 1, j + 1)) +
-              P.cv * (lv(i, j - 1) + lv(i + 1, j - 1) - lv(i + 1, j + 1) - lv(i, j + 1));
+  lh_out[o] = P.cu * (lu(li - 1, lj + 1) + lu(li - 1, lj) - lu(li + 1, lj) - lu(li + 1, lj + 1)) +
+              P.cv * (lv(li, lj - 1) + lv(li + 1, lj - 1) - lv(li + 1, lj + 1) - lv(li, lj + 1));
 }
 
 
@@ -1037,8 +1219,7 @@ void launch_finish_tl(const Plan& p, const T* dh, T* dh1, hipStream_t s) {
 template <typename T>
 void launch_stage_adj(const Plan& p, const T* h, const T* q, const T* s0, const T* s1,
                       const T* s2, T a0, T a1, T a2, const T* acc0, const T* acc1,
-                      const T* acc2, T* lq_out, T* lh_out, T* W[7], hipStream_t s) {
-  (void)W;
+                      const T* acc2, T* lq_out, T* lh_out, hipStream_t s) {
   Phys<T> P = make_phys<T>(p);
   dim3 grid((p.nx + kAX - 1) / kAX, (p.ny + kAY - 1) / kAY, p.nb);
   hipLaunchKernelGGL(k_adj_fused<T>, grid, dim3(kAX, kAY), 0, s, P, h, q, s0, s1, s2, a0, a1, a2,
@@ -1077,7 +1258,7 @@ void launch_adj_final(const Plan& p, const T* hcur, const T* lh, const T* lq, T*
   template void launch_finish_tl<T>(const Plan&, const T*, T*, hipStream_t);                  \
   template void launch_stage_adj<T>(const Plan&, const T*, const T*, const T*, const T*,      \
                                     const T*, T, T, T, const T*, const T*, const T*, T*, T*,  \
-                                    T* W[7], hipStream_t);                                    \
+                                    hipStream_t);                                             \
   template void launch_adj_init<T>(const Plan&, const T*, T*, T*, hipStream_t);               \
   template void launch_adj_final<T>(const Plan&, const T*, const T*, const T*, T*,            \
                                     hipStream_t);

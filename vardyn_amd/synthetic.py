@@ -270,3 +270,27 @@ def make_observations(case: QgCase, grid: Grid, n_steps=None, member=0):
         swaths(grid, n_days, spd, case.swath_width_km, case.swath_pixel_km, case.sigma_obs,
                case.seed + 2 + 1009 * member, obs, step_seed=case.seed + 2, max_step=n_steps)
     return obs.finalize()
+
+
+def config4_cases(n=48, seed=2024, dx=5e3, dt=900.0):
+    """``n`` independent regional problems of uneven size and window length:
+    sides drawn from {96, 128, 192, 256, 320, 384} (squares and rectangles),
+    windows uniform in 5-20 days at dt = 900 s, Ld uniform in 15-40 km
+    (c = f0 Ld), initial-field RMS uniform in 0.05-0.15 m, 5-40 nadir tracks
+    every 7 km; checkpoints about every sqrt(window) steps.  Each problem has
+    its own seed."""
+    rng = np.random.default_rng(seed)
+    sides = np.array([96, 128, 192, 256, 320, 384])
+    f0 = 1e-4
+    cases = []
+    for p in range(n):
+        ny, nx = (int(v) for v in rng.choice(sides, 2))
+        days = rng.uniform(5.0, 20.0)
+        n_steps = int(round(days * 86400.0 / dt))
+        ld = rng.uniform(15e3, 40e3)
+        cases.append(QgCase(f"c4_{p:02d}_{ny}x{nx}_{n_steps}", ny, nx, dx, f0 * ld, dt, n_steps,
+                            f0=f0, rms=float(rng.uniform(0.05, 0.15)),
+                            n_tracks=int(rng.integers(5, 41)), track_km=7.0,
+                            checkpoint_steps=int(round(np.sqrt(n_steps))),
+                            seed=int(seed + 7919 * (p + 1))))
+    return cases
