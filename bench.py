@@ -68,6 +68,8 @@ def parse():
                     help="only the stencil measurement and a copy of known size (for counter runs)")
     ap.add_argument("--problem-set", action="store_true",
                     help="the 48-problem uneven-size workload instead (aggregate gradients/s)")
+    ap.add_argument("--problem-set-size", type=int, default=48,
+                    help="first N problems of the problem set (profiling)")
     ap.add_argument("--streams", type=int, default=4,
                     help="HIP streams the problem set is spread over")
     ap.add_argument("--check-f32", action="store_true",
@@ -84,7 +86,10 @@ def gemm_roofline(plan, precision, n_inv=100, reps=5):
     (two 2-D sine transforms as dense products), a quarter of it per GEMM
     launch; ``achieved`` uses that count.  With folded sine matrices the
     kernels execute half of it: ``executed_tflops`` / ``executed_frac`` give
-    the MFMA utilisation on the flops actually issued."""
+    the MFMA utilisation on the flops actually issued.  ``traffic`` (bytes per
+    launch moved below L2) comes from the committed counter runs of this
+    configuration (FETCH_SIZE / WRITE_SIZE, calibrated on a copy of known size),
+    when there is one."""
     import torch
     q = torch.randn(plan.shape, dtype=plan.dtype, device=plan.device) * 1e-6
     hb = torch.zeros_like(q)
@@ -113,10 +118,15 @@ def gemm_roofline(plan, precision, n_inv=100, reps=5):
     peak = MFMA_PEAK["f64" if precision == "float64" else "f32"]
     achieved = alg_flops_launch / per_launch_s / 1e12
     executed = exec_flops_launch / per_launch_s / 1e12
-    return {"bound": "mfma", "kernel": "k_gemm / k_gemm_rd (sine-matrix GEMMs of the Helmholtz "
+    traffic, src = None, None
+    rec = os.path.join(ROOT, "profiles", f"gemm_traffic_qg{plan.ny}_b{plan.nb}.json")
+    if precision == "float64" and os.path.exists(rec):
+        with open(rec) as f:
+            traffic, src = json.load(f)["traffic_bytes_per_launch"], os.path.relpath(rec, ROOT)
+    return {"bound": "mfma", "kernel": "k_gemm / k_gemm2 (sine-matrix GEMMs of the Helmholtz "
                                        "inversion, 4 launches per inversion)",
             "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
-            "frac": round(achieved / peak, 4), "traffic": None,
+            "frac": round(achieved / peak, 4), "traffic": traffic, "traffic_source": src,
             "avg_launch_us": round(per_launch_s * 1e6, 3),
             "algorithmic_flops_per_launch": alg_flops_launch,
             "executed_flops_per_launch": exec_flops_launch, "folded_dst": plan.folded,
@@ -264,7 +274,7 @@ def problem_set(args):
     import torch
     from vardyn_amd import synthetic as syn
     from vardyn_amd.problemset import ProblemSet
-    cases = syn.config4_cases()
+    cases = syn.config4_cases()[:args.problem_set_size]
     t0 = time.time()
     ps = ProblemSet(cases, precision=args.precision, tape=args.tape, n_streams=args.streams,
                     autotune=not args.no_autotune).capture()
@@ -288,13 +298,13 @@ def problem_set(args):
     value = len(cases) * args.steps / dt
     done = np.array(done)
     return {
-        "metric": "fwd+adjoint 4D-Var gradients/sec, 48 uneven regional QG problems",
+        "metric": f"fwd+adjoint 4D-Var gradients/sec, {len(cases)} uneven regional QG problems",
         "value": round(value, 4), "unit": "gradients/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms, 2), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if args.precision == "float64" else "f32",
         "data": "synthetic (seeded fields and nadir tracks per problem)",
-        "config": {"workload": "48 problems, sides 96-384, 5-20 day windows, Ld 15-40 km",
+        "config": {"workload": f"{len(cases)} problems, sides 96-384, 5-20 day windows, Ld 15-40 km",
                    "streams": args.streams, "tapes": sorted({pb.engine.tape_mode for pb in ps.problems}),
                    "grid_point_steps_per_evaluation": gps},
         "grid_point_steps_per_s": round(gps * args.steps / dt, 1),

@@ -75,11 +75,12 @@ typedef struct vd_qg_params {
                             ny-2 and nx-2 are even (half the GEMM work);
                             1: always the full (n-2)^2 sine matrices */
   int gemm_config;       /* GEMM tile configuration: 0 = chosen from the
-                            problem shape; k in 1..10 forces configuration
+                            problem shape; k in 1..11 forces configuration
                             k-1: single-LDS-buffer 32x32 (split-K), 64x64,
                             64x64 (split-K 2), 128x64, 128x64 (split-K 2);
                             double-buffered 32x32 (split-K 4), 64x64,
-                            64x64 (split-K 2), 128x64, 64x32 (split-K 2) */
+                            64x64 (split-K 2), 128x64, 64x32 (split-K 2);
+                            single-LDS-buffer 128x128 */
   int nbatch;            /* independent problems on this grid (0 or 1 = one);
                             with nbatch > 1 every field argument of the step
                             and inversion calls is an (nbatch, ny, nx) array */
@@ -163,6 +164,65 @@ int vd_qg_rhs(vd_plan* plan, const void* h, const void* q, const void* base, dou
 /* field[i] = value where the plan mask is 0 (land); the reference fills the
  * initial state with NaN there (mapping/src/mod.py:697-700). */
 int vd_qg_fill_land(vd_plan* plan, void* field, double value, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * The pieces of one Qgm.step call (jqgm.py:517-557), for drivers that advance
+ * several problems step by step (ragged groups below).  vd_qg_step is
+ * prep -> nstep x (stages + inversions) -> finish; vd_qg_step_adj is
+ * adj_init -> reverse steps -> adj_final.
+ * ------------------------------------------------------------------------- */
+/* Segment start (jqgm.py:528-536): h = h0 (+ mdt), q = h2pv(h0, hb); hb NULL
+ * means zero. */
+int vd_qg_prep(vd_plan* plan, const void* h0, const void* hb, void* h, void* q, void* stream);
+/* Segment end (jqgm.py:549-555): h1 = h (- mdt), NaN on land. */
+int vd_qg_finish(vd_plan* plan, const void* h, void* h1, void* stream);
+/* Start of the reverse sweep (adjoint of vd_qg_finish): lh = adh1 with zeros
+ * on land, lq = 0. */
+int vd_qg_adj_init(vd_plan* plan, const void* adh1, void* lh, void* lq, void* stream);
+/* End of the reverse sweep (adjoint of vd_qg_prep at the segment start state
+ * hcur): adh0 = lh + h2pv^T lq. */
+int vd_qg_adj_final(vd_plan* plan, const void* hcur, const void* lh, const void* lq, void* adh0,
+                    void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Ragged groups: independent problems on grids of their own (own size, c,
+ * mask, window), advanced by shared launches -- one launch per stage or
+ * inversion covers every member that is still running.  No reference
+ * counterpart: the reference runs such problem sets one process per
+ * subdomain (mapping/run_assimilation.py:27-350).
+ *
+ * A flat field holds member b at elements offsets[b] .. offsets[b+1]-1 as an
+ * (ny_b, nx_b) array.  The calls act on members 0 .. nact-1, so a driver
+ * orders the members by window length (longest first) and passes the number
+ * still running.  Per member the arithmetic is exactly that of its own plan
+ * with GEMM configuration 0 (32 x 32 tiles).
+ * ------------------------------------------------------------------------- */
+typedef struct vd_group vd_group;
+
+/* Group n plans (nbatch 1 each; same precision, time scheme and folding of
+ * the sine transforms).  The plans must outlive the group. */
+int vd_group_create(vd_plan* const* plans, int n, vd_group** group);
+int vd_group_destroy(vd_group* group);
+/* offsets[0..n]: element offset of each member's fields; offsets[n] = total. */
+int vd_group_offsets(const vd_group* group, size_t* offsets);
+/* One explicit RK stage (jqgm.py:398-477) for members 0..nact-1:
+ * qout = qbase + c0 k0 + c1 k1 + c2 rhs(h, q); kstore (may be NULL)
+ * receives rhs.  k0, k1 may be NULL. */
+int vd_group_stage(vd_group* group, int nact, const void* h, const void* q, const void* qbase,
+                   const void* k0, const void* k1, double c0, double c1, double c2, void* qout,
+                   void* kstore, void* stream);
+/* Helmholtz inversion (jqgm.py:295-319) for members 0..nact-1:
+ * accumulate = 0: out = hb + P(F - Fsub) on the interior, hb on the ring
+ * (Fsub, hb may be NULL = zero); accumulate = 1: out += P(F). */
+int vd_group_inversion(vd_group* group, int nact, const void* F, const void* Fsub, const void* hb,
+                       void* out, int accumulate, void* stream);
+/* Adjoint of one RK stage for members 0..nact-1: with the cotangent
+ * a0 s0 + a1 s1 + a2 s2 of rhs(h, q), lq_out = acc0 + acc1 + acc2 + d/dq and
+ * lh_out = d/dh.  s0..s2 must not alias lq_out; acc* may. */
+int vd_group_stage_adj(vd_group* group, int nact, const void* h, const void* q, const void* s0,
+                       const void* s1, const void* s2, double a0, double a1, double a2,
+                       const void* acc0, const void* acc1, const void* acc2, void* lq_out,
+                       void* lh_out, void* stream);
 
 /* ---------------------------------------------------------------------------
  * L3 observation operator (Obsop_interp_l3_jax, mapping/src/obsop.py:375-678).

@@ -74,7 +74,7 @@ def test_error_paths_without_device():
     assert lib.vd_qg_plan_create(C.byref(p), C.byref(h)) == -1
     p.time_scheme, p.c = 1, -1.0
     assert lib.vd_qg_plan_create(C.byref(p), C.byref(h)) == -1
-    p.c, p.gemm_config = 2.0, 11
+    p.c, p.gemm_config = 2.0, 12
     assert lib.vd_qg_plan_create(C.byref(p), C.byref(h)) == -1
     assert b"gemm_config" in lib.vd_last_error()
     p.gemm_config, p.nbatch = 0, -2

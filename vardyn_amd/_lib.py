@@ -62,6 +62,17 @@ SIGNATURES = {
     "vd_qg_h2pv": (_I, [_VP, _VP, _VP, _VP, _VP]),
     "vd_qg_rhs": (_I, [_VP, _VP, _VP, _VP, C.c_double, _VP, _VP]),
     "vd_qg_fill_land": (_I, [_VP, _VP, _D, _VP]),
+    "vd_qg_prep": (_I, [_VP, _VP, _VP, _VP, _VP, _VP]),
+    "vd_qg_finish": (_I, [_VP, _VP, _VP, _VP]),
+    "vd_qg_adj_init": (_I, [_VP, _VP, _VP, _VP, _VP]),
+    "vd_qg_adj_final": (_I, [_VP, _VP, _VP, _VP, _VP, _VP]),
+    "vd_group_create": (_I, [C.POINTER(_VP), _I, C.POINTER(_VP)]),
+    "vd_group_destroy": (_I, [_VP]),
+    "vd_group_offsets": (_I, [_VP, C.POINTER(_SZ)]),
+    "vd_group_stage": (_I, [_VP, _I, _VP, _VP, _VP, _VP, _VP, _D, _D, _D, _VP, _VP, _VP]),
+    "vd_group_inversion": (_I, [_VP, _I, _VP, _VP, _VP, _VP, _I, _VP]),
+    "vd_group_stage_adj": (_I, [_VP, _I, _VP, _VP, _VP, _VP, _VP, _D, _D, _D, _VP, _VP, _VP,
+                                _VP, _VP, _VP]),
     "vd_obs_misfit": (_I, [C.POINTER(ObsTable), _I, _VP, _VP, _VP, _VP, _D, _VP]),
     "vd_obs_adjoint": (_I, [C.POINTER(ObsTable), _I, _VP, _VP, _VP, _VP]),
     "vd_dot_f64": (_I, [_I64, _VP, _VP, _D, _VP, _VP, _VP]),

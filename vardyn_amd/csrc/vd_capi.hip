@@ -397,9 +397,6 @@ static int autotune(Plan& p, int reps, hipStream_t s) {
 
 using namespace vd;
 
-struct vd_plan {
-  Plan p;
-};
 
 extern "C" {
 
@@ -423,7 +420,7 @@ int vd_qg_plan_create(const vd_qg_params* prm, vd_plan** out) {
   VD_REQUIRE(prm->nbatch >= 0 && prm->nbatch <= 4096, VD_ERR_INVALID_ARG,
              "nbatch must be in 0..4096");
   VD_REQUIRE(prm->gemm_config >= 0 && prm->gemm_config <= kNumTileCfg, VD_ERR_INVALID_ARG,
-             "gemm_config must be in 0..10");
+             "gemm_config must be in 0..11");
   vd_plan* h = new vd_plan();
   Plan& p = h->p;
   p.ny = prm->ny; p.nx = prm->nx;
@@ -595,6 +592,63 @@ int vd_qg_rhs(vd_plan* h, const void* hh, const void* q, const void* base, doubl
     launch_stage<float>(p, (const float*)hh, (const float*)q, b, nullptr, nullptr, 0.f, 0.f,
                         (float)coef, (float*)out, nullptr, s);
   }
+  VD_HIP_TRY(hipGetLastError());
+  return VD_OK;
+}
+
+int vd_qg_prep(vd_plan* h, const void* h0, const void* hb, void* hh, void* q, void* stream) {
+  VD_REQUIRE(h && h0 && hh && q, VD_ERR_INVALID_ARG, "null argument");
+  VD_REQUIRE(hh != h0 && q != h0, VD_ERR_INVALID_ARG,
+             "h and q must not alias h0 (q = h2pv(h0) reads h0 at neighbouring points)");
+  Plan& p = h->p;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (p.prec == VD_F64)
+    launch_prep<double>(p, (const double*)h0, hb ? (const double*)hb : fld<double>(p, F_ZERO),
+                        (double*)hh, fld<double>(p, F_HBM), (double*)q, fld<double>(p, F_QB), s);
+  else
+    launch_prep<float>(p, (const float*)h0, hb ? (const float*)hb : fld<float>(p, F_ZERO),
+                       (float*)hh, fld<float>(p, F_HBM), (float*)q, fld<float>(p, F_QB), s);
+  VD_HIP_TRY(hipGetLastError());
+  return VD_OK;
+}
+
+int vd_qg_finish(vd_plan* h, const void* hh, void* h1, void* stream) {
+  VD_REQUIRE(h && hh && h1, VD_ERR_INVALID_ARG, "null argument");
+  Plan& p = h->p;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (p.prec == VD_F64)
+    launch_finish<double>(p, (const double*)hh, (double*)h1, s);
+  else
+    launch_finish<float>(p, (const float*)hh, (float*)h1, s);
+  VD_HIP_TRY(hipGetLastError());
+  return VD_OK;
+}
+
+int vd_qg_adj_init(vd_plan* h, const void* adh1, void* lh, void* lq, void* stream) {
+  VD_REQUIRE(h && adh1 && lh && lq, VD_ERR_INVALID_ARG, "null argument");
+  Plan& p = h->p;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (p.prec == VD_F64)
+    launch_adj_init<double>(p, (const double*)adh1, (double*)lh, (double*)lq, s);
+  else
+    launch_adj_init<float>(p, (const float*)adh1, (float*)lh, (float*)lq, s);
+  VD_HIP_TRY(hipGetLastError());
+  return VD_OK;
+}
+
+int vd_qg_adj_final(vd_plan* h, const void* hcur, const void* lh, const void* lq, void* adh0,
+                    void* stream) {
+  VD_REQUIRE(h && hcur && lh && lq && adh0, VD_ERR_INVALID_ARG, "null argument");
+  VD_REQUIRE(adh0 != lq, VD_ERR_INVALID_ARG,
+             "adh0 must not alias lq (it is read at neighbouring points)");
+  Plan& p = h->p;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (p.prec == VD_F64)
+    launch_adj_final<double>(p, (const double*)hcur, (const double*)lh, (const double*)lq,
+                             (double*)adh0, s);
+  else
+    launch_adj_final<float>(p, (const float*)hcur, (const float*)lh, (const float*)lq,
+                            (float*)adh0, s);
   VD_HIP_TRY(hipGetLastError());
   return VD_OK;
 }

@@ -1,0 +1,117 @@
+// C ABI of ragged groups: independent QG problems on grids of their own,
+// advanced by shared launches over the members still running (see vardyn.h).
+#include "vd_internal.h"
+
+using namespace vd;
+
+int vd_group_create(vd_plan* const* plans, int n, vd_group** out) {
+  VD_REQUIRE(plans && out && n > 0, VD_ERR_INVALID_ARG, "null argument or empty group");
+  *out = nullptr;
+  for (int b = 0; b < n; ++b) {
+    VD_REQUIRE(plans[b], VD_ERR_INVALID_ARG, "null plan in group");
+    const Plan& p = plans[b]->p;
+    const Plan& p0 = plans[0]->p;
+    VD_REQUIRE(p.nb == 1, VD_ERR_INVALID_ARG, "group members must be single-problem plans");
+    VD_REQUIRE(p.prec == p0.prec && p.scheme == p0.scheme && p.fold == p0.fold &&
+                   p.device == p0.device,
+               VD_ERR_INVALID_ARG,
+               "group members need the same precision, time scheme, sine-transform folding "
+               "and device");
+  }
+  vd_group* grp = new vd_group();
+  Group& g = grp->g;
+  g.n = n;
+  g.prec = plans[0]->p.prec;
+  g.scheme = plans[0]->p.scheme;
+  g.nstage = plans[0]->p.nstage;
+  g.fold = plans[0]->p.fold;
+  for (int b = 0; b < n; ++b) g.plans.push_back(&plans[b]->p);
+  const int rc = g.prec == VD_F64 ? group_build<double>(g) : group_build<float>(g);
+  if (rc != VD_OK) {
+    group_release(g);
+    delete grp;
+    return rc;
+  }
+  *out = grp;
+  return VD_OK;
+}
+
+int vd_group_destroy(vd_group* grp) {
+  if (!grp) return VD_OK;
+  group_release(grp->g);
+  delete grp;
+  return VD_OK;
+}
+
+int vd_group_offsets(const vd_group* grp, size_t* offsets) {
+  VD_REQUIRE(grp && offsets, VD_ERR_INVALID_ARG, "null argument");
+  for (int b = 0; b <= grp->g.n; ++b) offsets[b] = grp->g.foff[b];
+  return VD_OK;
+}
+
+#define VD_GROUP_CHECK(grp, nact)                                                        \
+  VD_REQUIRE(grp, VD_ERR_INVALID_ARG, "null group");                                     \
+  VD_REQUIRE((nact) >= 0 && (nact) <= (grp)->g.n, VD_ERR_INVALID_ARG,                    \
+             "nact must be in 0..number of members")
+
+int vd_group_stage(vd_group* grp, int nact, const void* h, const void* q, const void* qbase,
+                   const void* k0, const void* k1, double c0, double c1, double c2, void* qout,
+                   void* kstore, void* stream) {
+  VD_GROUP_CHECK(grp, nact);
+  VD_REQUIRE(h && q && qbase && qout, VD_ERR_INVALID_ARG, "null argument");
+  VD_REQUIRE(qout != h && qout != q, VD_ERR_INVALID_ARG,
+             "qout must not alias h or q (the stencil reads them at neighbouring points)");
+  const Group& g = grp->g;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (g.prec == VD_F64)
+    launch_stage_rg<double>(g, nact, (const double*)h, (const double*)q, (const double*)qbase,
+                            (const double*)k0, (const double*)k1, c0, c1, c2, (double*)qout,
+                            (double*)kstore, s);
+  else
+    launch_stage_rg<float>(g, nact, (const float*)h, (const float*)q, (const float*)qbase,
+                           (const float*)k0, (const float*)k1, (float)c0, (float)c1, (float)c2,
+                           (float*)qout, (float*)kstore, s);
+  VD_HIP_TRY(hipGetLastError());
+  return VD_OK;
+}
+
+int vd_group_inversion(vd_group* grp, int nact, const void* F, const void* Fsub, const void* hb,
+                       void* out, int accumulate, void* stream) {
+  VD_GROUP_CHECK(grp, nact);
+  VD_REQUIRE(F && out, VD_ERR_INVALID_ARG, "null argument");
+  const Group& g = grp->g;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int mode = accumulate ? INV_ACC : INV_FWD;
+  if (g.prec == VD_F64)
+    launch_inversion_rg<double>(g, nact, (const double*)F, (const double*)Fsub,
+                                (const double*)hb, (double*)out, mode, s);
+  else
+    launch_inversion_rg<float>(g, nact, (const float*)F, (const float*)Fsub, (const float*)hb,
+                               (float*)out, mode, s);
+  VD_HIP_TRY(hipGetLastError());
+  return VD_OK;
+}
+
+int vd_group_stage_adj(vd_group* grp, int nact, const void* h, const void* q, const void* s0,
+                       const void* s1, const void* s2, double a0, double a1, double a2,
+                       const void* acc0, const void* acc1, const void* acc2, void* lq_out,
+                       void* lh_out, void* stream) {
+  VD_GROUP_CHECK(grp, nact);
+  VD_REQUIRE(h && q && s0 && lq_out && lh_out, VD_ERR_INVALID_ARG, "null argument");
+  VD_REQUIRE(lq_out != s0 && lq_out != s1 && lq_out != s2, VD_ERR_INVALID_ARG,
+             "lq_out must not alias s0..s2 (they are read at neighbouring points)");
+  const Group& g = grp->g;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (g.prec == VD_F64)
+    launch_stage_adj_rg<double>(g, nact, (const double*)h, (const double*)q, (const double*)s0,
+                                (const double*)s1, (const double*)s2, a0, a1, a2,
+                                (const double*)acc0, (const double*)acc1, (const double*)acc2,
+                                (double*)lq_out, (double*)lh_out, s);
+  else
+    launch_stage_adj_rg<float>(g, nact, (const float*)h, (const float*)q, (const float*)s0,
+                               (const float*)s1, (const float*)s2, (float)a0, (float)a1,
+                               (float)a2, (const float*)acc0, (const float*)acc1,
+                               (const float*)acc2, (float*)lq_out, (float*)lh_out, s);
+  VD_HIP_TRY(hipGetLastError());
+  return VD_OK;
+}

@@ -6,6 +6,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "../../include/vardyn.h"
 
@@ -39,10 +40,11 @@ constexpr int kTile = 32;  // smallest GEMM output tile and the K chunk
 struct TileCfg {
   int bm, bn, ks, kern;
 };
-constexpr int kNumTileCfg = 10;
+constexpr int kNumTileCfg = 11;
 inline constexpr TileCfg kTileCfgs[kNumTileCfg] = {
     {32, 32, 4, 0}, {64, 64, 1, 0}, {64, 64, 2, 0}, {128, 64, 1, 0}, {128, 64, 2, 0},
-    {32, 32, 4, 1}, {64, 64, 1, 1}, {64, 64, 2, 1}, {128, 64, 1, 1}, {64, 32, 2, 1}};
+    {32, 32, 4, 1}, {64, 64, 1, 1}, {64, 64, 2, 1}, {128, 64, 1, 1}, {64, 32, 2, 1},
+    {128, 128, 1, 0}};
 inline bool tile_cfg_fits(int c, int M, int N, int K) {
   return M % kTileCfgs[c].bm == 0 && N % kTileCfgs[c].bn == 0 &&
          (c == 0 || (K / kTile) % kTileCfgs[c].ks == 0);
@@ -103,6 +105,30 @@ struct Plan {
   void* gT[3] = {nullptr, nullptr, nullptr};
   void* fld[40] = {};
   int nfld = 0;
+};
+
+// A ragged group: independent problems on grids of their own (one plan each,
+// nbatch 1), advanced by shared launches.  Member fields are concatenated in
+// member order in every flat buffer; a launch covers the first `nact` members.
+template <typename T>
+struct RgMember {
+  Phys<T> P;
+  size_t foff;  // element offset of the member's fields
+};
+struct Group {
+  int n = 0, prec = 0, scheme = 0, nstage = 0, fold = 0;
+  std::vector<Plan*> plans;   // not owned
+  std::vector<size_t> foff;   // n + 1 prefix offsets
+  void* mem = nullptr;        // RgMember<T>[n]
+  int4* stage_tiles = nullptr;  // (member, tile row, tile column) of 64 x 4 point tiles
+  std::vector<int> stage_prefix;
+  int4* adj_tiles = nullptr;    // the same for 64 x 16 tiles of the adjoint stage
+  std::vector<int> adj_prefix;
+  void* gargs[4] = {nullptr, nullptr, nullptr, nullptr};  // RgGemm<T>[n] per inversion stage
+  // (member, product, tile) of the 32 x 32 GEMM tiles per stage and K-split
+  // class (4, 2, 1 K groups: configuration 0's choice for each member)
+  int4* gtiles[4][3] = {};
+  std::vector<int> gprefix[4][3];
 };
 
 // Named scratch fields inside Plan::fld.
@@ -166,6 +192,22 @@ template <typename T>
 void launch_stage_adj(const Plan& p, const T* h, const T* q, const T* s0, const T* s1,
                       const T* s2, T a0, T a1, T a2, const T* acc0, const T* acc1,
                       const T* acc2, T* lq_out, T* lh_out, hipStream_t s);
+// Ragged groups (vd_kernels.hip): build the device tables of a group whose
+// plans are set, free them, and the three per-step launches over the first
+// nact members.
+template <typename T> int group_build(Group& g);
+void group_release(Group& g);
+template <typename T>
+void launch_stage_rg(const Group& g, int nact, const T* h, const T* q, const T* qbase,
+                     const T* kprev0, const T* kprev1, T c0, T c1, T c2, T* qout, T* kstore,
+                     hipStream_t s);
+template <typename T>
+void launch_inversion_rg(const Group& g, int nact, const T* F, const T* Fsub, const T* hb, T* out,
+                         int mode, hipStream_t s);
+template <typename T>
+void launch_stage_adj_rg(const Group& g, int nact, const T* h, const T* q, const T* s0,
+                         const T* s1, const T* s2, T a0, T a1, T a2, const T* acc0,
+                         const T* acc1, const T* acc2, T* lq_out, T* lh_out, hipStream_t s);
 // lh = adh1 with zeros on land; lq = 0.
 template <typename T>
 void launch_adj_init(const Plan& p, const T* adh1, T* lh, T* lq, hipStream_t s);
@@ -175,3 +217,11 @@ void launch_adj_final(const Plan& p, const T* hcur, const T* lh, const T* lq, T*
                       hipStream_t s);
 
 }  // namespace vd
+
+// Opaque handles of the C ABI.
+struct vd_plan {
+  vd::Plan p;
+};
+struct vd_group {
+  vd::Group g;
+};

@@ -92,6 +92,22 @@ struct GemmArgs {
   int nz, tm, tn, nb;
 };
 
+// Ragged groups: per-call field pointers of a GEMM launch (member b's fields
+// start at its offset in every flat buffer), and the static operands of one
+// member for one inversion stage.
+template <typename T>
+struct RgCall {
+  const T* F;
+  const T* Fsub;
+  const T* hb;
+  T* out;
+};
+template <typename T>
+struct RgGemm {
+  GemmArgs<T> g;
+  size_t foff;
+};
+
 // Operand elements.  A(i, l): dense, or column-folded (sum/difference of
 // columns l and mx-1-l).  B(k, n): dense, the interior of field F (minus a
 // second field Fsub for the *_SUB kinds), or row-folded.  Indices past the
@@ -248,7 +264,7 @@ __device__ __forceinline__ void store_out(const GemmArgs<T>& g, const GemmPtrs<T
 // LDS-staged kernel: each K chunk of 32 is loaded by the whole group into
 // LDS (two chunks of register prefetch) and shared by its four waves.
 template <typename T, int AOP, int BOP, int KIND, int MAP, int BM, int BN, int KS>
-__global__ __launch_bounds__(256 * KS) void k_gemm(GemmArgs<T> g) {
+__device__ __forceinline__ void gemm_tile(const GemmArgs<T>& g, int z, int tile) {
   typedef typename Mfma<T>::acc_t acc_t;
   constexpr int LDT = kTile + 1;
   constexpr int MT = BM / 32, NT = BN / 32;          // 16x16 MFMA blocks per wave
@@ -263,8 +279,6 @@ __global__ __launch_bounds__(256 * KS) void k_gemm(GemmArgs<T> g) {
   const int lane = tid & 63;
   const int wq = (tid >> 6) & 3;     // quadrant of the tile owned by this wave
   const int wm = wq >> 1, wn = wq & 1;
-  int z, tile;
-  work_item(g, &z, &tile);
   const GemmPtrs<T> P = gemm_ptrs<T, AOP, BOP, MAP>(g, z);
   const int m0 = (tile / g.tn) * BM, n0 = (tile % g.tn) * BN;
   T* As = &sm[grp][0];               // [BM][LDT]
@@ -424,6 +438,29 @@ __global__ __launch_bounds__(256 * KS) void k_gemm(GemmArgs<T> g) {
       for (int r = 0; r < 4; ++r)
         store_out<T, KIND, MAP>(g, P, m0 + wm * (BM / 2) + a_ * 16 + Mfma<T>::out_row(lane, r),
                                 n0 + wn * (BN / 2) + b_ * 16 + (lane & 15), val[a_][b_][r]);
+}
+
+template <typename T, int AOP, int BOP, int KIND, int MAP, int BM, int BN, int KS>
+__global__ __launch_bounds__(256 * KS) void k_gemm(GemmArgs<T> g) {
+  int z, tile;
+  work_item(g, &z, &tile);
+  gemm_tile<T, AOP, BOP, KIND, MAP, BM, BN, KS>(g, z, tile);
+}
+
+// Ragged groups: workgroup b of the launch takes tile table entry b = (member,
+// product z, tile); the member's static operands come from its descriptor and
+// the per-call fields from `call` at the member's offset.
+template <typename T, int AOP, int BOP, int KIND, int MAP, int BM, int BN, int KS>
+__global__ __launch_bounds__(256 * KS) void k_gemm_rg(RgCall<T> call, const RgGemm<T>* __restrict__ rg,
+                                                      const int4* __restrict__ tiles) {
+  const int4 t = tiles[blockIdx.x];
+  GemmArgs<T> g = rg[t.x].g;
+  const size_t fo = rg[t.x].foff;
+  g.F = call.F ? call.F + fo : nullptr;
+  g.Fsub = call.Fsub ? call.Fsub + fo : nullptr;
+  g.hb = call.hb ? call.hb + fo : nullptr;
+  g.out = call.out ? call.out + fo : nullptr;
+  gemm_tile<T, AOP, BOP, KIND, MAP, BM, BN, KS>(g, t.y, t.z);
 }
 
 // Double-buffered kernel.  Per K chunk of 32 there is one barrier: while
@@ -622,87 +659,106 @@ static void gemm(GemmArgs<T> a, int M, int N, int nz, int cfg, hipStream_t s) {
     case 6: VD_G(k_gemm2<T, AOP, BOP, KIND, MAP, 64, 64, 1>); break;
     case 7: VD_G(k_gemm2<T, AOP, BOP, KIND, MAP, 64, 64, 2>); break;
     case 8: VD_G(k_gemm2<T, AOP, BOP, KIND, MAP, 128, 64, 1>); break;
-    default: VD_G(k_gemm2<T, AOP, BOP, KIND, MAP, 64, 32, 2>); break;
+    case 9: VD_G(k_gemm2<T, AOP, BOP, KIND, MAP, 64, 32, 2>); break;
+    default: VD_G(k_gemm<T, AOP, BOP, KIND, MAP, 128, 128, 1>); break;
   }
 #undef VD_G
 }
 
+// Static operands of inversion stage `stage` of plan p (the per-call field
+// pointers F, Fsub, hb, out are left null):
+//   0: T1 = S_y (F - Fsub)   1: T2 = (T1 S_x) ./ Lambda
+//   2: T1 = S_y T2           3: out = epilogue(T1 S_x)
 template <typename T>
-void launch_gemm_stage(const Plan& p, int stage, const T* F, const T* Fsub, const T* hb, T* out,
-                       int mode, int cfg, hipStream_t s) {
+GemmArgs<T> stage_args(const Plan& p, int stage) {
   T* T1 = static_cast<T*>(p.gT[0]);
   T* T2 = static_cast<T*>(p.gT[1]);
-  int cf[4] = {cfg, cfg, cfg, cfg};
   GemmArgs<T> a{};
   a.ny = p.ny; a.nx = p.nx; a.my = p.my; a.mx = p.mx; a.hy = p.hy; a.hx = p.hx;
   a.scale = static_cast<const T*>(p.lam_inv);
   const size_t NF = p.nf, NG = p.ng;
-  const int nb = p.nb;
-  a.nb = nb;
+  a.nb = p.nb;
   a.bsF = NF; a.bsOut = NF; a.bsScale = NG;
-  if (p.fold) {
-    const T* SyF = static_cast<const T*>(p.SyF);
-    const T* SxF = static_cast<const T*>(p.SxF);
+  const bool fold = p.fold;
+  const T* Ay = static_cast<const T*>(fold ? p.SyF : p.Sy);
+  const T* Bx = static_cast<const T*>(fold ? p.SxF : p.Sx);
+  const int ldy = fold ? p.HY : p.MY, ldx = fold ? p.HX : p.MX;
+  if (fold) {
     a.strideA = (size_t)p.HY * p.HY;
     a.strideB = (size_t)p.HX * p.HX;
-    if (stage == 0) {
-    // T1 = S_y (F - Fsub)
-    a.A = SyF; a.lda = p.HY; a.F = F; a.Fsub = Fsub; a.K = p.HY; a.C = T1; a.ldc = p.MX;
+  }
+  if (stage == 0) {
+    a.A = Ay; a.lda = ldy; a.K = ldy; a.C = T1; a.ldc = p.MX;
     a.bsA = 0; a.bsB = 0; a.bsC = NG;
-    if (Fsub)
-      gemm<T, AOP_DENSE, BOP_FOLDROW_FIELD_SUB, EK_STORE, MAP_ROWFOLD>(a, p.HY, p.MX, 2 * nb, cf[0], s);
-    else
-      gemm<T, AOP_DENSE, BOP_FOLDROW_FIELD, EK_STORE, MAP_ROWFOLD>(a, p.HY, p.MX, 2 * nb, cf[0], s);
-    } else if (stage == 1) {
-    // T2 = (T1 S_x) ./ Lambda
-    a.A = T1; a.lda = p.MX; a.B = SxF; a.ldb = p.HX; a.K = p.HX; a.C = T2; a.ldc = p.MX;
+  } else if (stage == 1) {
+    a.A = T1; a.lda = p.MX; a.B = Bx; a.ldb = ldx; a.K = ldx; a.C = T2; a.ldc = p.MX;
     a.bsA = NG; a.bsB = 0; a.bsC = NG;
-    gemm<T, AOP_FOLDCOL, BOP_DENSE, EK_SCALE, MAP_COLFOLD>(a, p.MY, p.HX, 2 * nb, cf[1], s);
-    } else if (stage == 2) {
-    // T1 = S_y T2
-    a.A = SyF; a.lda = p.HY; a.B = T2; a.ldb = p.MX; a.K = p.HY; a.C = T1; a.ldc = p.MX;
+  } else if (stage == 2) {
+    a.A = Ay; a.lda = ldy; a.B = T2; a.ldb = p.MX; a.K = ldy; a.C = T1; a.ldc = p.MX;
     a.bsA = 0; a.bsB = NG; a.bsC = NG;
-    gemm<T, AOP_DENSE, BOP_FOLDROW_DENSE, EK_STORE, MAP_ROWFOLD>(a, p.HY, p.MX, 2 * nb, cf[2], s);
-    } else {
-    // out = epilogue(T1 S_x)
-    a.A = T1; a.lda = p.MX; a.B = SxF; a.ldb = p.HX; a.K = p.HX; a.out = out; a.hb = hb;
+  } else {
+    a.A = T1; a.lda = p.MX; a.B = Bx; a.ldb = ldx; a.K = ldx;
     a.bsA = NG; a.bsB = 0; a.bsC = 0;
-    if (mode == INV_ACC)
-      gemm<T, AOP_FOLDCOL, BOP_DENSE, EK_FIELD_ACC, MAP_COLFOLD>(a, p.MY, p.HX, 2 * nb, cf[3], s);
-    else
-      gemm<T, AOP_FOLDCOL, BOP_DENSE, EK_FIELD_SET, MAP_COLFOLD>(a, p.MY, p.HX, 2 * nb, cf[3], s);
+  }
+  return a;
+}
+
+// Calls L::go<AOP, BOP, KIND, MAP>() with the operand and epilogue kinds of
+// inversion stage `stage` (fold or dense, subtracted field or not,
+// overwrite or accumulate).
+template <typename L>
+void dispatch_stage(bool fold, int stage, bool sub, int mode, L& l) {
+  const bool acc = mode == INV_ACC;
+  if (fold) {
+    if (stage == 0) {
+      if (sub) l.template go<AOP_DENSE, BOP_FOLDROW_FIELD_SUB, EK_STORE, MAP_ROWFOLD>();
+      else l.template go<AOP_DENSE, BOP_FOLDROW_FIELD, EK_STORE, MAP_ROWFOLD>();
+    } else if (stage == 1) {
+      l.template go<AOP_FOLDCOL, BOP_DENSE, EK_SCALE, MAP_COLFOLD>();
+    } else if (stage == 2) {
+      l.template go<AOP_DENSE, BOP_FOLDROW_DENSE, EK_STORE, MAP_ROWFOLD>();
+    } else if (acc) {
+      l.template go<AOP_FOLDCOL, BOP_DENSE, EK_FIELD_ACC, MAP_COLFOLD>();
+    } else {
+      l.template go<AOP_FOLDCOL, BOP_DENSE, EK_FIELD_SET, MAP_COLFOLD>();
     }
     return;
   }
-  const T* Sy = static_cast<const T*>(p.Sy);
-  const T* Sx = static_cast<const T*>(p.Sx);
   if (stage == 0) {
-  // T1 = S_y (F - Fsub)_interior
-  a.A = Sy; a.lda = p.MY; a.F = F; a.Fsub = Fsub; a.K = p.MY; a.C = T1; a.ldc = p.MX;
-  a.bsA = 0; a.bsB = 0; a.bsC = NG;
-  if (Fsub)
-    gemm<T, AOP_DENSE, BOP_FIELD_SUB, EK_STORE, MAP_PLAIN>(a, p.MY, p.MX, nb, cf[0], s);
-  else
-    gemm<T, AOP_DENSE, BOP_FIELD, EK_STORE, MAP_PLAIN>(a, p.MY, p.MX, nb, cf[0], s);
+    if (sub) l.template go<AOP_DENSE, BOP_FIELD_SUB, EK_STORE, MAP_PLAIN>();
+    else l.template go<AOP_DENSE, BOP_FIELD, EK_STORE, MAP_PLAIN>();
   } else if (stage == 1) {
-  // T2 = (T1 S_x) ./ Lambda
-  a.A = T1; a.lda = p.MX; a.B = Sx; a.ldb = p.MX; a.K = p.MX; a.C = T2; a.ldc = p.MX;
-  a.bsA = NG; a.bsB = 0; a.bsC = NG;
-  gemm<T, AOP_DENSE, BOP_DENSE, EK_SCALE, MAP_PLAIN>(a, p.MY, p.MX, nb, cf[1], s);
+    l.template go<AOP_DENSE, BOP_DENSE, EK_SCALE, MAP_PLAIN>();
   } else if (stage == 2) {
-  // T1 = S_y T2
-  a.A = Sy; a.lda = p.MY; a.B = T2; a.ldb = p.MX; a.K = p.MY; a.C = T1; a.ldc = p.MX;
-  a.bsA = 0; a.bsB = NG; a.bsC = NG;
-  gemm<T, AOP_DENSE, BOP_DENSE, EK_STORE, MAP_PLAIN>(a, p.MY, p.MX, nb, cf[2], s);
+    l.template go<AOP_DENSE, BOP_DENSE, EK_STORE, MAP_PLAIN>();
+  } else if (acc) {
+    l.template go<AOP_DENSE, BOP_DENSE, EK_FIELD_ACC, MAP_PLAIN>();
   } else {
-  // out = epilogue(T1 S_x)
-  a.A = T1; a.lda = p.MX; a.B = Sx; a.ldb = p.MX; a.K = p.MX; a.out = out; a.hb = hb;
-  a.bsA = NG; a.bsB = 0; a.bsC = 0;
-  if (mode == INV_ACC)
-    gemm<T, AOP_DENSE, BOP_DENSE, EK_FIELD_ACC, MAP_PLAIN>(a, p.MY, p.MX, nb, cf[3], s);
-  else
-    gemm<T, AOP_DENSE, BOP_DENSE, EK_FIELD_SET, MAP_PLAIN>(a, p.MY, p.MX, nb, cf[3], s);
+    l.template go<AOP_DENSE, BOP_DENSE, EK_FIELD_SET, MAP_PLAIN>();
   }
+}
+
+template <typename T>
+struct UniformLaunch {
+  GemmArgs<T> a;
+  int M, N, nz, cfg;
+  hipStream_t s;
+  template <int AOP, int BOP, int KIND, int MAP>
+  void go() { gemm<T, AOP, BOP, KIND, MAP>(a, M, N, nz, cfg, s); }
+};
+
+template <typename T>
+void launch_gemm_stage(const Plan& p, int stage, const T* F, const T* Fsub, const T* hb, T* out,
+                       int mode, int cfg, hipStream_t s) {
+  UniformLaunch<T> l;
+  l.a = stage_args<T>(p, stage);
+  if (stage == 0) { l.a.F = F; l.a.Fsub = Fsub; }
+  if (stage == 3) { l.a.out = out; l.a.hb = hb; }
+  int K;
+  gemm_stage_shape(p, stage, &l.M, &l.N, &K, &l.nz);
+  l.cfg = cfg;
+  l.s = s;
+  dispatch_stage(p.fold != 0, stage, Fsub != nullptr, mode, l);
 }
 
 template <typename T>
@@ -892,6 +948,21 @@ __global__ void k_finish(Phys<T> P, const T* __restrict__ h, const T* __restrict
 // then selects rhs or zero.  (Several rows per thread were measured and gave
 // no gain.)
 template <typename T>
+__device__ __forceinline__ void stage_point(const Phys<T>& P, int i, int j, const T* h,
+                                            const T* q, const T* qbase, const T* k0,
+                                            const T* k1, T c0, T c1, T c2, T* qout, T* kstore) {
+  const int nx = P.nx;
+  const size_t o = (size_t)i * nx + j;
+  const T kk = rhs_at(h, q, min(max(i, 2), P.ny - 3), min(max(j, 2), nx - 3), P);
+  const T k = P.mask[o] == 3 ? kk : T(0);
+  if (kstore) kstore[o] = k;
+  T v = qbase[o];
+  if (k0) v += c0 * k0[o];
+  if (k1) v += c1 * k1[o];
+  qout[o] = v + c2 * k;
+}
+
+template <typename T>
 __global__ __launch_bounds__(256) void k_stage(Phys<T> P, const T* __restrict__ h,
                                                const T* __restrict__ q,
                                                const T* __restrict__ qbase,
@@ -900,13 +971,28 @@ __global__ __launch_bounds__(256) void k_stage(Phys<T> P, const T* __restrict__ 
                                                T* __restrict__ qout, T* __restrict__ kstore) {
   VD_POINT_2D
   VD_B(h); VD_B(q); VD_B(qbase); VD_B(k0); VD_B(k1); VD_B(qout); VD_B(kstore);
-  const T kk = rhs_at(h, q, min(max(i, 2), P.ny - 3), min(max(j, 2), nx - 3), P);
-  const T k = P.mask[o] == 3 ? kk : T(0);
-  if (kstore) kstore[o] = k;
-  T v = qbase[o];
-  if (k0) v += c0 * k0[o];
-  if (k1) v += c1 * k1[o];
-  qout[o] = v + c2 * k;
+  (void)o;
+  stage_point(P, i, j, h, q, qbase, k0, k1, c0, c1, c2, qout, kstore);
+}
+
+// Ragged groups: block b takes tile (member, tile row, tile column) of the
+// table; the member's constants and field offset come from its descriptor.
+template <typename T>
+__global__ __launch_bounds__(256) void k_stage_rg(const RgMember<T>* __restrict__ mem,
+                                                  const int4* __restrict__ tiles,
+                                                  const T* __restrict__ h,
+                                                  const T* __restrict__ q,
+                                                  const T* __restrict__ qbase,
+                                                  const T* __restrict__ k0,
+                                                  const T* __restrict__ k1, T c0, T c1, T c2,
+                                                  T* __restrict__ qout, T* __restrict__ kstore) {
+  const int4 t = tiles[blockIdx.x];
+  const Phys<T> P = mem[t.x].P;
+  const int i = t.y * 4 + threadIdx.y, j = t.z * 64 + threadIdx.x;
+  if (i >= P.ny || j >= P.nx) return;
+  const size_t zo = mem[t.x].foff;
+  VD_B(h); VD_B(q); VD_B(qbase); VD_B(k0); VD_B(k1); VD_B(qout); VD_B(kstore);
+  stage_point(P, i, j, h, q, qbase, k0, k1, c0, c1, c2, qout, kstore);
 }
 
 // ---------------------------------------------------------------------------
@@ -1044,10 +1130,10 @@ constexpr int kAX = 64, kAY = 16, kAH = 2;
 constexpr int kRW = kAX + 2 * kAH, kRH = kAY + 2 * kAH;
 
 template <typename T>
-__global__ __launch_bounds__(kAX * kAY) void k_adj_fused(
-    Phys<T> P, const T* __restrict__ h, const T* __restrict__ q, const T* __restrict__ s0,
-    const T* __restrict__ s1, const T* __restrict__ s2, T a0, T a1, T a2, const T* acc0,
-    const T* acc1, const T* acc2, T* lq_out, T* __restrict__ lh_out) {
+__device__ __forceinline__ void adj_tile(const Phys<T>& P, int i0, int j0, const T* h, const T* q,
+                                         const T* s0, const T* s1, const T* s2, T a0, T a1, T a2,
+                                         const T* acc0, const T* acc1, const T* acc2, T* lq_out,
+                                         T* lh_out) {
   // acc* may alias lq_out (in-place accumulation): each thread reads and
   // writes only its own point of those fields.  s0..s2 are read at the halo
   // points of the tile, which other blocks write, so they must not alias
@@ -1056,10 +1142,6 @@ __global__ __launch_bounds__(kAX * kAY) void k_adj_fused(
   __shared__ uint8_t okU[kRH][kRW], okV[kRH][kRW];
   const int ny = P.ny, nx = P.nx;
   const int tid = threadIdx.y * kAX + threadIdx.x;
-  const int j0 = blockIdx.x * kAX, i0 = blockIdx.y * kAY;
-  const size_t zo = (size_t)blockIdx.z * P.nf;
-  VD_B(h); VD_B(q); VD_B(s0); VD_B(s1); VD_B(s2); VD_B(acc0); VD_B(acc1); VD_B(acc2);
-  VD_B(lq_out); VD_B(lh_out);
   for (int r = tid; r < kRH * kRW; r += kAX * kAY) {
     const int li = r / kRW, lj = r - li * kRW;
     const int i = i0 - kAH + li, j = j0 - kAH + lj;
@@ -1104,6 +1186,33 @@ __global__ __launch_bounds__(kAX * kAY) void k_adj_fused(
   };
   lh_out[o] = P.cu * (lu(li - 1, lj + 1) + lu(li - 1, lj) - lu(li + 1, lj) - lu(li + 1, lj + 1)) +
               P.cv * (lv(li, lj - 1) + lv(li + 1, lj - 1) - lv(li + 1, lj + 1) - lv(li, lj + 1));
+}
+
+template <typename T>
+__global__ __launch_bounds__(kAX * kAY) void k_adj_fused(
+    Phys<T> P, const T* __restrict__ h, const T* __restrict__ q, const T* __restrict__ s0,
+    const T* __restrict__ s1, const T* __restrict__ s2, T a0, T a1, T a2, const T* acc0,
+    const T* acc1, const T* acc2, T* lq_out, T* __restrict__ lh_out) {
+  const size_t zo = (size_t)blockIdx.z * P.nf;
+  VD_B(h); VD_B(q); VD_B(s0); VD_B(s1); VD_B(s2); VD_B(acc0); VD_B(acc1); VD_B(acc2);
+  VD_B(lq_out); VD_B(lh_out);
+  adj_tile(P, blockIdx.y * kAY, blockIdx.x * kAX, h, q, s0, s1, s2, a0, a1, a2, acc0, acc1, acc2,
+           lq_out, lh_out);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kAX * kAY) void k_adj_fused_rg(
+    const RgMember<T>* __restrict__ mem, const int4* __restrict__ tiles, const T* __restrict__ h,
+    const T* __restrict__ q, const T* __restrict__ s0, const T* __restrict__ s1,
+    const T* __restrict__ s2, T a0, T a1, T a2, const T* acc0, const T* acc1, const T* acc2,
+    T* lq_out, T* __restrict__ lh_out) {
+  const int4 t = tiles[blockIdx.x];
+  const Phys<T> P = mem[t.x].P;
+  const size_t zo = mem[t.x].foff;
+  VD_B(h); VD_B(q); VD_B(s0); VD_B(s1); VD_B(s2); VD_B(acc0); VD_B(acc1); VD_B(acc2);
+  VD_B(lq_out); VD_B(lh_out);
+  adj_tile(P, t.y * kAY, t.z * kAX, h, q, s0, s1, s2, a0, a1, a2, acc0, acc1, acc2, lq_out,
+           lh_out);
 }
 
 
@@ -1239,6 +1348,144 @@ void launch_adj_final(const Plan& p, const T* hcur, const T* lh, const T* lq, T*
   hipLaunchKernelGGL(k_adj_final<T>, grid2d(p.ny, p.nx, p.nb), kBlock2d, 0, s, P, hcur, lh, lq, adh0);
 }
 
+// ---------------------------------------------------------------------------
+// Ragged groups
+// ---------------------------------------------------------------------------
+template <typename T>
+struct RaggedLaunch {
+  RgCall<T> call;
+  const RgGemm<T>* rg;
+  const int4* tiles;
+  int ntiles, ks;
+  hipStream_t s;
+  template <int AOP, int BOP, int KIND, int MAP>
+  void go() {
+    const dim3 grid(ntiles), block(256 * ks);
+    if (ks == 4)
+      hipLaunchKernelGGL((k_gemm_rg<T, AOP, BOP, KIND, MAP, 32, 32, 4>), grid, block, 0, s, call, rg, tiles);
+    else if (ks == 2)
+      hipLaunchKernelGGL((k_gemm_rg<T, AOP, BOP, KIND, MAP, 32, 32, 2>), grid, block, 0, s, call, rg, tiles);
+    else
+      hipLaunchKernelGGL((k_gemm_rg<T, AOP, BOP, KIND, MAP, 32, 32, 1>), grid, block, 0, s, call, rg, tiles);
+  }
+};
+
+template <typename T>
+void launch_stage_rg(const Group& g, int nact, const T* h, const T* q, const T* qbase,
+                     const T* kprev0, const T* kprev1, T c0, T c1, T c2, T* qout, T* kstore,
+                     hipStream_t s) {
+  const int nb = g.stage_prefix[nact];
+  if (nb == 0) return;
+  hipLaunchKernelGGL(k_stage_rg<T>, dim3(nb), kBlock2d, 0, s, static_cast<const RgMember<T>*>(g.mem),
+                     g.stage_tiles, h, q, qbase, kprev0, kprev1, c0, c1, c2, qout, kstore);
+}
+
+template <typename T>
+void launch_stage_adj_rg(const Group& g, int nact, const T* h, const T* q, const T* s0,
+                         const T* s1, const T* s2, T a0, T a1, T a2, const T* acc0,
+                         const T* acc1, const T* acc2, T* lq_out, T* lh_out, hipStream_t s) {
+  const int nb = g.adj_prefix[nact];
+  if (nb == 0) return;
+  hipLaunchKernelGGL(k_adj_fused_rg<T>, dim3(nb), dim3(kAX, kAY), 0, s,
+                     static_cast<const RgMember<T>*>(g.mem), g.adj_tiles, h, q, s0, s1, s2, a0, a1,
+                     a2, acc0, acc1, acc2, lq_out, lh_out);
+}
+
+template <typename T>
+void launch_inversion_rg(const Group& g, int nact, const T* F, const T* Fsub, const T* hb, T* out,
+                         int mode, hipStream_t s) {
+  static const int kKs[3] = {4, 2, 1};
+  for (int st = 0; st < 4; ++st) {
+    for (int c = 0; c < 3; ++c) {
+      RaggedLaunch<T> l;
+      l.call = RgCall<T>{st == 0 ? F : nullptr, st == 0 ? Fsub : nullptr, st == 3 ? hb : nullptr,
+                         st == 3 ? out : nullptr};
+      l.rg = static_cast<const RgGemm<T>*>(g.gargs[st]);
+      l.tiles = g.gtiles[st][c];
+      l.ntiles = g.gprefix[st][c][nact];
+      l.ks = kKs[c];
+      l.s = s;
+      if (l.ntiles > 0) dispatch_stage(g.fold != 0, st, Fsub != nullptr, mode, l);
+    }
+  }
+}
+
+template <typename V>
+static int upload_vec(const std::vector<V>& v, void** d) {
+  VD_HIP_TRY(hipMalloc(d, std::max<size_t>(1, v.size()) * sizeof(V)));
+  if (!v.empty()) VD_HIP_TRY(hipMemcpy(*d, v.data(), v.size() * sizeof(V), hipMemcpyHostToDevice));
+  return VD_OK;
+}
+
+template <typename T>
+int group_build(Group& g) {
+  const int n = g.n;
+  g.foff.assign(n + 1, 0);
+  std::vector<RgMember<T>> mem(n);
+  std::vector<int4> st, at;
+  g.stage_prefix.assign(n + 1, 0);
+  g.adj_prefix.assign(n + 1, 0);
+  for (int b = 0; b < n; ++b) {
+    const Plan& p = *g.plans[b];
+    g.foff[b + 1] = g.foff[b] + p.nf;
+    mem[b].P = make_phys<T>(p);
+    mem[b].foff = g.foff[b];
+    for (int ty = 0; ty < (p.ny + 3) / 4; ++ty)
+      for (int tx = 0; tx < (p.nx + 63) / 64; ++tx) st.push_back(make_int4(b, ty, tx, 0));
+    for (int ty = 0; ty < (p.ny + kAY - 1) / kAY; ++ty)
+      for (int tx = 0; tx < (p.nx + kAX - 1) / kAX; ++tx) at.push_back(make_int4(b, ty, tx, 0));
+    g.stage_prefix[b + 1] = (int)st.size();
+    g.adj_prefix[b + 1] = (int)at.size();
+  }
+  int rc;
+  if ((rc = upload_vec(mem, &g.mem)) != VD_OK) return rc;
+  if ((rc = upload_vec(st, (void**)&g.stage_tiles)) != VD_OK) return rc;
+  if ((rc = upload_vec(at, (void**)&g.adj_tiles)) != VD_OK) return rc;
+  for (int stg = 0; stg < 4; ++stg) {
+    // 32 x 32 tiles with configuration 0's K-group count for each member (4, 2
+    // or 1 from its chunk count), so that every member computes exactly what
+    // its own plan computes; one tile table per K-split class
+    std::vector<RgGemm<T>> args(n);
+    std::vector<int4> tiles[3];
+    for (int c = 0; c < 3; ++c) g.gprefix[stg][c].assign(n + 1, 0);
+    for (int b = 0; b < n; ++b) {
+      const Plan& p = *g.plans[b];
+      int M, N, K, nz;
+      gemm_stage_shape(p, stg, &M, &N, &K, &nz);
+      const int nchunk = K / kTile;
+      const int c = nchunk % 4 == 0 ? 0 : (nchunk % 2 == 0 ? 1 : 2);
+      args[b].g = stage_args<T>(p, stg);
+      args[b].g.nz = nz;
+      args[b].g.tm = M / kTile;
+      args[b].g.tn = N / kTile;
+      args[b].foff = g.foff[b];
+      for (int z = 0; z < nz; ++z)
+        for (int t = 0; t < (M / kTile) * (N / kTile); ++t) tiles[c].push_back(make_int4(b, z, t, 0));
+      for (int cc = 0; cc < 3; ++cc) g.gprefix[stg][cc][b + 1] = (int)tiles[cc].size();
+    }
+    if ((rc = upload_vec(args, &g.gargs[stg])) != VD_OK) return rc;
+    for (int c = 0; c < 3; ++c)
+      if ((rc = upload_vec(tiles[c], (void**)&g.gtiles[stg][c])) != VD_OK) return rc;
+  }
+  return VD_OK;
+}
+
+void group_release(Group& g) {
+  hipFree(g.mem);
+  hipFree(g.stage_tiles);
+  hipFree(g.adj_tiles);
+  for (int st = 0; st < 4; ++st) {
+    hipFree(g.gargs[st]);
+    g.gargs[st] = nullptr;
+    for (int c = 0; c < 3; ++c) {
+      hipFree(g.gtiles[st][c]);
+      g.gtiles[st][c] = nullptr;
+    }
+  }
+  g.mem = nullptr;
+  g.stage_tiles = g.adj_tiles = nullptr;
+}
+
 #define VD_INSTANTIATE(T)                                                                     \
   template Phys<T> make_phys<T>(const Plan&);                                                \
   template void launch_inversion<T>(const Plan&, const T*, const T*, const T*, T*, int,       \
@@ -1256,6 +1503,14 @@ void launch_adj_final(const Plan& p, const T* hcur, const T* lh, const T* lq, T*
                                    hipStream_t);                                              \
   template void launch_prep_tl<T>(const Plan&, const T*, const T*, T*, T*, hipStream_t);      \
   template void launch_finish_tl<T>(const Plan&, const T*, T*, hipStream_t);                  \
+  template int group_build<T>(Group&);                                                       \
+  template void launch_stage_rg<T>(const Group&, int, const T*, const T*, const T*, const T*,  \
+                                   const T*, T, T, T, T*, T*, hipStream_t);                  \
+  template void launch_inversion_rg<T>(const Group&, int, const T*, const T*, const T*, T*,    \
+                                       int, hipStream_t);                                    \
+  template void launch_stage_adj_rg<T>(const Group&, int, const T*, const T*, const T*,       \
+                                       const T*, const T*, T, T, T, const T*, const T*,       \
+                                       const T*, T*, T*, hipStream_t);                       \
   template void launch_stage_adj<T>(const Plan&, const T*, const T*, const T*, const T*,      \
                                     const T*, T, T, T, const T*, const T*, const T*, T*, T*,  \
                                     hipStream_t);                                             \
