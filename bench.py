@@ -35,7 +35,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "fwd+adjoint 4D-Var gradients/sec, QG 512² 30-day window, grad rel.err≤1e-5"
 HBM_PEAK_GBS = 8000.0
-MFMA_PEAK = {"f64": 78.6, "f32": 157.3}  # dense TFLOP/s, MI355X
+MFMA_PEAK = {"f64": 78.6, "f32": 157.3, "i8": 5000.0}  # dense TFLOP/s (TOP/s for int8), MI355X
 
 
 def log(msg):
@@ -72,6 +72,10 @@ def parse():
                     help="first N problems of the problem set (profiling)")
     ap.add_argument("--streams", type=int, default=4,
                     help="HIP streams the problem set is spread over")
+    ap.add_argument("--dst-split", type=int, default=None,
+                    help="sine-matrix GEMMs as split products on int8 MFMA with this many "
+                         "digits per FP64 operand (0 = FP64 MFMA); sets VARDYN_DST_SPLIT for "
+                         "every plan of the run")
     ap.add_argument("--check-f32", action="store_true",
                     help="also evaluate the FP32 path and report its gradient error vs FP64")
     return ap.parse_args()
@@ -89,7 +93,13 @@ def gemm_roofline(plan, precision, n_inv=100, reps=5):
     the MFMA utilisation on the flops actually issued.  ``traffic`` (bytes per
     launch moved below L2) comes from the committed counter runs of this
     configuration (FETCH_SIZE / WRITE_SIZE, calibrated on a copy of known size),
-    when there is one."""
+    when there is one.
+
+    Split-product plans (``dst_split`` digits) run the products on int8 MFMA:
+    ``achieved`` is then the int8 work the kernels issue (digit pairs x 2 M N
+    K over the padded product shapes) against the int8 peak, and
+    ``fp64_equivalent_tflops`` the algorithmic FP64 flops per second the
+    inversion delivers at FP64-class accuracy."""
     import torch
     q = torch.randn(plan.shape, dtype=plan.dtype, device=plan.device) * 1e-6
     hb = torch.zeros_like(q)
@@ -115,14 +125,41 @@ def gemm_roofline(plan, precision, n_inv=100, reps=5):
     alg_flops_launch = my * mx * (my + mx) * plan.nb
     exec_flops_launch = alg_flops_launch / 2 if plan.folded else alg_flops_launch
     per_launch_s = ms * 1e-3 / (4 * n_inv * reps)
-    peak = MFMA_PEAK["f64" if precision == "float64" else "f32"]
-    achieved = alg_flops_launch / per_launch_s / 1e12
-    executed = exec_flops_launch / per_launch_s / 1e12
+    split = getattr(plan, "dst_split", 0)
     traffic, src = None, None
-    rec = os.path.join(ROOT, "profiles", f"gemm_traffic_qg{plan.ny}_b{plan.nb}.json")
+    rec = os.path.join(ROOT, "profiles", f"gemm_traffic_qg{plan.ny}_b{plan.nb}"
+                                         f"{f'_split{split}' if split else ''}.json")
     if precision == "float64" and os.path.exists(rec):
         with open(rec) as f:
             traffic, src = json.load(f)["traffic_bytes_per_launch"], os.path.relpath(rec, ROOT)
+    if split:
+        # int8 MFMA work of the split products: digit pairs x 2 M N K over the
+        # zero-padded product shapes (half sizes when folded), averaged over the
+        # four stages of an inversion
+        def rup(v):
+            return (v + 31) // 32 * 32
+        pairs = split * (split + 1) // 2
+        if plan.folded:
+            HY, HX, MY, MX = rup(my // 2), rup(mx // 2), rup(my), rup(mx)
+            macs, nz = [HY * MX * HY, MY * HX * HX] * 2, 2 * plan.nb
+        else:
+            MY, MX = rup(my), rup(mx)
+            macs, nz = [MY * MX * MY, MY * MX * MX] * 2, plan.nb
+        i8_ops = 2 * pairs * nz * sum(macs) / 4
+        achieved = i8_ops / per_launch_s / 1e12
+        fp64_eq = alg_flops_launch / per_launch_s / 1e12
+        return {"bound": "mfma", "kernel": "k_gemm_split (sine-matrix GEMMs as int8 split products "
+                                           "of FP64 operands, 4 launches per inversion)",
+                "achieved": round(achieved, 3), "peak": MFMA_PEAK["i8"], "unit": "TOP/s",
+                "frac": round(achieved / MFMA_PEAK["i8"], 4), "traffic": traffic,
+                "traffic_source": src, "avg_launch_us": round(per_launch_s * 1e6, 3),
+                "int8_ops_per_launch": i8_ops, "digits": split, "digit_pairs": pairs,
+                "algorithmic_flops_per_launch": alg_flops_launch, "folded_dst": plan.folded,
+                "fp64_equivalent_tflops": round(fp64_eq, 3),
+                "fp64_equivalent_over_fp64_mfma_peak": round(fp64_eq / MFMA_PEAK["f64"], 4)}
+    peak = MFMA_PEAK["f64" if precision == "float64" else "f32"]
+    achieved = alg_flops_launch / per_launch_s / 1e12
+    executed = exec_flops_launch / per_launch_s / 1e12
     return {"bound": "mfma", "kernel": "k_gemm / k_gemm2 (sine-matrix GEMMs of the Helmholtz "
                                        "inversion, 4 launches per inversion)",
             "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
@@ -268,6 +305,28 @@ def measure(pb, args, label):
     return r
 
 
+def split_check(case, pb1, precision, n_steps):
+    """Gradient and cost of a split-product problem against the same problem
+    (same observations) with FP64-MFMA sine GEMMs, over the whole window, at
+    the background state."""
+    import torch
+    from vardyn_amd.problem import build_problem
+    X0 = torch.zeros(pb1.engine.nctrl, dtype=torch.float64, device=pb1.plan.device)
+    Js, Gs = pb1.engine.run(X0)
+    Js, Gs = Js.clone(), Gs.clone()
+    pb64 = build_problem(case, precision=precision, n_steps=n_steps, control="ic", tape="full",
+                         truth_plan=pb1.plan, dst_split=0)
+    J64, G64 = pb64.engine.run(X0)
+    torch.cuda.synchronize()
+    r = {"grad_rel_err_vs_fp64_mfma": float((torch.linalg.vector_norm(Gs - G64) /
+                                             torch.linalg.vector_norm(G64)).item()),
+         "J_rel_err_vs_fp64_mfma": abs(Js[0].item() - J64[0].item()) / abs(J64[0].item()),
+         "window_steps": n_steps, "digits": pb1.plan.dst_split}
+    del pb64
+    release(None)
+    return r
+
+
 def problem_set(args):
     """Independent problems of uneven grid sizes and windows (48 by default).
 
@@ -341,6 +400,8 @@ def release(pb):
 
 def main():
     args = parse()
+    if args.dst_split is not None:
+        os.environ["VARDYN_DST_SPLIT"] = str(args.dst_split)
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
               f"(independent problems shard across GPUs with no communication)", file=sys.stderr)
@@ -367,6 +428,9 @@ def main():
         pb1 = build_problem(case, precision=args.precision, n_steps=n_steps, control="ic",
                             tape="full" if args.tape == "auto" else args.tape)
         plain = measure(pb1, args, "plain")
+        if pb1.plan.dst_split:
+            plain["split_check"] = split_check(case, pb1, args.precision, n_steps)
+            log(f"split check: {plain['split_check']}")
         if nb == 1:
             pb, head = pb1, plain
         else:
@@ -393,12 +457,14 @@ def main():
                    "obs_per_member": pb.H.n_total, "segments": len(pb.checkpoints) - 1,
                    "control": "initial SSH", "tape": head["tape"], "batch": nb,
                    "hip_graph": not args.no_graph, "folded_dst": pb.plan.folded,
-                   "gemm_tiles": head["tiles"]},
+                   "dst_split": pb.plan.dst_split, "gemm_tiles": head["tiles"]},
         "grid_point_steps_per_s": round(case.ny * case.nx * n_steps * value, 1),
         "ms_per_gradient": round(ms / nb, 2),
         "J": head["J0"], "grad_norm": head["gnorm0"],
         "roofline": roof,
     }
+    if plain is not None and "split_check" in plain:
+        out["split_check"] = plain["split_check"]
     if plain is not None and nb > 1:
         out["plain_path"] = {"value": round(plain["value"], 4), "unit": "gradients/s",
                              "ms_per_gradient": round(plain["ms_per_step"], 2),

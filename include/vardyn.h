@@ -86,6 +86,15 @@ typedef struct vd_qg_params {
                             and inversion calls is an (nbatch, ny, nx) array */
   const double* c_batch; /* (nbatch) gravity-wave speed per member, or NULL
                             for c everywhere */
+  int dst_split;         /* 0: sine-matrix GEMMs in the plan precision on
+                            FP64/FP32 MFMA; 5, 6 or 7 (FP64 plans): split
+                            products on int8 MFMA, each FP64 operand written
+                            as that many base-256 digits per line (Ozaki
+                            scheme), relative GEMM error about 2^(5-8n);
+                            interior sizes up to 1024 (folded) or 512; with
+                            it, gemm_config k in 1..4 forces split tile
+                            configuration k-1 (sine x data lines per
+                            workgroup: 128x32, 64x64, 64x32, 32x32) */
 } vd_qg_params;
 
 typedef struct vd_plan vd_plan;

@@ -79,3 +79,13 @@ def test_error_paths_without_device():
     assert b"gemm_config" in lib.vd_last_error()
     p.gemm_config, p.nbatch = 0, -2
     assert lib.vd_qg_plan_create(C.byref(p), C.byref(h)) == -1
+    # split-product GEMMs: 5..7 digits, FP64 plans only, their own tile range
+    p.nbatch, p.dst_split = 1, 4
+    assert lib.vd_qg_plan_create(C.byref(p), C.byref(h)) == -1
+    assert b"dst_split" in lib.vd_last_error()
+    p.dst_split, p.precision = 6, 0
+    assert lib.vd_qg_plan_create(C.byref(p), C.byref(h)) == -1
+    assert b"FP64" in lib.vd_last_error()
+    p.precision, p.gemm_config = 1, 5
+    assert lib.vd_qg_plan_create(C.byref(p), C.byref(h)) == -1
+    assert b"0..4" in lib.vd_last_error()

@@ -23,7 +23,7 @@ def rel(a, b):
 
 
 def make_pair(ny, nx, scheme, precision, land=False, beta=False, kdiff=False, mdt=False,
-              bathy=False, seed=0, dst_dense=False):
+              bathy=False, seed=0, dst_dense=False, dst_split=0):
     rng = np.random.default_rng(seed)
     dx, dy, dt, c = 10e3, 12e3, 1200.0, 2.7
     ssh = None
@@ -43,7 +43,7 @@ def make_pair(ny, nx, scheme, precision, land=False, beta=False, kdiff=False, md
     fb = None if b is None else float(orc.f) * b
     plan = QgPlan(ny, nx, dx, dy, dt, c, float(orc.f), time_scheme=scheme, precision=precision,
                   kdiffus=K, mask=build_mask(ny, nx, ssh), beta=orc.beta, fbathy=fb, mdt=m,
-                  dst_dense=dst_dense)
+                  dst_dense=dst_dense, dst_split=dst_split)
     h0 = 0.1 * rng.standard_normal((ny, nx))
     if land:
         h0[np.isnan(ssh)] = np.nan

@@ -31,6 +31,7 @@ class QgParams(C.Structure):
         ("mask", C.c_void_p), ("beta", C.c_void_p), ("fbathy", C.c_void_p), ("mdt", C.c_void_p),
         ("dst_dense", C.c_int), ("gemm_config", C.c_int),
         ("nbatch", C.c_int), ("c_batch", C.c_void_p),
+        ("dst_split", C.c_int),
     ]
 
 

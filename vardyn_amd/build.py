@@ -29,13 +29,20 @@ COMMON = [
 ]
 
 
+# Per-source flags.  The split-product GEMMs keep up to 112 int32 accumulators
+# per lane across their K loop; with them in VGPRs (the MFMA's VGPR form on
+# gfx950) the compiler no longer copies every accumulator between AGPRs and
+# VGPRs around each loop iteration.
+EXTRA = {"vd_split.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
+
+
 def sources():
     return sorted(CSRC.glob("*.hip"))
 
 
 def _compile(src: Path) -> Path:
     obj = BUILD / (src.stem + ".o")
-    cmd = [HIPCC, *COMMON, "-c", str(src), "-o", str(obj)]
+    cmd = [HIPCC, *COMMON, *EXTRA.get(src.name, []), "-c", str(src), "-o", str(obj)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src.name}:\n{r.stderr}")

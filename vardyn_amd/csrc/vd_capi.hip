@@ -5,6 +5,7 @@
 
 #include <utility>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -33,8 +34,46 @@ static int upload(const std::vector<double>& h, void** dptr) {
   return VD_OK;
 }
 
+// Digit planes of a sine operand for the split-product GEMMs (vd_split.hip):
+// op(par, line, k) for par < npar, line < lines, k < K is written as ns
+// balanced base-256 digits per element, the bytes of (Y + C) ^ C with
+// Y = rint(x 2^(8 ns - e)), e = ilogb(max |line|) + 3 and C = 0x80...80 --
+// the split the kernel applies to its data operand.  Planes [par][ns]
+// (digit 1 first) of lines x K bytes each, stored in MFMA fragment order:
+// element (line, k) at ((line/32 * K/32 + k/32) * 64 + lane) * 16 + k%16 with
+// lane = line%32 + 32 (k%32 / 16), the 16 bytes lane `lane` of a wave loads
+// for its 32-line block and K step (lines and K are multiples of 32).
+// Exponents [par][lines].
+template <typename Op>
+static int upload_split(int npar, int lines, int K, int ns, Op op, void** d8, int** de) {
+  std::vector<int8_t> pl((size_t)npar * ns * lines * K, 0);
+  std::vector<int> ex((size_t)npar * lines, 0);
+  const unsigned long long C = 0x8080808080808080ull >> (8 * (8 - ns));
+  for (int par = 0; par < npar; ++par)
+    for (int ln = 0; ln < lines; ++ln) {
+      double m = 0.0;
+      for (int k = 0; k < K; ++k) m = std::max(m, std::fabs(op(par, ln, k)));
+      if (m < 0x1p-900) continue;  // zero line: zero digits
+      const int e = std::ilogb(m) + 3;
+      ex[(size_t)par * lines + ln] = e;
+      for (int k = 0; k < K; ++k) {
+        const long long Y = std::llrint(std::ldexp(op(par, ln, k), 8 * ns - e));
+        const unsigned long long Z = ((unsigned long long)Y + C) ^ C;
+        const size_t frag = ((size_t)(ln / 32) * (K / 32) + k / 32) * 64 + (ln % 32) + 32 * ((k % 32) / 16);
+        for (int d = 0; d < ns; ++d)
+          pl[((size_t)par * ns + d) * lines * K + frag * 16 + k % 16] = (int8_t)(uint8_t)(Z >> (8 * (ns - 1 - d)));
+      }
+    }
+  VD_HIP_TRY(hipMalloc(d8, pl.size()));
+  VD_HIP_TRY(hipMemcpy(*d8, pl.data(), pl.size(), hipMemcpyHostToDevice));
+  VD_HIP_TRY(hipMalloc((void**)de, ex.size() * sizeof(int)));
+  VD_HIP_TRY(hipMemcpy(*de, ex.data(), ex.size() * sizeof(int), hipMemcpyHostToDevice));
+  return VD_OK;
+}
+
 static void release_plan_buffers(Plan& p) {
   hipFree(p.SyF); hipFree(p.SxF); hipFree(p.gfc2_b);
+  hipFree(p.S8y); hipFree(p.S8x); hipFree(p.eSy); hipFree(p.eSx);
   hipFree(p.Sy); hipFree(p.Sx); hipFree(p.lam_inv); hipFree(p.mask);
   hipFree(p.beta); hipFree(p.fbathy); hipFree(p.mdt);
   for (auto& g : p.gT) hipFree(g);
@@ -57,12 +96,15 @@ static int build_constants(Plan& p, const vd_qg_params& prm) {
     return S;
   };
   int rc;
-  if ((rc = upload<T>(sine(p.my, p.MY), &p.Sy)) != VD_OK) return rc;
-  if ((rc = upload<T>(sine(p.mx, p.MX), &p.Sx)) != VD_OK) return rc;
+  const std::vector<double> SyP = sine(p.my, p.MY), SxP = sine(p.mx, p.MX);
+  if ((rc = upload<T>(SyP, &p.Sy)) != VD_OK) return rc;
+  if ((rc = upload<T>(SxP, &p.Sx)) != VD_OK) return rc;
+  std::vector<double> fy, fx;
   if (p.fold) {
     // SyF[p][r][k] = S_y[2r+p][k], SxF[p][l][c] = S_x[l][2c+p] for r, k, l, c < m/2
     const std::vector<double> Sy = sine(p.my, p.my), Sx = sine(p.mx, p.mx);
-    std::vector<double> fy(2 * (size_t)p.HY * p.HY, 0.0), fx(2 * (size_t)p.HX * p.HX, 0.0);
+    fy.assign(2 * (size_t)p.HY * p.HY, 0.0);
+    fx.assign(2 * (size_t)p.HX * p.HX, 0.0);
     for (int par = 0; par < 2; ++par) {
       for (int r = 0; r < p.hy; ++r)
         for (int k = 0; k < p.hy; ++k)
@@ -73,6 +115,25 @@ static int build_constants(Plan& p, const vd_qg_params& prm) {
     }
     if ((rc = upload<T>(fy, &p.SyF)) != VD_OK) return rc;
     if ((rc = upload<T>(fx, &p.SxF)) != VD_OK) return rc;
+  }
+  if (p.split_ns > 0) {
+    // the sine operand of the split GEMMs: rows of S_y (left stages) and
+    // columns of S_x (right stages), K contiguous
+    const int ns = p.split_ns;
+    if (p.fold) {
+      const int HY = p.HY, HX = p.HX;
+      rc = upload_split(2, HY, HY, ns, [&](int par, int r, int k) { return fy[((size_t)par * HY + r) * HY + k]; },
+                        &p.S8y, &p.eSy);
+      if (rc != VD_OK) return rc;
+      rc = upload_split(2, HX, HX, ns, [&](int par, int c, int l) { return fx[((size_t)par * HX + l) * HX + c]; },
+                        &p.S8x, &p.eSx);
+    } else {
+      const int MY = p.MY, MX = p.MX;
+      rc = upload_split(1, MY, MY, ns, [&](int, int j, int k) { return SyP[(size_t)j * MY + k]; }, &p.S8y, &p.eSy);
+      if (rc != VD_OK) return rc;
+      rc = upload_split(1, MX, MX, ns, [&](int, int j, int l) { return SxP[(size_t)l * MX + j]; }, &p.S8x, &p.eSx);
+    }
+    if (rc != VD_OK) return rc;
   }
   // 1/Lambda with Lambda = g/f [2(cos(pi(j+1)/(nx-1))-1)/dx^2 + 2(cos(pi(i+1)/(ny-1))-1)/dy^2] - g f/c^2
   // (jqgm.py:165-172), one table per batch member (c may differ); zero in the padding.
@@ -370,9 +431,10 @@ static int autotune(Plan& p, int reps, hipStream_t s) {
     int M, N, K, nz;
     gemm_stage_shape(p, st, &M, &N, &K, &nz);
     float best = 1e30f;
-    int best_cfg = 0;
-    for (int c = 0; c < kNumTileCfg; ++c) {
-      if (!tile_cfg_fits(c, M, N, K)) continue;
+    int best_cfg = p.split_ns > 0 ? split_default_cfg(p, st) : 0;
+    const int ncfg = p.split_ns > 0 ? split_num_cfgs() : kNumTileCfg;
+    for (int c = 0; c < ncfg; ++c) {
+      if (p.split_ns > 0 ? !split_cfg_fits(p, st, c) : !tile_cfg_fits(c, M, N, K)) continue;
       launch_gemm_stage<T>(p, st, F, nullptr, nullptr, out, INV_FWD, c, s);  // warm
       VD_HIP_TRY(hipEventRecord(e0, s));
       for (int r = 0; r < reps; ++r) launch_gemm_stage<T>(p, st, F, nullptr, nullptr, out, INV_FWD, c, s);
@@ -419,8 +481,13 @@ int vd_qg_plan_create(const vd_qg_params* prm, vd_plan** out) {
              VD_ERR_INVALID_ARG, "dx, dy, c, g must be positive and f0 non-zero");
   VD_REQUIRE(prm->nbatch >= 0 && prm->nbatch <= 4096, VD_ERR_INVALID_ARG,
              "nbatch must be in 0..4096");
-  VD_REQUIRE(prm->gemm_config >= 0 && prm->gemm_config <= kNumTileCfg, VD_ERR_INVALID_ARG,
-             "gemm_config must be in 0..11");
+  VD_REQUIRE(prm->dst_split == 0 || (prm->dst_split >= 5 && prm->dst_split <= 7), VD_ERR_INVALID_ARG,
+             "dst_split must be 0, 5, 6 or 7");
+  VD_REQUIRE(prm->dst_split == 0 || prm->precision == VD_F64, VD_ERR_INVALID_ARG,
+             "dst_split needs an FP64 plan");
+  VD_REQUIRE(prm->gemm_config >= 0 && prm->gemm_config <= (prm->dst_split ? split_num_cfgs() : kNumTileCfg),
+             VD_ERR_INVALID_ARG,
+             prm->dst_split ? "gemm_config must be in 0..4 with dst_split" : "gemm_config must be in 0..11");
   vd_plan* h = new vd_plan();
   Plan& p = h->p;
   p.ny = prm->ny; p.nx = prm->nx;
@@ -431,6 +498,16 @@ int vd_qg_plan_create(const vd_qg_params* prm, vd_plan** out) {
   const int forced_cfg = prm->gemm_config - 1;
   p.nb = prm->nbatch <= 0 ? 1 : prm->nbatch;
   p.HY = round_up(p.hy, kTile); p.HX = round_up(p.hx, kTile);
+  p.split_ns = prm->dst_split;
+  {
+    const int ky = p.fold ? p.HY : p.MY, kx = p.fold ? p.HX : p.MX;
+    if (p.split_ns > 0 && (ky > split_max_k() || kx > split_max_k())) {
+      set_error("dst_split supports sine products with K up to " + std::to_string(split_max_k()) +
+                " (interior sizes up to 1024 folded, 512 dense)");
+      delete h;
+      return VD_ERR_UNSUPPORTED;
+    }
+  }
   p.prec = prm->precision;
   p.scheme = prm->time_scheme;
   p.nstage = p.scheme == VD_EULER ? 1 : (p.scheme == VD_RK2 ? 2 : 3);
@@ -449,7 +526,8 @@ int vd_qg_plan_create(const vd_qg_params* prm, vd_plan** out) {
     return VD_ERR_HIP;
   }
   for (int st = 0; st < 4; ++st)
-    p.gemm_cfg[st] = forced_cfg >= 0 ? forced_cfg : default_tile_cfg(p, st);
+    p.gemm_cfg[st] = forced_cfg >= 0 ? forced_cfg
+                                     : (p.split_ns > 0 ? split_default_cfg(p, st) : default_tile_cfg(p, st));
   int rc = p.prec == VD_F64 ? build_constants<double>(p, *prm) : build_constants<float>(p, *prm);
   if (rc != VD_OK) {
     const std::string msg = g_last_error;
@@ -482,6 +560,11 @@ int vd_qg_plan_bytes(const vd_plan* h, size_t* bytes) {
   const Plan& p = h->p;
   size_t b = ((size_t)p.MY * p.MY + (size_t)p.MX * p.MX + p.nb * p.ng * 4) * p.es + p.nf +
              (size_t)F_COUNT * p.nb * p.nf * p.es;
+  if (p.split_ns > 0) {
+    const int npar = p.fold ? 2 : 1;
+    const size_t ly = p.fold ? p.HY : p.MY, lx = p.fold ? p.HX : p.MX;
+    b += (size_t)npar * (p.split_ns * (ly * ly + lx * lx) + (ly + lx) * sizeof(int));
+  }
   if (p.has_beta) b += p.nf * p.es;
   if (p.has_bathy) b += p.nf * p.es;
   if (p.has_mdt) b += p.nf * p.es;

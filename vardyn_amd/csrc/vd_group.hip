@@ -13,15 +13,16 @@ int vd_group_create(vd_plan* const* plans, int n, vd_group** out) {
     const Plan& p0 = plans[0]->p;
     VD_REQUIRE(p.nb == 1, VD_ERR_INVALID_ARG, "group members must be single-problem plans");
     VD_REQUIRE(p.prec == p0.prec && p.scheme == p0.scheme && p.fold == p0.fold &&
-                   p.device == p0.device,
+                   p.device == p0.device && p.split_ns == p0.split_ns,
                VD_ERR_INVALID_ARG,
-               "group members need the same precision, time scheme, sine-transform folding "
-               "and device");
+               "group members need the same precision, time scheme, sine-transform folding, "
+               "split digits and device");
   }
   vd_group* grp = new vd_group();
   Group& g = grp->g;
   g.n = n;
   g.prec = plans[0]->p.prec;
+  g.split_ns = plans[0]->p.split_ns;
   g.scheme = plans[0]->p.scheme;
   g.nstage = plans[0]->p.nstage;
   g.fold = plans[0]->p.fold;

@@ -91,6 +91,15 @@ struct Plan {
   void* SyF = nullptr;  // [2][HY][HY]: rows 2r+p of S_y, columns 0..hy-1
   void* SxF = nullptr;  // [2][HX][HX]: SxF[p][l][c] = S_x[l][2c+p]
 
+  // split-product GEMMs on int8 MFMA (0 = off): digits per operand line,
+  // digit planes [par][split_ns][lines][K] and line exponents [par][lines] of
+  // the sine operand of the left (S_y) and right (S_x) stages
+  int split_ns = 0;
+  void* S8y = nullptr;
+  void* S8x = nullptr;
+  int* eSy = nullptr;
+  int* eSx = nullptr;
+
   // constants
   void* Sy = nullptr;
   void* Sx = nullptr;
@@ -129,6 +138,12 @@ struct Group {
   // class (4, 2, 1 K groups: configuration 0's choice for each member)
   int4* gtiles[4][3] = {};
   std::vector<int> gprefix[4][3];
+  // split-product groups (split_ns > 0): per-stage member descriptors and
+  // (member, product, 32 x 32 tile) tables
+  int split_ns = 0;
+  void* sargs[4] = {nullptr, nullptr, nullptr, nullptr};
+  int4* stiles[4] = {nullptr, nullptr, nullptr, nullptr};
+  std::vector<int> sprefix[4];
 };
 
 // Named scratch fields inside Plan::fld.
@@ -157,6 +172,17 @@ void launch_gemm_stage(const Plan& p, int stage, const T* F, const T* Fsub, cons
                        int mode, int cfg, hipStream_t s);
 // (M, N, K, products) of GEMM stage `stage` of a plan.
 void gemm_stage_shape(const Plan& p, int stage, int* M, int* N, int* K, int* nz);
+// Split-product stage on int8 MFMA (vd_split.hip), FP64 plans with split_ns > 0,
+// and its tile configurations.
+void launch_split_stage(const Plan& p, int stage, const double* F, const double* Fsub,
+                        const double* hb, double* out, int mode, int cfg, hipStream_t s);
+int split_num_cfgs();
+int split_max_k();
+int split_group_build(Group& g);
+void launch_inversion_split_rg(const Group& g, int nact, const double* F, const double* Fsub,
+                               const double* hb, double* out, int mode, hipStream_t s);
+bool split_cfg_fits(const Plan& p, int stage, int c);
+int split_default_cfg(const Plan& p, int stage);
 
 // Prepare a segment: hcur = h0 + mdt, hbm = hb + mdt, q0 = h2pv(hcur, hbm), qb = h2pv(hbm, hbm).
 template <typename T>

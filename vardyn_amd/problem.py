@@ -86,11 +86,12 @@ def build_batch(case: syn.QgCase, nbatch, precision="float64", n_steps=None, tap
 
 def build_problem(case: syn.QgCase, precision="float64", n_steps=None, control="ic",
                   tape="checkpoint", device=None, truth_plan: QgPlan | None = None,
-                  with_engine=True) -> Problem:
+                  with_engine=True, dst_split=0) -> Problem:
     n_steps = case.n_steps if n_steps is None else int(n_steps)
     grid = syn.make_grid(case.ny, case.nx, case.dx)
     plan = QgPlan(case.ny, case.nx, case.dx, case.dx, case.dt, case.c, case.f0, g=case.g,
-                  time_scheme=case.time_scheme, precision=precision, device=device)
+                  time_scheme=case.time_scheme, precision=precision, device=device,
+                  dst_split=dst_split)
     obs = syn.make_observations(case, grid, n_steps)
     truth0 = syn.gaussian_random_field(case.ny, case.nx, case.dx, case.wavelength, case.rms,
                                        case.seed)

@@ -65,7 +65,8 @@ class QgPlan:
 
     def __init__(self, ny, nx, dx, dy, dt, c, f0, g=9.81, time_scheme="rk2",
                  precision="float64", kdiffus=None, mask=None, beta=None, fbathy=None,
-                 mdt=None, device=None, dst_dense=None, gemm_config=None, nbatch=1):
+                 mdt=None, device=None, dst_dense=None, gemm_config=None, nbatch=1,
+                 dst_split=None):
         if not torch.cuda.is_available():
             raise VardynError("no HIP device available: the QG plan runs on the GPU only")
         self.device = torch.device("cuda", torch.cuda.current_device() if device is None
@@ -119,6 +120,11 @@ class QgPlan:
             # VARDYN_GEMM_CONFIG=k forces tile configuration k-1 (A/B comparisons)
             gemm_config = int(os.environ.get("VARDYN_GEMM_CONFIG", "0"))
         p.gemm_config = int(gemm_config)
+        if dst_split is None:
+            # VARDYN_DST_SPLIT=n selects the split-product GEMMs with n digits
+            dst_split = int(os.environ.get("VARDYN_DST_SPLIT", "0"))
+        p.dst_split = int(dst_split)
+        self.dst_split = p.dst_split
         self.params = p
         handle = C.c_void_p()
         with torch.cuda.device(self.device):
