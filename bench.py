@@ -72,10 +72,12 @@ def parse():
                     help="first N problems of the problem set (profiling)")
     ap.add_argument("--streams", type=int, default=4,
                     help="HIP streams the problem set is spread over")
-    ap.add_argument("--dst-split", type=int, default=None,
-                    help="sine-matrix GEMMs as split products on int8 MFMA with this many "
-                         "digits per FP64 operand (0 = FP64 MFMA); sets VARDYN_DST_SPLIT for "
-                         "every plan of the run")
+    ap.add_argument("--dst-split", type=int, default=6,
+                    help="batched path: sine-matrix GEMMs as split products on int8 MFMA with "
+                         "this many digits per FP64 operand (0 = FP64 MFMA)")
+    ap.add_argument("--plain-dst-split", type=int, default=0,
+                    help="plain single-problem path: the same choice (FP64 MFMA by default, "
+                         "the faster one for a single 512^2 problem)")
     ap.add_argument("--check-f32", action="store_true",
                     help="also evaluate the FP32 path and report its gradient error vs FP64")
     return ap.parse_args()
@@ -231,12 +233,13 @@ def copy_calibration(nbytes=256 << 20):
     return {"bytes_read": nbytes, "bytes_written": nbytes, "achieved_GBs": round(gbs, 1)}
 
 
-def cpu_baseline(case, sample_steps, precision):
+def cpu_baseline(case, sample_steps, precision, dst_split=0):
     """The FP64 oracle's ``cost_and_grad`` on one host thread over a bounded
     sample of the same workload (the first ``sample_steps`` steps of the
     window, with their observations), scaled to the full window.  The same
-    sample is evaluated by the GPU engine, and the relative error of its
-    gradient against the oracle's is reported beside the timing."""
+    sample is evaluated by the GPU engine (FP64-MFMA sine GEMMs, and with
+    ``dst_split`` digits of split products when given), and the relative
+    error of its gradient against the oracle's is reported beside the timing."""
     import torch
     from threadpoolctl import threadpool_limits
     from oracle.fourdvar import BasisInitialState, CostOracle, Cov, ModelOracle
@@ -247,6 +250,12 @@ def cpu_baseline(case, sample_steps, precision):
                        tape="full")
     X = 0.5 * pb.truth0.ravel()
     J, G = pb.engine.cost_and_grad(X)
+    Js = Gs = None
+    if dst_split:
+        pbs = build_problem(case, precision=precision, n_steps=sample_steps, control="ic",
+                            tape="full", truth_plan=pb.plan, dst_split=dst_split)
+        Js, Gs = pbs.engine.cost_and_grad(X)
+        del pbs
     qg = QgmOracle(dx=case.dx, dy=case.dx, dt=case.dt, c=case.c, f=case.f0,
                    time_scheme=case.time_scheme, shape=(case.ny, case.nx))
     H = ObsopOracle(pb.grid.lon, pb.grid.lat, pb.grid.DX, pb.grid.DY, pb.T, case.dt, pb.dict_obs,
@@ -262,13 +271,18 @@ def cpu_baseline(case, sample_steps, precision):
     del pb
     gc.collect()
     torch.cuda.empty_cache()
+    extra = {}
+    if Gs is not None:
+        extra = {"gpu_split_grad_rel_err_vs_oracle": float(np.linalg.norm(Gs - Gr) / np.linalg.norm(Gr)),
+                 "gpu_split_J_rel_err_vs_oracle": float(abs(Js - Jr) / abs(Jr)),
+                 "split_digits": dst_split}
     return {"value": round(1.0 / per_grad, 6), "unit": "gradients/s", "cores": 1, "kind": "port",
             "sample": f"FP64 NumPy/SciPy oracle cost_and_grad over the first {sample_steps} RK2 "
                       f"steps of the {case.ny}x{case.nx} window with their observations "
                       f"({t:.1f} s), scaled by {case.n_steps}/{sample_steps}",
             "host": platform.processor() or platform.machine(), "host_cpus": os.cpu_count(),
             "gpu_grad_rel_err_vs_oracle": float(np.linalg.norm(G - Gr) / np.linalg.norm(Gr)),
-            "gpu_J_rel_err_vs_oracle": float(abs(J - Jr) / abs(Jr))}
+            "gpu_J_rel_err_vs_oracle": float(abs(J - Jr) / abs(Jr)), **extra}
 
 
 def measure(pb, args, label):
@@ -305,24 +319,25 @@ def measure(pb, args, label):
     return r
 
 
-def split_check(case, pb1, precision, n_steps):
-    """Gradient and cost of a split-product problem against the same problem
-    (same observations) with FP64-MFMA sine GEMMs, over the whole window, at
-    the background state."""
+def split_check(case, pb_ref, precision, n_steps, digits):
+    """Gradient and cost of the workload with split-product sine GEMMs
+    (``digits`` per operand line) against the same problem -- same
+    observations -- with the other GEMM mode (``pb_ref``), over the whole
+    window, at the background state."""
     import torch
     from vardyn_amd.problem import build_problem
-    X0 = torch.zeros(pb1.engine.nctrl, dtype=torch.float64, device=pb1.plan.device)
-    Js, Gs = pb1.engine.run(X0)
-    Js, Gs = Js.clone(), Gs.clone()
-    pb64 = build_problem(case, precision=precision, n_steps=n_steps, control="ic", tape="full",
-                         truth_plan=pb1.plan, dst_split=0)
-    J64, G64 = pb64.engine.run(X0)
+    X0 = torch.zeros(pb_ref.engine.nctrl, dtype=torch.float64, device=pb_ref.plan.device)
+    Jr, Gr = pb_ref.engine.run(X0)
+    Jr, Gr = Jr.clone(), Gr.clone()
+    pbs = build_problem(case, precision=precision, n_steps=n_steps, control="ic", tape="full",
+                        truth_plan=pb_ref.plan, dst_split=digits)
+    Js, Gs = pbs.engine.run(X0)
     torch.cuda.synchronize()
-    r = {"grad_rel_err_vs_fp64_mfma": float((torch.linalg.vector_norm(Gs - G64) /
-                                             torch.linalg.vector_norm(G64)).item()),
-         "J_rel_err_vs_fp64_mfma": abs(Js[0].item() - J64[0].item()) / abs(J64[0].item()),
-         "window_steps": n_steps, "digits": pb1.plan.dst_split}
-    del pb64
+    r = {"grad_rel_err_vs_fp64_mfma": float((torch.linalg.vector_norm(Gs - Gr) /
+                                             torch.linalg.vector_norm(Gr)).item()),
+         "J_rel_err_vs_fp64_mfma": abs(Js[0].item() - Jr[0].item()) / abs(Jr[0].item()),
+         "window_steps": n_steps, "digits": digits}
+    del pbs
     release(None)
     return r
 
@@ -400,8 +415,8 @@ def release(pb):
 
 def main():
     args = parse()
-    if args.dst_split is not None:
-        os.environ["VARDYN_DST_SPLIT"] = str(args.dst_split)
+    if args.precision != "float64":
+        args.dst_split = args.plain_dst_split = 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
               f"(independent problems shard across GPUs with no communication)", file=sys.stderr)
@@ -424,12 +439,14 @@ def main():
 
     plain = None
     if nb == 1 or not args.no_plain:
-        log(f"plain path: one {case.ny}x{case.nx} problem, {n_steps} steps, {args.precision}")
+        log(f"plain path: one {case.ny}x{case.nx} problem, {n_steps} steps, {args.precision}, "
+            f"dst_split {args.plain_dst_split}")
         pb1 = build_problem(case, precision=args.precision, n_steps=n_steps, control="ic",
-                            tape="full" if args.tape == "auto" else args.tape)
+                            tape="full" if args.tape == "auto" else args.tape,
+                            dst_split=args.plain_dst_split)
         plain = measure(pb1, args, "plain")
-        if pb1.plan.dst_split:
-            plain["split_check"] = split_check(case, pb1, args.precision, n_steps)
+        if args.dst_split and not args.plain_dst_split and nb > 1:
+            plain["split_check"] = split_check(case, pb1, args.precision, n_steps, args.dst_split)
             log(f"split check: {plain['split_check']}")
         if nb == 1:
             pb, head = pb1, plain
@@ -438,7 +455,8 @@ def main():
             release(None)
     if nb > 1:
         t0 = time.time()
-        pb = build_batch(case, nb, precision=args.precision, n_steps=n_steps, tape=args.tape)
+        pb = build_batch(case, nb, precision=args.precision, n_steps=n_steps, tape=args.tape,
+                         dst_split=args.dst_split)
         log(f"batched path: {nb} problems built in {time.time() - t0:.1f}s")
         head = measure(pb, args, "batched")
 
@@ -468,7 +486,8 @@ def main():
     if plain is not None and nb > 1:
         out["plain_path"] = {"value": round(plain["value"], 4), "unit": "gradients/s",
                              "ms_per_gradient": round(plain["ms_per_step"], 2),
-                             "tape": plain["tape"], "gemm_tiles": plain["tiles"]}
+                             "tape": plain["tape"], "gemm_tiles": plain["tiles"],
+                             "dst_split": args.plain_dst_split}
         out["batched_over_plain"] = round(value / plain["value"], 3)
     if args.check_f32 and args.precision == "float64" and nb == 1:
         X0 = torch.zeros(pb.engine.nctrl, dtype=torch.float64, device=pb.plan.device)
@@ -494,7 +513,8 @@ def main():
         out["stencil_roofline"] = stencil_roofline(case, args.precision)
         log(f"stencil roofline: {out['stencil_roofline']}")
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(case, args.cpu_sample_steps, args.precision)
+        out["cpu_baseline"] = cpu_baseline(case, args.cpu_sample_steps, args.precision,
+                                           args.dst_split)
         log(f"cpu baseline: {out['cpu_baseline']}")
     print(json.dumps(out), flush=True)
 

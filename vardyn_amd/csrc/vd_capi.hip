@@ -74,7 +74,7 @@ static int upload_split(int npar, int lines, int K, int ns, Op op, void** d8, in
 static void release_plan_buffers(Plan& p) {
   hipFree(p.SyF); hipFree(p.SxF); hipFree(p.gfc2_b);
   hipFree(p.S8y); hipFree(p.S8x); hipFree(p.eSy); hipFree(p.eSx);
-  hipFree(p.Sy); hipFree(p.Sx); hipFree(p.lam_inv); hipFree(p.mask);
+  hipFree(p.Sy); hipFree(p.Sx); hipFree(p.lam_cx); hipFree(p.lam_cy); hipFree(p.gfc2_d); hipFree(p.mask);
   hipFree(p.beta); hipFree(p.fbathy); hipFree(p.mdt);
   for (auto& g : p.gT) hipFree(g);
   for (auto& f : p.fld) hipFree(f);
@@ -135,22 +135,26 @@ static int build_constants(Plan& p, const vd_qg_params& prm) {
     }
     if (rc != VD_OK) return rc;
   }
-  // 1/Lambda with Lambda = g/f [2(cos(pi(j+1)/(nx-1))-1)/dx^2 + 2(cos(pi(i+1)/(ny-1))-1)/dy^2] - g f/c^2
-  // (jqgm.py:165-172), one table per batch member (c may differ); zero in the padding.
-  std::vector<double> li((size_t)p.nb * p.MY * p.MX, 0.0), gfc2(p.nb);
+  // Lambda = g/f [2(cos(pi(j+1)/(nx-1))-1)/dx^2 + 2(cos(pi(i+1)/(ny-1))-1)/dy^2] - g f/c^2
+  // (jqgm.py:165-172) is separable: the x and y terms as two tables and g f/c^2 per batch
+  // member (c may differ); the GEMM epilogue forms 1/Lambda from them.
+  std::vector<double> cx(p.MX, 0.0), cy(p.MY, 0.0), gfc2(p.nb);
+  for (int j = 0; j < p.mx; ++j) cx[j] = 2 * (std::cos(M_PI / (p.nx - 1) * (j + 1)) - 1) / (p.dx * p.dx);
+  for (int i = 0; i < p.my; ++i) cy[i] = 2 * (std::cos(M_PI / (p.ny - 1) * (i + 1)) - 1) / (p.dy * p.dy);
   for (int b = 0; b < p.nb; ++b) {
     const double cb = prm.c_batch ? prm.c_batch[b] : p.c;
     VD_REQUIRE(cb > 0, VD_ERR_INVALID_ARG, "c_batch values must be positive");
     gfc2[b] = p.g * p.f0 / (cb * cb);
-    for (int i = 0; i < p.my; ++i)
-      for (int j = 0; j < p.mx; ++j) {
-        const double lap = 2 * (std::cos(M_PI / (p.nx - 1) * (j + 1)) - 1) / (p.dx * p.dx) +
-                           2 * (std::cos(M_PI / (p.ny - 1) * (i + 1)) - 1) / (p.dy * p.dy);
-        const double lam = p.g / p.f0 * lap - gfc2[b];
-        li[((size_t)b * p.MY + i) * p.MX + j] = 1.0 / lam;
-      }
   }
-  if ((rc = upload<T>(li, &p.lam_inv)) != VD_OK) return rc;
+  {
+    void *dcx = nullptr, *dcy = nullptr, *dg = nullptr;
+    if ((rc = upload<double>(cx, &dcx)) != VD_OK) return rc;
+    p.lam_cx = static_cast<double*>(dcx);
+    if ((rc = upload<double>(cy, &dcy)) != VD_OK) return rc;
+    p.lam_cy = static_cast<double*>(dcy);
+    if ((rc = upload<double>(gfc2, &dg)) != VD_OK) return rc;
+    p.gfc2_d = static_cast<double*>(dg);
+  }
   if ((rc = upload<T>(gfc2, &p.gfc2_b)) != VD_OK) return rc;
 
   std::vector<uint8_t> mask;
@@ -558,7 +562,8 @@ int vd_qg_tape_bytes(const vd_plan* h, int nstep, size_t* bytes) {
 int vd_qg_plan_bytes(const vd_plan* h, size_t* bytes) {
   VD_REQUIRE(h && bytes, VD_ERR_INVALID_ARG, "null argument");
   const Plan& p = h->p;
-  size_t b = ((size_t)p.MY * p.MY + (size_t)p.MX * p.MX + p.nb * p.ng * 4) * p.es + p.nf +
+  size_t b = ((size_t)p.MY * p.MY + (size_t)p.MX * p.MX + p.nb * p.ng * 3) * p.es + p.nf +
+             ((size_t)p.MY + p.MX + p.nb) * sizeof(double) +
              (size_t)F_COUNT * p.nb * p.nf * p.es;
   if (p.split_ns > 0) {
     const int npar = p.fold ? 2 : 1;

@@ -57,11 +57,16 @@ struct GemmArgs {
   int K;
   T* C;
   int ldc;
-  const T* scale;  // EK_SCALE factor, ld = ldc
+  // EK_SCALE factor 1/Lambda, Lambda = gof (lcx[col] + lcy[row]) - gfc2[member]
+  // (separable: two 1-D tables and one constant per member), zero in the padding
+  const double* lcx;
+  const double* lcy;
+  const double* gfc2;
+  double gof;
   const T* hb;     // EK_FIELD_SET offset / ring field (nullptr = 0)
   T* out;          // EK_FIELD_*: (ny, nx) destination
   // batch strides (elements) of each operand; 0 for operands shared by all members
-  size_t bsA, bsB, bsF, bsC, bsScale, bsOut;
+  size_t bsA, bsB, bsF, bsC, bsOut;
   // launch shape: nz products (parities x members), tm x tn tiles each, nb members
   int nz, tm, tn, nb;
 };
@@ -173,10 +178,11 @@ __device__ __forceinline__ void work_item(const GemmArgs<T>& g, int* z, int* til
 // Operand and output pointers of one product.
 template <typename T>
 struct GemmPtrs {
-  const T *A, *B, *Fld, *Fsub, *scale, *hbm;
+  const T *A, *B, *Fld, *Fsub, *hbm;
   T *Cout, *fout;
   int par;
   T sgn;
+  double gfc2;  // g f / c^2 of the member
 };
 template <typename T, int AOP, int BOP, int MAP>
 __device__ __forceinline__ GemmPtrs<T> gemm_ptrs(const GemmArgs<T>& g, int z) {
@@ -190,7 +196,7 @@ __device__ __forceinline__ GemmPtrs<T> gemm_ptrs(const GemmArgs<T>& g, int z) {
   P.Fld = g.F + mb * g.bsF;
   P.Fsub = g.Fsub ? g.Fsub + mb * g.bsF : nullptr;
   P.Cout = g.C + mb * g.bsC;
-  P.scale = g.scale + mb * g.bsScale;
+  P.gfc2 = g.gfc2 ? g.gfc2[mb] : 0.0;
   P.fout = g.out + mb * g.bsOut;
   P.hbm = g.hb ? g.hb + mb * g.bsOut : nullptr;
   return P;
@@ -208,7 +214,18 @@ __device__ __forceinline__ void store_out(const GemmArgs<T>& g, const GemmPtrs<T
   if (KIND == EK_STORE || KIND == EK_SCALE) {
     if (MAP == MAP_COLFOLD && col >= g.hx) return;
     const size_t o = (size_t)row * g.ldc + oc;
-    P.Cout[o] = KIND == EK_SCALE ? v * P.scale[o] : v;
+    if (KIND == EK_SCALE) {
+      // the same operations, in the same order and unfused, as the host's
+      // Lambda (so the factor is the correctly rounded 1/Lambda of before)
+      double lam;
+      {
+#pragma clang fp contract(off)
+        lam = g.gof * (g.lcx[oc] + g.lcy[row]) - P.gfc2;
+      }
+      P.Cout[o] = (row < g.my && oc < g.mx) ? v * T(1.0 / lam) : T(0);
+    } else {
+      P.Cout[o] = v;
+    }
     return;
   }
   const bool ok = row < g.my && (MAP == MAP_COLFOLD ? col < g.hx : col < g.mx);
@@ -246,10 +263,13 @@ GemmArgs<T> stage_args(const Plan& p, int stage) {
   T* T2 = static_cast<T*>(p.gT[1]);
   GemmArgs<T> a{};
   a.ny = p.ny; a.nx = p.nx; a.my = p.my; a.mx = p.mx; a.hy = p.hy; a.hx = p.hx;
-  a.scale = static_cast<const T*>(p.lam_inv);
+  a.lcx = p.lam_cx;
+  a.lcy = p.lam_cy;
+  a.gfc2 = p.gfc2_d;
+  a.gof = p.g / p.f0;
   const size_t NF = p.nf, NG = p.ng;
   a.nb = p.nb;
-  a.bsF = NF; a.bsOut = NF; a.bsScale = NG;
+  a.bsF = NF; a.bsOut = NF;
   const bool fold = p.fold;
   const T* Ay = static_cast<const T*>(fold ? p.SyF : p.Sy);
   const T* Bx = static_cast<const T*>(fold ? p.SxF : p.Sx);

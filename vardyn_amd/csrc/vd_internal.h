@@ -103,7 +103,9 @@ struct Plan {
   // constants
   void* Sy = nullptr;
   void* Sx = nullptr;
-  void* lam_inv = nullptr;  // [nb][MY][MX]
+  double* lam_cx = nullptr;  // [MX] 2 (cos(pi (j+1)/(nx-1)) - 1)/dx^2, zero past mx
+  double* lam_cy = nullptr;  // [MY] the same in y
+  double* gfc2_d = nullptr;  // [nb] g f / c^2 per member, FP64
   void* gfc2_b = nullptr;   // [nb]
   uint8_t* mask = nullptr;
   void* beta = nullptr;
@@ -129,7 +131,7 @@ struct Group {
   std::vector<Plan*> plans;   // not owned
   std::vector<size_t> foff;   // n + 1 prefix offsets
   void* mem = nullptr;        // RgMember<T>[n]
-  int4* stage_tiles = nullptr;  // (member, tile row, tile column) of 64 x 4 point tiles
+  int4* stage_tiles = nullptr;  // (member, tile row, tile column) of 64 x 16 point tiles
   std::vector<int> stage_prefix;
   int4* adj_tiles = nullptr;    // the same for 64 x 16 tiles of the adjoint stage
   std::vector<int> adj_prefix;

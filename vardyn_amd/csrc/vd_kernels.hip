@@ -533,14 +533,18 @@ __device__ __forceinline__ Vel<T> vel_T(const T* h, int nx, int i, int j, const 
   return r;
 }
 
-// rhs (jqgm.py:321-375) at an active point (mask == 3, so 2 <= i,j <= n-4).
+// rhs (jqgm.py:321-375) at an active point (mask == 3, so 2 <= i,j <= n-4),
+// with h and q read through views of row pitch `pitch` at (li, lj) -- the
+// fields themselves (pitch nx, (li, lj) = (i, j)) or an LDS tile -- and the
+// optional beta and f*bathymetry fields at (i, j).
 template <typename T>
-__device__ __forceinline__ T rhs_at(const T* h, const T* q, int i, int j, const Phys<T>& P) {
+__device__ __forceinline__ T rhs_view(const T* h, const T* q, int pitch, int li, int lj, int i,
+                                      int j, const Phys<T>& P) {
   const int nx = P.nx;
-  const Vel<T> V = vel_T(h, nx, i, j, P);
+  const Vel<T> V = vel_T(h, pitch, li, lj, P);
   const T up = V.uT < 0 ? T(0) : V.uT, um = V.uT > 0 ? T(0) : V.uT;
   const T vp = V.vT < 0 ? T(0) : V.vT, vm = V.vT > 0 ? T(0) : V.vT;
-  const Diffs<T> D = diffs(q, nx, i, j);
+  const Diffs<T> D = diffs(q, pitch, li, lj);
   T k = (um * D.xm - up * D.xp) * P.inv6dx + (vm * D.ym - vp * D.yp) * P.inv6dy;
   if (P.fbathy) {
     const Diffs<T> B = diffs(P.fbathy, nx, i, j);
@@ -548,11 +552,15 @@ __device__ __forceinline__ T rhs_at(const T* h, const T* q, int i, int j, const 
   }
   if (P.beta) k -= P.beta[i * nx + j] * V.vT;
   if (P.has_kdiff) {
-    const T c = q[i * nx + j];
-    k += P.kx * (q[i * nx + j + 1] + q[i * nx + j - 1] - 2 * c) +
-         P.ky * (q[(i + 1) * nx + j] + q[(i - 1) * nx + j] - 2 * c);
+    const T c = q[li * pitch + lj];
+    k += P.kx * (q[li * pitch + lj + 1] + q[li * pitch + lj - 1] - 2 * c) +
+         P.ky * (q[(li + 1) * pitch + lj] + q[(li - 1) * pitch + lj] - 2 * c);
   }
   return nz(k);
+}
+template <typename T>
+__device__ __forceinline__ T rhs_at(const T* h, const T* q, int i, int j, const Phys<T>& P) {
+  return rhs_view(h, q, P.nx, i, j, i, j, P);
 }
 
 // One thread per point; blockIdx.z is the batch member and zo the offset of
@@ -647,24 +655,47 @@ __global__ void k_finish(Phys<T> P, const T* __restrict__ h, const T* __restrict
 }
 
 // Fused RK stage: qout = qbase + c0 k0 + c1 k1 + c2 rhs(h, q) (rhs stored in
-// kstore when given), one point per thread.  The stencil loads are taken at
-// indices clamped into the band 2..n-3 (which holds every active point, mask
-// == 3), so they are always in bounds and issued without a branch; the mask
-// then selects rhs or zero.  (Several rows per thread were measured and gave
-// no gain.)
+// kstore when given); the mask selects rhs (active points) or zero.
+// LDS-tiled: a 64 x 16 tile of points per workgroup (64 x 4 threads, four
+// rows each).  h and q of the tile and its 2-point halo are loaded once into
+// LDS (rows and columns clamped into the field, so always in bounds); every
+// stencil read then comes from LDS, which leaves HBM as the bound (one global
+// read per point of h and q instead of about twenty).  Points outside the
+// active band (every active point lies in rows/columns 2..n-3) read halo
+// values that the mask then discards; the optional beta and f*bathymetry
+// fields are read at the point clamped into that band.
+constexpr int kSX = 64, kSY = 16, kSP = kSX + 4;
 template <typename T>
-__device__ __forceinline__ void stage_point(const Phys<T>& P, int i, int j, const T* h,
-                                            const T* q, const T* qbase, const T* k0,
-                                            const T* k1, T c0, T c1, T c2, T* qout, T* kstore) {
-  const int nx = P.nx;
-  const size_t o = (size_t)i * nx + j;
-  const T kk = rhs_at(h, q, min(max(i, 2), P.ny - 3), min(max(j, 2), nx - 3), P);
-  const T k = P.mask[o] == 3 ? kk : T(0);
-  if (kstore) kstore[o] = k;
-  T v = qbase[o];
-  if (k0) v += c0 * k0[o];
-  if (k1) v += c1 * k1[o];
-  qout[o] = v + c2 * k;
+__device__ __forceinline__ void stage_tile(const Phys<T>& P, int i0, int j0, const T* h,
+                                           const T* q, const T* qbase, const T* k0,
+                                           const T* k1, T c0, T c1, T c2, T* qout, T* kstore) {
+  __shared__ T hs[(kSY + 4) * kSP];
+  __shared__ T qs[(kSY + 4) * kSP];
+  const int nx = P.nx, ny = P.ny;
+  const int tid = threadIdx.x + 64 * threadIdx.y;
+  for (int e = tid; e < (kSY + 4) * kSP; e += 256) {
+    const int r = e / kSP, c = e % kSP;
+    const size_t g = (size_t)min(max(i0 - 2 + r, 0), ny - 1) * nx + min(max(j0 - 2 + c, 0), nx - 1);
+    hs[e] = h[g];
+    qs[e] = q[g];
+  }
+  __syncthreads();
+  const int j = j0 + threadIdx.x;
+  if (j >= nx) return;
+#pragma unroll
+  for (int rr = 0; rr < kSY / 4; ++rr) {
+    const int li = threadIdx.y + 4 * rr, i = i0 + li;
+    if (i >= ny) break;
+    const size_t o = (size_t)i * nx + j;
+    const int ic = min(max(i, 2), ny - 3), jc = min(max(j, 2), nx - 3);  // global optional fields
+    const T kk = rhs_view(hs, qs, kSP, li + 2, threadIdx.x + 2, ic, jc, P);
+    const T k = P.mask[o] == 3 ? kk : T(0);
+    if (kstore) kstore[o] = k;
+    T v = qbase[o];
+    if (k0) v += c0 * k0[o];
+    if (k1) v += c1 * k1[o];
+    qout[o] = v + c2 * k;
+  }
 }
 
 template <typename T>
@@ -674,10 +705,9 @@ __global__ __launch_bounds__(256) void k_stage(Phys<T> P, const T* __restrict__ 
                                                const T* __restrict__ k0,
                                                const T* __restrict__ k1, T c0, T c1, T c2,
                                                T* __restrict__ qout, T* __restrict__ kstore) {
-  VD_POINT_2D
+  const size_t zo = (size_t)blockIdx.z * P.nf;
   VD_B(h); VD_B(q); VD_B(qbase); VD_B(k0); VD_B(k1); VD_B(qout); VD_B(kstore);
-  (void)o;
-  stage_point(P, i, j, h, q, qbase, k0, k1, c0, c1, c2, qout, kstore);
+  stage_tile(P, blockIdx.y * kSY, blockIdx.x * kSX, h, q, qbase, k0, k1, c0, c1, c2, qout, kstore);
 }
 
 // Ragged groups: block b takes tile (member, tile row, tile column) of the
@@ -693,11 +723,9 @@ __global__ __launch_bounds__(256) void k_stage_rg(const RgMember<T>* __restrict_
                                                   T* __restrict__ qout, T* __restrict__ kstore) {
   const int4 t = tiles[blockIdx.x];
   const Phys<T> P = mem[t.x].P;
-  const int i = t.y * 4 + threadIdx.y, j = t.z * 64 + threadIdx.x;
-  if (i >= P.ny || j >= P.nx) return;
   const size_t zo = mem[t.x].foff;
   VD_B(h); VD_B(q); VD_B(qbase); VD_B(k0); VD_B(k1); VD_B(qout); VD_B(kstore);
-  stage_point(P, i, j, h, q, qbase, k0, k1, c0, c1, c2, qout, kstore);
+  stage_tile(P, t.y * kSY, t.z * kSX, h, q, qbase, k0, k1, c0, c1, c2, qout, kstore);
 }
 
 // ---------------------------------------------------------------------------
@@ -1005,7 +1033,8 @@ template <typename T>
 void launch_stage(const Plan& p, const T* h, const T* q, const T* qbase, const T* kprev0,
                   const T* kprev1, T c0, T c1, T c2, T* qout, T* kstore, hipStream_t s) {
   Phys<T> P = make_phys<T>(p);
-  hipLaunchKernelGGL(k_stage<T>, grid2d(p.ny, p.nx, p.nb), kBlock2d, 0, s, P, h, q, qbase, kprev0,
+  hipLaunchKernelGGL(k_stage<T>, dim3((p.nx + kSX - 1) / kSX, (p.ny + kSY - 1) / kSY, p.nb), kBlock2d, 0, s,
+                     P, h, q, qbase, kprev0,
                      kprev1, c0, c1, c2, qout, kstore);
 }
 
@@ -1141,8 +1170,8 @@ int group_build(Group& g) {
     g.foff[b + 1] = g.foff[b] + p.nf;
     mem[b].P = make_phys<T>(p);
     mem[b].foff = g.foff[b];
-    for (int ty = 0; ty < (p.ny + 3) / 4; ++ty)
-      for (int tx = 0; tx < (p.nx + 63) / 64; ++tx) st.push_back(make_int4(b, ty, tx, 0));
+    for (int ty = 0; ty < (p.ny + kSY - 1) / kSY; ++ty)
+      for (int tx = 0; tx < (p.nx + kSX - 1) / kSX; ++tx) st.push_back(make_int4(b, ty, tx, 0));
     for (int ty = 0; ty < (p.ny + kAY - 1) / kAY; ++ty)
       for (int tx = 0; tx < (p.nx + kAX - 1) / kAX; ++tx) at.push_back(make_int4(b, ty, tx, 0));
     g.stage_prefix[b + 1] = (int)st.size();

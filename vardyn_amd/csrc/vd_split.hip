@@ -167,6 +167,7 @@ __device__ __forceinline__ void split_tile(const SplitArgs& sa, int z, int tile)
   __shared__ double red[8 * BDL];
   __shared__ double dscale[BDL];
   __shared__ int dexp[BDL];
+  __shared__ int sexp[BSL];
 
   const GemmArgs<double>& g = sa.g;
   const GemmPtrs<double> P = gemm_ptrs<double, AOP, BOP, MAP>(g, z);
@@ -184,6 +185,9 @@ __device__ __forceinline__ void split_tile(const SplitArgs& sa, int z, int tile)
     const int Q = tid + NTHR * j;
     return DS == 0 ? 4 * (Q / BDL) : 4 * (Q % 8);
   };
+
+  // sine line exponents of the tile (read in the epilogue)
+  for (int i = tid; i < BSL; i += NTHR) sexp[i] = sa.eS[(size_t)P.par * sa.lines + s0 + i];
 
   // 1. fetch and fold everything (steps past nstep re-read the last one and are zeroed)
   double v[NSTEP][NQ][4];
@@ -346,28 +350,53 @@ __device__ __forceinline__ void split_tile(const SplitArgs& sa, int z, int tile)
                 acc[l][a][b][r] += park[(((q - 1) * (WS * WD) + slot) * PER + (a * NT + b) * 16 + r) * 64 + lane];
       }
     }
-    if (wk != 0) return;
   }
 
-  // 5. combine levels in FP64, scale by the line exponents, epilogue
-  const int* eS = sa.eS + (size_t)P.par * sa.lines;
+  // 5. combine levels in FP64 (Horner in 2^-8, exact conversions), scale by
+  // the line exponents, epilogue.  Left operands' products (DS = 1) have data
+  // lines as output rows: their tile goes through LDS so that consecutive
+  // threads store (and read Lambda / hb at) consecutive columns of a row.
+  auto value = [&](int a, int b, int r) {
+    const int dl = wd * WDL + b * 32 + (lane & 31);
+    const int sl = ws * WSL + a * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+    double x = (double)acc[NS - 1][a][b][r];
 #pragma unroll
-  for (int a = 0; a < MT; ++a)
+    for (int l = NS - 2; l >= 0; --l) x = fma(x, 0x1p-8, (double)acc[l][a][b][r]);
+    const int ed = dexp[dl];
+    return ed == INT_MIN ? (double)NAN : ldexp(x, sexp[sl] + ed - 16);
+  };
+  if constexpr (DS == 0) {
+    if (wk == 0) {
 #pragma unroll
-    for (int b = 0; b < NT; ++b) {
-      const int dl = wd * WDL + b * 32 + (lane & 31);
-      const int ed = dexp[dl];
+      for (int a = 0; a < MT; ++a)
 #pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int sl = s0 + ws * WSL + a * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-        double x = 0.0;
+        for (int b = 0; b < NT; ++b)
 #pragma unroll
-        for (int l = NS - 1; l >= 0; --l) x += ldexp((double)acc[l][a][b][r], -8 * (l + 2));
-        x = ed == INT_MIN ? NAN : ldexp(x, eS[sl] + ed);
-        if (DS == 0) store_out<double, KIND, MAP>(g, P, sl, d0 + dl, x);
-        else store_out<double, KIND, MAP>(g, P, d0 + dl, sl, x);
-      }
+          for (int r = 0; r < 16; ++r)
+            store_out<double, KIND, MAP>(g, P, s0 + ws * WSL + a * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3),
+                                         d0 + wd * WDL + b * 32 + (lane & 31), value(a, b, r));
     }
+  } else {
+    constexpr int TP = BSL + 1;  // row pitch (doubles) of the transposed tile
+    static_assert(BDL * TP * 8 <= NS * BDL * KP, "output tile fits the data planes");
+    double* T = reinterpret_cast<double*>(&Dl[0][0][0]);
+    __syncthreads();  // every wave is done with the data planes
+    if (wk == 0) {
+#pragma unroll
+      for (int a = 0; a < MT; ++a)
+#pragma unroll
+        for (int b = 0; b < NT; ++b)
+#pragma unroll
+          for (int r = 0; r < 16; ++r)
+            T[(wd * WDL + b * 32 + (lane & 31)) * TP + ws * WSL + a * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3)] =
+                value(a, b, r);
+    }
+    __syncthreads();
+    for (int e = tid; e < BDL * BSL; e += NTHR) {
+      const int dl = e / BSL, sl = e % BSL;
+      store_out<double, KIND, MAP>(g, P, d0 + dl, s0 + sl, T[dl * TP + sl]);
+    }
+  }
 }
 
 template <int AOP, int BOP, int KIND, int MAP, int BSL, int BDL, int WS, int WD, int NS, int NSTEP>

@@ -49,7 +49,7 @@ def truth_run(plan: QgPlan, h0, steps, dt):
 
 
 def build_batch(case: syn.QgCase, nbatch, precision="float64", n_steps=None, tape="full",
-                device=None, c=None):
+                device=None, c=None, dst_split=0):
     """``nbatch`` independent problems of ``case`` in one batched plan.
 
     Member b has its own seeded initial field, track/swath geometry and
@@ -60,7 +60,7 @@ def build_batch(case: syn.QgCase, nbatch, precision="float64", n_steps=None, tap
     grid = syn.make_grid(case.ny, case.nx, case.dx)
     plan = QgPlan(case.ny, case.nx, case.dx, case.dx, case.dt, case.c if c is None else c,
                   case.f0, g=case.g, time_scheme=case.time_scheme, precision=precision,
-                  device=device, nbatch=nbatch)
+                  device=device, nbatch=nbatch, dst_split=dst_split)
     truth0 = np.stack([syn.gaussian_random_field(case.ny, case.nx, case.dx, case.wavelength,
                                                  case.rms, case.seed + 7919 * b)
                        for b in range(nbatch)])
