@@ -64,6 +64,9 @@ def parse():
                     help="keep the shape-based GEMM tile choice instead of timing the candidates")
     ap.add_argument("--no-stencil", action="store_true",
                     help="skip the batched stencil HBM measurement")
+    ap.add_argument("--gemm-only", action="store_true",
+                    help="only the inversion-GEMM roofline of the batched plan (no problem, no tape): "
+                         "the command the GEMM counter runs profile")
     ap.add_argument("--stencil-only", action="store_true",
                     help="only the stencil measurement and a copy of known size (for counter runs)")
     ap.add_argument("--problem-set", action="store_true",
@@ -428,6 +431,15 @@ def main():
     case = syn.case_basin512()
     if args.problem_set:
         print(json.dumps(problem_set(args)), flush=True)
+        return
+    if args.gemm_only:
+        from vardyn_amd.qg import QgPlan
+        plan = QgPlan(case.ny, case.nx, case.dx, case.dx, case.dt, case.c, case.f0, g=case.g,
+                      time_scheme=case.time_scheme, precision=args.precision,
+                      nbatch=max(1, args.batch), dst_split=args.dst_split)
+        tiles = plan.autotune(reps=5) if not args.no_autotune else None
+        print(json.dumps({"gemm_roofline": gemm_roofline(plan, args.precision), "gemm_tiles": tiles,
+                          "batch": plan.nb, "dst_split": plan.dst_split}), flush=True)
         return
     if args.stencil_only:
         out = {"stencil_roofline": stencil_roofline(case, args.precision),

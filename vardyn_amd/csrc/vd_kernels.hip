@@ -656,7 +656,7 @@ __global__ void k_finish(Phys<T> P, const T* __restrict__ h, const T* __restrict
 
 // Fused RK stage: qout = qbase + c0 k0 + c1 k1 + c2 rhs(h, q) (rhs stored in
 // kstore when given); the mask selects rhs (active points) or zero.
-// LDS-tiled: a 64 x 16 tile of points per workgroup (64 x 4 threads, four
+// LDS-tiled: a 64 x 8 tile of points per workgroup (64 x 4 threads, two
 // rows each).  h and q of the tile and its 2-point halo are loaded once into
 // LDS (rows and columns clamped into the field, so always in bounds); every
 // stencil read then comes from LDS, which leaves HBM as the bound (one global
@@ -664,7 +664,7 @@ __global__ void k_finish(Phys<T> P, const T* __restrict__ h, const T* __restrict
 // active band (every active point lies in rows/columns 2..n-3) read halo
 // values that the mask then discards; the optional beta and f*bathymetry
 // fields are read at the point clamped into that band.
-constexpr int kSX = 64, kSY = 16, kSP = kSX + 4;
+constexpr int kSX = 64, kSY = 8, kSP = kSX + 4;
 template <typename T>
 __device__ __forceinline__ void stage_tile(const Phys<T>& P, int i0, int j0, const T* h,
                                            const T* q, const T* qbase, const T* k0,
@@ -824,17 +824,19 @@ __global__ void k_finish_tl(Phys<T> P, const T* __restrict__ dh, T* __restrict__
 // cotangent ak = a0 s0 + a1 s1 + a2 s2 of rhs at t:
 //   w[0] = lambda_uT, w[1] = lambda_vT (with the beta term),
 //   w[2..5] = up ak/(6dx), um ak/(6dx), vp ak/(6dy), vm ak/(6dy), w[6] = ak.
+// h and q are read through views of row pitch `pitch` at (hi, hj) (an LDS
+// tile); s0..s2, beta and f*bathymetry at the field point (i, j).
 template <typename T>
-__device__ __forceinline__ void adj_terms(const T* h, const T* q, const T* s0, const T* s1,
-                                          const T* s2, T a0, T a1, T a2, int i, int j,
-                                          const Phys<T>& P, T w[7]) {
+__device__ __forceinline__ void adj_terms(const T* h, const T* q, int pitch, int hi, int hj,
+                                          const T* s0, const T* s1, const T* s2, T a0, T a1,
+                                          T a2, int i, int j, const Phys<T>& P, T w[7]) {
   const int nx = P.nx;
   const size_t o = (size_t)i * nx + j;
   T ak = a0 * s0[o];
   if (s1) ak += a1 * s1[o];
   if (s2) ak += a2 * s2[o];
-  const Vel<T> V = vel_T(h, nx, i, j, P);
-  const Diffs<T> D = diffs(q, nx, i, j);
+  const Vel<T> V = vel_T(h, pitch, hi, hj, P);
+  const Diffs<T> D = diffs(q, pitch, hi, hj);
   T dxp = D.xp, dxm = D.xm, dyp = D.yp, dym = D.ym;
   if (P.fbathy) {
     const Diffs<T> B = diffs(P.fbathy, nx, i, j);
@@ -852,15 +854,18 @@ __device__ __forceinline__ void adj_terms(const T* h, const T* q, const T* s0, c
 }
 
 // Fused adjoint stage (one launch, deterministic, no atomics).  A block owns
-// an AX x AY tile of output points.  Phase 1 evaluates adj_terms on the tile
-// plus a halo of 2 into LDS (zero where the point is inactive or outside the
-// grid), together with the validity of u and v at every tile point (inside
-// their domains and not NaN, jqgm.py:238-262).  Phase 2 gathers, for each
-// output point p,
+// an AX x AY tile of output points.  Phase 0 loads h and q of the tile plus a
+// halo of 4 into LDS (rows and columns clamped into the field: in bounds, and
+// only read where the true neighbours exist).  Phase 1 evaluates adj_terms on
+// the tile plus a halo of 2 into LDS (zero where the point is inactive or
+// outside the grid), together with the validity of u and v at every tile
+// point (inside their domains and not NaN, jqgm.py:238-262), reading h and q
+// from LDS.  Phase 2 gathers, for each output point p,
 //   lambda_q(p) from the T points whose upwind / diffusion stencils read q(p),
 //   lambda_h(p) from lambda_u, lambda_v at the velocity points that read h(p).
 constexpr int kAX = 64, kAY = 16, kAH = 2;
 constexpr int kRW = kAX + 2 * kAH, kRH = kAY + 2 * kAH;
+constexpr int kHW = kAX + 4 * kAH, kHH = kAY + 4 * kAH;  // h, q tile: halo 4
 
 template <typename T>
 __device__ __forceinline__ void adj_tile(const Phys<T>& P, int i0, int j0, const T* h, const T* q,
@@ -872,19 +877,29 @@ __device__ __forceinline__ void adj_tile(const Phys<T>& P, int i0, int j0, const
   // points of the tile, which other blocks write, so they must not alias
   // lq_out.
   __shared__ T W[7][kRH][kRW];
+  __shared__ T Hs[kHH * kHW], Qs[kHH * kHW];
   __shared__ uint8_t okU[kRH][kRW], okV[kRH][kRW];
   const int ny = P.ny, nx = P.nx;
   const int tid = threadIdx.y * kAX + threadIdx.x;
+  for (int e = tid; e < kHH * kHW; e += kAX * kAY) {
+    const int r = e / kHW, c = e - r * kHW;
+    const size_t g = (size_t)min(max(i0 - 2 * kAH + r, 0), ny - 1) * nx +
+                     min(max(j0 - 2 * kAH + c, 0), nx - 1);
+    Hs[e] = h[g];
+    Qs[e] = q[g];
+  }
+  __syncthreads();
   for (int r = tid; r < kRH * kRW; r += kAX * kAY) {
     const int li = r / kRW, lj = r - li * kRW;
     const int i = i0 - kAH + li, j = j0 - kAH + lj;
+    const int hi = li + kAH, hj = lj + kAH;  // the same point in the h, q tile
     T w[7] = {0, 0, 0, 0, 0, 0, 0};
     if (i >= 0 && i < ny && j >= 0 && j < nx && P.mask[(size_t)i * nx + j] == 3)
-      adj_terms(h, q, s0, s1, s2, a0, a1, a2, i, j, P, w);
+      adj_terms(Hs, Qs, kHW, hi, hj, s0, s1, s2, a0, a1, a2, i, j, P, w);
 #pragma unroll
     for (int k = 0; k < 7; ++k) W[k][li][lj] = w[k];
-    okU[li][lj] = i >= 1 && i <= ny - 2 && j >= 1 && j <= nx - 1 && !u_isnan(h, nx, i, j);
-    okV[li][lj] = i >= 1 && i <= ny - 1 && j >= 1 && j <= nx - 2 && !v_isnan(h, nx, i, j);
+    okU[li][lj] = i >= 1 && i <= ny - 2 && j >= 1 && j <= nx - 1 && !u_isnan(Hs, kHW, hi, hj);
+    okV[li][lj] = i >= 1 && i <= ny - 1 && j >= 1 && j <= nx - 2 && !v_isnan(Hs, kHW, hi, hj);
   }
   __syncthreads();
   const int i = i0 + threadIdx.y, j = j0 + threadIdx.x;
