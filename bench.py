@@ -73,9 +73,12 @@ def parse():
                     help="the 48-problem uneven-size workload instead (aggregate gradients/s)")
     ap.add_argument("--problem-set-size", type=int, default=48,
                     help="first N problems of the problem set (profiling)")
+    ap.add_argument("--problem-set-split", type=int, default=5,
+                    help="problem set: split-product digits of the ragged engine's sine GEMMs "
+                         "(0 = FP64 MFMA)")
     ap.add_argument("--streams", type=int, default=4,
                     help="HIP streams the problem set is spread over")
-    ap.add_argument("--dst-split", type=int, default=6,
+    ap.add_argument("--dst-split", type=int, default=5,
                     help="batched path: sine-matrix GEMMs as split products on int8 MFMA with "
                          "this many digits per FP64 operand (0 = FP64 MFMA)")
     ap.add_argument("--plain-dst-split", type=int, default=0,
@@ -181,7 +184,8 @@ def stencil_roofline(case, precision, nbatch=64, n_launch=20, reps=5):
     workload's grid at once, so that its working set (three input fields and
     one output per problem) is far larger than the 256 MB Infinity Cache.
     Algorithmic bytes per point: read h, q, base and write out (4 words) plus
-    the shared int8 mask once per launch."""
+    the shared int8 mask once per launch; base is a field of its own (as in
+    the second RK2 stage), so all four streams go to memory."""
     import torch
     from vardyn_amd.qg import QgPlan
     plan = QgPlan(case.ny, case.nx, case.dx, case.dx, case.dt, case.c, case.f0,
@@ -189,14 +193,15 @@ def stencil_roofline(case, precision, nbatch=64, n_launch=20, reps=5):
     g = torch.Generator(device=plan.device).manual_seed(5)
     h = 0.1 * torch.randn(plan.shape, dtype=plan.dtype, device=plan.device, generator=g)
     q = plan.h2pv(h, torch.zeros_like(h))
+    base = q + 1e-9 * torch.randn(plan.shape, dtype=plan.dtype, device=plan.device, generator=g)
     out = torch.empty_like(q)
     for _ in range(3):
-        plan.rhs(h, q, q, case.dt, out)
+        plan.rhs(h, q, base, case.dt, out)
     torch.cuda.synchronize()
     graph = torch.cuda.CUDAGraph()
     with torch.cuda.graph(graph):
         for _ in range(n_launch):
-            plan.rhs(h, q, q, case.dt, out)
+            plan.rhs(h, q, base, case.dt, out)
     graph.replay()
     torch.cuda.synchronize()
     s = torch.cuda.current_stream()
@@ -210,7 +215,7 @@ def stencil_roofline(case, precision, nbatch=64, n_launch=20, reps=5):
     npts = case.ny * case.nx
     nbytes = nbatch * npts * 4 * q.element_size() + npts
     gbs = nbytes / per_launch_s / 1e9
-    del plan, h, q, out, graph
+    del plan, h, q, base, out, graph
     torch.cuda.empty_cache()
     return {"bound": "hbm", "kernel": "k_stage (fused h2uv + upwind advection + RK stage update)",
             "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -340,6 +345,11 @@ def split_check(case, pb_ref, precision, n_steps, digits):
                                              torch.linalg.vector_norm(Gr)).item()),
          "J_rel_err_vs_fp64_mfma": abs(Js[0].item() - Jr[0].item()) / abs(Jr[0].item()),
          "window_steps": n_steps, "digits": digits}
+    # the metric's accuracy bar: gradient within 1e-5 (relative) of FP64
+    r["within_1e-5"] = r["grad_rel_err_vs_fp64_mfma"] <= 1e-5
+    if not r["within_1e-5"]:
+        log(f"WARNING: split-product gradient error {r['grad_rel_err_vs_fp64_mfma']:.3e} exceeds "
+            f"the 1e-5 bar; the batched figure does not meet the metric's accuracy")
     del pbs
     release(None)
     return r
@@ -359,8 +369,8 @@ def problem_set(args):
     from vardyn_amd.ragged import RaggedFourDVar
     cases = syn.config4_cases()[:args.problem_set_size]
     t0 = time.time()
-    pbs = [build_problem(c, precision=args.precision, control="ic", with_engine=False)
-           for c in cases]
+    pbs = [build_problem(c, precision=args.precision, control="ic", with_engine=False,
+                         dst_split=args.problem_set_split) for c in cases]
     eng = RaggedFourDVar(pbs).capture()
     log(f"problem set: {len(cases)} problems built and captured in {time.time() - t0:.1f}s, "
         f"tape {eng.tape_bytes / 2**30:.1f} GiB")
@@ -399,6 +409,7 @@ def problem_set(args):
         "data": "synthetic (seeded fields and nadir tracks per problem)",
         "config": {"workload": f"{len(cases)} problems, sides 96-384, 5-20 day windows, "
                                f"Ld 15-40 km, dt 900 s", "engine": "ragged (shared launches)",
+                   "dst_split": args.problem_set_split,
                    "grid_point_steps_per_evaluation": gps},
         "grid_point_steps_per_s": round(gps * args.steps / dt, 1),
         "per_problem_latency_ms": round(ms, 2),

@@ -63,8 +63,15 @@ typedef struct vd_qg_params {
   double g, f0, c;       /* gravity, Coriolis, gravity-wave speed */
   int has_kdiffus;       /* PV diffusion (jqgm.py:362-369) */
   double kdiffus;
-  const int8_t* mask;    /* (ny, nx) mask 0..3 (jqgm.py:179-217) or NULL for
-                            the land-free ring mask */
+  const int8_t* mask;    /* (ny, nx) mask or NULL for the land-free ring mask:
+                            0 land, 1/2 boundary (q = -g f/c^2 hb, no
+                            tendency), 3 active (rows/columns 2..n-3), as
+                            jqgm.py:179-217 builds them; 4 interior point
+                            without tendency (q from the Laplacian, rows and
+                            columns 1..n-2) and 5 point with q = 0 and no
+                            tendency: the rows next to and on the ring of a
+                            QgmWithTiles tile inside the domain
+                            (jqgm.py:1129-1230) */
   const double* beta;    /* (ny-2, nx) beta = (f[2:]-f[:-2])/(2 dy)
                             (jqgm.py:144) or NULL (f plane) */
   const double* fbathy;  /* (ny, nx) f * bathymetry_PV_term (jqgm.py:353-354)
@@ -268,6 +275,31 @@ int vd_obs_misfit(const vd_obs_table* tab, int precision, const void* x, double*
  * gathered per grid point in a fixed order (bitwise reproducible). */
 int vd_obs_adjoint(const vd_obs_table* tab, int precision, const double* cot,
                    const uint8_t* valid, void* adx, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Overlapping tiles of QgmWithTiles (jqgm.py:1033-1258).  Tile t = ty * ntx +
+ * tx starts at (ys[ty], xs[tx]) (device int32 arrays of nty and ntx starts,
+ * each in 0..n - ts; jqgm.py:1098-1113) and spans ts x ts points; a tile
+ * stack `flat` holds nty * ntx tiles of ts * ts values.  Fields are (ny, nx)
+ * of the given precision; w is the ts x ts window and wsum the (ny, nx) sum of
+ * the windows over the tiles (jqgm.py:1073-1081).
+ * ------------------------------------------------------------------------- */
+/* flat_t = (full + add)[tile t] (add may be NULL): the tile slices of
+ * QgmWithTiles.step_tile (jqgm.py:1177-1187, the MDT added at :1229-1231). */
+int vd_tiles_gather(int precision, int ny, int nx, int ts, int nty, int ntx, const int* ys,
+                    const int* xs, const void* full, const void* add, void* flat, void* stream);
+/* full = sum_t w flat_t / wsum - sub (sub may be NULL), tiles summed in tile
+ * order: the blend of QgmWithTiles.step (jqgm.py:1233-1243). */
+int vd_tiles_blend(int precision, int ny, int nx, int ts, int nty, int ntx, const int* ys,
+                   const int* xs, const void* w, const void* wsum, const void* flat, const void* sub,
+                   void* full, void* stream);
+/* Transposes for the adjoint: flat_t = w full[tile t] / wsum[tile t] ... */
+int vd_tiles_blend_adj(int precision, int ny, int nx, int ts, int nty, int ntx, const int* ys,
+                       const int* xs, const void* w, const void* wsum, const void* full, void* flat,
+                       void* stream);
+/* ... and full = sum_t flat_t placed at tile t (overwrites full). */
+int vd_tiles_gather_adj(int precision, int ny, int nx, int ts, int nty, int ntx, const int* ys,
+                        const int* xs, const void* flat, void* full, void* stream);
 
 /* ---------------------------------------------------------------------------
  * Small device helpers for the 4D-Var driver (tools_4Dvar.py:252-400).

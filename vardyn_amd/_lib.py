@@ -76,6 +76,10 @@ SIGNATURES = {
                                 _VP, _VP, _VP]),
     "vd_obs_misfit": (_I, [C.POINTER(ObsTable), _I, _VP, _VP, _VP, _VP, _D, _VP]),
     "vd_obs_adjoint": (_I, [C.POINTER(ObsTable), _I, _VP, _VP, _VP, _VP]),
+    "vd_tiles_gather": (_I, [_I, _I, _I, _I, _I, _I, _VP, _VP, _VP, _VP, _VP, _VP]),
+    "vd_tiles_blend": (_I, [_I, _I, _I, _I, _I, _I, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP]),
+    "vd_tiles_blend_adj": (_I, [_I, _I, _I, _I, _I, _I, _VP, _VP, _VP, _VP, _VP, _VP, _VP]),
+    "vd_tiles_gather_adj": (_I, [_I, _I, _I, _I, _I, _I, _VP, _VP, _VP, _VP, _VP]),
     "vd_dot_f64": (_I, [_I64, _VP, _VP, _D, _VP, _VP, _VP]),
     "vd_axpby": (_I, [_I, _I64, _D, _VP, _D, _VP, _VP]),
     "vd_convert": (_I, [_I, _VP, _I, _VP, _I64, _I, _VP]),

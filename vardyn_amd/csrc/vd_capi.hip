@@ -161,19 +161,26 @@ static int build_constants(Plan& p, const vd_qg_params& prm) {
   if (prm.mask) {
     mask.resize(p.nf);
     for (size_t i = 0; i < p.nf; ++i) {
-      VD_REQUIRE(prm.mask[i] >= 0 && prm.mask[i] <= 3, VD_ERR_INVALID_ARG, "mask values must be 0..3");
+      VD_REQUIRE(prm.mask[i] >= 0 && prm.mask[i] <= 5, VD_ERR_INVALID_ARG, "mask values must be 0..5");
       mask[i] = (uint8_t)prm.mask[i];
     }
   } else {
     ring_mask(p.ny, p.nx, mask);
   }
-  // The stencils read up to two cells around an active (mask 3) point; the
-  // reference's ring layout keeps active points in rows/columns 2..n-4.
+  // The stencils read up to two cells around an active (mask 3) point and
+  // one around a Laplacian-only (mask 4) point: active points lie in rows and
+  // columns 2..n-3 (the reference's full-domain ring keeps them in 2..n-4;
+  // QgmWithTiles tiles use 2..n-3), mask-4 points in 1..n-2.
   for (int i = 0; i < p.ny; ++i)
-    for (int j = 0; j < p.nx; ++j)
-      if (mask[(size_t)i * p.nx + j] == 3)
-        VD_REQUIRE(i >= 2 && i <= p.ny - 4 && j >= 2 && j <= p.nx - 4, VD_ERR_INVALID_ARG,
-                   "mask value 3 (active) is only allowed in rows/columns 2..n-4");
+    for (int j = 0; j < p.nx; ++j) {
+      const uint8_t m = mask[(size_t)i * p.nx + j];
+      if (m == 3)
+        VD_REQUIRE(i >= 2 && i <= p.ny - 3 && j >= 2 && j <= p.nx - 3, VD_ERR_INVALID_ARG,
+                   "mask value 3 (active) is only allowed in rows/columns 2..n-3");
+      if (m == 4)
+        VD_REQUIRE(i >= 1 && i <= p.ny - 2 && j >= 1 && j <= p.nx - 2, VD_ERR_INVALID_ARG,
+                   "mask value 4 is only allowed in rows/columns 1..n-2");
+    }
   VD_HIP_TRY(hipMalloc((void**)&p.mask, p.nf));
   VD_HIP_TRY(hipMemcpy(p.mask, mask.data(), p.nf, hipMemcpyHostToDevice));
 

@@ -533,7 +533,7 @@ __device__ __forceinline__ Vel<T> vel_T(const T* h, int nx, int i, int j, const 
   return r;
 }
 
-// rhs (jqgm.py:321-375) at an active point (mask == 3, so 2 <= i,j <= n-4),
+// rhs (jqgm.py:321-375) at an active point (mask == 3, so 2 <= i,j <= n-3),
 // with h and q read through views of row pitch `pitch` at (li, lj) -- the
 // fields themselves (pitch nx, (li, lj) = (i, j)) or an LDS tile -- and the
 // optional beta and f*bathymetry fields at (i, j).
@@ -589,8 +589,8 @@ template <typename T>
 __device__ __forceinline__ T h2pv_at(const T* h, const T* hb, int i, int j, uint8_t m,
                                      const Phys<T>& P) {
   const int nx = P.nx;
-  if (m == 0) return T(0);
-  if (m != 3) return -P.gfc2_b[blockIdx.z] * hb[i * nx + j];
+  if (m == 0 || m == 5) return T(0);
+  if (m != 3 && m != 4) return -P.gfc2_b[blockIdx.z] * hb[i * nx + j];
   const T c = h[i * nx + j];
   const T lap = P.gf_dy2 * (h[(i + 1) * nx + j] + h[(i - 1) * nx + j] - 2 * c) +
                 P.gf_dx2 * (h[i * nx + j + 1] + h[i * nx + j - 1] - 2 * c);
@@ -611,13 +611,13 @@ __global__ void k_prep(Phys<T> P, const T* __restrict__ h0, const T* __restrict_
     hcur[o] = H(i, j);
     hbm[o] = HB(i, j);
     T qv = 0, qbv = 0;
-    if (m == 3) {
+    if (m == 3 || m == 4) {
       const T c = H(i, j), cb = HB(i, j);
       qv = nz(P.gf_dy2 * (H(i + 1, j) + H(i - 1, j) - 2 * c) +
               P.gf_dx2 * (H(i, j + 1) + H(i, j - 1) - 2 * c) - P.gfc2_b[blockIdx.z] * c);
       qbv = nz(P.gf_dy2 * (HB(i + 1, j) + HB(i - 1, j) - 2 * cb) +
                P.gf_dx2 * (HB(i, j + 1) + HB(i, j - 1) - 2 * cb) - P.gfc2_b[blockIdx.z] * cb);
-    } else if (m != 0) {
+    } else if (m == 1 || m == 2) {
       qv = -P.gfc2_b[blockIdx.z] * HB(i, j);
       qbv = qv;
     }
@@ -673,6 +673,18 @@ __device__ __forceinline__ void stage_tile(const Phys<T>& P, int i0, int j0, con
   __shared__ T qs[(kSY + 4) * kSP];
   const int nx = P.nx, ny = P.ny;
   const int tid = threadIdx.x + 64 * threadIdx.y;
+  // this thread's point-wise inputs, issued before the tile loads so that
+  // they are in flight together (clamped: always in bounds).  q_base and the
+  // output are streamed once per launch: non-temporal load and store.
+  const int j = j0 + threadIdx.x;
+  T base[kSY / 4];
+  uint8_t msk[kSY / 4];
+#pragma unroll
+  for (int rr = 0; rr < kSY / 4; ++rr) {
+    const size_t o = (size_t)min(i0 + (int)threadIdx.y + 4 * rr, ny - 1) * nx + min(j, nx - 1);
+    base[rr] = __builtin_nontemporal_load(&qbase[o]);
+    msk[rr] = P.mask[o];
+  }
   for (int e = tid; e < (kSY + 4) * kSP; e += 256) {
     const int r = e / kSP, c = e % kSP;
     const size_t g = (size_t)min(max(i0 - 2 + r, 0), ny - 1) * nx + min(max(j0 - 2 + c, 0), nx - 1);
@@ -680,7 +692,6 @@ __device__ __forceinline__ void stage_tile(const Phys<T>& P, int i0, int j0, con
     qs[e] = q[g];
   }
   __syncthreads();
-  const int j = j0 + threadIdx.x;
   if (j >= nx) return;
 #pragma unroll
   for (int rr = 0; rr < kSY / 4; ++rr) {
@@ -689,12 +700,12 @@ __device__ __forceinline__ void stage_tile(const Phys<T>& P, int i0, int j0, con
     const size_t o = (size_t)i * nx + j;
     const int ic = min(max(i, 2), ny - 3), jc = min(max(j, 2), nx - 3);  // global optional fields
     const T kk = rhs_view(hs, qs, kSP, li + 2, threadIdx.x + 2, ic, jc, P);
-    const T k = P.mask[o] == 3 ? kk : T(0);
+    const T k = msk[rr] == 3 ? kk : T(0);
     if (kstore) kstore[o] = k;
-    T v = qbase[o];
+    T v = base[rr];
     if (k0) v += c0 * k0[o];
     if (k1) v += c1 * k1[o];
-    qout[o] = v + c2 * k;
+    __builtin_nontemporal_store(v + c2 * k, &qout[o]);
   }
 }
 
@@ -802,7 +813,7 @@ __global__ void k_prep_tl(Phys<T> P, const T* __restrict__ dh0, const T* __restr
   VD_B(dh0); VD_B(hcur); VD_B(dhcur); VD_B(dq0);
   dhcur[o] = dh0[o];
   T r = T(0);
-  if (P.mask[o] == 3 && !q_isnan(hcur, nx, i, j)) {
+  if ((P.mask[o] == 3 || P.mask[o] == 4) && !q_isnan(hcur, nx, i, j)) {
     const T c = dh0[o];
     r = P.gf_dy2 * (dh0[o + nx] + dh0[o - nx] - 2 * c) +
         P.gf_dx2 * (dh0[o + 1] + dh0[o - 1] - 2 * c) - P.gfc2_b[blockIdx.z] * c;
@@ -979,11 +990,11 @@ __global__ void k_adj_final(Phys<T> P, const T* __restrict__ hcur, const T* __re
   VD_POINT_2D
   VD_B(hcur); VD_B(lh); VD_B(lq); VD_B(adh0);
   const int ny = P.ny;
-  // a(t) = lq(t) where h2pv used its Laplacian formula (mask 3, q not NaN)
+  // a(t) = lq(t) where h2pv used its Laplacian formula (mask 3 or 4, q not NaN)
   auto A = [&](int a, int b) -> T {
     if (a < 1 || a > ny - 2 || b < 1 || b > nx - 2) return T(0);
     const size_t t = (size_t)a * nx + b;
-    if (P.mask[t] != 3 || q_isnan(hcur, nx, a, b)) return T(0);
+    if ((P.mask[t] != 3 && P.mask[t] != 4) || q_isnan(hcur, nx, a, b)) return T(0);
     return lq[t];
   };
   const T r = P.gf_dy2 * (A(i + 1, j) + A(i - 1, j)) + P.gf_dx2 * (A(i, j + 1) + A(i, j - 1)) +
