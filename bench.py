@@ -84,6 +84,13 @@ def parse():
     ap.add_argument("--plain-dst-split", type=int, default=0,
                     help="plain single-problem path: the same choice (FP64 MFMA by default, "
                          "the faster one for a single 512^2 problem)")
+    ap.add_argument("--tiled", action="store_true",
+                    help="the tiled QG model (QgmWithTiles) on the 512x512 grid instead: step and "
+                         "adjoint of all tiles in shared launches (grid-point-steps/s)")
+    ap.add_argument("--tile-size", type=int, default=32)
+    ap.add_argument("--tile-overlap", type=int, default=16)
+    ap.add_argument("--tiled-steps", type=int, default=16,
+                    help="tiled model: steps per step / step_adj call (one segment)")
     ap.add_argument("--check-f32", action="store_true",
                     help="also evaluate the FP32 path and report its gradient error vs FP64")
     return ap.parse_args()
@@ -418,6 +425,146 @@ def problem_set(args):
         "ragged_over_separate": round(sep_ms / ms, 3)}
 
 
+def tiled(args):
+    """The tiled QG model (``QgmWithTiles``, jqgm.py:1033-1258) on the 512 x 512
+    grid of the headline workload with the reference's default tiling (32-point
+    tiles overlapping by 16: 1024 tiles).  One unit is a ``step`` plus a
+    ``step_adj`` call over a segment of ``--tiled-steps`` RK2 steps, each call
+    one replayed HIP graph; ``value`` is domain grid points x steps per second
+    of that pair.  Beside it: the same calls tile by tile (``engine="plans"``),
+    the FP64 tiled oracle on one host thread, parity of both calls against the
+    oracle, and the rooflines of the shared-launch inversion GEMMs and stage
+    stencil (graph-timed with HIP events on the calls' stream)."""
+    import numpy as np
+    import torch
+    from oracle.tiles import TiledQgmOracle
+    from vardyn_amd import synthetic as syn
+    from vardyn_amd.tiles import QgmWithTiles
+    case = syn.case_basin512()
+    ny, nx, ts, ov, n = case.ny, case.nx, args.tile_size, args.tile_overlap, args.tiled_steps
+    kw = dict(dx=case.dx * np.ones((ny, nx)), dy=case.dx * np.ones((ny, nx)), dt=case.dt,
+              SSH=np.zeros((ny, nx)), c=case.c, f=case.f0, time_scheme="rk2", tile_size=ts,
+              tile_overlap=ov)
+    t0 = time.time()
+    m = QgmWithTiles(**kw, engine="group")
+    log(f"tiled: {m.ntile} tiles of {ts}x{ts} (overlap {ov}) built in {time.time() - t0:.1f}s")
+    h0n = syn.gaussian_random_field(ny, nx, case.dx, 100e3, 0.1, seed=11)
+    rng = np.random.default_rng(12)
+    hbn = 0.01 * rng.standard_normal((ny, nx))
+    adn = rng.standard_normal((ny, nx))
+    dev = m.device
+    h0, hb, ad = (torch.as_tensor(a, device=dev) for a in (h0n, hbn, adn))
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            out = fn()
+        g.replay()
+        torch.cuda.synchronize()
+        s_ = torch.cuda.current_stream()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(s_)
+        for _ in range(reps):
+            g.replay()
+        e1.record(s_)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps, out, g
+
+    ms_f, _, gf = timed(lambda: m.step(h0, hb, n), max(3, args.steps))
+    ms_a, _, ga = timed(lambda: m.step_adj(ad, h0, hb, n), max(3, args.steps))
+    del gf, ga
+    gps = ny * nx * n
+    value = gps / ((ms_f + ms_a) * 1e-3)
+    log(f"tiled, shared launches: step {ms_f:.2f} ms, step_adj {ms_a:.2f} ms per {n}-step call")
+    # rooflines of the shared launches: inversion GEMMs (FP64 MFMA) and stage stencil
+    grp, T = m.group, m.ntile
+    F, H, Q, Q2 = (m._stack().normal_() for _ in range(4))
+    qb = torch.zeros_like(F)
+    reps = 50
+
+    def inv():
+        for _ in range(reps):
+            grp.inversion(T, ptr_(F), ptr_(H), Fsub=ptr_(qb), hb=ptr_(qb))
+
+    def stage():
+        for _ in range(reps):
+            grp.stage(T, ptr_(H), ptr_(Q), ptr_(Q), None, None, 0, 0, case.dt, ptr_(Q2))
+
+    ms_inv = timed(inv, 3)[0] / reps
+    ms_st = timed(stage, 3)[0] / reps
+    my = ts - 2
+    flops_launch = T * my * my * (2 * my)
+    inv_tf = flops_launch / (ms_inv / 4 * 1e-3) / 1e12
+    st_bytes = T * ts * ts * (4 * 8 + 1)
+    st_gbs = st_bytes / (ms_st * 1e-3) / 1e9
+    roof = {"bound": "mfma", "kernel": "k_gemm_rg (sine-matrix GEMMs of all tiles, FP64 MFMA, "
+                                       "4 launches per inversion)",
+            "achieved": round(inv_tf, 3), "peak": MFMA_PEAK["f64"], "unit": "TFLOP/s",
+            "frac": round(inv_tf / MFMA_PEAK["f64"], 4), "traffic": None,
+            "avg_launch_us": round(ms_inv / 4 * 1e3, 3), "algorithmic_flops_per_launch": flops_launch}
+    st_roof = {"bound": "hbm", "kernel": "k_stage_rg (stage stencil of all tiles)",
+               "achieved": round(st_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+               "frac": round(st_gbs / HBM_PEAK_GBS, 4), "bytes_per_launch": st_bytes,
+               "avg_launch_us": round(ms_st * 1e3, 3),
+               "note": "working set of one launch (33 MB) is cache-resident"}
+    del F, H, Q, Q2, qb
+    # parity against the oracle over two steps (the oracle takes seconds per step)
+    orc = TiledQgmOracle(case.dx, case.dx, case.dt, (ny, nx), c=case.c, f=case.f0,
+                         SSH=np.zeros((ny, nx)), time_scheme="rk2", tile_size=ts, tile_overlap=ov)
+    k = 2
+    t1 = time.perf_counter()
+    ref_h = orc.step(h0n, hbn, k)
+    ref_a = orc.step_adj(adn, h0n, hbn, k)
+    cpu_s = time.perf_counter() - t1
+    gpu_h = m.step(h0n, hbn, k)
+    gpu_a = m.step_adj(adn, h0n, hbn, k)
+    err_h = float(np.linalg.norm(gpu_h - ref_h) / np.linalg.norm(ref_h))
+    err_a = float(np.linalg.norm(gpu_a - ref_a) / np.linalg.norm(ref_a))
+    log(f"tiled parity vs oracle ({k} steps): step {err_h:.2e}, step_adj {err_a:.2e}; "
+        f"oracle {cpu_s:.1f} s")
+    # the same calls tile by tile
+    one = QgmWithTiles(**kw, engine="plans")
+    k1 = 2
+    one.step(h0, hb, k1)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    one.step(h0, hb, k1)
+    one.step_adj(ad, h0, hb, k1)
+    torch.cuda.synchronize()
+    ms_one = (time.perf_counter() - t2) * 1e3
+    del one
+    release(None)
+    return {
+        "metric": f"tiled QG (QgmWithTiles) step + step_adj grid-point-steps/sec, {ny}x{nx}, "
+                  f"{ts}-point tiles overlapping by {ov}",
+        "value": round(value, 1), "unit": "grid-point-steps/s", "n_gpus": 1,
+        "steps": max(3, args.steps), "warmup": 1, "ms_per_step": round(ms_f + ms_a, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded Gaussian random SSH, random boundary field and adjoint input)",
+        "config": {"workload": f"tiled_qg{ny}", "grid": [ny, nx], "dx_m": case.dx,
+                   "dt_s": case.dt, "time_scheme": "rk2", "tile_size": ts, "tile_overlap": ov,
+                   "tiles": m.ntile, "steps_per_call": n, "engine": "ragged group (shared launches)",
+                   "hip_graph": True},
+        "ms_step": round(ms_f, 3), "ms_step_adj": round(ms_a, 3),
+        "roofline": roof, "stencil_roofline": st_roof,
+        "tile_by_tile": {"value": round(ny * nx * k1 / (ms_one * 1e-3), 1),
+                         "ms_step_plus_adj": round(ms_one, 2), "steps_per_call": k1,
+                         "launches": "eager"},
+        "group_over_tile_by_tile": round(value / (ny * nx * k1 / (ms_one * 1e-3)), 2),
+        "parity_vs_oracle": {"steps": k, "step_rel_err": err_h, "step_adj_rel_err": err_a},
+        "cpu_baseline": {"value": round(ny * nx * k / cpu_s, 1), "unit": "grid-point-steps/s",
+                         "cores": 1, "kind": "port",
+                         "sample": f"FP64 tiled oracle, step + step_adj over {k} steps of the "
+                                   f"same tiling ({cpu_s:.1f} s)"}}
+
+
+def ptr_(t):
+    from vardyn_amd.qg import ptr
+    return ptr(t)
+
+
 def release(pb):
     import torch
     if pb is not None and pb.engine is not None:
@@ -442,6 +589,9 @@ def main():
     case = syn.case_basin512()
     if args.problem_set:
         print(json.dumps(problem_set(args)), flush=True)
+        return
+    if args.tiled:
+        print(json.dumps(tiled(args)), flush=True)
         return
     if args.gemm_only:
         from vardyn_amd.qg import QgPlan

@@ -239,6 +239,19 @@ int vd_group_stage_adj(vd_group* group, int nact, const void* h, const void* q, 
                        const void* s1, const void* s2, double a0, double a1, double a2,
                        const void* acc0, const void* acc1, const void* acc2, void* lq_out,
                        void* lh_out, void* stream);
+/* Segment start and end of members 0..nact-1, each as vd_qg_prep /
+ * vd_qg_finish / vd_qg_adj_init / vd_qg_adj_final compute it with the member's
+ * own plan (jqgm.py:528-536, 549-555 and their adjoints), in one launch.
+ * vd_group_prep: h = h0 (+ mdt), q = h2pv(h0, hb) and, where hbm / qb are not
+ * NULL, hbm = hb (+ mdt), qb = h2pv(hb, hb); hb is required (a zero field for
+ * none).  vd_group_adj_final: adh0 = lh + h2pv^T lq at the start state hcur. */
+int vd_group_prep(vd_group* group, int nact, const void* h0, const void* hb, void* h, void* q,
+                  void* hbm, void* qb, void* stream);
+int vd_group_finish(vd_group* group, int nact, const void* h, void* h1, void* stream);
+int vd_group_adj_init(vd_group* group, int nact, const void* adh1, void* lh, void* lq,
+                      void* stream);
+int vd_group_adj_final(vd_group* group, int nact, const void* hcur, const void* lh,
+                       const void* lq, void* adh0, void* stream);
 
 /* ---------------------------------------------------------------------------
  * L3 observation operator (Obsop_interp_l3_jax, mapping/src/obsop.py:375-678).

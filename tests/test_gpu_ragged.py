@@ -68,3 +68,55 @@ def test_ragged_graph_replay_matches_eager(monkeypatch):
             J, G = eng.evaluate(x)
             torch.cuda.synchronize()
             assert torch.equal(J, Je) and torch.equal(G, Ge)
+
+
+def test_group_segment_start_end_equal_plan_calls():
+    """vd_group_prep / finish / adj_init / adj_final on a group of uneven
+    members (land, an MDT, per-member c) equal the members' own plan calls bit
+    for bit."""
+    from vardyn_amd import _lib
+    from vardyn_amd.qg import QgPlan, build_mask, current_stream_ptr, ptr
+    from vardyn_amd.ragged import QgGroup
+    rng = np.random.default_rng(7)
+    shapes = [(40, 72), (66, 34), (24, 52)]  # even interiors: folded sine transforms
+    plans = []
+    for k, (ny, nx) in enumerate(shapes):
+        ssh = np.zeros((ny, nx))
+        ssh[ny // 2, nx // 3] = np.nan
+        mdt = 0.05 * rng.standard_normal((ny, nx)) if k == 1 else None
+        plans.append(QgPlan(ny, nx, 8e3, 9e3, 900.0, 2.0 + 0.4 * k, 1e-4, time_scheme="rk2",
+                            mask=build_mask(ny, nx, ssh), mdt=mdt))
+    g = QgGroup(plans)
+    tot = g.offsets[-1]
+    dev = plans[0].device
+
+    def flat(scale=1.0):
+        return scale * torch.randn(tot, dtype=torch.float64, device=dev)
+
+    def view(t, b):
+        ny, nx = shapes[b]
+        return t[g.offsets[b]:g.offsets[b + 1]].view(ny, nx)
+
+    h0, hb, adh1, lq_in, lh_in = flat(0.1), flat(0.02), flat(), flat(), flat()
+    outs = [torch.full((tot,), 7.0, dtype=torch.float64, device=dev) for _ in range(8)]
+    h, q, hbm, qb, h1, lh, lq, adh0 = outs
+    g.prep(g.n, ptr(h0), ptr(hb), ptr(h), ptr(q), ptr(hbm), ptr(qb))
+    g.finish(g.n, ptr(h), ptr(h1))
+    g.adj_init(g.n, ptr(adh1), ptr(lh), ptr(lq))
+    g.adj_final(g.n, ptr(h), ptr(lh_in), ptr(lq_in), ptr(adh0))
+    s = current_stream_ptr(dev)
+    for b, p in enumerate(plans):
+        e = [torch.empty(shapes[b], dtype=torch.float64, device=dev) for _ in range(6)]
+        _lib.call("vd_qg_prep", p._h, ptr(view(h0, b)), ptr(view(hb, b)), ptr(e[0]), ptr(e[1]), s)
+        _lib.call("vd_qg_prep", p._h, ptr(view(hb, b)), ptr(view(hb, b)), ptr(e[2]), ptr(e[3]), s)
+        _lib.call("vd_qg_finish", p._h, ptr(e[0]), ptr(e[4]), s)
+        for got, want in ((h, e[0]), (q, e[1]), (qb, e[3]), (h1, e[4])):
+            assert torch.equal(view(got, b).isnan(), want.isnan())
+            assert torch.equal(torch.nan_to_num(view(got, b)), torch.nan_to_num(want))
+        # hbm = hb (+ mdt): the second prep's h
+        assert torch.equal(view(hbm, b), e[2])
+        _lib.call("vd_qg_adj_init", p._h, ptr(view(adh1, b)), ptr(e[4]), ptr(e[5]), s)
+        assert torch.equal(view(lh, b), e[4]) and torch.equal(view(lq, b), e[5])
+        _lib.call("vd_qg_adj_final", p._h, ptr(e[0]), ptr(view(lh_in, b)), ptr(view(lq_in, b)),
+                  ptr(e[4]), s)
+        assert torch.equal(view(adh0, b), e[4])

@@ -2,7 +2,9 @@
 against the FP64 tiled oracle (jqgm.py:1033-1258).
 
 Bars: FP64 step, tangent-linear and adjoint within 1e-10 relative of the oracle
-(the sine transforms are GEMMs here and FFTs in the oracle); the gather/blend
+(the sine transforms are GEMMs here and FFTs in the oracle), with the tiles in
+shared launches (ragged group) and tile by tile; the two engines within 1e-12
+of each other on the reference's default tiling (32-point tiles, overlap 16); the gather/blend
 kernels exactly adjoint to each other up to rounding (1e-13); blending the tiles
 of a field gives the field back (1e-14).
 """
@@ -25,7 +27,8 @@ def rel(a, b):
     return np.linalg.norm(a[m] - b[m]) / max(np.linalg.norm(b[m]), 1e-300)
 
 
-def pair(ny, nx, scheme, ts, ov, land=False, mdt=False, kdiff=False, fvar=False, seed=0):
+def pair(ny, nx, scheme, ts, ov, land=False, mdt=False, kdiff=False, fvar=False, seed=0,
+         engine="group"):
     rng = np.random.default_rng(seed)
     dx = 10e3
     ssh = np.zeros((ny, nx))
@@ -40,17 +43,18 @@ def pair(ny, nx, scheme, ts, ov, land=False, mdt=False, kdiff=False, fvar=False,
                          time_scheme=scheme, tile_size=ts, tile_overlap=ov)
     gpu = QgmWithTiles(dx=dx * np.ones((ny, nx)), dy=dx * np.ones((ny, nx)), dt=1200.0, SSH=ssh,
                        c=c, Kdiffus=K, f=f, time_scheme=scheme, mdt=m, tile_size=ts,
-                       tile_overlap=ov)
+                       tile_overlap=ov, engine=engine)
     h0 = 0.05 * rng.standard_normal((ny, nx))
     h0[np.isnan(ssh)] = np.nan
     hb = 0.01 * rng.standard_normal((ny, nx))
     return orc, gpu, h0, hb, rng, np.isnan(ssh)
 
 
+@pytest.mark.parametrize("engine", ["group", "plans"])
 @pytest.mark.parametrize("scheme", ["Euler", "rk2"])
 @pytest.mark.parametrize("opts", [dict(), dict(land=True, mdt=True, kdiff=True, fvar=True)])
-def test_step_tgl_adj_against_oracle(scheme, opts):
-    orc, gpu, h0, hb, rng, land = pair(76, 58, scheme, 32, 8, **opts)
+def test_step_tgl_adj_against_oracle(scheme, opts, engine):
+    orc, gpu, h0, hb, rng, land = pair(76, 58, scheme, 32, 8, engine=engine, **opts)
     n = 6
     h1 = gpu.step(h0, hb, n)
     assert isinstance(h1, np.ndarray)
@@ -89,3 +93,17 @@ def test_gather_blend_kernels():
     _lib.call("vd_tiles_blend_adj", *gpu._dims(), ptr(gpu._w), ptr(gpu._wsum), ptr(y), ptr(bT), s)
     lhs, rhs = float((gpu._blend(F) * y).sum()), float((F * bT).sum())
     assert abs(lhs - rhs) <= 1e-13 * abs(lhs)
+
+
+@pytest.mark.parametrize("scheme", ["Euler", "rk2"])
+def test_group_engine_matches_tile_by_tile(scheme):
+    """The reference's default tiling (tile_size 32, tile_overlap 16; config_default.py:398-400):
+    shared launches against tile by tile, with land, over several steps."""
+    _, grp, h0, hb, rng, land = pair(96, 80, scheme, 32, 16, land=True)
+    _, one = pair(96, 80, scheme, 32, 16, land=True, engine="plans")[:2]
+    assert grp.ntile == one.ntile == 6 * 5
+    n = 5
+    assert rel(grp.step(h0, hb, n), one.step(h0, hb, n)) < 1e-12
+    ad = rng.standard_normal(h0.shape)
+    ad[land] = 0
+    assert rel(grp.step_adj(ad, h0, hb, n), one.step_adj(ad, h0, hb, n)) < 1e-12

@@ -102,3 +102,16 @@ def test_tiled_tangent_adjoint_pair(scheme):
     tl = model.step_tgl(d, h0, hb, 4)
     ok = ~np.isnan(fd)
     assert np.linalg.norm((fd - tl)[ok]) / np.linalg.norm(tl[ok]) < 1e-5
+
+
+def test_plugin_module_exposes_tiled_class():
+    """The reference instantiates getattr(module, config.MOD.name_class) of the
+    jqgm.py it loads by path (mod.py:479-486); name_class may be 'QgmWithTiles'."""
+    import os
+    from importlib.machinery import SourceFileLoader
+    import vardyn_amd
+    path = os.path.join(os.path.dirname(vardyn_amd.__file__), "jqgm.py")
+    mod = SourceFileLoader("jqgm_plugin_check", path).load_module()
+    from vardyn_amd.tiles import QgmWithTiles
+    assert getattr(mod, "QgmWithTiles") is QgmWithTiles
+    assert hasattr(mod, "Qgm")

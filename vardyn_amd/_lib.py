@@ -74,6 +74,10 @@ SIGNATURES = {
     "vd_group_inversion": (_I, [_VP, _I, _VP, _VP, _VP, _VP, _I, _VP]),
     "vd_group_stage_adj": (_I, [_VP, _I, _VP, _VP, _VP, _VP, _VP, _D, _D, _D, _VP, _VP, _VP,
                                 _VP, _VP, _VP]),
+    "vd_group_prep": (_I, [_VP, _I, _VP, _VP, _VP, _VP, _VP, _VP, _VP]),
+    "vd_group_finish": (_I, [_VP, _I, _VP, _VP, _VP]),
+    "vd_group_adj_init": (_I, [_VP, _I, _VP, _VP, _VP, _VP]),
+    "vd_group_adj_final": (_I, [_VP, _I, _VP, _VP, _VP, _VP, _VP]),
     "vd_obs_misfit": (_I, [C.POINTER(ObsTable), _I, _VP, _VP, _VP, _VP, _D, _VP]),
     "vd_obs_adjoint": (_I, [C.POINTER(ObsTable), _I, _VP, _VP, _VP, _VP]),
     "vd_tiles_gather": (_I, [_I, _I, _I, _I, _I, _I, _VP, _VP, _VP, _VP, _VP, _VP]),

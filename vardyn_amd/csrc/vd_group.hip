@@ -116,3 +116,68 @@ int vd_group_stage_adj(vd_group* grp, int nact, const void* h, const void* q, co
   VD_HIP_TRY(hipGetLastError());
   return VD_OK;
 }
+
+// Segment start and end of the members, as vd_qg_prep / vd_qg_finish /
+// vd_qg_adj_init / vd_qg_adj_final compute them for each member's own plan.
+int vd_group_prep(vd_group* grp, int nact, const void* h0, const void* hb, void* h, void* q,
+                  void* hbm, void* qb, void* stream) {
+  VD_GROUP_CHECK(grp, nact);
+  VD_REQUIRE(h0 && hb && h && q, VD_ERR_INVALID_ARG, "null argument");
+  for (const void* out : {(const void*)h, (const void*)q, (const void*)hbm, (const void*)qb})
+    VD_REQUIRE(out != h0 && out != hb, VD_ERR_INVALID_ARG,
+               "outputs must not alias h0 or hb (h2pv reads them at neighbouring points)");
+  const Group& g = grp->g;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (g.prec == VD_F64)
+    launch_prep_rg<double>(g, nact, (const double*)h0, (const double*)hb, (double*)h, (double*)hbm,
+                           (double*)q, (double*)qb, s);
+  else
+    launch_prep_rg<float>(g, nact, (const float*)h0, (const float*)hb, (float*)h, (float*)hbm,
+                          (float*)q, (float*)qb, s);
+  VD_HIP_TRY(hipGetLastError());
+  return VD_OK;
+}
+
+int vd_group_finish(vd_group* grp, int nact, const void* h, void* h1, void* stream) {
+  VD_GROUP_CHECK(grp, nact);
+  VD_REQUIRE(h && h1, VD_ERR_INVALID_ARG, "null argument");
+  const Group& g = grp->g;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (g.prec == VD_F64)
+    launch_finish_rg<double>(g, nact, (const double*)h, (double*)h1, s);
+  else
+    launch_finish_rg<float>(g, nact, (const float*)h, (float*)h1, s);
+  VD_HIP_TRY(hipGetLastError());
+  return VD_OK;
+}
+
+int vd_group_adj_init(vd_group* grp, int nact, const void* adh1, void* lh, void* lq, void* stream) {
+  VD_GROUP_CHECK(grp, nact);
+  VD_REQUIRE(adh1 && lh && lq, VD_ERR_INVALID_ARG, "null argument");
+  const Group& g = grp->g;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (g.prec == VD_F64)
+    launch_adj_init_rg<double>(g, nact, (const double*)adh1, (double*)lh, (double*)lq, s);
+  else
+    launch_adj_init_rg<float>(g, nact, (const float*)adh1, (float*)lh, (float*)lq, s);
+  VD_HIP_TRY(hipGetLastError());
+  return VD_OK;
+}
+
+int vd_group_adj_final(vd_group* grp, int nact, const void* hcur, const void* lh, const void* lq,
+                       void* adh0, void* stream) {
+  VD_GROUP_CHECK(grp, nact);
+  VD_REQUIRE(hcur && lh && lq && adh0, VD_ERR_INVALID_ARG, "null argument");
+  VD_REQUIRE(adh0 != lq, VD_ERR_INVALID_ARG,
+             "adh0 must not alias lq (h2pv^T reads it at neighbouring points)");
+  const Group& g = grp->g;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (g.prec == VD_F64)
+    launch_adj_final_rg<double>(g, nact, (const double*)hcur, (const double*)lh,
+                                (const double*)lq, (double*)adh0, s);
+  else
+    launch_adj_final_rg<float>(g, nact, (const float*)hcur, (const float*)lh, (const float*)lq,
+                               (float*)adh0, s);
+  VD_HIP_TRY(hipGetLastError());
+  return VD_OK;
+}

@@ -124,7 +124,8 @@ struct Plan {
 template <typename T>
 struct RgMember {
   Phys<T> P;
-  size_t foff;  // element offset of the member's fields
+  size_t foff;         // element offset of the member's fields
+  const T* mdt;        // the member's MDT field, or NULL
 };
 struct Group {
   int n = 0, prec = 0, scheme = 0, nstage = 0, fold = 0;
@@ -236,6 +237,18 @@ template <typename T>
 void launch_stage_adj_rg(const Group& g, int nact, const T* h, const T* q, const T* s0,
                          const T* s1, const T* s2, T a0, T a1, T a2, const T* acc0,
                          const T* acc1, const T* acc2, T* lq_out, T* lh_out, hipStream_t s);
+// Segment start/end of members 0..nact-1 of a group (the plan launches'
+// arithmetic per member; hbm and qb may be NULL).
+template <typename T>
+void launch_prep_rg(const Group& g, int nact, const T* h0, const T* hb, T* hcur, T* hbm, T* q0,
+                    T* qb, hipStream_t s);
+template <typename T>
+void launch_finish_rg(const Group& g, int nact, const T* h, T* h1, hipStream_t s);
+template <typename T>
+void launch_adj_init_rg(const Group& g, int nact, const T* adh1, T* lh, T* lq, hipStream_t s);
+template <typename T>
+void launch_adj_final_rg(const Group& g, int nact, const T* hcur, const T* lh, const T* lq,
+                         T* adh0, hipStream_t s);
 // lh = adh1 with zeros on land; lq = 0.
 template <typename T>
 void launch_adj_init(const Plan& p, const T* adh1, T* lh, T* lq, hipStream_t s);

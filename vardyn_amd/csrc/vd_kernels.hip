@@ -597,19 +597,23 @@ __device__ __forceinline__ T h2pv_at(const T* h, const T* hb, int i, int j, uint
   return nz(lap - P.gfc2_b[blockIdx.z] * c);
 }
 
+// Point-wise bodies of the segment start/end kernels, shared by the plan
+// kernels (grid over columns, rows and batch members) and the ragged-group
+// kernels (tile table over members).  Field pointers are already offset to the
+// member's field; hbm and qb of the segment start may be NULL (not wanted).
 template <typename T>
-__global__ void k_prep(Phys<T> P, const T* __restrict__ h0, const T* __restrict__ hb,
-                       const T* __restrict__ mdt, T* __restrict__ hcur, T* __restrict__ hbm,
-                       T* __restrict__ q0, T* __restrict__ qb) {
-  VD_POINT_2D
-  VD_B(h0); VD_B(hb); VD_B(hcur); VD_B(hbm); VD_B(q0); VD_B(qb);
+__device__ __forceinline__ void prep_point(const Phys<T>& P, int i, int j, const T* h0,
+                                           const T* hb, const T* mdt, T* hcur, T* hbm, T* q0,
+                                           T* qb) {
+  const int nx = P.nx;
+  const size_t o = (size_t)i * nx + j;
   const uint8_t m = P.mask[o];
   if (mdt) {
     // q needs h + mdt at neighbours: evaluate the stencil on the fly.
     auto H = [&](int ii, int jj) { return h0[ii * nx + jj] + mdt[ii * nx + jj]; };
     auto HB = [&](int ii, int jj) { return hb[ii * nx + jj] + mdt[ii * nx + jj]; };
     hcur[o] = H(i, j);
-    hbm[o] = HB(i, j);
+    if (hbm) hbm[o] = HB(i, j);
     T qv = 0, qbv = 0;
     if (m == 3 || m == 4) {
       const T c = H(i, j), cb = HB(i, j);
@@ -622,13 +626,22 @@ __global__ void k_prep(Phys<T> P, const T* __restrict__ h0, const T* __restrict_
       qbv = qv;
     }
     q0[o] = qv;
-    qb[o] = qbv;
+    if (qb) qb[o] = qbv;
   } else {
     hcur[o] = h0[o];
-    hbm[o] = hb[o];
+    if (hbm) hbm[o] = hb[o];
     q0[o] = h2pv_at(h0, hb, i, j, m, P);
-    qb[o] = h2pv_at(hb, hb, i, j, m, P);
+    if (qb) qb[o] = h2pv_at(hb, hb, i, j, m, P);
   }
+}
+
+template <typename T>
+__global__ void k_prep(Phys<T> P, const T* __restrict__ h0, const T* __restrict__ hb,
+                       const T* __restrict__ mdt, T* __restrict__ hcur, T* __restrict__ hbm,
+                       T* __restrict__ q0, T* __restrict__ qb) {
+  VD_POINT_2D
+  VD_B(h0); VD_B(hb); VD_B(hcur); VD_B(hbm); VD_B(q0); VD_B(qb);
+  prep_point(P, i, j, h0, hb, mdt, hcur, hbm, q0, qb);
 }
 
 template <typename T>
@@ -647,11 +660,18 @@ __global__ void k_fill_land(Phys<T> P, T* __restrict__ f, T value) {
 }
 
 template <typename T>
+__device__ __forceinline__ void finish_point(const Phys<T>& P, int i, int j, const T* h,
+                                             const T* mdt, T* h1) {
+  const size_t o = (size_t)i * P.nx + j;
+  h1[o] = P.mask[o] == 0 ? T(NAN) : (mdt ? h[o] - mdt[o] : h[o]);
+}
+
+template <typename T>
 __global__ void k_finish(Phys<T> P, const T* __restrict__ h, const T* __restrict__ mdt,
                          T* __restrict__ h1) {
   VD_POINT_2D
   VD_B(h); VD_B(h1);
-  h1[o] = P.mask[o] == 0 ? T(NAN) : (mdt ? h[o] - mdt[o] : h[o]);
+  finish_point(P, i, j, h, mdt, h1);
 }
 
 // Fused RK stage: qout = qbase + c0 k0 + c1 k1 + c2 rhs(h, q) (rhs stored in
@@ -976,20 +996,26 @@ __global__ __launch_bounds__(kAX * kAY) void k_adj_fused_rg(
 
 
 template <typename T>
-__global__ void k_adj_init(Phys<T> P, const T* __restrict__ adh1, T* __restrict__ lh,
-                           T* __restrict__ lq) {
-  VD_POINT_2D
-  VD_B(adh1); VD_B(lh); VD_B(lq);
+__device__ __forceinline__ void adj_init_point(const Phys<T>& P, int i, int j, const T* adh1,
+                                               T* lh, T* lq) {
+  const size_t o = (size_t)i * P.nx + j;
   lh[o] = P.mask[o] == 0 ? T(0) : adh1[o];
   lq[o] = T(0);
 }
 
 template <typename T>
-__global__ void k_adj_final(Phys<T> P, const T* __restrict__ hcur, const T* __restrict__ lh,
-                            const T* __restrict__ lq, T* __restrict__ adh0) {
+__global__ void k_adj_init(Phys<T> P, const T* __restrict__ adh1, T* __restrict__ lh,
+                           T* __restrict__ lq) {
   VD_POINT_2D
-  VD_B(hcur); VD_B(lh); VD_B(lq); VD_B(adh0);
-  const int ny = P.ny;
+  VD_B(adh1); VD_B(lh); VD_B(lq);
+  adj_init_point(P, i, j, adh1, lh, lq);
+}
+
+template <typename T>
+__device__ __forceinline__ void adj_final_point(const Phys<T>& P, int i, int j, const T* hcur,
+                                                const T* lh, const T* lq, T* adh0) {
+  const int nx = P.nx, ny = P.ny;
+  const size_t o = (size_t)i * nx + j;
   // a(t) = lq(t) where h2pv used its Laplacian formula (mask 3 or 4, q not NaN)
   auto A = [&](int a, int b) -> T {
     if (a < 1 || a > ny - 2 || b < 1 || b > nx - 2) return T(0);
@@ -1000,6 +1026,69 @@ __global__ void k_adj_final(Phys<T> P, const T* __restrict__ hcur, const T* __re
   const T r = P.gf_dy2 * (A(i + 1, j) + A(i - 1, j)) + P.gf_dx2 * (A(i, j + 1) + A(i, j - 1)) +
               (-2 * P.gf_dy2 - 2 * P.gf_dx2 - P.gfc2_b[blockIdx.z]) * A(i, j);
   adh0[o] = lh[o] + r;
+}
+
+template <typename T>
+__global__ void k_adj_final(Phys<T> P, const T* __restrict__ hcur, const T* __restrict__ lh,
+                            const T* __restrict__ lq, T* __restrict__ adh0) {
+  VD_POINT_2D
+  VD_B(hcur); VD_B(lh); VD_B(lq); VD_B(adh0);
+  adj_final_point(P, i, j, hcur, lh, lq, adh0);
+}
+
+// Ragged-group versions of the segment start/end kernels: block b takes the
+// 64 x 8 point tile stage_tiles[b] of its member (64 x 4 threads, two rows
+// each) with the member's constants, MDT and field offset.  Every point runs
+// the same body as in the member's own plan; the member's gfc2_b has one
+// entry and blockIdx.z is 0 here, so the bodies read the member's value.
+#define VD_POINT_RG                                              \
+  const int4 t = tiles[blockIdx.x];                              \
+  const Phys<T> P = mem[t.x].P;                                  \
+  const T* mdt = mem[t.x].mdt;                                   \
+  (void)mdt;                                                     \
+  const size_t zo = mem[t.x].foff;
+#define VD_POINT_RG_LOOP(BODY)                                   \
+  const int j = t.z * kSX + threadIdx.x;                         \
+  _Pragma("unroll") for (int rr = 0; rr < kSY / 4; ++rr) {       \
+    const int i = t.y * kSY + threadIdx.y + 4 * rr;              \
+    if (i < P.ny && j < P.nx) BODY;                              \
+  }
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_prep_rg(const RgMember<T>* __restrict__ mem,
+                                                 const int4* __restrict__ tiles, const T* h0,
+                                                 const T* hb, T* hcur, T* hbm, T* q0, T* qb) {
+  VD_POINT_RG
+  VD_B(h0); VD_B(hb); VD_B(hcur); VD_B(hbm); VD_B(q0); VD_B(qb);
+  VD_POINT_RG_LOOP(prep_point(P, i, j, h0, hb, mdt, hcur, hbm, q0, qb))
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_finish_rg(const RgMember<T>* __restrict__ mem,
+                                                   const int4* __restrict__ tiles, const T* h,
+                                                   T* h1) {
+  VD_POINT_RG
+  VD_B(h); VD_B(h1);
+  VD_POINT_RG_LOOP(finish_point(P, i, j, h, mdt, h1))
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_adj_init_rg(const RgMember<T>* __restrict__ mem,
+                                                     const int4* __restrict__ tiles,
+                                                     const T* adh1, T* lh, T* lq) {
+  VD_POINT_RG
+  VD_B(adh1); VD_B(lh); VD_B(lq);
+  VD_POINT_RG_LOOP(adj_init_point(P, i, j, adh1, lh, lq))
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_adj_final_rg(const RgMember<T>* __restrict__ mem,
+                                                      const int4* __restrict__ tiles,
+                                                      const T* hcur, const T* lh, const T* lq,
+                                                      T* adh0) {
+  VD_POINT_RG
+  VD_B(hcur); VD_B(lh); VD_B(lq); VD_B(adh0);
+  VD_POINT_RG_LOOP(adj_final_point(P, i, j, hcur, lh, lq, adh0))
 }
 
 // ---------------------------------------------------------------------------
@@ -1140,6 +1229,31 @@ void launch_stage_rg(const Group& g, int nact, const T* h, const T* q, const T* 
                      g.stage_tiles, h, q, qbase, kprev0, kprev1, c0, c1, c2, qout, kstore);
 }
 
+#define VD_RG_LAUNCH(KERNEL, ...)                                                      \
+  const int nb = g.stage_prefix[nact];                                                 \
+  if (nb == 0) return;                                                                 \
+  hipLaunchKernelGGL(KERNEL<T>, dim3(nb), kBlock2d, 0, s,                              \
+                     static_cast<const RgMember<T>*>(g.mem), g.stage_tiles, __VA_ARGS__)
+
+template <typename T>
+void launch_prep_rg(const Group& g, int nact, const T* h0, const T* hb, T* hcur, T* hbm, T* q0,
+                    T* qb, hipStream_t s) {
+  VD_RG_LAUNCH(k_prep_rg, h0, hb, hcur, hbm, q0, qb);
+}
+template <typename T>
+void launch_finish_rg(const Group& g, int nact, const T* h, T* h1, hipStream_t s) {
+  VD_RG_LAUNCH(k_finish_rg, h, h1);
+}
+template <typename T>
+void launch_adj_init_rg(const Group& g, int nact, const T* adh1, T* lh, T* lq, hipStream_t s) {
+  VD_RG_LAUNCH(k_adj_init_rg, adh1, lh, lq);
+}
+template <typename T>
+void launch_adj_final_rg(const Group& g, int nact, const T* hcur, const T* lh, const T* lq,
+                         T* adh0, hipStream_t s) {
+  VD_RG_LAUNCH(k_adj_final_rg, hcur, lh, lq, adh0);
+}
+
 template <typename T>
 void launch_stage_adj_rg(const Group& g, int nact, const T* h, const T* q, const T* s0,
                          const T* s1, const T* s2, T a0, T a1, T a2, const T* acc0,
@@ -1196,6 +1310,7 @@ int group_build(Group& g) {
     g.foff[b + 1] = g.foff[b] + p.nf;
     mem[b].P = make_phys<T>(p);
     mem[b].foff = g.foff[b];
+    mem[b].mdt = p.has_mdt ? static_cast<const T*>(p.mdt) : nullptr;
     for (int ty = 0; ty < (p.ny + kSY - 1) / kSY; ++ty)
       for (int tx = 0; tx < (p.nx + kSX - 1) / kSX; ++tx) st.push_back(make_int4(b, ty, tx, 0));
     for (int ty = 0; ty < (p.ny + kAY - 1) / kAY; ++ty)
@@ -1286,6 +1401,12 @@ void group_release(Group& g) {
                                     const T*, T, T, T, const T*, const T*, const T*, T*, T*,  \
                                     hipStream_t);                                             \
   template void launch_adj_init<T>(const Plan&, const T*, T*, T*, hipStream_t);               \
+  template void launch_prep_rg<T>(const Group&, int, const T*, const T*, T*, T*, T*, T*,       \
+                                  hipStream_t);                                               \
+  template void launch_finish_rg<T>(const Group&, int, const T*, T*, hipStream_t);             \
+  template void launch_adj_init_rg<T>(const Group&, int, const T*, T*, T*, hipStream_t);       \
+  template void launch_adj_final_rg<T>(const Group&, int, const T*, const T*, const T*, T*,    \
+                                       hipStream_t);                                          \
   template void launch_adj_final<T>(const Plan&, const T*, const T*, const T*, T*,            \
                                     hipStream_t);
 

@@ -13,6 +13,10 @@ Arrays may be numpy (results are returned as numpy) or torch tensors
 reference (mapping/src/config.py:6-8): float32 unless ``precision='float64'``
 is given or the environment variable ``VARDYN_FLOAT64=1`` is set.
 
+``QgmWithTiles`` (jqgm.py:1033-1258, ``name_class='QgmWithTiles'``) is the
+tiled model of ``vardyn_amd.tiles``; it computes in FP64 unless
+``precision='float32'`` is given.
+
 This module uses absolute imports only, so it also works when loaded by
 path as a top-level module.
 """
@@ -24,6 +28,7 @@ import numpy as np
 import torch
 
 from vardyn_amd.qg import QgPlan, build_mask
+from vardyn_amd.tiles import QgmWithTiles  # noqa: F401  (name_class='QgmWithTiles')
 
 
 def _default_precision():

@@ -77,6 +77,18 @@ class QgGroup:
         _lib.call("vd_group_stage_adj", self._h, int(nact), h, q, s0, s1, s2, float(a0),
                   float(a1), float(a2), acc0, acc1, acc2, lq_out, lh_out, self._s())
 
+    def prep(self, nact, h0, hb, h, q, hbm=None, qb=None):
+        _lib.call("vd_group_prep", self._h, int(nact), h0, hb, h, q, hbm, qb, self._s())
+
+    def finish(self, nact, h, h1):
+        _lib.call("vd_group_finish", self._h, int(nact), h, h1, self._s())
+
+    def adj_init(self, nact, adh1, lh, lq):
+        _lib.call("vd_group_adj_init", self._h, int(nact), adh1, lh, lq, self._s())
+
+    def adj_final(self, nact, hcur, lh, lq, adh0):
+        _lib.call("vd_group_adj_final", self._h, int(nact), hcur, lh, lq, adh0, self._s())
+
 
 class RaggedFourDVar:
     """Cost and gradient of every problem of ``problems`` in one step-synchronous sweep.
