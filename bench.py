@@ -185,14 +185,15 @@ def gemm_roofline(plan, precision, n_inv=100, reps=5):
             "executed_tflops": round(executed, 3), "executed_frac": round(executed / peak, 4)}
 
 
-def stencil_roofline(case, precision, nbatch=64, n_launch=20, reps=5):
+def stencil_roofline(case, precision, nbatch=64, n_launch=20, reps=5, rounds_n=5):
     """HBM roofline of the fused RK stage stencil (``k_stage`` behind
     ``vd_qg_rhs``: out = base + dt * rhs(h, q)) on ``nbatch`` problems of the
     workload's grid at once, so that its working set (three input fields and
     one output per problem) is far larger than the 256 MB Infinity Cache.
     Algorithmic bytes per point: read h, q, base and write out (4 words) plus
     the shared int8 mask once per launch; base is a field of its own (as in
-    the second RK2 stage), so all four streams go to memory."""
+    the second RK2 stage), so all four streams go to memory.  The launch time is
+    the median of ``rounds_n`` timed rounds of ``reps`` graph replays."""
     import torch
     from vardyn_amd.qg import QgPlan
     plan = QgPlan(case.ny, case.nx, case.dx, case.dx, case.dt, case.c, case.f0,
@@ -213,12 +214,15 @@ def stencil_roofline(case, precision, nbatch=64, n_launch=20, reps=5):
     torch.cuda.synchronize()
     s = torch.cuda.current_stream()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record(s)
-    for _ in range(reps):
-        graph.replay()
-    e1.record(s)
-    torch.cuda.synchronize()
-    per_launch_s = e0.elapsed_time(e1) * 1e-3 / (n_launch * reps)
+    rounds = []
+    for _ in range(rounds_n):
+        e0.record(s)
+        for _ in range(reps):
+            graph.replay()
+        e1.record(s)
+        torch.cuda.synchronize()
+        rounds.append(e0.elapsed_time(e1) * 1e-3 / (n_launch * reps))
+    per_launch_s = sorted(rounds)[len(rounds) // 2]  # median round
     npts = case.ny * case.nx
     nbytes = nbatch * npts * 4 * q.element_size() + npts
     gbs = nbytes / per_launch_s / 1e9
@@ -227,7 +231,8 @@ def stencil_roofline(case, precision, nbatch=64, n_launch=20, reps=5):
     return {"bound": "hbm", "kernel": "k_stage (fused h2uv + upwind advection + RK stage update)",
             "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(gbs / HBM_PEAK_GBS, 4), "batch": nbatch,
-            "bytes_per_launch": nbytes, "avg_launch_us": round(per_launch_s * 1e6, 2)}
+            "bytes_per_launch": nbytes, "avg_launch_us": round(per_launch_s * 1e6, 2),
+            "rounds_us": [round(r * 1e6, 2) for r in rounds]}
 
 
 def copy_calibration(nbytes=256 << 20):

@@ -92,3 +92,24 @@ def test_problem_set_equals_standalone_engines():
         eng = build_problem(c, precision="float64", control="ic", tape="full").engine
         Jr, Gr = eng.run(x)
         assert torch.equal(J, Jr) and torch.equal(G, Gr)
+
+
+def test_banded_stencil_order_equals_single():
+    """A batched plan whose stencil fields exceed the 256 MB Infinity Cache runs
+    the stage and adjoint-stage stencils in the banded XCD tile order; its
+    members stay bit-identical to single plans (row-major order)."""
+    B, ny, nx, n = 3, 1536, 2048, 2
+    assert B * ny * nx * 8 * 4 > 256 << 20 and ny * nx * 8 * 6 < 256 << 20
+    cs = [2.0, 2.4, 2.9]
+    rng = np.random.default_rng(1)
+    h0 = 0.1 * rng.standard_normal((B, ny, nx))
+    hb = 0.01 * rng.standard_normal((B, ny, nx))
+    ay = rng.standard_normal((B, ny, nx))
+    bp = QgPlan(ny, nx, 5e3, 5e3, 900.0, cs, 1e-4, time_scheme="rk2", nbatch=B, gemm_config=1)
+    H1 = bp.step(bp.field(h0), bp.field(hb), n)
+    AD = bp.step_adj(bp.field(ay), bp.field(h0), bp.field(hb), n)
+    del bp
+    for b in range(B):
+        sp = QgPlan(ny, nx, 5e3, 5e3, 900.0, cs[b], 1e-4, time_scheme="rk2", gemm_config=1)
+        assert torch.equal(H1[b], sp.step(sp.field(h0[b]), sp.field(hb[b]), n))
+        assert torch.equal(AD[b], sp.step_adj(sp.field(ay[b]), sp.field(h0[b]), sp.field(hb[b]), n))

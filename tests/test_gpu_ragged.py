@@ -120,3 +120,44 @@ def test_group_segment_start_end_equal_plan_calls():
         _lib.call("vd_qg_adj_final", p._h, ptr(e[0]), ptr(view(lh_in, b)), ptr(view(lq_in, b)),
                   ptr(e[4]), s)
         assert torch.equal(view(adh0, b), e[4])
+
+
+def test_banded_group_tables_equal_row_major():
+    """A group whose members' stencil fields exceed the 256 MB Infinity Cache
+    lays its stencil tiles out in the banded XCD order (with padding entries);
+    every member's stage, adjoint stage and segment start equal those of a
+    one-member group (row-major table) bit for bit."""
+    from vardyn_amd.qg import QgPlan, ptr
+    from vardyn_amd.ragged import QgGroup
+    ny, nx = 1536, 2048
+    plans = [QgPlan(ny, nx, 5e3, 5e3, 900.0, 2.0 + 0.3 * k, 1e-4, time_scheme="rk2",
+                    gemm_config=1) for k in range(3)]
+    assert 3 * ny * nx * 8 * 4 > 256 << 20 and ny * nx * 8 * 4 < 256 << 20
+    g3 = QgGroup(plans)
+    tot, m = g3.offsets[-1], ny * nx
+    gen = torch.Generator(device="cuda").manual_seed(9)
+
+    def rnd(scale=1.0):
+        return scale * torch.randn(tot, dtype=torch.float64, device="cuda", generator=gen)
+
+    h, hb, s0 = rnd(0.1), rnd(0.01), rnd()
+    q, hc = torch.empty_like(h), torch.empty_like(h)  # named: the calls must not see freed blocks
+    g3.prep(3, ptr(h), ptr(hb), ptr(hc), ptr(q))
+    out = [torch.empty_like(h) for _ in range(3)]
+    g3.stage(3, ptr(h), ptr(q), ptr(q), None, None, 0, 0, 900.0, ptr(out[0]))
+    g3.stage_adj(3, ptr(h), ptr(q), ptr(s0), None, None, 450.0, 0, 0, ptr(s0), None, None,
+                 ptr(out[1]), ptr(out[2]))
+    for b, p in enumerate(plans):
+        g1 = QgGroup([p])
+        sl = slice(b * m, (b + 1) * m)
+        h1, q1, s1 = h[sl].clone(), torch.empty(m, dtype=torch.float64, device="cuda"), s0[sl].clone()
+        hb1, hc1 = hb[sl].clone(), torch.empty_like(h1)
+        g1.prep(1, ptr(h1), ptr(hb1), ptr(hc1), ptr(q1))
+        assert torch.equal(q1, q[sl])
+        o = [torch.empty_like(h1) for _ in range(3)]
+        g1.stage(1, ptr(h1), ptr(q1), ptr(q1), None, None, 0, 0, 900.0, ptr(o[0]))
+        g1.stage_adj(1, ptr(h1), ptr(q1), ptr(s1), None, None, 450.0, 0, 0, ptr(s1), None, None,
+                     ptr(o[1]), ptr(o[2]))
+        for k in range(3):
+            assert torch.equal(o[k], out[k][sl]), (b, k)
+        del g1

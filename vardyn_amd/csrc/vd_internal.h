@@ -132,7 +132,7 @@ struct Group {
   std::vector<Plan*> plans;   // not owned
   std::vector<size_t> foff;   // n + 1 prefix offsets
   void* mem = nullptr;        // RgMember<T>[n]
-  int4* stage_tiles = nullptr;  // (member, tile row, tile column) of 64 x 8 point tiles
+  int4* stage_tiles = nullptr;  // (member, tile row, tile column) of 64 x 16 point tiles
   std::vector<int> stage_prefix;
   int4* adj_tiles = nullptr;    // the same for 64 x 16 tiles of the adjoint stage
   std::vector<int> adj_prefix;

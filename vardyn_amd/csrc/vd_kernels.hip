@@ -676,16 +676,21 @@ __global__ void k_finish(Phys<T> P, const T* __restrict__ h, const T* __restrict
 
 // Fused RK stage: qout = qbase + c0 k0 + c1 k1 + c2 rhs(h, q) (rhs stored in
 // kstore when given); the mask selects rhs (active points) or zero.
-// LDS-tiled: a 64 x 8 tile of points per workgroup (64 x 4 threads, two
-// rows each).  h and q of the tile and its 2-point halo are loaded once into
+// LDS-tiled: a 64 x kSY tile of points per workgroup (64 x 4 threads, kSY / 4
+// rows each; 8 rows where the fields fit in cache, 16 rows on HBM-bound
+// batches, where the taller tile halves the halo rows read per point: each
+// measured faster than the other there).  h and q of the tile and its 2-point halo are loaded once into
 // LDS (rows and columns clamped into the field, so always in bounds); every
 // stencil read then comes from LDS, which leaves HBM as the bound (one global
 // read per point of h and q instead of about twenty).  Points outside the
 // active band (every active point lies in rows/columns 2..n-3) read halo
 // values that the mask then discards; the optional beta and f*bathymetry
 // fields are read at the point clamped into that band.
-constexpr int kSX = 64, kSY = 8, kSP = kSX + 4;
-template <typename T>
+// kSY rows for launches whose fields fit in the Infinity Cache (and the
+// ragged-group tables), kSYB for banded, HBM-bound launches (below).
+constexpr int kSX = 64, kSY = 8, kSYB = 16, kSP = kSX + 4;
+constexpr int kSCX = kSX / 64;  // columns per thread (64 threads across)
+template <typename T, int kSY>
 __device__ __forceinline__ void stage_tile(const Phys<T>& P, int i0, int j0, const T* h,
                                            const T* q, const T* qbase, const T* k0,
                                            const T* k1, T c0, T c1, T c2, T* qout, T* kstore) {
@@ -696,14 +701,17 @@ __device__ __forceinline__ void stage_tile(const Phys<T>& P, int i0, int j0, con
   // this thread's point-wise inputs, issued before the tile loads so that
   // they are in flight together (clamped: always in bounds).  q_base and the
   // output are streamed once per launch: non-temporal load and store.
-  const int j = j0 + threadIdx.x;
-  T base[kSY / 4];
-  uint8_t msk[kSY / 4];
+  T base[kSCX][kSY / 4];
+  uint8_t msk[kSCX][kSY / 4];
 #pragma unroll
-  for (int rr = 0; rr < kSY / 4; ++rr) {
-    const size_t o = (size_t)min(i0 + (int)threadIdx.y + 4 * rr, ny - 1) * nx + min(j, nx - 1);
-    base[rr] = __builtin_nontemporal_load(&qbase[o]);
-    msk[rr] = P.mask[o];
+  for (int cx = 0; cx < kSCX; ++cx) {
+    const int j = j0 + threadIdx.x + 64 * cx;
+#pragma unroll
+    for (int rr = 0; rr < kSY / 4; ++rr) {
+      const size_t o = (size_t)min(i0 + (int)threadIdx.y + 4 * rr, ny - 1) * nx + min(j, nx - 1);
+      base[cx][rr] = __builtin_nontemporal_load(&qbase[o]);
+      msk[cx][rr] = P.mask[o];
+    }
   }
   for (int e = tid; e < (kSY + 4) * kSP; e += 256) {
     const int r = e / kSP, c = e % kSP;
@@ -712,33 +720,79 @@ __device__ __forceinline__ void stage_tile(const Phys<T>& P, int i0, int j0, con
     qs[e] = q[g];
   }
   __syncthreads();
-  if (j >= nx) return;
 #pragma unroll
-  for (int rr = 0; rr < kSY / 4; ++rr) {
-    const int li = threadIdx.y + 4 * rr, i = i0 + li;
-    if (i >= ny) break;
-    const size_t o = (size_t)i * nx + j;
-    const int ic = min(max(i, 2), ny - 3), jc = min(max(j, 2), nx - 3);  // global optional fields
-    const T kk = rhs_view(hs, qs, kSP, li + 2, threadIdx.x + 2, ic, jc, P);
-    const T k = msk[rr] == 3 ? kk : T(0);
-    if (kstore) kstore[o] = k;
-    T v = base[rr];
-    if (k0) v += c0 * k0[o];
-    if (k1) v += c1 * k1[o];
-    __builtin_nontemporal_store(v + c2 * k, &qout[o]);
+  for (int cx = 0; cx < kSCX; ++cx) {
+    const int lj = threadIdx.x + 64 * cx, j = j0 + lj;
+    if (j >= nx) break;
+#pragma unroll
+    for (int rr = 0; rr < kSY / 4; ++rr) {
+      const int li = threadIdx.y + 4 * rr, i = i0 + li;
+      if (i >= ny) break;
+      const size_t o = (size_t)i * nx + j;
+      const int ic = min(max(i, 2), ny - 3), jc = min(max(j, 2), nx - 3);  // global optional fields
+      const T kk = rhs_view(hs, qs, kSP, li + 2, lj + 2, ic, jc, P);
+      const T k = msk[cx][rr] == 3 ? kk : T(0);
+      if (kstore) kstore[o] = k;
+      T v = base[cx][rr];
+      if (k0) v += c0 * k0[o];
+      if (k1) v += c1 * k1[o];
+      __builtin_nontemporal_store(v + c2 * k, &qout[o]);
+    }
   }
 }
 
-template <typename T>
+// Banded XCD order of the stencil launches.  The GPU deals the workgroups of
+// a 1-D grid round-robin to its 8 XCDs (block b on XCD b % 8), each with its
+// own L2.  In row-major tile order the left and right neighbours of a tile
+// run on other XCDs, so the cache lines of their shared halo columns are
+// fetched from memory twice (measured: 1.34 x the algorithmic reads at 64 x
+// 512^2).  Here the 8 XCDs work on one problem at a time, XCD x taking the
+// x-th contiguous eighth of its row-major tiles: a band of rows in which
+// every left/right and most upper/lower neighbours share its L2 (measured:
+// 1.04 x), while all XCDs still stream through the same problem together.
+// The last band of a problem may hold fewer tiles; its spare blocks exit.
+// The order pays where the fields stream from HBM; when they fit in the
+// 256 MB Infinity Cache, the plain (columns, rows, members) grid measured
+// faster (8 x 512^2: adjoint stage 53.4 against 55.7 us), so the launcher
+// picks it by working set (band = 0: the tile is the 3-D block index).
+template <bool BAND>
+__device__ __forceinline__ bool band_tile(int gx, int gy, int* bx, int* by, int* bz) {
+  if constexpr (!BAND) {
+    *bx = blockIdx.x;
+    *by = blockIdx.y;
+    *bz = blockIdx.z;
+    return true;
+  } else {
+    const int T = gx * gy, G8 = (T + 7) >> 3;
+    const int b = blockIdx.x, x = b & 7, k = b >> 3;
+    const int z = k / G8, t = x * G8 + (k - z * G8);
+    if (t >= T) return false;
+    *bx = t % gx;
+    *by = t / gx;
+    *bz = z;
+    return true;
+  }
+}
+static inline dim3 band_grid(int band, int gx, int gy, int nb) {
+  return band ? dim3(8 * ((gx * gy + 7) / 8) * nb) : dim3(gx, gy, nb);
+}
+// Banded order when the fields a launch touches exceed the Infinity Cache.
+static inline int band_for(const Plan& p, int fields, size_t elem) {
+  return (double)p.nb * (double)p.nf * (double)elem * fields > 256.0 * (1 << 20) ? 1 : 0;
+}
+
+template <typename T, int KSY, bool BAND>
 __global__ __launch_bounds__(256) void k_stage(Phys<T> P, const T* __restrict__ h,
                                                const T* __restrict__ q,
                                                const T* __restrict__ qbase,
                                                const T* __restrict__ k0,
                                                const T* __restrict__ k1, T c0, T c1, T c2,
                                                T* __restrict__ qout, T* __restrict__ kstore) {
-  const size_t zo = (size_t)blockIdx.z * P.nf;
+  int bx, by, bz;
+  if (!band_tile<BAND>((P.nx + kSX - 1) / kSX, (P.ny + KSY - 1) / KSY, &bx, &by, &bz)) return;
+  const size_t zo = (size_t)bz * P.nf;
   VD_B(h); VD_B(q); VD_B(qbase); VD_B(k0); VD_B(k1); VD_B(qout); VD_B(kstore);
-  stage_tile(P, blockIdx.y * kSY, blockIdx.x * kSX, h, q, qbase, k0, k1, c0, c1, c2, qout, kstore);
+  stage_tile<T, KSY>(P, by * KSY, bx * kSX, h, q, qbase, k0, k1, c0, c1, c2, qout, kstore);
 }
 
 // Ragged groups: block b takes tile (member, tile row, tile column) of the
@@ -753,10 +807,11 @@ __global__ __launch_bounds__(256) void k_stage_rg(const RgMember<T>* __restrict_
                                                   const T* __restrict__ k1, T c0, T c1, T c2,
                                                   T* __restrict__ qout, T* __restrict__ kstore) {
   const int4 t = tiles[blockIdx.x];
+  if (t.x < 0) return;  // padding entry of a banded member
   const Phys<T> P = mem[t.x].P;
   const size_t zo = mem[t.x].foff;
   VD_B(h); VD_B(q); VD_B(qbase); VD_B(k0); VD_B(k1); VD_B(qout); VD_B(kstore);
-  stage_tile(P, t.y * kSY, t.z * kSX, h, q, qbase, k0, k1, c0, c1, c2, qout, kstore);
+  stage_tile<T, kSY>(P, t.y * kSY, t.z * kSX, h, q, qbase, k0, k1, c0, c1, c2, qout, kstore);
 }
 
 // ---------------------------------------------------------------------------
@@ -967,16 +1022,17 @@ __device__ __forceinline__ void adj_tile(const Phys<T>& P, int i0, int j0, const
               P.cv * (lv(li, lj - 1) + lv(li + 1, lj - 1) - lv(li + 1, lj + 1) - lv(li, lj + 1));
 }
 
-template <typename T>
+template <typename T, bool BAND>
 __global__ __launch_bounds__(kAX * kAY) void k_adj_fused(
     Phys<T> P, const T* __restrict__ h, const T* __restrict__ q, const T* __restrict__ s0,
     const T* __restrict__ s1, const T* __restrict__ s2, T a0, T a1, T a2, const T* acc0,
     const T* acc1, const T* acc2, T* lq_out, T* __restrict__ lh_out) {
-  const size_t zo = (size_t)blockIdx.z * P.nf;
+  int bx, by, bz;
+  if (!band_tile<BAND>((P.nx + kAX - 1) / kAX, (P.ny + kAY - 1) / kAY, &bx, &by, &bz)) return;
+  const size_t zo = (size_t)bz * P.nf;
   VD_B(h); VD_B(q); VD_B(s0); VD_B(s1); VD_B(s2); VD_B(acc0); VD_B(acc1); VD_B(acc2);
   VD_B(lq_out); VD_B(lh_out);
-  adj_tile(P, blockIdx.y * kAY, blockIdx.x * kAX, h, q, s0, s1, s2, a0, a1, a2, acc0, acc1, acc2,
-           lq_out, lh_out);
+  adj_tile(P, by * kAY, bx * kAX, h, q, s0, s1, s2, a0, a1, a2, acc0, acc1, acc2, lq_out, lh_out);
 }
 
 template <typename T>
@@ -986,6 +1042,7 @@ __global__ __launch_bounds__(kAX * kAY) void k_adj_fused_rg(
     const T* __restrict__ s2, T a0, T a1, T a2, const T* acc0, const T* acc1, const T* acc2,
     T* lq_out, T* __restrict__ lh_out) {
   const int4 t = tiles[blockIdx.x];
+  if (t.x < 0) return;  // padding entry of a banded member
   const Phys<T> P = mem[t.x].P;
   const size_t zo = mem[t.x].foff;
   VD_B(h); VD_B(q); VD_B(s0); VD_B(s1); VD_B(s2); VD_B(acc0); VD_B(acc1); VD_B(acc2);
@@ -1037,21 +1094,23 @@ __global__ void k_adj_final(Phys<T> P, const T* __restrict__ hcur, const T* __re
 }
 
 // Ragged-group versions of the segment start/end kernels: block b takes the
-// 64 x 8 point tile stage_tiles[b] of its member (64 x 4 threads, two rows
-// each) with the member's constants, MDT and field offset.  Every point runs
+// kSX x kSY point tile stage_tiles[b] of its member (64 x 4 threads) with the member's constants, MDT and field offset.  Every point runs
 // the same body as in the member's own plan; the member's gfc2_b has one
 // entry and blockIdx.z is 0 here, so the bodies read the member's value.
 #define VD_POINT_RG                                              \
   const int4 t = tiles[blockIdx.x];                              \
+  if (t.x < 0) return;                                           \
   const Phys<T> P = mem[t.x].P;                                  \
   const T* mdt = mem[t.x].mdt;                                   \
   (void)mdt;                                                     \
   const size_t zo = mem[t.x].foff;
 #define VD_POINT_RG_LOOP(BODY)                                   \
-  const int j = t.z * kSX + threadIdx.x;                         \
-  _Pragma("unroll") for (int rr = 0; rr < kSY / 4; ++rr) {       \
-    const int i = t.y * kSY + threadIdx.y + 4 * rr;              \
-    if (i < P.ny && j < P.nx) BODY;                              \
+  for (int cx = 0; cx < kSCX; ++cx) {                            \
+    const int j = t.z * kSX + threadIdx.x + 64 * cx;             \
+    _Pragma("unroll") for (int rr = 0; rr < kSY / 4; ++rr) {     \
+      const int i = t.y * kSY + threadIdx.y + 4 * rr;            \
+      if (i < P.ny && j < P.nx) BODY;                            \
+    }                                                            \
   }
 
 template <typename T>
@@ -1148,9 +1207,14 @@ template <typename T>
 void launch_stage(const Plan& p, const T* h, const T* q, const T* qbase, const T* kprev0,
                   const T* kprev1, T c0, T c1, T c2, T* qout, T* kstore, hipStream_t s) {
   Phys<T> P = make_phys<T>(p);
-  hipLaunchKernelGGL(k_stage<T>, dim3((p.nx + kSX - 1) / kSX, (p.ny + kSY - 1) / kSY, p.nb), kBlock2d, 0, s,
-                     P, h, q, qbase, kprev0,
-                     kprev1, c0, c1, c2, qout, kstore);
+  const int band = band_for(p, 4, sizeof(T));  // h, q, q_base in; q_out
+  const int gx = (p.nx + kSX - 1) / kSX;
+  if (band)
+    hipLaunchKernelGGL((k_stage<T, kSYB, true>), band_grid(1, gx, (p.ny + kSYB - 1) / kSYB, p.nb),
+                       kBlock2d, 0, s, P, h, q, qbase, kprev0, kprev1, c0, c1, c2, qout, kstore);
+  else
+    hipLaunchKernelGGL((k_stage<T, kSY, false>), band_grid(0, gx, (p.ny + kSY - 1) / kSY, p.nb),
+                       kBlock2d, 0, s, P, h, q, qbase, kprev0, kprev1, c0, c1, c2, qout, kstore);
 }
 
 template <typename T>
@@ -1179,9 +1243,14 @@ void launch_stage_adj(const Plan& p, const T* h, const T* q, const T* s0, const 
                       const T* s2, T a0, T a1, T a2, const T* acc0, const T* acc1,
                       const T* acc2, T* lq_out, T* lh_out, hipStream_t s) {
   Phys<T> P = make_phys<T>(p);
-  dim3 grid((p.nx + kAX - 1) / kAX, (p.ny + kAY - 1) / kAY, p.nb);
-  hipLaunchKernelGGL(k_adj_fused<T>, grid, dim3(kAX, kAY), 0, s, P, h, q, s0, s1, s2, a0, a1, a2,
-                     acc0, acc1, acc2, lq_out, lh_out);
+  const int band = band_for(p, 6, sizeof(T));  // h, q, cotangent, accumulator in; lq, lh out
+  const dim3 grid = band_grid(band, (p.nx + kAX - 1) / kAX, (p.ny + kAY - 1) / kAY, p.nb);
+  if (band)
+    hipLaunchKernelGGL((k_adj_fused<T, true>), grid, dim3(kAX, kAY), 0, s, P, h, q, s0, s1, s2, a0,
+                       a1, a2, acc0, acc1, acc2, lq_out, lh_out);
+  else
+    hipLaunchKernelGGL((k_adj_fused<T, false>), grid, dim3(kAX, kAY), 0, s, P, h, q, s0, s1, s2, a0,
+                       a1, a2, acc0, acc1, acc2, lq_out, lh_out);
 }
 
 template <typename T>
@@ -1297,12 +1366,38 @@ static int upload_vec(const std::vector<V>& v, void** d) {
   return VD_OK;
 }
 
+// Stencil tile table of member b with gx x gy tiles.  With band set, members
+// of at least 64 tiles are laid out in the banded XCD order of the
+// single-problem launches (table entry e runs on XCD e % 8): padding entries
+// (member -1) first align the member to a multiple of 8, then the 8 bands
+// interleave.  Otherwise, and for smaller members, row-major order.
+static void rg_tiles(int b, int gx, int gy, bool band, std::vector<int4>& out) {
+  const int T = gx * gy;
+  if (!band || T < 64) {
+    for (int ty = 0; ty < gy; ++ty)
+      for (int tx = 0; tx < gx; ++tx) out.push_back(make_int4(b, ty, tx, 0));
+    return;
+  }
+  while (out.size() % 8) out.push_back(make_int4(-1, 0, 0, 0));
+  const int G8 = (T + 7) / 8;
+  for (int k = 0; k < G8; ++k)
+    for (int x = 0; x < 8; ++x) {
+      const int t = x * G8 + k;
+      out.push_back(t < T ? make_int4(b, t / gx, t % gx, 0) : make_int4(-1, 0, 0, 0));
+    }
+}
+
 template <typename T>
 int group_build(Group& g) {
   const int n = g.n;
   g.foff.assign(n + 1, 0);
   std::vector<RgMember<T>> mem(n);
   std::vector<int4> st, at;
+  // banded stencil order when the members' fields (four per stage launch)
+  // exceed the 256 MB Infinity Cache, as for single-problem launches
+  double total = 0;
+  for (int b = 0; b < n; ++b) total += (double)g.plans[b]->nf * sizeof(T) * 4;
+  const bool band = total > 256.0 * (1 << 20);
   g.stage_prefix.assign(n + 1, 0);
   g.adj_prefix.assign(n + 1, 0);
   for (int b = 0; b < n; ++b) {
@@ -1311,10 +1406,8 @@ int group_build(Group& g) {
     mem[b].P = make_phys<T>(p);
     mem[b].foff = g.foff[b];
     mem[b].mdt = p.has_mdt ? static_cast<const T*>(p.mdt) : nullptr;
-    for (int ty = 0; ty < (p.ny + kSY - 1) / kSY; ++ty)
-      for (int tx = 0; tx < (p.nx + kSX - 1) / kSX; ++tx) st.push_back(make_int4(b, ty, tx, 0));
-    for (int ty = 0; ty < (p.ny + kAY - 1) / kAY; ++ty)
-      for (int tx = 0; tx < (p.nx + kAX - 1) / kAX; ++tx) at.push_back(make_int4(b, ty, tx, 0));
+    rg_tiles(b, (p.nx + kSX - 1) / kSX, (p.ny + kSY - 1) / kSY, band, st);
+    rg_tiles(b, (p.nx + kAX - 1) / kAX, (p.ny + kAY - 1) / kAY, band, at);
     g.stage_prefix[b + 1] = (int)st.size();
     g.adj_prefix[b + 1] = (int)at.size();
   }
