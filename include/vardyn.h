@@ -254,6 +254,55 @@ int vd_group_adj_final(vd_group* group, int nact, const void* hcur, const void* 
                        const void* lq, void* adh0, void* stream);
 
 /* ---------------------------------------------------------------------------
+ * Balanced-motion wavelet basis (Basis_bm, mapping/src/basis.py:472-936):
+ * SSH fields from wavelet coefficients (operg, basis.py:863-901) and the
+ * transpose (operg_transpose, basis.py:903-936) for a batch of T basis times.
+ * The host builds the tables (vardyn_amd/basis_bm.py): a rectilinear grid
+ * (lon1 along x, lat1 along y, mask 1 = masked), per frequency DX (km), the
+ * wave vectors kx, ky (nf x ntheta), the first coefficient, centre count and
+ * time-centre count; per centre its frequency, index, lon, lat, cos(lat) and
+ * point box (i0, i1, j0, j1 inclusive); per grid point the centres whose box
+ * covers it (CSR).  Coefficients are laid out as in the reference:
+ * frequency, time centre, space centre, direction, cos/sin.  The time factors
+ * gt of a call are, per frequency, a T x nt_f row-major block, concatenated.
+ * ------------------------------------------------------------------------- */
+typedef struct vd_bm_basis vd_bm_basis;
+typedef struct vd_bm_params {
+  int ny, nx, nf, ntheta, nc;
+  const double* lon1;
+  const double* lat1;
+  const uint8_t* mask;
+  const double* dxf;
+  const double* kx;
+  const double* ky;
+  const int32_t* cf;
+  const int32_t* cP;
+  const double* clon;
+  const double* clat;
+  const double* ccos;
+  const int32_t* cbox;
+  const int64_t* fbase;
+  const int32_t* fnp;
+  const int32_t* fnt;
+  const int64_t* pptr;
+  const int32_t* plist;
+} vd_bm_params;
+
+/* Upload the tables (host arrays) to the current device. */
+int vd_bm_create(const vd_bm_params* params, vd_bm_basis** basis);
+int vd_bm_destroy(vd_bm_basis* basis);
+/* Number of coefficients and of time-factor columns (sum of nt_f). */
+int vd_bm_info(const vd_bm_basis* basis, long long* nbasis, int* gt_total);
+/* Device workspace of one call over T times. */
+int vd_bm_workspace_size(const vd_bm_basis* basis, int T, size_t* bytes);
+/* ssh[t] (T fields of ny x nx) = operg(time t, X); device pointers. */
+int vd_bm_operg(vd_bm_basis* basis, int T, const double* gt, const double* X, double* ssh,
+                void* work, void* stream);
+/* adX += sum over the T times of operg_transpose(time t, adssh[t]). */
+int vd_bm_operg_adj(vd_bm_basis* basis, int T, const double* gt, const double* adssh,
+                    double* adX, void* work, void* stream);
+
+/* ---------------------------------------------------------------------------
  * L3 observation operator (Obsop_interp_l3_jax, mapping/src/obsop.py:375-678).
  * One table per observation time:
  *   row_ptr[n_obs+1], col[nnz] (int32 flat grid index), w[nnz] (double)

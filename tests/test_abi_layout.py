@@ -29,3 +29,8 @@ def test_params_layout_matches_header():
 
 def test_obs_table_layout_matches_header():
     assert [f for f, _ in _lib.ObsTable._fields_] == struct_fields("vd_obs_table")
+
+
+def test_bm_params_layout_matches_header():
+    from vardyn_amd.basis_bm import BmParams
+    assert [f for f, _ in BmParams._fields_] == struct_fields("vd_bm_params")

@@ -1,0 +1,102 @@
+"""The balanced-motion wavelet basis on the GPU (vd_bm_operg / vd_bm_operg_adj
+behind vardyn_amd.basis_bm.Basis_bm) against the FP64 oracle of Basis_bm
+(mapping/src/basis.py:472-936).
+
+Bars: the host tables (frequencies, centres, coefficient layout, Q) equal the
+oracle's exactly; operg and operg_transpose within 1e-12 relative of the
+oracle (trigonometric functions of the device math library against numpy's);
+a batch of times equals the single-time calls (synthesis bit for bit, the
+transposed sum within 1e-13); the dot-product identity of the GPU pair; two
+runs bit-identical (no atomics).
+"""
+from types import SimpleNamespace
+
+import numpy as np
+import pytest
+import torch
+
+from oracle.basis_bm import BasisBmOracle
+from vardyn_amd.basis_bm import Basis_bm
+
+pytestmark = pytest.mark.gpu
+
+
+def grid(ny, nx, d, land):
+    lon, lat = np.meshgrid(-60 + d * np.arange(nx), 35 + d * np.arange(ny))
+    mask = None
+    if land:
+        mask = np.zeros((ny, nx), dtype=bool)
+        mask[:ny // 5, :nx // 4] = True
+        mask[ny // 2, nx // 2:nx // 2 + 3] = True
+    return lon, lat, mask
+
+
+def rel(a, b):
+    a, b = np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+CASES = [dict(ny=28, nx=32, d=0.05, land=False, flux=False, lmin=30.0, lmax=110.0),
+         dict(ny=28, nx=32, d=0.05, land=True, flux=True, lmin=30.0, lmax=110.0),
+         dict(ny=60, nx=44, d=0.045, land=True, flux=False, lmin=60.0, lmax=250.0)]
+
+
+def pair(c, times):
+    lon, lat, mask = grid(c["ny"], c["nx"], c["d"], c["land"])
+    kw = dict(flux=c["flux"], lmin=c["lmin"], lmax=c["lmax"], delta_time_ref=7.3)
+    orc = BasisBmOracle(lon, lat, mask, **kw)
+    Xb_o, Q_o = orc.set_basis(times, return_q=True)
+    gpu = Basis_bm(SimpleNamespace(lon=lon, lat=lat, mask=mask), **kw)
+    Xb_g, Q_g = gpu.set_basis(times, return_q=True)
+    return orc, gpu, Q_o, Q_g
+
+
+@pytest.mark.parametrize("c", CASES)
+def test_tables_and_operators_against_oracle(c):
+    times = np.arange(0.0, 9.0, 0.5)
+    orc, gpu, Q_o, Q_g = pair(c, times)
+    assert gpu.nbasis == orc.nbasis > 0 and np.array_equal(Q_g, Q_o)
+    assert list(gpu.iff_wavebounds) == list(orc.iff_wavebounds)
+    for f in range(orc.nf):
+        assert np.array_equal(gpu.centres[f][0], orc.ENSLON[f])
+        assert np.array_equal(gpu.centres[f][1], orc.ENSLAT[f])
+        assert np.array_equal(gpu.enst[f], orc.enst[f]) and gpu.norm_fact[f] == orc.norm_fact[f]
+    rng = np.random.default_rng(1)
+    X = rng.standard_normal(gpu.nbasis)
+    for t in (0.0, 3.5, 8.5):
+        assert rel(gpu.operg(t, X), orc.operg(t, X)) < 1e-12
+        phi = rng.standard_normal(gpu.shape_phys)
+        adState = {"ssh": phi.copy()}
+        got = gpu.operg_transpose(t, adState)
+        assert rel(got, orc.operg_transpose(t, phi)) < 1e-12
+        assert not np.any(adState["ssh"])             # consumed, as in the reference
+
+
+@pytest.mark.parametrize("c", CASES[1:])
+def test_batch_adjoint_and_determinism(c):
+    times = np.arange(0.0, 9.0, 0.5)
+    _, gpu, _, _ = pair(c, times)
+    rng = np.random.default_rng(2)
+    X = torch.as_tensor(rng.standard_normal(gpu.nbasis), device="cuda")
+    F = gpu.operg_batch(times, X)
+    for k in (0, 5, len(times) - 1):
+        assert torch.equal(F[k], gpu.operg_batch([times[k]], X)[0])
+    assert torch.equal(F, gpu.operg_batch(times, X))
+    A = torch.as_tensor(rng.standard_normal((len(times),) + gpu.shape_phys), device="cuda")
+    adX = gpu.operg_transpose_batch(times, A)
+    assert torch.equal(adX, gpu.operg_transpose_batch(times, A))
+    one = sum(gpu.operg_transpose_batch([t], A[k]) for k, t in enumerate(times))
+    assert rel(adX.cpu(), one.cpu()) < 1e-13
+    lhs = float((F * A).sum())
+    rhs = float((X * adX).sum())
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+
+
+def test_state_protocol():
+    c = CASES[0]
+    times = np.arange(0.0, 4.0)
+    _, gpu, _, _ = pair(c, times)
+    X = np.random.default_rng(3).standard_normal(gpu.nbasis)
+    State = {"ssh": torch.zeros(gpu.shape_phys, dtype=torch.float64, device="cuda")}
+    gpu.operg(1.0, X, State=State)
+    assert torch.equal(State["ssh"].cpu(), torch.as_tensor(gpu.operg(1.0, X)))

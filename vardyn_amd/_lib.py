@@ -84,6 +84,12 @@ SIGNATURES = {
     "vd_tiles_blend": (_I, [_I, _I, _I, _I, _I, _I, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP]),
     "vd_tiles_blend_adj": (_I, [_I, _I, _I, _I, _I, _I, _VP, _VP, _VP, _VP, _VP, _VP, _VP]),
     "vd_tiles_gather_adj": (_I, [_I, _I, _I, _I, _I, _I, _VP, _VP, _VP, _VP, _VP]),
+    "vd_bm_create": (_I, [_VP, _VP]),
+    "vd_bm_destroy": (_I, [_VP]),
+    "vd_bm_info": (_I, [_VP, C.POINTER(C.c_longlong), C.POINTER(C.c_int)]),
+    "vd_bm_workspace_size": (_I, [_VP, _I, C.POINTER(_SZ)]),
+    "vd_bm_operg": (_I, [_VP, _I, _VP, _VP, _VP, _VP, _VP]),
+    "vd_bm_operg_adj": (_I, [_VP, _I, _VP, _VP, _VP, _VP, _VP]),
     "vd_dot_f64": (_I, [_I64, _VP, _VP, _D, _VP, _VP, _VP]),
     "vd_axpby": (_I, [_I, _I64, _D, _VP, _D, _VP, _VP]),
     "vd_convert": (_I, [_I, _VP, _I, _VP, _I64, _I, _VP]),

@@ -1,0 +1,374 @@
+// Balanced-motion wavelet basis (Basis_bm, mapping/src/basis.py:472-936):
+// synthesis of SSH fields from wavelet coefficients (operg) and its transpose
+// (operg_transpose), for a batch of basis times at once.
+//
+// A coefficient is (frequency f, time centre it, space centre P, direction,
+// cos/sin); its field at time t is
+//   g_f(t, it) * sqrt(2) * w(dx/DX_f) w(dy/DX_f) * cos|sin(kx dx + ky dy)
+// on the points of the centre's box (|dx|, |dy| <= DX_f km, unmasked), where
+// g_f is the time factor and w(x) = cos^2(pi x / 2) (basis.py:726-820).  The
+// host passes the centres with their boxes (a rectilinear grid makes the
+// reference's point selection an index box), the per-point lists of centres
+// covering each point, and the time factors of the batch.
+//
+// Matrix-free: the space values are recomputed from the formula in each
+// direction (no stored CSC), and every direction is a gather with a fixed
+// summation order, so results are deterministic:
+//   k_bm_time   C[c][q][t] = sum_it g_f(t, it) X[f, it, P, q]   (q: direction, cos/sin)
+//   k_bm_synth  ssh[t][p] = sum over centres c covering p, q of G(p, c, q) C[c][q][t]
+//   k_bm_proj   D[c][q][t] = sum over points p of c's box of G(p, c, q) adssh[t][p]
+//   k_bm_time_adj  adX[f, it, P, q] += sum_t g_f(t, it) D[c][q][t]
+// The trigonometric values of a (point, centre) pair are computed once for
+// the whole batch of times.
+#include <cmath>
+#include <cstring>
+
+#include "vd_internal.h"
+
+namespace vd {
+
+constexpr double kKm2Deg = 1.0 / 110;
+constexpr int kBmMaxTheta = 16;  // directions per frequency (9 with the reference defaults)
+constexpr int kBmTB = 4;         // times per pass of the synthesis gather
+constexpr int kBmPB = 2;         // times per pass of the projection
+
+struct BmTables {
+  int ny, nx, nf, ntheta, nc;
+  const double* lon1;      // nx
+  const double* lat1;      // ny
+  const uint8_t* mask;     // ny * nx (1 = masked) or null
+  const double* dxf;       // nf: DX_f (km)
+  const double* kx;        // nf * ntheta
+  const double* ky;
+  const int* cf;           // nc: frequency of centre c
+  const int* cP;           // nc: index of c among its frequency's centres
+  const double* clon;      // nc
+  const double* clat;
+  const double* ccos;      // cos(lat_c pi / 180)
+  const int4* cbox;        // nc: i0, i1, j0, j1 (inclusive; i0 > i1 = empty)
+  const int64_t* fbase;    // nf: first coefficient of frequency f
+  const int* fnp;          // nf: centres of frequency f
+  const int* fnt;          // nf: time centres of frequency f
+  const int* gtoff;        // nf: offset of frequency f's (T x nt_f) block of time factors / T
+  const int64_t* pptr;     // ny * nx + 1: centres covering point p
+  const int* plist;
+};
+
+__device__ __forceinline__ double bm_window(double x) {
+  const double c = cos(x * 0.5 * M_PI);
+  return c * c;
+}
+
+// sqrt(2) * taper of centre c at grid point (i, j), and the offsets (km)
+__device__ __forceinline__ void bm_point(const BmTables& B, int c, int i, int j, double* xx,
+                                         double* yy, double* s2f) {
+#pragma clang fp contract(off)
+  const double dx = B.dxf[B.cf[c]];
+  *xx = (B.lon1[j] - B.clon[c]) / kKm2Deg * B.ccos[c];
+  *yy = (B.lat1[i] - B.clat[c]) / kKm2Deg;
+  *s2f = M_SQRT2 * (bm_window(*xx / dx) * bm_window(*yy / dx));
+}
+
+__device__ __forceinline__ void bm_trig(const BmTables& B, int f, int th, double xx, double yy,
+                                        double* s, double* co) {
+#pragma clang fp contract(off)
+  const double arg = B.kx[f * B.ntheta + th] * xx + B.ky[f * B.ntheta + th] * yy;
+  sincos(arg, s, co);
+}
+
+// C[c][q][t] = sum_it g_f(t, it) X[fbase + it * blk + P * 2 ntheta + q]
+__global__ void k_bm_time(BmTables B, int T, const double* __restrict__ gt,
+                          const double* __restrict__ X, double* __restrict__ C) {
+  const int q2 = 2 * B.ntheta;
+  const long long n = (long long)B.nc * q2;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int c = (int)(e / q2), q = (int)(e % q2);
+    const int f = B.cf[c], nt = B.fnt[f];
+    const long long blk = (long long)B.fnp[f] * q2;
+    const double* g = gt + (long long)B.gtoff[f] * T;
+    const double* x = X + B.fbase[f] + (long long)B.cP[c] * q2 + q;
+    for (int t = 0; t < T; ++t) {
+      double acc = 0;
+      for (int it = 0; it < nt; ++it) {
+        const double gv = g[t * nt + it];
+        if (gv != 0) acc += gv * x[it * blk];
+      }
+      C[e * T + t] = acc;
+    }
+  }
+}
+
+// ssh[t][p] for times t0 .. t0 + TB - 1 (fewer at the end of the batch)
+__global__ __launch_bounds__(256) void k_bm_synth(BmTables B, int T, int t0,
+                                                  const double* __restrict__ C,
+                                                  double* __restrict__ ssh) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int np = B.ny * B.nx;
+  if (p >= np) return;
+  const int i = p / B.nx, j = p % B.nx;
+  const int nt = min(kBmTB, T - t0);
+  double acc[kBmTB] = {};
+  const int q2 = 2 * B.ntheta;
+  for (int64_t e = B.pptr[p]; e < B.pptr[p + 1]; ++e) {
+    const int c = B.plist[e], f = B.cf[c];
+    double xx, yy, s2f;
+    bm_point(B, c, i, j, &xx, &yy, &s2f);
+    const double* cc = C + ((long long)c * q2) * T + t0;
+    for (int th = 0; th < B.ntheta; ++th) {
+      double s, co;
+      bm_trig(B, f, th, xx, yy, &s, &co);
+      const double vc = s2f * co, vs = s2f * s;
+#pragma unroll
+      for (int t = 0; t < kBmTB; ++t)
+        if (t < nt) acc[t] += vc * cc[(2 * th) * T + t] + vs * cc[(2 * th + 1) * T + t];
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < kBmTB; ++t)
+    if (t < nt) ssh[(long long)(t0 + t) * np + p] = acc[t];
+}
+
+// D[c][q][t] for times t0 .. t0 + PB - 1: one workgroup of 4 waves per
+// centre, fixed reduction order (per-thread sums over a strided set of box
+// points, a shuffle tree per wave, then the four waves in a fixed order).
+__global__ __launch_bounds__(256) void k_bm_proj(BmTables B, int T, int t0,
+                                                 const double* __restrict__ adssh,
+                                                 double* __restrict__ D) {
+  __shared__ double red[4][2 * kBmMaxTheta * kBmPB];
+  const int c = blockIdx.x;
+  const int f = B.cf[c];
+  const int4 bx = B.cbox[c];
+  const int nt = min(kBmPB, T - t0);
+  const int np = B.ny * B.nx;
+  const int q2 = 2 * B.ntheta;
+  double part[2 * kBmMaxTheta][kBmPB];
+#pragma unroll
+  for (int q = 0; q < 2 * kBmMaxTheta; ++q)
+#pragma unroll
+    for (int t = 0; t < kBmPB; ++t) part[q][t] = 0;
+  const int bw = bx.w - bx.z + 1, bh = bx.y - bx.x + 1;
+  const int npts = (bw > 0 && bh > 0) ? bw * bh : 0;
+  for (int e = threadIdx.x; e < npts; e += blockDim.x) {
+    const int i = bx.x + e / bw, j = bx.z + e % bw;
+    const int p = i * B.nx + j;
+    if (B.mask && B.mask[p]) continue;
+    double a[kBmPB];
+#pragma unroll
+    for (int t = 0; t < kBmPB; ++t) a[t] = t < nt ? adssh[(int64_t)(t0 + t) * np + p] : 0.0;
+    double xx, yy, s2f;
+    bm_point(B, c, i, j, &xx, &yy, &s2f);
+#pragma unroll
+    for (int th = 0; th < kBmMaxTheta; ++th) {
+      if (th < B.ntheta) {
+        double s, co;
+        bm_trig(B, f, th, xx, yy, &s, &co);
+        const double vc = s2f * co, vs = s2f * s;
+#pragma unroll
+        for (int t = 0; t < kBmPB; ++t) {
+          part[2 * th][t] += vc * a[t];
+          part[2 * th + 1][t] += vs * a[t];
+        }
+      }
+    }
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < 2 * kBmMaxTheta; ++q) {
+    if (q < q2) {
+#pragma unroll
+      for (int t = 0; t < kBmPB; ++t) {
+        double v = part[q][t];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+        if (lane == 0) red[wv][q * kBmPB + t] = v;
+      }
+    }
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < q2 * kBmPB; e += blockDim.x) {
+    const int q = e / kBmPB, t = e % kBmPB;
+    if (t < nt)
+      D[((int64_t)c * q2 + q) * T + t0 + t] = (red[0][e] + red[1][e]) + (red[2][e] + red[3][e]);
+  }
+}
+
+// adX[fbase + it * blk + P * 2 ntheta + q] += sum_t g_f(t, it) D[c][q][t]
+__global__ void k_bm_time_adj(BmTables B, int T, const double* __restrict__ gt,
+                              const double* __restrict__ D, double* __restrict__ adX) {
+  const int q2 = 2 * B.ntheta;
+  const long long n = (long long)B.nc * q2;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int c = (int)(e / q2), q = (int)(e % q2);
+    const int f = B.cf[c], nt = B.fnt[f];
+    const long long blk = (long long)B.fnp[f] * q2;
+    const double* g = gt + (long long)B.gtoff[f] * T;
+    double* x = adX + B.fbase[f] + (long long)B.cP[c] * q2 + q;
+    const double* d = D + e * T;
+    for (int it = 0; it < nt; ++it) {
+      double acc = 0;
+      bool any = false;
+      for (int t = 0; t < T; ++t) {
+        const double gv = g[t * nt + it];
+        if (gv != 0) {
+          acc += gv * d[t];
+          any = true;
+        }
+      }
+      if (any) x[it * blk] += acc;
+    }
+  }
+}
+
+}  // namespace vd
+
+using namespace vd;
+
+struct vd_bm_basis {
+  BmTables B;
+  std::vector<void*> owned;
+  long long nbasis;
+  int gt_total;  // sum of nt_f
+  int device;
+};
+
+template <typename V>
+static int bm_upload(vd_bm_basis* b, const V* src, size_t n, const V** dst) {
+  if (n == 0) {
+    *dst = nullptr;
+    return VD_OK;
+  }
+  void* d = nullptr;
+  VD_HIP_TRY(hipMalloc(&d, n * sizeof(V)));
+  b->owned.push_back(d);
+  VD_HIP_TRY(hipMemcpy(d, src, n * sizeof(V), hipMemcpyHostToDevice));
+  *dst = static_cast<const V*>(d);
+  return VD_OK;
+}
+
+extern "C" {
+
+int vd_bm_create(const vd_bm_params* prm, vd_bm_basis** out) {
+  VD_REQUIRE(prm && out, VD_ERR_INVALID_ARG, "null argument");
+  *out = nullptr;
+  const vd_bm_params& P = *prm;
+  VD_REQUIRE(P.ny > 0 && P.nx > 0 && P.nf > 0 && P.nc >= 0, VD_ERR_SHAPE, "bad basis sizes");
+  VD_REQUIRE(P.ntheta > 0 && P.ntheta <= kBmMaxTheta, VD_ERR_UNSUPPORTED,
+             "ntheta must be 1..16 (directions per frequency)");
+  VD_REQUIRE(P.lon1 && P.lat1 && P.dxf && P.kx && P.ky && P.fbase && P.fnp && P.fnt && P.pptr,
+             VD_ERR_INVALID_ARG, "null table");
+  VD_REQUIRE(P.nc == 0 || (P.cf && P.cP && P.clon && P.clat && P.ccos && P.cbox && P.plist),
+             VD_ERR_INVALID_ARG, "null centre table");
+  const size_t np = (size_t)P.ny * P.nx;
+  for (int c = 0; c < P.nc; ++c) {
+    const int* bx = P.cbox + 4 * c;
+    VD_REQUIRE(P.cf[c] >= 0 && P.cf[c] < P.nf && P.cP[c] >= 0 && P.cP[c] < P.fnp[P.cf[c]],
+               VD_ERR_INVALID_ARG, "centre frequency / index out of range");
+    VD_REQUIRE(bx[0] > bx[1] || (bx[0] >= 0 && bx[1] < P.ny && bx[2] >= 0 && bx[3] < P.nx),
+               VD_ERR_INVALID_ARG, "centre box outside the grid");
+  }
+  VD_REQUIRE(P.pptr[0] == 0, VD_ERR_INVALID_ARG, "point lists must start at 0");
+  for (size_t p = 0; p < np; ++p)
+    VD_REQUIRE(P.pptr[p + 1] >= P.pptr[p], VD_ERR_INVALID_ARG, "point lists must be ordered");
+  for (int64_t e = 0; e < P.pptr[np]; ++e)
+    VD_REQUIRE(P.plist[e] >= 0 && P.plist[e] < P.nc, VD_ERR_INVALID_ARG, "bad centre in point list");
+  long long nb = 0;
+  for (int f = 0; f < P.nf; ++f) {
+    VD_REQUIRE(P.fnp[f] >= 0 && P.fnt[f] >= 0, VD_ERR_INVALID_ARG, "bad frequency table");
+    VD_REQUIRE(P.fbase[f] == nb, VD_ERR_INVALID_ARG, "frequency blocks must be contiguous");
+    nb += 2LL * P.ntheta * P.fnp[f] * P.fnt[f];
+  }
+  int device = 0;
+  VD_HIP_TRY(hipGetDevice(&device));
+  vd_bm_basis* b = new vd_bm_basis();
+  b->nbasis = nb;
+  b->device = device;
+  BmTables& B = b->B;
+  B.ny = P.ny; B.nx = P.nx; B.nf = P.nf; B.ntheta = P.ntheta; B.nc = P.nc;
+  std::vector<int> gtoff(P.nf);
+  int acc = 0;
+  for (int f = 0; f < P.nf; ++f) {
+    gtoff[f] = acc;
+    acc += P.fnt[f];
+  }
+  b->gt_total = acc;
+  std::vector<int4> box(P.nc);
+  for (int c = 0; c < P.nc; ++c)
+    box[c] = make_int4(P.cbox[4 * c], P.cbox[4 * c + 1], P.cbox[4 * c + 2], P.cbox[4 * c + 3]);
+  int rc = VD_OK;
+  if ((rc = bm_upload(b, P.lon1, P.nx, &B.lon1)) || (rc = bm_upload(b, P.lat1, P.ny, &B.lat1)) ||
+      (rc = bm_upload(b, P.mask, P.mask ? np : 0, &B.mask)) ||
+      (rc = bm_upload(b, P.dxf, P.nf, &B.dxf)) ||
+      (rc = bm_upload(b, P.kx, (size_t)P.nf * P.ntheta, &B.kx)) ||
+      (rc = bm_upload(b, P.ky, (size_t)P.nf * P.ntheta, &B.ky)) ||
+      (rc = bm_upload(b, P.cf, P.nc, &B.cf)) || (rc = bm_upload(b, P.cP, P.nc, &B.cP)) ||
+      (rc = bm_upload(b, P.clon, P.nc, &B.clon)) || (rc = bm_upload(b, P.clat, P.nc, &B.clat)) ||
+      (rc = bm_upload(b, P.ccos, P.nc, &B.ccos)) ||
+      (rc = bm_upload(b, (const int4*)box.data(), P.nc, &B.cbox)) ||
+      (rc = bm_upload(b, P.fbase, P.nf, &B.fbase)) || (rc = bm_upload(b, P.fnp, P.nf, &B.fnp)) ||
+      (rc = bm_upload(b, P.fnt, P.nf, &B.fnt)) ||
+      (rc = bm_upload(b, (const int*)gtoff.data(), P.nf, &B.gtoff)) ||
+      (rc = bm_upload(b, P.pptr, np + 1, &B.pptr)) ||
+      (rc = bm_upload(b, P.plist, (size_t)P.pptr[np], &B.plist))) {
+    vd_bm_destroy(b);
+    return rc;
+  }
+  *out = b;
+  return VD_OK;
+}
+
+int vd_bm_destroy(vd_bm_basis* b) {
+  if (!b) return VD_OK;
+  for (void* p : b->owned) hipFree(p);
+  delete b;
+  return VD_OK;
+}
+
+int vd_bm_info(const vd_bm_basis* b, long long* nbasis, int* gt_total) {
+  VD_REQUIRE(b, VD_ERR_INVALID_ARG, "null basis");
+  if (nbasis) *nbasis = b->nbasis;
+  if (gt_total) *gt_total = b->gt_total;
+  return VD_OK;
+}
+
+int vd_bm_workspace_size(const vd_bm_basis* b, int T, size_t* bytes) {
+  VD_REQUIRE(b && bytes && T > 0, VD_ERR_INVALID_ARG, "null argument or T < 1");
+  *bytes = (size_t)b->B.nc * 2 * b->B.ntheta * T * sizeof(double);
+  return VD_OK;
+}
+
+int vd_bm_operg(vd_bm_basis* b, int T, const double* gt, const double* X, double* ssh,
+                void* work, void* stream) {
+  VD_REQUIRE(b && gt && X && ssh && work && T > 0, VD_ERR_INVALID_ARG, "null argument or T < 1");
+  const BmTables& B = b->B;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  double* C = static_cast<double*>(work);
+  const long long n = (long long)B.nc * 2 * B.ntheta;
+  if (n > 0)
+    hipLaunchKernelGGL(k_bm_time, dim3((unsigned)std::min<long long>((n + 255) / 256, 65535)),
+                       dim3(256), 0, s, B, T, gt, X, C);
+  const int np = B.ny * B.nx;
+  for (int t0 = 0; t0 < T; t0 += kBmTB)
+    hipLaunchKernelGGL(k_bm_synth, dim3((np + 255) / 256), dim3(256), 0, s, B, T, t0, C, ssh);
+  VD_HIP_TRY(hipGetLastError());
+  return VD_OK;
+}
+
+int vd_bm_operg_adj(vd_bm_basis* b, int T, const double* gt, const double* adssh, double* adX,
+                    void* work, void* stream) {
+  VD_REQUIRE(b && gt && adssh && adX && work && T > 0, VD_ERR_INVALID_ARG,
+             "null argument or T < 1");
+  const BmTables& B = b->B;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  double* D = static_cast<double*>(work);
+  if (B.nc == 0) return VD_OK;
+  for (int t0 = 0; t0 < T; t0 += kBmPB)
+    hipLaunchKernelGGL(k_bm_proj, dim3(B.nc), dim3(256), 0, s, B, T, t0, adssh, D);
+  const long long n = (long long)B.nc * 2 * B.ntheta;
+  hipLaunchKernelGGL(k_bm_time_adj, dim3((unsigned)std::min<long long>((n + 255) / 256, 65535)),
+                     dim3(256), 0, s, B, T, gt, D, adX);
+  VD_HIP_TRY(hipGetLastError());
+  return VD_OK;
+}
+
+}  // extern "C"
