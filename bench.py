@@ -87,6 +87,9 @@ def parse():
     ap.add_argument("--tiled", action="store_true",
                     help="the tiled QG model (QgmWithTiles) on the 512x512 grid instead: step and "
                          "adjoint of all tiles in shared launches (grid-point-steps/s)")
+    ap.add_argument("--basis", action="store_true",
+                    help="the balanced-motion wavelet basis (Basis_bm) on a 512x512 lon/lat grid "
+                         "instead: operg + operg_transpose over the window's basis times")
     ap.add_argument("--tile-size", type=int, default=32)
     ap.add_argument("--tile-overlap", type=int, default=16)
     ap.add_argument("--tiled-steps", type=int, default=16,
@@ -565,6 +568,118 @@ def tiled(args):
                                    f"same tiling ({cpu_s:.1f} s)"}}
 
 
+def basis(args):
+    """The balanced-motion wavelet basis (``Basis_bm``, basis.py:472-936) with
+    the reference's default settings (BASIS_BM, config_default.py:804-860) on
+    a regular 512 x 512 lon/lat grid of 0.045 degrees (about 5 km), at the
+    54 basis times of a 30-day window with a basis time every 13.5 h (every
+    checkpoint of the QG workload).  One unit is ``operg`` of all 54 times
+    plus the summed ``operg_transpose`` of all 54 (the forward and reverse
+    use of the reduced control in one gradient evaluation); ``value`` is grid
+    points x basis times per second of that pair.  Beside it: the FP64 oracle
+    (the reference's sparse-matrix algorithm) on one host thread over a
+    sample of times, and parity of the GPU fields and coefficients against
+    it."""
+    import numpy as np
+    import torch
+    from types import SimpleNamespace
+    from oracle.basis_bm import BasisBmOracle
+    from vardyn_amd import _lib
+    from vardyn_amd.basis_bm import Basis_bm
+    from vardyn_amd.qg import current_stream_ptr, ptr
+    ny = nx = 512
+    d = 0.045
+    lon, lat = np.meshgrid(-60 + d * np.arange(nx), 35 + d * np.arange(ny))
+    times = np.arange(54) * 0.5625
+    t0 = time.time()
+    b = Basis_bm(SimpleNamespace(lon=lon, lat=lat, mask=None))
+    _, Q = b.set_basis(times, return_q=True)
+    setup_s = time.time() - t0
+    log(f"basis: {b.nbasis} coefficients, {b.nf} frequencies x {b.ntheta} directions, "
+        f"{b.nc} centres, {b.nnz_space:.3e} space-matrix entries; set-up {setup_s:.1f}s")
+    T = times.size
+    dev = b.device
+    rng = np.random.default_rng(21)
+    X = torch.as_tensor(rng.standard_normal(b.nbasis) * np.sqrt(Q), device=dev)
+    gt = torch.as_tensor(b.time_factors(times), device=dev)
+    ws = b._workspace(T)
+    out = torch.empty((T, ny, nx), dtype=torch.float64, device=dev)
+    ad = torch.as_tensor(rng.standard_normal((T, ny, nx)), device=dev)
+    adX = torch.zeros(b.nbasis, dtype=torch.float64, device=dev)
+    s_ = current_stream_ptr(dev)
+
+    def fwd():
+        _lib.call("vd_bm_operg", b._h, T, ptr(gt), ptr(X), ptr(out), ptr(ws), s_)
+
+    def adj():
+        _lib.call("vd_bm_operg_adj", b._h, T, ptr(gt), ptr(ad), ptr(adX), ptr(ws), s_)
+
+    def clock(fn, reps=5):
+        fn()
+        torch.cuda.synchronize()
+        st = torch.cuda.current_stream()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(reps):
+            fn()
+        e1.record(st)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    ms_f, ms_a = clock(fwd), clock(adj)
+    value = ny * nx * T / ((ms_f + ms_a) * 1e-3)
+    flops = 2 * b.nnz_space * T  # one multiply-add per space-matrix entry and time
+    tf = 2 * flops / ((ms_f + ms_a) * 1e-3) / 1e12
+    log(f"basis: operg {ms_f:.2f} ms, operg_transpose {ms_a:.2f} ms for {T} times")
+    # oracle on a sample of times (its space matrix is built once, untimed)
+    orc = BasisBmOracle(lon, lat, None)
+    t1 = time.time()
+    orc.set_basis(times)
+    orc_setup = time.time() - t1
+    sample = [0, T // 2]
+    Xh = X.cpu().numpy()
+    t2 = time.perf_counter()
+    ref_f = [orc.operg(times[k], Xh) for k in sample]
+    ref_a = sum(orc.operg_transpose(times[k], ad[k].cpu().numpy()) for k in sample)
+    cpu_s = time.perf_counter() - t2
+    got_f = b.operg_batch(times[sample], X).cpu().numpy()
+    got_a = b.operg_transpose_batch(times[sample], ad[sample]).cpu().numpy()
+    err_f = max(float(np.max(np.abs(got_f[i] - ref_f[i])) / np.max(np.abs(ref_f[i])))
+                for i in range(len(sample)))
+    err_a = float(np.max(np.abs(got_a - ref_a)) / np.max(np.abs(ref_a)))
+    log(f"basis parity vs oracle ({len(sample)} times): operg {err_f:.2e}, "
+        f"operg_transpose {err_a:.2e}; oracle {cpu_s:.1f} s (+{orc_setup:.1f} s set-up)")
+    del orc
+    return {
+        "metric": "balanced-motion basis operg + operg_transpose grid-point-times/sec, 512x512, "
+                  "54 basis times",
+        "value": round(value, 1), "unit": "grid-point-times/s", "n_gpus": 1, "steps": 5,
+        "warmup": 1, "ms_per_step": round(ms_f + ms_a, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded coefficients scaled by sqrt(Q), seeded adjoint fields)",
+        "config": {"workload": "bm_basis512", "grid": [ny, nx], "dlon_dlat_deg": d,
+                   "times": int(T), "window_days": 30, "lmin_km": 80.0, "lmax_km": 970.0,
+                   "nbasis": int(b.nbasis), "frequencies": int(b.nf), "directions": int(b.ntheta),
+                   "centres": int(b.nc), "space_matrix_entries": int(b.nnz_space)},
+        "ms_operg": round(ms_f, 3), "ms_operg_transpose": round(ms_a, 3),
+        "roofline": {"bound": "fp64 (matrix-free: trigonometric values recomputed)",
+                     "kernel": "k_bm_synth + k_bm_proj", "achieved": round(tf, 3),
+                     "peak": MFMA_PEAK["f64"], "unit": "TFLOP/s",
+                     "frac": round(tf / MFMA_PEAK["f64"], 4), "traffic": None,
+                     "algorithmic_flops": 2 * flops,
+                     "note": "2 flops per space-matrix entry, time and direction of use; the "
+                             "sin/cos evaluations (one per entry per pass of 4 or 2 times) are "
+                             "not counted"},
+        "parity_vs_oracle": {"times": len(sample), "operg_rel_err": err_f,
+                             "operg_transpose_rel_err": err_a},
+        "setup_s": round(setup_s, 2),
+        "cpu_baseline": {"value": round(ny * nx * len(sample) / cpu_s, 1),
+                         "unit": "grid-point-times/s", "cores": 1, "kind": "port",
+                         "sample": f"FP64 oracle (sparse space matrix, as basis.py) operg + "
+                                   f"operg_transpose at {len(sample)} of the {T} times "
+                                   f"({cpu_s:.1f} s; matrix set-up {orc_setup:.1f} s not timed)"}}
+
+
 def ptr_(t):
     from vardyn_amd.qg import ptr
     return ptr(t)
@@ -597,6 +712,9 @@ def main():
         return
     if args.tiled:
         print(json.dumps(tiled(args)), flush=True)
+        return
+    if args.basis:
+        print(json.dumps(basis(args)), flush=True)
         return
     if args.gemm_only:
         from vardyn_amd.qg import QgPlan

@@ -56,6 +56,9 @@ class BasisBmOracle:
     """``Basis_bm`` on a grid given by 2-D ``lon``, ``lat`` (degrees) and an
     optional boolean ``mask`` (True = masked), SSH component only."""
 
+    controls_initial_state = False
+    time_dependent = True  # operg(t_days, X) / operg_transpose(t_days, ad)
+
     def __init__(self, lon, lat, mask=None, flux=False, facns=1.0, facnlt=2.0, npsp=3.5,
                  facpsp=1.5, lmin=80.0, lmax=970.0, lmeso=300.0, tmeso=20.0, sloptdec=-1.28,
                  factdec=0.5, tdecmin=2.5, tdecmax=40.0, Qmax=1e-3, slopQ=-5.0, norm_time=True,

@@ -169,7 +169,9 @@ class CostOracle:
                 misfits[t] = m
                 Jo += m.dot(self.R.inv(m))
             if (not self.basis.controls_initial_state) and cps[i] % self.dtbasis == 0:
-                params = self.basis.operg(X)
+                # time-dependent bases take the time in days (tools_4Dvar.py:297)
+                params = self.basis.operg(t / 3600 / 24, X) \
+                    if getattr(self.basis, "time_dependent", False) else self.basis.operg(X)
             states[t] = (var.copy(), params.copy())
             var = self.M.step(var, params, nstep)
         t = self.T[-1]
@@ -195,7 +197,9 @@ class CostOracle:
             t = self.T[cps[i]]
             advar, adparams = self.M.step_adj(advar, adparams, states[t][0], nstep)
             if (not self.basis.controls_initial_state) and cps[i] % self.dtbasis == 0:
-                adX = adX + self.basis.operg_transpose(adparams)
+                adX = adX + (self.basis.operg_transpose(t / 3600 / 24, adparams)
+                             if getattr(self.basis, "time_dependent", False)
+                             else self.basis.operg_transpose(adparams))
                 adparams = np.zeros_like(adparams)
             if self.H.is_obs_time(t):
                 advar = advar + self.H.adj(t, states[t][0], misfits[t])

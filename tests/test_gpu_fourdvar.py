@@ -121,3 +121,43 @@ def test_graph_replay_matches_eager():
             # unrelated eager work between replays
             pb.plan.step(pb.plan.field(pb.truth0), None, 3)
             torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("prec", [True, False])
+def test_cost_grad_bm_basis_f64(prec):
+    """SSH-flux control through the balanced-motion wavelet basis (Basis_bm,
+    basis.py:472-936) with its per-coefficient Q as sigma_b: GPU engine
+    (batched synthesis / transpose over the basis times) against the oracle's
+    cost_and_grad with the oracle basis (operg at each basis time)."""
+    from types import SimpleNamespace
+    from oracle.basis_bm import BasisBmOracle
+    from vardyn_amd.basis_bm import Basis_bm
+    c = small_case()
+    pb = build_problem(c, precision="float64", control="offset")
+    cps = np.asarray(pb.checkpoints)
+    bseg = [i for i in range(len(cps) - 1) if cps[i] % c.checkpoint_steps == 0]
+    tb = pb.T[cps[bseg]] / 3600 / 24
+    lon, lat = np.meshgrid(-60 + 0.09 * np.arange(c.nx), 35 + 0.09 * np.arange(c.ny))
+    kw = dict(lmin=60.0, lmax=250.0, flux=True)
+    gb = Basis_bm(SimpleNamespace(lon=lon, lat=lat, mask=None), **kw)
+    _, Q = gb.set_basis(tb, return_q=True)
+    orc = BasisBmOracle(lon, lat, None, **kw)
+    orc.set_basis(tb)
+    eng = FourDVar(pb.plan, pb.H, cps, pb.T, control="basis", basis=gb, sigma_b=Q, prec=prec,
+                   dtbasis=c.checkpoint_steps, tape="full")
+    rng = np.random.default_rng(4)
+    X = rng.standard_normal(gb.nbasis) * (1.0 if prec else Q) * 30.0
+    J, G = eng.cost_and_grad(X)
+    qg = QgmOracle(dx=c.dx, dy=c.dx, dt=c.dt, c=c.c, f=c.f0, time_scheme=c.time_scheme,
+                   shape=(c.ny, c.nx))
+    H = ObsopOracle(pb.grid.lon, pb.grid.lat, pb.grid.DX, pb.grid.DY, pb.T, c.dt, pb.dict_obs,
+                    Npix=4)
+    ref = CostOracle(ModelOracle(qg), H, orc, cps, pb.T, dtbasis=c.checkpoint_steps, B=Cov(Q),
+                     Xb=np.zeros(gb.nbasis), prec=prec)
+    Jr, Gr = ref.cost_and_grad(X)
+    assert J == pytest.approx(Jr, rel=1e-10)
+    assert rel(G, Gr) < 1e-9
+    # the same evaluation replayed from a captured graph
+    eng.capture()
+    J2, G2 = eng.cost_and_grad(X)
+    assert J2 == J and np.array_equal(G2, G)

@@ -27,6 +27,7 @@ from __future__ import annotations
 
 import ctypes as C
 import datetime
+from types import SimpleNamespace
 
 import numpy as np
 import torch
@@ -67,6 +68,7 @@ class Basis_bm:
     from 1950-01-01 to the experiment's start (basis.py:528)."""
 
     controls_initial_state = False
+    time_dependent = True
 
     def __init__(self, State, flux=False, facns=1.0, facnlt=2.0, npsp=3.5, facpsp=1.5,
                  lmin=80.0, lmax=970.0, lmeso=300.0, tmeso=20.0, sloptdec=-1.28, factdec=0.5,
@@ -276,34 +278,50 @@ class Basis_bm:
         return torch.empty(max(1, b.value // 8), dtype=torch.float64, device=self.device)
 
     # ----------------------------------------------------------------- operators
-    def operg_batch(self, times, X):
-        """SSH fields (T, ny, nx) at ``times`` from coefficients ``X`` (device tensor out)."""
+    def prepare(self, times):
+        """Time factors and workspace of a fixed set of basis times, on the
+        device, so that ``operg_prepared`` / ``operg_transpose_prepared`` only
+        launch kernels (legal inside a captured HIP graph)."""
         times = np.atleast_1d(np.asarray(times, dtype=np.float64))
-        T = times.size
-        x = torch.as_tensor(X, dtype=torch.float64).reshape(-1).to(self.device).contiguous()
-        if x.numel() != self.nbasis:
-            raise VardynError(f"expected {self.nbasis} coefficients, got {x.numel()}")
-        gt = torch.as_tensor(self.time_factors(times), device=self.device)
-        out = torch.empty((T, self.ny, self.nx), dtype=torch.float64, device=self.device)
-        ws = self._workspace(T)
-        _lib.call("vd_bm_operg", self._h, T, ptr(gt), ptr(x), ptr(out), ptr(ws),
+        return SimpleNamespace(times=times, T=int(times.size),
+                               gt=torch.as_tensor(self.time_factors(times), device=self.device),
+                               ws=self._workspace(times.size))
+
+    def operg_prepared(self, prep, X, out):
+        """out (T, ny, nx) float64 = the fields at the prepared times from the
+        float64 device coefficients ``X``."""
+        if X.numel() != self.nbasis or out.numel() != prep.T * self.ny * self.nx:
+            raise VardynError("operg_prepared: coefficient or output size mismatch")
+        _lib.call("vd_bm_operg", self._h, prep.T, ptr(prep.gt), ptr(X), ptr(out), ptr(prep.ws),
                   current_stream_ptr(self.device))
         return out
 
+    def operg_transpose_prepared(self, prep, adssh, adX):
+        """adX += sum over the prepared times of operg_transpose(t, adssh[t])."""
+        if adX.numel() != self.nbasis or adssh.numel() != prep.T * self.ny * self.nx:
+            raise VardynError("operg_transpose_prepared: size mismatch")
+        _lib.call("vd_bm_operg_adj", self._h, prep.T, ptr(prep.gt), ptr(adssh), ptr(adX),
+                  ptr(prep.ws), current_stream_ptr(self.device))
+        return adX
+
+    def operg_batch(self, times, X):
+        """SSH fields (T, ny, nx) at ``times`` from coefficients ``X`` (device tensor out)."""
+        prep = self.prepare(times)
+        x = torch.as_tensor(X, dtype=torch.float64).reshape(-1).to(self.device).contiguous()
+        if x.numel() != self.nbasis:
+            raise VardynError(f"expected {self.nbasis} coefficients, got {x.numel()}")
+        out = torch.empty((prep.T, self.ny, self.nx), dtype=torch.float64, device=self.device)
+        return self.operg_prepared(prep, x, out)
+
     def operg_transpose_batch(self, times, adssh, adX=None):
         """Sum over ``times`` of operg_transpose(t, adssh[t]); ``adssh`` (T, ny, nx)."""
-        times = np.atleast_1d(np.asarray(times, dtype=np.float64))
-        T = times.size
-        a = torch.as_tensor(adssh, dtype=torch.float64).to(self.device).reshape(T, self.ny,
+        prep = self.prepare(times)
+        a = torch.as_tensor(adssh, dtype=torch.float64).to(self.device).reshape(prep.T, self.ny,
                                                                                 self.nx)
         a = torch.nan_to_num(a).contiguous()
         if adX is None:
             adX = torch.zeros(self.nbasis, dtype=torch.float64, device=self.device)
-        gt = torch.as_tensor(self.time_factors(times), device=self.device)
-        ws = self._workspace(T)
-        _lib.call("vd_bm_operg_adj", self._h, T, ptr(gt), ptr(a), ptr(adX), ptr(ws),
-                  current_stream_ptr(self.device))
-        return adX
+        return self.operg_transpose_prepared(prep, a, adX)
 
     def operg(self, t, X, State=None):
         """basis.py:863-901 (SSH): the field at time ``t``; into

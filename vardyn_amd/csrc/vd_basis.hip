@@ -16,7 +16,11 @@
 // summation order, so results are deterministic:
 //   k_bm_time   C[c][q][t] = sum_it g_f(t, it) X[f, it, P, q]   (q: direction, cos/sin)
 //   k_bm_synth  ssh[t][p] = sum over centres c covering p, q of G(p, c, q) C[c][q][t]
-//   k_bm_proj   D[c][q][t] = sum over points p of c's box of G(p, c, q) adssh[t][p]
+//   k_bm_proj   Dp[k][q][t] = sum over the points of chunk k of a centre's box of
+//               G(p, c, q) adssh[t][p] (boxes cut into chunks of kBmChunk points,
+//               so the large scales' boxes, up to the whole grid, spread over
+//               many workgroups)
+//   k_bm_proj_sum  D[c][q][t] = sum of c's chunks in order
 //   k_bm_time_adj  adX[f, it, P, q] += sum_t g_f(t, it) D[c][q][t]
 // The trigonometric values of a (point, centre) pair are computed once for
 // the whole batch of times.
@@ -31,6 +35,7 @@ constexpr double kKm2Deg = 1.0 / 110;
 constexpr int kBmMaxTheta = 16;  // directions per frequency (9 with the reference defaults)
 constexpr int kBmTB = 4;         // times per pass of the synthesis gather
 constexpr int kBmPB = 2;         // times per pass of the projection
+constexpr int kBmChunk = 1024;   // box points per projection workgroup
 
 struct BmTables {
   int ny, nx, nf, ntheta, nc;
@@ -52,6 +57,10 @@ struct BmTables {
   const int* gtoff;        // nf: offset of frequency f's (T x nt_f) block of time factors / T
   const int64_t* pptr;     // ny * nx + 1: centres covering point p
   const int* plist;
+  int nk;                  // projection chunks
+  const int* kc;           // nk: centre of chunk k
+  const int* kstart;       // nk: first box point of chunk k (row-major in the box)
+  const int* kptr;         // nc + 1: chunks of centre c
 };
 
 __device__ __forceinline__ double bm_window(double x) {
@@ -129,14 +138,15 @@ __global__ __launch_bounds__(256) void k_bm_synth(BmTables B, int T, int t0,
     if (t < nt) ssh[(long long)(t0 + t) * np + p] = acc[t];
 }
 
-// D[c][q][t] for times t0 .. t0 + PB - 1: one workgroup of 4 waves per
-// centre, fixed reduction order (per-thread sums over a strided set of box
-// points, a shuffle tree per wave, then the four waves in a fixed order).
+// Dp[k][q][t] for times t0 .. t0 + PB - 1: one workgroup of 4 waves per
+// chunk of a centre's box, fixed reduction order (per-thread sums over a
+// strided set of the chunk's points, a shuffle tree per wave, then the four
+// waves in a fixed order).
 __global__ __launch_bounds__(256) void k_bm_proj(BmTables B, int T, int t0,
                                                  const double* __restrict__ adssh,
                                                  double* __restrict__ D) {
   __shared__ double red[4][2 * kBmMaxTheta * kBmPB];
-  const int c = blockIdx.x;
+  const int k = blockIdx.x, c = B.kc[k];
   const int f = B.cf[c];
   const int4 bx = B.cbox[c];
   const int nt = min(kBmPB, T - t0);
@@ -149,7 +159,8 @@ __global__ __launch_bounds__(256) void k_bm_proj(BmTables B, int T, int t0,
     for (int t = 0; t < kBmPB; ++t) part[q][t] = 0;
   const int bw = bx.w - bx.z + 1, bh = bx.y - bx.x + 1;
   const int npts = (bw > 0 && bh > 0) ? bw * bh : 0;
-  for (int e = threadIdx.x; e < npts; e += blockDim.x) {
+  const int e1 = min(npts, B.kstart[k] + kBmChunk);
+  for (int e = B.kstart[k] + threadIdx.x; e < e1; e += blockDim.x) {
     const int i = bx.x + e / bw, j = bx.z + e % bw;
     const int p = i * B.nx + j;
     if (B.mask && B.mask[p]) continue;
@@ -188,7 +199,22 @@ __global__ __launch_bounds__(256) void k_bm_proj(BmTables B, int T, int t0,
   for (int e = threadIdx.x; e < q2 * kBmPB; e += blockDim.x) {
     const int q = e / kBmPB, t = e % kBmPB;
     if (t < nt)
-      D[((int64_t)c * q2 + q) * T + t0 + t] = (red[0][e] + red[1][e]) + (red[2][e] + red[3][e]);
+      D[((int64_t)k * q2 + q) * T + t0 + t] = (red[0][e] + red[1][e]) + (red[2][e] + red[3][e]);
+  }
+}
+
+// D[c][q][t] = sum over the chunks of centre c, in order, of Dp[k][q][t]
+__global__ void k_bm_proj_sum(BmTables B, int T, const double* __restrict__ Dp,
+                              double* __restrict__ D) {
+  const int q2 = 2 * B.ntheta;
+  const int64_t n = (int64_t)B.nc * q2 * T;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int c = (int)(e / ((int64_t)q2 * T));
+    const int64_t r = e - (int64_t)c * q2 * T;  // q * T + t
+    double acc = 0;
+    for (int k = B.kptr[c]; k < B.kptr[c + 1]; ++k) acc += Dp[(int64_t)k * q2 * T + r];
+    D[e] = acc;
   }
 }
 
@@ -226,6 +252,7 @@ using namespace vd;
 
 struct vd_bm_basis {
   BmTables B;
+  std::vector<int> kc, kstart, kptr;
   std::vector<void*> owned;
   long long nbasis;
   int gt_total;  // sum of nt_f
@@ -293,8 +320,18 @@ int vd_bm_create(const vd_bm_params* prm, vd_bm_basis** out) {
   }
   b->gt_total = acc;
   std::vector<int4> box(P.nc);
-  for (int c = 0; c < P.nc; ++c)
+  b->kptr.assign(P.nc + 1, 0);
+  for (int c = 0; c < P.nc; ++c) {
     box[c] = make_int4(P.cbox[4 * c], P.cbox[4 * c + 1], P.cbox[4 * c + 2], P.cbox[4 * c + 3]);
+    const long long npts = box[c].x > box[c].y ? 0
+                           : (long long)(box[c].y - box[c].x + 1) * (box[c].w - box[c].z + 1);
+    for (long long e = 0; e < npts; e += kBmChunk) {
+      b->kc.push_back(c);
+      b->kstart.push_back((int)e);
+    }
+    b->kptr[c + 1] = (int)b->kc.size();
+  }
+  B.nk = (int)b->kc.size();
   int rc = VD_OK;
   if ((rc = bm_upload(b, P.lon1, P.nx, &B.lon1)) || (rc = bm_upload(b, P.lat1, P.ny, &B.lat1)) ||
       (rc = bm_upload(b, P.mask, P.mask ? np : 0, &B.mask)) ||
@@ -309,7 +346,10 @@ int vd_bm_create(const vd_bm_params* prm, vd_bm_basis** out) {
       (rc = bm_upload(b, P.fnt, P.nf, &B.fnt)) ||
       (rc = bm_upload(b, (const int*)gtoff.data(), P.nf, &B.gtoff)) ||
       (rc = bm_upload(b, P.pptr, np + 1, &B.pptr)) ||
-      (rc = bm_upload(b, P.plist, (size_t)P.pptr[np], &B.plist))) {
+      (rc = bm_upload(b, P.plist, (size_t)P.pptr[np], &B.plist)) ||
+      (rc = bm_upload(b, (const int*)b->kc.data(), b->kc.size(), &B.kc)) ||
+      (rc = bm_upload(b, (const int*)b->kstart.data(), b->kstart.size(), &B.kstart)) ||
+      (rc = bm_upload(b, (const int*)b->kptr.data(), b->kptr.size(), &B.kptr))) {
     vd_bm_destroy(b);
     return rc;
   }
@@ -333,7 +373,8 @@ int vd_bm_info(const vd_bm_basis* b, long long* nbasis, int* gt_total) {
 
 int vd_bm_workspace_size(const vd_bm_basis* b, int T, size_t* bytes) {
   VD_REQUIRE(b && bytes && T > 0, VD_ERR_INVALID_ARG, "null argument or T < 1");
-  *bytes = (size_t)b->B.nc * 2 * b->B.ntheta * T * sizeof(double);
+  // C (or D) per centre, plus the projection's per-chunk partial sums
+  *bytes = (size_t)(b->B.nc + b->B.nk) * 2 * b->B.ntheta * T * sizeof(double);
   return VD_OK;
 }
 
@@ -361,9 +402,14 @@ int vd_bm_operg_adj(vd_bm_basis* b, int T, const double* gt, const double* adssh
   const BmTables& B = b->B;
   hipStream_t s = static_cast<hipStream_t>(stream);
   double* D = static_cast<double*>(work);
+  double* Dp = D + (size_t)B.nc * 2 * B.ntheta * T;
   if (B.nc == 0) return VD_OK;
-  for (int t0 = 0; t0 < T; t0 += kBmPB)
-    hipLaunchKernelGGL(k_bm_proj, dim3(B.nc), dim3(256), 0, s, B, T, t0, adssh, D);
+  if (B.nk > 0)
+    for (int t0 = 0; t0 < T; t0 += kBmPB)
+      hipLaunchKernelGGL(k_bm_proj, dim3(B.nk), dim3(256), 0, s, B, T, t0, adssh, Dp);
+  const int64_t nd = (int64_t)B.nc * 2 * B.ntheta * T;
+  hipLaunchKernelGGL(k_bm_proj_sum, dim3((unsigned)std::min<int64_t>((nd + 255) / 256, 65535)),
+                     dim3(256), 0, s, B, T, Dp, D);
   const long long n = (long long)B.nc * 2 * B.ntheta;
   hipLaunchKernelGGL(k_bm_time_adj, dim3((unsigned)std::min<long long>((n + 255) / 256, 65535)),
                      dim3(256), 0, s, B, T, gt, D, adX);

@@ -21,7 +21,12 @@ reverse, from the last segment to the first:
 
 Controls: ``"ic"`` -- the control vector is the initial SSH (the reference
 computes this adjoint and drops it); ``"offset"`` -- the reference's
-``Basis_offset`` flux (basis.py:3939-4000).
+``Basis_offset`` flux (basis.py:3939-4000); ``"basis"`` -- the SSH flux of a
+time-dependent reduced basis (``basis=``, e.g. the balanced-motion wavelets of
+``vardyn_amd.basis_bm.Basis_bm``) with per-coefficient prior standard
+deviations ``sigma_b`` (its Q): the fields of all basis times come from one
+batched synthesis before the forward sweep, and the adjoint fields of all
+basis times go through one batched transpose after the reverse sweep.
 
 Tape: ``"full"`` keeps every stage state of the window in HBM during the
 forward sweep (no recomputation); ``"checkpoint"`` keeps only the segment
@@ -52,13 +57,15 @@ from .qg import QgPlan, current_stream_ptr, ptr
 class FourDVar:
     def __init__(self, plan: QgPlan, obsop, checkpoints, T, control="ic", sigma_b=None,
                  Xb=None, sigma_r=None, prec=False, dtbasis=None, tape="checkpoint", hb=None,
-                 land=None):
-        if control not in ("ic", "offset"):
-            raise ValueError("control must be 'ic' or 'offset'")
+                 land=None, basis=None):
+        if control not in ("ic", "offset", "basis"):
+            raise ValueError("control must be 'ic', 'offset' or 'basis'")
+        if (control == "basis") != (basis is not None):
+            raise ValueError("control 'basis' takes a basis (and only it does)")
         if tape not in ("checkpoint", "full", "auto"):
             raise ValueError("tape must be 'checkpoint', 'full' or 'auto'")
-        if sigma_b is not None and not np.isscalar(sigma_b):
-            raise NotImplementedError("only a scalar background standard deviation is supported")
+        if sigma_b is not None and not np.isscalar(sigma_b) and control != "basis":
+            raise NotImplementedError("a per-coefficient sigma_b needs control 'basis'")
         if prec and sigma_b is None:
             raise ValueError("preconditioning needs sigma_b")
         self.plan = plan
@@ -84,7 +91,7 @@ class FourDVar:
         self.nseg = len(cps) - 1
         self.nsteps = np.diff(cps).astype(int)
         self.n = plan.ny * plan.nx
-        self.nctrl = self.n if control == "ic" else 1
+        self.nctrl = self.n if control == "ic" else (basis.nbasis if control == "basis" else 1)
         if tape == "auto":
             # the whole-window tape when it takes at most 80% of the free HBM
             need = sum(plan.tape_bytes(int(k)) for k in self.nsteps)
@@ -132,6 +139,22 @@ class FourDVar:
         self.var0 = torch.zeros(fshape, dtype=self.dtype, **dd)
         self.land = land
         self._graph = None
+        self.basis = basis
+        self.sig_vec = None
+        if control == "basis":
+            if sigma_b is None:
+                raise ValueError("control 'basis' needs sigma_b (the basis's Q)")
+            sb = np.broadcast_to(np.asarray(sigma_b, dtype=np.float64), (self.nctrl,))
+            self.sig_vec = torch.as_tensor(np.ascontiguousarray(sb), **dd)
+            self.inv_s2_vec = 1.0 / self.sig_vec ** 2
+            self.wtmp = torch.empty((self.B, self.nctrl), dtype=torch.float64, **dd)
+            # segments that start at a basis time, and their times in days
+            self.bseg = [i for i in range(self.nseg) if self.cps[i] % self.dtbasis == 0]
+            self.bindex = {i: k for k, i in enumerate(self.bseg)}
+            self.bprep = basis.prepare(self.T[self.cps[self.bseg]] / (3600 * 24))
+            nb = len(self.bseg)
+            self.bfields = torch.empty((self.B, nb, plan.ny, plan.nx), dtype=torch.float64, **dd)
+            self.badj = torch.zeros((self.B, nb, plan.ny, plan.nx), dtype=torch.float64, **dd)
 
     # ------------------------------------------------------------ helpers
     def _s(self):
@@ -184,10 +207,22 @@ class FourDVar:
         X0 = X0.reshape(self.B, self.nctrl).contiguous()
         J, X = self.J, self.X
         J.zero_()
-        inv_sb2 = 0.0 if self.sigma_b is None else 1.0 / float(self.sigma_b) ** 2
+        inv_sb2 = 0.0 if (self.sigma_b is None or self.sig_vec is not None) \
+            else 1.0 / float(self.sigma_b) ** 2
         # background term (tools_4Dvar.py:262-271); J accumulates (Jo + Jb) / 2
         self._convert(self.Xb, X)
-        if self.sigma_b is not None:
+        if self.sig_vec is not None:
+            # per-coefficient standard deviations (tools_4Dvar.py:262-271)
+            if self.prec:
+                X.addcmul_(self.sig_vec, X0)
+                for b in range(self.B):
+                    self._dot(X0[b], X0[b], 0.5, J[b:b + 1])
+            else:
+                X.add_(X0)
+                torch.mul(X0, self.inv_s2_vec, out=self.wtmp)
+                for b in range(self.B):
+                    self._dot(X0[b], self.wtmp[b], 0.5, J[b:b + 1])
+        elif self.sigma_b is not None:
             self._axpby(self.sigma_b if self.prec else 1.0, X0, 1.0, X)
             for b in range(self.B):
                 self._dot(X0[b], X0[b], 0.5 if self.prec else 0.5 * inv_sb2, J[b:b + 1])
@@ -203,6 +238,9 @@ class FourDVar:
             s0.copy_(self.var0)
         params = self.params
         params.zero_()
+        if self.control == "basis":
+            for b in range(self.B):
+                self.basis.operg_prepared(self.bprep, X[b], self.bfields[b])
         # forward sweep
         for i in range(self.nseg):
             st, nxt = self.states[i], self.states[i + 1]
@@ -212,9 +250,12 @@ class FourDVar:
                 # Basis_offset.operg: params = X everywhere (basis.py:3968-3984)
                 params.copy_(X[:, 0].to(params.dtype).reshape(self.B, 1, 1)
                              .expand(self.B, plan.ny, plan.nx).reshape(self.fshape))
+            if self.control == "basis" and i in self.bindex:
+                # basis.operg(t, X, State=params) (tools_4Dvar.py:295-299)
+                params.copy_(self.bfields[:, self.bindex[i]].reshape(self.fshape))
             tape = self._tape(i) if self.tape_mode == "full" else None
             plan.step(st, self.hb, int(self.nsteps[i]), h1=nxt, tape=tape)
-            if self.control == "offset":
+            if self.control in ("offset", "basis"):
                 self._axpby(self._flux_scale(i), params, 1.0, nxt)
         if self.final_obs[0] >= 0:
             self._misfits(self.final_obs, self.states[-1], J, inv_r2)
@@ -229,7 +270,7 @@ class FourDVar:
         if self.final_obs[0] >= 0:
             self._obs_adj(self.final_obs, lam)
         for i in reversed(range(self.nseg)):
-            if self.control == "offset":
+            if self.control in ("offset", "basis"):
                 self._axpby(self._flux_scale(i), lam, 1.0, adpar)
             plan.step_adj(lam, self.states[i], self.hb, int(self.nsteps[i]), adh0=lam2,
                           tape=self._tape(i), tape_valid=self.tape_mode == "full")
@@ -239,11 +280,25 @@ class FourDVar:
                 for b in range(self.B):
                     self._dot(self.p64[b], self.ones64, 1.0, G[b])
                 adpar.zero_()
+            if self.control == "basis" and i in self.bindex:
+                # adX += basis.operg_transpose(t, adState.params) (tools_4Dvar.py:369-374),
+                # gathered here and transposed for all basis times after the sweep
+                self.badj[:, self.bindex[i]].copy_(adpar.reshape(self.B, plan.ny, plan.nx))
+                adpar.zero_()
             if self.seg_obs[i][0] >= 0:
                 self._obs_adj(self.seg_obs[i], lam)
         # gradient (tools_4Dvar.py:388-398)
         if self.control == "ic":
             self._convert(lam, G, nan_to_zero=True)
+        if self.control == "basis":
+            for b in range(self.B):
+                self.basis.operg_transpose_prepared(self.bprep, self.badj[b], G[b])
+            if self.prec:
+                G.mul_(self.sig_vec)
+                G.add_(X0)
+            else:
+                G.addcmul_(X0, self.inv_s2_vec)
+            return J, G
         if self.prec:
             self._axpby(0.0, G, float(self.sigma_b), G)
             self._axpby(1.0, X0, 1.0, G)
