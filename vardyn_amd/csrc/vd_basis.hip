@@ -22,8 +22,13 @@
 //               many workgroups)
 //   k_bm_proj_sum  D[c][q][t] = sum of c's chunks in order
 //   k_bm_time_adj  adX[f, it, P, q] += sum_t g_f(t, it) D[c][q][t]
-// The trigonometric values of a (point, centre) pair are computed once for
-// the whole batch of times.
+// The phase is separable on the rectilinear grid: dx depends on the column
+// only and dy on the row only, so cos|sin(kx dx + ky dy) are products of
+// per-column and per-row cos/sin (angle addition), and the taper is the
+// product of w(dx/DX) and w(dy/DX).  k_bm_tables evaluates those once per
+// centre, column or row, and direction when the basis is created; the
+// operators then read them (a few loads and multiply-adds per entry instead
+// of a sin/cos evaluation).
 #include <cmath>
 #include <cstring>
 
@@ -57,6 +62,15 @@ struct BmTables {
   const int* gtoff;        // nf: offset of frequency f's (T x nt_f) block of time factors / T
   const int64_t* pptr;     // ny * nx + 1: centres covering point p
   const int* plist;
+  // separable tables per centre: columns xoff[c] .. xoff[c] + bw - 1 and rows
+  // yoff[c] .. yoff[c] + bh - 1 of the global column / row tables
+  int ncol, nrow;          // table lengths
+  const int* xoff;
+  const int* yoff;
+  double* wx;              // ncol: w(dx / DX)
+  double* wy;              // nrow
+  double2* cx;             // ntheta * ncol: (cos, sin)(kx dx), direction-major
+  double2* cy;             // ntheta * nrow: (cos, sin)(ky dy)
   int nk;                  // projection chunks
   const int* kc;           // nk: centre of chunk k
   const int* kstart;       // nk: first box point of chunk k (row-major in the box)
@@ -68,21 +82,50 @@ __device__ __forceinline__ double bm_window(double x) {
   return c * c;
 }
 
-// sqrt(2) * taper of centre c at grid point (i, j), and the offsets (km)
-__device__ __forceinline__ void bm_point(const BmTables& B, int c, int i, int j, double* xx,
-                                         double* yy, double* s2f) {
+// Separable tables of centre c (one workgroup per centre): the column part
+// dx = (lon_j - lon_c) / km2deg * cos(lat_c), the row part
+// dy = (lat_i - lat_c) / km2deg, their tapers and (cos, sin)(kx dx), (cos,
+// sin)(ky dy) for every direction.
+__global__ void k_bm_tables(BmTables B) {
 #pragma clang fp contract(off)
-  const double dx = B.dxf[B.cf[c]];
-  *xx = (B.lon1[j] - B.clon[c]) / kKm2Deg * B.ccos[c];
-  *yy = (B.lat1[i] - B.clat[c]) / kKm2Deg;
-  *s2f = M_SQRT2 * (bm_window(*xx / dx) * bm_window(*yy / dx));
+  const int c = blockIdx.x, f = B.cf[c];
+  const int4 bx = B.cbox[c];
+  if (bx.x > bx.y) return;
+  const double DX = B.dxf[f];
+  const int bw = bx.w - bx.z + 1, bh = bx.y - bx.x + 1;
+  for (int jx = threadIdx.x; jx < bw; jx += blockDim.x) {
+    const double xx = (B.lon1[bx.z + jx] - B.clon[c]) / kKm2Deg * B.ccos[c];
+    const int x = B.xoff[c] + jx;
+    B.wx[x] = bm_window(xx / DX);
+    for (int th = 0; th < B.ntheta; ++th) {
+      double sn, cs;
+      sincos(B.kx[f * B.ntheta + th] * xx, &sn, &cs);
+      B.cx[(size_t)th * B.ncol + x] = make_double2(cs, sn);
+    }
+  }
+  for (int iy = threadIdx.x; iy < bh; iy += blockDim.x) {
+    const double yy = (B.lat1[bx.x + iy] - B.clat[c]) / kKm2Deg;
+    const int y = B.yoff[c] + iy;
+    B.wy[y] = bm_window(yy / DX);
+    for (int th = 0; th < B.ntheta; ++th) {
+      double sn, cs;
+      sincos(B.ky[f * B.ntheta + th] * yy, &sn, &cs);
+      B.cy[(size_t)th * B.nrow + y] = make_double2(cs, sn);
+    }
+  }
 }
 
-__device__ __forceinline__ void bm_trig(const BmTables& B, int f, int th, double xx, double yy,
-                                        double* s, double* co) {
-#pragma clang fp contract(off)
-  const double arg = B.kx[f * B.ntheta + th] * xx + B.ky[f * B.ntheta + th] * yy;
-  sincos(arg, s, co);
+// sqrt(2) * taper of centre c at its box point (row iy, column jx)
+__device__ __forceinline__ double bm_s2f(const BmTables& B, int x, int y) {
+  return M_SQRT2 * (B.wx[x] * B.wy[y]);
+}
+
+// (cos, sin)(kx dx + ky dy) of direction th from the separable tables
+__device__ __forceinline__ void bm_trig(const BmTables& B, int th, int x, int y, double* s,
+                                        double* co) {
+  const double2 a = B.cx[(size_t)th * B.ncol + x], b = B.cy[(size_t)th * B.nrow + y];
+  *co = a.x * b.x - a.y * b.y;
+  *s = a.y * b.x + a.x * b.y;
 }
 
 // C[c][q][t] = sum_it g_f(t, it) X[fbase + it * blk + P * 2 ntheta + q]
@@ -120,13 +163,14 @@ __global__ __launch_bounds__(256) void k_bm_synth(BmTables B, int T, int t0,
   double acc[kBmTB] = {};
   const int q2 = 2 * B.ntheta;
   for (int64_t e = B.pptr[p]; e < B.pptr[p + 1]; ++e) {
-    const int c = B.plist[e], f = B.cf[c];
-    double xx, yy, s2f;
-    bm_point(B, c, i, j, &xx, &yy, &s2f);
+    const int c = B.plist[e];
+    const int4 bx = B.cbox[c];
+    const int x = B.xoff[c] + (j - bx.z), y = B.yoff[c] + (i - bx.x);
+    const double s2f = bm_s2f(B, x, y);
     const double* cc = C + ((long long)c * q2) * T + t0;
     for (int th = 0; th < B.ntheta; ++th) {
       double s, co;
-      bm_trig(B, f, th, xx, yy, &s, &co);
+      bm_trig(B, th, x, y, &s, &co);
       const double vc = s2f * co, vs = s2f * s;
 #pragma unroll
       for (int t = 0; t < kBmTB; ++t)
@@ -147,7 +191,6 @@ __global__ __launch_bounds__(256) void k_bm_proj(BmTables B, int T, int t0,
                                                  double* __restrict__ D) {
   __shared__ double red[4][2 * kBmMaxTheta * kBmPB];
   const int k = blockIdx.x, c = B.kc[k];
-  const int f = B.cf[c];
   const int4 bx = B.cbox[c];
   const int nt = min(kBmPB, T - t0);
   const int np = B.ny * B.nx;
@@ -167,13 +210,13 @@ __global__ __launch_bounds__(256) void k_bm_proj(BmTables B, int T, int t0,
     double a[kBmPB];
 #pragma unroll
     for (int t = 0; t < kBmPB; ++t) a[t] = t < nt ? adssh[(int64_t)(t0 + t) * np + p] : 0.0;
-    double xx, yy, s2f;
-    bm_point(B, c, i, j, &xx, &yy, &s2f);
+    const int x = B.xoff[c] + (j - bx.z), y = B.yoff[c] + (i - bx.x);
+    const double s2f = bm_s2f(B, x, y);
 #pragma unroll
     for (int th = 0; th < kBmMaxTheta; ++th) {
       if (th < B.ntheta) {
         double s, co;
-        bm_trig(B, f, th, xx, yy, &s, &co);
+        bm_trig(B, th, x, y, &s, &co);
         const double vc = s2f * co, vs = s2f * s;
 #pragma unroll
         for (int t = 0; t < kBmPB; ++t) {
@@ -273,6 +316,17 @@ static int bm_upload(vd_bm_basis* b, const V* src, size_t n, const V** dst) {
   return VD_OK;
 }
 
+template <typename V>
+static int bm_alloc(vd_bm_basis* b, size_t n, V** dst) {
+  *dst = nullptr;
+  if (n == 0) return VD_OK;
+  void* d = nullptr;
+  VD_HIP_TRY(hipMalloc(&d, n * sizeof(V)));
+  b->owned.push_back(d);
+  *dst = static_cast<V*>(d);
+  return VD_OK;
+}
+
 extern "C" {
 
 int vd_bm_create(const vd_bm_params* prm, vd_bm_basis** out) {
@@ -332,6 +386,18 @@ int vd_bm_create(const vd_bm_params* prm, vd_bm_basis** out) {
     b->kptr[c + 1] = (int)b->kc.size();
   }
   B.nk = (int)b->kc.size();
+  std::vector<int> xoff(P.nc), yoff(P.nc);
+  int ncol = 0, nrow = 0;
+  for (int c = 0; c < P.nc; ++c) {
+    xoff[c] = ncol;
+    yoff[c] = nrow;
+    if (box[c].x <= box[c].y) {
+      ncol += box[c].w - box[c].z + 1;
+      nrow += box[c].y - box[c].x + 1;
+    }
+  }
+  B.ncol = ncol;
+  B.nrow = nrow;
   int rc = VD_OK;
   if ((rc = bm_upload(b, P.lon1, P.nx, &B.lon1)) || (rc = bm_upload(b, P.lat1, P.ny, &B.lat1)) ||
       (rc = bm_upload(b, P.mask, P.mask ? np : 0, &B.mask)) ||
@@ -349,9 +415,23 @@ int vd_bm_create(const vd_bm_params* prm, vd_bm_basis** out) {
       (rc = bm_upload(b, P.plist, (size_t)P.pptr[np], &B.plist)) ||
       (rc = bm_upload(b, (const int*)b->kc.data(), b->kc.size(), &B.kc)) ||
       (rc = bm_upload(b, (const int*)b->kstart.data(), b->kstart.size(), &B.kstart)) ||
-      (rc = bm_upload(b, (const int*)b->kptr.data(), b->kptr.size(), &B.kptr))) {
+      (rc = bm_upload(b, (const int*)b->kptr.data(), b->kptr.size(), &B.kptr)) ||
+      (rc = bm_upload(b, (const int*)xoff.data(), xoff.size(), &B.xoff)) ||
+      (rc = bm_upload(b, (const int*)yoff.data(), yoff.size(), &B.yoff)) ||
+      (rc = bm_alloc(b, (size_t)ncol, &B.wx)) || (rc = bm_alloc(b, (size_t)nrow, &B.wy)) ||
+      (rc = bm_alloc(b, (size_t)ncol * P.ntheta, &B.cx)) ||
+      (rc = bm_alloc(b, (size_t)nrow * P.ntheta, &B.cy))) {
     vd_bm_destroy(b);
     return rc;
+  }
+  if (P.nc > 0) {
+    hipLaunchKernelGGL(k_bm_tables, dim3(P.nc), dim3(256), 0, 0, B);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+      vd_bm_destroy(b);
+      VD_HIP_TRY(e);
+    }
   }
   *out = b;
   return VD_OK;
