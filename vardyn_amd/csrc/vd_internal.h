@@ -134,7 +134,8 @@ struct Group {
   void* mem = nullptr;        // RgMember<T>[n]
   int4* stage_tiles = nullptr;  // (member, tile row, tile column) of 64 x 16 point tiles
   std::vector<int> stage_prefix;
-  int4* adj_tiles = nullptr;    // the same for 64 x 16 tiles of the adjoint stage
+  int4* adj_tiles = nullptr;    // the same for the adjoint stage's tiles: 64 x 16,
+  bool adj_narrow = false;      // or 32 x 32 when every member is at most 32 points wide
   std::vector<int> adj_prefix;
   void* gargs[4] = {nullptr, nullptr, nullptr, nullptr};  // RgGemm<T>[n] per inversion stage
   // (member, product, tile) of the 32 x 32 GEMM tiles per stage and K-split

@@ -949,11 +949,14 @@ __device__ __forceinline__ void adj_terms(const T* h, const T* q, int pitch, int
 // from LDS.  Phase 2 gathers, for each output point p,
 //   lambda_q(p) from the T points whose upwind / diffusion stencils read q(p),
 //   lambda_h(p) from lambda_u, lambda_v at the velocity points that read h(p).
+// The tile is AX x AY points (AX x AY threads): 64 x 16 in general, 32 x 32
+// for ragged groups whose members are all at most 32 points wide (the tiles
+// of QgmWithTiles), where a 64-point-wide tile would leave half its threads
+// idle.
 constexpr int kAX = 64, kAY = 16, kAH = 2;
-constexpr int kRW = kAX + 2 * kAH, kRH = kAY + 2 * kAH;
-constexpr int kHW = kAX + 4 * kAH, kHH = kAY + 4 * kAH;  // h, q tile: halo 4
+constexpr int kAXN = 32, kAYN = 32;  // narrow members
 
-template <typename T>
+template <typename T, int AX = kAX, int AY = kAY>
 __device__ __forceinline__ void adj_tile(const Phys<T>& P, int i0, int j0, const T* h, const T* q,
                                          const T* s0, const T* s1, const T* s2, T a0, T a1, T a2,
                                          const T* acc0, const T* acc1, const T* acc2, T* lq_out,
@@ -962,12 +965,14 @@ __device__ __forceinline__ void adj_tile(const Phys<T>& P, int i0, int j0, const
   // writes only its own point of those fields.  s0..s2 are read at the halo
   // points of the tile, which other blocks write, so they must not alias
   // lq_out.
+  constexpr int kRW = AX + 2 * kAH, kRH = AY + 2 * kAH;
+  constexpr int kHW = AX + 4 * kAH, kHH = AY + 4 * kAH;  // h, q tile: halo 4
   __shared__ T W[7][kRH][kRW];
   __shared__ T Hs[kHH * kHW], Qs[kHH * kHW];
   __shared__ uint8_t okU[kRH][kRW], okV[kRH][kRW];
   const int ny = P.ny, nx = P.nx;
-  const int tid = threadIdx.y * kAX + threadIdx.x;
-  for (int e = tid; e < kHH * kHW; e += kAX * kAY) {
+  const int tid = threadIdx.y * AX + threadIdx.x;
+  for (int e = tid; e < kHH * kHW; e += AX * AY) {
     const int r = e / kHW, c = e - r * kHW;
     const size_t g = (size_t)min(max(i0 - 2 * kAH + r, 0), ny - 1) * nx +
                      min(max(j0 - 2 * kAH + c, 0), nx - 1);
@@ -975,7 +980,7 @@ __device__ __forceinline__ void adj_tile(const Phys<T>& P, int i0, int j0, const
     Qs[e] = q[g];
   }
   __syncthreads();
-  for (int r = tid; r < kRH * kRW; r += kAX * kAY) {
+  for (int r = tid; r < kRH * kRW; r += AX * AY) {
     const int li = r / kRW, lj = r - li * kRW;
     const int i = i0 - kAH + li, j = j0 - kAH + lj;
     const int hi = li + kAH, hj = lj + kAH;  // the same point in the h, q tile
@@ -1035,8 +1040,8 @@ __global__ __launch_bounds__(kAX * kAY) void k_adj_fused(
   adj_tile(P, by * kAY, bx * kAX, h, q, s0, s1, s2, a0, a1, a2, acc0, acc1, acc2, lq_out, lh_out);
 }
 
-template <typename T>
-__global__ __launch_bounds__(kAX * kAY) void k_adj_fused_rg(
+template <typename T, int AX, int AY>
+__global__ __launch_bounds__(AX * AY) void k_adj_fused_rg(
     const RgMember<T>* __restrict__ mem, const int4* __restrict__ tiles, const T* __restrict__ h,
     const T* __restrict__ q, const T* __restrict__ s0, const T* __restrict__ s1,
     const T* __restrict__ s2, T a0, T a1, T a2, const T* acc0, const T* acc1, const T* acc2,
@@ -1047,8 +1052,8 @@ __global__ __launch_bounds__(kAX * kAY) void k_adj_fused_rg(
   const size_t zo = mem[t.x].foff;
   VD_B(h); VD_B(q); VD_B(s0); VD_B(s1); VD_B(s2); VD_B(acc0); VD_B(acc1); VD_B(acc2);
   VD_B(lq_out); VD_B(lh_out);
-  adj_tile(P, t.y * kAY, t.z * kAX, h, q, s0, s1, s2, a0, a1, a2, acc0, acc1, acc2, lq_out,
-           lh_out);
+  adj_tile<T, AX, AY>(P, t.y * AY, t.z * AX, h, q, s0, s1, s2, a0, a1, a2, acc0, acc1, acc2,
+                      lq_out, lh_out);
 }
 
 
@@ -1329,7 +1334,12 @@ void launch_stage_adj_rg(const Group& g, int nact, const T* h, const T* q, const
                          const T* acc1, const T* acc2, T* lq_out, T* lh_out, hipStream_t s) {
   const int nb = g.adj_prefix[nact];
   if (nb == 0) return;
-  hipLaunchKernelGGL(k_adj_fused_rg<T>, dim3(nb), dim3(kAX, kAY), 0, s,
+  if (g.adj_narrow)
+    hipLaunchKernelGGL((k_adj_fused_rg<T, kAXN, kAYN>), dim3(nb), dim3(kAXN, kAYN), 0, s,
+                       static_cast<const RgMember<T>*>(g.mem), g.adj_tiles, h, q, s0, s1, s2, a0, a1,
+                       a2, acc0, acc1, acc2, lq_out, lh_out);
+  else
+  hipLaunchKernelGGL((k_adj_fused_rg<T, kAX, kAY>), dim3(nb), dim3(kAX, kAY), 0, s,
                      static_cast<const RgMember<T>*>(g.mem), g.adj_tiles, h, q, s0, s1, s2, a0, a1,
                      a2, acc0, acc1, acc2, lq_out, lh_out);
 }
@@ -1398,6 +1408,8 @@ int group_build(Group& g) {
   double total = 0;
   for (int b = 0; b < n; ++b) total += (double)g.plans[b]->nf * sizeof(T) * 4;
   const bool band = total > 256.0 * (1 << 20);
+  g.adj_narrow = true;
+  for (int b = 0; b < n; ++b) g.adj_narrow = g.adj_narrow && g.plans[b]->nx <= kAXN;
   g.stage_prefix.assign(n + 1, 0);
   g.adj_prefix.assign(n + 1, 0);
   for (int b = 0; b < n; ++b) {
@@ -1407,7 +1419,8 @@ int group_build(Group& g) {
     mem[b].foff = g.foff[b];
     mem[b].mdt = p.has_mdt ? static_cast<const T*>(p.mdt) : nullptr;
     rg_tiles(b, (p.nx + kSX - 1) / kSX, (p.ny + kSY - 1) / kSY, band, st);
-    rg_tiles(b, (p.nx + kAX - 1) / kAX, (p.ny + kAY - 1) / kAY, band, at);
+    const int ax = g.adj_narrow ? kAXN : kAX, ay = g.adj_narrow ? kAYN : kAY;
+    rg_tiles(b, (p.nx + ax - 1) / ax, (p.ny + ay - 1) / ay, band, at);
     g.stage_prefix[b + 1] = (int)st.size();
     g.adj_prefix[b + 1] = (int)at.size();
   }
