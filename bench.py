@@ -503,15 +503,24 @@ def tiled(args):
     ms_inv = timed(inv, 3)[0] / reps
     ms_st = timed(stage, 3)[0] / reps
     my = ts - 2
-    flops_launch = T * my * my * (2 * my)
-    inv_tf = flops_launch / (ms_inv / 4 * 1e-3) / 1e12
     st_bytes = T * ts * ts * (4 * 8 + 1)
     st_gbs = st_bytes / (ms_st * 1e-3) / 1e9
-    roof = {"bound": "mfma", "kernel": "k_gemm_rg (sine-matrix GEMMs of all tiles, FP64 MFMA, "
-                                       "4 launches per inversion)",
+    # tiles with interiors of at most 45 points invert in one fused launch
+    # (k_inv_small: the four sine products of each tile in LDS, FP64 FMA);
+    # larger ones in four GEMM launches (k_gemm_rg, FP64 MFMA)
+    fused = my <= 45
+    nlaunch = 1 if fused else 4
+    flops_launch = T * 4 * my * my * (2 * my) // nlaunch
+    inv_tf = flops_launch / (ms_inv / nlaunch * 1e-3) / 1e12
+    roof = {"bound": "fp64" if fused else "mfma",
+            "kernel": "k_inv_small (all four sine products of each tile in one workgroup, one "
+                      "launch per inversion)" if fused else
+                      "k_gemm_rg (sine-matrix GEMMs of all tiles, FP64 MFMA, 4 launches per "
+                      "inversion)",
             "achieved": round(inv_tf, 3), "peak": MFMA_PEAK["f64"], "unit": "TFLOP/s",
             "frac": round(inv_tf / MFMA_PEAK["f64"], 4), "traffic": None,
-            "avg_launch_us": round(ms_inv / 4 * 1e3, 3), "algorithmic_flops_per_launch": flops_launch}
+            "avg_launch_us": round(ms_inv / nlaunch * 1e3, 3),
+            "algorithmic_flops_per_launch": flops_launch}
     st_roof = {"bound": "hbm", "kernel": "k_stage_rg (stage stencil of all tiles)",
                "achieved": round(st_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                "frac": round(st_gbs / HBM_PEAK_GBS, 4), "bytes_per_launch": st_bytes,

@@ -161,3 +161,32 @@ def test_banded_group_tables_equal_row_major():
         for k in range(3):
             assert torch.equal(o[k], out[k][sl]), (b, k)
         del g1
+
+
+@pytest.mark.parametrize("n", [30, 44])
+def test_small_member_fused_inversion(n):
+    """Groups of equal-size small members (interiors up to 45 points: the tiles
+    of QgmWithTiles) invert each member in one workgroup; forward (with q_b and
+    h_b) and accumulating inversions against each member's own plan."""
+    from vardyn_amd.qg import QgPlan, ptr
+    from vardyn_amd.ragged import QgGroup
+    ny = nx = n + 2
+    plans = [QgPlan(ny, nx, 10e3, 9e3, 1200.0, 2.0 + 0.25 * k, 1e-4 * (1 + 0.1 * k),
+                    time_scheme="rk2") for k in range(5)]
+    g = QgGroup(plans)
+    tot, m = g.offsets[-1], ny * nx
+    gen = torch.Generator(device="cuda").manual_seed(13)
+    F, Fs, hb, acc = (torch.randn(tot, dtype=torch.float64, device="cuda", generator=gen)
+                      for _ in range(4))
+    out = torch.empty_like(F)
+    g.inversion(len(plans), ptr(F), ptr(out), Fsub=ptr(Fs), hb=ptr(hb))
+    acc2 = acc.clone()
+    g.inversion(len(plans), ptr(F), ptr(acc2), accumulate=True)
+    for b, p in enumerate(plans):
+        sl = slice(b * m, (b + 1) * m)
+        want = p.pv2h(F[sl].view(ny, nx), hb[sl].view(ny, nx), Fs[sl].view(ny, nx))
+        got = out[sl].view(ny, nx)
+        assert torch.linalg.vector_norm(got - want) <= 1e-13 * torch.linalg.vector_norm(want)
+        pz = p.pv2h(F[sl].view(ny, nx), None, None)
+        wacc = acc[sl].view(ny, nx) + pz
+        assert torch.linalg.vector_norm(acc2[sl].view(ny, nx) - wacc) <= 1e-13 * torch.linalg.vector_norm(wacc)

@@ -136,6 +136,8 @@ struct Group {
   std::vector<int> stage_prefix;
   int4* adj_tiles = nullptr;    // the same for the adjoint stage's tiles: 64 x 16,
   bool adj_narrow = false;      // or 32 x 32 when every member is at most 32 points wide
+  bool small = false;           // equal-size small FP64 members: fused inversion per member
+  void* small_mem = nullptr;    // RgSmall[n]
   std::vector<int> adj_prefix;
   void* gargs[4] = {nullptr, nullptr, nullptr, nullptr};  // RgGemm<T>[n] per inversion stage
   // (member, product, tile) of the 32 x 32 GEMM tiles per stage and K-split

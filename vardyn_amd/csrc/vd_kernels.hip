@@ -1344,12 +1344,114 @@ void launch_stage_adj_rg(const Group& g, int nact, const T* h, const T* q, const
                      a2, acc0, acc1, acc2, lq_out, lh_out);
 }
 
+// Helmholtz inversion of a small member in one workgroup (ragged groups of
+// equal-size members with interiors of at most kSmallM points, the tiles of
+// QgmWithTiles): the four sine products and the division by Lambda run on
+// the member's interior held in LDS, instead of four GEMM launches whose
+// 32 x 32 tiles would be mostly padding.  Per member:
+//   A = F - Fsub, B = S_y A, A = (B S_x) / Lambda, B = S_y A,
+//   out = hb + B S_x (interior; ring = hb) or out += B S_x (accumulate).
+// Lambda = gof (lcx[j] + lcy[i]) - gfc2 as in the GEMM epilogue.
+constexpr int kSmallM = 45;  // 2 m_y m_x + m_y^2 + m_x^2 doubles fit 64 KB of LDS
+struct RgSmall {
+  size_t foff;
+  const double* lcx;
+  const double* lcy;
+  double gof, gfc2;
+};
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_inv_small(const RgSmall* __restrict__ mem,
+                                                   const double* __restrict__ Sy, int ldy,
+                                                   const double* __restrict__ Sx, int ldx, int my,
+                                                   int mx, const double* F, const double* Fsub,
+                                                   const double* hb, double* out) {
+  extern __shared__ double sm[];
+  double* A = sm;
+  double* B = A + my * mx;
+  double* SY = B + my * mx;
+  double* SX = SY + my * my;
+  const RgSmall m = mem[blockIdx.x];
+  const int nx = mx + 2, ny = my + 2, tid = threadIdx.x;
+  F += m.foff;
+  if (Fsub) Fsub += m.foff;
+  if (hb) hb += m.foff;
+  out += m.foff;
+  for (int e = tid; e < my * my; e += 256) SY[e] = Sy[(size_t)(e / my) * ldy + e % my];
+  for (int e = tid; e < mx * mx; e += 256) SX[e] = Sx[(size_t)(e / mx) * ldx + e % mx];
+  for (int e = tid; e < my * mx; e += 256) {
+    const size_t o = (size_t)(e / mx + 1) * nx + e % mx + 1;
+    A[e] = Fsub ? F[o] - Fsub[o] : F[o];
+  }
+  __syncthreads();
+  for (int e = tid; e < my * mx; e += 256) {  // B = S_y A
+    const int i = e / mx, j = e % mx;
+    double v = 0;
+    for (int k = 0; k < my; ++k) v = fma(SY[i * my + k], A[k * mx + j], v);
+    B[e] = v;
+  }
+  __syncthreads();
+  for (int e = tid; e < my * mx; e += 256) {  // A = (B S_x) / Lambda
+    const int i = e / mx, j = e % mx;
+    double v = 0;
+    for (int k = 0; k < mx; ++k) v = fma(B[i * mx + k], SX[k * mx + j], v);
+    double lam;
+    {
+#pragma clang fp contract(off)
+      lam = m.gof * (m.lcx[j] + m.lcy[i]) - m.gfc2;
+    }
+    A[e] = v * (1.0 / lam);
+  }
+  __syncthreads();
+  for (int e = tid; e < my * mx; e += 256) {  // B = S_y A
+    const int i = e / mx, j = e % mx;
+    double v = 0;
+    for (int k = 0; k < my; ++k) v = fma(SY[i * my + k], A[k * mx + j], v);
+    B[e] = v;
+  }
+  __syncthreads();
+  for (int e = tid; e < my * mx; e += 256) {  // out = hb + B S_x, or out += B S_x
+    const int i = e / mx, j = e % mx;
+    double v = 0;
+    for (int k = 0; k < mx; ++k) v = fma(B[i * mx + k], SX[k * mx + j], v);
+    const size_t o = (size_t)(i + 1) * nx + j + 1;
+    if (MODE == INV_ACC)
+      out[o] += v;
+    else
+      out[o] = hb ? hb[o] + v : v;
+  }
+  if (MODE != INV_ACC)  // the ring from hb
+    for (int e = tid; e < 2 * nx + 2 * my; e += 256) {
+      const size_t o = e < nx ? (size_t)e
+                     : e < 2 * nx ? (size_t)(ny - 1) * nx + (e - nx)
+                     : e < 2 * nx + my ? (size_t)(e - 2 * nx + 1) * nx
+                                       : (size_t)(e - 2 * nx - my + 1) * nx + nx - 1;
+      out[o] = hb ? hb[o] : 0.0;
+    }
+}
+
 template <typename T>
 void launch_inversion_rg(const Group& g, int nact, const T* F, const T* Fsub, const T* hb, T* out,
                          int mode, hipStream_t s) {
   if constexpr (sizeof(T) == 8) {
     if (g.split_ns > 0) {
       launch_inversion_split_rg(g, nact, F, Fsub, hb, out, mode, s);
+      return;
+    }
+    if (g.small && mode != INV_TL) {
+      if (nact == 0) return;
+      const Plan& p0 = *g.plans[0];
+      const size_t lds = (size_t)(2 * p0.my * p0.mx + p0.my * p0.my + p0.mx * p0.mx) * 8;
+      if (mode == INV_ACC)
+        hipLaunchKernelGGL(k_inv_small<INV_ACC>, dim3(nact), dim3(256), lds, s,
+                           static_cast<const RgSmall*>(g.small_mem), static_cast<const double*>(p0.Sy),
+                           p0.MY, static_cast<const double*>(p0.Sx), p0.MX, p0.my, p0.mx, F, Fsub, hb,
+                           out);
+      else
+        hipLaunchKernelGGL(k_inv_small<INV_FWD>, dim3(nact), dim3(256), lds, s,
+                           static_cast<const RgSmall*>(g.small_mem), static_cast<const double*>(p0.Sy),
+                           p0.MY, static_cast<const double*>(p0.Sx), p0.MX, p0.my, p0.mx, F, Fsub, hb,
+                           out);
       return;
     }
   }
@@ -1429,6 +1531,24 @@ int group_build(Group& g) {
   if ((rc = upload_vec(st, (void**)&g.stage_tiles)) != VD_OK) return rc;
   if ((rc = upload_vec(at, (void**)&g.adj_tiles)) != VD_OK) return rc;
   if (g.split_ns > 0) return split_group_build(g);
+  // small equal-size FP64 members: fused inversion in one workgroup each
+  if constexpr (sizeof(T) == 8) {
+    const Plan& p0 = *g.plans[0];
+    bool small = p0.my <= kSmallM && p0.mx <= kSmallM && p0.nb == 1;
+    for (int b = 0; b < n && small; ++b)
+      small = g.plans[b]->ny == p0.ny && g.plans[b]->nx == p0.nx && g.plans[b]->nb == 1;
+    if (small) {
+      std::vector<RgSmall> sm(n);
+      for (int b = 0; b < n; ++b) {
+        const Plan& p = *g.plans[b];
+        double gfc2 = 0;
+        VD_HIP_TRY(hipMemcpy(&gfc2, p.gfc2_d, sizeof(double), hipMemcpyDeviceToHost));
+        sm[b] = RgSmall{g.foff[b], p.lam_cx, p.lam_cy, p.g / p.f0, gfc2};
+      }
+      if ((rc = upload_vec(sm, &g.small_mem)) != VD_OK) return rc;
+      g.small = true;
+    }
+  }
   for (int stg = 0; stg < 4; ++stg) {
     // 32 x 32 tiles with configuration 0's K-group count for each member (4, 2
     // or 1 from its chunk count), so that every member computes exactly what
@@ -1474,6 +1594,9 @@ void group_release(Group& g) {
       g.gtiles[st][c] = nullptr;
     }
   }
+  hipFree(g.small_mem);
+  g.small_mem = nullptr;
+  g.small = false;
   g.mem = nullptr;
   g.stage_tiles = g.adj_tiles = nullptr;
 }
