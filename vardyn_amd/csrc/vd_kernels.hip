@@ -1360,6 +1360,29 @@ struct RgSmall {
   double gof, gfc2;
 };
 
+// C[i][j] = sum_k L[i][k] R[k][j] (M x N, row-major operands in LDS), each
+// thread four rows of one column: the L values are the same for every lane
+// of a row block (broadcast reads) and each R value is read once for four
+// multiply-adds.  Every output sums over k in order with FMA.
+template <typename F>
+__device__ __forceinline__ void small_mm4(const double* L, int ldl, const double* R, int ldr,
+                                          int K, int M, int N, F&& store) {
+  const int nb = (M + 3) / 4;
+  for (int e = threadIdx.x; e < nb * N; e += 256) {
+    const int i0 = 4 * (e / N), j = e % N;
+    double acc[4] = {0, 0, 0, 0};
+    for (int k = 0; k < K; ++k) {
+      const double r = R[k * ldr + j];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (i0 + u < M) acc[u] = fma(L[(i0 + u) * ldl + k], r, acc[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (i0 + u < M) store(i0 + u, j, acc[u]);
+  }
+}
+
 template <int MODE>
 __global__ __launch_bounds__(256) void k_inv_small(const RgSmall* __restrict__ mem,
                                                    const double* __restrict__ Sy, int ldy,
@@ -1384,42 +1407,30 @@ __global__ __launch_bounds__(256) void k_inv_small(const RgSmall* __restrict__ m
     A[e] = Fsub ? F[o] - Fsub[o] : F[o];
   }
   __syncthreads();
-  for (int e = tid; e < my * mx; e += 256) {  // B = S_y A
-    const int i = e / mx, j = e % mx;
-    double v = 0;
-    for (int k = 0; k < my; ++k) v = fma(SY[i * my + k], A[k * mx + j], v);
-    B[e] = v;
-  }
+  // B = S_y A
+  small_mm4(SY, my, A, mx, my, my, mx, [&](int i, int j, double v) { B[i * mx + j] = v; });
   __syncthreads();
-  for (int e = tid; e < my * mx; e += 256) {  // A = (B S_x) / Lambda
-    const int i = e / mx, j = e % mx;
-    double v = 0;
-    for (int k = 0; k < mx; ++k) v = fma(B[i * mx + k], SX[k * mx + j], v);
+  // A = (B S_x) / Lambda
+  small_mm4(B, mx, SX, mx, mx, my, mx, [&](int i, int j, double v) {
     double lam;
     {
 #pragma clang fp contract(off)
       lam = m.gof * (m.lcx[j] + m.lcy[i]) - m.gfc2;
     }
-    A[e] = v * (1.0 / lam);
-  }
+    A[i * mx + j] = v * (1.0 / lam);
+  });
   __syncthreads();
-  for (int e = tid; e < my * mx; e += 256) {  // B = S_y A
-    const int i = e / mx, j = e % mx;
-    double v = 0;
-    for (int k = 0; k < my; ++k) v = fma(SY[i * my + k], A[k * mx + j], v);
-    B[e] = v;
-  }
+  // B = S_y A
+  small_mm4(SY, my, A, mx, my, my, mx, [&](int i, int j, double v) { B[i * mx + j] = v; });
   __syncthreads();
-  for (int e = tid; e < my * mx; e += 256) {  // out = hb + B S_x, or out += B S_x
-    const int i = e / mx, j = e % mx;
-    double v = 0;
-    for (int k = 0; k < mx; ++k) v = fma(B[i * mx + k], SX[k * mx + j], v);
+  // out = hb + B S_x, or out += B S_x
+  small_mm4(B, mx, SX, mx, mx, my, mx, [&](int i, int j, double v) {
     const size_t o = (size_t)(i + 1) * nx + j + 1;
     if (MODE == INV_ACC)
       out[o] += v;
     else
       out[o] = hb ? hb[o] + v : v;
-  }
+  });
   if (MODE != INV_ACC)  // the ring from hb
     for (int e = tid; e < 2 * nx + 2 * my; e += 256) {
       const size_t o = e < nx ? (size_t)e
