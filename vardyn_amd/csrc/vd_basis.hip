@@ -15,7 +15,10 @@
 // direction (no stored CSC), and every direction is a gather with a fixed
 // summation order, so results are deterministic:
 //   k_bm_time   C[c][q][t] = sum_it g_f(t, it) X[f, it, P, q]   (q: direction, cos/sin)
-//   k_bm_synth  ssh[t][p] = sum over centres c covering p, q of G(p, c, q) C[c][q][t]
+//   k_bm_synth  ssh[t][p] = sum over centres c covering p, q of G(p, c, q) C[c][q][t],
+//               per 16 x 16 tile of grid points: the coefficients of the centres
+//               covering the tile are staged in LDS for 8 times at a time, so
+//               that every point reads them as LDS broadcasts
 //   k_bm_proj   Dp[k][q][t] = sum over the points of chunk k of a centre's box of
 //               G(p, c, q) adssh[t][p] (boxes cut into chunks of kBmChunk points,
 //               so the large scales' boxes, up to the whole grid, spread over
@@ -38,8 +41,9 @@ namespace vd {
 
 constexpr double kKm2Deg = 1.0 / 110;
 constexpr int kBmMaxTheta = 16;  // directions per frequency (9 with the reference defaults)
-constexpr int kBmTB = 4;         // times per pass of the synthesis gather
-constexpr int kBmPB = 2;         // times per pass of the projection
+constexpr int kBmTB = 8;         // times per pass of the synthesis gather
+constexpr int kBmTile = 16;      // grid-point tile side of the synthesis
+constexpr int kBmCC = 24;        // centres staged in LDS at a time
 constexpr int kBmChunk = 1024;   // box points per projection workgroup
 
 struct BmTables {
@@ -71,6 +75,9 @@ struct BmTables {
   double* wy;              // nrow
   double2* cx;             // ntheta * ncol: (cos, sin)(kx dx), direction-major
   double2* cy;             // ntheta * nrow: (cos, sin)(ky dy)
+  int ntx, nty;            // synthesis tiles (kBmTile x kBmTile points)
+  const int* tptr;         // ntx * nty + 1: centres whose box meets tile t (ascending)
+  const int* tlist;
   int nk;                  // projection chunks
   const int* kc;           // nk: centre of chunk k
   const int* kstart;       // nk: first box point of chunk k (row-major in the box)
@@ -151,44 +158,77 @@ __global__ void k_bm_time(BmTables B, int T, const double* __restrict__ gt,
   }
 }
 
-// ssh[t][p] for times t0 .. t0 + TB - 1 (fewer at the end of the batch)
+// ssh[t][p] for times t0 .. t0 + TB - 1 (fewer at the end of the batch): a
+// workgroup per 16 x 16 tile of points; the centres meeting the tile, in
+// ascending order, are staged kBmCC at a time with their coefficients for the
+// pass's times.  Each point adds the centres whose box holds it in that same
+// order (so its sum is the one over its own list of centres).
 __global__ __launch_bounds__(256) void k_bm_synth(BmTables B, int T, int t0,
                                                   const double* __restrict__ C,
                                                   double* __restrict__ ssh) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  const int np = B.ny * B.nx;
-  if (p >= np) return;
-  const int i = p / B.nx, j = p % B.nx;
+  __shared__ double Cs[kBmCC * 2 * kBmMaxTheta * kBmTB];
+  __shared__ int4 Bx[kBmCC];
+  __shared__ int Xo[kBmCC], Yo[kBmCC];
+  const int tile = blockIdx.x;
+  const int ti = tile / B.ntx, tj = tile % B.ntx;
+  const int i = ti * kBmTile + threadIdx.x / kBmTile, j = tj * kBmTile + threadIdx.x % kBmTile;
+  const bool in = i < B.ny && j < B.nx && !(B.mask && B.mask[(size_t)i * B.nx + j]);
   const int nt = min(kBmTB, T - t0);
-  double acc[kBmTB] = {};
   const int q2 = 2 * B.ntheta;
-  for (int64_t e = B.pptr[p]; e < B.pptr[p + 1]; ++e) {
-    const int c = B.plist[e];
-    const int4 bx = B.cbox[c];
-    const int x = B.xoff[c] + (j - bx.z), y = B.yoff[c] + (i - bx.x);
-    const double s2f = bm_s2f(B, x, y);
-    const double* cc = C + ((long long)c * q2) * T + t0;
-    for (int th = 0; th < B.ntheta; ++th) {
-      double s, co;
-      bm_trig(B, th, x, y, &s, &co);
-      const double vc = s2f * co, vs = s2f * s;
+  double acc[kBmTB] = {};
+  const int e0 = B.tptr[tile], e1 = B.tptr[tile + 1];
+  for (int c0 = e0; c0 < e1; c0 += kBmCC) {
+    const int ncc = min(kBmCC, e1 - c0);
+    __syncthreads();
+    for (int e = threadIdx.x; e < ncc * q2 * kBmTB; e += 256) {
+      const int cc = e / (q2 * kBmTB), r = e % (q2 * kBmTB), q = r / kBmTB, t = r % kBmTB;
+      const int c = B.tlist[c0 + cc];
+      Cs[(cc * q2 + q) * kBmTB + t] = t < nt ? C[((int64_t)c * q2 + q) * T + t0 + t] : 0.0;
+    }
+    if (threadIdx.x < ncc) {
+      const int c = B.tlist[c0 + threadIdx.x];
+      Bx[threadIdx.x] = B.cbox[c];
+      Xo[threadIdx.x] = B.xoff[c];
+      Yo[threadIdx.x] = B.yoff[c];
+    }
+    __syncthreads();
+    if (!in) continue;
+    for (int cc = 0; cc < ncc; ++cc) {
+      const int4 bx = Bx[cc];
+      if (i < bx.x || i > bx.y || j < bx.z || j > bx.w) continue;
+      const int x = Xo[cc] + (j - bx.z), y = Yo[cc] + (i - bx.x);
+      const double s2f = bm_s2f(B, x, y);
+      const double* cs = Cs + cc * q2 * kBmTB;
+      for (int th = 0; th < B.ntheta; ++th) {
+        double sn, co;
+        bm_trig(B, th, x, y, &sn, &co);
+        const double vc = s2f * co, vs = s2f * sn;
 #pragma unroll
-      for (int t = 0; t < kBmTB; ++t)
-        if (t < nt) acc[t] += vc * cc[(2 * th) * T + t] + vs * cc[(2 * th + 1) * T + t];
+        for (int t = 0; t < kBmTB; ++t)
+          acc[t] += vc * cs[(2 * th) * kBmTB + t] + vs * cs[(2 * th + 1) * kBmTB + t];
+      }
     }
   }
+  if (i < B.ny && j < B.nx) {
+    const int np = B.ny * B.nx, p = i * B.nx + j;
 #pragma unroll
-  for (int t = 0; t < kBmTB; ++t)
-    if (t < nt) ssh[(long long)(t0 + t) * np + p] = acc[t];
+    for (int t = 0; t < kBmTB; ++t)
+      if (t < nt) ssh[(int64_t)(t0 + t) * np + p] = acc[t];
+  }
 }
 
 // Dp[k][q][t] for times t0 .. t0 + PB - 1: one workgroup of 4 waves per
 // chunk of a centre's box, fixed reduction order (per-thread sums over a
 // strided set of the chunk's points, a shuffle tree per wave, then the four
 // waves in a fixed order).
+// MAXTH bounds the directions (register arrays are sized by it) and PB is the
+// number of times per pass: <10, 4> for up to 10 directions (the reference's
+// defaults give 9), <16, 2> otherwise.
+template <int MAXTH, int PB>
 __global__ __launch_bounds__(256) void k_bm_proj(BmTables B, int T, int t0,
                                                  const double* __restrict__ adssh,
                                                  double* __restrict__ D) {
+  constexpr int kBmMaxTheta = MAXTH, kBmPB = PB;
   __shared__ double red[4][2 * kBmMaxTheta * kBmPB];
   const int k = blockIdx.x, c = B.kc[k];
   const int4 bx = B.cbox[c];
@@ -398,6 +438,21 @@ int vd_bm_create(const vd_bm_params* prm, vd_bm_basis** out) {
   }
   B.ncol = ncol;
   B.nrow = nrow;
+  // synthesis tiles: the centres whose box meets each 16 x 16 tile, ascending
+  B.ntx = (P.nx + kBmTile - 1) / kBmTile;
+  B.nty = (P.ny + kBmTile - 1) / kBmTile;
+  std::vector<std::vector<int>> per((size_t)B.ntx * B.nty);
+  for (int c = 0; c < P.nc; ++c) {
+    if (box[c].x > box[c].y) continue;
+    for (int ti = box[c].x / kBmTile; ti <= box[c].y / kBmTile; ++ti)
+      for (int tj = box[c].z / kBmTile; tj <= box[c].w / kBmTile; ++tj)
+        per[(size_t)ti * B.ntx + tj].push_back(c);
+  }
+  std::vector<int> tptr(per.size() + 1, 0), tlist;
+  for (size_t t = 0; t < per.size(); ++t) {
+    tlist.insert(tlist.end(), per[t].begin(), per[t].end());
+    tptr[t + 1] = (int)tlist.size();
+  }
   int rc = VD_OK;
   if ((rc = bm_upload(b, P.lon1, P.nx, &B.lon1)) || (rc = bm_upload(b, P.lat1, P.ny, &B.lat1)) ||
       (rc = bm_upload(b, P.mask, P.mask ? np : 0, &B.mask)) ||
@@ -417,6 +472,8 @@ int vd_bm_create(const vd_bm_params* prm, vd_bm_basis** out) {
       (rc = bm_upload(b, (const int*)b->kstart.data(), b->kstart.size(), &B.kstart)) ||
       (rc = bm_upload(b, (const int*)b->kptr.data(), b->kptr.size(), &B.kptr)) ||
       (rc = bm_upload(b, (const int*)xoff.data(), xoff.size(), &B.xoff)) ||
+      (rc = bm_upload(b, (const int*)tptr.data(), tptr.size(), &B.tptr)) ||
+      (rc = bm_upload(b, (const int*)tlist.data(), tlist.size(), &B.tlist)) ||
       (rc = bm_upload(b, (const int*)yoff.data(), yoff.size(), &B.yoff)) ||
       (rc = bm_alloc(b, (size_t)ncol, &B.wx)) || (rc = bm_alloc(b, (size_t)nrow, &B.wy)) ||
       (rc = bm_alloc(b, (size_t)ncol * P.ntheta, &B.cx)) ||
@@ -468,9 +525,8 @@ int vd_bm_operg(vd_bm_basis* b, int T, const double* gt, const double* X, double
   if (n > 0)
     hipLaunchKernelGGL(k_bm_time, dim3((unsigned)std::min<long long>((n + 255) / 256, 65535)),
                        dim3(256), 0, s, B, T, gt, X, C);
-  const int np = B.ny * B.nx;
   for (int t0 = 0; t0 < T; t0 += kBmTB)
-    hipLaunchKernelGGL(k_bm_synth, dim3((np + 255) / 256), dim3(256), 0, s, B, T, t0, C, ssh);
+    hipLaunchKernelGGL(k_bm_synth, dim3(B.ntx * B.nty), dim3(256), 0, s, B, T, t0, C, ssh);
   VD_HIP_TRY(hipGetLastError());
   return VD_OK;
 }
@@ -484,9 +540,14 @@ int vd_bm_operg_adj(vd_bm_basis* b, int T, const double* gt, const double* adssh
   double* D = static_cast<double*>(work);
   double* Dp = D + (size_t)B.nc * 2 * B.ntheta * T;
   if (B.nc == 0) return VD_OK;
-  if (B.nk > 0)
-    for (int t0 = 0; t0 < T; t0 += kBmPB)
-      hipLaunchKernelGGL(k_bm_proj, dim3(B.nk), dim3(256), 0, s, B, T, t0, adssh, Dp);
+  if (B.nk > 0) {
+    if (B.ntheta <= 10)
+      for (int t0 = 0; t0 < T; t0 += 4)
+        hipLaunchKernelGGL((k_bm_proj<10, 4>), dim3(B.nk), dim3(256), 0, s, B, T, t0, adssh, Dp);
+    else
+      for (int t0 = 0; t0 < T; t0 += 2)
+        hipLaunchKernelGGL((k_bm_proj<16, 2>), dim3(B.nk), dim3(256), 0, s, B, T, t0, adssh, Dp);
+  }
   const int64_t nd = (int64_t)B.nc * 2 * B.ntheta * T;
   hipLaunchKernelGGL(k_bm_proj_sum, dim3((unsigned)std::min<int64_t>((nd + 255) / 256, 65535)),
                      dim3(256), 0, s, B, T, Dp, D);
