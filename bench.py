@@ -671,14 +671,14 @@ def basis(args):
                    "nbasis": int(b.nbasis), "frequencies": int(b.nf), "directions": int(b.ntheta),
                    "centres": int(b.nc), "space_matrix_entries": int(b.nnz_space)},
         "ms_operg": round(ms_f, 3), "ms_operg_transpose": round(ms_a, 3),
-        "roofline": {"bound": "fp64 (matrix-free: trigonometric values recomputed)",
-                     "kernel": "k_bm_synth + k_bm_proj", "achieved": round(tf, 3),
+        "roofline": {"bound": "fp64 (matrix-free: space values from separable tables)",
+                     "kernel": "k_bm_synth + k_bm_proj + k_bm_proj_sum", "achieved": round(tf, 3),
                      "peak": MFMA_PEAK["f64"], "unit": "TFLOP/s",
                      "frac": round(tf / MFMA_PEAK["f64"], 4), "traffic": None,
                      "algorithmic_flops": 2 * flops,
                      "note": "2 flops per space-matrix entry, time and direction of use; the "
-                             "sin/cos evaluations (one per entry per pass of 4 or 2 times) are "
-                             "not counted"},
+                             "angle-addition and taper products that rebuild each entry "
+                             "from the separable tables are not counted"},
         "parity_vs_oracle": {"times": len(sample), "operg_rel_err": err_f,
                              "operg_transpose_rel_err": err_a},
         "setup_s": round(setup_s, 2),

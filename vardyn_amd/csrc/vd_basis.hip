@@ -19,11 +19,11 @@
 //               per 16 x 16 tile of grid points: the coefficients of the centres
 //               covering the tile are staged in LDS for 8 times at a time, so
 //               that every point reads them as LDS broadcasts
-//   k_bm_proj   Dp[k][q][t] = sum over the points of chunk k of a centre's box of
-//               G(p, c, q) adssh[t][p] (boxes cut into chunks of kBmChunk points,
-//               so the large scales' boxes, up to the whole grid, spread over
-//               many workgroups)
-//   k_bm_proj_sum  D[c][q][t] = sum of c's chunks in order
+//   k_bm_proj   Dt[e][q][t] = sum over the points of a 16 x 16 tile in the box of
+//               centre c of G(p, c, q) adssh[t][p], for every (tile, centre)
+//               entry e of the tile lists (the tile's adjoint fields in LDS,
+//               8 times per pass)
+//   k_bm_proj_sum  D[c][q][t] = sum of c's entries in ascending tile order
 //   k_bm_time_adj  adX[f, it, P, q] += sum_t g_f(t, it) D[c][q][t]
 // The phase is separable on the rectilinear grid: dx depends on the column
 // only and dy on the row only, so cos|sin(kx dx + ky dy) are products of
@@ -44,7 +44,6 @@ constexpr int kBmMaxTheta = 16;  // directions per frequency (9 with the referen
 constexpr int kBmTB = 8;         // times per pass of the synthesis gather
 constexpr int kBmTile = 16;      // grid-point tile side of the synthesis
 constexpr int kBmCC = 24;        // centres staged in LDS at a time
-constexpr int kBmChunk = 1024;   // box points per projection workgroup
 
 struct BmTables {
   int ny, nx, nf, ntheta, nc;
@@ -78,10 +77,9 @@ struct BmTables {
   int ntx, nty;            // synthesis tiles (kBmTile x kBmTile points)
   const int* tptr;         // ntx * nty + 1: centres whose box meets tile t (ascending)
   const int* tlist;
-  int nk;                  // projection chunks
-  const int* kc;           // nk: centre of chunk k
-  const int* kstart;       // nk: first box point of chunk k (row-major in the box)
-  const int* kptr;         // nc + 1: chunks of centre c
+  int ntl;                 // length of the tile lists
+  const int* cptr;         // nc + 1: entries (positions in tlist) of centre c,
+  const int* clist;        //   in ascending tile order
 };
 
 __device__ __forceinline__ double bm_window(double x) {
@@ -217,87 +215,86 @@ __global__ __launch_bounds__(256) void k_bm_synth(BmTables B, int T, int t0,
   }
 }
 
-// Dp[k][q][t] for times t0 .. t0 + PB - 1: one workgroup of 4 waves per
-// chunk of a centre's box, fixed reduction order (per-thread sums over a
-// strided set of the chunk's points, a shuffle tree per wave, then the four
-// waves in a fixed order).
-// MAXTH bounds the directions (register arrays are sized by it) and PB is the
-// number of times per pass: <10, 4> for up to 10 directions (the reference's
-// defaults give 9), <16, 2> otherwise.
-template <int MAXTH, int PB>
+// Dt[e][q][t] for times t0 .. t0 + 7 of the tile's entries e: a workgroup per
+// 16 x 16 tile of points holds their adjoint fields in LDS (zero on masked
+// points and outside the grid); each thread takes (entry, direction) pairs and
+// sweeps the tile's points inside the centre's box in row-major order, one
+// table lookup per point for the cosine and sine columns of all 8 times.
 __global__ __launch_bounds__(256) void k_bm_proj(BmTables B, int T, int t0,
                                                  const double* __restrict__ adssh,
-                                                 double* __restrict__ D) {
-  constexpr int kBmMaxTheta = MAXTH, kBmPB = PB;
-  __shared__ double red[4][2 * kBmMaxTheta * kBmPB];
-  const int k = blockIdx.x, c = B.kc[k];
-  const int4 bx = B.cbox[c];
-  const int nt = min(kBmPB, T - t0);
-  const int np = B.ny * B.nx;
-  const int q2 = 2 * B.ntheta;
-  double part[2 * kBmMaxTheta][kBmPB];
+                                                 double* __restrict__ Dt) {
+  __shared__ double As[kBmTB][kBmTile * kBmTile];
+  const int tile = blockIdx.x;
+  const int ti = tile / B.ntx, tj = tile % B.ntx;
+  const int i0 = ti * kBmTile, j0 = tj * kBmTile;
+  const int np = B.ny * B.nx, nt = min(kBmTB, T - t0), q2 = 2 * B.ntheta;
+  {
+    const int i = i0 + threadIdx.x / kBmTile, j = j0 + threadIdx.x % kBmTile;
+    const bool in = i < B.ny && j < B.nx && !(B.mask && B.mask[(size_t)i * B.nx + j]);
 #pragma unroll
-  for (int q = 0; q < 2 * kBmMaxTheta; ++q)
+    for (int t = 0; t < kBmTB; ++t)
+      As[t][threadIdx.x] = (in && t < nt) ? adssh[(int64_t)(t0 + t) * np + i * B.nx + j] : 0.0;
+  }
+  __syncthreads();
+  const int e0 = B.tptr[tile], ne = B.tptr[tile + 1] - e0;
+  for (int w = threadIdx.x; w < ne * B.ntheta; w += 256) {
+    const int e = e0 + w / B.ntheta, th = w % B.ntheta;
+    const int c = B.tlist[e];
+    const int4 bx = B.cbox[c];
+    const int ia = max(bx.x, i0), ib = min(min(bx.y, i0 + kBmTile - 1), B.ny - 1);
+    const int ja = max(bx.z, j0), jb = min(min(bx.w, j0 + kBmTile - 1), B.nx - 1);
+    double ac[kBmTB] = {}, as[kBmTB] = {};
+    for (int i = ia; i <= ib; ++i) {
+      const int y = B.yoff[c] + (i - bx.x);
+      const double wy = B.wy[y];
+      const double2 b = B.cy[(size_t)th * B.nrow + y];
+      for (int j = ja; j <= jb; ++j) {
+        const int x = B.xoff[c] + (j - bx.z);
+        const double2 a = B.cx[(size_t)th * B.ncol + x];
+        const double s2f = M_SQRT2 * (B.wx[x] * wy);
+        const double co = a.x * b.x - a.y * b.y, sn = a.y * b.x + a.x * b.y;
+        const double vc = s2f * co, vs = s2f * sn;
+        const int lp = (i - i0) * kBmTile + (j - j0);
 #pragma unroll
-    for (int t = 0; t < kBmPB; ++t) part[q][t] = 0;
-  const int bw = bx.w - bx.z + 1, bh = bx.y - bx.x + 1;
-  const int npts = (bw > 0 && bh > 0) ? bw * bh : 0;
-  const int e1 = min(npts, B.kstart[k] + kBmChunk);
-  for (int e = B.kstart[k] + threadIdx.x; e < e1; e += blockDim.x) {
-    const int i = bx.x + e / bw, j = bx.z + e % bw;
-    const int p = i * B.nx + j;
-    if (B.mask && B.mask[p]) continue;
-    double a[kBmPB];
-#pragma unroll
-    for (int t = 0; t < kBmPB; ++t) a[t] = t < nt ? adssh[(int64_t)(t0 + t) * np + p] : 0.0;
-    const int x = B.xoff[c] + (j - bx.z), y = B.yoff[c] + (i - bx.x);
-    const double s2f = bm_s2f(B, x, y);
-#pragma unroll
-    for (int th = 0; th < kBmMaxTheta; ++th) {
-      if (th < B.ntheta) {
-        double s, co;
-        bm_trig(B, th, x, y, &s, &co);
-        const double vc = s2f * co, vs = s2f * s;
-#pragma unroll
-        for (int t = 0; t < kBmPB; ++t) {
-          part[2 * th][t] += vc * a[t];
-          part[2 * th + 1][t] += vs * a[t];
+        for (int t = 0; t < kBmTB; ++t) {
+          ac[t] += vc * As[t][lp];
+          as[t] += vs * As[t][lp];
         }
       }
     }
-  }
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    double* d = Dt + ((int64_t)e * q2 + 2 * th) * kBmTB;
 #pragma unroll
-  for (int q = 0; q < 2 * kBmMaxTheta; ++q) {
-    if (q < q2) {
-#pragma unroll
-      for (int t = 0; t < kBmPB; ++t) {
-        double v = part[q][t];
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-        if (lane == 0) red[wv][q * kBmPB + t] = v;
-      }
+    for (int t = 0; t < kBmTB; ++t) {
+      d[t] = ac[t];
+      d[kBmTB + t] = as[t];
     }
-  }
-  __syncthreads();
-  for (int e = threadIdx.x; e < q2 * kBmPB; e += blockDim.x) {
-    const int q = e / kBmPB, t = e % kBmPB;
-    if (t < nt)
-      D[((int64_t)k * q2 + q) * T + t0 + t] = (red[0][e] + red[1][e]) + (red[2][e] + red[3][e]);
   }
 }
 
-// D[c][q][t] = sum over the chunks of centre c, in order, of Dp[k][q][t]
-__global__ void k_bm_proj_sum(BmTables B, int T, const double* __restrict__ Dp,
+// D[c][q][t0 + t] = sum over the entries of centre c (ascending tile order) of
+// Dt[e][q][t]: eight interleaved partial sums (entries k = u mod 8) combined
+// in a fixed tree, so that eight loads are in flight and the result is the
+// same on every run
+__global__ void k_bm_proj_sum(BmTables B, int T, int t0, const double* __restrict__ Dt,
                               double* __restrict__ D) {
-  const int q2 = 2 * B.ntheta;
-  const int64_t n = (int64_t)B.nc * q2 * T;
+  const int q2 = 2 * B.ntheta, nt = min(kBmTB, T - t0);
+  const int64_t n = (int64_t)B.nc * q2 * kBmTB;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
        e += (int64_t)gridDim.x * blockDim.x) {
-    const int c = (int)(e / ((int64_t)q2 * T));
-    const int64_t r = e - (int64_t)c * q2 * T;  // q * T + t
-    double acc = 0;
-    for (int k = B.kptr[c]; k < B.kptr[c + 1]; ++k) acc += Dp[(int64_t)k * q2 * T + r];
-    D[e] = acc;
+    const int c = (int)(e / ((int64_t)q2 * kBmTB));
+    const int r = (int)(e - (int64_t)c * q2 * kBmTB), q = r / kBmTB, t = r % kBmTB;
+    if (t >= nt) continue;
+    double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int k = B.cptr[c];
+    const int kend = B.cptr[c + 1];
+    for (; k + 8 <= kend; k += 8) {
+#pragma unroll
+      for (int u = 0; u < 8; ++u) a[u] += Dt[(int64_t)B.clist[k + u] * q2 * kBmTB + r];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (k + u < kend) a[u] += Dt[(int64_t)B.clist[k + u] * q2 * kBmTB + r];
+    D[((int64_t)c * q2 + q) * T + t0 + t] = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
   }
 }
 
@@ -335,7 +332,7 @@ using namespace vd;
 
 struct vd_bm_basis {
   BmTables B;
-  std::vector<int> kc, kstart, kptr;
+  std::vector<int> cptr, clist;
   std::vector<void*> owned;
   long long nbasis;
   int gt_total;  // sum of nt_f
@@ -414,18 +411,8 @@ int vd_bm_create(const vd_bm_params* prm, vd_bm_basis** out) {
   }
   b->gt_total = acc;
   std::vector<int4> box(P.nc);
-  b->kptr.assign(P.nc + 1, 0);
-  for (int c = 0; c < P.nc; ++c) {
+  for (int c = 0; c < P.nc; ++c)
     box[c] = make_int4(P.cbox[4 * c], P.cbox[4 * c + 1], P.cbox[4 * c + 2], P.cbox[4 * c + 3]);
-    const long long npts = box[c].x > box[c].y ? 0
-                           : (long long)(box[c].y - box[c].x + 1) * (box[c].w - box[c].z + 1);
-    for (long long e = 0; e < npts; e += kBmChunk) {
-      b->kc.push_back(c);
-      b->kstart.push_back((int)e);
-    }
-    b->kptr[c + 1] = (int)b->kc.size();
-  }
-  B.nk = (int)b->kc.size();
   std::vector<int> xoff(P.nc), yoff(P.nc);
   int ncol = 0, nrow = 0;
   for (int c = 0; c < P.nc; ++c) {
@@ -453,6 +440,17 @@ int vd_bm_create(const vd_bm_params* prm, vd_bm_basis** out) {
     tlist.insert(tlist.end(), per[t].begin(), per[t].end());
     tptr[t + 1] = (int)tlist.size();
   }
+  B.ntl = (int)tlist.size();
+  // per centre, its entries of the tile lists in ascending tile order
+  {
+    std::vector<int> cnt(P.nc + 1, 0);
+    for (int c : tlist) ++cnt[c + 1];
+    b->cptr.assign(P.nc + 1, 0);
+    for (int c = 0; c < P.nc; ++c) b->cptr[c + 1] = b->cptr[c] + cnt[c + 1];
+    b->clist.assign(tlist.size(), 0);
+    std::vector<int> fill(b->cptr.begin(), b->cptr.end() - 1);
+    for (int e = 0; e < (int)tlist.size(); ++e) b->clist[fill[tlist[e]]++] = e;
+  }
   int rc = VD_OK;
   if ((rc = bm_upload(b, P.lon1, P.nx, &B.lon1)) || (rc = bm_upload(b, P.lat1, P.ny, &B.lat1)) ||
       (rc = bm_upload(b, P.mask, P.mask ? np : 0, &B.mask)) ||
@@ -468,9 +466,8 @@ int vd_bm_create(const vd_bm_params* prm, vd_bm_basis** out) {
       (rc = bm_upload(b, (const int*)gtoff.data(), P.nf, &B.gtoff)) ||
       (rc = bm_upload(b, P.pptr, np + 1, &B.pptr)) ||
       (rc = bm_upload(b, P.plist, (size_t)P.pptr[np], &B.plist)) ||
-      (rc = bm_upload(b, (const int*)b->kc.data(), b->kc.size(), &B.kc)) ||
-      (rc = bm_upload(b, (const int*)b->kstart.data(), b->kstart.size(), &B.kstart)) ||
-      (rc = bm_upload(b, (const int*)b->kptr.data(), b->kptr.size(), &B.kptr)) ||
+      (rc = bm_upload(b, (const int*)b->cptr.data(), b->cptr.size(), &B.cptr)) ||
+      (rc = bm_upload(b, (const int*)b->clist.data(), b->clist.size(), &B.clist)) ||
       (rc = bm_upload(b, (const int*)xoff.data(), xoff.size(), &B.xoff)) ||
       (rc = bm_upload(b, (const int*)tptr.data(), tptr.size(), &B.tptr)) ||
       (rc = bm_upload(b, (const int*)tlist.data(), tlist.size(), &B.tlist)) ||
@@ -510,8 +507,9 @@ int vd_bm_info(const vd_bm_basis* b, long long* nbasis, int* gt_total) {
 
 int vd_bm_workspace_size(const vd_bm_basis* b, int T, size_t* bytes) {
   VD_REQUIRE(b && bytes && T > 0, VD_ERR_INVALID_ARG, "null argument or T < 1");
-  // C (or D) per centre, plus the projection's per-chunk partial sums
-  *bytes = (size_t)(b->B.nc + b->B.nk) * 2 * b->B.ntheta * T * sizeof(double);
+  // C (or D) per centre and time, plus the projection's per-(tile, centre)
+  // partial sums of one pass of kBmTB times
+  *bytes = ((size_t)b->B.nc * T + (size_t)b->B.ntl * kBmTB) * 2 * b->B.ntheta * sizeof(double);
   return VD_OK;
 }
 
@@ -538,19 +536,14 @@ int vd_bm_operg_adj(vd_bm_basis* b, int T, const double* gt, const double* adssh
   const BmTables& B = b->B;
   hipStream_t s = static_cast<hipStream_t>(stream);
   double* D = static_cast<double*>(work);
-  double* Dp = D + (size_t)B.nc * 2 * B.ntheta * T;
+  double* Dt = D + (size_t)B.nc * 2 * B.ntheta * T;
   if (B.nc == 0) return VD_OK;
-  if (B.nk > 0) {
-    if (B.ntheta <= 10)
-      for (int t0 = 0; t0 < T; t0 += 4)
-        hipLaunchKernelGGL((k_bm_proj<10, 4>), dim3(B.nk), dim3(256), 0, s, B, T, t0, adssh, Dp);
-    else
-      for (int t0 = 0; t0 < T; t0 += 2)
-        hipLaunchKernelGGL((k_bm_proj<16, 2>), dim3(B.nk), dim3(256), 0, s, B, T, t0, adssh, Dp);
+  const int64_t nd = (int64_t)B.nc * 2 * B.ntheta * kBmTB;
+  for (int t0 = 0; t0 < T; t0 += kBmTB) {
+    hipLaunchKernelGGL(k_bm_proj, dim3(B.ntx * B.nty), dim3(256), 0, s, B, T, t0, adssh, Dt);
+    hipLaunchKernelGGL(k_bm_proj_sum, dim3((unsigned)std::min<int64_t>((nd + 255) / 256, 65535)),
+                       dim3(256), 0, s, B, T, t0, Dt, D);
   }
-  const int64_t nd = (int64_t)B.nc * 2 * B.ntheta * T;
-  hipLaunchKernelGGL(k_bm_proj_sum, dim3((unsigned)std::min<int64_t>((nd + 255) / 256, 65535)),
-                     dim3(256), 0, s, B, T, Dp, D);
   const long long n = (long long)B.nc * 2 * B.ntheta;
   hipLaunchKernelGGL(k_bm_time_adj, dim3((unsigned)std::min<long long>((n + 255) / 256, 65535)),
                      dim3(256), 0, s, B, T, gt, D, adX);
