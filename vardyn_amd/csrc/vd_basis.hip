@@ -44,6 +44,8 @@ constexpr int kBmMaxTheta = 16;  // directions per frequency (9 with the referen
 constexpr int kBmTB = 8;         // times per pass of the synthesis gather
 constexpr int kBmTile = 16;      // grid-point tile side of the synthesis
 constexpr int kBmCC = 24;        // centres staged in LDS at a time
+constexpr int kBmParts = 4;      // lanes per (centre, direction) pair of the projection
+constexpr bool kBmPartCols = true;  // those lanes take interleaved columns (else row bands)
 
 struct BmTables {
   int ny, nx, nf, ntheta, nc;
@@ -217,12 +219,19 @@ __global__ __launch_bounds__(256) void k_bm_synth(BmTables B, int T, int t0,
 
 // Dt[e][q][t] for times t0 .. t0 + 7 of the tile's entries e: a workgroup per
 // 16 x 16 tile of points holds their adjoint fields in LDS (zero on masked
-// points and outside the grid); each thread takes (entry, direction) pairs and
-// sweeps the tile's points inside the centre's box in row-major order, one
-// table lookup per point for the cosine and sine columns of all 8 times.
+// points and outside the grid).  An (entry, direction) pair is taken by PARTS
+// consecutive lanes, each sweeping 16 / PARTS rows of the tile's points inside
+// the centre's box in row-major order (one table lookup per point for the
+// cosine and sine columns of all 8 times); the lanes' partial sums are combined
+// by butterfly shuffles, a fixed tree (the same bits on every run).  With COLS
+// the lanes of a pair take interleaved columns instead of row bands, so that
+// they read adjacent LDS words.
+template <int PARTS, bool COLS>
 __global__ __launch_bounds__(256) void k_bm_proj(BmTables B, int T, int t0,
                                                  const double* __restrict__ adssh,
                                                  double* __restrict__ Dt) {
+  static_assert(kBmTile % PARTS == 0 && 64 % PARTS == 0, "parts of a pair share a wave");
+  constexpr int RP = kBmTile / PARTS;
   __shared__ double As[kBmTB][kBmTile * kBmTile];
   const int tile = blockIdx.x;
   const int ti = tile / B.ntx, tj = tile % B.ntx;
@@ -237,36 +246,58 @@ __global__ __launch_bounds__(256) void k_bm_proj(BmTables B, int T, int t0,
   }
   __syncthreads();
   const int e0 = B.tptr[tile], ne = B.tptr[tile + 1] - e0;
-  for (int w = threadIdx.x; w < ne * B.ntheta; w += 256) {
-    const int e = e0 + w / B.ntheta, th = w % B.ntheta;
-    const int c = B.tlist[e];
-    const int4 bx = B.cbox[c];
-    const int ia = max(bx.x, i0), ib = min(min(bx.y, i0 + kBmTile - 1), B.ny - 1);
-    const int ja = max(bx.z, j0), jb = min(min(bx.w, j0 + kBmTile - 1), B.nx - 1);
+  const int total = ne * B.ntheta * PARTS;
+  // the loop bound is the same for the whole workgroup, so every lane reaches
+  // the shuffles
+  for (int base = 0; base < total; base += 256) {
+    const int w = base + threadIdx.x;
+    const int pair = w / PARTS, part = w % PARTS;
+    const bool valid = w < total;
+    const int e = e0 + pair / B.ntheta, th = pair % B.ntheta;
     double ac[kBmTB] = {}, as[kBmTB] = {};
-    for (int i = ia; i <= ib; ++i) {
-      const int y = B.yoff[c] + (i - bx.x);
-      const double wy = B.wy[y];
-      const double2 b = B.cy[(size_t)th * B.nrow + y];
-      for (int j = ja; j <= jb; ++j) {
-        const int x = B.xoff[c] + (j - bx.z);
-        const double2 a = B.cx[(size_t)th * B.ncol + x];
-        const double s2f = M_SQRT2 * (B.wx[x] * wy);
-        const double co = a.x * b.x - a.y * b.y, sn = a.y * b.x + a.x * b.y;
-        const double vc = s2f * co, vs = s2f * sn;
-        const int lp = (i - i0) * kBmTile + (j - j0);
+    if (valid) {
+      const int c = B.tlist[e];
+      const int4 bx = B.cbox[c];
+      const int r0 = COLS ? i0 : i0 + part * RP;
+      const int ia = max(bx.x, r0), ib = min(min(bx.y, r0 + (COLS ? kBmTile : RP) - 1), B.ny - 1);
+      int ja = max(bx.z, j0);
+      const int jb = min(min(bx.w, j0 + kBmTile - 1), B.nx - 1);
+      if (COLS) ja += ((part - (ja - j0)) % PARTS + PARTS) % PARTS;
+      constexpr int JS = COLS ? PARTS : 1;
+      for (int i = ia; i <= ib; ++i) {
+        const int y = B.yoff[c] + (i - bx.x);
+        const double wy = B.wy[y];
+        const double2 b = B.cy[(size_t)th * B.nrow + y];
+        for (int j = ja; j <= jb; j += JS) {
+          const int x = B.xoff[c] + (j - bx.z);
+          const double2 a = B.cx[(size_t)th * B.ncol + x];
+          const double s2f = M_SQRT2 * (B.wx[x] * wy);
+          const double co = a.x * b.x - a.y * b.y, sn = a.y * b.x + a.x * b.y;
+          const double vc = s2f * co, vs = s2f * sn;
+          const int lp = (i - i0) * kBmTile + (j - j0);
 #pragma unroll
-        for (int t = 0; t < kBmTB; ++t) {
-          ac[t] += vc * As[t][lp];
-          as[t] += vs * As[t][lp];
+          for (int t = 0; t < kBmTB; ++t) {
+            ac[t] += vc * As[t][lp];
+            as[t] += vs * As[t][lp];
+          }
         }
       }
     }
-    double* d = Dt + ((int64_t)e * q2 + 2 * th) * kBmTB;
 #pragma unroll
-    for (int t = 0; t < kBmTB; ++t) {
-      d[t] = ac[t];
-      d[kBmTB + t] = as[t];
+    for (int off = 1; off < PARTS; off <<= 1) {
+#pragma unroll
+      for (int t = 0; t < kBmTB; ++t) {
+        ac[t] += __shfl_xor(ac[t], off);
+        as[t] += __shfl_xor(as[t], off);
+      }
+    }
+    if (valid && part == 0) {
+      double* d = Dt + ((int64_t)e * q2 + 2 * th) * kBmTB;
+#pragma unroll
+      for (int t = 0; t < kBmTB; ++t) {
+        d[t] = ac[t];
+        d[kBmTB + t] = as[t];
+      }
     }
   }
 }
@@ -540,7 +571,7 @@ int vd_bm_operg_adj(vd_bm_basis* b, int T, const double* gt, const double* adssh
   if (B.nc == 0) return VD_OK;
   const int64_t nd = (int64_t)B.nc * 2 * B.ntheta * kBmTB;
   for (int t0 = 0; t0 < T; t0 += kBmTB) {
-    hipLaunchKernelGGL(k_bm_proj, dim3(B.ntx * B.nty), dim3(256), 0, s, B, T, t0, adssh, Dt);
+    hipLaunchKernelGGL((k_bm_proj<kBmParts, kBmPartCols>), dim3(B.ntx * B.nty), dim3(256), 0, s, B, T, t0, adssh, Dt);
     hipLaunchKernelGGL(k_bm_proj_sum, dim3((unsigned)std::min<int64_t>((nd + 255) / 256, 65535)),
                        dim3(256), 0, s, B, T, t0, Dt, D);
   }
