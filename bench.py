@@ -87,6 +87,9 @@ def parse():
     ap.add_argument("--tiled", action="store_true",
                     help="the tiled QG model (QgmWithTiles) on the 512x512 grid instead: step and "
                          "adjoint of all tiles in shared launches (grid-point-steps/s)")
+    ap.add_argument("--merge", action="store_true",
+                    help="time the merge of 9 overlapping subwindow analyses onto a 512x512 grid "
+                         "over 64 dates")
     ap.add_argument("--basis", action="store_true",
                     help="the balanced-motion wavelet basis (Basis_bm) on a 512x512 lon/lat grid "
                          "instead: operg + operg_transpose over the window's basis times")
@@ -689,6 +692,94 @@ def basis(args):
                                    f"({cpu_s:.1f} s; matrix set-up {orc_setup:.1f} s not timed)"}}
 
 
+def merge(args):
+    """The merge of subwindow analyses onto the target grid
+    (compute_weights_map / merge_output_date, run_assimilation.py:351-607):
+    a 512 x 512 target grid of 0.045 degrees, 3 x 3 overlapping subwindows of
+    200 x 200 points on grids of 0.05 degrees (offset from the target's, so every
+    value is a true bilinear interpolation), 64 output dates merged per call.
+    ``value`` is target grid points x dates per second; beside it the FP64
+    oracle (scipy, as the reference) on one host thread over 2 dates, and
+    parity against it."""
+    import numpy as np
+    import torch
+    from oracle.merge import merge_date, weights_map
+    from vardyn_amd.merge import SubwindowMerge
+    ny = nx = 512
+    d = 0.045
+    lon, lat = np.meshgrid(-10.0 + d * np.arange(nx), 30.0 + d * np.arange(ny))
+    subs = []
+    for a in range(3):
+        for c in range(3):
+            lo0, la0 = -10.3 + 6.7 * c, 29.8 + 6.7 * a  # 9.95 degrees wide, 3.25 overlapping
+            subs.append((lo0 + 0.05 * np.arange(200), la0 + 0.05 * np.arange(200)))
+    T = 64
+    rng = np.random.default_rng(11)
+    dev = torch.device("cuda")
+    t0 = time.time()
+    m = SubwindowMerge(lon, lat, subs)
+    torch.cuda.synchronize()
+    setup_s = time.time() - t0
+    fields = [torch.as_tensor(rng.standard_normal((T, 200, 200)), device=dev) for _ in subs]
+    packed, _ = m.pack(fields)
+    out = torch.empty((T, ny, nx), dtype=torch.float64, device=dev)
+    st = torch.cuda.current_stream()
+    m.merge_packed(packed, T, out)
+    torch.cuda.synchronize()
+    reps = 20
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(reps):
+        m.merge_packed(packed, T, out)
+    e1.record(st)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    launches = 1  # all dates in one launch, 8 per workgroup row
+    w = m.weights_space.cpu().numpy()
+    valid = int(np.isfinite(w).sum())
+    np_ = ny * nx
+    # per launch: each subwindow value read once and each output written once per
+    # date, and per 8 dates the tables (wsum; cell per pair; fractions and weight
+    # per valid pair)
+    per_launch = 8 * T * (m.sub_points + np_) + (T // 8) * (8 * np_ + 8 * m.nsub * np_ + 24 * valid)
+    gbs = per_launch / (ms * 1e-3 / launches) / 1e9
+    log(f"merge: {m.nsub} subwindows of 200x200 onto {ny}x{nx}, {T} dates: {ms:.3f} ms per call "
+        f"({launches} launches); set-up {setup_s:.2f}s")
+    wr, wsr = weights_map(lon, lat, subs)
+    sample = [0, T - 1]
+    fh = [f.cpu().numpy() for f in fields]
+    t1 = time.perf_counter()
+    refs = [merge_date(lon, lat, subs, [f[k] for f in fh], wr, wsr) for k in sample]
+    cpu_s = time.perf_counter() - t1
+    got = out.cpu().numpy()
+    err = 0.0
+    for k, ref in zip(sample, refs):
+        ok = np.isfinite(ref)
+        assert np.array_equal(ok, np.isfinite(got[k]))
+        err = max(err, float(np.max(np.abs(got[k][ok] - ref[ok])) / np.max(np.abs(ref[ok]))))
+    log(f"merge parity vs oracle ({len(sample)} dates): {err:.2e}; oracle {cpu_s:.2f} s")
+    return {
+        "metric": "subwindow merge target-grid-point-dates/sec, 9 subwindows onto 512x512, 64 dates",
+        "value": round(np_ * T / (ms * 1e-3), 1), "unit": "grid-point-dates/s", "n_gpus": 1,
+        "steps": reps, "warmup": 1, "ms_per_step": round(ms, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded subwindow fields)",
+        "config": {"workload": "merge512_s9_t64", "grid": [ny, nx], "subwindows": m.nsub,
+                   "subwindow_grid": [200, 200], "dates": T, "sub_points": m.sub_points},
+        "roofline": {"bound": "hbm", "kernel": "k_merge_apply", "achieved": round(gbs, 1),
+                     "peak": 8000.0, "unit": "GB/s", "frac": round(gbs / 8000.0, 4), "traffic": None,
+                     "bytes_per_launch": per_launch, "avg_launch_us": round(ms * 1e3 / launches, 2),
+                     "note": "algorithmic bytes: every subwindow value read once and every target "
+                             "value written once per date, plus the tables once per 8 dates"},
+        "parity_vs_oracle": {"dates": len(sample), "rel_err": err},
+        "setup_s": round(setup_s, 3),
+        "cpu_baseline": {"value": round(np_ * len(sample) / cpu_s, 1), "unit": "grid-point-dates/s",
+                         "cores": 1, "kind": "port",
+                         "sample": f"FP64 oracle (scipy RegularGridInterpolator, as the reference) "
+                                   f"merge of {len(sample)} dates ({cpu_s:.2f} s; weights untimed)"},
+    }
+
+
 def ptr_(t):
     from vardyn_amd.qg import ptr
     return ptr(t)
@@ -724,6 +815,9 @@ def main():
         return
     if args.basis:
         print(json.dumps(basis(args)), flush=True)
+        return
+    if args.merge:
+        print(json.dumps(merge(args)), flush=True)
         return
     if args.gemm_only:
         from vardyn_amd.qg import QgPlan

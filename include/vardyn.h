@@ -303,6 +303,49 @@ int vd_bm_operg_adj(vd_bm_basis* basis, int T, const double* gt, const double* a
                     double* adX, void* work, void* stream);
 
 /* ---------------------------------------------------------------------------
+ * Merge of subwindow analyses onto the target grid.  Replaces the numerical
+ * core of compute_weights_map (mapping/src/run_assimilation.py:351-412, the
+ * regular-grid branch of _build_interpolator :414-446), merge_output_date
+ * (:537-607, without the netCDF load/save and with kernel None) and the blend
+ * of merge_time_windows_outputs (:732-830).
+ *
+ * Subwindow s has ascending 1-D axes slat (sny[s] values) and slon (snx[s]),
+ * concatenated over s.  Per target point and subwindow: scipy
+ * RegularGridInterpolator semantics (linear, NaN outside), the raised-cosine
+ * weight (ones with a single subwindow), its sum over the subwindows.
+ * ------------------------------------------------------------------------- */
+typedef struct vd_merge vd_merge;
+typedef struct vd_merge_params {
+  int ny, nx;              /* target grid */
+  const double* lon;       /* ny * nx target longitudes (degrees) */
+  const double* lat;       /* ny * nx target latitudes */
+  const uint8_t* mask;     /* ny * nx, 1 = masked (NaN in the output), or null */
+  int nsub;                /* subwindows */
+  const int32_t* sny;      /* nsub */
+  const int32_t* snx;      /* nsub */
+  const double* slat;      /* sum of sny: latitude axes, strictly ascending each */
+  const double* slon;      /* sum of snx: longitude axes, strictly ascending each */
+} vd_merge_params;
+
+/* Upload the grids (host arrays) and build the interpolation tables and
+ * weights on the current device. */
+int vd_merge_create(const vd_merge_params* params, vd_merge** merge);
+int vd_merge_destroy(vd_merge* merge);
+/* Total subwindow points (sum of sny * snx). */
+int vd_merge_info(const vd_merge* merge, long long* sub_points);
+/* Copy the interpolated weights (nsub x ny x nx, NaN outside a subwindow)
+ * and their sum (ny x nx) into device buffers; either may be null. */
+int vd_merge_weights(const vd_merge* merge, double* weights, double* weights_sum, void* stream);
+/* out[t] (T fields of ny x nx) = merge of the subwindows' fields of date t.
+ * fields: subwindow s's T fields of sny[s] x snx[s] start at
+ * T * (points of the subwindows before s).  Device pointers. */
+int vd_merge_apply(vd_merge* merge, int T, const double* fields, double* out, void* stream);
+/* out = w1 * a + w2 * b over n values (the blend of two time windows,
+ * run_assimilation.py:797-818). */
+int vd_merge_blend(long long n, double w1, double w2, const double* a, const double* b, double* out,
+                   void* stream);
+
+/* ---------------------------------------------------------------------------
  * L3 observation operator (Obsop_interp_l3_jax, mapping/src/obsop.py:375-678).
  * One table per observation time:
  *   row_ptr[n_obs+1], col[nnz] (int32 flat grid index), w[nnz] (double)

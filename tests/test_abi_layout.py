@@ -34,3 +34,8 @@ def test_obs_table_layout_matches_header():
 def test_bm_params_layout_matches_header():
     from vardyn_amd.basis_bm import BmParams
     assert [f for f, _ in BmParams._fields_] == struct_fields("vd_bm_params")
+
+
+def test_merge_params_layout_matches_header():
+    from vardyn_amd.merge import MergeParams
+    assert [f for f, _ in MergeParams._fields_] == struct_fields("vd_merge_params")

@@ -90,6 +90,12 @@ SIGNATURES = {
     "vd_bm_workspace_size": (_I, [_VP, _I, C.POINTER(_SZ)]),
     "vd_bm_operg": (_I, [_VP, _I, _VP, _VP, _VP, _VP, _VP]),
     "vd_bm_operg_adj": (_I, [_VP, _I, _VP, _VP, _VP, _VP, _VP]),
+    "vd_merge_create": (_I, [_VP, _VP]),
+    "vd_merge_destroy": (_I, [_VP]),
+    "vd_merge_info": (_I, [_VP, C.POINTER(C.c_longlong)]),
+    "vd_merge_weights": (_I, [_VP, _VP, _VP, _VP]),
+    "vd_merge_apply": (_I, [_VP, _I, _VP, _VP, _VP]),
+    "vd_merge_blend": (_I, [C.c_longlong, C.c_double, C.c_double, _VP, _VP, _VP, _VP]),
     "vd_dot_f64": (_I, [_I64, _VP, _VP, _D, _VP, _VP, _VP]),
     "vd_axpby": (_I, [_I, _I64, _D, _VP, _D, _VP, _VP]),
     "vd_convert": (_I, [_I, _VP, _I, _VP, _I64, _I, _VP]),

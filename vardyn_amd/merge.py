@@ -1,0 +1,190 @@
+"""Merge of subwindow analyses onto the target grid, on the GPU.
+
+Reference: ``compute_weights_map``, ``_build_interpolator``,
+``merge_output_date`` and ``merge_time_windows_outputs``
+(mapping/src/run_assimilation.py:351-446, 537-607, 732-830),
+datlas-ocean/VarDyn.  The reference splits a large domain into overlapping
+subwindows and a long period into overlapping time windows, runs one
+assimilation per piece and merges the outputs date by date:
+
+* space (``compute_weights_map`` + ``merge_output_date``): every subwindow's
+  field is interpolated onto the target grid (scipy ``RegularGridInterpolator``,
+  linear, NaN outside the subwindow) and added with the subwindow's weight,
+  the raised-cosine window ``0.5 (1 - cos)`` over ``linspace(0, 2 pi, n)`` on
+  each axis (ones for a single subwindow) interpolated the same way, divided
+  by the sum of the weights; values that interpolate to NaN are skipped;
+  masked target points become NaN;
+* time (``merge_time_windows_outputs``): a date inside one window takes that
+  window's output; a date inside two or more takes ``W1 a + W2 b`` of the first
+  two, ``W2 = 0.5 (1 - cos(pi alpha))``, ``alpha`` the clamped position of the
+  date in their overlap (0.5 / 0.5 for an overlap of zero length).
+
+Here the outputs come in as device tensors instead of netCDF files, and a
+batch of dates is merged in one call.  Scope: regular (``GRID_GEO``)
+subwindow grids with ascending axes and no longitude wrap (the reference's
+Delaunay branches for irregular grids and split longitudes are not built);
+NaN gaps are not filled first (``kernel`` None; the reference's astropy fill
+is not available); H-grid variables (the C-grid averaging of U/V variables
+belongs to the SW model).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import VardynError
+from .qg import current_stream_ptr, ptr
+
+
+class MergeParams(C.Structure):
+    _fields_ = [("ny", C.c_int), ("nx", C.c_int), ("lon", C.c_void_p), ("lat", C.c_void_p),
+                ("mask", C.c_void_p), ("nsub", C.c_int), ("sny", C.c_void_p), ("snx", C.c_void_p),
+                ("slat", C.c_void_p), ("slon", C.c_void_p)]
+
+
+def _axes(lon, lat):
+    """1-D axes of a subwindow grid: ``lon_in[0, :]`` and ``lat_in[:, 0]`` of
+    2-D arrays (run_assimilation.py:432-433), or 1-D arrays as they are."""
+    lon, lat = np.asarray(lon, dtype=np.float64), np.asarray(lat, dtype=np.float64)
+    if lon.ndim == 2:
+        lon = lon[0, :]
+    if lat.ndim == 2:
+        lat = lat[:, 0]
+    return np.ascontiguousarray(lon), np.ascontiguousarray(lat)
+
+
+class SubwindowMerge:
+    """``compute_weights_map`` for a target grid (2-D ``lon``, ``lat``,
+    optional boolean ``mask``) and subwindow grids ``[(lon, lat), ...]``; the
+    tables and weights live on ``device``.  ``merge`` then plays
+    ``merge_output_date`` for a batch of dates."""
+
+    def __init__(self, lon, lat, subwindows, mask=None, device="cuda"):
+        lon = np.ascontiguousarray(np.asarray(lon, dtype=np.float64))
+        lat = np.ascontiguousarray(np.asarray(lat, dtype=np.float64))
+        if lon.ndim != 2 or lon.shape != lat.shape:
+            raise VardynError("the target grid needs 2-D lon and lat of one shape")
+        self.ny, self.nx = lon.shape
+        self.device = torch.device(device)
+        axes = [_axes(lo, la) for lo, la in subwindows]
+        if not axes:
+            raise VardynError("no subwindow")
+        self.shapes = [(la.size, lo.size) for lo, la in axes]
+        self.nsub = len(axes)
+        sny = np.array([s[0] for s in self.shapes], dtype=np.int32)
+        snx = np.array([s[1] for s in self.shapes], dtype=np.int32)
+        slat = np.ascontiguousarray(np.concatenate([la for _, la in axes]))
+        slon = np.ascontiguousarray(np.concatenate([lo for lo, _ in axes]))
+        m = None
+        if mask is not None and np.any(mask):
+            m = np.ascontiguousarray(np.asarray(mask, dtype=np.uint8))
+        p = MergeParams(self.ny, self.nx, lon.ctypes.data, lat.ctypes.data,
+                        m.ctypes.data if m is not None else None, self.nsub,
+                        sny.ctypes.data, snx.ctypes.data, slat.ctypes.data, slon.ctypes.data)
+        h = C.c_void_p()
+        with torch.cuda.device(self.device):
+            _lib.call("vd_merge_create", C.byref(p), C.byref(h))
+        self._h = h
+        tot = C.c_longlong()
+        _lib.call("vd_merge_info", h, C.byref(tot))
+        self.sub_points = int(tot.value)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            _lib.load().vd_merge_destroy(h)
+            self._h = None
+
+    @property
+    def weights_space(self):
+        """Interpolated weight of every subwindow on the target grid
+        (nsub x ny x nx, NaN outside the subwindow)."""
+        w = torch.empty((self.nsub, self.ny, self.nx), dtype=torch.float64, device=self.device)
+        _lib.call("vd_merge_weights", self._h, ptr(w), None, current_stream_ptr(self.device))
+        return w
+
+    @property
+    def weights_space_sum(self):
+        ws = torch.empty((self.ny, self.nx), dtype=torch.float64, device=self.device)
+        _lib.call("vd_merge_weights", self._h, None, ptr(ws), current_stream_ptr(self.device))
+        return ws
+
+    def pack(self, fields):
+        """One device buffer holding every subwindow's (T, ny_s, nx_s) fields."""
+        if len(fields) != self.nsub:
+            raise VardynError(f"expected {self.nsub} subwindow field stacks, got {len(fields)}")
+        T = None
+        parts = []
+        for f, (ny, nx) in zip(fields, self.shapes):
+            f = torch.as_tensor(f, dtype=torch.float64, device=self.device)
+            if f.dim() == 2:
+                f = f.unsqueeze(0)
+            if f.shape[1:] != (ny, nx):
+                raise VardynError(f"subwindow field shape {tuple(f.shape[1:])}, grid {(ny, nx)}")
+            if T is None:
+                T = f.shape[0]
+            elif f.shape[0] != T:
+                raise VardynError("every subwindow needs the same number of dates")
+            parts.append(f.reshape(-1))
+        return torch.cat(parts), T
+
+    def merge_packed(self, packed, T, out=None):
+        """Merge from a buffer made by ``pack``; ``out`` (T, ny, nx)."""
+        if packed.dtype != torch.float64 or not packed.is_contiguous() or packed.numel() != T * self.sub_points:
+            raise VardynError("packed fields must be a contiguous float64 buffer of T x subwindow points")
+        if out is None:
+            out = torch.empty((T, self.ny, self.nx), dtype=torch.float64, device=self.device)
+        _lib.call("vd_merge_apply", self._h, int(T), ptr(packed), ptr(out), current_stream_ptr(self.device))
+        return out
+
+    def merge(self, fields, out=None):
+        """``merge_output_date`` for T dates at once: ``fields`` is one
+        (T, ny_s, nx_s) (or (ny_s, nx_s)) array per subwindow; returns the
+        merged (T, ny, nx) fields."""
+        packed, T = self.pack(fields)
+        return self.merge_packed(packed, T, out)
+
+
+def compute_weights_map(lon, lat, subwindows, mask=None, device="cuda"):
+    """run_assimilation.py:351-412: the merger with its weight maps."""
+    return SubwindowMerge(lon, lat, subwindows, mask=mask, device=device)
+
+
+def time_window_weights(date, list_date_start, list_date_end):
+    """Which windows a date takes and their weights (run_assimilation.py:785-812):
+    ``[]`` outside every window, ``[(i, 1.0)]`` inside one, ``[(i, W1), (j, W2)]``
+    for the first two windows that hold it."""
+    active = [i for i in range(len(list_date_start)) if list_date_start[i] <= date <= list_date_end[i]]
+    if not active:
+        return []
+    if len(active) == 1:
+        return [(active[0], 1.0)]
+    i, j = active[0], active[1]
+    start, end = list_date_start[j], list_date_end[i]
+    dur = end - start
+    dur = dur.total_seconds() if hasattr(dur, "total_seconds") else float(dur)
+    if dur > 0:
+        a = date - start
+        a = a.total_seconds() if hasattr(a, "total_seconds") else float(a)
+        alpha = min(max(a / dur, 0.0), 1.0)
+        w2 = 0.5 * (1.0 - math.cos(math.pi * alpha))
+        w1 = 1.0 - w2
+    else:
+        w1, w2 = 0.5, 0.5
+    return [(i, w1), (j, w2)]
+
+
+def blend(a, b, w1, w2, out=None):
+    """``W1 a + W2 b`` of two windows' outputs on the GPU (run_assimilation.py:815-818)."""
+    if a.shape != b.shape or a.dtype != torch.float64 or b.dtype != torch.float64:
+        raise VardynError("blend needs two float64 tensors of one shape")
+    a, b = a.contiguous(), b.contiguous()
+    if out is None:
+        out = torch.empty_like(a)
+    _lib.call("vd_merge_blend", C.c_longlong(a.numel()), float(w1), float(w2), ptr(a), ptr(b), ptr(out),
+              current_stream_ptr(a.device))
+    return out
