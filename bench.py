@@ -743,6 +743,12 @@ def merge(args):
     # per valid pair)
     per_launch = 8 * T * (m.sub_points + np_) + (T // 8) * (8 * np_ + 8 * m.nsub * np_ + 24 * valid)
     gbs = per_launch / (ms * 1e-3 / launches) / 1e9
+    traffic, tsrc = None, None
+    rec = os.path.join(ROOT, "profiles", f"merge_traffic_merge{ny}_s{m.nsub}_t{T}.json")
+    if os.path.exists(rec):
+        with open(rec) as fh_:
+            traffic = json.load(fh_)["traffic_bytes_per_launch"]
+        tsrc = os.path.relpath(rec, ROOT)
     log(f"merge: {m.nsub} subwindows of 200x200 onto {ny}x{nx}, {T} dates: {ms:.3f} ms per call "
         f"({launches} launches); set-up {setup_s:.2f}s")
     wr, wsr = weights_map(lon, lat, subs)
@@ -767,7 +773,8 @@ def merge(args):
         "config": {"workload": "merge512_s9_t64", "grid": [ny, nx], "subwindows": m.nsub,
                    "subwindow_grid": [200, 200], "dates": T, "sub_points": m.sub_points},
         "roofline": {"bound": "hbm", "kernel": "k_merge_apply", "achieved": round(gbs, 1),
-                     "peak": 8000.0, "unit": "GB/s", "frac": round(gbs / 8000.0, 4), "traffic": None,
+                     "peak": 8000.0, "unit": "GB/s", "frac": round(gbs / 8000.0, 4), "traffic": traffic,
+                     "traffic_source": tsrc,
                      "bytes_per_launch": per_launch, "avg_launch_us": round(ms * 1e3 / launches, 2),
                      "note": "algorithmic bytes: every subwindow value read once and every target "
                              "value written once per date, plus the tables once per 8 dates"},
