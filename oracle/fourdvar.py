@@ -19,7 +19,12 @@ initial SSH and the gradient is that adjoint.
 
 Pinning: the schedule is checked against hand-worked integer cases; the cost
 against closed forms on quadratic toy problems (zero model steps) and the
-gradient against centred finite differences along seeded directions.
+gradient against centred finite differences along seeded directions.  The
+flux increment of ``ModelOracle``: from rest one segment gives exactly
+``nstep*dt/86400 * F`` for two segment lengths, the increment is additive
+after the segment, the flux adjoint is the pre-propagation state adjoint
+times the same scale, and (state, flux) -> state passes the dot-product test
+(``tests/test_oracle_pins.py``).
 """
 from __future__ import annotations
 

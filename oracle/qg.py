@@ -27,8 +27,13 @@ Pinning (tests/test_oracle_*.py):
   ``lambda_kl`` times the mode; ``pv2h o h2pv`` is the identity on such modes.
 * Masks: known-answer test of the ring pattern written out by hand.
 * ``h2uv``/``h2pv``/``adv``: polynomial exactness (bilinear / quadratic /
-  cubic fields give the analytic derivatives) and a brute-force scalar loop
-  on tiny grids.
+  cubic fields give the analytic derivatives) and point-by-point loops on
+  7x9 and 8x10 grids (``tests/test_oracle_pins.py``).
+* ``rhs`` optional branches (``tests/test_oracle_pins.py``): the beta plane
+  by ``f = f0 + b0 y + g2 y^2`` (centred beta exact, row of v checked), PV
+  diffusion by the 5-point Laplacian of a cubic, the bathymetry term by the
+  upwind derivative of a cubic ``f_mean * bathymetry`` for both wind signs,
+  and all of them superposed.
 * Time schemes: amplification polynomials on the linear test equation.
 * TLM: Taylor remainder O(lambda^2); adjoint: dot-product identity to 1e-12;
   both against finite differences of the forward.
