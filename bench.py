@@ -4,16 +4,20 @@ Workload: 1.5-layer QG in a closed 512x512 basin, dx = 5 km, Ld = 20 km
 (c = 2 m/s), dt = 900 s, 30-day window (2880 RK2 steps); 60 seeded nadir
 tracks every 6 km plus one 120 km swath per day at 2 km pixels (sigma =
 0.02 m); initial-state control with x_b = 0, sigma_b = 0.1 m; checkpoints
-every 54 steps plus every observation time.  One "step" of this benchmark is
-one full ``cost_and_grad`` evaluation (forward sweep with misfits, reverse
-sweep with the hand-written adjoint), replayed from a HIP graph, with the
-control vectors already resident in HBM.
+every 54 steps (about sqrt(2880)) plus every observation time.  One "step" of
+this benchmark is one full ``cost_and_grad`` evaluation (forward sweep with
+misfits, reverse sweep with the hand-written adjoint), replayed from a HIP
+graph, with the control vectors already resident in HBM.  Every evaluation
+is taken at X = 0.5 x the truth initial SSH of each member, a flowing state.
 
 ``value`` is the batched path: ``--batch`` independent problems of this
 workload (own initial fields, tracks and noise) advanced in the same kernel
-launches.  The plain single-problem path is measured in the same run and
-reported beside it with the ratio.  ``--batch 1`` makes the plain path the
-headline.
+launches, with the whole trajectory tape kept in HBM.  Measured beside it in
+the same run: the plain single-problem path, both paths with checkpointed
+trajectories (segment start states kept, each segment recomputed in the
+reverse sweep), the ratios, the split-product gradient against FP64-MFMA
+over the whole window, the 48-problem uneven set, the stencil rooflines and
+the CPU oracle.  ``--batch 1`` makes the plain path the headline.
 
 Prints one JSON line.  ``--gpus N`` with N > 1 exits: this build measures a
 single GPU.
@@ -57,7 +61,15 @@ def parse():
     ap.add_argument("--no-plain", action="store_true",
                     help="skip the single-problem measurement")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample-steps", type=int, default=24)
+    ap.add_argument("--cpu-sample-steps", type=int, default=96,
+                    help="steps of the window the CPU oracle is timed on (one day by default)")
+    ap.add_argument("--no-checkpoint", action="store_true",
+                    help="skip the checkpointed (segment start states + recompute) legs")
+    ap.add_argument("--no-problem-set", action="store_true",
+                    help="skip the 48-problem uneven set in the default run")
+    ap.add_argument("--oracle-full", action="store_true",
+                    help="one full-window FP64 oracle gradient (all host cores, minutes) against "
+                         "the GPU gradients of both sine-GEMM modes, at X = 0.5 x truth")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch kernels eagerly instead of replaying a captured HIP graph")
     ap.add_argument("--no-autotune", action="store_true",
@@ -117,10 +129,9 @@ def gemm_roofline(plan, precision, n_inv=100, reps=5):
     when there is one.
 
     Split-product plans (``dst_split`` digits) run the products on int8 MFMA:
-    ``achieved`` is then the int8 work the kernels issue (digit pairs x 2 M N
-    K over the padded product shapes) against the int8 peak, and
-    ``fp64_equivalent_tflops`` the algorithmic FP64 flops per second the
-    inversion delivers at FP64-class accuracy."""
+    ``achieved`` is then the same algorithmic flops per second against the
+    int8 peak, and ``executed_int8_tops`` the int8 work the kernels issue
+    (digit pairs x 2 M N K over the padded product shapes)."""
     import torch
     q = torch.randn(plan.shape, dtype=plan.dtype, device=plan.device) * 1e-6
     hb = torch.zeros_like(q)
@@ -154,9 +165,11 @@ def gemm_roofline(plan, precision, n_inv=100, reps=5):
         with open(rec) as f:
             traffic, src = json.load(f)["traffic_bytes_per_launch"], os.path.relpath(rec, ROOT)
     if split:
-        # int8 MFMA work of the split products: digit pairs x 2 M N K over the
+        # int8 MFMA work the split products issue: digit pairs x 2 M N K over the
         # zero-padded product shapes (half sizes when folded), averaged over the
-        # four stages of an inversion
+        # four stages of an inversion.  ``achieved`` / ``frac`` count only the
+        # algorithmic DST work (the same flops as the FP64 path) against the int8
+        # MFMA peak the kernels run on; the executed int8 work has its own keys.
         def rup(v):
             return (v + 31) // 32 * 32
         pairs = split * (split + 1) // 2
@@ -167,17 +180,20 @@ def gemm_roofline(plan, precision, n_inv=100, reps=5):
             MY, MX = rup(my), rup(mx)
             macs, nz = [MY * MX * MY, MY * MX * MX] * 2, plan.nb
         i8_ops = 2 * pairs * nz * sum(macs) / 4
-        achieved = i8_ops / per_launch_s / 1e12
-        fp64_eq = alg_flops_launch / per_launch_s / 1e12
+        executed = i8_ops / per_launch_s / 1e12
+        achieved = alg_flops_launch / per_launch_s / 1e12
         return {"bound": "mfma", "kernel": "k_gemm_split (sine-matrix GEMMs as int8 split products "
                                            "of FP64 operands, 4 launches per inversion)",
-                "achieved": round(achieved, 3), "peak": MFMA_PEAK["i8"], "unit": "TOP/s",
+                "achieved": round(achieved, 3), "peak": MFMA_PEAK["i8"], "unit": "TFLOP/s",
                 "frac": round(achieved / MFMA_PEAK["i8"], 4), "traffic": traffic,
                 "traffic_source": src, "avg_launch_us": round(per_launch_s * 1e6, 3),
-                "int8_ops_per_launch": i8_ops, "digits": split, "digit_pairs": pairs,
                 "algorithmic_flops_per_launch": alg_flops_launch, "folded_dst": plan.folded,
-                "fp64_equivalent_tflops": round(fp64_eq, 3),
-                "fp64_equivalent_over_fp64_mfma_peak": round(fp64_eq / MFMA_PEAK["f64"], 4)}
+                "note": "achieved = algorithmic DST flops per launch / launch time, against the "
+                        "dense int8 MFMA peak the products run on",
+                "executed_int8_ops_per_launch": i8_ops, "digits": split, "digit_pairs": pairs,
+                "executed_int8_tops": round(executed, 3),
+                "executed_int8_frac": round(executed / MFMA_PEAK["i8"], 4),
+                "achieved_over_fp64_mfma_peak": round(achieved / MFMA_PEAK["f64"], 4)}
     peak = MFMA_PEAK["f64" if precision == "float64" else "f32"]
     achieved = alg_flops_launch / per_launch_s / 1e12
     executed = exec_flops_launch / per_launch_s / 1e12
@@ -241,6 +257,56 @@ def stencil_roofline(case, precision, nbatch=64, n_launch=20, reps=5, rounds_n=5
             "rounds_us": [round(r * 1e6, 2) for r in rounds]}
 
 
+def adj_stencil_roofline(case, precision, nbatch=64, n_launch=20, reps=5, rounds_n=5):
+    """HBM roofline of the adjoint RK stage stencil (``k_adj_fused`` behind
+    ``vd_qg_rhs_adj``: lq = acc + coef drhs/dq^T lam, lh = coef drhs/dh^T lam)
+    on ``nbatch`` problems of the workload's grid at once (working set far above
+    the 256 MB Infinity Cache).  Algorithmic bytes per point: read h, q, lam,
+    acc and write lq, lh (6 words) plus the shared int8 mask once per launch.
+    Launch time: median of ``rounds_n`` rounds of ``reps`` graph replays."""
+    import torch
+    from vardyn_amd.qg import QgPlan
+    plan = QgPlan(case.ny, case.nx, case.dx, case.dx, case.dt, case.c, case.f0,
+                  time_scheme=case.time_scheme, precision=precision, nbatch=nbatch)
+    g = torch.Generator(device=plan.device).manual_seed(6)
+    h = 0.1 * torch.randn(plan.shape, dtype=plan.dtype, device=plan.device, generator=g)
+    q = plan.h2pv(h, torch.zeros_like(h))
+    lam = torch.randn(plan.shape, dtype=plan.dtype, device=plan.device, generator=g)
+    acc = torch.randn(plan.shape, dtype=plan.dtype, device=plan.device, generator=g)
+    lq, lh = torch.empty_like(q), torch.empty_like(q)
+    for _ in range(3):
+        plan.rhs_adj(h, q, lam, case.dt, acc, lq, lh)
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        for _ in range(n_launch):
+            plan.rhs_adj(h, q, lam, case.dt, acc, lq, lh)
+    graph.replay()
+    torch.cuda.synchronize()
+    s = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    rounds = []
+    for _ in range(rounds_n):
+        e0.record(s)
+        for _ in range(reps):
+            graph.replay()
+        e1.record(s)
+        torch.cuda.synchronize()
+        rounds.append(e0.elapsed_time(e1) * 1e-3 / (n_launch * reps))
+    per_launch_s = sorted(rounds)[len(rounds) // 2]
+    npts = case.ny * case.nx
+    nbytes = nbatch * npts * 6 * q.element_size() + npts
+    gbs = nbytes / per_launch_s / 1e9
+    del plan, h, q, lam, acc, lq, lh, graph
+    torch.cuda.empty_cache()
+    return {"bound": "hbm", "kernel": "k_adj_fused (adjoint RK stage: transposed upwind advection, "
+                                      "beta / diffusion terms and h2uv)",
+            "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "batch": nbatch,
+            "bytes_per_launch": nbytes, "avg_launch_us": round(per_launch_s * 1e6, 2),
+            "rounds_us": [round(r * 1e6, 2) for r in rounds]}
+
+
 def copy_calibration(nbytes=256 << 20):
     """A plain device copy of known size, for scaling the FETCH_SIZE / WRITE_SIZE
     counters of a profiled run (bytes moved: nbytes read + nbytes written)."""
@@ -259,22 +325,77 @@ def copy_calibration(nbytes=256 << 20):
     return {"bytes_read": nbytes, "bytes_written": nbytes, "achieved_GBs": round(gbs, 1)}
 
 
-def cpu_baseline(case, sample_steps, precision, dst_split=0):
-    """The FP64 oracle's ``cost_and_grad`` on one host thread over a bounded
-    sample of the same workload (the first ``sample_steps`` steps of the
-    window, with their observations), scaled to the full window.  The same
-    sample is evaluated by the GPU engine (FP64-MFMA sine GEMMs, and with
-    ``dst_split`` digits of split products when given), and the relative
-    error of its gradient against the oracle's is reported beside the timing."""
-    import torch
-    from threadpoolctl import threadpool_limits
+def control_point(truth):
+    """The control the evaluations are timed and checked at: half the truth
+    initial SSH of each member (a flowing state, away from the background
+    x_b = 0 where the forward stays at rest and the tangent is time-invariant)."""
+    return 0.5 * np.asarray(truth, dtype=np.float64).ravel()
+
+
+def oracle_cost(pb, case):
     from oracle.fourdvar import BasisInitialState, CostOracle, Cov, ModelOracle
     from oracle.obsop import ObsopOracle
     from oracle.qg import QgmOracle
+    qg = QgmOracle(dx=case.dx, dy=case.dx, dt=case.dt, c=case.c, f=case.f0,
+                   time_scheme=case.time_scheme, shape=(case.ny, case.nx))
+    H = ObsopOracle(pb.grid.lon, pb.grid.lat, pb.grid.DX, pb.grid.DY, pb.T, case.dt, pb.dict_obs,
+                    Npix=4)
+    return CostOracle(ModelOracle(qg), H, BasisInitialState((case.ny, case.nx), case.sigma_b),
+                      pb.checkpoints, pb.T, dtbasis=case.checkpoint_steps, B=Cov(case.sigma_b),
+                      Xb=np.zeros(case.ny * case.nx))
+
+
+def timed_oracle(cost, X, threads, progress=False):
+    """The oracle's cost_and_grad on ``threads`` host threads (BLAS and scipy.fft
+    workers), optionally logging each model segment so that long runs report."""
+    import scipy.fft
+    from threadpoolctl import threadpool_limits
+    M = cost.M
+    step0, adj0 = M.step, M.step_adj
+    t_start = time.perf_counter()
+    if progress:
+        count = {"f": 0, "a": 0}
+
+        def step(var, params, nstep):
+            count["f"] += 1
+            if count["f"] % 12 == 0:
+                log(f"oracle: forward segment {count['f']} ({time.perf_counter() - t_start:.0f} s)")
+            return step0(var, params, nstep)
+
+        def step_adj(advar, adparams, var, nstep):
+            count["a"] += 1
+            if count["a"] % 12 == 0:
+                log(f"oracle: adjoint segment {count['a']} ({time.perf_counter() - t_start:.0f} s)")
+            return adj0(advar, adparams, var, nstep)
+        M.step, M.step_adj = step, step_adj
+    try:
+        with threadpool_limits(limits=threads), scipy.fft.set_workers(threads):
+            t0 = time.perf_counter()
+            J, G = cost.cost_and_grad(X)
+            t = time.perf_counter() - t0
+    finally:
+        M.step, M.step_adj = step0, adj0
+    return J, G, t
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+def cpu_baseline(case, sample_steps, precision, dst_split=0):
+    """The FP64 oracle's ``cost_and_grad`` over a bounded sample of the same
+    workload (the first ``sample_steps`` steps of the window with their
+    observations, at the control of the timed evaluations), on one host thread
+    (the reference pins MKL/OMP to one thread, mapping/src/__init__.py:3-5) and
+    on all host cores, each scaled to the full window.  The same sample is
+    evaluated by the GPU engine (FP64-MFMA sine GEMMs, and with ``dst_split``
+    digits of split products when given), and the relative errors of its
+    gradient and cost against the oracle's are reported beside the timing."""
+    import torch
     from vardyn_amd.problem import build_problem
     pb = build_problem(case, precision=precision, n_steps=sample_steps, control="ic",
                        tape="full")
-    X = 0.5 * pb.truth0.ravel()
+    X = control_point(pb.truth0)
     J, G = pb.engine.cost_and_grad(X)
     Js = Gs = None
     if dst_split:
@@ -282,77 +403,93 @@ def cpu_baseline(case, sample_steps, precision, dst_split=0):
                             tape="full", truth_plan=pb.plan, dst_split=dst_split)
         Js, Gs = pbs.engine.cost_and_grad(X)
         del pbs
-    qg = QgmOracle(dx=case.dx, dy=case.dx, dt=case.dt, c=case.c, f=case.f0,
-                   time_scheme=case.time_scheme, shape=(case.ny, case.nx))
-    H = ObsopOracle(pb.grid.lon, pb.grid.lat, pb.grid.DX, pb.grid.DY, pb.T, case.dt, pb.dict_obs,
-                    Npix=4)
-    cost = CostOracle(ModelOracle(qg), H, BasisInitialState((case.ny, case.nx), case.sigma_b),
-                      pb.checkpoints, pb.T, dtbasis=case.checkpoint_steps, B=Cov(case.sigma_b),
-                      Xb=np.zeros(case.ny * case.nx))
-    with threadpool_limits(limits=1):
-        t0 = time.perf_counter()
-        Jr, Gr = cost.cost_and_grad(X)
-        t = time.perf_counter() - t0
-    per_grad = t * case.n_steps / sample_steps
+    cost = oracle_cost(pb, case)
+    ncpu = os.cpu_count() or 1
+    Jr, Gr, t1 = timed_oracle(cost, X, 1)
+    _, Ga, tn = timed_oracle(cost, X, ncpu)
+    scale = case.n_steps / sample_steps
     del pb
     gc.collect()
     torch.cuda.empty_cache()
     extra = {}
     if Gs is not None:
-        extra = {"gpu_split_grad_rel_err_vs_oracle": float(np.linalg.norm(Gs - Gr) / np.linalg.norm(Gr)),
+        extra = {"gpu_split_grad_rel_err_vs_oracle": rel(Gs, Gr),
                  "gpu_split_J_rel_err_vs_oracle": float(abs(Js - Jr) / abs(Jr)),
                  "split_digits": dst_split}
-    return {"value": round(1.0 / per_grad, 6), "unit": "gradients/s", "cores": 1, "kind": "port",
+    return {"value": round(1.0 / (tn * scale), 6), "unit": "gradients/s", "cores": ncpu,
+            "kind": "port",
             "sample": f"FP64 NumPy/SciPy oracle cost_and_grad over the first {sample_steps} RK2 "
-                      f"steps of the {case.ny}x{case.nx} window with their observations "
-                      f"({t:.1f} s), scaled by {case.n_steps}/{sample_steps}",
-            "host": platform.processor() or platform.machine(), "host_cpus": os.cpu_count(),
-            "gpu_grad_rel_err_vs_oracle": float(np.linalg.norm(G - Gr) / np.linalg.norm(Gr)),
+                      f"steps of the {case.ny}x{case.nx} window with their observations at "
+                      f"X = 0.5 x truth ({tn:.1f} s on {ncpu} threads, {t1:.1f} s on 1), "
+                      f"scaled by {case.n_steps}/{sample_steps}",
+            "one_thread": {"value": round(1.0 / (t1 * scale), 6), "cores": 1,
+                           "sample_s": round(t1, 2)},
+            "all_cores": {"value": round(1.0 / (tn * scale), 6), "cores": ncpu,
+                          "sample_s": round(tn, 2)},
+            "host": platform.processor() or platform.machine(), "host_cpus": ncpu,
+            "threads_agree": rel(Ga, Gr),
+            "gpu_grad_rel_err_vs_oracle": rel(G, Gr),
             "gpu_J_rel_err_vs_oracle": float(abs(J - Jr) / abs(Jr)), **extra}
 
 
-def measure(pb, args, label):
-    """Autotune, capture and time ``args.steps`` evaluations of ``pb.engine``."""
+def measure(pb, args, label, X, tiles=None, steps=None, warmup=None):
+    """Autotune (unless ``tiles`` are given), capture and time evaluations of
+    ``pb.engine`` at the control ``X`` (host array of B x nctrl values)."""
     import torch
     eng = pb.engine
     nb = pb.plan.nb
+    steps = args.steps if steps is None else steps
+    warmup = args.warmup if warmup is None else warmup
     t0 = time.time()
-    tiles = pb.plan.autotune(reps=5) if not args.no_autotune else None
+    if tiles is None:
+        tiles = pb.plan.autotune(reps=5) if not args.no_autotune else None
     log(f"{label}: GEMM tile configurations {tiles} ({time.time() - t0:.2f}s); "
         f"{pb.H.n_total} obs at {len(pb.H.t_obs)} times per member, "
         f"{len(pb.checkpoints) - 1} segments, tape {eng.tape_mode}")
-    X0 = torch.zeros(nb * eng.nctrl, dtype=torch.float64, device=pb.plan.device)
+    X0 = torch.as_tensor(np.asarray(X, dtype=np.float64), device=pb.plan.device).reshape(-1)
     if not args.no_graph:
         t0 = time.time()
         eng.capture()
         log(f"{label}: captured one gradient evaluation in a HIP graph ({time.time() - t0:.1f}s)")
-    for w in range(args.warmup):
+    for w in range(warmup):
         J, G = eng.evaluate(X0)
         torch.cuda.synchronize()
         log(f"{label}: warmup {w}: J[0]={J[0].item():.6e}")
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         J, G = eng.evaluate(X0)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t1
-    ms = dt * 1e3 / args.steps
-    r = {"value": nb * args.steps / dt, "ms_per_step": ms, "batch": nb, "tiles": tiles,
+    ms = dt * 1e3 / steps
+    r = {"value": nb * steps / dt, "ms_per_step": ms, "batch": nb, "tiles": tiles,
          "tape": eng.tape_mode, "J0": J[0].item(),
-         "gnorm0": float(torch.linalg.vector_norm(G[0]).item())}
-    log(f"{label}: timed {args.steps} evaluations of {nb} problem(s): {ms:.1f} ms each; "
+         "gnorm0": float(torch.linalg.vector_norm(G[0]).item()), "steps": steps}
+    log(f"{label}: timed {steps} evaluations of {nb} problem(s): {ms:.1f} ms each; "
         f"J[0]={r['J0']:.10e} |G[0]|={r['gnorm0']:.10e}")
     return r
 
 
-def split_check(case, pb_ref, precision, n_steps, digits):
+def retape(pb, tape):
+    """Replace ``pb.engine`` by an engine of the same problem(s) with another
+    trajectory storage ("full" or "checkpoint"); the old tape is freed first."""
+    from vardyn_amd.fourdvar import FourDVar
+    H = pb.Hs if getattr(pb, "Hs", None) is not None and pb.plan.nb > 1 else pb.H
+    pb.engine = None
+    release(None)
+    pb.engine = FourDVar(pb.plan, H, pb.checkpoints, pb.T, control="ic",
+                         sigma_b=pb.case.sigma_b, tape=tape, dtbasis=pb.case.checkpoint_steps)
+    return pb
+
+
+def split_check(case, pb_ref, precision, n_steps, digits, X):
     """Gradient and cost of the workload with split-product sine GEMMs
     (``digits`` per operand line) against the same problem -- same
-    observations -- with the other GEMM mode (``pb_ref``), over the whole
-    window, at the background state."""
+    observations -- with FP64-MFMA GEMMs (``pb_ref``), over the whole window, at
+    the control ``X`` of the timed evaluations."""
     import torch
     from vardyn_amd.problem import build_problem
-    X0 = torch.zeros(pb_ref.engine.nctrl, dtype=torch.float64, device=pb_ref.plan.device)
+    X0 = torch.as_tensor(X, device=pb_ref.plan.device)
     Jr, Gr = pb_ref.engine.run(X0)
     Jr, Gr = Jr.clone(), Gr.clone()
     pbs = build_problem(case, precision=precision, n_steps=n_steps, control="ic", tape="full",
@@ -362,7 +499,7 @@ def split_check(case, pb_ref, precision, n_steps, digits):
     r = {"grad_rel_err_vs_fp64_mfma": float((torch.linalg.vector_norm(Gs - Gr) /
                                              torch.linalg.vector_norm(Gr)).item()),
          "J_rel_err_vs_fp64_mfma": abs(Js[0].item() - Jr[0].item()) / abs(Jr[0].item()),
-         "window_steps": n_steps, "digits": digits}
+         "window_steps": n_steps, "digits": digits, "control": "0.5 x truth initial SSH"}
     # the metric's accuracy bar: gradient within 1e-5 (relative) of FP64
     r["within_1e-5"] = r["grad_rel_err_vs_fp64_mfma"] <= 1e-5
     if not r["within_1e-5"]:
@@ -373,44 +510,109 @@ def split_check(case, pb_ref, precision, n_steps, digits):
     return r
 
 
-def problem_set(args):
+def oracle_full(args):
+    """One full-window gradient of the 512 x 512 workload by the FP64 oracle on
+    all host cores, at X = 0.5 x truth, against the GPU engine's gradients with
+    FP64-MFMA sine GEMMs and with split products of 5 and 6 digits (same
+    problem, same observations).  Several minutes of host time: a run of its
+    own, not part of the default benchmark."""
+    import torch
+    from vardyn_amd import synthetic as syn
+    from vardyn_amd.problem import build_problem
+    case = syn.case_basin512()
+    n = args.window_steps or case.n_steps
+    pb = build_problem(case, precision="float64", n_steps=n, control="ic", tape="full")
+    X = control_point(pb.truth0)
+    t0 = time.perf_counter()
+    J0, G0 = pb.engine.cost_and_grad(X)
+    torch.cuda.synchronize()
+    gpu_s = time.perf_counter() - t0
+    gpu = {"fp64_mfma": (J0, G0)}
+    for d in (5, 6):
+        pbs = build_problem(case, precision="float64", n_steps=n, control="ic", tape="full",
+                            truth_plan=pb.plan, dst_split=d)
+        gpu[f"split{d}"] = pbs.engine.cost_and_grad(X)
+        del pbs
+        release(None)
+    log(f"oracle: GPU gradients done ({gpu_s:.1f} s for the FP64-MFMA one); oracle over {n} "
+        f"steps, {len(pb.checkpoints) - 1} segments")
+    cost = oracle_cost(pb, case)
+    ncpu = os.cpu_count() or 1
+    Jr, Gr, t = timed_oracle(cost, X, ncpu, progress=True)
+    out = {"workload": f"{case.name}: {case.ny}x{case.nx}, {n} RK2 steps, "
+                       f"{pb.H.n_total} observations", "control": "X = 0.5 x truth initial SSH",
+           "oracle": {"J": float(Jr), "grad_norm": float(np.linalg.norm(Gr)), "seconds": round(t, 1),
+                      "threads": ncpu, "host": platform.processor() or platform.machine(),
+                      "gradients_per_s": round(1.0 / t, 6)},
+           "gpu": {}}
+    for k, (J, G) in gpu.items():
+        out["gpu"][k] = {"J": float(J), "J_rel_err_vs_oracle": float(abs(J - Jr) / abs(Jr)),
+                         "grad_rel_err_vs_oracle": rel(G, Gr),
+                         "grad_max_abs_err_over_max": float(np.max(np.abs(G - Gr)) /
+                                                            np.max(np.abs(Gr))),
+                         "within_1e-5": rel(G, Gr) <= 1e-5}
+    log(f"oracle comparison: {json.dumps(out['gpu'])}")
+    return out
+
+
+def problem_set(args, steps=None, warmup=None, separate=True):
     """Independent problems of uneven grid sizes and windows (48 by default).
 
     Headline: the ragged engine -- one launch per RK stage / inversion / adjoint
     stage covers every problem still running, one HIP graph for the whole
-    evaluation.  Beside it, the same problems as separate single-problem engines,
-    one graph each, replayed concurrently on ``args.streams`` streams."""
+    evaluation -- at the control X = 0.5 x truth of every problem.  Beside it
+    (``separate``), the same problems as separate single-problem engines, one
+    graph each, replayed concurrently on ``args.streams`` streams."""
     import torch
     from vardyn_amd import synthetic as syn
     from vardyn_amd.problem import build_problem
     from vardyn_amd.problemset import ProblemSet
     from vardyn_amd.ragged import RaggedFourDVar
+    steps = args.steps if steps is None else steps
+    warmup = args.warmup if warmup is None else warmup
     cases = syn.config4_cases()[:args.problem_set_size]
     t0 = time.time()
     pbs = [build_problem(c, precision=args.precision, control="ic", with_engine=False,
                          dst_split=args.problem_set_split) for c in cases]
     eng = RaggedFourDVar(pbs).capture()
-    log(f"problem set: {len(cases)} problems built and captured in {time.time() - t0:.1f}s, "
+    build_s = time.time() - t0
+    log(f"problem set: {len(cases)} problems built and captured in {build_s:.1f}s, "
         f"tape {eng.tape_bytes / 2**30:.1f} GiB")
-    X = torch.zeros_like(eng.X)
-    for _ in range(args.warmup):
+    X = torch.as_tensor(np.concatenate([control_point(pb.truth0) for pb in pbs]),
+                        device=eng.dev)
+    for _ in range(warmup):
         eng.evaluate(X)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         eng.evaluate(X)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t1
-    ms = dt * 1e3 / args.steps
+    ms = dt * 1e3 / steps
     gps = sum(c.ny * c.nx * c.n_steps for c in cases)
-    value = len(cases) * args.steps / dt
+    value = len(cases) * steps / dt
     log(f"ragged engine: {ms:.1f} ms per evaluation of {len(cases)} problems")
     del eng, pbs
     release(None)
+    out = {
+        "metric": f"fwd+adjoint 4D-Var gradients/sec, {len(cases)} uneven regional QG problems",
+        "value": round(value, 4), "unit": "gradients/s", "n_gpus": 1, "steps": steps,
+        "warmup": warmup, "ms_per_step": round(ms, 2), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if args.precision == "float64" else "f32",
+        "data": "synthetic (seeded fields and nadir tracks per problem)",
+        "config": {"workload": f"{len(cases)} problems, sides 96-384, 5-20 day windows, "
+                               f"Ld 15-40 km, dt 900 s", "engine": "ragged (shared launches)",
+                   "dst_split": args.problem_set_split, "control": "0.5 x truth initial SSH",
+                   "grid_point_steps_per_evaluation": gps},
+        "grid_point_steps_per_s": round(gps * steps / dt, 1),
+        "per_problem_latency_ms": round(ms, 2), "build_s": round(build_s, 1)}
+    if not separate:
+        return out
     # the same problems as separate engines on several streams
     ps = ProblemSet(cases, precision=args.precision, tape="full", n_streams=args.streams,
                     autotune=False).capture()
-    Xs = ps.zeros()
+    Xs = [torch.as_tensor(control_point(pb.truth0), device=ps.device) for pb in ps.problems]
     ps.evaluate(Xs)
     t2 = time.perf_counter()
     ps.evaluate(Xs)
@@ -418,22 +620,10 @@ def problem_set(args):
     log(f"separate engines on {args.streams} streams: {sep_ms:.1f} ms per evaluation")
     del ps
     release(None)
-    return {
-        "metric": f"fwd+adjoint 4D-Var gradients/sec, {len(cases)} uneven regional QG problems",
-        "value": round(value, 4), "unit": "gradients/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(ms, 2), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if args.precision == "float64" else "f32",
-        "data": "synthetic (seeded fields and nadir tracks per problem)",
-        "config": {"workload": f"{len(cases)} problems, sides 96-384, 5-20 day windows, "
-                               f"Ld 15-40 km, dt 900 s", "engine": "ragged (shared launches)",
-                   "dst_split": args.problem_set_split,
-                   "grid_point_steps_per_evaluation": gps},
-        "grid_point_steps_per_s": round(gps * args.steps / dt, 1),
-        "per_problem_latency_ms": round(ms, 2),
-        "separate_engines": {"ms_per_step": round(sep_ms, 2), "streams": args.streams,
-                             "value": round(len(cases) * 1e3 / sep_ms, 4)},
-        "ragged_over_separate": round(sep_ms / ms, 3)}
+    out["separate_engines"] = {"ms_per_step": round(sep_ms, 2), "streams": args.streams,
+                               "value": round(len(cases) * 1e3 / sep_ms, 4)}
+    out["ragged_over_separate"] = round(sep_ms / ms, 3)
+    return out
 
 
 def tiled(args):
@@ -820,6 +1010,9 @@ def main():
     if args.tiled:
         print(json.dumps(tiled(args)), flush=True)
         return
+    if args.oracle_full:
+        print(json.dumps(oracle_full(args)), flush=True)
+        return
     if args.basis:
         print(json.dumps(basis(args)), flush=True)
         return
@@ -842,18 +1035,25 @@ def main():
         return
     n_steps = args.window_steps or case.n_steps
     nb = max(1, args.batch)
+    extra_steps = max(2, min(args.steps, 10))  # timed evaluations of the side measurements
+    legs = {}
 
-    plain = None
+    plain = split = None
     if nb == 1 or not args.no_plain:
         log(f"plain path: one {case.ny}x{case.nx} problem, {n_steps} steps, {args.precision}, "
             f"dst_split {args.plain_dst_split}")
         pb1 = build_problem(case, precision=args.precision, n_steps=n_steps, control="ic",
                             tape="full" if args.tape == "auto" else args.tape,
                             dst_split=args.plain_dst_split)
-        plain = measure(pb1, args, "plain")
+        X1 = control_point(pb1.truth0)
+        plain = measure(pb1, args, "plain", X1)
         if args.dst_split and not args.plain_dst_split and nb > 1:
-            plain["split_check"] = split_check(case, pb1, args.precision, n_steps, args.dst_split)
-            log(f"split check: {plain['split_check']}")
+            split = split_check(case, pb1, args.precision, n_steps, args.dst_split, X1)
+            log(f"split check: {split}")
+        if nb > 1 and not args.no_checkpoint and plain["tape"] == "full":
+            retape(pb1, "checkpoint")
+            legs["plain_checkpoint"] = measure(pb1, args, "plain, checkpointed", X1,
+                                               tiles=plain["tiles"], steps=extra_steps, warmup=1)
         if nb == 1:
             pb, head = pb1, plain
         else:
@@ -864,7 +1064,12 @@ def main():
         pb = build_batch(case, nb, precision=args.precision, n_steps=n_steps, tape=args.tape,
                          dst_split=args.dst_split)
         log(f"batched path: {nb} problems built in {time.time() - t0:.1f}s")
-        head = measure(pb, args, "batched")
+        Xb = control_point(pb.truth)
+        head = measure(pb, args, "batched", Xb)
+        if not args.no_checkpoint and head["tape"] == "full":
+            retape(pb, "checkpoint")
+            legs["batched_checkpoint"] = measure(pb, args, "batched, checkpointed", Xb,
+                                                 tiles=head["tiles"], steps=extra_steps, warmup=1)
 
     roof = gemm_roofline(pb.plan, args.precision)
     log(f"roofline: {roof}")
@@ -879,7 +1084,9 @@ def main():
                    "grid": [case.ny, case.nx], "dx_m": case.dx, "dt_s": case.dt,
                    "window_steps": n_steps, "time_scheme": case.time_scheme,
                    "obs_per_member": pb.H.n_total, "segments": len(pb.checkpoints) - 1,
-                   "control": "initial SSH", "tape": head["tape"], "batch": nb,
+                   "checkpoint_steps": case.checkpoint_steps,
+                   "control": "initial SSH", "evaluated_at": "X = 0.5 x truth initial SSH",
+                   "tape": head["tape"], "batch": nb,
                    "hip_graph": not args.no_graph, "folded_dst": pb.plan.folded,
                    "dst_split": pb.plan.dst_split, "gemm_tiles": head["tiles"]},
         "grid_point_steps_per_s": round(case.ny * case.nx * n_steps * value, 1),
@@ -887,16 +1094,30 @@ def main():
         "J": head["J0"], "grad_norm": head["gnorm0"],
         "roofline": roof,
     }
-    if plain is not None and "split_check" in plain:
-        out["split_check"] = plain["split_check"]
+    if split is not None:
+        out["split_check"] = split
     if plain is not None and nb > 1:
         out["plain_path"] = {"value": round(plain["value"], 4), "unit": "gradients/s",
                              "ms_per_gradient": round(plain["ms_per_step"], 2),
                              "tape": plain["tape"], "gemm_tiles": plain["tiles"],
                              "dst_split": args.plain_dst_split}
         out["batched_over_plain"] = round(value / plain["value"], 3)
+    for k, r in legs.items():
+        out[k] = {"value": round(r["value"], 4), "unit": "gradients/s",
+                  "ms_per_gradient": round(r["ms_per_step"] / r["batch"], 2),
+                  "ms_per_step": round(r["ms_per_step"], 2), "batch": r["batch"],
+                  "tape": r["tape"], "steps": r["steps"], "J": r["J0"]}
+    if "batched_checkpoint" in legs and plain is not None:
+        bc = legs["batched_checkpoint"]["value"]
+        out["ratios"] = {
+            "batched_full_over_plain_full": round(value / plain["value"], 3),
+            "batched_checkpoint_over_plain_full": round(bc / plain["value"], 3)}
+        if "plain_checkpoint" in legs:
+            out["ratios"]["batched_checkpoint_over_plain_checkpoint"] = round(
+                bc / legs["plain_checkpoint"]["value"], 3)
     if args.check_f32 and args.precision == "float64" and nb == 1:
-        X0 = torch.zeros(pb.engine.nctrl, dtype=torch.float64, device=pb.plan.device)
+        X0 = torch.as_tensor(X1, device=pb.plan.device)
+        retape(pb, "full")
         J64, G64 = pb.engine.run(X0)
         G64 = G64.clone()
         pb32 = build_problem(case, precision="float32", n_steps=n_steps, control="ic",
@@ -913,11 +1134,21 @@ def main():
             "J_rel_err_vs_f64": abs(J32.item() - J64.item()) / abs(J64.item())}
         log(f"f32 path: {out['f32_path']}")
         del pb32
+    pb.engine = None
     del pb
     release(None)
+    if not args.no_problem_set:
+        ps = problem_set(args, steps=extra_steps, warmup=1, separate=False)
+        out["problem_set"] = {k: ps[k] for k in ("value", "unit", "ms_per_step", "steps",
+                                                 "grid_point_steps_per_s", "build_s")}
+        out["problem_set"]["workload"] = ps["config"]["workload"]
+        out["problem_set"]["dst_split"] = ps["config"]["dst_split"]
+        log(f"problem set: {out['problem_set']}")
     if not args.no_stencil:
         out["stencil_roofline"] = stencil_roofline(case, args.precision)
         log(f"stencil roofline: {out['stencil_roofline']}")
+        out["adjoint_stencil_roofline"] = adj_stencil_roofline(case, args.precision)
+        log(f"adjoint stencil roofline: {out['adjoint_stencil_roofline']}")
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(case, args.cpu_sample_steps, args.precision,
                                            args.dst_split)

@@ -177,6 +177,16 @@ int vd_qg_h2pv(vd_plan* plan, const void* h, const void* hb, void* q, void* stre
 int vd_qg_rhs(vd_plan* plan, const void* h, const void* q, const void* base, double coef,
               void* out, void* stream);
 
+/* Transpose of vd_qg_rhs at (h, q): the vjp of coef * rhs(h2uv(h), q)
+ * (Qgm.h2uv + Qgm.rhs, jqgm.py:238-262, 321-375; the reference gets it from
+ * jax.vjp inside step_adj, jqgm.py:565-569) applied to the cotangent lam:
+ *   lq = acc + coef * (d rhs / d q)^T lam,   lh = coef * (d rhs / d h)^T lam,
+ * where h enters through the velocities.  acc NULL means zero and may alias
+ * lq; lam is read at neighbouring points, so neither output may alias it or
+ * h or q.  This is the adjoint RK stage kernel of vd_qg_step_adj. */
+int vd_qg_rhs_adj(vd_plan* plan, const void* h, const void* q, const void* lam, double coef,
+                  const void* acc, void* lq, void* lh, void* stream);
+
 /* field[i] = value where the plan mask is 0 (land); the reference fills the
  * initial state with NaN there (mapping/src/mod.py:697-700). */
 int vd_qg_fill_land(vd_plan* plan, void* field, double value, void* stream);

@@ -1,0 +1,241 @@
+"""The four QG workloads of the benchmark configurations, on the GPU, against
+the FP64 oracle (through the C ABI).
+
+* 128 x 128, 7 days (504 RK2 steps), FP64: the full-window gradient at a
+  flowing state against the oracle's; 50 L-BFGS-B iterations through
+  ``inv.minimize`` with the GPU engine, whose first iterates match an
+  oracle-driven run.
+* 256 x 256, 14 days (1008 steps), FP32 model with FP64 cost accumulation:
+  full tape and checkpoints every 32 steps give the same bits; centred finite
+  differences along 20 seeded directions with an eps sweep 1e-2 ... 1e-6;
+  the FP32 gradient's error against the FP64 oracle is stated.
+* 512 x 512 (dx 5 km, Ld 20 km, dt 900 s) on a 96-step window: the
+  FP64-MFMA path and the 5- and 6-digit split-product paths against the
+  oracle.
+* The 48 uneven regional problems (windows truncated to 40-72 steps so the
+  set builds in seconds): every member of the ragged engine equals its own
+  single-problem engine bit for bit, and the two smallest members match the
+  oracle.
+
+Bars are written per test.  Every evaluation is taken at X = 0.5 x truth (a
+flowing state; at the background x_b = 0 the forward stays at rest).
+"""
+import dataclasses
+import os
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from test_gpu_fourdvar import oracle_cost, rel
+from vardyn_amd import synthetic as syn
+from vardyn_amd.fourdvar import FourDVar
+from vardyn_amd.inv import minimize
+from vardyn_amd.problem import build_problem
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[configs] {msg}", file=sys.stderr, flush=True)
+
+
+def threads():
+    import scipy.fft
+    from threadpoolctl import threadpool_limits
+    n = min(16, os.cpu_count() or 1)
+
+    class Ctx:
+        def __enter__(self):
+            self.a = threadpool_limits(limits=n)
+            self.b = scipy.fft.set_workers(n)
+            self.a.__enter__()
+            self.b.__enter__()
+
+        def __exit__(self, *e):
+            self.b.__exit__(*e)
+            self.a.__exit__(*e)
+    return Ctx()
+
+
+# ------------------------------------------------------------------ 128^2, 7 days
+def test_basin128_full_window_gradient_against_oracle():
+    case = syn.case_basin128()
+    pb = build_problem(case, precision="float64", control="ic", tape="full")
+    X = 0.5 * pb.truth0.ravel()
+    J, G = pb.engine.cost_and_grad(X)
+    say(f"basin128: GPU J={J:.10e}; oracle over {case.n_steps} steps")
+    with threads():
+        Jr, Gr = oracle_cost(pb, "ic").cost_and_grad(X)
+    say(f"basin128: J rel err {abs(J - Jr) / abs(Jr):.2e}, G rel err {rel(G, Gr):.2e}")
+    assert abs(J - Jr) <= 1e-10 * abs(Jr)
+    assert rel(G, Gr) < 1e-9
+
+
+def test_basin128_lbfgs_50_iterations():
+    """50 iterations of the reference's minimiser loop on the GPU engine from the
+    background x_b = 0 (inv.py:266-415, scipy L-BFGS-B): the cost falls
+    monotonically through the accepted iterates, the best state is kept, and
+    the first two iterates equal those of the same loop driven by the oracle."""
+    case = syn.case_basin128()
+    pb = build_problem(case, precision="float64", control="ic", tape="full")
+    pb.engine.capture()
+    xs = []
+    x, info = minimize(pb.engine.cost_and_grad, np.zeros(pb.engine.nctrl), maxiter=50,
+                       callback=lambda xk: xs.append(np.array(xk)))
+    J = info["J"]
+    say(f"basin128 L-BFGS: {info['nit']} iterations, {info['nfev']} evaluations, "
+        f"J {J[0]:.6e} -> {info['best_cost']:.6e} ({info['status']})")
+    assert info["nit"] == 50 or info["status"].startswith("scipy:CONVERGENCE")
+    J0 = pb.engine.cost_and_grad(np.zeros(pb.engine.nctrl))[0]
+    Jx = pb.engine.cost_and_grad(x)[0]
+    assert Jx < 0.5 * J0
+    accepted = [pb.engine.cost_and_grad(xk)[0] for xk in xs]
+    assert all(b <= a * (1 + 1e-12) for a, b in zip(accepted, accepted[1:]))
+    assert info["best_cost"] <= accepted[-1] * (1 + 1e-12)
+    # the same loop on the oracle, two iterations
+    orc = oracle_cost(pb, "ic")
+    xo = []
+    with threads():
+        minimize(orc.cost_and_grad, np.zeros(pb.engine.nctrl), maxiter=2,
+                 callback=lambda xk: xo.append(np.array(xk)))
+    # iterate 1 is a scaled gradient step; iterate 2 uses y = g1 - g0, whose
+    # cancellation amplifies the gradients' 1e-10 agreement (measured 4e-7)
+    for k, bar in enumerate((1e-12, 1e-5)):
+        e = rel(xs[k], xo[k])
+        say(f"basin128 L-BFGS iterate {k + 1}: GPU vs oracle rel diff {e:.2e}")
+        assert e < bar
+
+
+# ------------------------------------------------------------------ 256^2, 14 days, FP32
+@pytest.fixture(scope="module")
+def basin256():
+    case = dataclasses.replace(syn.case_basin256(), checkpoint_steps=32)
+    pb32 = build_problem(case, precision="float32", control="ic", tape="full")
+    return case, pb32
+
+
+def test_basin256_f32_full_tape_equals_checkpoint_every_32(basin256):
+    case, pb = basin256
+    assert np.all(np.diff(pb.checkpoints) <= 32)
+    ck = FourDVar(pb.plan, pb.H, pb.checkpoints, pb.T, control="ic", sigma_b=case.sigma_b,
+                  tape="checkpoint", dtbasis=case.checkpoint_steps)
+    X = torch.as_tensor(0.5 * pb.truth0.ravel(), device=pb.plan.device)
+    J1, G1 = pb.engine.run(X)
+    J1, G1 = J1.clone(), G1.clone()
+    J2, G2 = ck.run(X)
+    torch.cuda.synchronize()
+    assert torch.equal(J1, J2) and torch.equal(G1, G2)
+
+
+def fd_sweep(eng, X, G0, label, n_dir=20):
+    """Best-eps relative error of the centred difference along ``n_dir`` seeded
+    directions (1 cm RMS) over eps = 1e-2 ... 1e-6, per direction."""
+    rng = np.random.default_rng(2024)
+    best = []
+    for k in range(n_dir):
+        d = torch.as_tensor(0.01 * rng.standard_normal(eng.nctrl), device=X.device)
+        gd = float(torch.dot(G0, d))
+        errs = []
+        for eps in (1e-2, 1e-3, 1e-4, 1e-5, 1e-6):
+            jp = float(eng.evaluate(X + eps * d)[0][0])
+            jm = float(eng.evaluate(X - eps * d)[0][0])
+            errs.append(abs((jp - jm) / (2 * eps) - gd) / abs(gd))
+        best.append(min(errs))
+        if k < 2:
+            say(f"{label} FD direction {k}: rel err per eps {['%.1e' % e for e in errs]}")
+    say(f"{label} FD: best-eps error over {n_dir} directions: median {np.median(best):.1e}, "
+        f"worst {max(best):.1e}")
+    return np.array(best)
+
+
+@pytest.mark.parametrize("precision,bar", [("float64", 1e-5), ("float32", 2e-2)])
+def test_basin256_finite_differences_20_directions(basin256, precision, bar):
+    """Centred differences of J along 20 seeded directions approach G.d.  The
+    upwind switches (where a T-point velocity changes sign) make J only
+    piecewise smooth, so the difference error falls linearly with eps (1e-2 at
+    eps = 1e-2, 1e-3 at 1e-3 in the FP32 runs); FP32 trajectory rounding sets
+    a floor near eps = 1e-4.  Bars: best eps within 1e-5 (FP64 model) and 2e-2
+    (FP32 model) for every direction."""
+    case, pb32 = basin256
+    if precision == "float64":
+        pb = build_problem(case, precision="float64", control="ic", tape="full",
+                           truth_plan=pb32.plan)
+    else:
+        pb = pb32
+    eng = pb.engine
+    eng.capture()
+    X = torch.as_tensor(0.5 * pb32.truth0.ravel(), device=pb.plan.device)
+    J, G = eng.evaluate(X)
+    best = fd_sweep(eng, X, G[0].clone(), f"basin256 {precision}")
+    assert np.all(best < bar), best
+
+
+def test_basin256_f32_gradient_error_against_oracle(basin256):
+    """FP32 model, FP64 accumulation: stated against the FP64 oracle over the
+    14-day window; bar 1e-2 (the FP32 trajectory's rounding grows with the flow)."""
+    case, pb = basin256
+    X = 0.5 * pb.truth0.ravel()
+    J, G = pb.engine.cost_and_grad(X)
+    with threads():
+        Jr, Gr = oracle_cost(pb, "ic").cost_and_grad(X)
+    e = rel(G, Gr)
+    say(f"basin256 FP32 vs FP64 oracle: J rel err {abs(J - Jr) / abs(Jr):.2e}, G rel err {e:.2e}")
+    assert abs(J - Jr) <= 1e-3 * abs(Jr)
+    assert e < 1e-2
+
+
+# ------------------------------------------------------------------ 512^2, 96 steps
+@pytest.mark.parametrize("split,tol", [(0, 1e-10), (5, 5e-8), (6, 1e-9)])
+def test_basin512_short_window_against_oracle(split, tol):
+    case = syn.case_basin512()
+    pb = build_problem(case, precision="float64", n_steps=96, control="ic", tape="full",
+                       dst_split=split)
+    X = 0.5 * pb.truth0.ravel()
+    J, G = pb.engine.cost_and_grad(X)
+    with threads():
+        Jr, Gr = oracle_cost(pb, "ic").cost_and_grad(X)
+    say(f"basin512 96 steps, dst_split {split}: J rel err {abs(J - Jr) / abs(Jr):.2e}, "
+        f"G rel err {rel(G, Gr):.2e}")
+    assert abs(J - Jr) <= tol * abs(Jr)
+    assert rel(G, Gr) < tol
+
+
+# ------------------------------------------------------------------ 48 uneven problems
+def truncated_config4():
+    out = []
+    for p, c in enumerate(syn.config4_cases()):
+        n = min(c.n_steps, 40 + 8 * (p % 5))
+        out.append(dataclasses.replace(c, n_steps=n, checkpoint_steps=max(2, int(round(n ** 0.5)))))
+    return out
+
+
+def test_config4_ragged_members_equal_single_engines(monkeypatch):
+    from vardyn_amd.ragged import RaggedFourDVar
+    monkeypatch.setenv("VARDYN_GEMM_CONFIG", "1")  # the ragged GEMMs run configuration 0
+    cases = truncated_config4()
+    pbs = [build_problem(c, precision="float64", control="ic", with_engine=False) for c in cases]
+    eng = RaggedFourDVar(pbs)
+    X = torch.as_tensor(np.concatenate([0.5 * pb.truth0.ravel() for pb in pbs]), device=eng.dev)
+    J, G = eng.run(X)
+    J, Gs = J.clone(), [g.clone() for g in eng.split(G)]
+    say(f"config4: {len(pbs)} problems, sides {sorted({c.ny for c in cases})}")
+    for i, pb in enumerate(pbs):
+        ref = FourDVar(pb.plan, pb.H, pb.checkpoints, pb.T, control="ic",
+                       sigma_b=pb.case.sigma_b, tape="full", dtbasis=pb.case.checkpoint_steps)
+        Jr, Gr = ref.run(eng.split(X)[i].contiguous())
+        assert torch.equal(J[i:i + 1], Jr), f"member {i}: J differs"
+        assert torch.equal(Gs[i], Gr[0]), f"member {i}: G differs"
+        del ref
+    small = sorted(range(len(pbs)), key=lambda i: cases[i].ny * cases[i].nx)[:2]
+    for i in small:
+        pb = pbs[i]
+        Xi = 0.5 * pb.truth0.ravel()
+        with threads():
+            Jr, Gr = oracle_cost(pb, "ic").cost_and_grad(Xi)
+        e = rel(Gs[i].cpu().numpy(), Gr)
+        say(f"config4 member {i} ({cases[i].ny}x{cases[i].nx}, {cases[i].n_steps} steps) vs "
+            f"oracle: G rel err {e:.2e}")
+        assert abs(float(J[i]) - Jr) <= 1e-10 * abs(Jr)
+        assert e < 1e-10

@@ -219,3 +219,37 @@ def test_rhs_stage_f64(opts):
     e = plan.rhs(H, Q, base=Q, coef=900.0).cpu().numpy()
     assert rel(e, q + 900.0 * ref) < 1e-13
 
+
+
+@pytest.mark.parametrize("opts", [dict(), dict(land=True, beta=True, kdiff=True, bathy=True)])
+@pytest.mark.parametrize("precision,tol", [("float64", 1e-12), ("float32", 2e-5)])
+def test_rhs_adj_against_oracle(precision, tol, opts):
+    """vd_qg_rhs_adj is the transpose of vd_qg_rhs: the oracle's hand vjp of
+    rhs(h2uv(h), q) (its own dot-product test pins it), with an accumulator."""
+    ny, nx = 66, 98
+    orc, plan, h0, hb, rng = make_pair(ny, nx, "rk2", precision, **opts)
+    q0 = orc.h2pv(h0, hb)
+    lam = rng.standard_normal((ny, nx))
+    acc = rng.standard_normal((ny, nx))
+    coef = 450.0
+    adh_ref, adq_ref = orc._k_adj(coef * lam, h0, q0)
+    lq, lh = plan.rhs_adj(plan.field(h0), plan.field(q0), plan.field(lam), coef, plan.field(acc))
+    torch.cuda.synchronize()
+    assert rel(lq.cpu().numpy(), acc + adq_ref) < tol
+    assert rel(lh.cpu().numpy(), adh_ref) < tol
+    # in place accumulation (acc aliases lq)
+    lq2 = plan.field(acc)
+    plan.rhs_adj(plan.field(h0), plan.field(q0), plan.field(lam), coef, lq2, lq2)
+    torch.cuda.synchronize()
+    assert torch.equal(lq2, lq)
+
+
+def test_rhs_adj_rejects_aliasing():
+    from vardyn_amd._lib import VardynError
+    orc, plan, h0, hb, rng = make_pair(40, 48, "rk2", "float64")
+    h, q = plan.field(h0), plan.field(orc.h2pv(h0, hb))
+    lam = plan.field(rng.standard_normal((40, 48)))
+    with pytest.raises(VardynError):
+        plan.rhs_adj(h, q, lam, 1.0, None, lam, plan.empty())
+    with pytest.raises(VardynError):
+        plan.rhs_adj(h, q, lam, 1.0, None, plan.empty(), lam)

@@ -138,3 +138,54 @@ def test_group_rejects_mixed_split():
     orc, plan64, _, _, _ = make_pair(40, 48, "rk2", "float64")
     with pytest.raises(VardynError):
         QgGroup([split_plan(orc, 40, 48, 6), plan64])
+
+
+# ---------------------------------------------------------------- against the oracle
+# The headline batched path runs 5 digits; these compare the split plans with the
+# FP64 oracle directly (not with the FP64-MFMA plan).  Bars per digit count: the
+# model step, tangent and adjoint over 12 steps within 5e-8 / 5e-10 relative
+# (5 / 6 digits: the per-product error of about 2^(5-8n) of the largest line
+# products, through 36 inversions), the 4D-Var cost within 1e-9 / 1e-11 and the
+# gradient within 5e-8 / 5e-10.
+ORACLE_TOL = {5: (5e-8, 1e-9), 6: (5e-10, 1e-11)}
+
+
+@pytest.mark.parametrize("ns", [5, 6])
+@pytest.mark.parametrize("scheme", ["Euler", "rk2", "rk3"])
+def test_step_tangent_adjoint_against_oracle(scheme, ns):
+    ny, nx, nstep = 66, 98, 12
+    orc, sp, h0, hb, rng = make_pair(ny, nx, scheme, "float64", land=True, beta=True, seed=5,
+                                     dst_split=ns)
+    H0, HB = sp.field(h0), sp.field(hb)
+    tol = ORACLE_TOL[ns][0]
+    h1 = sp.step(H0, HB, nstep)
+    torch.cuda.synchronize()
+    assert rel(h1.cpu().numpy(), orc.step(h0, hb, nstep)) < tol
+    dhn = np.nan_to_num(rng.standard_normal((ny, nx)) * 1e-3)
+    adn = rng.standard_normal((ny, nx))
+    tl = sp.step_tgl(sp.field(dhn), H0, HB, nstep)
+    adj = sp.step_adj(sp.field(adn), H0, HB, nstep)
+    torch.cuda.synchronize()
+    tl_ref, _ = orc.step_tgl(dhn, h0, hb, nstep)
+    ad_ref = orc.step_adj(adn, h0, hb, nstep)
+    ok = ~np.isnan(h0)
+    tl_h = tl.cpu().numpy()
+    assert np.linalg.norm((tl_h - tl_ref)[ok]) / np.linalg.norm(tl_ref[ok]) < tol
+    adj_h = adj.cpu().numpy()
+    assert np.linalg.norm((adj_h - ad_ref)[ok]) / np.linalg.norm(ad_ref[ok]) < tol
+
+
+@pytest.mark.parametrize("ns", [5, 6])
+def test_gradient_against_oracle(ns):
+    from test_gpu_fourdvar import oracle_cost
+    from vardyn_amd import synthetic as syn
+    from vardyn_amd.problem import build_problem
+    case = syn.QgCase("split", 130, 130, 10e3, 2.5, 1200.0, 96, n_tracks=6, track_km=7.0,
+                      checkpoint_steps=24, time_scheme="rk2", seed=11)
+    pb = build_problem(case, precision="float64", dst_split=ns)
+    X = 0.5 * pb.truth0.ravel()
+    J, G = pb.engine.cost_and_grad(X)
+    Jr, Gr = oracle_cost(pb, "ic").cost_and_grad(X)
+    tg, tj = ORACLE_TOL[ns][0], ORACLE_TOL[ns][1]
+    assert abs(J - Jr) <= tj * abs(Jr)
+    assert np.linalg.norm(G - Gr) / np.linalg.norm(Gr) < tg

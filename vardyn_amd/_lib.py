@@ -62,6 +62,7 @@ SIGNATURES = {
     "vd_qg_pv2h": (_I, [_VP, _VP, _VP, _VP, _VP, _VP]),
     "vd_qg_h2pv": (_I, [_VP, _VP, _VP, _VP, _VP]),
     "vd_qg_rhs": (_I, [_VP, _VP, _VP, _VP, C.c_double, _VP, _VP]),
+    "vd_qg_rhs_adj": (_I, [_VP, _VP, _VP, _VP, C.c_double, _VP, _VP, _VP, _VP]),
     "vd_qg_fill_land": (_I, [_VP, _VP, _D, _VP]),
     "vd_qg_prep": (_I, [_VP, _VP, _VP, _VP, _VP, _VP]),
     "vd_qg_finish": (_I, [_VP, _VP, _VP, _VP]),

@@ -698,6 +698,29 @@ int vd_qg_rhs(vd_plan* h, const void* hh, const void* q, const void* base, doubl
   return VD_OK;
 }
 
+int vd_qg_rhs_adj(vd_plan* h, const void* hh, const void* q, const void* lam, double coef,
+                  const void* acc, void* lq, void* lh, void* stream) {
+  VD_REQUIRE(h && hh && q && lam && lq && lh, VD_ERR_INVALID_ARG, "null argument");
+  VD_REQUIRE(lq != lam && lh != lam && lq != hh && lq != q && lh != hh && lh != q && lq != lh &&
+                 acc != lh,
+             VD_ERR_INVALID_ARG,
+             "lq and lh must not alias lam, h, q or each other (the stencil reads lam, h and q at "
+             "neighbouring points)");
+  Plan& p = h->p;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (p.prec == VD_F64) {
+    launch_stage_adj<double>(p, (const double*)hh, (const double*)q, (const double*)lam, nullptr,
+                             nullptr, coef, 0.0, 0.0, (const double*)acc, nullptr, nullptr,
+                             (double*)lq, (double*)lh, s);
+  } else {
+    launch_stage_adj<float>(p, (const float*)hh, (const float*)q, (const float*)lam, nullptr,
+                            nullptr, (float)coef, 0.f, 0.f, (const float*)acc, nullptr, nullptr,
+                            (float*)lq, (float*)lh, s);
+  }
+  VD_HIP_TRY(hipGetLastError());
+  return VD_OK;
+}
+
 int vd_qg_prep(vd_plan* h, const void* h0, const void* hb, void* hh, void* q, void* stream) {
   VD_REQUIRE(h && h0 && hh && q, VD_ERR_INVALID_ARG, "null argument");
   VD_REQUIRE(hh != h0 && q != h0, VD_ERR_INVALID_ARG,

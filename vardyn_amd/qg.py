@@ -254,3 +254,14 @@ class QgPlan:
         _lib.call("vd_qg_rhs", self._h, ptr(h), ptr(q), ptr(base), float(coef), ptr(out),
                   current_stream_ptr(self.device))
         return out
+
+    def rhs_adj(self, h, q, lam, coef=1.0, acc=None, lq=None, lh=None):
+        """Transpose of ``rhs`` at (h, q) applied to ``lam`` (the vjp of
+        ``coef * rhs(h2uv(h), q)``, jqgm.py:238-262, 321-375): returns
+        ``(acc + coef * drhs/dq^T lam, coef * drhs/dh^T lam)``."""
+        lq = self.empty() if lq is None else lq
+        lh = self.empty() if lh is None else lh
+        self._check(h, q, lam, acc, lq, lh)
+        _lib.call("vd_qg_rhs_adj", self._h, ptr(h), ptr(q), ptr(lam), float(coef), ptr(acc),
+                  ptr(lq), ptr(lh), current_stream_ptr(self.device))
+        return lq, lh
