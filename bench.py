@@ -85,12 +85,12 @@ def parse():
                     help="the 48-problem uneven-size workload instead (aggregate gradients/s)")
     ap.add_argument("--problem-set-size", type=int, default=48,
                     help="first N problems of the problem set (profiling)")
-    ap.add_argument("--problem-set-split", type=int, default=5,
+    ap.add_argument("--problem-set-split", type=int, default=6,
                     help="problem set: split-product digits of the ragged engine's sine GEMMs "
                          "(0 = FP64 MFMA)")
     ap.add_argument("--streams", type=int, default=4,
                     help="HIP streams the problem set is spread over")
-    ap.add_argument("--dst-split", type=int, default=5,
+    ap.add_argument("--dst-split", type=int, default=6,
                     help="batched path: sine-matrix GEMMs as split products on int8 MFMA with "
                          "this many digits per FP64 operand (0 = FP64 MFMA)")
     ap.add_argument("--plain-dst-split", type=int, default=0,

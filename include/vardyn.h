@@ -423,6 +423,7 @@ int vd_tiles_gather_adj(int precision, int ny, int nx, int ts, int nty, int ntx,
  * holds VD_REDUCE_SCRATCH doubles of device memory. */
 int vd_dot_f64(int64_t n, const double* x, const double* y, double alpha, double* out,
                double* scratch, void* stream);
+
 /* y[i] = a * x[i] + b * y[i] in the given precision (x, y same precision). */
 int vd_axpby(int precision, int64_t n, double a, const void* x, double b, void* y, void* stream);
 /* Precision conversion: dst[i] = (dst type) src[i]; NaN -> 0 if nan_to_zero. */

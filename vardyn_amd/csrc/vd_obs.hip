@@ -10,23 +10,34 @@ namespace vd {
 constexpr int kRedThreads = 256;
 constexpr int kRedBlocks = VD_REDUCE_SCRATCH;
 
-// Deterministic block sum: every block writes its partial to scratch[block];
-// a single-block pass adds them in index order.
+// Deterministic block sum, valid in thread 0: each 64-lane wavefront adds
+// its values by a fixed shuffle tree (lane l takes lane l + 32, 16, ..., 1),
+// then thread 0 adds the four wavefront sums in order.  Every block writes
+// its partial to part[block]; a single-block pass adds those in index order.
+constexpr int kWaves = kRedThreads / 64;
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
 __device__ __forceinline__ double block_sum(double v, double* sh) {
-  const int tid = threadIdx.x;
-  sh[tid] = v;
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
   __syncthreads();
-  for (int s = kRedThreads / 2; s > 0; s >>= 1) {
-    if (tid < s) sh[tid] += sh[tid + s];
-    __syncthreads();
+  double s = 0;
+  if (threadIdx.x == 0) {
+    s = sh[0];
+#pragma unroll
+    for (int w = 1; w < kWaves; ++w) s += sh[w];
   }
-  return sh[0];
+  return s;
 }
 
 __global__ __launch_bounds__(kRedThreads) void k_sum_partials(const double* __restrict__ part,
                                                               int n, double alpha,
                                                               double* __restrict__ out) {
-  __shared__ double sh[kRedThreads];
+  __shared__ double sh[kWaves];
   double v = 0;
   for (int i = threadIdx.x; i < n; i += kRedThreads) v += part[i];
   const double s = block_sum(v, sh);
@@ -47,7 +58,7 @@ __global__ __launch_bounds__(kRedThreads) void k_obs_misfit(
     const double* __restrict__ w, const double* __restrict__ y, const double* __restrict__ err,
     const TX* __restrict__ x, double* __restrict__ misfit, uint8_t* __restrict__ valid,
     double* __restrict__ part) {
-  __shared__ double sh[kRedThreads];
+  __shared__ double sh[kWaves];
   double m2 = 0;
   for (int o = blockIdx.x * kRedThreads + threadIdx.x; o < n_obs; o += gridDim.x * kRedThreads) {
     double hx = 0;
@@ -94,7 +105,7 @@ __global__ void k_obs_adjoint(int n_cols, const int32_t* __restrict__ tcol,
 __global__ __launch_bounds__(kRedThreads) void k_dot(int64_t n, const double* __restrict__ x,
                                                      const double* __restrict__ y,
                                                      double* __restrict__ part) {
-  __shared__ double sh[kRedThreads];
+  __shared__ double sh[kWaves];
   double v = 0;
   for (int64_t i = (int64_t)blockIdx.x * kRedThreads + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * kRedThreads)
