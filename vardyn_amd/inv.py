@@ -144,3 +144,26 @@ def minimize(fun, x0, maxiter=10, ftol=None, gtol=None, convergence_nit=None,
     nfev += w.nfev
     return x, {"J": w.J_list, "G": w.G_list, "nit": nit, "nfev": nfev, "status": status,
                "retries": retry, "best_cost": w.best_cost}
+
+
+def minimize_device(engine, X0, maxiter=10, ftol=None, gtol=None, m=10, maxls=20, callback=None):
+    """``minimize`` with the iterates, gradients and correction pairs in HBM
+    (``vardyn_amd.lbfgs.DeviceLbfgs``) for a ``FourDVar``-like ``engine``
+    (``evaluate(X) -> (J, G)`` on float64 device tensors; a batched engine
+    minimises each member on its own).  Options as the reference sets them
+    (inv.py:281-293): ``ftol`` as scipy's (default 2.22e-9), ``gtol`` relative
+    to ``max|g|`` at X = 0.  Returns ``(X, info)``: X (B, nctrl) on the device,
+    ``info["members"]`` the per-member cost history, iteration and evaluation
+    counts and stopping reason (scipy's wording)."""
+    import torch
+    from .lbfgs import DeviceLbfgs
+    B, n = engine.B, engine.nctrl
+    dev = engine.dev
+    X0 = torch.as_tensor(X0, dtype=torch.float64, device=dev).reshape(B, n)
+    gtol_abs = 1e-5
+    if gtol is not None:
+        _, g0 = engine.evaluate(torch.zeros_like(X0))
+        gtol_abs = gtol * float(torch.max(torch.abs(g0)))
+    lb = DeviceLbfgs(engine.evaluate, B, n, dev, m=m, maxls=maxls)
+    return lb.minimize(X0, maxiter=maxiter, ftol=2.220446049250313e-09 if ftol is None else ftol,
+                       gtol=gtol_abs, callback=callback)
