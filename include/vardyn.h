@@ -424,6 +424,21 @@ int vd_tiles_gather_adj(int precision, int ny, int nx, int ts, int nty, int ntx,
 int vd_dot_f64(int64_t n, const double* x, const double* y, double alpha, double* out,
                double* scratch, void* stream);
 
+/* The same dot product, allocating nothing: the partial sums go to a device
+ * array inside the library (VD_REDUCE_SCRATCH doubles, 8 KiB, one copy per
+ * device, held until the process ends), so the caller passes no scratch.  No
+ * counterpart in the reference: it is for callers that cannot allocate device
+ * memory (code under HIP-graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  The first call on a device looks the array's address up
+ * (hipGetSymbolAddress); make it outside graph capture.  n must not exceed
+ * VD_DOT_NOALLOC_MAX_N (2^36 doubles, more than the device memory) and any
+ * n up to it fits, since a call uses at most VD_REDUCE_SCRATCH partials.  All
+ * calls on one device share the array: calls that overlap on one device (on
+ * different streams) race. */
+#define VD_DOT_NOALLOC_MAX_N ((int64_t)1 << 36)
+int vd_dot_f64_noalloc(int64_t n, const double* x, const double* y, double alpha, double* out,
+                       void* stream);
+
 /* y[i] = a * x[i] + b * y[i] in the given precision (x, y same precision). */
 int vd_axpby(int precision, int64_t n, double a, const void* x, double b, void* y, void* stream);
 /* Precision conversion: dst[i] = (dst type) src[i]; NaN -> 0 if nan_to_zero. */

@@ -98,6 +98,7 @@ SIGNATURES = {
     "vd_merge_apply": (_I, [_VP, _I, _VP, _VP, _VP]),
     "vd_merge_blend": (_I, [C.c_longlong, C.c_double, C.c_double, _VP, _VP, _VP, _VP]),
     "vd_dot_f64": (_I, [_I64, _VP, _VP, _D, _VP, _VP, _VP]),
+    "vd_dot_f64_noalloc": (_I, [_I64, _VP, _VP, _D, _VP, _VP]),
     "vd_axpby": (_I, [_I, _I64, _D, _VP, _D, _VP, _VP]),
     "vd_convert": (_I, [_I, _VP, _I, _VP, _I64, _I, _VP]),
 }

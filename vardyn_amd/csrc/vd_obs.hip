@@ -5,6 +5,8 @@
 // datlas-ocean/VarDyn.
 #include "vd_internal.h"
 
+#include <mutex>
+
 namespace vd {
 
 constexpr int kRedThreads = 256;
@@ -138,6 +140,35 @@ static int grid1d(int64_t n) {
   return (int)b;
 }
 
+// Device scratch of vd_dot_f64_noalloc: one partial sum per block of a call,
+// at most kRedBlocks of them (8 KiB).  In host code `scratch` is a host-side
+// stand-in, not a device address: kernels get the address that
+// hipGetSymbolAddress returns for the current device (each device holds its
+// own copy of the array).
+static __device__ __align__(256) double scratch[VD_REDUCE_SCRATCH];
+
+constexpr int kMaxDevices = 64;
+static double* g_scratch[kMaxDevices];
+static std::mutex g_scratch_mu;
+
+// The current device's address of `scratch`, looked up on the first call on
+// that device and kept.
+static int device_scratch(double** out) {
+  int dev = -1;
+  VD_HIP_TRY(hipGetDevice(&dev));
+  VD_REQUIRE(dev >= 0 && dev < kMaxDevices, VD_ERR_UNSUPPORTED, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    VD_HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    VD_REQUIRE(p != nullptr, VD_ERR_HIP, "hipGetSymbolAddress returned no address for the "
+                                         "dot-product scratch");
+    g_scratch[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch[dev];
+  return VD_OK;
+}
+
 }  // namespace vd
 
 using namespace vd;
@@ -193,6 +224,25 @@ int vd_dot_f64(int64_t n, const double* x, const double* y, double alpha, double
   const int nb = red_blocks(n);
   hipLaunchKernelGGL(k_dot, dim3(nb), dim3(kRedThreads), 0, s, n, x, y, scratch);
   hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kRedThreads), 0, s, scratch, nb, alpha, out);
+  VD_HIP_TRY(hipGetLastError());
+  return VD_OK;
+}
+
+int vd_dot_f64_noalloc(int64_t n, const double* x, const double* y, double alpha, double* out,
+                       void* stream) {
+  VD_REQUIRE(x && y && out, VD_ERR_INVALID_ARG, "null argument");
+  VD_REQUIRE(n >= 0 && n <= VD_DOT_NOALLOC_MAX_N, VD_ERR_INVALID_ARG,
+             "vd_dot_f64_noalloc: n must be in [0, VD_DOT_NOALLOC_MAX_N]");
+  double* part = nullptr;
+  int rc = device_scratch(&part);
+  if (rc != VD_OK) return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (n == 0) return VD_OK;
+  // the same two launches as vd_dot_f64; every partial the second one reads
+  // is written by the first, so nothing in the scratch needs clearing
+  const int nb = red_blocks(n);
+  hipLaunchKernelGGL(k_dot, dim3(nb), dim3(kRedThreads), 0, s, n, x, y, part);
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kRedThreads), 0, s, part, nb, alpha, out);
   VD_HIP_TRY(hipGetLastError());
   return VD_OK;
 }
