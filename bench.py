@@ -67,6 +67,10 @@ def parse():
                     help="skip the checkpointed (segment start states + recompute) legs")
     ap.add_argument("--no-problem-set", action="store_true",
                     help="skip the 48-problem uneven set in the default run")
+    ap.add_argument("--minimize", action="store_true",
+                    help="the 4D-Var minimisation instead: --iterations L-BFGS iterations with the "
+                         "vectors in HBM, batched and plain paths")
+    ap.add_argument("--iterations", type=int, default=30)
     ap.add_argument("--oracle-full", action="store_true",
                     help="one full-window FP64 oracle gradient (all host cores, minutes) against "
                          "the GPU gradients of both sine-GEMM modes, at X = 0.5 x truth")
@@ -555,6 +559,64 @@ def oracle_full(args):
     return out
 
 
+def minimisation(args):
+    """The 4D-Var minimisation of the configured workload: ``--iterations``
+    L-BFGS iterations (30 by default) from the background X = 0 with the
+    iterates, gradients and correction pairs in HBM (``inv.minimize_device``:
+    FP64 device inner products, Moré-Thuente line search as L-BFGS-B's), for
+    the batched path (``--batch`` problems, each its own minimisation in
+    lockstep) and for the plain single-problem path.  Reports wall time per
+    minimisation, evaluations, and the cost reduction of every member."""
+    import torch
+    from vardyn_amd import synthetic as syn
+    from vardyn_amd.inv import minimize_device
+    from vardyn_amd.problem import build_batch, build_problem
+    case = syn.case_basin512()
+    n_steps = args.window_steps or case.n_steps
+    out = {"metric": f"4D-Var minimisation, {args.iterations} L-BFGS iterations, QG 512^2 "
+                     f"{n_steps}-step window", "unit": "minimisations/s",
+           "config": {"workload": case.name, "window_steps": n_steps, "iterations": args.iterations,
+                      "start": "X = 0 (background)", "memory_pairs": 10}}
+    legs = []
+    if not args.no_plain:
+        legs.append(("plain", 1, args.plain_dst_split))
+    if args.batch > 1:
+        legs.append(("batched", args.batch, args.dst_split))
+    for label, nb, split in legs:
+        t0 = time.time()
+        if nb == 1:
+            pb = build_problem(case, precision=args.precision, n_steps=n_steps, control="ic",
+                               tape="full", dst_split=split)
+        else:
+            pb = build_batch(case, nb, precision=args.precision, n_steps=n_steps, tape="full",
+                             dst_split=split)
+        tiles = pb.plan.autotune(reps=5) if not args.no_autotune else None
+        eng = pb.engine.capture()
+        log(f"minimise {label}: {nb} problem(s) built and captured in {time.time() - t0:.1f}s")
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        X, info = minimize_device(eng, np.zeros((nb, eng.nctrl)), maxiter=args.iterations)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t1
+        mem = info["members"]
+        r = {"problems": nb, "dst_split": split, "gemm_tiles": tiles, "wall_s": round(wall, 2),
+             "rounds": info["rounds"], "s_per_round": round(wall / info["rounds"], 4),
+             "minimisations_per_s": round(nb / wall, 5),
+             "gradients_per_s": round(sum(m["nfev"] for m in mem) / wall, 4),
+             "iterations": [m["nit"] for m in mem], "evaluations": [m["nfev"] for m in mem],
+             "status": sorted({m["status"] for m in mem}),
+             "J_first_last": [[m["J"][0], m["J"][-1]] for m in mem]}
+        log(f"minimise {label}: {r}")
+        out[label] = r
+        pb.engine = None
+        del pb, eng, X
+        release(None)
+    if "batched" in out and "plain" in out:
+        out["batched_over_plain"] = round(out["batched"]["minimisations_per_s"] /
+                                          out["plain"]["minimisations_per_s"], 3)
+    return out
+
+
 def problem_set(args, steps=None, warmup=None, separate=True):
     """Independent problems of uneven grid sizes and windows (48 by default).
 
@@ -1012,6 +1074,9 @@ def main():
         return
     if args.oracle_full:
         print(json.dumps(oracle_full(args)), flush=True)
+        return
+    if args.minimize:
+        print(json.dumps(minimisation(args)), flush=True)
         return
     if args.basis:
         print(json.dumps(basis(args)), flush=True)
