@@ -1095,6 +1095,7 @@ def main():
         return
     if args.stencil_only:
         out = {"stencil_roofline": stencil_roofline(case, args.precision),
+               "adjoint_stencil_roofline": adj_stencil_roofline(case, args.precision),
                "copy_calibration": copy_calibration()}
         print(json.dumps(out), flush=True)
         return

@@ -909,13 +909,18 @@ __global__ void k_finish_tl(Phys<T> P, const T* __restrict__ dh, T* __restrict__
 // Adjoint terms at an active T point t = (i, j) (mask == 3), for the
 // cotangent ak = a0 s0 + a1 s1 + a2 s2 of rhs at t:
 //   w[0] = lambda_uT, w[1] = lambda_vT (with the beta term),
-//   w[2..5] = up ak/(6dx), um ak/(6dx), vp ak/(6dy), vm ak/(6dy), w[6] = ak.
-// h and q are read through views of row pitch `pitch` at (hi, hj) (an LDS
-// tile); s0..s2, beta and f*bathymetry at the field point (i, j).
+//   w[2] = uT ak/(6dx), w[3] = vT ak/(6dy), w[4] = ak,
+// and the upwind side of each velocity (pu: uT > 0, pv: vT > 0).  The
+// transposed upwind stencils weight q by up ak/(6dx) = (pu ? w[2] : 0) and
+// um ak/(6dx) = (pu ? 0 : w[2]) (both are zero where uT == 0), so one value
+// and a flag carry both.  h and q are read through views of row pitch
+// `pitch` at (hi, hj) (an LDS tile); s0..s2, beta and f*bathymetry at the
+// field point (i, j).
 template <typename T>
 __device__ __forceinline__ void adj_terms(const T* h, const T* q, int pitch, int hi, int hj,
                                           const T* s0, const T* s1, const T* s2, T a0, T a1,
-                                          T a2, int i, int j, const Phys<T>& P, T w[7]) {
+                                          T a2, int i, int j, const Phys<T>& P, T w[5], bool* pu,
+                                          bool* pv) {
   const int nx = P.nx;
   const size_t o = (size_t)i * nx + j;
   T ak = a0 * s0[o];
@@ -932,11 +937,11 @@ __device__ __forceinline__ void adj_terms(const T* h, const T* q, int pitch, int
   w[0] = (V.uT < 0 ? T(0) : -aux * dxp) + (V.uT > 0 ? T(0) : aux * dxm);
   w[1] = (V.vT < 0 ? T(0) : -auy * dyp) + (V.vT > 0 ? T(0) : auy * dym);
   if (P.beta) w[1] -= P.beta[o] * ak;
-  w[2] = (V.uT < 0 ? T(0) : V.uT) * aux;
-  w[3] = (V.uT > 0 ? T(0) : V.uT) * aux;
-  w[4] = (V.vT < 0 ? T(0) : V.vT) * auy;
-  w[5] = (V.vT > 0 ? T(0) : V.vT) * auy;
-  w[6] = ak;
+  w[2] = V.uT * aux;
+  w[3] = V.vT * auy;
+  w[4] = ak;
+  *pu = V.uT > 0;
+  *pv = V.vT > 0;
 }
 
 // Fused adjoint stage (one launch, deterministic, no atomics).  A block owns
@@ -956,7 +961,7 @@ __device__ __forceinline__ void adj_terms(const T* h, const T* q, int pitch, int
 constexpr int kAX = 64, kAY = 16, kAH = 2;
 constexpr int kAXN = 32, kAYN = 32;  // narrow members
 
-template <typename T, int AX = kAX, int AY = kAY>
+template <typename T, int AX = kAX, int AY = kAY, bool KD = true>
 __device__ __forceinline__ void adj_tile(const Phys<T>& P, int i0, int j0, const T* h, const T* q,
                                          const T* s0, const T* s1, const T* s2, T a0, T a1, T a2,
                                          const T* acc0, const T* acc1, const T* acc2, T* lq_out,
@@ -964,12 +969,15 @@ __device__ __forceinline__ void adj_tile(const Phys<T>& P, int i0, int j0, const
   // acc* may alias lq_out (in-place accumulation): each thread reads and
   // writes only its own point of those fields.  s0..s2 are read at the halo
   // points of the tile, which other blocks write, so they must not alias
-  // lq_out.
+  // lq_out.  KD: the plan has PV diffusion (the ak array is kept for its
+  // transposed Laplacian).
   constexpr int kRW = AX + 2 * kAH, kRH = AY + 2 * kAH;
   constexpr int kHW = AX + 4 * kAH, kHH = AY + 4 * kAH;  // h, q tile: halo 4
-  __shared__ T W[7][kRH][kRW];
+  constexpr int NWA = KD ? 5 : 4;
+  __shared__ T W[NWA][kRH][kRW];
   __shared__ T Hs[kHH * kHW], Qs[kHH * kHW];
-  __shared__ uint8_t okU[kRH][kRW], okV[kRH][kRW];
+  // bit 0: u valid, 1: v valid, 2: uT > 0, 3: vT > 0 at each tile point
+  __shared__ uint8_t fl[kRH][kRW];
   const int ny = P.ny, nx = P.nx;
   const int tid = threadIdx.y * AX + threadIdx.x;
   for (int e = tid; e < kHH * kHW; e += AX * AY) {
@@ -984,13 +992,15 @@ __device__ __forceinline__ void adj_tile(const Phys<T>& P, int i0, int j0, const
     const int li = r / kRW, lj = r - li * kRW;
     const int i = i0 - kAH + li, j = j0 - kAH + lj;
     const int hi = li + kAH, hj = lj + kAH;  // the same point in the h, q tile
-    T w[7] = {0, 0, 0, 0, 0, 0, 0};
+    T w[5] = {0, 0, 0, 0, 0};
+    bool pu = false, pv = false;
     if (i >= 0 && i < ny && j >= 0 && j < nx && P.mask[(size_t)i * nx + j] == 3)
-      adj_terms(Hs, Qs, kHW, hi, hj, s0, s1, s2, a0, a1, a2, i, j, P, w);
+      adj_terms(Hs, Qs, kHW, hi, hj, s0, s1, s2, a0, a1, a2, i, j, P, w, &pu, &pv);
 #pragma unroll
-    for (int k = 0; k < 7; ++k) W[k][li][lj] = w[k];
-    okU[li][lj] = i >= 1 && i <= ny - 2 && j >= 1 && j <= nx - 1 && !u_isnan(Hs, kHW, hi, hj);
-    okV[li][lj] = i >= 1 && i <= ny - 1 && j >= 1 && j <= nx - 2 && !v_isnan(Hs, kHW, hi, hj);
+    for (int k = 0; k < NWA; ++k) W[k][li][lj] = w[k];
+    const bool oku = i >= 1 && i <= ny - 2 && j >= 1 && j <= nx - 1 && !u_isnan(Hs, kHW, hi, hj);
+    const bool okv = i >= 1 && i <= ny - 1 && j >= 1 && j <= nx - 2 && !v_isnan(Hs, kHW, hi, hj);
+    fl[li][lj] = (oku ? 1 : 0) | (okv ? 2 : 0) | (pu ? 4 : 0) | (pv ? 8 : 0);
   }
   __syncthreads();
   const int i = i0 + threadIdx.y, j = j0 + threadIdx.x;
@@ -1001,34 +1011,42 @@ __device__ __forceinline__ void adj_tile(const Phys<T>& P, int i0, int j0, const
   if (acc0) lq += acc0[o];
   if (acc1) lq += acc1[o];
   if (acc2) lq += acc2[o];
+  // the up / um (vp / vm) weights at LDS point (a, b)
+  auto wp = [&](int k, int bit, int a, int b) -> T { return (fl[a][b] & bit) ? W[k][a][b] : T(0); };
+  auto wm = [&](int k, int bit, int a, int b) -> T { return (fl[a][b] & bit) ? T(0) : W[k][a][b]; };
   // transpose of the x / y upwind stencils: T point t = p + d reads q(p)
-  lq += (-2) * W[2][li][lj - 1] + (-6) * W[3][li][lj - 1] + (-3) * W[2][li][lj] +
-        3 * W[3][li][lj] + 6 * W[2][li][lj + 1] + 2 * W[3][li][lj + 1] - W[2][li][lj + 2] +
-        W[3][li][lj - 2];
-  lq += (-2) * W[4][li - 1][lj] + (-6) * W[5][li - 1][lj] + (-3) * W[4][li][lj] +
-        3 * W[5][li][lj] + 6 * W[4][li + 1][lj] + 2 * W[5][li + 1][lj] - W[4][li + 2][lj] +
-        W[5][li - 2][lj];
-  if (P.has_kdiff) {
-    const T c = W[6][li][lj];
-    lq += P.kx * (W[6][li][lj + 1] + W[6][li][lj - 1] - 2 * c) +
-          P.ky * (W[6][li + 1][lj] + W[6][li - 1][lj] - 2 * c);
+  lq += (-2) * wp(2, 4, li, lj - 1) + (-6) * wm(2, 4, li, lj - 1) + (-3) * wp(2, 4, li, lj) +
+        3 * wm(2, 4, li, lj) + 6 * wp(2, 4, li, lj + 1) + 2 * wm(2, 4, li, lj + 1) -
+        wp(2, 4, li, lj + 2) + wm(2, 4, li, lj - 2);
+  lq += (-2) * wp(3, 8, li - 1, lj) + (-6) * wm(3, 8, li - 1, lj) + (-3) * wp(3, 8, li, lj) +
+        3 * wm(3, 8, li, lj) + 6 * wp(3, 8, li + 1, lj) + 2 * wm(3, 8, li + 1, lj) -
+        wp(3, 8, li + 2, lj) + wm(3, 8, li - 2, lj);
+  if constexpr (KD) {
+    if (P.has_kdiff) {
+      const T c = W[NWA - 1][li][lj];
+      lq += P.kx * (W[NWA - 1][li][lj + 1] + W[NWA - 1][li][lj - 1] - 2 * c) +
+            P.ky * (W[NWA - 1][li + 1][lj] + W[NWA - 1][li - 1][lj] - 2 * c);
+    }
   }
   lq_out[o] = lq;
   // lambda_u at u point (a, b) = 0.5 (W0[a][b] + W0[a][b-1]) where u is valid;
   // the arguments are LDS coordinates (every one used below is inside the tile
   // plus halo)
   auto lu = [&](int la, int lb) -> T {
-    return okU[la][lb] ? T(0.5) * (W[0][la][lb] + W[0][la][lb - 1]) : T(0);
+    return (fl[la][lb] & 1) ? T(0.5) * (W[0][la][lb] + W[0][la][lb - 1]) : T(0);
   };
   auto lv = [&](int la, int lb) -> T {
-    return okV[la][lb] ? T(0.5) * (W[1][la][lb] + W[1][la - 1][lb]) : T(0);
+    return (fl[la][lb] & 2) ? T(0.5) * (W[1][la][lb] + W[1][la - 1][lb]) : T(0);
   };
   lh_out[o] = P.cu * (lu(li - 1, lj + 1) + lu(li - 1, lj) - lu(li + 1, lj) - lu(li + 1, lj + 1)) +
               P.cv * (lv(li, lj - 1) + lv(li + 1, lj - 1) - lv(li + 1, lj + 1) - lv(li, lj + 1));
 }
 
-template <typename T, bool BAND>
-__global__ __launch_bounds__(kAX * kAY) void k_adj_fused(
+// Without diffusion the FP64 tile takes 77 KB of LDS, so two workgroups share
+// a CU when each thread keeps to 64 VGPRs (eight waves per SIMD).
+template <typename T, bool BAND, bool KD>
+__global__ __launch_bounds__(kAX * kAY) __attribute__((amdgpu_waves_per_eu(KD ? 4 : 8)))
+void k_adj_fused(
     Phys<T> P, const T* __restrict__ h, const T* __restrict__ q, const T* __restrict__ s0,
     const T* __restrict__ s1, const T* __restrict__ s2, T a0, T a1, T a2, const T* acc0,
     const T* acc1, const T* acc2, T* lq_out, T* __restrict__ lh_out) {
@@ -1037,7 +1055,8 @@ __global__ __launch_bounds__(kAX * kAY) void k_adj_fused(
   const size_t zo = (size_t)bz * P.nf;
   VD_B(h); VD_B(q); VD_B(s0); VD_B(s1); VD_B(s2); VD_B(acc0); VD_B(acc1); VD_B(acc2);
   VD_B(lq_out); VD_B(lh_out);
-  adj_tile(P, by * kAY, bx * kAX, h, q, s0, s1, s2, a0, a1, a2, acc0, acc1, acc2, lq_out, lh_out);
+  adj_tile<T, kAX, kAY, KD>(P, by * kAY, bx * kAX, h, q, s0, s1, s2, a0, a1, a2, acc0, acc1, acc2,
+                            lq_out, lh_out);
 }
 
 template <typename T, int AX, int AY>
@@ -1250,12 +1269,17 @@ void launch_stage_adj(const Plan& p, const T* h, const T* q, const T* s0, const 
   Phys<T> P = make_phys<T>(p);
   const int band = band_for(p, 6, sizeof(T));  // h, q, cotangent, accumulator in; lq, lh out
   const dim3 grid = band_grid(band, (p.nx + kAX - 1) / kAX, (p.ny + kAY - 1) / kAY, p.nb);
-  if (band)
-    hipLaunchKernelGGL((k_adj_fused<T, true>), grid, dim3(kAX, kAY), 0, s, P, h, q, s0, s1, s2, a0,
-                       a1, a2, acc0, acc1, acc2, lq_out, lh_out);
-  else
-    hipLaunchKernelGGL((k_adj_fused<T, false>), grid, dim3(kAX, kAY), 0, s, P, h, q, s0, s1, s2, a0,
-                       a1, a2, acc0, acc1, acc2, lq_out, lh_out);
+#define VD_ADJ(BAND, KD)                                                                        \
+  hipLaunchKernelGGL((k_adj_fused<T, BAND, KD>), grid, dim3(kAX, kAY), 0, s, P, h, q, s0, s1, s2, \
+                     a0, a1, a2, acc0, acc1, acc2, lq_out, lh_out)
+  if (band) {
+    if (p.has_kdiff) VD_ADJ(true, true);
+    else VD_ADJ(true, false);
+  } else {
+    if (p.has_kdiff) VD_ADJ(false, true);
+    else VD_ADJ(false, false);
+  }
+#undef VD_ADJ
 }
 
 template <typename T>
