@@ -501,18 +501,26 @@ __device__ __forceinline__ bool v_isnan(const T* h, int nx, int i, int j) {
                h[i * nx + j - 1]);
 }
 
+// View of a field for the stencil helpers: the value at offset (di, dj) from
+// the view's point of a row-major array (a global field or an LDS tile).
+template <typename T>
+struct PView {
+  const T* p;
+  int pitch;
+  __device__ __forceinline__ T operator()(int di, int dj) const { return p[di * pitch + dj]; }
+};
 // Upwind difference sums of adv (jqgm.py:387-394), without the 1/(6 d) factor.
 template <typename T>
 struct Diffs {
   T xp, xm, yp, ym;
 };
-template <typename T>
-__device__ __forceinline__ Diffs<T> diffs(const T* q, int nx, int i, int j) {
-  const T c = q[i * nx + j];
-  const T e1 = q[i * nx + j + 1], e2 = q[i * nx + j + 2];
-  const T w1 = q[i * nx + j - 1], w2 = q[i * nx + j - 2];
-  const T n1 = q[(i + 1) * nx + j], n2 = q[(i + 2) * nx + j];
-  const T s1 = q[(i - 1) * nx + j], s2 = q[(i - 2) * nx + j];
+template <typename T, typename V>
+__device__ __forceinline__ Diffs<T> diffs_v(const V& q) {
+  const T c = q(0, 0);
+  const T e1 = q(0, 1), e2 = q(0, 2);
+  const T w1 = q(0, -1), w2 = q(0, -2);
+  const T n1 = q(1, 0), n2 = q(2, 0);
+  const T s1 = q(-1, 0), s2 = q(-2, 0);
   Diffs<T> d;
   d.xp = 2 * e1 + 3 * c - 6 * w1 + w2;
   d.xm = e2 - 6 * e1 + 3 * c + 2 * w1;
@@ -520,17 +528,43 @@ __device__ __forceinline__ Diffs<T> diffs(const T* q, int nx, int i, int j) {
   d.ym = n2 - 6 * n1 + 3 * c + 2 * s1;
   return d;
 }
+template <typename T>
+__device__ __forceinline__ Diffs<T> diffs(const T* q, int nx, int i, int j) {
+  return diffs_v<T>(PView<T>{q + i * nx + j, nx});
+}
 
 template <typename T>
 struct Vel {
   T uT, vT;
 };
+// u at column offset dj of the view's row, v at row offset di of its column
+// (u_at / v_at of the view's field)
+template <typename T, typename V>
+__device__ __forceinline__ T u_v(const V& h, int dj, T cu) {
+  return nz(cu * (h(1, dj - 1) + h(1, dj) - h(-1, dj) - h(-1, dj - 1)));
+}
+template <typename T, typename V>
+__device__ __forceinline__ T v_v(const V& h, int di, T cv) {
+  return nz(cv * (h(di, 1) + h(di - 1, 1) - h(di - 1, -1) - h(di, -1)));
+}
+template <typename T, typename V>
+__device__ __forceinline__ Vel<T> vel_v(const V& h, const Phys<T>& P) {
+  Vel<T> r;
+  r.uT = T(0.5) * (u_v<T>(h, 0, P.cu) + u_v<T>(h, 1, P.cu));
+  r.vT = T(0.5) * (v_v<T>(h, 0, P.cv) + v_v<T>(h, 1, P.cv));
+  return r;
+}
 template <typename T>
 __device__ __forceinline__ Vel<T> vel_T(const T* h, int nx, int i, int j, const Phys<T>& P) {
-  Vel<T> r;
-  r.uT = T(0.5) * (u_at(h, nx, i, j, P.cu) + u_at(h, nx, i, j + 1, P.cu));
-  r.vT = T(0.5) * (v_at(h, nx, i, j, P.cv) + v_at(h, nx, i + 1, j, P.cv));
-  return r;
+  return vel_v<T>(PView<T>{h + i * nx + j, nx}, P);
+}
+template <typename T, typename V>
+__device__ __forceinline__ bool u_isnan_v(const V& h) {
+  return isnan(h(1, -1) + h(1, 0) + h(-1, 0) + h(-1, -1));
+}
+template <typename T, typename V>
+__device__ __forceinline__ bool v_isnan_v(const V& h) {
+  return isnan(h(0, 1) + h(-1, 1) + h(-1, -1) + h(0, -1));
 }
 
 // rhs (jqgm.py:321-375) at an active point (mask == 3, so 2 <= i,j <= n-3),
@@ -916,18 +950,17 @@ __global__ void k_finish_tl(Phys<T> P, const T* __restrict__ dh, T* __restrict__
 // and a flag carry both.  h and q are read through views of row pitch
 // `pitch` at (hi, hj) (an LDS tile); s0..s2, beta and f*bathymetry at the
 // field point (i, j).
-template <typename T>
-__device__ __forceinline__ void adj_terms(const T* h, const T* q, int pitch, int hi, int hj,
-                                          const T* s0, const T* s1, const T* s2, T a0, T a1,
-                                          T a2, int i, int j, const Phys<T>& P, T w[5], bool* pu,
-                                          bool* pv) {
+template <typename T, typename HV, typename QV>
+__device__ __forceinline__ void adj_terms(const HV& h, const QV& q, const T* s0, const T* s1,
+                                          const T* s2, T a0, T a1, T a2, int i, int j,
+                                          const Phys<T>& P, T w[5], bool* pu, bool* pv) {
   const int nx = P.nx;
   const size_t o = (size_t)i * nx + j;
   T ak = a0 * s0[o];
   if (s1) ak += a1 * s1[o];
   if (s2) ak += a2 * s2[o];
-  const Vel<T> V = vel_T(h, pitch, hi, hj, P);
-  const Diffs<T> D = diffs(q, pitch, hi, hj);
+  const Vel<T> V = vel_v<T>(h, P);
+  const Diffs<T> D = diffs_v<T>(q);
   T dxp = D.xp, dxm = D.xm, dyp = D.yp, dym = D.ym;
   if (P.fbathy) {
     const Diffs<T> B = diffs(P.fbathy, nx, i, j);
@@ -961,6 +994,43 @@ __device__ __forceinline__ void adj_terms(const T* h, const T* q, int pitch, int
 constexpr int kAX = 64, kAY = 16, kAH = 2;
 constexpr int kAXN = 32, kAYN = 32;  // narrow members
 
+// Phase 2 of an adjoint stage at output point p: lambda_q(p) (added to lq)
+// from the T points whose upwind / diffusion stencils read q(p), lambda_h(p)
+// from lambda_u, lambda_v at the velocity points that read h(p).  W(k, di,
+// dj) is adjoint term k at offset (di, dj) from p (0: lambda_uT, 1: lambda_vT,
+// 2: uT ak/(6dx), 3: vT ak/(6dy), 4: ak) and F(di, dj) its flags (bit 0 u
+// valid, 1 v valid, 2 uT > 0, 3 vT > 0).
+template <typename T, bool KD, typename WV, typename FV>
+__device__ __forceinline__ void adj_gather(const Phys<T>& P, const WV& W, const FV& F, T lq,
+                                           T* lq_out, T* lh_out) {
+  // the up / um (vp / vm) weights at offset (a, b)
+  auto wp = [&](int k, int bit, int a, int b) -> T { return (F(a, b) & bit) ? W(k, a, b) : T(0); };
+  auto wm = [&](int k, int bit, int a, int b) -> T { return (F(a, b) & bit) ? T(0) : W(k, a, b); };
+  // transpose of the x / y upwind stencils: T point t = p + d reads q(p)
+  lq += (-2) * wp(2, 4, 0, -1) + (-6) * wm(2, 4, 0, -1) + (-3) * wp(2, 4, 0, 0) +
+        3 * wm(2, 4, 0, 0) + 6 * wp(2, 4, 0, 1) + 2 * wm(2, 4, 0, 1) - wp(2, 4, 0, 2) +
+        wm(2, 4, 0, -2);
+  lq += (-2) * wp(3, 8, -1, 0) + (-6) * wm(3, 8, -1, 0) + (-3) * wp(3, 8, 0, 0) +
+        3 * wm(3, 8, 0, 0) + 6 * wp(3, 8, 1, 0) + 2 * wm(3, 8, 1, 0) - wp(3, 8, 2, 0) +
+        wm(3, 8, -2, 0);
+  if constexpr (KD) {
+    if (P.has_kdiff) {
+      const T c = W(4, 0, 0);
+      lq += P.kx * (W(4, 0, 1) + W(4, 0, -1) - 2 * c) + P.ky * (W(4, 1, 0) + W(4, -1, 0) - 2 * c);
+    }
+  }
+  *lq_out = lq;
+  // lambda_u at u point (a, b) = 0.5 (W0(a, b) + W0(a, b-1)) where u is valid
+  auto lu = [&](int a, int b) -> T {
+    return (F(a, b) & 1) ? T(0.5) * (W(0, a, b) + W(0, a, b - 1)) : T(0);
+  };
+  auto lv = [&](int a, int b) -> T {
+    return (F(a, b) & 2) ? T(0.5) * (W(1, a, b) + W(1, a - 1, b)) : T(0);
+  };
+  *lh_out = P.cu * (lu(-1, 1) + lu(-1, 0) - lu(1, 0) - lu(1, 1)) +
+            P.cv * (lv(0, -1) + lv(1, -1) - lv(1, 1) - lv(0, 1));
+}
+
 template <typename T, int AX = kAX, int AY = kAY, bool KD = true>
 __device__ __forceinline__ void adj_tile(const Phys<T>& P, int i0, int j0, const T* h, const T* q,
                                          const T* s0, const T* s1, const T* s2, T a0, T a1, T a2,
@@ -992,14 +1062,15 @@ __device__ __forceinline__ void adj_tile(const Phys<T>& P, int i0, int j0, const
     const int li = r / kRW, lj = r - li * kRW;
     const int i = i0 - kAH + li, j = j0 - kAH + lj;
     const int hi = li + kAH, hj = lj + kAH;  // the same point in the h, q tile
+    const PView<T> hv{Hs + hi * kHW + hj, kHW}, qv{Qs + hi * kHW + hj, kHW};
     T w[5] = {0, 0, 0, 0, 0};
     bool pu = false, pv = false;
     if (i >= 0 && i < ny && j >= 0 && j < nx && P.mask[(size_t)i * nx + j] == 3)
-      adj_terms(Hs, Qs, kHW, hi, hj, s0, s1, s2, a0, a1, a2, i, j, P, w, &pu, &pv);
+      adj_terms(hv, qv, s0, s1, s2, a0, a1, a2, i, j, P, w, &pu, &pv);
 #pragma unroll
     for (int k = 0; k < NWA; ++k) W[k][li][lj] = w[k];
-    const bool oku = i >= 1 && i <= ny - 2 && j >= 1 && j <= nx - 1 && !u_isnan(Hs, kHW, hi, hj);
-    const bool okv = i >= 1 && i <= ny - 1 && j >= 1 && j <= nx - 2 && !v_isnan(Hs, kHW, hi, hj);
+    const bool oku = i >= 1 && i <= ny - 2 && j >= 1 && j <= nx - 1 && !u_isnan_v<T>(hv);
+    const bool okv = i >= 1 && i <= ny - 1 && j >= 1 && j <= nx - 2 && !v_isnan_v<T>(hv);
     fl[li][lj] = (oku ? 1 : 0) | (okv ? 2 : 0) | (pu ? 4 : 0) | (pv ? 8 : 0);
   }
   __syncthreads();
@@ -1011,35 +1082,9 @@ __device__ __forceinline__ void adj_tile(const Phys<T>& P, int i0, int j0, const
   if (acc0) lq += acc0[o];
   if (acc1) lq += acc1[o];
   if (acc2) lq += acc2[o];
-  // the up / um (vp / vm) weights at LDS point (a, b)
-  auto wp = [&](int k, int bit, int a, int b) -> T { return (fl[a][b] & bit) ? W[k][a][b] : T(0); };
-  auto wm = [&](int k, int bit, int a, int b) -> T { return (fl[a][b] & bit) ? T(0) : W[k][a][b]; };
-  // transpose of the x / y upwind stencils: T point t = p + d reads q(p)
-  lq += (-2) * wp(2, 4, li, lj - 1) + (-6) * wm(2, 4, li, lj - 1) + (-3) * wp(2, 4, li, lj) +
-        3 * wm(2, 4, li, lj) + 6 * wp(2, 4, li, lj + 1) + 2 * wm(2, 4, li, lj + 1) -
-        wp(2, 4, li, lj + 2) + wm(2, 4, li, lj - 2);
-  lq += (-2) * wp(3, 8, li - 1, lj) + (-6) * wm(3, 8, li - 1, lj) + (-3) * wp(3, 8, li, lj) +
-        3 * wm(3, 8, li, lj) + 6 * wp(3, 8, li + 1, lj) + 2 * wm(3, 8, li + 1, lj) -
-        wp(3, 8, li + 2, lj) + wm(3, 8, li - 2, lj);
-  if constexpr (KD) {
-    if (P.has_kdiff) {
-      const T c = W[NWA - 1][li][lj];
-      lq += P.kx * (W[NWA - 1][li][lj + 1] + W[NWA - 1][li][lj - 1] - 2 * c) +
-            P.ky * (W[NWA - 1][li + 1][lj] + W[NWA - 1][li - 1][lj] - 2 * c);
-    }
-  }
-  lq_out[o] = lq;
-  // lambda_u at u point (a, b) = 0.5 (W0[a][b] + W0[a][b-1]) where u is valid;
-  // the arguments are LDS coordinates (every one used below is inside the tile
-  // plus halo)
-  auto lu = [&](int la, int lb) -> T {
-    return (fl[la][lb] & 1) ? T(0.5) * (W[0][la][lb] + W[0][la][lb - 1]) : T(0);
-  };
-  auto lv = [&](int la, int lb) -> T {
-    return (fl[la][lb] & 2) ? T(0.5) * (W[1][la][lb] + W[1][la - 1][lb]) : T(0);
-  };
-  lh_out[o] = P.cu * (lu(li - 1, lj + 1) + lu(li - 1, lj) - lu(li + 1, lj) - lu(li + 1, lj + 1)) +
-              P.cv * (lv(li, lj - 1) + lv(li + 1, lj - 1) - lv(li + 1, lj + 1) - lv(li, lj + 1));
+  auto Wv = [&](int k, int a, int b) -> T { return W[k < NWA ? k : 0][li + a][lj + b]; };
+  auto Fv = [&](int a, int b) -> int { return fl[li + a][lj + b]; };
+  adj_gather<T, KD>(P, Wv, Fv, lq, lq_out + o, lh_out + o);
 }
 
 // Without diffusion the FP64 tile takes 77 KB of LDS, so two workgroups share
