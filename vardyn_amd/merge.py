@@ -80,8 +80,12 @@ class SubwindowMerge:
         slat = np.ascontiguousarray(np.concatenate([la for _, la in axes]))
         slon = np.ascontiguousarray(np.concatenate([lo for lo, _ in axes]))
         m = None
-        if mask is not None and np.any(mask):
-            m = np.ascontiguousarray(np.asarray(mask, dtype=np.uint8))
+        if mask is not None:
+            mask = np.asarray(mask)
+            if mask.shape != (self.ny, self.nx):
+                raise VardynError(f"mask shape {mask.shape}, target grid {(self.ny, self.nx)}")
+            if np.any(mask):
+                m = np.ascontiguousarray(mask.astype(np.uint8))
         p = MergeParams(self.ny, self.nx, lon.ctypes.data, lat.ctypes.data,
                         m.ctypes.data if m is not None else None, self.nsub,
                         sny.ctypes.data, snx.ctypes.data, slat.ctypes.data, slon.ctypes.data)
@@ -132,13 +136,26 @@ class SubwindowMerge:
             parts.append(f.reshape(-1))
         return torch.cat(parts), T
 
+    def _check_device_tensor(self, t, what, shape=None, numel=None):
+        if not isinstance(t, torch.Tensor):
+            raise VardynError(f"{what} must be a torch tensor on {self.device}")
+        if t.device != self.device or t.dtype != torch.float64 or not t.is_contiguous():
+            raise VardynError(f"{what} must be a contiguous float64 tensor on {self.device}")
+        if shape is not None and tuple(t.shape) != tuple(shape):
+            raise VardynError(f"{what} has shape {tuple(t.shape)}, expected {tuple(shape)}")
+        if numel is not None and t.numel() != numel:
+            raise VardynError(f"{what} holds {t.numel()} values, expected {numel}")
+
     def merge_packed(self, packed, T, out=None):
         """Merge from a buffer made by ``pack``; ``out`` (T, ny, nx)."""
-        if packed.dtype != torch.float64 or not packed.is_contiguous() or packed.numel() != T * self.sub_points:
-            raise VardynError("packed fields must be a contiguous float64 buffer of T x subwindow points")
+        T = int(T)
+        if T < 1:
+            raise VardynError("T must be at least 1")
+        self._check_device_tensor(packed, "packed fields", numel=T * self.sub_points)
         if out is None:
             out = torch.empty((T, self.ny, self.nx), dtype=torch.float64, device=self.device)
-        _lib.call("vd_merge_apply", self._h, int(T), ptr(packed), ptr(out), current_stream_ptr(self.device))
+        self._check_device_tensor(out, "out", shape=(T, self.ny, self.nx))
+        _lib.call("vd_merge_apply", self._h, T, ptr(packed), ptr(out), current_stream_ptr(self.device))
         return out
 
     def merge(self, fields, out=None):
@@ -180,11 +197,18 @@ def time_window_weights(date, list_date_start, list_date_end):
 
 def blend(a, b, w1, w2, out=None):
     """``W1 a + W2 b`` of two windows' outputs on the GPU (run_assimilation.py:815-818)."""
-    if a.shape != b.shape or a.dtype != torch.float64 or b.dtype != torch.float64:
-        raise VardynError("blend needs two float64 tensors of one shape")
+    for t, what in ((a, "a"), (b, "b")):
+        if not isinstance(t, torch.Tensor) or t.dtype != torch.float64 or not t.is_cuda:
+            raise VardynError(f"blend: {what} must be a float64 device tensor")
+    if a.shape != b.shape or a.device != b.device:
+        raise VardynError("blend needs two tensors of one shape on one device")
     a, b = a.contiguous(), b.contiguous()
     if out is None:
         out = torch.empty_like(a)
+    if (not isinstance(out, torch.Tensor) or out.shape != a.shape or out.dtype != torch.float64
+            or out.device != a.device or not out.is_contiguous()):
+        raise VardynError("blend: out must be a contiguous float64 tensor of the inputs' shape "
+                          "and device")
     _lib.call("vd_merge_blend", C.c_longlong(a.numel()), float(w1), float(w2), ptr(a), ptr(b), ptr(out),
               current_stream_ptr(a.device))
     return out
