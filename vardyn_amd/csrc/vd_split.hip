@@ -293,15 +293,34 @@ __device__ __forceinline__ void split_tile(const SplitArgs& sa, int z, int tile)
       for (int b = 0; b < NT; ++b)
 #pragma unroll
         for (int r = 0; r < 16; ++r) acc[l][a][b][r] = 0;
-  auto step = [&](int st, const SineFrag& cur, SineFrag& nxt) {
-    i32x4 df[NS][NT];
+  // The loop is unrolled over the K steps of this wave (st = wk + i WK) so that
+  // the fragment buffers are registers: sine fragments are loaded two steps
+  // ahead (three buffers) and data fragments one step ahead (two buffers), which
+  // keeps the L2 and LDS latency of the loads off the MFMA issue.
+  typedef i32x4 DataFrag[NS][NT];
+  auto load_data = [&](int st, DataFrag& df) {
 #pragma unroll
     for (int q = 0; q < NS; ++q)
 #pragma unroll
       for (int b = 0; b < NT; ++b)
         df[q][b] = *reinterpret_cast<const i32x4*>(
             &Dl[q][wd * WDL + b * 32 + (lane & 31)][st * 32 + 16 * (lane >> 5)]);
-    if (st + WK < nstep) load_sine(st + WK, nxt);
+  };
+  SineFrag sf[3];
+  DataFrag dfr[2];
+  if (wk < nstep) {
+    load_sine(wk, sf[0]);
+    load_data(wk, dfr[0]);
+  }
+  if (wk + WK < nstep) load_sine(wk + WK, sf[1]);
+#pragma unroll
+  for (int i = 0; i < NSTEP; ++i) {
+    const int st = wk + i * WK;
+    if (st >= nstep) break;
+    if (st + 2 * WK < nstep) load_sine(st + 2 * WK, sf[(i + 2) % 3]);
+    if (st + WK < nstep) load_data(st + WK, dfr[(i + 1) % 2]);
+    const SineFrag& cur = sf[i % 3];
+    const DataFrag& df = dfr[i % 2];
 #pragma unroll
     for (int p = 0; p < NS; ++p)
 #pragma unroll
@@ -312,12 +331,6 @@ __device__ __forceinline__ void split_tile(const SplitArgs& sa, int z, int tile)
           for (int b = 0; b < NT; ++b)
             acc[p + q][a][b] =
                 __builtin_amdgcn_mfma_i32_32x32x32_i8(cur[p][a], df[q][b], acc[p + q][a][b], 0, 0, 0);
-  };
-  SineFrag sf0, sf1;
-  if (wk < nstep) load_sine(wk, sf0);
-  for (int st = wk; st < nstep; st += 2 * WK) {
-    step(st, sf0, sf1);
-    if (st + WK < nstep) step(st + WK, sf1, sf0);
   }
   if constexpr (WK > 1) {
     // K-split partials: groups 1..WK-1 park each level in LDS (over the data
