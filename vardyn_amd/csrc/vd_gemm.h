@@ -202,10 +202,46 @@ __device__ __forceinline__ GemmPtrs<T> gemm_ptrs(const GemmArgs<T>& g, int z) {
   return P;
 }
 
-// Epilogue for output element (row, col) of a product.
+// Epilogue for output element (row, col) of a product, in three parts so that
+// a kernel can issue the loads of all its elements before the first store and
+// keep the few boundary elements' extra stores out of the store sequence (an
+// output pointer may alias an input, so the compiler keeps every load behind
+// the stores before it, and a load between stores waits for all of them):
+//   epi_input  the loads: hb for EK_FIELD_SET, the field for EK_FIELD_ACC,
+//              the two Lambda tables for EK_SCALE;
+//   epi_store  the element's own store, given those values;
+//   epi_ring   EK_FIELD_SET's ring stores (h = hb on the boundary rows and
+//              columns) of an element next to the boundary.
+template <typename T>
+struct EpiIn {
+  T x;             // hb or the field
+  double lcx, lcy;  // Lambda tables at the element's column and row
+};
+
 template <typename T, int KIND, int MAP>
-__device__ __forceinline__ void store_out(const GemmArgs<T>& g, const GemmPtrs<T>& P, int row,
-                                          int col, T v) {
+__device__ __forceinline__ EpiIn<T> epi_input(const GemmArgs<T>& g, const GemmPtrs<T>& P, int row, int col) {
+  // Straight-line code: every load is unconditional, from an index clamped
+  // into its array (an element outside the interior reads element 0), and
+  // without hb the output field stands in; epi_store discards what it does
+  // not use.  So all of a thread's loads are in flight together.
+  EpiIn<T> in{T(0), 0.0, 0.0};
+  if (MAP == MAP_ROWFOLD || KIND == EK_STORE) return in;
+  const int oc = MAP == MAP_COLFOLD ? 2 * col + P.par : col;
+  const bool ok = row < g.my && (MAP == MAP_COLFOLD ? col < g.hx : col < g.mx) && oc < g.mx;
+  if (KIND == EK_SCALE) {
+    in.lcx = g.lcx[ok ? oc : 0];
+    in.lcy = g.lcy[ok ? row : 0];
+    return in;
+  }
+  const T* src = KIND == EK_FIELD_SET && P.hbm ? P.hbm : P.fout;
+  const size_t i = ok ? row + 1 : 1, j = ok ? oc + 1 : 1;
+  in.x = src[i * g.nx + j];
+  return in;
+}
+
+template <typename T, int KIND, int MAP>
+__device__ __forceinline__ void epi_store(const GemmArgs<T>& g, const GemmPtrs<T>& P, int row, int col, T v,
+                                          const EpiIn<T>& in) {
   if (MAP == MAP_ROWFOLD) {
     if (row < g.hy) P.Cout[(size_t)(2 * row + P.par) * g.ldc + col] = v;
     return;
@@ -220,7 +256,7 @@ __device__ __forceinline__ void store_out(const GemmArgs<T>& g, const GemmPtrs<T
       double lam;
       {
 #pragma clang fp contract(off)
-        lam = g.gof * (g.lcx[oc] + g.lcy[row]) - P.gfc2;
+        lam = g.gof * (in.lcx + in.lcy) - P.gfc2;
       }
       P.Cout[o] = (row < g.my && oc < g.mx) ? v * T(1.0 / lam) : T(0);
     } else {
@@ -230,18 +266,23 @@ __device__ __forceinline__ void store_out(const GemmArgs<T>& g, const GemmPtrs<T
   }
   const bool ok = row < g.my && (MAP == MAP_COLFOLD ? col < g.hx : col < g.mx);
   if (!ok) return;
+  const size_t i = row + 1, j = oc + 1;
+  P.fout[i * g.nx + j] = (KIND == EK_FIELD_ACC || P.hbm ? in.x : T(0)) + v;
+}
+
+template <typename T, int KIND, int MAP>
+__device__ __forceinline__ void epi_ring(const GemmArgs<T>& g, const GemmPtrs<T>& P, int row, int col) {
+  if (MAP == MAP_ROWFOLD || KIND != EK_FIELD_SET) return;
+  const bool ok = row < g.my && (MAP == MAP_COLFOLD ? col < g.hx : col < g.mx);
+  const int oc = MAP == MAP_COLFOLD ? 2 * col + P.par : col;
+  const bool top = row == 0, bot = row == g.my - 1, lft = oc == 0, rgt = oc == g.mx - 1;
+  if (!ok || !(top || bot || lft || rgt)) return;
   const int nx = g.nx, ny = g.ny;
   const size_t i = row + 1, j = oc + 1;
   T* fout = P.fout;
-  if (KIND == EK_FIELD_ACC) {
-    fout[i * nx + j] += v;
-    return;
-  }
   const T* hb = P.hbm;
-  fout[i * nx + j] = (hb ? hb[i * nx + j] : T(0)) + v;
   // ring: h = hb on rows 0, ny-1 and columns 0, nx-1 (jqgm.py:316-317)
   auto ring = [&](size_t ii, size_t jj) { fout[ii * nx + jj] = hb ? hb[ii * nx + jj] : T(0); };
-  const bool top = row == 0, bot = row == g.my - 1, lft = oc == 0, rgt = oc == g.mx - 1;
   if (top) ring(0, j);
   if (bot) ring(ny - 1, j);
   if (lft) ring(i, 0);
@@ -250,6 +291,12 @@ __device__ __forceinline__ void store_out(const GemmArgs<T>& g, const GemmPtrs<T
   if (top && rgt) ring(0, nx - 1);
   if (bot && lft) ring(ny - 1, 0);
   if (bot && rgt) ring(ny - 1, nx - 1);
+}
+
+template <typename T, int KIND, int MAP>
+__device__ __forceinline__ void store_out(const GemmArgs<T>& g, const GemmPtrs<T>& P, int row, int col, T v) {
+  epi_store<T, KIND, MAP>(g, P, row, col, v, epi_input<T, KIND, MAP>(g, P, row, col));
+  epi_ring<T, KIND, MAP>(g, P, row, col);
 }
 
 

@@ -202,14 +202,32 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs<T>& g, int z, int tile)
     }
   }
 
+  // epilogue inputs of every element first, so that their loads are in flight together
+  EpiIn<T> in[MT][NT][4];
 #pragma unroll
   for (int a_ = 0; a_ < MT; ++a_)
 #pragma unroll
     for (int b_ = 0; b_ < NT; ++b_)
 #pragma unroll
       for (int r = 0; r < 4; ++r)
-        store_out<T, KIND, MAP>(g, P, m0 + wm * (BM / 2) + a_ * 16 + Mfma<T>::out_row(lane, r),
-                                n0 + wn * (BN / 2) + b_ * 16 + (lane & 15), val[a_][b_][r]);
+        in[a_][b_][r] = epi_input<T, KIND, MAP>(g, P, m0 + wm * (BM / 2) + a_ * 16 + Mfma<T>::out_row(lane, r),
+                                                n0 + wn * (BN / 2) + b_ * 16 + (lane & 15));
+#pragma unroll
+  for (int a_ = 0; a_ < MT; ++a_)
+#pragma unroll
+    for (int b_ = 0; b_ < NT; ++b_)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        epi_store<T, KIND, MAP>(g, P, m0 + wm * (BM / 2) + a_ * 16 + Mfma<T>::out_row(lane, r),
+                                n0 + wn * (BN / 2) + b_ * 16 + (lane & 15), val[a_][b_][r], in[a_][b_][r]);
+#pragma unroll
+  for (int a_ = 0; a_ < MT; ++a_)
+#pragma unroll
+    for (int b_ = 0; b_ < NT; ++b_)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        epi_ring<T, KIND, MAP>(g, P, m0 + wm * (BM / 2) + a_ * 16 + Mfma<T>::out_row(lane, r),
+                               n0 + wn * (BN / 2) + b_ * 16 + (lane & 15));
 }
 
 template <typename T, int AOP, int BOP, int KIND, int MAP, int BM, int BN, int KS>
@@ -391,14 +409,32 @@ __global__ __launch_bounds__(256 * KS) void k_gemm2(GemmArgs<T> g) {
           for (int r = 0; r < 4; ++r) val[a_][b_][r] += src[((a_ * NT + b_) * 4 + r) * 256 + gt];
     }
   }
+  // epilogue inputs of every element first, so that their loads are in flight together
+  EpiIn<T> in[MT][NT][4];
 #pragma unroll
   for (int a_ = 0; a_ < MT; ++a_)
 #pragma unroll
     for (int b_ = 0; b_ < NT; ++b_)
 #pragma unroll
       for (int r = 0; r < 4; ++r)
-        store_out<T, KIND, MAP>(g, P, m0 + wm * (BM / 2) + a_ * 16 + Mfma<T>::out_row(lane, r),
-                                n0 + wn * (BN / 2) + b_ * 16 + r16, val[a_][b_][r]);
+        in[a_][b_][r] = epi_input<T, KIND, MAP>(g, P, m0 + wm * (BM / 2) + a_ * 16 + Mfma<T>::out_row(lane, r),
+                                                n0 + wn * (BN / 2) + b_ * 16 + r16);
+#pragma unroll
+  for (int a_ = 0; a_ < MT; ++a_)
+#pragma unroll
+    for (int b_ = 0; b_ < NT; ++b_)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        epi_store<T, KIND, MAP>(g, P, m0 + wm * (BM / 2) + a_ * 16 + Mfma<T>::out_row(lane, r),
+                                n0 + wn * (BN / 2) + b_ * 16 + r16, val[a_][b_][r], in[a_][b_][r]);
+#pragma unroll
+  for (int a_ = 0; a_ < MT; ++a_)
+#pragma unroll
+    for (int b_ = 0; b_ < NT; ++b_)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        epi_ring<T, KIND, MAP>(g, P, m0 + wm * (BM / 2) + a_ * 16 + Mfma<T>::out_row(lane, r),
+                               n0 + wn * (BN / 2) + b_ * 16 + r16);
 }
 
 template <typename T, int AOP, int BOP, int KIND, int MAP>

@@ -172,7 +172,8 @@ __device__ __forceinline__ void split_tile(const SplitArgs& sa, int z, int tile)
   const GemmArgs<double>& g = sa.g;
   const GemmPtrs<double> P = gemm_ptrs<double, AOP, BOP, MAP>(g, z);
   const int s0 = (tile / g.tn) * BSL, d0 = (tile % g.tn) * BDL;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // the wave index is the same for every lane of a wave: in a scalar register
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int ws = wave % WS, wd = (wave / WS) % WD, wk = wave / (WS * WD);
   const int nstep = g.K / 32;  // 1 .. NSTEP
 
@@ -391,7 +392,17 @@ __device__ __forceinline__ void split_tile(const SplitArgs& sa, int z, int tile)
     }
   } else {
     constexpr int TP = BSL + 1;  // row pitch (doubles) of the transposed tile
+    constexpr int NE = BDL * BSL / NTHR;  // output elements per thread
     static_assert(BDL * TP * 8 <= NS * BDL * KP, "output tile fits the data planes");
+    static_assert(NE * NTHR == BDL * BSL, "the threads cover the output tile");
+    // the epilogue's global inputs (hb, the Lambda tables) of this thread's
+    // elements, all in flight while the tile goes through LDS
+    EpiIn<double> in[NE];
+#pragma unroll
+    for (int it = 0; it < NE; ++it) {
+      const int e = tid + NTHR * it;
+      in[it] = epi_input<double, KIND, MAP>(g, P, d0 + e / BSL, s0 + e % BSL);
+    }
     double* T = reinterpret_cast<double*>(&Dl[0][0][0]);
     __syncthreads();  // every wave is done with the data planes
     if (wk == 0) {
@@ -405,9 +416,15 @@ __device__ __forceinline__ void split_tile(const SplitArgs& sa, int z, int tile)
                 value(a, b, r);
     }
     __syncthreads();
-    for (int e = tid; e < BDL * BSL; e += NTHR) {
-      const int dl = e / BSL, sl = e % BSL;
-      store_out<double, KIND, MAP>(g, P, d0 + dl, s0 + sl, T[dl * TP + sl]);
+#pragma unroll
+    for (int it = 0; it < NE; ++it) {
+      const int e = tid + NTHR * it, dl = e / BSL, sl = e % BSL;
+      epi_store<double, KIND, MAP>(g, P, d0 + dl, s0 + sl, T[dl * TP + sl], in[it]);
+    }
+#pragma unroll
+    for (int it = 0; it < NE; ++it) {
+      const int e = tid + NTHR * it;
+      epi_ring<double, KIND, MAP>(g, P, d0 + e / BSL, s0 + e % BSL);
     }
   }
 }
