@@ -441,6 +441,60 @@ int vd_dot_f64_noalloc(int64_t n, const double* x, const double* y, double alpha
 
 /* y[i] = a * x[i] + b * y[i] in the given precision (x, y same precision). */
 int vd_axpby(int precision, int64_t n, double a, const void* x, double b, void* y, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Multilayer rotating shallow-water model on the Arakawa C grid (class SW,
+ * mapping/models/model_qgsw/sw.py:67-937): the constructor (:117-323, with
+ * the masks of masks.py:28-159), step (:752-904, RK3-SSP), step_tgl
+ * (:906-923, the reference's jax.jvp of step) and step_adj (:925-937, its
+ * jax.vjp with respect to the initial state).
+ *
+ * FP64.  Arrays are the reference's, x before y, y contiguous, one plane per
+ * layer: u (nl, nx+1, ny), v (nl, nx, ny+1), h (nl, nx, ny) in physical
+ * units (velocities in m/s, h the layer-thickness perturbation in m); f on
+ * the (nx+1) x (ny+1) vorticity grid.  Supported: per-layer constant depths,
+ * scalar dx / dy, any binary ocean mask, slip coefficient, linear bottom
+ * drag on the last layer, the smooth thickness clamp.  Viscosity and
+ * thickness diffusion (sw.py:512-573) are rejected with VD_ERR_UNSUPPORTED;
+ * wind stress, sponge, open boundaries, the barotropic filter and external
+ * forcing have no parameter (the reference's defaults: none of them).
+ * ------------------------------------------------------------------------- */
+typedef struct vd_sw_model vd_sw_model;
+typedef struct vd_sw_params {
+  int nx, ny, nl;
+  double dx, dy, dt;
+  const double* H;        /* nl layer depths (m) */
+  const double* g_prime;  /* nl reduced gravities (m/s^2), g_prime[0] the surface one */
+  const double* f;        /* (nx+1) x (ny+1) Coriolis parameter on the vorticity grid */
+  const uint8_t* mask;    /* nx x ny, 1 = ocean; null: all ocean */
+  double slip_coef;       /* 1 free slip, 0 no slip */
+  double bottom_drag_coef;
+  double h_min, h_min_sharpness; /* smooth clamp of the total thickness (defaults 0.1 m, 10) */
+  double visc_coef, diff_coef;   /* must be 0 */
+} vd_sw_params;
+
+/* Build the masks and upload the grid constants to the current device. */
+int vd_sw_create(const vd_sw_params* params, vd_sw_model** model);
+int vd_sw_destroy(vd_sw_model* model);
+/* Number of values of u, v and h (all layers). */
+int vd_sw_info(const vd_sw_model* model, long long* nu, long long* nv, long long* nh);
+/* (du, dv, dh) = time derivatives at the SCALED state (u dx, v dy, h dx dy),
+ * compute_time_derivatives (sw.py:723-750); device pointers. */
+int vd_sw_rhs(vd_sw_model* model, const double* u, const double* v, const double* h, double* du,
+              double* dv, double* dh, void* stream);
+/* (u1, v1, h1) = nstep RK3-SSP steps from (u0, v0, h0); device pointers. */
+int vd_sw_step(vd_sw_model* model, const double* u0, const double* v0, const double* h0, double* u1,
+               double* v1, double* h1, int nstep, void* stream);
+/* (du1, dv1, dh1) = tangent of step at (u0, v0, h0) applied to (du0, dv0, dh0). */
+int vd_sw_step_tgl(vd_sw_model* model, const double* u0, const double* v0, const double* h0,
+                   const double* du0, const double* dv0, const double* dh0, double* du1, double* dv1,
+                   double* dh1, int nstep, void* stream);
+/* (au0, av0, ah0) = adjoint of step at (u0, v0, h0) applied to (au1, av1, ah1).
+ * The trajectory is recomputed in segments of about sqrt(nstep) steps; the
+ * model keeps a device tape of (segments + segment length) states. */
+int vd_sw_step_adj(vd_sw_model* model, const double* u0, const double* v0, const double* h0,
+                   const double* au1, const double* av1, const double* ah1, double* au0, double* av0,
+                   double* ah0, int nstep, void* stream);
 /* Precision conversion: dst[i] = (dst type) src[i]; NaN -> 0 if nan_to_zero. */
 int vd_convert(int src_precision, const void* src, int dst_precision, void* dst, int64_t n,
                int nan_to_zero, void* stream);

@@ -39,3 +39,7 @@ def test_bm_params_layout_matches_header():
 def test_merge_params_layout_matches_header():
     from vardyn_amd.merge import MergeParams
     assert [f for f, _ in MergeParams._fields_] == struct_fields("vd_merge_params")
+
+
+def test_sw_params_layout_matches_header():
+    assert [f for f, _ in _lib.SwParams._fields_] == struct_fields("vd_sw_params")

@@ -45,6 +45,17 @@ class ObsTable(C.Structure):
     ]
 
 
+class SwParams(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int), ("nl", C.c_int),
+        ("dx", C.c_double), ("dy", C.c_double), ("dt", C.c_double),
+        ("H", C.c_void_p), ("g_prime", C.c_void_p), ("f", C.c_void_p), ("mask", C.c_void_p),
+        ("slip_coef", C.c_double), ("bottom_drag_coef", C.c_double),
+        ("h_min", C.c_double), ("h_min_sharpness", C.c_double),
+        ("visc_coef", C.c_double), ("diff_coef", C.c_double),
+    ]
+
+
 # name -> (restype, argtypes)
 _VP, _I, _D, _SZ, _I64 = C.c_void_p, C.c_int, C.c_double, C.c_size_t, C.c_int64
 SIGNATURES = {
@@ -91,6 +102,13 @@ SIGNATURES = {
     "vd_bm_workspace_size": (_I, [_VP, _I, C.POINTER(_SZ)]),
     "vd_bm_operg": (_I, [_VP, _I, _VP, _VP, _VP, _VP, _VP]),
     "vd_bm_operg_adj": (_I, [_VP, _I, _VP, _VP, _VP, _VP, _VP]),
+    "vd_sw_create": (_I, [C.POINTER(SwParams), C.POINTER(_VP)]),
+    "vd_sw_destroy": (_I, [_VP]),
+    "vd_sw_info": (_I, [_VP, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
+    "vd_sw_rhs": (_I, [_VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP]),
+    "vd_sw_step": (_I, [_VP, _VP, _VP, _VP, _VP, _VP, _VP, _I, _VP]),
+    "vd_sw_step_tgl": (_I, [_VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _I, _VP]),
+    "vd_sw_step_adj": (_I, [_VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _I, _VP]),
     "vd_merge_create": (_I, [_VP, _VP]),
     "vd_merge_destroy": (_I, [_VP]),
     "vd_merge_info": (_I, [_VP, C.POINTER(C.c_longlong)]),

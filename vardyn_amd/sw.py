@@ -1,0 +1,230 @@
+"""Multilayer rotating shallow-water model on the GPU (class ``SW`` of
+mapping/models/model_qgsw/sw.py:67-1035, datlas-ocean/VarDyn).
+
+The same constructor dictionary and the same calls as the reference:
+``step(u0, v0, h0, nstep)``, ``step_tgl(u0, v0, h0, du0, dv0, dh0, nstep)``
+(the reference's ``jax.jvp`` of ``step``, sw.py:906-923) and
+``step_adj(u0, v0, h0, wuT, wvT, whT, nstep)`` (its ``jax.vjp``, sw.py:925-937,
+returned in the same nesting: ``((au0, av0, ah0, None, None),)``; the H and
+h_wind cotangents are not computed), plus ``adjoint_test_sw`` (:940-1035) and
+the physical/scaled conversions.  Fields are the reference's: u (nl, nx+1,
+ny), v (nl, nx, ny+1), h (nl, nx, ny), optionally with the ensemble axis of
+length 1 in front; f on the (nx+1, ny+1) vorticity grid.
+
+The model runs in libvardyn_hip (``vd_sw_*``, FP64).  Parameters the GPU
+model does not implement are rejected rather than ignored: viscosity and
+thickness diffusion, wind stress, sponge, open boundaries, the barotropic
+filter, external forcing, spatially varying depths or metrics, and
+ensembles of more than one member.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import SwParams, VardynError
+from .qg import current_stream_ptr, ptr
+
+
+def _scalar(x, what):
+    a = np.asarray(x, dtype=np.float64)
+    if a.size != 1 and not np.all(a == a.flat[0]):
+        raise VardynError(f"sw: {what} must be a scalar (spatially varying {what} is not supported)")
+    return float(a.flat[0])
+
+
+class SW:
+    """``SW(param)`` with the reference's parameter dictionary (sw.py:117-323)."""
+
+    _UNSUPPORTED_TRUE = ("barotropic_filter", "flag_obc", "flag_linear")
+
+    def __init__(self, param, device=None):
+        p = dict(param)
+        self.device = torch.device(device if device is not None else "cuda")
+        self.nx, self.ny, self.nl = int(p["nx"]), int(p["ny"]), int(p["nl"])
+        self.dx = _scalar(p["dx"], "dx")
+        self.dy = _scalar(p["dy"], "dy")
+        self.area = self.dx * self.dy
+        H = np.asarray(p["H"], dtype=np.float64)
+        if H.ndim >= 3 and (H.shape[-1] != 1 or H.shape[-2] != 1):
+            raise VardynError("sw: H must be constant per layer (shape (nl, 1, 1))")
+        self.H = H.reshape(-1)
+        gp = np.asarray(p["g_prime"], dtype=np.float64).reshape(-1)
+        if self.H.size != self.nl or gp.size != self.nl:
+            raise VardynError(f"sw: H and g_prime need nl = {self.nl} values")
+        self.g_prime = gp
+        f = np.asarray(p["f"], dtype=np.float64)
+        if f.shape != (self.nx + 1, self.ny + 1):
+            raise VardynError(f"sw: f must be ({self.nx + 1}, {self.ny + 1}), got {f.shape}")
+        self.f = np.ascontiguousarray(f)
+        self.dt = float(p["dt"])
+        mask = p.get("mask")
+        if mask is not None:
+            mask = np.asarray(mask)
+            if mask.shape != (self.nx, self.ny):
+                raise VardynError(f"sw: mask must be ({self.nx}, {self.ny}), got {mask.shape}")
+            vals = np.unique(mask)
+            if not np.all(np.isin(vals, (0, 1))) or vals.tolist() == [0]:
+                raise VardynError("sw: the mask must be binary with at least one ocean cell")
+            mask = np.ascontiguousarray(mask, dtype=np.uint8)
+        self.mask = mask
+        for key in ("visc_coef", "diff_coef", "sponge_coef"):
+            if float(p.get(key, 0.0) or 0.0) != 0.0:
+                raise VardynError(f"sw: {key} != 0 is not implemented")
+        for key in ("taux", "tauy"):
+            t = np.asarray(p.get(key, 0.0), dtype=np.float64)
+            if np.any(t != 0):
+                raise VardynError(f"sw: wind stress ({key}) is not implemented")
+        for key in self._UNSUPPORTED_TRUE:
+            if p.get(key, False):
+                raise VardynError(f"sw: {key} is not implemented")
+        if int(p.get("n_ens", 1)) != 1:
+            raise VardynError("sw: one ensemble member")
+        if p.get("forcing_momentum", "direct") != "direct":
+            raise VardynError("sw: only direct momentum forcing")
+        self.slip_coef = float(p.get("slip_coef", 1.0))
+        self.bottom_drag_coef = float(p.get("bottom_drag_coef", 0.0))
+        self.h_min = float(p.get("h_min", 0.1))
+        self.h_min_sharpness = float(p.get("h_min_sharpness", 10.0))
+        self._keep = (self.H, self.g_prime, self.f, self.mask)
+        sp = SwParams(self.nx, self.ny, self.nl, self.dx, self.dy, self.dt,
+                      self.H.ctypes.data, self.g_prime.ctypes.data, self.f.ctypes.data,
+                      self.mask.ctypes.data if self.mask is not None else None,
+                      self.slip_coef, self.bottom_drag_coef, self.h_min, self.h_min_sharpness, 0.0, 0.0)
+        lib = _lib.load()
+        h = C.c_void_p()
+        with torch.cuda.device(self.device):
+            _lib.check(lib.vd_sw_create(C.byref(sp), C.byref(h)), "vd_sw_create")
+        self._h = h
+        self.u_shape = (self.nl, self.nx + 1, self.ny)
+        self.v_shape = (self.nl, self.nx, self.ny + 1)
+        self.h_shape = (self.nl, self.nx, self.ny)
+        m = _host_masks(self)
+        self.mask_u, self.mask_v, self.mask_h = m
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            try:
+                _lib.load().vd_sw_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    # ------------------------------------------------------------ fields
+    def _field(self, a, shape, what):
+        t = torch.as_tensor(a, dtype=torch.float64, device=self.device)
+        ens = t.dim() == len(shape) + 1
+        if ens:
+            if t.shape[0] != 1:
+                raise VardynError("sw: one ensemble member")
+            t = t[0]
+        if tuple(t.shape) != shape:
+            raise VardynError(f"sw: {what} must be {shape}, got {tuple(t.shape)}")
+        return t.contiguous(), ens
+
+    def _state(self, u, v, h, names=("u", "v", "h")):
+        (u, e1), (v, e2), (h, e3) = (self._field(u, self.u_shape, names[0]),
+                                     self._field(v, self.v_shape, names[1]),
+                                     self._field(h, self.h_shape, names[2]))
+        return u, v, h, e1 or e2 or e3
+
+    @staticmethod
+    def _out(x, ens):
+        return x.unsqueeze(0) if ens else x
+
+    # ------------------------------------------------------------ model
+    def compute_time_derivatives(self, u, v, h):
+        """Time derivatives (dt_u, dt_v, dt_h) at the scaled state (sw.py:723-750)."""
+        u, v, h, ens = self._state(u, v, h)
+        out = [torch.empty_like(a) for a in (u, v, h)]
+        _lib.check(_lib.load().vd_sw_rhs(self._h, ptr(u), ptr(v), ptr(h), *(ptr(o) for o in out),
+                                         current_stream_ptr()), "vd_sw_rhs")
+        return tuple(self._out(o, ens) for o in out)
+
+    def step(self, u0, v0, h0, nstep=1, **kw):
+        """nstep RK3-SSP steps (sw.py:752-904); physical fields in and out."""
+        _no_extras(kw)
+        u, v, h, ens = self._state(u0, v0, h0)
+        out = [torch.empty_like(a) for a in (u, v, h)]
+        _lib.check(_lib.load().vd_sw_step(self._h, ptr(u), ptr(v), ptr(h), *(ptr(o) for o in out), int(nstep),
+                                          current_stream_ptr()), "vd_sw_step")
+        return tuple(self._out(o, ens) for o in out)
+
+    def step_tgl(self, u0, v0, h0, du0, dv0, dh0, H=None, dH=None, nstep=1, **kw):
+        """Tangent of ``step`` at (u0, v0, h0) applied to (du0, dv0, dh0)."""
+        if H is not None or dH is not None:
+            raise VardynError("sw: tangents with respect to H are not implemented")
+        _no_extras(kw)
+        u, v, h, ens = self._state(u0, v0, h0)
+        du, dv, dh, _ = self._state(du0, dv0, dh0, ("du0", "dv0", "dh0"))
+        out = [torch.empty_like(a) for a in (u, v, h)]
+        _lib.check(_lib.load().vd_sw_step_tgl(self._h, ptr(u), ptr(v), ptr(h), ptr(du), ptr(dv), ptr(dh),
+                                              *(ptr(o) for o in out), int(nstep), current_stream_ptr()),
+                   "vd_sw_step_tgl")
+        return tuple(self._out(o, ens) for o in out)
+
+    def step_adj(self, u0, v0, h0, wuT, wvT, whT, H=None, nstep=1, **kw):
+        """Adjoint of ``step`` at (u0, v0, h0) applied to (wuT, wvT, whT), as
+        the reference's ``vjp_fn(cotangents)``: ``((au0, av0, ah0, None, None),)``."""
+        if H is not None:
+            raise VardynError("sw: adjoints with respect to H are not implemented")
+        _no_extras(kw)
+        u, v, h, ens = self._state(u0, v0, h0)
+        wu, wv, wh, _ = self._state(wuT, wvT, whT, ("wuT", "wvT", "whT"))
+        out = [torch.empty_like(a) for a in (u, v, h)]
+        _lib.check(_lib.load().vd_sw_step_adj(self._h, ptr(u), ptr(v), ptr(h), ptr(wu), ptr(wv), ptr(wh),
+                                              *(ptr(o) for o in out), int(nstep), current_stream_ptr()),
+                   "vd_sw_step_adj")
+        au, av, ah = (self._out(o, ens) for o in out)
+        return ((au, av, ah, None, None),)
+
+    # ------------------------------------------------------------ conversions
+    def get_physical_uvh(self, u, v, h, numpy=False):
+        """sw.py:423-429: scaled -> physical."""
+        up, vp, hp = (torch.as_tensor(u) / self.dx, torch.as_tensor(v) / self.dy, torch.as_tensor(h) / self.area)
+        return tuple(x.cpu().numpy() for x in (up, vp, hp)) if numpy else (up, vp, hp)
+
+    def set_input_uvh(self, u_phys, v_phys, h_phys):
+        """sw.py:431-449: physical -> scaled (masked)."""
+        u, v, h, _ = self._state(u_phys, v_phys, h_phys)
+        return (u * self.mask_u) * self.dx, (v * self.mask_v) * self.dy, (h * self.mask_h) * self.area
+
+    def adjoint_test_sw(self, nstep=1, seed=42):
+        """<M dx, y> / <dx, M* y> with masked random vectors scaled by 1e-4
+        (sw.py:940-1035; the reference draws them with jax.random, here a
+        seeded NumPy generator)."""
+        rng = np.random.default_rng(seed)
+
+        def rand(shape, m):
+            return torch.as_tensor(rng.standard_normal(shape) * 1e-4, device=self.device) * m
+        ms = (self.mask_u, self.mask_v, self.mask_h)
+        shapes = (self.u_shape, self.v_shape, self.h_shape)
+        x0 = [rand(s, m) for s, m in zip(shapes, ms)]
+        dx0 = [rand(s, m) for s, m in zip(shapes, ms)]
+        w = [rand(s, m) for s, m in zip(shapes, ms)]
+        dx1 = self.step_tgl(*x0, *dx0, nstep=nstep)
+        ad = self.step_adj(*x0, *w, nstep=nstep)[0][:3]
+        ps1 = sum(float(torch.sum(a * b)) for a, b in zip(dx1, w))
+        ps2 = sum(float(torch.sum(a * b)) for a, b in zip(dx0, ad))
+        return ps1 / ps2
+
+
+def _no_extras(kw):
+    extra = {k: v for k, v in kw.items() if v is not None}
+    if extra:
+        raise VardynError(f"sw: {sorted(extra)} not implemented (boundary fields, forcing, wind, h_wind)")
+
+
+def _host_masks(model):
+    """u, v, h masks on the device (masks.py:64-66), for the conversions."""
+    h = np.ones((model.nx, model.ny)) if model.mask is None else model.mask.astype(np.float64)
+    hp = np.pad(h, ((1, 1), (0, 0)))
+    mu = ((hp[1:] + hp[:-1]) / 2 > 3 / 4).astype(np.float64)
+    hq = np.pad(h, ((0, 0), (1, 1)))
+    mv = ((hq[:, 1:] + hq[:, :-1]) / 2 > 3 / 4).astype(np.float64)
+    dev = model.device
+    return (torch.as_tensor(mu, device=dev), torch.as_tensor(mv, device=dev), torch.as_tensor(h, device=dev))
