@@ -109,6 +109,10 @@ def parse():
     ap.add_argument("--basis", action="store_true",
                     help="the balanced-motion wavelet basis (Basis_bm) on a 512x512 lon/lat grid "
                          "instead: operg + operg_transpose over the window's basis times")
+    ap.add_argument("--sw", action="store_true",
+                    help="the 3-layer shallow-water model (256x256, dx 8 km, beta plane, dt 120 s) instead: "
+                         "forward steps and the gradient of a final-state cost over the 10-day window")
+    ap.add_argument("--sw-days", type=float, default=10.0, help="shallow-water window (days)")
     ap.add_argument("--tile-size", type=int, default=32)
     ap.add_argument("--tile-overlap", type=int, default=16)
     ap.add_argument("--tiled-steps", type=int, default=16,
@@ -944,6 +948,102 @@ def basis(args):
                                    f"({cpu_s:.1f} s; matrix set-up {orc_setup:.1f} s not timed)"}}
 
 
+def sw_case(nx=256, ny=256):
+    """The 3-layer shallow-water configuration: 256 x 256 cells of 8 km, beta
+    plane f = 1e-4 + 1.6e-11 y, layers 500 / 1000 / 3000 m, reduced gravities
+    0.02 (surface) / 0.02 / 0.01 m/s^2, dt = 120 s, no-slip walls.  The
+    initial state: a seeded smooth thickness anomaly in the top layer (RMS
+    5 cm, 50 km Gaussian smoothing) and the geostrophic velocities of its
+    pressure, zero in the layers below."""
+    import numpy as np
+    from scipy.ndimage import gaussian_filter
+    dx = dy = 8e3
+    y = (np.arange(ny + 1) - ny / 2) * dy
+    f = np.broadcast_to(1e-4 + 1.6e-11 * y, (nx + 1, ny + 1)).copy()
+    H = np.array([500.0, 1000.0, 3000.0])
+    gp = np.array([0.02, 0.02, 0.01])
+    param = dict(nx=nx, ny=ny, nl=3, dx=dx, dy=dy, H=H.reshape(3, 1, 1), g_prime=gp, f=f, dt=120.0,
+                 slip_coef=0.0, bottom_drag_coef=0.0)
+    rng = np.random.default_rng(2024)
+    eta = gaussian_filter(rng.standard_normal((nx, ny)), 50e3 / dx)
+    eta *= 0.05 / eta.std()
+    h = np.zeros((3, nx, ny)); h[0] = eta
+    # geostrophic velocities of the top-layer pressure g'_0 eta: f u = -g' deta/dy, f v = g' deta/dx
+    f_u = 0.5 * (f[:, 1:] + f[:, :-1]); f_v = 0.5 * (f[1:, :] + f[:-1, :])
+    u = np.zeros((3, nx + 1, ny)); v = np.zeros((3, nx, ny + 1))
+    ey = np.gradient(eta, dy, axis=1); ex = np.gradient(eta, dx, axis=0)
+    u[0, 1:nx, :] = -gp[0] * 0.5 * (ey[1:] + ey[:-1]) / f_u[1:nx]
+    v[0, :, 1:ny] = gp[0] * 0.5 * (ex[:, 1:] + ex[:, :-1]) / f_v[:, 1:ny]
+    return param, (u, v, h)
+
+
+def sw_model(args):
+    """The multilayer shallow-water model (sw.py:67-937) on the 256 x 256,
+    3-layer configuration of ``sw_case``: forward RK3-SSP steps per second,
+    and the gradient of the final-state cost J = 1/2 |x(T)|^2 over the window
+    (``step`` then ``step_adj``: a forward sweep keeping sqrt(n) segment
+    states, each segment recomputed, three adjoint RHS evaluations per step).
+    Beside them the FP64 oracle (NumPy, one host thread) over a few steps of
+    the same grid, and parity against it."""
+    import numpy as np
+    import torch
+    from oracle.sw import SwOracle
+    from vardyn_amd.sw import SW
+    param, x0 = sw_case()
+    model = SW(param)
+    dev = torch.device("cuda")
+    X = [torch.as_tensor(a, device=dev) for a in x0]
+    n_steps = int(round(args.sw_days * 86400 / param["dt"]))
+    st = torch.cuda.current_stream()
+    ev = lambda: torch.cuda.Event(enable_timing=True)
+    # forward
+    model.step(*X, nstep=8)
+    torch.cuda.synchronize()
+    e0, e1 = ev(), ev()
+    e0.record(st)
+    xT = model.step(*X, nstep=n_steps)
+    e1.record(st)
+    torch.cuda.synchronize()
+    fwd_s = e0.elapsed_time(e1) / 1e3
+    # gradient of J = 1/2 |x(T)|^2 with respect to x0
+    e2, e3 = ev(), ev()
+    e2.record(st)
+    xT = model.step(*X, nstep=n_steps)
+    (g,) = model.step_adj(*X, *xT, nstep=n_steps)
+    e3.record(st)
+    torch.cuda.synchronize()
+    grad_s = e2.elapsed_time(e3) / 1e3
+    finite = all(bool(torch.isfinite(t).all()) for t in list(xT) + list(g[:3]))
+    # oracle on the host: a few steps, timed and compared
+    k = 6
+    orc = SwOracle(param["nx"], param["ny"], param["dx"], param["dy"], param["H"].reshape(-1), param["g_prime"],
+                   param["f"], param["dt"], slip_coef=param["slip_coef"], bottom_drag_coef=0.0)
+    t0 = time.time()
+    ref = orc.step(*x0, nstep=k)
+    cpu_s = time.time() - t0
+    got = model.step(*X, nstep=k)
+    torch.cuda.synchronize()
+    err = max(float(np.linalg.norm(a.cpu().numpy() - b) / np.linalg.norm(b)) for a, b in zip(got, ref))
+    pts = param["nx"] * param["ny"] * 3
+    return {
+        "metric": "sw_gradients_per_sec",
+        "value": round(1.0 / grad_s, 4),
+        "unit": "gradients/s",
+        "workload": f"3-layer shallow water {param['nx']}x{param['ny']}, dx 8 km, dt 120 s, "
+                    f"{args.sw_days:g}-day window ({n_steps} steps), FP64",
+        "gradient_s": round(grad_s, 3),
+        "forward_steps_per_s": round(n_steps / fwd_s, 1),
+        "forward_window_s": round(fwd_s, 3),
+        "cell_steps_per_s": round(pts * n_steps / fwd_s, 0),
+        "finite": finite,
+        "oracle_rel_err": float(f"{err:.3e}"),
+        "oracle_steps": k,
+        "cpu_baseline": {"value": round(k / cpu_s, 3), "unit": "forward steps/s", "cores": 1, "kind": "port",
+                         "sample": f"{k} forward steps of the same grid with the NumPy oracle"},
+        "gpu_over_cpu_forward": round((n_steps / fwd_s) / (k / cpu_s), 1),
+    }
+
+
 def merge(args):
     """The merge of subwindow analyses onto the target grid
     (compute_weights_map / merge_output_date, run_assimilation.py:351-607):
@@ -1088,6 +1188,9 @@ def main():
         return
     if args.merge:
         print(json.dumps(merge(args)), flush=True)
+        return
+    if args.sw:
+        print(json.dumps(sw_model(args)), flush=True)
         return
     if args.gemm_only:
         from vardyn_amd.qg import QgPlan

@@ -4,9 +4,10 @@
 //
 // One RHS evaluation (compute_time_derivatives, sw.py:723-750) is two
 // kernels:
-//   k_sw_diag  per (i, j) column: the vorticity on the w grid, the clamped
+//   k_sw_diag  per (i, j, layer): the vorticity on the w grid, the clamped
 //              total thickness h_tot and kinetic energy + pressure (the
-//              pressure couples the layers, so a thread walks all of them)
+//              pressure couples the layers: a thread reads the column's
+//              thicknesses)
 //   k_sw_tend  per (i, j, layer): the thickness tendency (the divergence of
 //              the two upwind-reconstructed fluxes of h_tot), the u and v
 //              tendencies (vortex force from the reconstructed vorticity,
@@ -22,9 +23,9 @@
 //                     the partial derivatives of the flux with respect to its
 //                     six stencil values (6-component dual numbers through the
 //                     same reconstruction code) and its velocity
-//   k_sw_adj_gather1  per w point: the vorticity cotangent; per h cell: the
-//                     h_tot, kinetic-energy and pressure cotangents (the
-//                     pressure's layer sums in reverse)
+//   k_sw_adj_gather1  per w point and layer: the vorticity cotangent; per h
+//                     cell and layer: the h_tot, kinetic-energy and pressure
+//                     cotangents (the pressure's layer sums in reverse)
 //   k_sw_adj_gather2  per u / v point and h cell: the state cotangents
 // and the RK3 stages run in reverse with the stage states recomputed from the
 // step's start state.  Steps are reversed segment by segment: the start
@@ -111,35 +112,42 @@ __device__ __forceinline__ void w_sten_x(const SwGrid& g, Fld om, int l, int i, 
   }
 }
 
+// Diagnostics of layer blockIdx.z at (i, j): vorticity on the w grid, h_tot
+// and kinetic energy + pressure on the h grid.  The pressure of layer l
+// needs the thicknesses of every layer at the column; a thread reads them
+// all (the same sums, in the same order, as one column walk).
 template <typename S>
 __global__ __launch_bounds__(256) void k_sw_diag(SwGrid g, St x, FldOut om, FldOut ht, FldOut kep) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y * blockDim.y + threadIdx.y;
+  const int l = blockIdx.z;
   if (i > g.nx || j > g.ny) return;
-  for (int l = 0; l < g.nl; ++l) Ld<S>::put(om.v, om.t, iw_(g, l, i, j), omega_at<S>(g, x.u, x.v, l, i, j));
+  Ld<S>::put(om.v, om.t, iw_(g, l, i, j), omega_at<S>(g, x.u, x.v, l, i, j));
   if (i >= g.nx || j >= g.ny) return;
   // eta = reverse cumsum of h / area over the layers, p = cumsum of g' eta
   // (sw.py:686-687, reverse_cumsum :46-54)
-  S xs[8];
   S tot = Cst<S>::make(0.0);
-  for (int l = 0; l < g.nl; ++l) {
-    xs[l] = Ld<S>::at(x.h.v, x.h.t, ih_(g, l, i, j)) / g.area;
-    tot = l == 0 ? xs[0] : tot + xs[l];
+  for (int k = 0; k < g.nl; ++k) {
+    const S xk = Ld<S>::at(x.h.v, x.h.t, ih_(g, k, i, j)) / g.area;
+    tot = k == 0 ? xk : tot + xk;
   }
-  S cum = Cst<S>::make(0.0), p = Cst<S>::make(0.0);
+  S cum = Cst<S>::make(0.0), p = Cst<S>::make(0.0), xl = Cst<S>::make(0.0);
+  for (int k = 0; k <= l; ++k) {
+    const S xk = Ld<S>::at(x.h.v, x.h.t, ih_(g, k, i, j)) / g.area;
+    cum = k == 0 ? xk : cum + xk;
+    const S eta = (xk + tot) - cum;
+    p = k == 0 ? g.gp[0] * eta : p + g.gp[k] * eta;
+    xl = xk;
+  }
+  (void)xl;
   const double mh = g.mask_h[i * g.ny + j];
   const double idx2 = g.dx * g.dx, idy2 = g.dy * g.dy;
-  for (int l = 0; l < g.nl; ++l) {
-    cum = l == 0 ? xs[0] : cum + xs[l];
-    const S eta = (xs[l] + tot) - cum;
-    p = l == 0 ? g.gp[0] * eta : p + g.gp[l] * eta;
-    // comp_ke (finite_diff.py:26-30): 0.25 (u U + u U + v V + v V)
-    const S ua = Ld<S>::at(x.u.v, x.u.t, iu_(g, l, i + 1, j)), ub = Ld<S>::at(x.u.v, x.u.t, iu_(g, l, i, j));
-    const S va = Ld<S>::at(x.v.v, x.v.t, iv_(g, l, i, j + 1)), vb = Ld<S>::at(x.v.v, x.v.t, iv_(g, l, i, j));
-    const S ke = 0.25 * (((ua * (ua / idx2) + ub * (ub / idx2)) + va * (va / idy2)) + vb * (vb / idy2));
-    Ld<S>::put(kep.v, kep.t, ih_(g, l, i, j), ke * mh + p);
-    const S hh = Ld<S>::at(x.h.v, x.h.t, ih_(g, l, i, j));
-    Ld<S>::put(ht.v, ht.t, ih_(g, l, i, j), smooth_clamp(g.href[l] + hh, g.h_min_area, g.sharp));
-  }
+  // comp_ke (finite_diff.py:26-30): 0.25 (u U + u U + v V + v V)
+  const S ua = Ld<S>::at(x.u.v, x.u.t, iu_(g, l, i + 1, j)), ub = Ld<S>::at(x.u.v, x.u.t, iu_(g, l, i, j));
+  const S va = Ld<S>::at(x.v.v, x.v.t, iv_(g, l, i, j + 1)), vb = Ld<S>::at(x.v.v, x.v.t, iv_(g, l, i, j));
+  const S ke = 0.25 * (((ua * (ua / idx2) + ub * (ub / idx2)) + va * (va / idy2)) + vb * (vb / idy2));
+  Ld<S>::put(kep.v, kep.t, ih_(g, l, i, j), ke * mh + p);
+  const S hh = Ld<S>::at(x.h.v, x.h.t, ih_(g, l, i, j));
+  Ld<S>::put(ht.v, ht.t, ih_(g, l, i, j), smooth_clamp(g.href[l] + hh, g.h_min_area, g.sharp));
 }
 
 // RK3-SSP stage STAGE at state x (sw.py:834-857): k = F(x), then
@@ -354,12 +362,12 @@ __global__ __launch_bounds__(256) void k_sw_adj_face(SwGrid g, const double* xu,
 __global__ __launch_bounds__(256) void k_sw_adj_gather1(SwGrid g, const double* xh, const double* lu,
                                                         const double* lv, AdjWs ws) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y * blockDim.y + threadIdx.y;
-  const int nl = g.nl, W = g.ny + 1;
+  const int l = blockIdx.z, nl = g.nl, W = g.ny + 1;
   if (i > g.nx || j > g.ny) return;
   const int w = i * W + j;
   // vorticity cotangent at w point (i, j): the u points (i, j+2-k) of the flux
   // along y and the v points (i+2-k, j) of the flux along x
-  for (int l = 0; l < nl; ++l) {
+  {
     double a = 0.0;
     for (int k = 0; k < 6; ++k) {
       const int jj = j + 2 - k;
@@ -372,38 +380,38 @@ __global__ __launch_bounds__(256) void k_sw_adj_gather1(SwGrid g, const double* 
     ws.pl(kAom, l, nl)[w] = a;
   }
   if (i >= g.nx || j >= g.ny) return;
-  // h cell (i, j): h_tot through the faces (i+3-k, j) and (i, j+3-k),
-  // kinetic energy + pressure through the u points i, i+1 and v points j, j+1
-  double ap[8], ahp[8];
-  const double mh = g.mask_h[i * g.ny + j];
-  for (int l = 0; l < nl; ++l) {
-    double a = 0.0;
-    for (int k = 0; k < 6; ++k) {
-      const int ii = i + 3 - k;
-      if (ii >= 0 && ii <= g.nx) a += ws.pl(kGx + k, l, nl)[ii * W + j];
-    }
-    for (int k = 0; k < 6; ++k) {
-      const int jj = j + 3 - k;
-      if (jj >= 0 && jj <= g.ny) a += ws.pl(kGy + k, l, nl)[i * W + jj];
-    }
-    const double hh = xh[ih_(g, l, i, j)];
-    ahp[l] = a * smooth_clamp_deriv(g.href[l] + hh, g.h_min_area, g.sharp);
-    const double akep = -mu_u(g, lu, l, i, j) + mu_u(g, lu, l, i + 1, j) - mu_v(g, lv, l, i, j) +
-                        mu_v(g, lv, l, i, j + 1);
-    ws.pl(kAKE, l, nl)[w] = akep * mh;
-    ap[l] = akep;
+  // h cell (i, j) of layer l: h_tot through the faces (i+3-k, j) and
+  // (i, j+3-k), kinetic energy + pressure through the u points i, i+1 and
+  // v points j, j+1 of every layer (the pressure of layer k depends on the
+  // thickness of layer l for all k)
+  double a = 0.0;
+  for (int k = 0; k < 6; ++k) {
+    const int ii = i + 3 - k;
+    if (ii >= 0 && ii <= g.nx) a += ws.pl(kGx + k, l, nl)[ii * W + j];
   }
-  // p_l = sum_{k <= l} g'_k eta_k, eta_k = sum_{m >= k} h_m / area
+  for (int k = 0; k < 6; ++k) {
+    const int jj = j + 3 - k;
+    if (jj >= 0 && jj <= g.ny) a += ws.pl(kGy + k, l, nl)[i * W + jj];
+  }
+  const double mh = g.mask_h[i * g.ny + j];
+  const double hh = xh[ih_(g, l, i, j)];
+  const double ahp = a * smooth_clamp_deriv(g.href[l] + hh, g.h_min_area, g.sharp);
+  auto akep = [&](int k) {
+    return -mu_u(g, lu, k, i, j) + mu_u(g, lu, k, i + 1, j) - mu_v(g, lv, k, i, j) + mu_v(g, lv, k, i, j + 1);
+  };
+  ws.pl(kAKE, l, nl)[w] = akep(l) * mh;
+  // p_k = sum_{m <= k} g'_m eta_m, eta_m = sum_{n >= m} h_n / area:
+  // d/dh_l = (1/area) sum_{m <= l} g'_m sum_{k >= m} akep_k
+  double ap[8];
+  for (int k = 0; k < nl; ++k) ap[k] = akep(k);
   double rs = 0.0, cum = 0.0;
   double aeta[8];
   for (int k = nl - 1; k >= 0; --k) {
     rs += ap[k];
     aeta[k] = g.gp[k] * rs;
   }
-  for (int m = 0; m < nl; ++m) {
-    cum += aeta[m];
-    ws.pl(kAhp, m, nl)[w] = ahp[m] + cum / g.area;
-  }
+  for (int m = 0; m <= l; ++m) cum += aeta[m];
+  ws.pl(kAhp, l, nl)[w] = ahp + cum / g.area;
 }
 
 // lam_out (u, v, h) += F'(x)^T mu, given gather1's planes.
@@ -638,7 +646,7 @@ int sw_rk3(vd_sw_model* m, double* xv, double* xt, hipStream_t s) {
   for (int stg = 0; stg < 3; ++stg) {
     const St x = st_of(m, bv[stg], bt[stg]);
     const StOut xn = sto_of(m, bv[(stg + 1) % 3], bt[(stg + 1) % 3]);
-    hipLaunchKernelGGL(k_sw_diag<S>, grd(g, 1), blk(), 0, s, g, x, omo, hto, kepo);
+    hipLaunchKernelGGL(k_sw_diag<S>, grd(g, g.nl), blk(), 0, s, g, x, omo, hto, kepo);
     if (stg == 0) hipLaunchKernelGGL((k_sw_tend<S, 0>), grd(g, g.nl), blk(), 0, s, g, x, omi, hti, kepi, k0, k1, k0o, xn);
     else if (stg == 1)
       hipLaunchKernelGGL((k_sw_tend<S, 1>), grd(g, g.nl), blk(), 0, s, g, x, omi, hti, kepi, k0, k1, k1o, xn);
@@ -669,12 +677,12 @@ void scale3(const vd_sw_model* m, const double* iu, const double* iv, const doub
 int sw_rhs_adj(vd_sw_model* m, const double* x, const double* mu, hipStream_t s) {
   const SwGrid& g = m->g;
   const St xs = st_of(m, x, nullptr);
-  hipLaunchKernelGGL(k_sw_diag<double>, grd(g, 1), blk(), 0, s, g, xs, FldOut{m->om, nullptr},
+  hipLaunchKernelGGL(k_sw_diag<double>, grd(g, g.nl), blk(), 0, s, g, xs, FldOut{m->om, nullptr},
                      FldOut{m->ht, nullptr}, FldOut{m->kep, nullptr});
   AdjWs ws{m->ws, (int)m->nwl};
   const double *mu_u = mu, *mu_v = mu + m->nu, *mu_h = mu + m->nu + m->nv;
   hipLaunchKernelGGL(k_sw_adj_face, grd(g, g.nl), blk(), 0, s, g, x, x + m->nu, m->om, m->ht, mu_u, mu_v, mu_h, ws);
-  hipLaunchKernelGGL(k_sw_adj_gather1, grd(g, 1), blk(), 0, s, g, x + m->nu + m->nv, mu_u, mu_v, ws);
+  hipLaunchKernelGGL(k_sw_adj_gather1, grd(g, g.nl), blk(), 0, s, g, x + m->nu + m->nv, mu_u, mu_v, ws);
   hipLaunchKernelGGL(k_sw_adj_gather2, grd(g, g.nl), blk(), 0, s, g, x, x + m->nu, mu_u, mu_v, ws, m->L,
                      m->L + m->nu, m->L + m->nu + m->nv);
   VD_HIP_TRY(hipGetLastError());
@@ -698,11 +706,11 @@ int sw_rk3_adj(vd_sw_model* m, const double* x0, hipStream_t s) {
     const FldOut omo{m->om, nullptr}, hto{m->ht, nullptr}, kepo{m->kep, nullptr};
     const Fld omi{m->om, nullptr}, hti{m->ht, nullptr}, kepi{m->kep, nullptr};
     St x = st_of(m, x0, nullptr);
-    hipLaunchKernelGGL(k_sw_diag<double>, grd(g, 1), blk(), 0, s, g, x, omo, hto, kepo);
+    hipLaunchKernelGGL(k_sw_diag<double>, grd(g, g.nl), blk(), 0, s, g, x, omo, hto, kepo);
     hipLaunchKernelGGL((k_sw_tend<double, 0>), grd(g, g.nl), blk(), 0, s, g, x, omi, hti, kepi, k0, k1, k0o,
                        sto_of(m, m->SB, nullptr));
     x = st_of(m, m->SB, nullptr);
-    hipLaunchKernelGGL(k_sw_diag<double>, grd(g, 1), blk(), 0, s, g, x, omo, hto, kepo);
+    hipLaunchKernelGGL(k_sw_diag<double>, grd(g, g.nl), blk(), 0, s, g, x, omo, hto, kepo);
     hipLaunchKernelGGL((k_sw_tend<double, 1>), grd(g, g.nl), blk(), 0, s, g, x, omi, hti, kepi, k0, k1, k1o,
                        sto_of(m, m->SC, nullptr));
   }
@@ -811,7 +819,7 @@ int vd_sw_rhs(vd_sw_model* m, const double* u, const double* v, const double* h,
   VD_HIP_TRY(hipMemcpyAsync(m->SB + m->nu, v, m->nv * sizeof(double), hipMemcpyDeviceToDevice, s));
   VD_HIP_TRY(hipMemcpyAsync(m->SB + m->nu + m->nv, h, m->nh * sizeof(double), hipMemcpyDeviceToDevice, s));
   const St x = st_of(m, m->SB, nullptr);
-  hipLaunchKernelGGL(k_sw_diag<double>, grd(g, 1), blk(), 0, s, g, x, FldOut{m->om, nullptr},
+  hipLaunchKernelGGL(k_sw_diag<double>, grd(g, g.nl), blk(), 0, s, g, x, FldOut{m->om, nullptr},
                      FldOut{m->ht, nullptr}, FldOut{m->kep, nullptr});
   // stage 0 with the tendency stored in K0 (the updated state goes to SC)
   hipLaunchKernelGGL((k_sw_tend<double, 0>), grd(g, g.nl), blk(), 0, s, g, x, Fld{m->om, nullptr},
