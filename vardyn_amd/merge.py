@@ -70,6 +70,9 @@ class SubwindowMerge:
             raise VardynError("the target grid needs 2-D lon and lat of one shape")
         self.ny, self.nx = lon.shape
         self.device = torch.device(device)
+        if self.device.type == "cuda" and self.device.index is None:
+            # tensors report their device with its index: compare against that
+            self.device = torch.device("cuda", torch.cuda.current_device())
         axes = [_axes(lo, la) for lo, la in subwindows]
         if not axes:
             raise VardynError("no subwindow")
