@@ -313,6 +313,60 @@ int vd_bm_operg_adj(vd_bm_basis* basis, int T, const double* gt, const double* a
                     double* adX, void* work, void* stream);
 
 /* ---------------------------------------------------------------------------
+ * 3-D wavelet basis (Basis_wavelet3d, mapping/src/basis.py:2640-2985): fields
+ * from wavelet coefficients (operg, basis.py:2936-2960) and the transpose
+ * (operg_transpose, basis.py:2962-2985) for a batch of T basis times, as
+ * dense FP64 MFMA products over 16 x 16-point tiles.  The host builds the
+ * tables (vardyn_amd/basis_wavelet3d.py): the grid's lon and lat (ny x nx,
+ * any 2-D grid), per space frequency DX (km), the wave vectors kx, ky
+ * (nf x ntheta), its first coefficient, first family and family count; per
+ * family (a space centre with its 2 ntheta waves; families frequency-major)
+ * its frequency, lon, lat and cos(lat); per tile (row-major over the
+ * ceil(ny/16) x ceil(nx/16) tiles) a CSR of slots (tslot), each slot a family
+ * whose support meets the tile (ascending) with the 256-bit membership of
+ * the tile's points (point p = 16 r + c, bit p % 64 of word p / 64; the
+ * reference's box test, masked points excluded).  Coefficients are laid out
+ * as in the reference: space frequency, time block (time frequency, time
+ * centre), space centre, direction, cos/sin.  The time factors gt of a call
+ * are T x nf x ntb (ntb: time blocks per frequency), row-major.
+ * ------------------------------------------------------------------------- */
+typedef struct vd_w3_basis vd_w3_basis;
+typedef struct vd_w3_params {
+  int ny, nx, nf, ntheta, ntb, nfam;
+  const double* lon;
+  const double* lat;
+  const double* dxs;
+  const double* kx;
+  const double* ky;
+  const int32_t* fam_f;
+  const double* fam_lon;
+  const double* fam_lat;
+  const double* fam_cos;
+  const int64_t* fbase;
+  const int32_t* ffam0;
+  const int32_t* fnfam;
+  const int64_t* tslot;
+  const int32_t* sfam;
+  const uint64_t* sbits;
+} vd_w3_params;
+
+/* Upload the tables and build the tile blocks of the space matrix on the
+ * current device (synchronises once). */
+int vd_w3_create(const vd_w3_params* params, vd_w3_basis** basis);
+int vd_w3_destroy(vd_w3_basis* basis);
+/* Number of coefficients, of tile entries (the stored block rows: the tile
+ * blocks hold 256 doubles per entry) and of waves. */
+int vd_w3_info(const vd_w3_basis* basis, long long* nbasis, long long* nentries, long long* nwave);
+/* Device workspace of one call (any T: the calls run 16 times per pass). */
+int vd_w3_workspace_size(const vd_w3_basis* basis, size_t* bytes);
+/* out[t] (T fields of ny x nx) = operg(time t, X); device pointers. */
+int vd_w3_operg(vd_w3_basis* basis, int T, const double* gt, const double* X, double* out, void* work,
+                void* stream);
+/* adX += sum over the T times of operg_transpose(time t, ad[t]). */
+int vd_w3_operg_adj(vd_w3_basis* basis, int T, const double* gt, const double* ad, double* adX,
+                    void* work, void* stream);
+
+/* ---------------------------------------------------------------------------
  * Merge of subwindow analyses onto the target grid.  Replaces the numerical
  * core of compute_weights_map (mapping/src/run_assimilation.py:351-412, the
  * regular-grid branch of _build_interpolator :414-446), merge_output_date
@@ -456,8 +510,9 @@ int vd_axpby(int precision, int64_t n, double a, const void* x, double b, void* 
  * scalar dx / dy, any binary ocean mask, slip coefficient, linear bottom
  * drag on the last layer, the smooth thickness clamp.  Viscosity and
  * thickness diffusion (sw.py:512-573) are rejected with VD_ERR_UNSUPPORTED;
- * wind stress, sponge, open boundaries, the barotropic filter and external
- * forcing have no parameter (the reference's defaults: none of them).
+ * wind stress, sponge, open boundaries and the barotropic filter have no
+ * parameter (the reference's defaults: none of them).  External forcing
+ * (Fu, Fv, Fh, direct momentum mode) is an argument of the step calls.
  * ------------------------------------------------------------------------- */
 typedef struct vd_sw_model vd_sw_model;
 typedef struct vd_sw_params {
@@ -482,19 +537,30 @@ int vd_sw_info(const vd_sw_model* model, long long* nu, long long* nv, long long
  * compute_time_derivatives (sw.py:723-750); device pointers. */
 int vd_sw_rhs(vd_sw_model* model, const double* u, const double* v, const double* h, double* du,
               double* dv, double* dh, void* stream);
-/* (u1, v1, h1) = nstep RK3-SSP steps from (u0, v0, h0); device pointers. */
-int vd_sw_step(vd_sw_model* model, const double* u0, const double* v0, const double* h0, double* u1,
-               double* v1, double* h1, int nstep, void* stream);
-/* (du1, dv1, dh1) = tangent of step at (u0, v0, h0) applied to (du0, dv0, dh0). */
-int vd_sw_step_tgl(vd_sw_model* model, const double* u0, const double* v0, const double* h0,
-                   const double* du0, const double* dv0, const double* dh0, double* du1, double* dv1,
+/* (u1, v1, h1) = nstep RK3-SSP steps from (u0, v0, h0); device pointers.
+ * (Fu, Fv, Fh): external forcing in physical units per second, per layer
+ * (u, v and h shapes), added as x += dt F after every step (sw.py:752-904,
+ * the Fu / Fv / Fh arguments of SW.step); a null field is zero. */
+int vd_sw_step(vd_sw_model* model, const double* u0, const double* v0, const double* h0, const double* Fu,
+               const double* Fv, const double* Fh, double* u1, double* v1, double* h1, int nstep, void* stream);
+/* (du1, dv1, dh1) = tangent of step at (u0, v0, h0; F) applied to (du0, dv0, dh0; dF)
+ * (jax.jvp of step, sw.py:906-923, with the forcing among the differentiated
+ * inputs as in Model_qgsw.jstep_tgl, mod.py:3222-3253); null forcing fields
+ * and null forcing tangents are zero. */
+int vd_sw_step_tgl(vd_sw_model* model, const double* u0, const double* v0, const double* h0, const double* Fu,
+                   const double* Fv, const double* Fh, const double* du0, const double* dv0, const double* dh0,
+                   const double* dFu, const double* dFv, const double* dFh, double* du1, double* dv1,
                    double* dh1, int nstep, void* stream);
-/* (au0, av0, ah0) = adjoint of step at (u0, v0, h0) applied to (au1, av1, ah1).
- * The trajectory is recomputed in segments of about sqrt(nstep) steps; the
- * model keeps a device tape of (segments + segment length) states. */
-int vd_sw_step_adj(vd_sw_model* model, const double* u0, const double* v0, const double* h0,
-                   const double* au1, const double* av1, const double* ah1, double* au0, double* av0,
-                   double* ah0, int nstep, void* stream);
+/* (au0, av0, ah0) = adjoint of step at (u0, v0, h0; F) applied to (au1, av1, ah1),
+ * and (aFu, aFv, aFh) = the forcing cotangents (overwritten; null outputs are
+ * not computed) -- jax.vjp of step (sw.py:925-937) as Model_qgsw.jstep_adj
+ * uses it (mod.py:3255-3290).  The trajectory is recomputed in segments of
+ * about sqrt(nstep) steps; the model keeps a device tape of (segments +
+ * segment length) states, allocated on the first call that needs it. */
+int vd_sw_step_adj(vd_sw_model* model, const double* u0, const double* v0, const double* h0, const double* Fu,
+                   const double* Fv, const double* Fh, const double* au1, const double* av1, const double* ah1,
+                   double* au0, double* av0, double* ah0, double* aFu, double* aFv, double* aFh, int nstep,
+                   void* stream);
 /* Precision conversion: dst[i] = (dst type) src[i]; NaN -> 0 if nan_to_zero. */
 int vd_convert(int src_precision, const void* src, int dst_precision, void* dst, int64_t n,
                int nan_to_zero, void* stream);

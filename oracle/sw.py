@@ -21,15 +21,16 @@ Follows ``mapping/models/model_qgsw/`` of datlas-ocean/VarDyn:
   boundary classes), ``compute_diagnostic_variables`` (:673-698),
   ``advection_h`` (:465-479), ``advection_momentum`` (:481-510, vortex force,
   Coriolis, kinetic energy + pressure gradient, bottom drag), RK3-SSP
-  ``step`` (:752-904) with ``set_input_uvh`` / ``get_physical_uvh``
+  ``step`` (:752-904, with the external forcing Fu / Fv / Fh added after
+  every step, :875-877) with ``set_input_uvh`` / ``get_physical_uvh``
   (:423-449)
 
 Scope (the options of the reference's default configuration,
 config_default.py:500-560): per-layer constant depths ``H``, scalar ``dx`` /
 ``dy``, any binary ocean mask, any ``f`` on the vorticity grid, slip
 coefficient, bottom drag.  Not restated: viscosity / thickness diffusion,
-wind stress, sponge, open boundaries, the barotropic filter and external
-forcing (all zero by default there; the GPU model rejects them).
+wind stress, sponge, open boundaries and the barotropic filter (all zero
+by default there; the GPU model rejects them).
 
 The reference quirks are kept: the vorticity flux along y on the u grid
 blends the 6-point reconstruction with ``u_sten_wy_gt4`` (so where the
@@ -301,17 +302,29 @@ class SwOracle:
         k2 = self.rhs(u2, v2, h2)
         return tuple(a + (dt / 12.0) * (8.0 * b - c - d) for a, b, c, d in zip((u2, v2, h2), k2, k1, k0))
 
-    def step(self, u0, v0, h0, nstep=1):
-        """step (sw.py:752-904), physical variables in and out."""
+    def step(self, u0, v0, h0, nstep=1, F=None):
+        """step (sw.py:752-904), physical variables in and out.  ``F`` =
+        (Fu, Fv, Fh): external forcing per layer in physical units per
+        second, scaled like the state and added as x += dt F after every
+        RK3 step (sw.py:791-796, 875-877); None: no forcing."""
         s = self.to_scaled(u0, v0, h0)
+        Fs = None if F is None else self.to_scaled(*F)
         for _ in range(nstep):
             s = self.rk3(*s)
+            if Fs is not None:
+                s = tuple(a + self.dt * f for a, f in zip(s, Fs))
         return self.to_physical(*s)
 
-    def step_tgl(self, u0, v0, h0, du0, dv0, dh0, nstep=1, eps=1e-30):
+    def step_tgl(self, u0, v0, h0, du0, dv0, dh0, nstep=1, eps=1e-30, F=None, dF=None):
         """Tangent of ``step`` by the complex step (the reference takes
-        ``jax.jvp`` of ``step``, sw.py:906-923)."""
+        ``jax.jvp`` of ``step``, sw.py:906-923; with the forcing among the
+        differentiated inputs as Model_qgsw.jstep_tgl, mod.py:3222-3253)."""
         z = [np.asarray(a, dtype=complex) + 1j * eps * np.asarray(d) for a, d in
              ((u0, du0), (v0, dv0), (h0, dh0))]
-        out = self.step(*z, nstep=nstep)
+        Fz = None
+        if F is not None or dF is not None:
+            F = F if F is not None else tuple(np.zeros_like(a) for a in (u0, v0, h0))
+            dF = dF if dF is not None else tuple(np.zeros_like(a) for a in F)
+            Fz = tuple(np.asarray(a, dtype=complex) + 1j * eps * np.asarray(d) for a, d in zip(F, dF))
+        out = self.step(*z, nstep=nstep, F=Fz)
         return tuple(np.imag(o) / eps for o in out)

@@ -275,3 +275,59 @@ def test_mirror_symmetry_y():
     np.testing.assert_allclose(b[0][:, :, ::-1], a[0], rtol=0, atol=1e-12 * np.abs(a[0]).max())
     np.testing.assert_allclose(-b[1][:, :, ::-1], a[1], rtol=0, atol=1e-12 * np.abs(a[1]).max())
     np.testing.assert_allclose(b[2][:, :, ::-1], a[2], rtol=0, atol=1e-12 * np.abs(a[2]).max())
+
+
+def _basin(nx=12, ny=10, nl=3, fplane=False):
+    dx = dy = 8e3
+    y = (np.arange(ny + 1) - ny / 2) * dy
+    f = np.broadcast_to(1e-4 + (0.0 if fplane else 1.6e-11) * y, (nx + 1, ny + 1)).copy()
+    H = np.array([500.0, 1000.0, 3000.0][:nl])
+    gp = np.array([0.02, 0.02, 0.01][:nl])
+    return SwOracle(nx, ny, dx, dy, H, gp, f, 120.0)
+
+
+def test_forcing_uniform_thickness_is_exact():
+    """A uniform thickness forcing of layer 0 from rest has no pressure
+    gradient and no flux: after n steps h_0 = n dt F and nothing else moves
+    (the forcing is x += dt F after every RK3 step, sw.py:875-877)."""
+    o = _basin()
+    nl, nx, ny = o.nl, o.nx, o.ny
+    z = (np.zeros((nl, nx + 1, ny)), np.zeros((nl, nx, ny + 1)), np.zeros((nl, nx, ny)))
+    Fh = np.zeros((nl, nx, ny)); Fh[0] = 3e-6
+    for n in (1, 4):
+        u, v, h = o.step(*z, nstep=n, F=(np.zeros_like(z[0]), np.zeros_like(z[1]), Fh))
+        assert np.all(u == 0) and np.all(v == 0) and np.all(h[1:] == 0)
+        np.testing.assert_allclose(h[0], n * o.dt * 3e-6, rtol=1e-13)
+
+
+def test_forcing_one_step_from_rest_is_dt_F_masked():
+    """From rest the RK3 stages have zero tendencies, so one step gives
+    dt F on the ocean points of each grid (u and v zero on the walls)."""
+    o = _basin(nl=2)
+    rng = np.random.default_rng(4)
+    nl, nx, ny = o.nl, o.nx, o.ny
+    F = (rng.standard_normal((nl, nx + 1, ny)), rng.standard_normal((nl, nx, ny + 1)),
+         rng.standard_normal((nl, nx, ny)))
+    z = tuple(np.zeros_like(a) for a in F)
+    out = o.step(*z, nstep=1, F=F)
+    for got, Fi, m in zip(out, F, (o.m["u"], o.m["v"], o.m["h"])):
+        np.testing.assert_allclose(got, o.dt * Fi * m, rtol=1e-13, atol=1e-30)
+
+
+def test_forcing_tangent_equals_finite_differences():
+    o = _basin(nl=2)
+    rng = np.random.default_rng(8)
+    nl, nx, ny = o.nl, o.nx, o.ny
+    m = o.m
+    x = [0.05 * rng.standard_normal((nl, nx + 1, ny)) * m["u"],
+         0.05 * rng.standard_normal((nl, nx, ny + 1)) * m["v"], 0.3 * rng.standard_normal((nl, nx, ny)) * m["h"]]
+    F = [1e-6 * rng.standard_normal(a.shape) for a in x]
+    dF = [1e-6 * rng.standard_normal(a.shape) for a in x]
+    zero = [np.zeros_like(a) for a in x]
+    tl = o.step_tgl(*x, *zero, nstep=3, F=F, dF=dF)
+    eps = 1e-3
+    p = o.step(*x, nstep=3, F=[a + eps * d for a, d in zip(F, dF)])
+    q = o.step(*x, nstep=3, F=[a - eps * d for a, d in zip(F, dF)])
+    for t, a, b in zip(tl, p, q):
+        fd = (a - b) / (2 * eps)
+        assert np.linalg.norm(t - fd) <= 1e-6 * np.linalg.norm(fd)

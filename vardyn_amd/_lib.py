@@ -102,13 +102,19 @@ SIGNATURES = {
     "vd_bm_workspace_size": (_I, [_VP, _I, C.POINTER(_SZ)]),
     "vd_bm_operg": (_I, [_VP, _I, _VP, _VP, _VP, _VP, _VP]),
     "vd_bm_operg_adj": (_I, [_VP, _I, _VP, _VP, _VP, _VP, _VP]),
+    "vd_w3_create": (_I, [_VP, _VP]),
+    "vd_w3_destroy": (_I, [_VP]),
+    "vd_w3_info": (_I, [_VP, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
+    "vd_w3_workspace_size": (_I, [_VP, C.POINTER(_SZ)]),
+    "vd_w3_operg": (_I, [_VP, _I, _VP, _VP, _VP, _VP, _VP]),
+    "vd_w3_operg_adj": (_I, [_VP, _I, _VP, _VP, _VP, _VP, _VP]),
     "vd_sw_create": (_I, [C.POINTER(SwParams), C.POINTER(_VP)]),
     "vd_sw_destroy": (_I, [_VP]),
     "vd_sw_info": (_I, [_VP, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
     "vd_sw_rhs": (_I, [_VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP]),
-    "vd_sw_step": (_I, [_VP, _VP, _VP, _VP, _VP, _VP, _VP, _I, _VP]),
-    "vd_sw_step_tgl": (_I, [_VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _I, _VP]),
-    "vd_sw_step_adj": (_I, [_VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _I, _VP]),
+    "vd_sw_step": (_I, [_VP] * 10 + [_I, _VP]),
+    "vd_sw_step_tgl": (_I, [_VP] * 16 + [_I, _VP]),
+    "vd_sw_step_adj": (_I, [_VP] * 16 + [_I, _VP]),
     "vd_merge_create": (_I, [_VP, _VP]),
     "vd_merge_destroy": (_I, [_VP]),
     "vd_merge_info": (_I, [_VP, C.POINTER(C.c_longlong)]),

@@ -7,6 +7,30 @@
 
 namespace vd {
 
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
+// 16x16x4 MFMA tiles.  Inputs, both precisions: lane l supplies A[l%16][l/16]
+// and B[l/16][l%16].  Result register r of lane l holds column l%16 and row
+// out_row(l, r), which differs between the two instructions (measured on
+// gfx950: the f64 form interleaves the four registers across row groups).
+template <typename T> struct Mfma;
+template <> struct Mfma<float> {
+  typedef floatx4 acc_t;
+  static __device__ __forceinline__ acc_t run(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int out_row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+template <> struct Mfma<double> {
+  typedef doublex4 acc_t;
+  static __device__ __forceinline__ acc_t run(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int out_row(int lane, int r) { return 4 * r + (lane >> 4); }
+};
+
+
 // ---------------------------------------------------------------------------
 // Sine-matrix GEMMs of the Helmholtz inversion.
 //

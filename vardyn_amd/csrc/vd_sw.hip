@@ -230,7 +230,7 @@ __global__ void k_sw_scale_field(const double* in, double* out, const double* ma
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
   const double m = mask[e % plane];
-  const double x = in[e];
+  const double x = in ? in[e] : 0.0;  // no input: a zero field
   out[e] = mode == 0 ? (x * m) * d : (mode == 1 ? x / d : (x * d) * m);
 }
 
@@ -589,6 +589,7 @@ struct vd_sw_model {
   double *om = nullptr, *ht = nullptr, *kep = nullptr, *tom = nullptr, *tht = nullptr, *tkep = nullptr;
   double* ws = nullptr;
   double *L = nullptr, *LK0 = nullptr, *LK1 = nullptr, *LK2 = nullptr;
+  double *F = nullptr, *dF = nullptr, *aF = nullptr;  // scaled forcing, its tangent and cotangent
   double* tape = nullptr;
   size_t tape_states = 0;
   ~vd_sw_model() {
@@ -668,6 +669,7 @@ void scale3(const vd_sw_model* m, const double* iu, const double* iv, const doub
   const size_t cnt[3] = {m->nu, m->nv, m->nh};
   for (int f = 0; f < 3; ++f) {
     const int c = (int)cnt[f];
+    if (!outs[f]) continue;
     hipLaunchKernelGGL(k_sw_scale_field, dim3((c + 255) / 256), dim3(256), 0, s, ins[f], outs[f], masks[f], planes[f],
                        fac[f], mode, c);
   }
@@ -693,6 +695,20 @@ void comb(const vd_sw_model* m, const double* x, double* y0, double a0, double b
           double* y2, double a2, double b2, hipStream_t s) {
   const int n = (int)m->ns;
   hipLaunchKernelGGL(k_sw_comb, dim3((n + 255) / 256), dim3(256), 0, s, n, x, y0, a0, b0, y1, a1, b1, y2, a2, b2);
+}
+
+// Scaled forcing (set_input_uvh of Fu, Fv, Fh; a missing field is zero)
+// written to dst; false when all three are missing.
+bool load_forcing(const vd_sw_model* m, const double* Fu, const double* Fv, const double* Fh, double* dst,
+                  hipStream_t s) {
+  if (!Fu && !Fv && !Fh) return false;
+  scale3(m, Fu, Fv, Fh, dst, dst + m->nu, dst + m->nu + m->nv, 0, s);
+  return true;
+}
+
+// x <- x + dt F: the external forcing after each RK3 step (sw.py:875-877).
+void add_forcing(const vd_sw_model* m, double* x, const double* F, hipStream_t s) {
+  comb(m, F, x, 1.0, m->g.dt, nullptr, 0, 0, nullptr, 0, 0, s);
 }
 
 // L <- (one RK3 step at x0)^T L
@@ -789,7 +805,8 @@ int vd_sw_create(const vd_sw_params* P, vd_sw_model** out) {
       (rc = sw_alloc(m, m->nh, &m->kep)) || (rc = sw_alloc(m, nw, &m->tom)) || (rc = sw_alloc(m, m->nh, &m->tht)) ||
       (rc = sw_alloc(m, m->nh, &m->tkep)) || (rc = sw_alloc(m, (size_t)kAdjPlanes * nw, &m->ws)) ||
       (rc = sw_alloc(m, m->ns, &m->L)) || (rc = sw_alloc(m, m->ns, &m->LK0)) || (rc = sw_alloc(m, m->ns, &m->LK1)) ||
-      (rc = sw_alloc(m, m->ns, &m->LK2))) {
+      (rc = sw_alloc(m, m->ns, &m->LK2)) || (rc = sw_alloc(m, m->ns, &m->F)) || (rc = sw_alloc(m, m->ns, &m->dF)) ||
+      (rc = sw_alloc(m, m->ns, &m->aF))) {
     delete m;
     return rc;
   }
@@ -832,45 +849,55 @@ int vd_sw_rhs(vd_sw_model* m, const double* u, const double* v, const double* h,
   return VD_OK;
 }
 
-int vd_sw_step(vd_sw_model* m, const double* u0, const double* v0, const double* h0, double* u1, double* v1,
-               double* h1, int nstep, void* stream) {
+int vd_sw_step(vd_sw_model* m, const double* u0, const double* v0, const double* h0, const double* Fu,
+               const double* Fv, const double* Fh, double* u1, double* v1, double* h1, int nstep, void* stream) {
   VD_REQUIRE(m && u0 && v0 && h0 && u1 && v1 && h1, VD_ERR_INVALID_ARG, "null argument");
   VD_REQUIRE(nstep >= 0, VD_ERR_INVALID_ARG, "sw: nstep must be >= 0");
   hipStream_t s = static_cast<hipStream_t>(stream);
+  const bool forced = load_forcing(m, Fu, Fv, Fh, m->F, s);
   scale3(m, u0, v0, h0, m->SA, m->SA + m->nu, m->SA + m->nu + m->nv, 0, s);
   for (int k = 0; k < nstep; ++k) {
     int rc = sw_rk3<false>(m, m->SA, nullptr, s);
     if (rc) return rc;
+    if (forced) add_forcing(m, m->SA, m->F, s);
   }
   scale3(m, m->SA, m->SA + m->nu, m->SA + m->nu + m->nv, u1, v1, h1, 1, s);
   VD_HIP_TRY(hipGetLastError());
   return VD_OK;
 }
 
-int vd_sw_step_tgl(vd_sw_model* m, const double* u0, const double* v0, const double* h0, const double* du0,
-                   const double* dv0, const double* dh0, double* du1, double* dv1, double* dh1, int nstep,
-                   void* stream) {
+int vd_sw_step_tgl(vd_sw_model* m, const double* u0, const double* v0, const double* h0, const double* Fu,
+                   const double* Fv, const double* Fh, const double* du0, const double* dv0, const double* dh0,
+                   const double* dFu, const double* dFv, const double* dFh, double* du1, double* dv1, double* dh1,
+                   int nstep, void* stream) {
   VD_REQUIRE(m && u0 && v0 && h0 && du0 && dv0 && dh0 && du1 && dv1 && dh1, VD_ERR_INVALID_ARG, "null argument");
   VD_REQUIRE(nstep >= 0, VD_ERR_INVALID_ARG, "sw: nstep must be >= 0");
   hipStream_t s = static_cast<hipStream_t>(stream);
+  const bool forced = load_forcing(m, Fu, Fv, Fh, m->F, s);
+  const bool dforced = load_forcing(m, dFu, dFv, dFh, m->dF, s);
   scale3(m, u0, v0, h0, m->SA, m->SA + m->nu, m->SA + m->nu + m->nv, 0, s);
   scale3(m, du0, dv0, dh0, m->TA, m->TA + m->nu, m->TA + m->nu + m->nv, 0, s);
   for (int k = 0; k < nstep; ++k) {
     int rc = sw_rk3<true>(m, m->SA, m->TA, s);
     if (rc) return rc;
+    if (forced) add_forcing(m, m->SA, m->F, s);
+    if (dforced) add_forcing(m, m->TA, m->dF, s);
   }
   scale3(m, m->TA, m->TA + m->nu, m->TA + m->nu + m->nv, du1, dv1, dh1, 1, s);
   VD_HIP_TRY(hipGetLastError());
   return VD_OK;
 }
 
-int vd_sw_step_adj(vd_sw_model* m, const double* u0, const double* v0, const double* h0, const double* au1,
-                   const double* av1, const double* ah1, double* au0, double* av0, double* ah0, int nstep,
+int vd_sw_step_adj(vd_sw_model* m, const double* u0, const double* v0, const double* h0, const double* Fu,
+                   const double* Fv, const double* Fh, const double* au1, const double* av1, const double* ah1,
+                   double* au0, double* av0, double* ah0, double* aFu, double* aFv, double* aFh, int nstep,
                    void* stream) {
   VD_REQUIRE(m && u0 && v0 && h0 && au1 && av1 && ah1 && au0 && av0 && ah0, VD_ERR_INVALID_ARG, "null argument");
   VD_REQUIRE(nstep >= 0, VD_ERR_INVALID_ARG, "sw: nstep must be >= 0");
   hipStream_t s = static_cast<hipStream_t>(stream);
   const size_t ns = m->ns;
+  const bool forced = load_forcing(m, Fu, Fv, Fh, m->F, s);
+  const bool want_aF = aFu || aFv || aFh;
   // segments of about sqrt(nstep) steps: their start states, then one
   // segment's step states at a time
   const int seg = std::max(1, (int)std::ceil(std::sqrt((double)nstep)));
@@ -891,9 +918,12 @@ int vd_sw_step_adj(vd_sw_model* m, const double* u0, const double* v0, const dou
                                                 hipMemcpyDeviceToDevice, s));
     int rc = sw_rk3<false>(m, m->SA, nullptr, s);
     if (rc) return rc;
+    if (forced) add_forcing(m, m->SA, m->F, s);
   }
   // cotangent of the scaled final state: get_physical_uvh divides by dx, dy, dx dy
   scale3(m, au1, av1, ah1, m->L, m->L + m->nu, m->L + m->nu + m->nv, 1, s);
+  // the forcing cotangent starts at zero (a missing input scales to zero)
+  if (want_aF) scale3(m, nullptr, nullptr, nullptr, m->aF, m->aF + m->nu, m->aF + m->nu + m->nv, 0, s);
   for (int q = nseg - 1; q >= 0; --q) {
     const int s0 = q * seg, s1 = std::min(nstep, s0 + seg);
     VD_HIP_TRY(hipMemcpyAsync(m->SA, ck + (size_t)q * ns, ns * sizeof(double), hipMemcpyDeviceToDevice, s));
@@ -902,15 +932,19 @@ int vd_sw_step_adj(vd_sw_model* m, const double* u0, const double* v0, const dou
       if (k + 1 < s1) {
         int rc = sw_rk3<false>(m, m->SA, nullptr, s);
         if (rc) return rc;
+        if (forced) add_forcing(m, m->SA, m->F, s);
       }
     }
     for (int k = s1 - 1; k >= s0; --k) {
+      // x_{k+1} = RK3(x_k) + dt F: F collects dt L_{k+1} before L goes back through the step
+      if (want_aF) add_forcing(m, m->aF, m->L, s);
       int rc = sw_rk3_adj(m, st + (size_t)(k - s0) * ns, s);
       if (rc) return rc;
     }
   }
   // set_input_uvh: u_scaled = (u mask_u) dx
   scale3(m, m->L, m->L + m->nu, m->L + m->nu + m->nv, au0, av0, ah0, 2, s);
+  if (want_aF) scale3(m, m->aF, m->aF + m->nu, m->aF + m->nu + m->nv, aFu, aFv, aFh, 2, s);
   VD_HIP_TRY(hipGetLastError());
   return VD_OK;
 }

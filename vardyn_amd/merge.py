@@ -70,7 +70,7 @@ class SubwindowMerge:
             raise VardynError("the target grid needs 2-D lon and lat of one shape")
         self.ny, self.nx = lon.shape
         self.device = torch.device(device)
-        if self.device.type == "cuda" and self.device.index is None:
+        if self.device.type == "cuda" and self.device.index is None and torch.cuda.is_available():
             # tensors report their device with its index: compare against that
             self.device = torch.device("cuda", torch.cuda.current_device())
         axes = [_axes(lo, la) for lo, la in subwindows]

@@ -14,8 +14,10 @@ length 1 in front; f on the (nx+1, ny+1) vorticity grid.
 The model runs in libvardyn_hip (``vd_sw_*``, FP64).  Parameters the GPU
 model does not implement are rejected rather than ignored: viscosity and
 thickness diffusion, wind stress, sponge, open boundaries, the barotropic
-filter, external forcing, spatially varying depths or metrics, and
-ensembles of more than one member.
+filter, spatially varying depths or metrics, and ensembles of more than
+one member.  External forcing (Fu, Fv, Fh; the direct momentum mode) is
+supported: ``step`` / ``step_tgl`` take it, ``step_adj_forced`` returns its
+cotangents.
 """
 from __future__ import annotations
 
@@ -145,26 +147,46 @@ class SW:
                                          current_stream_ptr()), "vd_sw_rhs")
         return tuple(self._out(o, ens) for o in out)
 
-    def step(self, u0, v0, h0, nstep=1, **kw):
-        """nstep RK3-SSP steps (sw.py:752-904); physical fields in and out."""
-        _no_extras(kw)
+    def _forcing(self, Fu, Fv, Fh, names=("Fu", "Fv", "Fh")):
+        """Forcing fields (physical units per second, per layer) as device
+        tensors; None stays None (a zero field)."""
+        shapes = (self.u_shape, self.v_shape, self.h_shape)
+        return tuple(None if a is None else self._field(a, shp, nm)[0]
+                     for a, shp, nm in zip((Fu, Fv, Fh), shapes, names))
+
+    def step(self, u0, v0, h0, H=None, nstep=1, u_b=None, v_b=None, h_b=None, Fu=None, Fv=None, Fh=None,
+             taux=None, tauy=None, h_wind=None):
+        """nstep RK3-SSP steps (sw.py:752-904); physical fields in and out.
+        The forcing (Fu, Fv, Fh) is added as x += dt F after every step.
+        The boundary fields u_b, v_b, h_b only enter through the sponge,
+        which this model does not have (sponge_coef = 0), so they change
+        nothing."""
+        if H is not None:
+            raise VardynError("sw: a depth perturbation H is not implemented")
+        _no_extras(dict(taux=taux, tauy=tauy, h_wind=h_wind))
         u, v, h, ens = self._state(u0, v0, h0)
+        F = self._forcing(Fu, Fv, Fh)
         out = [torch.empty_like(a) for a in (u, v, h)]
-        _lib.check(_lib.load().vd_sw_step(self._h, ptr(u), ptr(v), ptr(h), *(ptr(o) for o in out), int(nstep),
-                                          current_stream_ptr()), "vd_sw_step")
+        _lib.check(_lib.load().vd_sw_step(self._h, ptr(u), ptr(v), ptr(h), *(ptr(x) for x in F),
+                                          *(ptr(o) for o in out), int(nstep), current_stream_ptr()), "vd_sw_step")
         return tuple(self._out(o, ens) for o in out)
 
-    def step_tgl(self, u0, v0, h0, du0, dv0, dh0, H=None, dH=None, nstep=1, **kw):
-        """Tangent of ``step`` at (u0, v0, h0) applied to (du0, dv0, dh0)."""
+    def step_tgl(self, u0, v0, h0, du0, dv0, dh0, H=None, dH=None, nstep=1, Fu=None, Fv=None, Fh=None,
+                 dFu=None, dFv=None, dFh=None, **kw):
+        """Tangent of ``step`` at (u0, v0, h0; F) applied to (du0, dv0, dh0; dF)
+        (the reference's ``jax.jvp`` of ``step``, sw.py:906-923; the forcing
+        tangent as in Model_qgsw.jstep_tgl, mod.py:3222-3253)."""
         if H is not None or dH is not None:
             raise VardynError("sw: tangents with respect to H are not implemented")
         _no_extras(kw)
         u, v, h, ens = self._state(u0, v0, h0)
         du, dv, dh, _ = self._state(du0, dv0, dh0, ("du0", "dv0", "dh0"))
+        F = self._forcing(Fu, Fv, Fh)
+        dF = self._forcing(dFu, dFv, dFh, ("dFu", "dFv", "dFh"))
         out = [torch.empty_like(a) for a in (u, v, h)]
-        _lib.check(_lib.load().vd_sw_step_tgl(self._h, ptr(u), ptr(v), ptr(h), ptr(du), ptr(dv), ptr(dh),
-                                              *(ptr(o) for o in out), int(nstep), current_stream_ptr()),
-                   "vd_sw_step_tgl")
+        _lib.check(_lib.load().vd_sw_step_tgl(self._h, ptr(u), ptr(v), ptr(h), *(ptr(x) for x in F), ptr(du),
+                                              ptr(dv), ptr(dh), *(ptr(x) for x in dF), *(ptr(o) for o in out),
+                                              int(nstep), current_stream_ptr()), "vd_sw_step_tgl")
         return tuple(self._out(o, ens) for o in out)
 
     def step_adj(self, u0, v0, h0, wuT, wvT, whT, H=None, nstep=1, **kw):
@@ -173,14 +195,23 @@ class SW:
         if H is not None:
             raise VardynError("sw: adjoints with respect to H are not implemented")
         _no_extras(kw)
+        au, av, ah, _, _, _ = self.step_adj_forced(u0, v0, h0, wuT, wvT, whT, nstep=nstep, forcing_adjoint=False)
+        return ((au, av, ah, None, None),)
+
+    def step_adj_forced(self, u0, v0, h0, wuT, wvT, whT, nstep=1, Fu=None, Fv=None, Fh=None,
+                        forcing_adjoint=True):
+        """Adjoint of the forced ``step`` (Model_qgsw.jstep_adj, mod.py:3255-3290):
+        (au0, av0, ah0, aFu, aFv, aFh), the forcing cotangents None unless
+        ``forcing_adjoint``."""
         u, v, h, ens = self._state(u0, v0, h0)
         wu, wv, wh, _ = self._state(wuT, wvT, whT, ("wuT", "wvT", "whT"))
+        F = self._forcing(Fu, Fv, Fh)
         out = [torch.empty_like(a) for a in (u, v, h)]
-        _lib.check(_lib.load().vd_sw_step_adj(self._h, ptr(u), ptr(v), ptr(h), ptr(wu), ptr(wv), ptr(wh),
-                                              *(ptr(o) for o in out), int(nstep), current_stream_ptr()),
-                   "vd_sw_step_adj")
-        au, av, ah = (self._out(o, ens) for o in out)
-        return ((au, av, ah, None, None),)
+        aF = [torch.empty_like(a) for a in (u, v, h)] if forcing_adjoint else [None, None, None]
+        _lib.check(_lib.load().vd_sw_step_adj(self._h, ptr(u), ptr(v), ptr(h), *(ptr(x) for x in F), ptr(wu),
+                                              ptr(wv), ptr(wh), *(ptr(o) for o in out), *(ptr(x) for x in aF),
+                                              int(nstep), current_stream_ptr()), "vd_sw_step_adj")
+        return tuple(self._out(o, ens) for o in out) + tuple(None if a is None else self._out(a, ens) for a in aF)
 
     # ------------------------------------------------------------ conversions
     def get_physical_uvh(self, u, v, h, numpy=False):
