@@ -236,7 +236,8 @@ class BasisWavelet3dOracle:
 
     def operg(self, it, X):
         """Field at basis time index ``it`` (basis.py:2936-2960)."""
-        phi = np.zeros(self.nphys)
+        X = np.asarray(X)
+        phi = np.zeros(self.nphys, dtype=np.result_type(X.dtype, np.float64))
         for iff in range(self.nfs):
             b0, b1 = self.fs_wavebounds[iff], self.fs_wavebounds[iff + 1]
             g = self.Gt[it][iff]
