@@ -113,6 +113,10 @@ def parse():
                     help="the 3-layer shallow-water model (256x256, dx 8 km, beta plane, dt 120 s) instead: "
                          "forward steps and the gradient of a final-state cost over the 10-day window")
     ap.add_argument("--sw-days", type=float, default=10.0, help="shallow-water window (days)")
+    ap.add_argument("--sw4dvar", action="store_true",
+                    help="the 3-layer shallow-water 4D-Var with the 3-D wavelet forcing control instead "
+                         "(256x256, 10-day window, 30 nadir tracks): gradient evaluations per second, the "
+                         "basis products' roofline, parity against the oracle cost on short windows")
     ap.add_argument("--tile-size", type=int, default=32)
     ap.add_argument("--tile-overlap", type=int, default=16)
     ap.add_argument("--tiled-steps", type=int, default=16,
@@ -1158,6 +1162,156 @@ def release(pb):
     torch.cuda.empty_cache()
 
 
+def sw_oracle_cost(pb, prec=False):
+    """The FP64 oracle cost (oracle/sw4dvar.py) of a shallow-water problem."""
+    from oracle.basis_wavelet3d import BasisWavelet3dOracle
+    from oracle.obsop import ObsopOracle
+    from oracle.sw import SwOracle
+    from oracle.sw4dvar import SwCostOracle
+    from vardyn_amd.sw_problem import basis_kwargs
+    c, m = pb.case, pb.model
+    sw = SwOracle(m.nx, m.ny, m.dx, m.dy, m.H, m.g_prime, m.f, m.dt)
+    lon, lat = pb.grid.lon.T, pb.grid.lat.T
+    H = ObsopOracle(lon, lat, np.full(lon.shape, c.dx), np.full(lon.shape, c.dx), pb.T, c.dt, pb.dict_obs, Npix=4)
+    basis = BasisWavelet3dOracle(lon, lat, **basis_kwargs(c))
+    basis.set_basis(pb.time_basis)
+    return SwCostOracle(sw, H, basis, pb.checkpoints, pb.T, c.checkpoint_steps, pb.x0, sigma_b=pb.Q, prec=prec)
+
+
+def sw_parity(pb, seed=3):
+    """J and one directional derivative of the GPU engine against the oracle
+    cost (complex step) at X = 0.5 x truth coefficients."""
+    import torch
+    orc = sw_oracle_cost(pb)
+    X0 = 0.5 * pb.X_true
+    J, G = pb.engine.run(torch.as_tensor(X0, device=pb.model.device))
+    J, G = float(J[0]), G.cpu().numpy()
+    t0 = time.time()
+    Jo = orc.cost(X0)
+    cost_s = time.time() - t0
+    d = np.random.default_rng(seed).standard_normal(G.size)
+    dd = orc.directional_derivative(X0, d)
+    return {"steps": int(pb.T.size - 1), "grid": [pb.case.nx, pb.case.ny], "J_rel_err": abs(J - Jo) / abs(Jo),
+            "dir_deriv_rel_err": abs(G @ d - dd) / abs(dd), "oracle_cost_s": round(cost_s, 2)}
+
+
+def w3_roofline(basis, prep, X, reps=20):
+    """Average durations of the basis synthesis (all prepared times) and of its
+    transpose, HIP events on their stream.  Both stream the tile blocks of the
+    space matrix once per pass of up to 16 times: algorithmic bytes = 256
+    doubles per stored tile entry and pass (plus the coefficients, fields and
+    partials, small beside it)."""
+    import torch
+    st = torch.cuda.current_stream()
+    out = torch.empty((prep.T, basis.ny, basis.nx), dtype=torch.float64, device=basis.device)
+    adX = torch.zeros(basis.nbasis, dtype=torch.float64, device=basis.device)
+    res = {}
+    for name, fn in (("synthesis", lambda: basis.operg_prepared(prep, X, out)),
+                     ("transpose", lambda: basis.operg_transpose_prepared(prep, out, adX))):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(reps):
+            fn()
+        e1.record(st)
+        torch.cuda.synchronize()
+        res[name + "_us"] = round(e0.elapsed_time(e1) / reps * 1e3, 2)
+    passes = -(-prep.T // 16)
+    bytes_ = basis.nentries * 256 * 8 * passes
+    flops = 2.0 * 16 * basis.nentries * 256 * passes
+    us = res["synthesis_us"]
+    res.update({"bound": "hbm", "achieved": round(bytes_ / (us * 1e-6) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(bytes_ / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 3),
+                "traffic": None, "block_bytes": bytes_, "times": prep.T,
+                "mfma_f64_tflops_issued": round(flops / (us * 1e-6) / 1e12, 2),
+                "space_matrix_nnz": basis.nnz_space, "stored_entries": basis.nentries * 256})
+    return res
+
+
+def sw4dvar(args):
+    """The 3-layer shallow-water 4D-Var of the wavelet-forcing configuration:
+    one evaluation = forward sweep with the misfits of 30 nadir tracks, reverse
+    sweep with the hand-written adjoint (segment recomputation inside
+    vd_sw_step_adj), the basis synthesis of every basis time before and its
+    transpose after; at X = 0.5 x the truth's coefficients (a forced, flowing
+    state).  Beside it: the basis products' roofline, parity of J and of a
+    directional derivative against the oracle cost on two short windows (the
+    40 x 36 test grid and the 256 x 256 grid over 30 steps), and the oracle's
+    forward speed on one host thread."""
+    import torch
+    from vardyn_amd.sw_problem import SwCase, build_sw_problem, small_case
+    case = SwCase(days=args.sw_days)
+    t0 = time.time()
+    pb = build_sw_problem(case)
+    build_s = time.time() - t0
+    log(f"sw4dvar: {case.nx}x{case.ny}x{len(case.H)}, {case.n_steps} steps, {pb.basis.nbasis} coefficients, "
+        f"{pb.H.n_total} obs, {len(pb.checkpoints) - 1} segments, built in {build_s:.1f}s")
+    eng = pb.engine
+    X0 = torch.as_tensor(0.5 * pb.X_true, device=pb.model.device)
+    for _ in range(max(1, args.warmup)):
+        J, G = eng.run(X0)
+    torch.cuda.synchronize()
+    st = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    n = max(1, min(args.steps, 3))
+    e0.record(st)
+    for _ in range(n):
+        J, G = eng.run(X0)
+    e1.record(st)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / n
+    log(f"sw4dvar: {ms:.1f} ms per gradient")
+    roof = w3_roofline(pb.basis, eng.bprep, eng.X)
+    log(f"basis roofline: {roof}")
+    # forward-only speed (the same window, the engine's model)
+    xs = [torch.as_tensor(a, device=pb.model.device) for a in pb.x0]
+    e0.record(st)
+    pb.model.step(*xs, nstep=case.n_steps)
+    e1.record(st)
+    torch.cuda.synchronize()
+    fwd_s = e0.elapsed_time(e1) / 1e3
+    out = {
+        "metric": "sw_4dvar_gradients_per_sec", "value": round(1e3 / ms, 4), "unit": "gradients/s", "n_gpus": 1,
+        "steps": n, "warmup": args.warmup, "ms_per_step": round(ms, 1), "higher_is_better": True,
+        "dtype": "f64", "data": "synthetic (seeded eddy initial state, forced truth run, nadir tracks)",
+        "config": {"workload": f"{case.name}: 3-layer shallow water {case.nx}x{case.ny}, dx 8 km, beta plane, "
+                               f"dt {case.dt:g} s, {case.days:g}-day window ({case.n_steps} steps), wavelet forcing "
+                               f"control ({pb.basis.nbasis} coefficients, {pb.basis.nfs} scales "
+                               f"{1 / pb.basis.fs.max():g}-{1 / pb.basis.fs.min():g} km, {pb.basis.ntheta} "
+                               f"directions), {case.n_tracks} nadir tracks",
+                   "obs": pb.H.n_total, "segments": len(pb.checkpoints) - 1, "basis_times": int(eng.bprep.T),
+                   "evaluated_at": "X = 0.5 x truth coefficients", "J": float(J[0]),
+                   "grad_norm": float(torch.linalg.vector_norm(G))},
+        "cell_steps_per_s": round(case.nx * case.ny * len(case.H) * case.n_steps * 1e3 / ms, 0),
+        "forward_window_s": round(fwd_s, 3), "build_s": round(build_s, 1),
+        "roofline": roof,
+    }
+    del pb, eng
+    release(None)
+    small = build_sw_problem(small_case())
+    out["parity_small"] = sw_parity(small)
+    log(f"parity (40x36, 240 steps): {out['parity_small']}")
+    del small
+    short = build_sw_problem(SwCase(days=30 * 120.0 / 86400, n_tracks=6))
+    par = sw_parity(short)
+    out["parity_256_30steps"] = par
+    log(f"parity (256x256, 30 steps): {par}")
+    del short
+    release(None)
+    if not args.no_cpu_baseline:
+        # the oracle's forward steps on one host thread (it has no adjoint: its
+        # gradient is taken by complex steps, one direction at a time)
+        steps_s = 30 / par["oracle_cost_s"]
+        grad_est = steps_s / (case.n_steps * 3.0)  # forward + recomputation + adjoint ~ 3 forwards
+        out["cpu_baseline"] = {"value": round(steps_s, 2), "unit": "forward steps/s (256x256x3)", "cores": 1,
+                               "kind": "port", "sample": "oracle cost (forward sweep, misfits, basis) over the "
+                                                         "30-step 256x256 window",
+                               "gradients_per_s_if_adjoint_cost_3_forwards": float(f"{grad_est:.3e}")}
+        out["gpu_over_cpu_forward"] = round((case.n_steps / fwd_s) / steps_s, 1)
+    return out
+
+
 def main():
     args = parse()
     if args.precision != "float64":
@@ -1191,6 +1345,9 @@ def main():
         return
     if args.sw:
         print(json.dumps(sw_model(args)), flush=True)
+        return
+    if args.sw4dvar:
+        print(json.dumps(sw4dvar(args)), flush=True)
         return
     if args.gemm_only:
         from vardyn_amd.qg import QgPlan
