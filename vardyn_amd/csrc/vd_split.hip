@@ -309,19 +309,7 @@ __device__ __forceinline__ void split_tile(const SplitArgs& sa, int z, int tile)
   };
   SineFrag sf[3];
   DataFrag dfr[2];
-  if (wk < nstep) {
-    load_sine(wk, sf[0]);
-    load_data(wk, dfr[0]);
-  }
-  if (wk + WK < nstep) load_sine(wk + WK, sf[1]);
-#pragma unroll
-  for (int i = 0; i < NSTEP; ++i) {
-    const int st = wk + i * WK;
-    if (st >= nstep) break;
-    if (st + 2 * WK < nstep) load_sine(st + 2 * WK, sf[(i + 2) % 3]);
-    if (st + WK < nstep) load_data(st + WK, dfr[(i + 1) % 2]);
-    const SineFrag& cur = sf[i % 3];
-    const DataFrag& df = dfr[i % 2];
+  auto mfma_step = [&](const SineFrag& cur, const DataFrag& df) {
 #pragma unroll
     for (int p = 0; p < NS; ++p)
 #pragma unroll
@@ -332,6 +320,44 @@ __device__ __forceinline__ void split_tile(const SplitArgs& sa, int z, int tile)
           for (int b = 0; b < NT; ++b)
             acc[p + q][a][b] =
                 __builtin_amdgcn_mfma_i32_32x32x32_i8(cur[p][a], df[q][b], acc[p + q][a][b], 0, 0, 0);
+  };
+  if (nstep == NSTEP) {
+    // The whole K range of the template: every load below is unconditional
+    // (its condition is a compile-time function of the unrolled iteration),
+    // so the compiler's wait counts track each prefetch batch exactly and an
+    // MFMA waits only for its own step's fragments.  (Under run-time
+    // conditions the waits had to assume the worst at every join and drained
+    // all outstanding loads once per step.)
+    constexpr int NI = NSTEP / WK;
+    static_assert(NI * WK == NSTEP, "K steps divide among the wave groups");
+    load_sine(wk, sf[0]);
+    load_data(wk, dfr[0]);
+    if constexpr (NI > 1) load_sine(wk + WK, sf[1]);
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+      const int st = wk + i * WK;
+      if (i + 2 < NI) load_sine(st + 2 * WK, sf[(i + 2) % 3]);
+      if (i + 1 < NI) load_data(st + WK, dfr[(i + 1) % 2]);
+      // keep the prefetches ahead of this step's MFMAs: the scheduler would
+      // otherwise sink each load next to its first use to shorten live ranges
+      __builtin_amdgcn_sched_barrier(0);
+      mfma_step(sf[i % 3], dfr[i % 2]);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  } else {
+    if (wk < nstep) {
+      load_sine(wk, sf[0]);
+      load_data(wk, dfr[0]);
+    }
+    if (wk + WK < nstep) load_sine(wk + WK, sf[1]);
+#pragma unroll
+    for (int i = 0; i < NSTEP; ++i) {
+      const int st = wk + i * WK;
+      if (st >= nstep) break;
+      if (st + 2 * WK < nstep) load_sine(st + 2 * WK, sf[(i + 2) % 3]);
+      if (st + WK < nstep) load_data(st + WK, dfr[(i + 1) % 2]);
+      mfma_step(sf[i % 3], dfr[i % 2]);
+    }
   }
   if constexpr (WK > 1) {
     // K-split partials: groups 1..WK-1 park each level in LDS (over the data
