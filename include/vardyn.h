@@ -561,6 +561,24 @@ int vd_sw_step_adj(vd_sw_model* model, const double* u0, const double* v0, const
                    const double* Fv, const double* Fh, const double* au1, const double* av1, const double* ah1,
                    double* au0, double* av0, double* ah0, double* aFu, double* aFv, double* aFh, int nstep,
                    void* stream);
+/* vd_sw_step that also keeps the scaled states (u dx, v dy, h dx dy, one
+ * buffer [u | v | h] of nu + nv + nh values each) before steps 0, stride,
+ * 2 stride, ... into `keep` (ceil(nstep / stride) states); stride 0 keeps
+ * every step's start state and its two RK3 stage states (3 nstep states):
+ * the trajectory a 4D-Var driver's forward sweep leaves for the reverse
+ * sweep.  No reference counterpart (the reference's jax.vjp of a chunk
+ * recomputes it). */
+int vd_sw_step_keep(vd_sw_model* model, const double* u0, const double* v0, const double* h0, const double* Fu,
+                    const double* Fv, const double* Fh, double* u1, double* v1, double* h1, int nstep, int stride,
+                    double* keep, void* stream);
+/* vd_sw_step_adj from states kept by vd_sw_step_keep with the same forcing,
+ * nstep and stride: no forward pass; stride 0 recomputes nothing, stride 1
+ * only each step's two stage states, a larger stride also each run of
+ * `stride` steps once. */
+int vd_sw_step_adj_kept(vd_sw_model* model, const double* Fu, const double* Fv, const double* Fh,
+                        const double* keep, int stride, const double* au1, const double* av1, const double* ah1,
+                        double* au0, double* av0, double* ah0, double* aFu, double* aFv, double* aFh, int nstep,
+                        void* stream);
 /* Precision conversion: dst[i] = (dst type) src[i]; NaN -> 0 if nan_to_zero. */
 int vd_convert(int src_precision, const void* src, int dst_precision, void* dst, int64_t n,
                int nan_to_zero, void* stream);

@@ -70,3 +70,24 @@ def test_gradient_against_gpu_finite_differences():
     jm = float(pb.engine.run(X0 - eps * d)[0][0])
     fd = (jp - jm) / (2 * eps)
     assert abs(float(G @ d) - fd) <= 1e-5 * abs(fd)
+
+
+def test_tape_modes_bitwise_equal():
+    """Every step's state kept (stage states recomputed), sqrt-spaced states
+    with each run recomputed, and every step with its stage states (nothing
+    recomputed) give the same bits."""
+    pb = build_sw_problem(small_case(), tape="full")
+    X0 = torch.as_tensor(0.5 * pb.X_true, device="cuda")
+    J1, G1 = (t.clone() for t in pb.engine.run(X0))
+    assert pb.engine.tape_mode == "full" and int(pb.engine.stride.max()) == 1
+    from vardyn_amd.sw4dvar import SwFourDVar
+    e2 = SwFourDVar(pb.model, pb.H, pb.checkpoints, pb.T, pb.basis, pb.x0, sigma_b=pb.Q,
+                    dtbasis=pb.case.checkpoint_steps, tape="checkpoint")
+    assert int(e2.stride.max()) > 1
+    J2, G2 = e2.run(X0)
+    assert torch.equal(J1, J2) and torch.equal(G1, G2)
+    e3 = SwFourDVar(pb.model, pb.H, pb.checkpoints, pb.T, pb.basis, pb.x0, sigma_b=pb.Q,
+                    dtbasis=pb.case.checkpoint_steps, tape="stages")
+    assert int(e3.stride.max()) == 0
+    J3, G3 = e3.run(X0)
+    assert torch.equal(J1, J3) and torch.equal(G1, G3)

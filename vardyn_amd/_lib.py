@@ -115,6 +115,8 @@ SIGNATURES = {
     "vd_sw_step": (_I, [_VP] * 10 + [_I, _VP]),
     "vd_sw_step_tgl": (_I, [_VP] * 16 + [_I, _VP]),
     "vd_sw_step_adj": (_I, [_VP] * 16 + [_I, _VP]),
+    "vd_sw_step_keep": (_I, [_VP] * 10 + [_I, _I, _VP, _VP]),
+    "vd_sw_step_adj_kept": (_I, [_VP] * 5 + [_I] + [_VP] * 9 + [_I, _VP]),
     "vd_merge_create": (_I, [_VP, _VP]),
     "vd_merge_destroy": (_I, [_VP]),
     "vd_merge_info": (_I, [_VP, C.POINTER(C.c_longlong)]),

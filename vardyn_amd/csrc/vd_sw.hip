@@ -590,11 +590,14 @@ struct vd_sw_model {
   double* ws = nullptr;
   double *L = nullptr, *LK0 = nullptr, *LK1 = nullptr, *LK2 = nullptr;
   double *F = nullptr, *dF = nullptr, *aF = nullptr;  // scaled forcing, its tangent and cotangent
-  double* tape = nullptr;
+  double* tape = nullptr;    // segment start states of vd_sw_step_adj
   size_t tape_states = 0;
+  double* segbuf = nullptr;  // the step states of one segment in the reverse sweep
+  size_t seg_states = 0;
   ~vd_sw_model() {
     for (void* p : allocs) (void)hipFree(p);
     if (tape) (void)hipFree(tape);
+    if (segbuf) (void)hipFree(segbuf);
   }
 };
 
@@ -712,8 +715,23 @@ void add_forcing(const vd_sw_model* m, double* x, const double* F, hipStream_t s
 }
 
 // L <- (one RK3 step at x0)^T L
-int sw_rk3_adj(vd_sw_model* m, const double* x0, hipStream_t s) {
+// L <- (one RK3 step at x0)^T L with the stage states x1, x2 given.
+int sw_rk3_adj_states(vd_sw_model* m, const double* x0, const double* x1, const double* x2, hipStream_t s) {
   const double dt = m->g.dt;
+  int rc;
+  // x3 = x2 + dt/12 (8 k2 - k1 - k0)
+  comb(m, m->L, m->LK2, 0.0, (dt / 12.0) * 8.0, m->LK1, 0.0, -(dt / 12.0), m->LK0, 0.0, -(dt / 12.0), s);
+  if ((rc = sw_rhs_adj(m, x2, m->LK2, s))) return rc;
+  // x2 = x1 + dt/4 (k1 - 3 k0)
+  comb(m, m->L, m->LK1, 1.0, dt / 4.0, m->LK0, 1.0, -3.0 * (dt / 4.0), nullptr, 0, 0, s);
+  if ((rc = sw_rhs_adj(m, x1, m->LK1, s))) return rc;
+  // x1 = x0 + dt k0
+  comb(m, m->L, m->LK0, 1.0, dt, nullptr, 0, 0, nullptr, 0, 0, s);
+  return sw_rhs_adj(m, x0, m->LK0, s);
+}
+
+// L <- (one RK3 step at x0)^T L, the stage states recomputed from x0.
+int sw_rk3_adj(vd_sw_model* m, const double* x0, hipStream_t s) {
   // stage states x1 (SB), x2 (SC) from x0: the forward stages 0 and 1
   {
     const SwGrid& g = m->g;
@@ -730,16 +748,7 @@ int sw_rk3_adj(vd_sw_model* m, const double* x0, hipStream_t s) {
     hipLaunchKernelGGL((k_sw_tend<double, 1>), grd(g, g.nl), blk(), 0, s, g, x, omi, hti, kepi, k0, k1, k1o,
                        sto_of(m, m->SC, nullptr));
   }
-  int rc;
-  // x3 = x2 + dt/12 (8 k2 - k1 - k0)
-  comb(m, m->L, m->LK2, 0.0, (dt / 12.0) * 8.0, m->LK1, 0.0, -(dt / 12.0), m->LK0, 0.0, -(dt / 12.0), s);
-  if ((rc = sw_rhs_adj(m, m->SC, m->LK2, s))) return rc;
-  // x2 = x1 + dt/4 (k1 - 3 k0)
-  comb(m, m->L, m->LK1, 1.0, dt / 4.0, m->LK0, 1.0, -3.0 * (dt / 4.0), nullptr, 0, 0, s);
-  if ((rc = sw_rhs_adj(m, m->SB, m->LK1, s))) return rc;
-  // x1 = x0 + dt k0
-  comb(m, m->L, m->LK0, 1.0, dt, nullptr, 0, 0, nullptr, 0, 0, s);
-  return sw_rhs_adj(m, x0, m->LK0, s);
+  return sw_rk3_adj_states(m, x0, m->SB, m->SC, s);
 }
 
 }  // namespace
@@ -888,6 +897,61 @@ int vd_sw_step_tgl(vd_sw_model* m, const double* u0, const double* v0, const dou
   return VD_OK;
 }
 
+// Reverse sweep over nstep steps from kept scaled states: state k*seg at
+// ck + k*ns (k < ceil(nstep/seg)); L holds the cotangent of the final scaled
+// state and is overwritten by the initial one; aF (when wanted) collects the
+// scaled forcing cotangent.  Inside a sub-segment the step states are
+// recomputed into the model's segment buffer (none for seg = 1).
+static int sw_reverse(vd_sw_model* m, bool forced, const double* ck, int seg, int nstep, bool want_aF,
+                      hipStream_t s) {
+  const size_t ns = m->ns;
+  if (seg == 0) {
+    // every step's start state and its two RK3 stage states were kept
+    for (int k = nstep - 1; k >= 0; --k) {
+      if (want_aF) add_forcing(m, m->aF, m->L, s);
+      const double* xk = ck + (size_t)3 * k * ns;
+      int rc = sw_rk3_adj_states(m, xk, xk + ns, xk + 2 * ns, s);
+      if (rc) return rc;
+    }
+    return VD_OK;
+  }
+  const int nseg = nstep > 0 ? (nstep + seg - 1) / seg : 0;
+  double* st = nullptr;
+  if (seg > 1) {
+    if (m->seg_states < (size_t)seg) {
+      if (m->segbuf) VD_HIP_TRY(hipFree(m->segbuf));
+      m->segbuf = nullptr;
+      m->seg_states = 0;
+      VD_HIP_TRY(hipMalloc(&m->segbuf, (size_t)seg * ns * sizeof(double)));
+      m->seg_states = seg;
+    }
+    st = m->segbuf;
+  }
+  for (int q = nseg - 1; q >= 0; --q) {
+    const int s0 = q * seg, s1 = std::min(nstep, s0 + seg);
+    if (seg > 1) {
+      VD_HIP_TRY(hipMemcpyAsync(m->SA, ck + (size_t)q * ns, ns * sizeof(double), hipMemcpyDeviceToDevice, s));
+      for (int k = s0; k < s1; ++k) {
+        VD_HIP_TRY(
+            hipMemcpyAsync(st + (size_t)(k - s0) * ns, m->SA, ns * sizeof(double), hipMemcpyDeviceToDevice, s));
+        if (k + 1 < s1) {
+          int rc = sw_rk3<false>(m, m->SA, nullptr, s);
+          if (rc) return rc;
+          if (forced) add_forcing(m, m->SA, m->F, s);
+        }
+      }
+    }
+    for (int k = s1 - 1; k >= s0; --k) {
+      // x_{k+1} = RK3(x_k) + dt F: F collects dt L_{k+1} before L goes back through the step
+      if (want_aF) add_forcing(m, m->aF, m->L, s);
+      const double* xk = seg > 1 ? st + (size_t)(k - s0) * ns : ck + (size_t)q * ns;
+      int rc = sw_rk3_adj(m, xk, s);
+      if (rc) return rc;
+    }
+  }
+  return VD_OK;
+}
+
 int vd_sw_step_adj(vd_sw_model* m, const double* u0, const double* v0, const double* h0, const double* Fu,
                    const double* Fv, const double* Fh, const double* au1, const double* av1, const double* ah1,
                    double* au0, double* av0, double* ah0, double* aFu, double* aFv, double* aFh, int nstep,
@@ -898,20 +962,18 @@ int vd_sw_step_adj(vd_sw_model* m, const double* u0, const double* v0, const dou
   const size_t ns = m->ns;
   const bool forced = load_forcing(m, Fu, Fv, Fh, m->F, s);
   const bool want_aF = aFu || aFv || aFh;
-  // segments of about sqrt(nstep) steps: their start states, then one
-  // segment's step states at a time
+  // segments of about sqrt(nstep) steps: their start states from a forward
+  // pass, then one segment's step states at a time
   const int seg = std::max(1, (int)std::ceil(std::sqrt((double)nstep)));
   const int nseg = nstep > 0 ? (nstep + seg - 1) / seg : 0;
-  const size_t need = (size_t)nseg + seg;
-  if (m->tape_states < need) {
+  if (m->tape_states < (size_t)nseg) {
     if (m->tape) VD_HIP_TRY(hipFree(m->tape));
     m->tape = nullptr;
     m->tape_states = 0;
-    VD_HIP_TRY(hipMalloc(&m->tape, need * ns * sizeof(double)));
-    m->tape_states = need;
+    VD_HIP_TRY(hipMalloc(&m->tape, std::max(nseg, 1) * ns * sizeof(double)));
+    m->tape_states = nseg;
   }
   double* ck = m->tape;
-  double* st = m->tape + (size_t)nseg * ns;
   scale3(m, u0, v0, h0, m->SA, m->SA + m->nu, m->SA + m->nu + m->nv, 0, s);
   for (int k = 0; k < nstep; ++k) {
     if (k % seg == 0) VD_HIP_TRY(hipMemcpyAsync(ck + (size_t)(k / seg) * ns, m->SA, ns * sizeof(double),
@@ -924,25 +986,58 @@ int vd_sw_step_adj(vd_sw_model* m, const double* u0, const double* v0, const dou
   scale3(m, au1, av1, ah1, m->L, m->L + m->nu, m->L + m->nu + m->nv, 1, s);
   // the forcing cotangent starts at zero (a missing input scales to zero)
   if (want_aF) scale3(m, nullptr, nullptr, nullptr, m->aF, m->aF + m->nu, m->aF + m->nu + m->nv, 0, s);
-  for (int q = nseg - 1; q >= 0; --q) {
-    const int s0 = q * seg, s1 = std::min(nstep, s0 + seg);
-    VD_HIP_TRY(hipMemcpyAsync(m->SA, ck + (size_t)q * ns, ns * sizeof(double), hipMemcpyDeviceToDevice, s));
-    for (int k = s0; k < s1; ++k) {
-      VD_HIP_TRY(hipMemcpyAsync(st + (size_t)(k - s0) * ns, m->SA, ns * sizeof(double), hipMemcpyDeviceToDevice, s));
-      if (k + 1 < s1) {
-        int rc = sw_rk3<false>(m, m->SA, nullptr, s);
-        if (rc) return rc;
-        if (forced) add_forcing(m, m->SA, m->F, s);
-      }
-    }
-    for (int k = s1 - 1; k >= s0; --k) {
-      // x_{k+1} = RK3(x_k) + dt F: F collects dt L_{k+1} before L goes back through the step
-      if (want_aF) add_forcing(m, m->aF, m->L, s);
-      int rc = sw_rk3_adj(m, st + (size_t)(k - s0) * ns, s);
-      if (rc) return rc;
-    }
-  }
+  int rc = sw_reverse(m, forced, ck, seg, nstep, want_aF, s);
+  if (rc) return rc;
   // set_input_uvh: u_scaled = (u mask_u) dx
+  scale3(m, m->L, m->L + m->nu, m->L + m->nu + m->nv, au0, av0, ah0, 2, s);
+  if (want_aF) scale3(m, m->aF, m->aF + m->nu, m->aF + m->nu + m->nv, aFu, aFv, aFh, 2, s);
+  VD_HIP_TRY(hipGetLastError());
+  return VD_OK;
+}
+
+int vd_sw_step_keep(vd_sw_model* m, const double* u0, const double* v0, const double* h0, const double* Fu,
+                    const double* Fv, const double* Fh, double* u1, double* v1, double* h1, int nstep, int stride,
+                    double* keep, void* stream) {
+  VD_REQUIRE(m && u0 && v0 && h0 && u1 && v1 && h1 && keep, VD_ERR_INVALID_ARG, "null argument");
+  VD_REQUIRE(nstep >= 0 && stride >= 0, VD_ERR_INVALID_ARG, "sw: nstep >= 0 and stride >= 0");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const size_t ns = m->ns;
+  const bool forced = load_forcing(m, Fu, Fv, Fh, m->F, s);
+  scale3(m, u0, v0, h0, m->SA, m->SA + m->nu, m->SA + m->nu + m->nv, 0, s);
+  for (int k = 0; k < nstep; ++k) {
+    if (stride == 0)
+      VD_HIP_TRY(hipMemcpyAsync(keep + (size_t)3 * k * ns, m->SA, ns * sizeof(double), hipMemcpyDeviceToDevice, s));
+    else if (k % stride == 0)
+      VD_HIP_TRY(hipMemcpyAsync(keep + (size_t)(k / stride) * ns, m->SA, ns * sizeof(double),
+                                hipMemcpyDeviceToDevice, s));
+    int rc = sw_rk3<false>(m, m->SA, nullptr, s);
+    if (rc) return rc;
+    if (stride == 0) {
+      // the stage states the step left in SB (x1) and SC (x2)
+      VD_HIP_TRY(hipMemcpyAsync(keep + ((size_t)3 * k + 1) * ns, m->SB, ns * sizeof(double),
+                                hipMemcpyDeviceToDevice, s));
+      VD_HIP_TRY(hipMemcpyAsync(keep + ((size_t)3 * k + 2) * ns, m->SC, ns * sizeof(double),
+                                hipMemcpyDeviceToDevice, s));
+    }
+    if (forced) add_forcing(m, m->SA, m->F, s);
+  }
+  scale3(m, m->SA, m->SA + m->nu, m->SA + m->nu + m->nv, u1, v1, h1, 1, s);
+  VD_HIP_TRY(hipGetLastError());
+  return VD_OK;
+}
+
+int vd_sw_step_adj_kept(vd_sw_model* m, const double* Fu, const double* Fv, const double* Fh, const double* keep,
+                        int stride, const double* au1, const double* av1, const double* ah1, double* au0,
+                        double* av0, double* ah0, double* aFu, double* aFv, double* aFh, int nstep, void* stream) {
+  VD_REQUIRE(m && keep && au1 && av1 && ah1 && au0 && av0 && ah0, VD_ERR_INVALID_ARG, "null argument");
+  VD_REQUIRE(nstep >= 0 && stride >= 0, VD_ERR_INVALID_ARG, "sw: nstep >= 0 and stride >= 0");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const bool forced = load_forcing(m, Fu, Fv, Fh, m->F, s);
+  const bool want_aF = aFu || aFv || aFh;
+  scale3(m, au1, av1, ah1, m->L, m->L + m->nu, m->L + m->nu + m->nv, 1, s);
+  if (want_aF) scale3(m, nullptr, nullptr, nullptr, m->aF, m->aF + m->nu, m->aF + m->nu + m->nv, 0, s);
+  int rc = sw_reverse(m, forced, keep, stride, nstep, want_aF, s);
+  if (rc) return rc;
   scale3(m, m->L, m->L + m->nu, m->L + m->nu + m->nv, au0, av0, ah0, 2, s);
   if (want_aF) scale3(m, m->aF, m->aF + m->nu, m->aF + m->nu + m->nv, aFu, aFv, aFh, 2, s);
   VD_HIP_TRY(hipGetLastError());

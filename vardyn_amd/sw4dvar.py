@@ -20,9 +20,11 @@ datlas-ocean/VarDyn:
   parameter adjoint, which the basis transpose consumes at basis times
   (tools_4Dvar.py:369-374).
 
-The segment adjoint is ``vd_sw_step_adj`` (forward sweep with sqrt-spaced
-segment states, each segment recomputed, the hand-written adjoint RK3
-stages).  The fields of all basis times come from one batched synthesis
+The forward sweep keeps the trajectory (``vd_sw_step_keep``): every step's
+state and RK3 stage states with ``tape="stages"``, every step's state with
+``tape="full"``, or every ceil(sqrt(n))-th state of each segment with
+``tape="checkpoint"``; the segment adjoint (``vd_sw_step_adj_kept``) runs
+the hand-written adjoint RK3 stages from it, recomputing what was not kept.  The fields of all basis times come from one batched synthesis
 before the forward sweep; the adjoint fields of all basis times go through
 one batched transpose after the reverse sweep.  Cost and gradient are FP64;
 the model runs in FP64.
@@ -45,7 +47,7 @@ from .sw import SW
 
 class SwFourDVar:
     def __init__(self, model: SW, obsop, checkpoints, T, basis, x0, sigma_b, Xb=None, sigma_r=None,
-                 prec=False, dtbasis=None):
+                 prec=False, dtbasis=None, tape="auto"):
         self.model = model
         self.H = obsop
         self.basis = basis
@@ -80,6 +82,25 @@ class SwFourDVar:
             raise VardynError("x0 must be (u, v, h) of the model's shapes")
         self.states = torch.empty((self.nseg + 1, self.ns), dtype=torch.float64, **dd)
         self.x0 = torch.cat(x0)
+        # the trajectory the forward sweep keeps for the reverse sweep: every
+        # step's scaled state with its two RK3 stage states ("stages", no
+        # recomputation), every step's state ("full", the stage states
+        # recomputed), or every ceil(sqrt(n))-th step of each segment
+        # ("checkpoint", each run also recomputed once); "auto": the first
+        # that takes at most 80% of the free device memory
+        if tape not in ("auto", "stages", "full", "checkpoint"):
+            raise ValueError("tape must be 'auto', 'stages', 'full' or 'checkpoint'")
+        if tape == "auto":
+            free, _ = torch.cuda.mem_get_info(self.dev)
+            need = int(self.nsteps.sum()) * self.ns * 8
+            tape = "stages" if 3 * need <= 0.8 * free else ("full" if need <= 0.8 * free else "checkpoint")
+        self.tape_mode = tape
+        # kept-state stride per segment: 0 = every step with its RK3 stage states
+        self.stride = np.array([0 if tape == "stages" else 1 if tape == "full" else max(1, int(np.ceil(np.sqrt(n))))
+                                for n in self.nsteps])
+        kept = np.where(self.stride == 0, 3 * self.nsteps, -(-self.nsteps // np.maximum(self.stride, 1)))
+        self.keep_off = np.concatenate([[0], np.cumsum(kept)]).astype(np.int64) * self.ns
+        self.keep = torch.empty(int(self.keep_off[-1]), dtype=torch.float64, **dd)
         # observation table index at each segment start / the final time
         self.seg_obs = [self.H.index(self.T[c]) if self.H.is_obs_time(self.T[c]) else -1 for c in cps[:-1]]
         self.final_obs = self.H.index(self.T[cps[-1]]) if self.H.is_obs_time(self.T[cps[-1]]) else -1
@@ -152,8 +173,10 @@ class SwFourDVar:
             if self.seg_obs[i] >= 0:
                 self.H.misfit_dev(self.seg_obs[i], self._surface(st), jo=J, inv_sigma_r2=inv_r2)
             F = self.Fh[self.seg_basis[i]]
-            _lib.check(lib.vd_sw_step(m._h, *(ptr(a) for a in self._parts(st)), None, None, ptr(F),
-                                      *(ptr(a) for a in self._parts(nxt)), int(self.nsteps[i]), s), "vd_sw_step")
+            keep = self.keep[int(self.keep_off[i]):int(self.keep_off[i + 1])]
+            _lib.check(lib.vd_sw_step_keep(m._h, *(ptr(a) for a in self._parts(st)), None, None, ptr(F),
+                                           *(ptr(a) for a in self._parts(nxt)), int(self.nsteps[i]),
+                                           int(self.stride[i]), ptr(keep), s), "vd_sw_step_keep")
         if self.final_obs >= 0:
             self.H.misfit_dev(self.final_obs, self._surface(self.states[-1]), jo=J, inv_sigma_r2=inv_r2)
 
@@ -170,11 +193,12 @@ class SwFourDVar:
             lh = self._parts(lam)[2].view(nl, m.nx, m.ny)
             lh.add_(self.adssh.unsqueeze(0))
             self.adssh.zero_()
-            st = self.states[i]
             F = self.Fh[self.seg_basis[i]]
-            _lib.check(lib.vd_sw_step_adj(m._h, *(ptr(a) for a in self._parts(st)), None, None, ptr(F),
-                                          *(ptr(a) for a in self._parts(lam)), *(ptr(a) for a in self._parts(lam2)),
-                                          None, None, ptr(self.aF), int(self.nsteps[i]), s), "vd_sw_step_adj")
+            keep = self.keep[int(self.keep_off[i]):int(self.keep_off[i + 1])]
+            _lib.check(lib.vd_sw_step_adj_kept(m._h, None, None, ptr(F), ptr(keep), int(self.stride[i]),
+                                               *(ptr(a) for a in self._parts(lam)),
+                                               *(ptr(a) for a in self._parts(lam2)), None, None, ptr(self.aF),
+                                               int(self.nsteps[i]), s), "vd_sw_step_adj_kept")
             lam, lam2 = lam2, lam
             # Fh = F / 86400 on the top layer: adF += aFh[0] / 86400
             self.adF.add_(self.aF[0], alpha=1.0 / 86400.0)

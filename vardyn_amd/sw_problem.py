@@ -7,8 +7,8 @@ The workload of the 3-layer configuration: 256 x 256 cells of 8 km on a beta
 plane (f = 1e-4 + 1.6e-11 y), layers 500 / 1000 / 3000 m, reduced gravities
 0.02 (surface) / 0.02 / 0.01 m/s^2, dt = 120 s, a 10-day window; the control
 is the SSH forcing of the top layer in a wavelet basis of 4 scales
-(wavelengths 50-400 km, 9 directions) with 5-day time windows (47,952
-coefficients); 30 seeded nadir tracks with sigma = 0.02 m.
+(wavelengths 50-400 km, 9 directions) with 5-day time windows (55,404
+coefficients on this grid); 30 seeded nadir tracks with sigma = 0.02 m.
 
 Twin experiment: the initial state is a known balanced eddy field (the same
 for truth and analysis); the truth is forced by a seeded coefficient vector
@@ -152,7 +152,8 @@ def _forced_run(model, basis, x0, X, steps, case: SwCase, n_steps):
     return out
 
 
-def build_sw_problem(case: SwCase | None = None, device=None, with_engine=True, prec=False) -> SwProblem:
+def build_sw_problem(case: SwCase | None = None, device=None, with_engine=True, prec=False,
+                     tape="auto") -> SwProblem:
     case = SwCase() if case is None else case
     dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
     n_steps = case.n_steps
@@ -183,5 +184,6 @@ def build_sw_problem(case: SwCase | None = None, device=None, with_engine=True, 
     cps = checkpoint_schedule(n_steps + 1, case.checkpoint_steps, lambda i: H.is_obs(T[i]))
     pb = SwProblem(case, grid, model, basis, H, T, cps, time_basis, x0, X_true, Q, dict_obs, obs)
     if with_engine:
-        pb.engine = SwFourDVar(model, H, cps, T, basis, x0, sigma_b=Q, prec=prec, dtbasis=case.checkpoint_steps)
+        pb.engine = SwFourDVar(model, H, cps, T, basis, x0, sigma_b=Q, prec=prec, dtbasis=case.checkpoint_steps,
+                               tape=tape)
     return pb
