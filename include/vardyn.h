@@ -389,10 +389,22 @@ typedef struct vd_merge_params {
   const int32_t* snx;      /* nsub */
   const double* slat;      /* sum of sny: latitude axes, strictly ascending each */
   const double* slon;      /* sum of snx: longitude axes, strictly ascending each */
+  const int32_t* tri_slot; /* nsub or null: k >= 0 marks subwindow s as an irregular grid
+                              (run_assimilation.py:436-446, LinearNDInterpolator) whose
+                              triangle tables are block k of tri_idx / tri_w; -1 a
+                              rectilinear grid (its axes in slat / slon; an irregular
+                              subwindow's axes are not read) */
+  const int32_t* tri_idx;  /* (irregular subwindows) x ny*nx x 3: the vertices (flat
+                              indices into the subwindow's sny x snx points) of the
+                              Delaunay triangle holding each target point, -1 outside
+                              the convex hull */
+  const double* tri_w;     /* the same shape: the barycentric weights (c0, c1, c2) */
 } vd_merge_params;
 
 /* Upload the grids (host arrays) and build the interpolation tables and
- * weights on the current device. */
+ * weights on the current device.  An irregular subwindow's value at a
+ * target point is ((0 + c0 v0) + c1 v1) + c2 v2 over its triangle, in
+ * LinearNDInterpolator's order. */
 int vd_merge_create(const vd_merge_params* params, vd_merge** merge);
 int vd_merge_destroy(vd_merge* merge);
 /* Total subwindow points (sum of sny * snx). */

@@ -1,7 +1,8 @@
 """GPU subwindow / time-window merge against the oracle
 (run_assimilation.py:351-446, 537-607, 732-830): weights, weight sums and
-merged fields within 1e-13 relative, with the oracle's NaN pattern; the
-blend and its weights."""
+merged fields within 1e-13 relative, with the oracle's NaN pattern, for
+regular and irregular (Delaunay) subwindow grids; the blend and its
+weights."""
 import numpy as np
 import pytest
 import torch
@@ -61,6 +62,53 @@ def test_merge_matches_oracle(seed, nsub, T, irregular, masked, nan_fields):
     out = m.merge([torch.from_numpy(f).cuda() for f in fields]).cpu().numpy()
     for t in range(T):
         ref = merge_date(lon, lat, subs, [f[t] for f in fields], w_ref, ws_ref, mask=mask)
+        _close(out[t], ref)
+
+
+def _curvilinear(rng, lon0, lat0, nx, ny, d=0.35):
+    """An irregular (GRID_CAR) subwindow grid: a sheared, jittered lattice."""
+    i, j = np.meshgrid(np.arange(nx), np.arange(ny))
+    lo = lon0 + d * (i + 0.2 * j) + 0.04 * rng.standard_normal(i.shape)
+    la = lat0 + d * (j - 0.15 * i) + 0.04 * rng.standard_normal(i.shape)
+    return lo, la
+
+
+@pytest.mark.parametrize("seed,kinds,T,masked,nan_fields", [
+    (11, "c", 4, False, False),
+    (12, "cgc", 9, True, True),
+    (13, "ccgcg", 13, False, True),
+])
+def test_irregular_subwindows_match_oracle(seed, kinds, T, masked, nan_fields):
+    """Irregular subwindows (run_assimilation.py:436-446: LinearNDInterpolator
+    over the subwindow's points), alone and mixed with regular ones."""
+    from vardyn_amd.merge import SubwindowMerge
+    rng = np.random.default_rng(seed)
+    lon, lat = np.meshgrid(np.linspace(-4.0, 6.0, 61), np.linspace(30.0, 38.0, 47))
+    subs, geo = [], []
+    for k in kinds:
+        lo0, la0 = rng.uniform(-5.0, 2.0), rng.uniform(29.0, 34.0)
+        if k == "c":
+            subs.append(_curvilinear(rng, lo0, la0, int(rng.integers(8, 20)), int(rng.integers(7, 16))))
+        else:
+            subs.append((np.linspace(lo0, lo0 + 5.0, int(rng.integers(6, 25))),
+                         np.linspace(la0, la0 + 4.0, int(rng.integers(5, 20)))))
+        geo.append(k == "g")
+    mask = (rng.random(lon.shape) < 0.05) if masked else None
+    w_ref, ws_ref = weights_map(lon, lat, subs, geo=geo)
+    assert sum(np.isfinite(w).sum() for w in w_ref) > 200
+    m = SubwindowMerge(lon, lat, subs, mask=mask, geo_grid=geo)
+    _close(m.weights_space.cpu().numpy(), np.stack(w_ref))
+    _close(m.weights_space_sum.cpu().numpy(), ws_ref)
+    fields = []
+    for (lo, la), g in zip(subs, geo):
+        shape = (la.size, lo.size) if g else lo.shape
+        f = rng.standard_normal((T,) + shape)
+        if nan_fields:
+            f[rng.random(f.shape) < 0.03] = np.nan
+        fields.append(f)
+    out = m.merge([torch.from_numpy(f).cuda() for f in fields]).cpu().numpy()
+    for t in range(T):
+        ref = merge_date(lon, lat, subs, [f[t] for f in fields], w_ref, ws_ref, mask=mask, geo=geo)
         _close(out[t], ref)
 
 

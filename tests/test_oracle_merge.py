@@ -90,3 +90,32 @@ def test_time_blend_weights():
     assert time_blend(5.0, 5.0, 5.0) == (0.5, 0.5)
     w1, w2 = time_blend(12.5, 10.0, 20.0)
     assert abs(w2 - np.sin(np.pi * 0.25 / 2) ** 2) < 1e-15 and abs(w1 + w2 - 1) < 1e-15
+
+
+def _irregular(seed, lon0, lat0, nx, ny, d=0.4):
+    """A curvilinear subwindow grid: a rotated, sheared and jittered lattice."""
+    rng = np.random.default_rng(seed)
+    i, j = np.meshgrid(np.arange(nx), np.arange(ny))
+    lo = lon0 + d * (i + 0.2 * j) + 0.05 * rng.standard_normal(i.shape)
+    la = lat0 + d * (j - 0.15 * i) + 0.05 * rng.standard_normal(i.shape)
+    return lo, la
+
+
+def test_irregular_branch_linear_exact_and_hull():
+    from scipy.spatial import ConvexHull
+    lon, lat = _target()
+    subs = [_irregular(1, 0.5, 40.5, 14, 12), _irregular(2, 4.0, 43.0, 11, 10)]
+    geo = [False, False]
+    lin = lambda lo, la: 0.3 + 2.0 * lo - 0.7 * la
+    w, ws = weights_map(lon, lat, subs, geo=geo)
+    out = merge_date(lon, lat, subs, [lin(lo, la) for lo, la in subs], w, ws, geo=geo)
+    ok = ws > 0
+    assert ok.sum() > 50
+    assert np.allclose(out[ok], lin(lon, lat)[ok], rtol=0, atol=1e-12)
+    for (lo, la), wi in zip(subs, w):
+        hull = ConvexHull(np.column_stack([lo.ravel(), la.ravel()]))
+        pts = np.column_stack([lon.ravel(), lat.ravel()])
+        dist = pts @ hull.equations[:, :2].T + hull.equations[:, 2]
+        inside = np.all(dist <= 1e-12, axis=1).reshape(lon.shape)
+        outside = np.any(dist > 1e-9, axis=1).reshape(lon.shape)
+        assert np.all(~np.isnan(wi[inside])) and np.all(np.isnan(wi[outside]))

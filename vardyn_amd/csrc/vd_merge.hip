@@ -3,7 +3,9 @@
 // mapping/src/run_assimilation.py:351-412, 537-607, 732-830).
 //
 // Each subwindow s has a rectilinear grid (ascending 1-D latitude and
-// longitude axes) and carries, per date, a field on it.  Per target point p
+// longitude axes), or an irregular 2-D grid with host-built Delaunay
+// triangle tables (the reference's LinearNDInterpolator branch), and
+// carries, per date, a field on it.  Per target point p
 // and subwindow s the merge uses scipy's RegularGridInterpolator (linear,
 // bounds_error False, fill NaN) semantics: the cell [a_i, a_{i+1}) holding
 // the coordinate on each axis (the last cell closed on the right), NaN
@@ -43,7 +45,21 @@ struct MergeTables {
   double2* frac;          // nsub * np: (fy, fx)
   double* w;              // nsub * np: interpolated weight (NaN outside)
   double* wsum;           // np
+  const int* tri_slot;    // nsub: block of an irregular subwindow's triangle tables, -1 rectilinear
+  const int* tri_idx;     // blocks of np * 3 vertices (-1 outside)
+  const double* tri_w;    // blocks of np * 3 barycentric weights
 };
+
+// LinearNDInterpolator's evaluation on one triangle (scipy interpnd:
+// out = 0; out += c_j v_j for j = 0, 1, 2), unfused like its C loop
+__device__ __forceinline__ double mg_tri(const double* v, const int* id, const double* c) {
+#pragma clang fp contract(off)
+  double out = 0.0;
+  out = out + c[0] * v[id[0]];
+  out = out + c[1] * v[id[1]];
+  out = out + c[2] * v[id[2]];
+  return out;
+}
 
 // scipy's interval search on an ascending axis a[0..n-1]: the i with
 // a[i] <= v < a[i+1], n-2 for v == a[n-1], -1 outside or for NaN
@@ -78,6 +94,23 @@ __global__ void k_merge_setup(MergeTables M) {
        e += (long long)gridDim.x * blockDim.x) {
     const int s = (int)(e / np), p = (int)(e % np);
     const int ny = M.sny[s], nx = M.snx[s];
+    if (M.tri_slot && M.tri_slot[s] >= 0) {
+      // irregular grid: the window (ones for a single subwindow) interpolated on the triangle
+      const size_t t3 = ((size_t)M.tri_slot[s] * np + p) * 3;
+      const int* id = M.tri_idx + t3;
+      M.cell[e] = make_int2(-1, -1);
+      M.frac[e] = make_double2(0, 0);
+      if (id[0] < 0) {
+        M.w[e] = NAN;
+        continue;
+      }
+      double wv[3];
+      const int loc[3] = {0, 1, 2};
+      for (int k = 0; k < 3; ++k)
+        wv[k] = M.nsub == 1 ? 1.0 : mg_hann(id[k] / nx, ny) * mg_hann(id[k] % nx, nx);
+      M.w[e] = mg_tri(wv, loc, M.tri_w + t3);
+      continue;
+    }
     const double* ay = M.slat + M.yoff[s];
     const double* ax = M.slon + M.xoff[s];
     const double la = M.lat[p], lo = M.lon[p];
@@ -127,6 +160,22 @@ __global__ __launch_bounds__(256) void k_merge_apply(MergeTables M, int T,
   const double ws = M.wsum[p];
   for (int s = 0; s < M.nsub; ++s) {
     const size_t e = (size_t)s * np + p;
+    if (M.tri_slot && M.tri_slot[s] >= 0) {
+      const size_t t3 = ((size_t)M.tri_slot[s] * np + p) * 3;
+      const int* id = M.tri_idx + t3;
+      if (id[0] < 0) continue;
+      const double* c = M.tri_w + t3;
+      const double w = M.w[e];
+      const long long plane = (long long)M.sny[s] * M.snx[s];
+      const double* base = fields + (long long)T * M.foff[s];
+#pragma unroll
+      for (int t = 0; t < kMgTB; ++t) {
+        if (t >= nt) break;
+        const double val = mg_tri(base + (long long)(t0 + t) * plane, id, c);
+        if (!isnan(val)) acc[t] += w * val / ws;
+      }
+      continue;
+    }
     const int2 c = M.cell[e];
     if (c.x < 0) continue;
     const double2 f = M.frac[e];
@@ -203,17 +252,32 @@ int vd_merge_create(const vd_merge_params* params, vd_merge** out) {
   std::vector<int> yoff(P.nsub), xoff(P.nsub);
   std::vector<long long> foff(P.nsub);
   long long ny_tot = 0, nx_tot = 0, tot = 0;
+  int n_irr = 0;
+  const size_t np_target = (size_t)P.ny * P.nx;
   for (int s = 0; s < P.nsub; ++s) {
     VD_REQUIRE(P.sny[s] >= 2 && P.snx[s] >= 2, VD_ERR_INVALID_ARG,
                "merge: a subwindow axis needs at least 2 points");
+    const bool irr = P.tri_slot && P.tri_slot[s] >= 0;
     const double* ay = P.slat + ny_tot;
     const double* ax = P.slon + nx_tot;
-    for (int i = 0; i + 1 < P.sny[s]; ++i)
-      VD_REQUIRE(ay[i] < ay[i + 1], VD_ERR_INVALID_ARG,
-                 "merge: subwindow latitudes must be strictly ascending");
-    for (int j = 0; j + 1 < P.snx[s]; ++j)
-      VD_REQUIRE(ax[j] < ax[j + 1], VD_ERR_INVALID_ARG,
-                 "merge: subwindow longitudes must be strictly ascending");
+    if (irr) {
+      VD_REQUIRE(P.tri_slot[s] == n_irr, VD_ERR_INVALID_ARG,
+                 "merge: irregular subwindows take triangle blocks 0, 1, ... in order");
+      VD_REQUIRE(P.tri_idx && P.tri_w, VD_ERR_INVALID_ARG, "merge: null triangle tables");
+      const long long npts = (long long)P.sny[s] * P.snx[s];
+      const int32_t* id = P.tri_idx + (size_t)n_irr * np_target * 3;
+      for (size_t q = 0; q < np_target * 3; ++q)
+        VD_REQUIRE(id[q] >= -1 && id[q] < npts && ((q % 3 == 0) || (id[q] < 0) == (id[q - q % 3] < 0)),
+                   VD_ERR_INVALID_ARG, "merge: triangle vertex outside its subwindow");
+      ++n_irr;
+    } else {
+      for (int i = 0; i + 1 < P.sny[s]; ++i)
+        VD_REQUIRE(ay[i] < ay[i + 1], VD_ERR_INVALID_ARG,
+                   "merge: subwindow latitudes must be strictly ascending");
+      for (int j = 0; j + 1 < P.snx[s]; ++j)
+        VD_REQUIRE(ax[j] < ax[j + 1], VD_ERR_INVALID_ARG,
+                   "merge: subwindow longitudes must be strictly ascending");
+    }
     yoff[s] = (int)ny_tot;
     xoff[s] = (int)nx_tot;
     foff[s] = tot;
@@ -239,7 +303,10 @@ int vd_merge_create(const vd_merge_params* params, vd_merge** out) {
       (rc = mg_upload(m, P.slat, (size_t)ny_tot, &M.slat)) ||
       (rc = mg_upload(m, P.slon, (size_t)nx_tot, &M.slon)) ||
       (rc = mg_alloc(m, np * P.nsub, &M.cell)) || (rc = mg_alloc(m, np * P.nsub, &M.frac)) ||
-      (rc = mg_alloc(m, np * P.nsub, &M.w)) || (rc = mg_alloc(m, np, &M.wsum))) {
+      (rc = mg_alloc(m, np * P.nsub, &M.w)) || (rc = mg_alloc(m, np, &M.wsum)) ||
+      (rc = mg_upload(m, (const int*)P.tri_slot, n_irr ? (size_t)P.nsub : 0, &M.tri_slot)) ||
+      (rc = mg_upload(m, (const int*)P.tri_idx, (size_t)n_irr * np * 3, &M.tri_idx)) ||
+      (rc = mg_upload(m, P.tri_w, (size_t)n_irr * np * 3, &M.tri_w))) {
     vd_merge_destroy(m);
     return rc;
   }

@@ -21,8 +21,11 @@ assimilation per piece and merges the outputs date by date:
 
 Here the outputs come in as device tensors instead of netCDF files, and a
 batch of dates is merged in one call.  Scope: regular (``GRID_GEO``)
-subwindow grids with ascending axes and no longitude wrap (the reference's
-Delaunay branches for irregular grids and split longitudes are not built);
+subwindow grids with ascending axes and irregular (``GRID_CAR``, ``geo_grid``
+False) grids through the reference's Delaunay interpolation (the triangles
+and barycentric weights found on the host as LinearNDInterpolator finds
+them, applied on the GPU), no longitude wrap (the reference's split-longitude
+branch is not built);
 NaN gaps are not filled first (``kernel`` None; the reference's astropy fill
 is not available); H-grid variables (the C-grid averaging of U/V variables
 belongs to the SW model).
@@ -43,7 +46,36 @@ from .qg import current_stream_ptr, ptr
 class MergeParams(C.Structure):
     _fields_ = [("ny", C.c_int), ("nx", C.c_int), ("lon", C.c_void_p), ("lat", C.c_void_p),
                 ("mask", C.c_void_p), ("nsub", C.c_int), ("sny", C.c_void_p), ("snx", C.c_void_p),
-                ("slat", C.c_void_p), ("slon", C.c_void_p)]
+                ("slat", C.c_void_p), ("slon", C.c_void_p), ("tri_slot", C.c_void_p),
+                ("tri_idx", C.c_void_p), ("tri_w", C.c_void_p)]
+
+
+def delaunay_tables(lon_in, lat_in, lon_out, lat_out):
+    """Triangle of every target point in the Delaunay triangulation of the
+    points (lon, lat) of an irregular subwindow, and its barycentric weights,
+    as ``LinearNDInterpolator(points, values)(pts_out)`` finds and computes
+    them (run_assimilation.py:436-446; scipy's find_simplex with its default
+    tolerance, c_i = sum_j T_ij (x_j - r_j), c_2 = (1 - c_0) - c_1).
+    Returns (idx (np, 3) int32, -1 outside the hull; w (np, 3) float64)."""
+    from scipy.spatial import Delaunay
+    points = np.column_stack([np.ravel(lon_in), np.ravel(lat_in)])
+    pts = np.column_stack([np.ravel(lon_out), np.ravel(lat_out)])
+    tri = Delaunay(points)
+    simplex = tri.find_simplex(pts)
+    inside = simplex >= 0
+    idx = np.full((pts.shape[0], 3), -1, dtype=np.int32)
+    w = np.zeros((pts.shape[0], 3))
+    sx = simplex[inside]
+    idx[inside] = tri.simplices[sx]
+    T = tri.transform[sx]
+    d0 = pts[inside, 0] - T[:, 2, 0]
+    d1 = pts[inside, 1] - T[:, 2, 1]
+    c0 = T[:, 0, 0] * d0 + T[:, 0, 1] * d1
+    c1 = T[:, 1, 0] * d0 + T[:, 1, 1] * d1
+    w[inside, 0] = c0
+    w[inside, 1] = c1
+    w[inside, 2] = (1.0 - c0) - c1
+    return idx, w
 
 
 def _axes(lon, lat):
@@ -63,7 +95,7 @@ class SubwindowMerge:
     tables and weights live on ``device``.  ``merge`` then plays
     ``merge_output_date`` for a batch of dates."""
 
-    def __init__(self, lon, lat, subwindows, mask=None, device="cuda"):
+    def __init__(self, lon, lat, subwindows, mask=None, device="cuda", geo_grid=None):
         lon = np.ascontiguousarray(np.asarray(lon, dtype=np.float64))
         lat = np.ascontiguousarray(np.asarray(lat, dtype=np.float64))
         if lon.ndim != 2 or lon.shape != lat.shape:
@@ -73,11 +105,32 @@ class SubwindowMerge:
         if self.device.type == "cuda" and self.device.index is None and torch.cuda.is_available():
             # tensors report their device with its index: compare against that
             self.device = torch.device("cuda", torch.cuda.current_device())
-        axes = [_axes(lo, la) for lo, la in subwindows]
-        if not axes:
+        if not subwindows:
             raise VardynError("no subwindow")
-        self.shapes = [(la.size, lo.size) for lo, la in axes]
+        geo = [True] * len(subwindows) if geo_grid is None else [bool(g) for g in geo_grid]
+        if len(geo) != len(subwindows):
+            raise VardynError("geo_grid needs one flag per subwindow")
+        axes, self.shapes, tri_idx, tri_w = [], [], [], []
+        tri_slot = np.full(len(subwindows), -1, dtype=np.int32)
+        for s, ((lo, la), g) in enumerate(zip(subwindows, geo)):
+            if g:
+                ax = _axes(lo, la)
+                axes.append(ax)
+                self.shapes.append((ax[1].size, ax[0].size))
+                continue
+            # irregular grid (GRID_CAR): Delaunay triangles of its points
+            lo2, la2 = np.asarray(lo, dtype=np.float64), np.asarray(la, dtype=np.float64)
+            if lo2.ndim != 2 or lo2.shape != la2.shape:
+                raise VardynError("an irregular subwindow needs 2-D lon and lat of one shape")
+            i_, w_ = delaunay_tables(lo2, la2, lon, lat)
+            tri_slot[s] = len(tri_idx)
+            tri_idx.append(i_)
+            tri_w.append(w_)
+            self.shapes.append(lo2.shape)
+            # axes are not read for irregular subwindows: any ascending ones
+            axes.append((np.arange(lo2.shape[1], dtype=np.float64), np.arange(lo2.shape[0], dtype=np.float64)))
         self.nsub = len(axes)
+        self.geo_grid = geo
         sny = np.array([s[0] for s in self.shapes], dtype=np.int32)
         snx = np.array([s[1] for s in self.shapes], dtype=np.int32)
         slat = np.ascontiguousarray(np.concatenate([la for _, la in axes]))
@@ -89,9 +142,14 @@ class SubwindowMerge:
                 raise VardynError(f"mask shape {mask.shape}, target grid {(self.ny, self.nx)}")
             if np.any(mask):
                 m = np.ascontiguousarray(mask.astype(np.uint8))
+        tri = None
+        if tri_idx:
+            tri = (tri_slot, np.ascontiguousarray(np.concatenate(tri_idx)),
+                   np.ascontiguousarray(np.concatenate(tri_w)))
         p = MergeParams(self.ny, self.nx, lon.ctypes.data, lat.ctypes.data,
                         m.ctypes.data if m is not None else None, self.nsub,
-                        sny.ctypes.data, snx.ctypes.data, slat.ctypes.data, slon.ctypes.data)
+                        sny.ctypes.data, snx.ctypes.data, slat.ctypes.data, slon.ctypes.data,
+                        *((a.ctypes.data for a in tri) if tri is not None else (None, None, None)))
         h = C.c_void_p()
         with torch.cuda.device(self.device):
             _lib.call("vd_merge_create", C.byref(p), C.byref(h))
@@ -169,9 +227,9 @@ class SubwindowMerge:
         return self.merge_packed(packed, T, out)
 
 
-def compute_weights_map(lon, lat, subwindows, mask=None, device="cuda"):
+def compute_weights_map(lon, lat, subwindows, mask=None, device="cuda", geo_grid=None):
     """run_assimilation.py:351-412: the merger with its weight maps."""
-    return SubwindowMerge(lon, lat, subwindows, mask=mask, device=device)
+    return SubwindowMerge(lon, lat, subwindows, mask=mask, device=device, geo_grid=geo_grid)
 
 
 def time_window_weights(date, list_date_start, list_date_end):
