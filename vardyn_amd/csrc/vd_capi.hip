@@ -43,11 +43,13 @@ static int upload(const std::vector<double>& h, void** dptr) {
 // element (line, k) at ((line/32 * K/32 + k/32) * 64 + lane) * 16 + k%16 with
 // lane = line%32 + 32 (k%32 / 16), the 16 bytes lane `lane` of a wave loads
 // for its 32-line block and K step (lines and K are multiples of 32).
-// Exponents [par][lines].
+// Exponents [par][lines].  The planes hold split_line_alloc(lines) lines,
+// the ones past `lines` zero.
 template <typename Op>
 static int upload_split(int npar, int lines, int K, int ns, Op op, void** d8, int** de) {
-  std::vector<int8_t> pl((size_t)npar * ns * lines * K, 0);
-  std::vector<int> ex((size_t)npar * lines, 0);
+  const int LA = split_line_alloc(lines);
+  std::vector<int8_t> pl((size_t)npar * ns * LA * K, 0);
+  std::vector<int> ex((size_t)npar * LA, 0);
   const unsigned long long C = 0x8080808080808080ull >> (8 * (8 - ns));
   for (int par = 0; par < npar; ++par)
     for (int ln = 0; ln < lines; ++ln) {
@@ -55,13 +57,13 @@ static int upload_split(int npar, int lines, int K, int ns, Op op, void** d8, in
       for (int k = 0; k < K; ++k) m = std::max(m, std::fabs(op(par, ln, k)));
       if (m < 0x1p-900) continue;  // zero line: zero digits
       const int e = std::ilogb(m) + 3;
-      ex[(size_t)par * lines + ln] = e;
+      ex[(size_t)par * LA + ln] = e;
       for (int k = 0; k < K; ++k) {
         const long long Y = std::llrint(std::ldexp(op(par, ln, k), 8 * ns - e));
         const unsigned long long Z = ((unsigned long long)Y + C) ^ C;
         const size_t frag = ((size_t)(ln / 32) * (K / 32) + k / 32) * 64 + (ln % 32) + 32 * ((k % 32) / 16);
         for (int d = 0; d < ns; ++d)
-          pl[((size_t)par * ns + d) * lines * K + frag * 16 + k % 16] = (int8_t)(uint8_t)(Z >> (8 * (ns - 1 - d)));
+          pl[((size_t)par * ns + d) * LA * K + frag * 16 + k % 16] = (int8_t)(uint8_t)(Z >> (8 * (ns - 1 - d)));
       }
     }
   VD_HIP_TRY(hipMalloc(d8, pl.size()));
@@ -582,7 +584,8 @@ int vd_qg_plan_bytes(const vd_plan* h, size_t* bytes) {
   if (p.split_ns > 0) {
     const int npar = p.fold ? 2 : 1;
     const size_t ly = p.fold ? p.HY : p.MY, lx = p.fold ? p.HX : p.MX;
-    b += (size_t)npar * (p.split_ns * (ly * ly + lx * lx) + (ly + lx) * sizeof(int));
+    const size_t ay = split_line_alloc((int)ly), ax = split_line_alloc((int)lx);
+    b += (size_t)npar * (p.split_ns * (ay * ly + ax * lx) + (ay + ax) * sizeof(int));
   }
   if (p.has_beta) b += p.nf * p.es;
   if (p.has_bathy) b += p.nf * p.es;

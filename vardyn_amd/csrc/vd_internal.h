@@ -93,7 +93,8 @@ struct Plan {
 
   // split-product GEMMs on int8 MFMA (0 = off): digits per operand line,
   // digit planes [par][split_ns][lines][K] and line exponents [par][lines] of
-  // the sine operand of the left (S_y) and right (S_x) stages
+  // the sine operand of the left (S_y) and right (S_x) stages, lines
+  // allocated to split_line_alloc(sine lines) (zero lines past the product)
   int split_ns = 0;
   void* S8y = nullptr;
   void* S8x = nullptr;
@@ -184,6 +185,9 @@ void launch_split_stage(const Plan& p, int stage, const double* F, const double*
                         const double* hb, double* out, int mode, int cfg, hipStream_t s);
 int split_num_cfgs();
 int split_max_k();
+// Sine lines allocated per digit plane: a multiple of 64 (the zero lines past
+// a product's own let ragged groups run every member on 64-line tiles).
+inline int split_line_alloc(int lines) { return (lines + 63) / 64 * 64; }
 int split_group_build(Group& g);
 void launch_inversion_split_rg(const Group& g, int nact, const double* F, const double* Fsub,
                                const double* hb, double* out, int mode, hipStream_t s);

@@ -50,7 +50,8 @@ struct SplitArgs {
   const int8_t* S8;    // [par][NS] digit planes of the sine operand, fragment-major (lines x K each)
   const int* eS;       // [par][lines] line exponents
   size_t pstride;      // bytes between parity blocks of S8
-  int lines;           // sine lines per parity
+  int lines;           // sine lines per parity in the digit planes (split_line_alloc)
+  int nsl;             // sine lines of the product: output past them is not stored
   int ns;
 };
 
@@ -412,9 +413,10 @@ __device__ __forceinline__ void split_tile(const SplitArgs& sa, int z, int tile)
 #pragma unroll
         for (int b = 0; b < NT; ++b)
 #pragma unroll
-          for (int r = 0; r < 16; ++r)
-            store_out<double, KIND, MAP>(g, P, s0 + ws * WSL + a * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3),
-                                         d0 + wd * WDL + b * 32 + (lane & 31), value(a, b, r));
+          for (int r = 0; r < 16; ++r) {
+            const int sl = s0 + ws * WSL + a * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            if (sl < sa.nsl) store_out<double, KIND, MAP>(g, P, sl, d0 + wd * WDL + b * 32 + (lane & 31), value(a, b, r));
+          }
     }
   } else {
     constexpr int TP = BSL + 1;  // row pitch (doubles) of the transposed tile
@@ -445,12 +447,12 @@ __device__ __forceinline__ void split_tile(const SplitArgs& sa, int z, int tile)
 #pragma unroll
     for (int it = 0; it < NE; ++it) {
       const int e = tid + NTHR * it, dl = e / BSL, sl = e % BSL;
-      epi_store<double, KIND, MAP>(g, P, d0 + dl, s0 + sl, T[dl * TP + sl], in[it]);
+      if (s0 + sl < sa.nsl) epi_store<double, KIND, MAP>(g, P, d0 + dl, s0 + sl, T[dl * TP + sl], in[it]);
     }
 #pragma unroll
     for (int it = 0; it < NE; ++it) {
       const int e = tid + NTHR * it;
-      epi_ring<double, KIND, MAP>(g, P, d0 + e / BSL, s0 + e % BSL);
+      if (s0 + e % BSL < sa.nsl) epi_ring<double, KIND, MAP>(g, P, d0 + e / BSL, s0 + e % BSL);
     }
   }
 }
@@ -479,7 +481,8 @@ __global__ __launch_bounds__(256) void k_gemm_split_rg(RgCall<double> call, cons
   sa.g.Fsub = call.Fsub ? call.Fsub + fo : nullptr;
   sa.g.hb = call.hb ? call.hb + fo : nullptr;
   sa.g.out = call.out ? call.out + fo : nullptr;
-  split_tile<AOP, BOP, KIND, MAP, 32, 32, 1, 1, NS, 8>(sa, t.y, t.z);
+  // the tiles of configuration 2 (kRgSplitCfg below)
+  split_tile<AOP, BOP, KIND, MAP, 64, 32, 2, 1, NS, 8>(sa, t.y, t.z);
 }
 
 // Tile configurations of the split kernel: sine lines x data lines per
@@ -575,9 +578,10 @@ void launch_split_stage(const Plan& p, int stage, const double* F, const double*
   const bool left = stage == 0 || stage == 2;
   l.a.S8 = static_cast<const int8_t*>(left ? p.S8y : p.S8x);
   l.a.eS = left ? p.eSy : p.eSx;
-  l.a.lines = nsl;
+  l.a.lines = split_line_alloc(nsl);
+  l.a.nsl = nsl;
   l.a.ns = p.split_ns;
-  l.a.pstride = (size_t)p.split_ns * nsl * K;
+  l.a.pstride = (size_t)p.split_ns * l.a.lines * K;
   l.a.g.nz = nz;
   l.a.g.tm = nsl / kSplitCfgs[cfg].bsl;
   l.a.g.tn = ndl / kSplitCfgs[cfg].bdl;
@@ -587,8 +591,15 @@ void launch_split_stage(const Plan& p, int stage, const double* F, const double*
 }
 
 // Split-product GEMMs of a ragged group: one launch per inversion stage over
-// the 32 x 32 tiles (any shape) of every active member, K up to 256.
-static SplitArgs split_stage_args(const Plan& p, int stage, int cfg) {
+// the 64 sine x 32 data line tiles (configuration 2) of every active member,
+// K up to 256.  A member's sine lines run to split_line_alloc(lines), a
+// multiple of 64; the tiles' rows past the product's own multiply zero digit
+// lines, and the epilogues discard them.  (On 32 x 32 tiles every data line
+// was fetched and split once per 32 sine lines, and the four waves split K
+// four ways; 64 x 32 tiles halve both, for 6.7% more MFMA work on the
+// 48-problem set's shapes.)
+constexpr int kRgSplitCfg = 2;
+static SplitArgs split_stage_args(const Plan& p, int stage) {
   SplitArgs a;
   a.g = stage_args<double>(p, stage);
   int nsl, ndl, K, nz;
@@ -596,12 +607,13 @@ static SplitArgs split_stage_args(const Plan& p, int stage, int cfg) {
   const bool left = stage == 0 || stage == 2;
   a.S8 = static_cast<const int8_t*>(left ? p.S8y : p.S8x);
   a.eS = left ? p.eSy : p.eSx;
-  a.lines = nsl;
+  a.lines = split_line_alloc(nsl);
+  a.nsl = nsl;
   a.ns = p.split_ns;
-  a.pstride = (size_t)p.split_ns * nsl * K;
+  a.pstride = (size_t)p.split_ns * a.lines * K;
   a.g.nz = nz;
-  a.g.tm = nsl / kSplitCfgs[cfg].bsl;
-  a.g.tn = ndl / kSplitCfgs[cfg].bdl;
+  a.g.tm = a.lines / kSplitCfgs[kRgSplitCfg].bsl;
+  a.g.tn = ndl / kSplitCfgs[kRgSplitCfg].bdl;
   return a;
 }
 
@@ -616,9 +628,9 @@ int split_group_build(Group& g) {
       split_lines(p, stg, &nsl, &ndl, &K, &nz);
       VD_REQUIRE(K <= 256, VD_ERR_UNSUPPORTED,
                  "split-product groups need sine products with K up to 256 (interior sizes up to 512 folded)");
-      args[b].sa = split_stage_args(p, stg, 3);
+      args[b].sa = split_stage_args(p, stg);
       args[b].foff = g.foff[b];
-      const int ntile = (nsl / 32) * (ndl / 32);
+      const int ntile = args[b].sa.g.tm * args[b].sa.g.tn;
       for (int z = 0; z < nz; ++z)
         for (int t = 0; t < ntile; ++t) tiles.push_back(make_int4(b, z, t, 0));
       g.sprefix[stg][b + 1] = (int)tiles.size();
