@@ -644,7 +644,8 @@ def problem_set(args, steps=None, warmup=None, separate=True):
     t0 = time.time()
     pbs = [build_problem(c, precision=args.precision, control="ic", with_engine=False,
                          dst_split=args.problem_set_split) for c in cases]
-    eng = RaggedFourDVar(pbs).capture()
+    eng = RaggedFourDVar(pbs, autotune=not args.no_autotune).capture()
+    tuning = eng.tuning
     build_s = time.time() - t0
     log(f"problem set: {len(cases)} problems built and captured in {build_s:.1f}s, "
         f"tape {eng.tape_bytes / 2**30:.1f} GiB")
@@ -676,7 +677,8 @@ def problem_set(args, steps=None, warmup=None, separate=True):
                    "dst_split": args.problem_set_split, "control": "0.5 x truth initial SSH",
                    "grid_point_steps_per_evaluation": gps},
         "grid_point_steps_per_s": round(gps * steps / dt, 1),
-        "per_problem_latency_ms": round(ms, 2), "build_s": round(build_s, 1)}
+        "per_problem_latency_ms": round(ms, 2), "build_s": round(build_s, 1),
+        "group_tuning": tuning}
     if not separate:
         return out
     # the same problems as separate engines on several streams
@@ -718,7 +720,9 @@ def tiled(args):
               tile_overlap=ov)
     t0 = time.time()
     m = QgmWithTiles(**kw, engine="group")
-    log(f"tiled: {m.ntile} tiles of {ts}x{ts} (overlap {ov}) built in {time.time() - t0:.1f}s")
+    tuning = m.autotune() if not args.no_autotune else None
+    log(f"tiled: {m.ntile} tiles of {ts}x{ts} (overlap {ov}) built in {time.time() - t0:.1f}s, "
+        f"group tuning {tuning}")
     h0n = syn.gaussian_random_field(ny, nx, case.dx, 100e3, 0.1, seed=11)
     rng = np.random.default_rng(12)
     hbn = 0.01 * rng.standard_normal((ny, nx))
@@ -827,7 +831,7 @@ def tiled(args):
                    "dt_s": case.dt, "time_scheme": "rk2", "tile_size": ts, "tile_overlap": ov,
                    "tiles": m.ntile, "steps_per_call": n, "engine": "ragged group (shared launches)",
                    "hip_graph": True},
-        "ms_step": round(ms_f, 3), "ms_step_adj": round(ms_a, 3),
+        "ms_step": round(ms_f, 3), "ms_step_adj": round(ms_a, 3), "group_tuning": tuning,
         "roofline": roof, "stencil_roofline": st_roof,
         "tile_by_tile": {"value": round(ny * nx * k1 / (ms_one * 1e-3), 1),
                          "ms_step_plus_adj": round(ms_one, 2), "steps_per_call": k1,
@@ -1474,6 +1478,7 @@ def main():
                                                  "grid_point_steps_per_s", "build_s")}
         out["problem_set"]["workload"] = ps["config"]["workload"]
         out["problem_set"]["dst_split"] = ps["config"]["dst_split"]
+        out["problem_set"]["group_tuning"] = ps["group_tuning"]
         log(f"problem set: {out['problem_set']}")
     if not args.no_stencil:
         out["stencil_roofline"] = stencil_roofline(case, args.precision)

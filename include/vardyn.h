@@ -221,7 +221,8 @@ int vd_qg_adj_final(vd_plan* plan, const void* hcur, const void* lh, const void*
  * (ny_b, nx_b) array.  The calls act on members 0 .. nact-1, so a driver
  * orders the members by window length (longest first) and passes the number
  * still running.  Per member the arithmetic is exactly that of its own plan
- * with GEMM configuration 0 (32 x 32 tiles).
+ * with GEMM configuration 0 (32 x 32 tiles), except the fused inversion of
+ * equal-size small FP64 members (the same sums in another order).
  * ------------------------------------------------------------------------- */
 typedef struct vd_group vd_group;
 
@@ -262,6 +263,21 @@ int vd_group_adj_init(vd_group* group, int nact, const void* adh1, void* lh, voi
                       void* stream);
 int vd_group_adj_final(vd_group* group, int nact, const void* hcur, const void* lh,
                        const void* lq, void* adh0, void* stream);
+/* Tile shapes and inversion form of the group, picked by timing the
+ * alternatives on its own members (no reference counterpart: the problem-set
+ * engine's tuner).  chosen[0..5] receives the choices: [0..3] the split-product
+ * tile shape of each inversion stage (0: 64 sine x 32 data lines, 1: 32 x 32;
+ * -1 in groups without split products), [4] the adjoint-stage tile (0: 64 x 16
+ * points, 1: 32 x 32), [5] the inversion of equal-size small FP64 members as
+ * one fused launch (1) or as the four GEMM stages (0; -1 where the fused form
+ * does not apply).  Choices [0..4] leave every result bit for bit unchanged;
+ * [5] changes results within rounding.  Each candidate runs `reps` (1..1000)
+ * times in each of three rounds; the call allocates and frees scratch fields
+ * for the timing and synchronises the stream. */
+int vd_group_autotune(vd_group* group, int reps, int* chosen, void* stream);
+/* Apply choices in the form vd_group_autotune returns (for instance stored
+ * from an earlier run); a value that does not apply to the group is an error. */
+int vd_group_set_tuning(vd_group* group, const int* chosen);
 
 /* ---------------------------------------------------------------------------
  * Balanced-motion wavelet basis (Basis_bm, mapping/src/basis.py:472-936):

@@ -190,3 +190,53 @@ def test_small_member_fused_inversion(n):
         pz = p.pv2h(F[sl].view(ny, nx), None, None)
         wacc = acc[sl].view(ny, nx) + pz
         assert torch.linalg.vector_norm(acc2[sl].view(ny, nx) - wacc) <= 1e-13 * torch.linalg.vector_norm(wacc)
+
+
+def test_group_autotune_and_forced_adjoint_tiles(monkeypatch):
+    """The adjoint-stage tile shapes (64 x 16, 32 x 32) and the autotuned
+    choice give the same J and G bit for bit; inapplicable choices are
+    rejected."""
+    from vardyn_amd._lib import VardynError
+    monkeypatch.setenv("VARDYN_GEMM_CONFIG", "1")
+    pbs = [build_problem(c, precision="float64", with_engine=False) for c in cases("rk2")]
+    eng = RaggedFourDVar(pbs)
+    rng = np.random.default_rng(9)
+    X = torch.as_tensor(0.05 * rng.standard_normal(eng.X.numel()), device="cuda")
+    J0, G0 = (t.clone() for t in eng.run(X))
+    for tile in ("32x32", "64x16"):
+        eng.group.set_tuning({"split_tiles": [None] * 4, "adjoint_tile": tile,
+                              "fused_small_inversion": None})
+        J1, G1 = eng.run(X)
+        assert torch.equal(J0, J1) and torch.equal(G0, G1), tile
+    with pytest.raises(VardynError):
+        eng.group.set_tuning({"split_tiles": ["64x32"] * 4, "adjoint_tile": "64x16",
+                              "fused_small_inversion": None})
+    with pytest.raises(VardynError):
+        eng.group.set_tuning({"split_tiles": [None] * 4, "adjoint_tile": "64x16",
+                              "fused_small_inversion": True})
+    tuned = RaggedFourDVar(pbs, autotune=True)
+    assert tuned.tuning["split_tiles"] == [None] * 4 and tuned.tuning["fused_small_inversion"] is None
+    J2, G2 = tuned.run(X)
+    assert torch.equal(J0, J2) and torch.equal(G0, G2)
+
+
+def test_small_member_group_tuning():
+    """Equal-size small members: the fused inversion and the GEMM stages agree
+    within rounding, and the autotuned group picks one of them."""
+    from vardyn_amd.qg import QgPlan, ptr
+    from vardyn_amd.ragged import QgGroup
+    ny = nx = 34
+    plans = [QgPlan(ny, nx, 10e3, 9e3, 1200.0, 2.0 + 0.25 * k, 1e-4, time_scheme="rk2") for k in range(6)]
+    g = QgGroup(plans)
+    gen = torch.Generator(device="cuda").manual_seed(21)
+    F, hb = (torch.randn(g.offsets[-1], dtype=torch.float64, device="cuda", generator=gen) for _ in range(2))
+    outs = []
+    for fused in (True, False):
+        g.set_tuning({"split_tiles": [None] * 4, "adjoint_tile": "32x32", "fused_small_inversion": fused})
+        out = torch.empty_like(F)
+        g.inversion(len(plans), ptr(F), ptr(out), hb=ptr(hb))
+        outs.append(out)
+    torch.cuda.synchronize()
+    assert torch.linalg.vector_norm(outs[0] - outs[1]) <= 1e-13 * torch.linalg.vector_norm(outs[1])
+    t = g.autotune(reps=2)
+    assert t["fused_small_inversion"] in (True, False) and t["split_tiles"] == [None] * 4

@@ -133,6 +133,29 @@ def test_ragged_group_equals_single_plans():
         assert torch.equal(Gs[i], Gr[0]), f"member {i}: G differs"
 
 
+@pytest.mark.parametrize("shape", ["64x32", "32x32"])
+def test_ragged_group_split_tile_shapes_bit_identical(shape):
+    """Both split tile shapes of a ragged group (64 sine lines on zero-padded
+    digit planes, or 32) and the autotuned choice give every member the same
+    J and G, bit for bit."""
+    from test_gpu_ragged import cases
+    from vardyn_amd.problem import build_problem
+    from vardyn_amd.ragged import RaggedFourDVar
+    pbs = [build_problem(c, precision="float64", with_engine=False, dst_split=6) for c in cases("rk2")]
+    eng = RaggedFourDVar(pbs)
+    rng = np.random.default_rng(8)
+    X = torch.as_tensor(0.05 * rng.standard_normal(eng.X.numel()), device="cuda")
+    J0, G0 = (t.clone() for t in eng.run(X))
+    eng.group.set_tuning({"split_tiles": [shape] * 4, "adjoint_tile": "32x32",
+                          "fused_small_inversion": None})
+    J1, G1 = eng.run(X)
+    assert torch.equal(J0, J1) and torch.equal(G0, G1)
+    tuned = RaggedFourDVar(pbs, autotune=True)
+    assert all(t in ("64x32", "32x32") for t in tuned.tuning["split_tiles"])
+    J2, G2 = tuned.run(X)
+    assert torch.equal(J0, J2) and torch.equal(G0, G2)
+
+
 def test_group_rejects_mixed_split():
     from vardyn_amd.ragged import QgGroup
     orc, plan64, _, _, _ = make_pair(40, 48, "rk2", "float64")

@@ -90,6 +90,8 @@ SIGNATURES = {
     "vd_group_finish": (_I, [_VP, _I, _VP, _VP, _VP]),
     "vd_group_adj_init": (_I, [_VP, _I, _VP, _VP, _VP, _VP]),
     "vd_group_adj_final": (_I, [_VP, _I, _VP, _VP, _VP, _VP, _VP]),
+    "vd_group_autotune": (_I, [_VP, _I, C.POINTER(_I), _VP]),
+    "vd_group_set_tuning": (_I, [_VP, C.POINTER(_I)]),
     "vd_obs_misfit": (_I, [C.POINTER(ObsTable), _I, _VP, _VP, _VP, _VP, _D, _VP]),
     "vd_obs_adjoint": (_I, [C.POINTER(ObsTable), _I, _VP, _VP, _VP, _VP]),
     "vd_tiles_gather": (_I, [_I, _I, _I, _I, _I, _I, _VP, _VP, _VP, _VP, _VP, _VP]),

@@ -181,3 +181,111 @@ int vd_group_adj_final(vd_group* grp, int nact, const void* hcur, const void* lh
   VD_HIP_TRY(hipGetLastError());
   return VD_OK;
 }
+
+// Tile shapes and the fused small-member inversion of a group, picked by
+// timing the alternatives on the group's own members (every member active).
+// Decisions, each kept only where it applies:
+//   split-product groups: per inversion stage, 64 x 32 or 32 x 32 split tiles
+//     (chosen[0..3]: 0 or 1; -1 for other groups) -- the same bits either way;
+//   the adjoint stage: 64 x 16 or 32 x 32 point tiles (chosen[4]: 0 or 1) --
+//     the same bits either way;
+//   equal-size small FP64 members: the inversion as one fused launch
+//     (k_inv_small) or as the four GEMM stage launches (chosen[5]: 1 or 0; -1
+//     where the fused form does not apply) -- results within rounding of
+//     each other (the FP64 sums run in another order).
+// Each candidate runs `reps` times per round, three rounds, fastest round
+// kept.  The timing inputs are scratch fields of the group's layout filled
+// with a finite constant; the call allocates and frees them, and synchronises
+// the stream.
+template <typename T>
+static int group_autotune(Group& g, int reps, int* chosen, hipStream_t s) {
+  const size_t nel = g.foff[g.n];
+  T* buf = nullptr;
+  VD_HIP_TRY(hipMalloc((void**)&buf, 6 * nel * sizeof(T)));
+  int rc = VD_OK;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  auto done = [&](int r) {
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    hipFree(buf);
+    return r;
+  };
+  if (hipMemsetAsync(buf, 0x3F, 6 * nel * sizeof(T), s) != hipSuccess ||
+      hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    set_error("autotune setup failed");
+    return done(VD_ERR_HIP);
+  }
+  T *f0 = buf, *f1 = buf + nel, *f2 = buf + 2 * nel, *f3 = buf + 3 * nel, *f4 = buf + 4 * nel;
+  // fastest of three rounds of `reps` calls of each candidate; returns the index
+  auto pick = [&](int ncand, auto&& run) -> int {
+    float best = 1e30f;
+    int bi = 0;
+    for (int round = 0; round < 3; ++round)
+      for (int c = 0; c < ncand; ++c) {
+        run(c);  // warm
+        hipEventRecord(e0, s);
+        for (int r = 0; r < reps; ++r) run(c);
+        hipEventRecord(e1, s);
+        if (hipEventSynchronize(e1) != hipSuccess) rc = VD_ERR_HIP;
+        float ms = 0;
+        hipEventElapsedTime(&ms, e0, e1);
+        if (ms < best) {
+          best = ms;
+          bi = c;
+        }
+      }
+    return bi;
+  };
+  for (int st = 0; st < 4; ++st) chosen[st] = -1;
+  if constexpr (sizeof(T) == 8) {
+    if (g.split_ns > 0)
+      for (int st = 0; st < 4; ++st) {
+        g.split_shape[st] = pick(2, [&](int shape) {
+          launch_split_stage_rg(g, st, shape, g.n, f0, nullptr, f1, f3, INV_FWD, s);
+        });
+        chosen[st] = g.split_shape[st];
+      }
+  }
+  g.adj_narrow = pick(2, [&](int narrow) {
+                   g.adj_narrow = narrow != 0;
+                   launch_stage_adj_rg<T>(g, g.n, f0, f1, f2, nullptr, nullptr, T(1), T(0), T(0), nullptr,
+                                          nullptr, nullptr, f3, f4, s);
+                 }) != 0;
+  chosen[4] = g.adj_narrow ? 1 : 0;
+  chosen[5] = -1;
+  if (g.small) {
+    g.use_small = pick(2, [&](int fused) {
+                    g.use_small = fused == 0;
+                    launch_inversion_rg<T>(g, g.n, f0, nullptr, f1, f3, INV_FWD, s);
+                  }) == 0;
+    chosen[5] = g.use_small ? 1 : 0;
+  }
+  if (rc == VD_OK && hipGetLastError() != hipSuccess) rc = VD_ERR_HIP;
+  if (rc != VD_OK) set_error("autotune: a timed launch failed");
+  return done(rc);
+}
+
+int vd_group_autotune(vd_group* grp, int reps, int* chosen, void* stream) {
+  VD_REQUIRE(grp && chosen, VD_ERR_INVALID_ARG, "null argument");
+  VD_REQUIRE(reps > 0 && reps <= 1000, VD_ERR_INVALID_ARG, "reps must be in 1..1000");
+  Group& g = grp->g;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  return g.prec == VD_F64 ? group_autotune<double>(g, reps, chosen, s)
+                          : group_autotune<float>(g, reps, chosen, s);
+}
+
+int vd_group_set_tuning(vd_group* grp, const int* chosen) {
+  VD_REQUIRE(grp && chosen, VD_ERR_INVALID_ARG, "null argument");
+  Group& g = grp->g;
+  for (int st = 0; st < 4; ++st)
+    VD_REQUIRE(g.split_ns > 0 ? (chosen[st] == 0 || chosen[st] == 1) : chosen[st] == -1, VD_ERR_INVALID_ARG,
+               "chosen[0..3] must be 0 or 1 in a split-product group and -1 otherwise");
+  VD_REQUIRE(chosen[4] == 0 || chosen[4] == 1, VD_ERR_INVALID_ARG, "chosen[4] must be 0 or 1");
+  VD_REQUIRE(g.small ? (chosen[5] == 0 || chosen[5] == 1) : chosen[5] == -1, VD_ERR_INVALID_ARG,
+             "chosen[5] must be 0 or 1 where the fused small-member inversion applies and -1 otherwise");
+  for (int st = 0; st < 4; ++st)
+    if (g.split_ns > 0) g.split_shape[st] = chosen[st];
+  g.adj_narrow = chosen[4] == 1;
+  if (g.small) g.use_small = chosen[5] == 1;
+  return VD_OK;
+}

@@ -135,22 +135,28 @@ struct Group {
   void* mem = nullptr;        // RgMember<T>[n]
   int4* stage_tiles = nullptr;  // (member, tile row, tile column) of 64 x 16 point tiles
   std::vector<int> stage_prefix;
-  int4* adj_tiles = nullptr;    // the same for the adjoint stage's tiles: 64 x 16,
-  bool adj_narrow = false;      // or 32 x 32 when every member is at most 32 points wide
-  bool small = false;           // equal-size small FP64 members: fused inversion per member
+  // the same for the adjoint stage's tiles, both shapes: [0] 64 x 16, [1] 32 x 32;
+  // adj_narrow picks [1] (by default when every member is at most 32 points
+  // wide, or by vd_group_autotune)
+  int4* adj_tiles[2] = {nullptr, nullptr};
+  std::vector<int> adj_prefix[2];
+  bool adj_narrow = false;
+  bool small = false;           // equal-size small FP64 members: the fused inversion applies
+  bool use_small = false;       // ... and is used (the default; vd_group_autotune may turn it off)
   void* small_mem = nullptr;    // RgSmall[n]
-  std::vector<int> adj_prefix;
   void* gargs[4] = {nullptr, nullptr, nullptr, nullptr};  // RgGemm<T>[n] per inversion stage
   // (member, product, tile) of the 32 x 32 GEMM tiles per stage and K-split
   // class (4, 2, 1 K groups: configuration 0's choice for each member)
   int4* gtiles[4][3] = {};
   std::vector<int> gprefix[4][3];
   // split-product groups (split_ns > 0): per-stage member descriptors and
-  // (member, product, 32 x 32 tile) tables
+  // (member, product, tile) tables for both tile shapes, [0] 64 sine x 32 data
+  // lines and [1] 32 x 32, and the shape each stage uses (split_shape)
   int split_ns = 0;
-  void* sargs[4] = {nullptr, nullptr, nullptr, nullptr};
-  int4* stiles[4] = {nullptr, nullptr, nullptr, nullptr};
-  std::vector<int> sprefix[4];
+  void* sargs[2][4] = {};
+  int4* stiles[2][4] = {};
+  std::vector<int> sprefix[2][4];
+  int split_shape[4] = {0, 0, 0, 0};
 };
 
 // Named scratch fields inside Plan::fld.
@@ -191,6 +197,9 @@ inline int split_line_alloc(int lines) { return (lines + 63) / 64 * 64; }
 int split_group_build(Group& g);
 void launch_inversion_split_rg(const Group& g, int nact, const double* F, const double* Fsub,
                                const double* hb, double* out, int mode, hipStream_t s);
+// One split-product stage of a ragged group with tile shape `shape` (0: 64 x 32, 1: 32 x 32).
+void launch_split_stage_rg(const Group& g, int stage, int shape, int nact, const double* F,
+                           const double* Fsub, const double* hb, double* out, int mode, hipStream_t s);
 bool split_cfg_fits(const Plan& p, int stage, int c);
 int split_default_cfg(const Plan& p, int stage);
 

@@ -1414,16 +1414,17 @@ template <typename T>
 void launch_stage_adj_rg(const Group& g, int nact, const T* h, const T* q, const T* s0,
                          const T* s1, const T* s2, T a0, T a1, T a2, const T* acc0,
                          const T* acc1, const T* acc2, T* lq_out, T* lh_out, hipStream_t s) {
-  const int nb = g.adj_prefix[nact];
+  const int sh = g.adj_narrow ? 1 : 0;
+  const int nb = g.adj_prefix[sh][nact];
   if (nb == 0) return;
   if (g.adj_narrow)
     hipLaunchKernelGGL((k_adj_fused_rg<T, kAXN, kAYN>), dim3(nb), dim3(kAXN, kAYN), 0, s,
-                       static_cast<const RgMember<T>*>(g.mem), g.adj_tiles, h, q, s0, s1, s2, a0, a1,
+                       static_cast<const RgMember<T>*>(g.mem), g.adj_tiles[1], h, q, s0, s1, s2, a0, a1,
                        a2, acc0, acc1, acc2, lq_out, lh_out);
   else
-  hipLaunchKernelGGL((k_adj_fused_rg<T, kAX, kAY>), dim3(nb), dim3(kAX, kAY), 0, s,
-                     static_cast<const RgMember<T>*>(g.mem), g.adj_tiles, h, q, s0, s1, s2, a0, a1,
-                     a2, acc0, acc1, acc2, lq_out, lh_out);
+    hipLaunchKernelGGL((k_adj_fused_rg<T, kAX, kAY>), dim3(nb), dim3(kAX, kAY), 0, s,
+                       static_cast<const RgMember<T>*>(g.mem), g.adj_tiles[0], h, q, s0, s1, s2, a0, a1,
+                       a2, acc0, acc1, acc2, lq_out, lh_out);
 }
 
 // Helmholtz inversion of a small member in one workgroup (ragged groups of
@@ -1531,7 +1532,7 @@ void launch_inversion_rg(const Group& g, int nact, const T* F, const T* Fsub, co
       launch_inversion_split_rg(g, nact, F, Fsub, hb, out, mode, s);
       return;
     }
-    if (g.small && mode != INV_TL) {
+    if (g.use_small && mode != INV_TL) {
       if (nact == 0) return;
       const Plan& p0 = *g.plans[0];
       const size_t lds = (size_t)(2 * p0.my * p0.mx + p0.my * p0.my + p0.mx * p0.mx) * 8;
@@ -1603,10 +1604,14 @@ int group_build(Group& g) {
   double total = 0;
   for (int b = 0; b < n; ++b) total += (double)g.plans[b]->nf * sizeof(T) * 4;
   const bool band = total > 256.0 * (1 << 20);
+  // adjoint-stage tiles: 32 x 32 by default when every member is at most 32
+  // points wide (a 64-wide tile would leave half its threads idle)
   g.adj_narrow = true;
   for (int b = 0; b < n; ++b) g.adj_narrow = g.adj_narrow && g.plans[b]->nx <= kAXN;
+  std::vector<int4> at2;
   g.stage_prefix.assign(n + 1, 0);
-  g.adj_prefix.assign(n + 1, 0);
+  g.adj_prefix[0].assign(n + 1, 0);
+  g.adj_prefix[1].assign(n + 1, 0);
   for (int b = 0; b < n; ++b) {
     const Plan& p = *g.plans[b];
     g.foff[b + 1] = g.foff[b] + p.nf;
@@ -1614,15 +1619,17 @@ int group_build(Group& g) {
     mem[b].foff = g.foff[b];
     mem[b].mdt = p.has_mdt ? static_cast<const T*>(p.mdt) : nullptr;
     rg_tiles(b, (p.nx + kSX - 1) / kSX, (p.ny + kSY - 1) / kSY, band, st);
-    const int ax = g.adj_narrow ? kAXN : kAX, ay = g.adj_narrow ? kAYN : kAY;
-    rg_tiles(b, (p.nx + ax - 1) / ax, (p.ny + ay - 1) / ay, band, at);
+    rg_tiles(b, (p.nx + kAX - 1) / kAX, (p.ny + kAY - 1) / kAY, band, at);
+    rg_tiles(b, (p.nx + kAXN - 1) / kAXN, (p.ny + kAYN - 1) / kAYN, band, at2);
     g.stage_prefix[b + 1] = (int)st.size();
-    g.adj_prefix[b + 1] = (int)at.size();
+    g.adj_prefix[0][b + 1] = (int)at.size();
+    g.adj_prefix[1][b + 1] = (int)at2.size();
   }
   int rc;
   if ((rc = upload_vec(mem, &g.mem)) != VD_OK) return rc;
   if ((rc = upload_vec(st, (void**)&g.stage_tiles)) != VD_OK) return rc;
-  if ((rc = upload_vec(at, (void**)&g.adj_tiles)) != VD_OK) return rc;
+  if ((rc = upload_vec(at, (void**)&g.adj_tiles[0])) != VD_OK) return rc;
+  if ((rc = upload_vec(at2, (void**)&g.adj_tiles[1])) != VD_OK) return rc;
   if (g.split_ns > 0) return split_group_build(g);
   // small equal-size FP64 members: fused inversion in one workgroup each
   if constexpr (sizeof(T) == 8) {
@@ -1639,7 +1646,7 @@ int group_build(Group& g) {
         sm[b] = RgSmall{g.foff[b], p.lam_cx, p.lam_cy, p.g / p.f0, gfc2};
       }
       if ((rc = upload_vec(sm, &g.small_mem)) != VD_OK) return rc;
-      g.small = true;
+      g.small = g.use_small = true;
     }
   }
   for (int stg = 0; stg < 4; ++stg) {
@@ -1674,12 +1681,17 @@ int group_build(Group& g) {
 void group_release(Group& g) {
   hipFree(g.mem);
   hipFree(g.stage_tiles);
-  hipFree(g.adj_tiles);
+  for (auto& t : g.adj_tiles) {
+    hipFree(t);
+    t = nullptr;
+  }
   for (int st = 0; st < 4; ++st) {
-    hipFree(g.sargs[st]);
-    hipFree(g.stiles[st]);
-    g.sargs[st] = nullptr;
-    g.stiles[st] = nullptr;
+    for (int sh = 0; sh < 2; ++sh) {
+      hipFree(g.sargs[sh][st]);
+      hipFree(g.stiles[sh][st]);
+      g.sargs[sh][st] = nullptr;
+      g.stiles[sh][st] = nullptr;
+    }
     hipFree(g.gargs[st]);
     g.gargs[st] = nullptr;
     for (int c = 0; c < 3; ++c) {
@@ -1689,9 +1701,9 @@ void group_release(Group& g) {
   }
   hipFree(g.small_mem);
   g.small_mem = nullptr;
-  g.small = false;
+  g.small = g.use_small = false;
   g.mem = nullptr;
-  g.stage_tiles = g.adj_tiles = nullptr;
+  g.stage_tiles = nullptr;
 }
 
 #define VD_INSTANTIATE(T)                                                                     \

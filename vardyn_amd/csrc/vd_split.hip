@@ -471,7 +471,7 @@ struct RgSplit {
   SplitArgs sa;
   size_t foff;
 };
-template <int AOP, int BOP, int KIND, int MAP, int NS>
+template <int AOP, int BOP, int KIND, int MAP, int NS, int BSL>
 __global__ __launch_bounds__(256) void k_gemm_split_rg(RgCall<double> call, const RgSplit* __restrict__ rg,
                                                        const int4* __restrict__ tiles) {
   const int4 t = tiles[blockIdx.x];
@@ -481,8 +481,8 @@ __global__ __launch_bounds__(256) void k_gemm_split_rg(RgCall<double> call, cons
   sa.g.Fsub = call.Fsub ? call.Fsub + fo : nullptr;
   sa.g.hb = call.hb ? call.hb + fo : nullptr;
   sa.g.out = call.out ? call.out + fo : nullptr;
-  // the tiles of configuration 2 (kRgSplitCfg below)
-  split_tile<AOP, BOP, KIND, MAP, 64, 32, 2, 1, NS, 8>(sa, t.y, t.z);
+  // BSL sine lines (64 or 32) x 32 data lines: configuration 2 or 3
+  split_tile<AOP, BOP, KIND, MAP, BSL, 32, BSL / 32, 1, NS, 8>(sa, t.y, t.z);
 }
 
 // Tile configurations of the split kernel: sine lines x data lines per
@@ -591,15 +591,18 @@ void launch_split_stage(const Plan& p, int stage, const double* F, const double*
 }
 
 // Split-product GEMMs of a ragged group: one launch per inversion stage over
-// the 64 sine x 32 data line tiles (configuration 2) of every active member,
-// K up to 256.  A member's sine lines run to split_line_alloc(lines), a
-// multiple of 64; the tiles' rows past the product's own multiply zero digit
-// lines, and the epilogues discard them.  (On 32 x 32 tiles every data line
-// was fetched and split once per 32 sine lines, and the four waves split K
-// four ways; 64 x 32 tiles halve both, for 6.7% more MFMA work on the
-// 48-problem set's shapes.)
-constexpr int kRgSplitCfg = 2;
-static SplitArgs split_stage_args(const Plan& p, int stage) {
+// the tiles of every active member, K up to 256, in one of two shapes:
+//   0: 64 sine x 32 data lines (configuration 2).  A member's sine lines run
+//      to split_line_alloc(lines), a multiple of 64; the tiles' rows past the
+//      product's own multiply zero digit lines, and the epilogues discard
+//      them.  Each data line is fetched and split once per 64 sine lines, and
+//      two wave groups split K (6.7% more MFMA work on the 48-problem set's
+//      shapes than shape 1).
+//   1: 32 x 32 (configuration 3): no padding, each data line fetched and
+//      split once per 32 sine lines, four wave groups split K.
+// Every shape gives the same bits; group_build's default is shape 0 and
+// vd_group_autotune times both per stage.
+static SplitArgs split_stage_args(const Plan& p, int stage, int shape) {
   SplitArgs a;
   a.g = stage_args<double>(p, stage);
   int nsl, ndl, K, nz;
@@ -612,34 +615,37 @@ static SplitArgs split_stage_args(const Plan& p, int stage) {
   a.ns = p.split_ns;
   a.pstride = (size_t)p.split_ns * a.lines * K;
   a.g.nz = nz;
-  a.g.tm = a.lines / kSplitCfgs[kRgSplitCfg].bsl;
-  a.g.tn = ndl / kSplitCfgs[kRgSplitCfg].bdl;
+  const int cfg = shape == 0 ? 2 : 3;
+  a.g.tm = (shape == 0 ? a.lines : nsl) / kSplitCfgs[cfg].bsl;
+  a.g.tn = ndl / kSplitCfgs[cfg].bdl;
   return a;
 }
 
 int split_group_build(Group& g) {
-  for (int stg = 0; stg < 4; ++stg) {
-    std::vector<RgSplit> args(g.n);
-    std::vector<int4> tiles;
-    g.sprefix[stg].assign(g.n + 1, 0);
-    for (int b = 0; b < g.n; ++b) {
-      const Plan& p = *g.plans[b];
-      int nsl, ndl, K, nz;
-      split_lines(p, stg, &nsl, &ndl, &K, &nz);
-      VD_REQUIRE(K <= 256, VD_ERR_UNSUPPORTED,
-                 "split-product groups need sine products with K up to 256 (interior sizes up to 512 folded)");
-      args[b].sa = split_stage_args(p, stg);
-      args[b].foff = g.foff[b];
-      const int ntile = args[b].sa.g.tm * args[b].sa.g.tn;
-      for (int z = 0; z < nz; ++z)
-        for (int t = 0; t < ntile; ++t) tiles.push_back(make_int4(b, z, t, 0));
-      g.sprefix[stg][b + 1] = (int)tiles.size();
+  for (int shape = 0; shape < 2; ++shape)
+    for (int stg = 0; stg < 4; ++stg) {
+      std::vector<RgSplit> args(g.n);
+      std::vector<int4> tiles;
+      g.sprefix[shape][stg].assign(g.n + 1, 0);
+      for (int b = 0; b < g.n; ++b) {
+        const Plan& p = *g.plans[b];
+        int nsl, ndl, K, nz;
+        split_lines(p, stg, &nsl, &ndl, &K, &nz);
+        VD_REQUIRE(K <= 256, VD_ERR_UNSUPPORTED,
+                   "split-product groups need sine products with K up to 256 (interior sizes up to 512 folded)");
+        args[b].sa = split_stage_args(p, stg, shape);
+        args[b].foff = g.foff[b];
+        const int ntile = args[b].sa.g.tm * args[b].sa.g.tn;
+        for (int z = 0; z < nz; ++z)
+          for (int t = 0; t < ntile; ++t) tiles.push_back(make_int4(b, z, t, 0));
+        g.sprefix[shape][stg][b + 1] = (int)tiles.size();
+      }
+      VD_HIP_TRY(hipMalloc(&g.sargs[shape][stg], args.size() * sizeof(RgSplit)));
+      VD_HIP_TRY(hipMemcpy(g.sargs[shape][stg], args.data(), args.size() * sizeof(RgSplit), hipMemcpyHostToDevice));
+      VD_HIP_TRY(hipMalloc((void**)&g.stiles[shape][stg], std::max<size_t>(1, tiles.size()) * sizeof(int4)));
+      VD_HIP_TRY(hipMemcpy(g.stiles[shape][stg], tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice));
     }
-    VD_HIP_TRY(hipMalloc(&g.sargs[stg], args.size() * sizeof(RgSplit)));
-    VD_HIP_TRY(hipMemcpy(g.sargs[stg], args.data(), args.size() * sizeof(RgSplit), hipMemcpyHostToDevice));
-    VD_HIP_TRY(hipMalloc((void**)&g.stiles[stg], std::max<size_t>(1, tiles.size()) * sizeof(int4)));
-    VD_HIP_TRY(hipMemcpy(g.stiles[stg], tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice));
-  }
+  for (int stg = 0; stg < 4; ++stg) g.split_shape[stg] = 0;
   return VD_OK;
 }
 
@@ -647,30 +653,39 @@ struct RaggedSplitLaunch {
   RgCall<double> call;
   const RgSplit* rg;
   const int4* tiles;
-  int ntiles, ns;
+  int ntiles, ns, shape;
   hipStream_t s;
   template <int AOP, int BOP, int KIND, int MAP>
   void go() {
+    if (shape == 0) go_bsl<AOP, BOP, KIND, MAP, 64>();
+    else go_bsl<AOP, BOP, KIND, MAP, 32>();
+  }
+  template <int AOP, int BOP, int KIND, int MAP, int BSL>
+  void go_bsl() {
     const dim3 grid(ntiles), block(256);
-    if (ns == 5) hipLaunchKernelGGL((k_gemm_split_rg<AOP, BOP, KIND, MAP, 5>), grid, block, 0, s, call, rg, tiles);
-    else if (ns == 6) hipLaunchKernelGGL((k_gemm_split_rg<AOP, BOP, KIND, MAP, 6>), grid, block, 0, s, call, rg, tiles);
-    else hipLaunchKernelGGL((k_gemm_split_rg<AOP, BOP, KIND, MAP, 7>), grid, block, 0, s, call, rg, tiles);
+    if (ns == 5) hipLaunchKernelGGL((k_gemm_split_rg<AOP, BOP, KIND, MAP, 5, BSL>), grid, block, 0, s, call, rg, tiles);
+    else if (ns == 6) hipLaunchKernelGGL((k_gemm_split_rg<AOP, BOP, KIND, MAP, 6, BSL>), grid, block, 0, s, call, rg, tiles);
+    else hipLaunchKernelGGL((k_gemm_split_rg<AOP, BOP, KIND, MAP, 7, BSL>), grid, block, 0, s, call, rg, tiles);
   }
 };
 
+void launch_split_stage_rg(const Group& g, int st, int shape, int nact, const double* F, const double* Fsub,
+                           const double* hb, double* out, int mode, hipStream_t s) {
+  RaggedSplitLaunch l;
+  l.call = RgCall<double>{st == 0 ? F : nullptr, st == 0 ? Fsub : nullptr, st == 3 ? hb : nullptr,
+                          st == 3 ? out : nullptr};
+  l.rg = static_cast<const RgSplit*>(g.sargs[shape][st]);
+  l.tiles = g.stiles[shape][st];
+  l.ntiles = g.sprefix[shape][st][nact];
+  l.ns = g.split_ns;
+  l.shape = shape;
+  l.s = s;
+  if (l.ntiles > 0) dispatch_stage(g.fold != 0, st, Fsub != nullptr, mode, l);
+}
+
 void launch_inversion_split_rg(const Group& g, int nact, const double* F, const double* Fsub,
                                const double* hb, double* out, int mode, hipStream_t s) {
-  for (int st = 0; st < 4; ++st) {
-    RaggedSplitLaunch l;
-    l.call = RgCall<double>{st == 0 ? F : nullptr, st == 0 ? Fsub : nullptr, st == 3 ? hb : nullptr,
-                            st == 3 ? out : nullptr};
-    l.rg = static_cast<const RgSplit*>(g.sargs[st]);
-    l.tiles = g.stiles[st];
-    l.ntiles = g.sprefix[st][nact];
-    l.ns = g.split_ns;
-    l.s = s;
-    if (l.ntiles > 0) dispatch_stage(g.fold != 0, st, Fsub != nullptr, mode, l);
-  }
+  for (int st = 0; st < 4; ++st) launch_split_stage_rg(g, st, g.split_shape[st], nact, F, Fsub, hb, out, mode, s);
 }
 
 }  // namespace vd

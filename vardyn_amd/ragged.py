@@ -89,6 +89,22 @@ class QgGroup:
     def adj_final(self, nact, hcur, lh, lq, adh0):
         _lib.call("vd_group_adj_final", self._h, int(nact), hcur, lh, lq, adh0, self._s())
 
+    def autotune(self, reps=5):
+        """Pick the group's tile shapes and inversion form by timing them on its
+        members (``vd_group_autotune``); returns the choices."""
+        ch = (C.c_int * 6)()
+        _lib.call("vd_group_autotune", self._h, int(reps), ch, self._s())
+        return {"split_tiles": [("64x32", "32x32")[c] if c >= 0 else None for c in ch[:4]],
+                "adjoint_tile": ("64x16", "32x32")[ch[4]],
+                "fused_small_inversion": None if ch[5] < 0 else bool(ch[5])}
+
+    def set_tuning(self, tuning):
+        """Apply choices in the form ``autotune`` returns."""
+        ch = (C.c_int * 6)(*[-1 if t is None else ("64x32", "32x32").index(t) for t in tuning["split_tiles"]],
+                           ("64x16", "32x32").index(tuning["adjoint_tile"]),
+                           -1 if tuning["fused_small_inversion"] is None else int(tuning["fused_small_inversion"]))
+        _lib.call("vd_group_set_tuning", self._h, ch)
+
 
 class RaggedFourDVar:
     """Cost and gradient of every problem of ``problems`` in one step-synchronous sweep.
@@ -98,7 +114,7 @@ class RaggedFourDVar:
     ``vardyn_amd.problem.build_problem`` returns.
     """
 
-    def __init__(self, problems):
+    def __init__(self, problems, autotune=False):
         self.problems = list(problems)
         n = len(self.problems)
         if n == 0:
@@ -120,6 +136,8 @@ class RaggedFourDVar:
         self.nst = np.array([nsteps[i] for i in self.order])
         self.S = int(self.nst.max())
         self.group = QgGroup([m.plan for m in self.members])
+        # tile shapes / inversion form timed on these members (None: defaults)
+        self.tuning = self.group.autotune() if autotune else None
         off = self.group.offsets
         self.nf = [off[j + 1] - off[j] for j in range(n)]
         es = torch.empty((), dtype=self.dtype).element_size()

@@ -148,6 +148,12 @@ class QgmWithTiles:
         self.step_tgl_jit = self.step_tgl
         self.step_adj_jit = self.step_adj
 
+    def autotune(self, reps=5):
+        """Pick the tile group's adjoint tile shape and inversion form (fused per
+        tile or GEMM stages) by timing them (``vd_group_autotune``); returns
+        the choices, or None for ``engine="plans"``."""
+        return None if self.group is None else self.group.autotune(reps)
+
     # ------------------------------------------------------------- helpers
     @property
     def ntile(self):
