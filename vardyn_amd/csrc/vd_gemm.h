@@ -245,11 +245,13 @@ struct EpiIn {
 template <typename T, int KIND, int MAP>
 __device__ __forceinline__ EpiIn<T> epi_input(const GemmArgs<T>& g, const GemmPtrs<T>& P, int row, int col) {
   // Straight-line code: every load is unconditional, from an index clamped
-  // into its array (an element outside the interior reads element 0), and
-  // without hb the output field stands in; epi_store discards what it does
-  // not use.  So all of a thread's loads are in flight together.
+  // into its array (an element outside the interior reads element 0);
+  // epi_store discards what it does not use.  So all of a thread's loads are
+  // in flight together.  EK_FIELD_SET without hb loads nothing (the branch is
+  // on a kernel argument, the same for every thread).
   EpiIn<T> in{T(0), 0.0, 0.0};
   if (MAP == MAP_ROWFOLD || KIND == EK_STORE) return in;
+  if (KIND == EK_FIELD_SET && !P.hbm) return in;
   const int oc = MAP == MAP_COLFOLD ? 2 * col + P.par : col;
   const bool ok = row < g.my && (MAP == MAP_COLFOLD ? col < g.hx : col < g.mx) && oc < g.mx;
   if (KIND == EK_SCALE) {
@@ -257,7 +259,7 @@ __device__ __forceinline__ EpiIn<T> epi_input(const GemmArgs<T>& g, const GemmPt
     in.lcy = g.lcy[ok ? row : 0];
     return in;
   }
-  const T* src = KIND == EK_FIELD_SET && P.hbm ? P.hbm : P.fout;
+  const T* src = KIND == EK_FIELD_SET ? P.hbm : P.fout;
   const size_t i = ok ? row + 1 : 1, j = ok ? oc + 1 : 1;
   in.x = src[i * g.nx + j];
   return in;
