@@ -327,6 +327,20 @@ int vd_bm_operg(vd_bm_basis* basis, int T, const double* gt, const double* X, do
 /* adX += sum over the T times of operg_transpose(time t, adssh[t]). */
 int vd_bm_operg_adj(vd_bm_basis* basis, int T, const double* gt, const double* adssh,
                     double* adX, void* work, void* stream);
+/* Geostrophic velocities of the basis fields (compute_velocities,
+ * basis.py:822-860; _ssh2uv / _ssh2uv_adj).  vd_bm_uv_tables uploads, once,
+ * au = -g / f_on_u and dy_on_u on the u grid (ny x (nx+1)) and bv = g / f_on_v
+ * and dx_on_v on the v grid ((ny+1) x nx), the reference's edge-padded
+ * midpoint tables (basis.py:514-526), as host arrays.  vd_bm_ssh2uv: for T
+ * fields, u = (au dP_y) / dy_on_u and v = (bv dP_x) / dx_on_v with P the SSH
+ * padded by one edge row and column; vd_bm_ssh2uv_adj: adssh += the
+ * transpose applied to (adu, adv), in the reference's order of operations.
+ * Device pointers for the fields. */
+int vd_bm_uv_tables(vd_bm_basis* basis, const double* au, const double* dy_on_u, const double* bv,
+                    const double* dx_on_v);
+int vd_bm_ssh2uv(vd_bm_basis* basis, int T, const double* ssh, double* u, double* v, void* stream);
+int vd_bm_ssh2uv_adj(vd_bm_basis* basis, int T, const double* adu, const double* adv, double* adssh,
+                     void* stream);
 
 /* ---------------------------------------------------------------------------
  * 3-D wavelet basis (Basis_wavelet3d, mapping/src/basis.py:2640-2985): fields

@@ -27,12 +27,22 @@ Follows ``mapping/src/basis.py`` of datlas-ocean/VarDyn:
   sum over frequencies of Gx_f (sum over active time centres of
   factor x coefficient block), and its transpose.
 
+* ``geostrophic_tables`` / ``ssh2uv`` / ``ssh2uv_adj`` -- basis.py:514-526,
+  822-860 (``compute_velocities``): f, dx, dy padded by one edge row and column
+  and averaged onto the u and v grids; u = -g/f_u dP_y / dy_u and
+  v = g/f_v dP_x / dx_v of the edge-padded SSH P, and the transpose with the
+  padded row, column and corner folded back onto the SSH's edge.
+
 Pinning (tests/test_oracle_basis_bm.py): the reference's own dot-product
 test of operg against operg_transpose (basis.py:4150-4158); closed forms of
 the windows, of one isolated wavelet's field and of the time factors; the
 counts of frequencies, directions and coefficients from the formulas; the Q
 profile (constant above lmeso, power law below); a dense brute-force
-construction of Gx on a tiny grid equal to the CSC one.
+construction of Gx on a tiny grid equal to the CSC one; for the velocities,
+the dot-product test of ssh2uv against ssh2uv_adj, the exact velocities of
+a tilted plane on a grid with dx != dy and f varying along y (interior
+values, zero first row of u and first column of v), and the midpoint tables
+of a linear f.
 """
 from __future__ import annotations
 
@@ -224,3 +234,40 @@ class BasisBmOracle:
                 sl = slice(self.iff_wavebounds[iff], self.iff_wavebounds[iff + 1])
                 adX[sl][ind0] += (gb * col[None, :]).ravel()
         return adX
+
+
+def geostrophic_tables(f, DX, DY, g=9.81):
+    """basis.py:514-526: (au, dy_on_u, bv, dx_on_v) with au = -g / f_on_u and
+    bv = g / f_on_v; f, DX, DY (ny, nx) padded by one edge row and column."""
+    pad = ((1, 0), (1, 0))
+    _f = np.pad(np.asarray(f, dtype=np.float64), pad_width=pad, mode="edge")
+    f_on_v = 0.5 * (_f[:, 1:] + _f[:, :-1])
+    f_on_u = 0.5 * (_f[1:, :] + _f[:-1, :])
+    dx = np.pad(np.asarray(DX, dtype=np.float64), pad_width=pad, mode="edge")
+    dy = np.pad(np.asarray(DY, dtype=np.float64), pad_width=pad, mode="edge")
+    dx_on_v = 0.5 * (dx[:, 1:] + dx[:, :-1])
+    dy_on_u = 0.5 * (dy[1:, :] + dy[:-1, :])
+    return -g / f_on_u, dy_on_u, g / f_on_v, dx_on_v
+
+
+def ssh2uv(ssh, au, dy_on_u, bv, dx_on_v):
+    """basis.py:822-833: u (ny, nx+1), v (ny+1, nx) of one SSH field."""
+    _ssh = np.pad(np.asarray(ssh, dtype=np.float64), pad_width=((1, 0), (1, 0)), mode="edge")
+    u = au * (_ssh[1:, :] - _ssh[:-1, :]) / dy_on_u
+    v = bv * (_ssh[:, 1:] - _ssh[:, :-1]) / dx_on_v
+    return u, v
+
+
+def ssh2uv_adj(adu, adv, au, dy_on_u, bv, dx_on_v):
+    """basis.py:835-860: the SSH adjoint of (adu, adv)."""
+    ny, nx1 = np.shape(adu)
+    _adssh = np.zeros((ny + 1, nx1))
+    _adssh[1:, :] += au * adu / dy_on_u
+    _adssh[:-1, :] += -au * adu / dy_on_u
+    _adssh[:, 1:] += bv * adv / dx_on_v
+    _adssh[:, :-1] += -bv * adv / dx_on_v
+    adssh = _adssh[1:, 1:].copy()
+    adssh[0, :] += _adssh[0, 1:]
+    adssh[:, 0] += _adssh[1:, 0]
+    adssh[0, 0] += _adssh[0, 0]
+    return adssh

@@ -100,3 +100,61 @@ def test_state_protocol():
     State = {"ssh": torch.zeros(gpu.shape_phys, dtype=torch.float64, device="cuda")}
     gpu.operg(1.0, X, State=State)
     assert torch.equal(State["ssh"].cpu(), torch.as_tensor(gpu.operg(1.0, X)))
+
+
+# ------------------------------------------------- geostrophic velocities (basis.py:822-860)
+def uv_pair(ny=31, nx=26, seed=4):
+    from oracle.basis_bm import geostrophic_tables
+    lon, lat, mask = grid(ny, nx, 0.05, False)
+    rng = np.random.default_rng(seed)
+    f = 1e-4 + 2e-6 * rng.standard_normal((ny, nx))
+    DX = 5e3 * (1 + 0.1 * rng.random((ny, nx)))
+    DY = 6e3 * (1 + 0.1 * rng.random((ny, nx)))
+    gpu = Basis_bm(SimpleNamespace(lon=lon, lat=lat, mask=mask, f=f, DX=DX, DY=DY), lmin=30.0, lmax=110.0,
+                   compute_velocities=True)
+    gpu.set_basis(np.arange(0.0, 6.0, 0.5))
+    return gpu, geostrophic_tables(f, DX, DY), rng
+
+
+def test_ssh2uv_and_adjoint_against_oracle():
+    """Device velocities and their adjoint equal the oracle's (the reference's
+    operation order, no contraction) for three fields at once."""
+    from oracle.basis_bm import ssh2uv, ssh2uv_adj
+    gpu, tab, rng = uv_pair()
+    ny, nx, T = gpu.ny, gpu.nx, 3
+    ssh = rng.standard_normal((T, ny, nx))
+    u, v = gpu.ssh2uv_batch(torch.as_tensor(ssh))
+    adu, adv = rng.standard_normal((T, ny, nx + 1)), rng.standard_normal((T, ny + 1, nx))
+    a0 = rng.standard_normal((T, ny, nx))
+    ad = gpu.ssh2uv_adj_batch(torch.as_tensor(adu), torch.as_tensor(adv), torch.as_tensor(a0, device="cuda").clone())
+    u, v, ad = u.cpu().numpy(), v.cpu().numpy(), ad.cpu().numpy()
+    for t in range(T):
+        uo, vo = ssh2uv(ssh[t], *tab)
+        assert rel(u[t], uo) <= 1e-15 and rel(v[t], vo) <= 1e-15
+        assert rel(ad[t], a0[t] + ssh2uv_adj(adu[t], adv[t], *tab)) <= 1e-15
+
+
+def test_operg_with_velocities():
+    """operg with compute_velocities fills the State's u and v with the
+    velocities of its SSH; operg_transpose folds the velocity adjoints into
+    the SSH adjoint before the transpose and zeroes all three."""
+    gpu, tab, rng = uv_pair()
+    X = rng.standard_normal(gpu.nbasis)
+    ssh, u, v = gpu.operg(1.5, X)
+    u2, v2 = gpu.ssh2uv_batch(torch.as_tensor(ssh)[None])
+    assert np.array_equal(u, u2[0].cpu().numpy()) and np.array_equal(v, v2[0].cpu().numpy())
+    st = {"ssh": np.zeros_like(ssh), "u": None, "v": None}
+    gpu.operg(1.5, torch.as_tensor(X, device="cuda"), State=st)
+    assert np.array_equal(st["u"].cpu().numpy(), u)
+    adssh = rng.standard_normal(ssh.shape)
+    adu, adv = rng.standard_normal(u.shape), rng.standard_normal(v.shape)
+    ad = {"ssh": adssh.copy(), "u": adu.copy(), "v": adv.copy()}
+    adX = gpu.operg_transpose(1.5, ad)
+    a = torch.as_tensor(adssh, device="cuda")[None].clone()
+    gpu.ssh2uv_adj_batch(torch.as_tensor(adu), torch.as_tensor(adv), a)
+    want = gpu.operg_transpose_batch([1.5], a).cpu().numpy()
+    assert np.array_equal(adX, want)
+    assert not ad["ssh"].any() and not ad["u"].any() and not ad["v"].any()
+    # the pair passes the dot-product test
+    lhs = np.sum(ssh * adssh) + np.sum(u * adu) + np.sum(v * adv)
+    assert abs(lhs - np.dot(X, adX)) <= 1e-12 * abs(lhs)

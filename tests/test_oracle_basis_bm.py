@@ -133,3 +133,60 @@ def test_single_wavelet_field_and_time_factor():
             b.Nt.update(b._time(np.array([t]))[1])
         got = b.operg(t, X).ravel()
         assert np.allclose(got, fac * wave / b.norm_fact[f], rtol=1e-12, atol=1e-15)
+
+
+# ------------------------------------------------- geostrophic velocities (basis.py:514-526, 822-860)
+def _uv_grid(ny=9, nx=12, seed=3):
+    from oracle.basis_bm import geostrophic_tables
+    rng = np.random.default_rng(seed)
+    f = 1e-4 + 2e-6 * rng.standard_normal((ny, nx))
+    DX = 5e3 * (1 + 0.1 * rng.random((ny, nx)))
+    DY = 7e3 * (1 + 0.1 * rng.random((ny, nx)))
+    return rng, geostrophic_tables(f, DX, DY)
+
+
+def test_ssh2uv_dot_product():
+    from oracle.basis_bm import ssh2uv, ssh2uv_adj
+    rng, tab = _uv_grid()
+    ssh = rng.standard_normal((9, 12))
+    adu, adv = rng.standard_normal((9, 13)), rng.standard_normal((10, 12))
+    u, v = ssh2uv(ssh, *tab)
+    lhs = np.sum(u * adu) + np.sum(v * adv)
+    rhs = np.sum(ssh * ssh2uv_adj(adu, adv, *tab))
+    assert abs(lhs - rhs) <= 1e-13 * abs(lhs)
+
+
+def test_ssh2uv_tilted_plane_closed_form():
+    """ssh = a x + b y on a grid with dx != dy and f linear in y: u = -g b / f_u
+    and v = g a / f_v away from the padded edge, u = 0 on the first row and
+    v = 0 on the first column (the edge padding repeats the first row and
+    column); f_u is the midpoint of consecutive rows of f, f_v the row's f."""
+    from oracle.basis_bm import geostrophic_tables, ssh2uv
+    ny, nx, dx, dy, g = 8, 11, 5e3, 7e3, 9.81
+    fr = 1e-4 + 3e-6 * np.arange(ny)
+    f = fr[:, None] * np.ones((1, nx))
+    tab = geostrophic_tables(f, dx * np.ones((ny, nx)), dy * np.ones((ny, nx)))
+    a, b = 2e-6, -3e-6
+    X, Y = np.meshgrid(dx * np.arange(nx), dy * np.arange(ny))
+    u, v = ssh2uv(a * X + b * Y, *tab)
+    fu = 0.5 * (fr + np.concatenate([fr[:1], fr[:-1]]))  # f_on_u row r: mean of f rows r and r-1
+    np.testing.assert_allclose(u[1:], (-g * b / fu[1:])[:, None] * np.ones((1, nx + 1)), rtol=1e-12)
+    assert np.all(u[0] == 0.0)
+    fv = np.concatenate([fr[:1], fr])  # f_on_v row r: f row max(r-1, 0)
+    np.testing.assert_allclose(v[:, 1:], (g * a / fv)[:, None] * np.ones((1, nx - 1)), rtol=1e-12)
+    assert np.all(v[:, 0] == 0.0)
+
+
+def test_geostrophic_tables_midpoints():
+    from oracle.basis_bm import geostrophic_tables
+    ny, nx = 5, 6
+    f = 1e-4 * (1 + 0.1 * np.arange(ny))[:, None] * np.ones((1, nx))
+    DX = np.ones((ny, nx)) * (1000.0 + 10 * np.arange(nx))[None, :]
+    DY = np.ones((ny, nx)) * (2000.0 + 20 * np.arange(ny))[:, None]
+    au, dyu, bv, dxv = geostrophic_tables(f, DX, DY)
+    assert au.shape == dyu.shape == (ny, nx + 1) and bv.shape == dxv.shape == (ny + 1, nx)
+    np.testing.assert_allclose(-9.81 / au[2, 3], 0.5 * (f[2, 0] + f[1, 0]), rtol=1e-15)
+    np.testing.assert_allclose(dyu[0, 0], DY[0, 0], rtol=1e-15)
+    np.testing.assert_allclose(dyu[3, 2], 0.5 * (DY[3, 0] + DY[2, 0]), rtol=1e-15)
+    np.testing.assert_allclose(dxv[2, 0], DX[0, 0], rtol=1e-15)
+    np.testing.assert_allclose(dxv[2, 4], 0.5 * (DX[0, 4] + DX[0, 3]), rtol=1e-15)

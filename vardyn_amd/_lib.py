@@ -104,6 +104,9 @@ SIGNATURES = {
     "vd_bm_workspace_size": (_I, [_VP, _I, C.POINTER(_SZ)]),
     "vd_bm_operg": (_I, [_VP, _I, _VP, _VP, _VP, _VP, _VP]),
     "vd_bm_operg_adj": (_I, [_VP, _I, _VP, _VP, _VP, _VP, _VP]),
+    "vd_bm_uv_tables": (_I, [_VP, _VP, _VP, _VP, _VP]),
+    "vd_bm_ssh2uv": (_I, [_VP, _I, _VP, _VP, _VP, _VP]),
+    "vd_bm_ssh2uv_adj": (_I, [_VP, _I, _VP, _VP, _VP, _VP]),
     "vd_w3_create": (_I, [_VP, _VP]),
     "vd_w3_destroy": (_I, [_VP]),
     "vd_w3_info": (_I, [_VP, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),

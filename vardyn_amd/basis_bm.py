@@ -19,9 +19,10 @@ Here ``set_basis`` builds the tables (centres, their point boxes on the
 rectilinear grid, the centres covering each grid point) and ``operg`` /
 ``operg_transpose`` run ``vd_bm_operg`` / ``vd_bm_operg_adj``: a matrix-free,
 deterministic gather in each direction, for one time or a batch of times
-(``operg_batch`` / ``operg_transpose_batch``).  Scope: the SSH component
-(``compute_velocities`` False), no depth attenuation and no background file
-(both read netCDF files).
+(``operg_batch`` / ``operg_transpose_batch``).  With ``compute_velocities``
+the fields also give their geostrophic velocities (basis.py:822-860) through
+``vd_bm_ssh2uv`` / ``vd_bm_ssh2uv_adj``.  Scope: no depth attenuation and no
+background file (both read netCDF files), no multi-mode accumulation.
 """
 from __future__ import annotations
 
@@ -73,10 +74,8 @@ class Basis_bm:
     def __init__(self, State, flux=False, facns=1.0, facnlt=2.0, npsp=3.5, facpsp=1.5,
                  lmin=80.0, lmax=970.0, lmeso=300.0, tmeso=20.0, sloptdec=-1.28, factdec=0.5,
                  tdecmin=2.5, tdecmax=40.0, facQ=1.0, Qmax=1e-3, slopQ=-5.0, norm_time=True,
-                 name_mod_var="ssh", delta_time_ref=0.0, compute_velocities=False, device=None,
-                 **kwargs):
-        if compute_velocities:
-            raise VardynError("Basis_bm: only the SSH component (compute_velocities=False) exists")
+                 name_mod_var="ssh", delta_time_ref=0.0, compute_velocities=False, name_mod_u="u",
+                 name_mod_v="v", device=None, **kwargs):
         if not torch.cuda.is_available():
             raise VardynError("Basis_bm needs a GPU (libvardyn_hip)")
         lon, lat = np.asarray(State.lon, dtype=np.float64), np.asarray(State.lat, dtype=np.float64)
@@ -97,6 +96,27 @@ class Basis_bm:
         self.sloptdec, self.factdec, self.tdecmin, self.tdecmax = sloptdec, factdec, tdecmin, tdecmax
         self.Qmax, self.slopQ, self.facQ, self.norm_time = Qmax, slopQ, facQ, norm_time
         self.name_mod_var = name_mod_var
+        self.compute_velocities = bool(compute_velocities)
+        self.name_mod_u, self.name_mod_v = name_mod_u, name_mod_v
+        self._uv = None
+        if self.compute_velocities:
+            # basis.py:514-526: f, DX, DY padded by one edge row and column,
+            # averaged onto the u and v grids; g = 9.81
+            try:
+                f, DX, DY = (np.asarray(getattr(State, n), dtype=np.float64) * np.ones(lon.shape)
+                             for n in ("f", "DX", "DY"))
+            except AttributeError as e:
+                raise VardynError("Basis_bm with compute_velocities needs State.f, State.DX and State.DY") from e
+            pad = ((1, 0), (1, 0))
+            _f = np.pad(f, pad_width=pad, mode="edge")
+            f_on_v = 0.5 * (_f[:, 1:] + _f[:, :-1])
+            f_on_u = 0.5 * (_f[1:, :] + _f[:-1, :])
+            dx = np.pad(DX, pad_width=pad, mode="edge")
+            dy = np.pad(DY, pad_width=pad, mode="edge")
+            self.g = 9.81
+            self._uv = tuple(np.ascontiguousarray(a) for a in (
+                -self.g / f_on_u, 0.5 * (dy[1:, :] + dy[:-1, :]), self.g / f_on_v,
+                0.5 * (dx[:, 1:] + dx[:, :-1])))
         self.delta_time_ref = float(delta_time_ref)
         self.window = _window_flux if flux else _window
         self.device = torch.device("cuda", torch.cuda.current_device()) if device is None \
@@ -109,7 +129,7 @@ class Basis_bm:
         B = config.BASIS
         names = ("flux", "facns", "facnlt", "npsp", "facpsp", "lmin", "lmax", "lmeso", "tmeso",
                  "sloptdec", "factdec", "tdecmin", "tdecmax", "facQ", "Qmax", "slopQ",
-                 "norm_time", "name_mod_var", "compute_velocities")
+                 "norm_time", "name_mod_var", "compute_velocities", "name_mod_u", "name_mod_v")
         args = {n: getattr(B, n) for n in names if hasattr(B, n)}
         ref = (config.EXP.init_date - datetime.datetime(1950, 1, 1, 0)).total_seconds() / 24 / 3600
         return cls(State, delta_time_ref=ref, **args, **kw)
@@ -255,6 +275,10 @@ class Basis_bm:
         with torch.cuda.device(self.device):
             _lib.call("vd_bm_create", C.byref(p), C.byref(h))
         self._h = h
+        if self._uv is not None:
+            au, dyu, bv, dxv = self._uv
+            _lib.call("vd_bm_uv_tables", self._h, au.ctypes.data, dyu.ctypes.data, bv.ctypes.data,
+                      dxv.ctypes.data)
 
     # ------------------------------------------------------------- time factors
     def time_factors(self, times):
@@ -323,21 +347,66 @@ class Basis_bm:
             adX = torch.zeros(self.nbasis, dtype=torch.float64, device=self.device)
         return self.operg_transpose_prepared(prep, a, adX)
 
+    def ssh2uv_batch(self, ssh):
+        """Geostrophic velocities (basis.py:822-833) of SSH fields (T, ny, nx)
+        on the device: u (T, ny, nx+1), v (T, ny+1, nx)."""
+        if self._uv is None:
+            raise VardynError("Basis_bm: built without compute_velocities")
+        s = torch.as_tensor(ssh, dtype=torch.float64).to(self.device).reshape(-1, self.ny, self.nx).contiguous()
+        T = s.shape[0]
+        u = torch.empty((T, self.ny, self.nx + 1), dtype=torch.float64, device=self.device)
+        v = torch.empty((T, self.ny + 1, self.nx), dtype=torch.float64, device=self.device)
+        _lib.call("vd_bm_ssh2uv", self._h, T, ptr(s), ptr(u), ptr(v), current_stream_ptr(self.device))
+        return u, v
+
+    def ssh2uv_adj_batch(self, adu, adv, adssh):
+        """adssh (T, ny, nx, device) += the transpose of ssh2uv (basis.py:835-860)
+        applied to adu (T, ny, nx+1), adv (T, ny+1, nx)."""
+        if self._uv is None:
+            raise VardynError("Basis_bm: built without compute_velocities")
+        T = adssh.shape[0]
+        if adssh.shape != (T, self.ny, self.nx) or not adssh.is_contiguous() or adssh.dtype != torch.float64:
+            raise VardynError("ssh2uv_adj_batch: adssh must be a contiguous float64 (T, ny, nx) tensor")
+        du = torch.as_tensor(adu, dtype=torch.float64).to(self.device).reshape(T, self.ny, self.nx + 1).contiguous()
+        dv = torch.as_tensor(adv, dtype=torch.float64).to(self.device).reshape(T, self.ny + 1, self.nx).contiguous()
+        _lib.call("vd_bm_ssh2uv_adj", self._h, T, ptr(du), ptr(dv), ptr(adssh), current_stream_ptr(self.device))
+        return adssh
+
     def operg(self, t, X, State=None):
-        """basis.py:863-901 (SSH): the field at time ``t``; into
-        ``State[name_mod_var]`` when a State is given, else returned (numpy
-        for numpy ``X``)."""
+        """basis.py:863-901: the field at time ``t`` (and with
+        ``compute_velocities`` its geostrophic velocities); into
+        ``State[name_mod_var]`` (``name_mod_u``, ``name_mod_v``) when a State
+        is given, else returned (numpy for numpy ``X``)."""
         ssh = self.operg_batch([t], X)[0]
+        out = [ssh]
+        if self.compute_velocities:
+            u, v = self.ssh2uv_batch(ssh[None])
+            out += [u[0], v[0]]
         if State is None:
-            return ssh.cpu().numpy() if isinstance(X, np.ndarray) else ssh
-        ref = State[self.name_mod_var]
-        State[self.name_mod_var] = ssh.to(ref.device if torch.is_tensor(ref) else ssh.device)
+            if isinstance(X, np.ndarray):
+                out = [a.cpu().numpy() for a in out]
+            return tuple(out) if self.compute_velocities else out[0]
+        names = [self.name_mod_var] + ([self.name_mod_u, self.name_mod_v] if self.compute_velocities else [])
+        for n, a in zip(names, out):
+            ref = State[n] if n in State else None
+            State[n] = a.to(ref.device if torch.is_tensor(ref) else a.device)
 
     def operg_transpose(self, t, adState):
-        """basis.py:903-936 (SSH): the coefficients' adjoint at time ``t``;
-        zeroes ``adState[name_mod_var]`` like the reference."""
+        """basis.py:903-936: the coefficients' adjoint at time ``t`` (with
+        ``compute_velocities`` the velocity adjoints are first added to the
+        SSH adjoint); zeroes the adjoint fields like the reference."""
         ad = adState[self.name_mod_var]
-        adX = self.operg_transpose_batch([t], torch.as_tensor(ad)[None])
-        adState[self.name_mod_var] = torch.zeros_like(torch.as_tensor(ad)) if torch.is_tensor(ad) \
-            else np.zeros_like(ad)
+        a = torch.as_tensor(ad, dtype=torch.float64).to(self.device).reshape(1, self.ny, self.nx).clone()
+        if self.compute_velocities:
+            du, dv = adState.get(self.name_mod_u), adState.get(self.name_mod_v)
+            if du is not None and dv is not None:
+                self.ssh2uv_adj_batch(du, dv, a)
+        adX = self.operg_transpose_batch([t], a)
+        def zero(x):
+            return torch.zeros_like(x) if torch.is_tensor(x) else np.zeros_like(x)
+        adState[self.name_mod_var] = zero(ad)
+        if self.compute_velocities:
+            for n in (self.name_mod_u, self.name_mod_v):
+                if adState.get(n) is not None:
+                    adState[n] = zero(adState[n])
         return adX.cpu().numpy()

@@ -368,7 +368,70 @@ struct vd_bm_basis {
   long long nbasis;
   int gt_total;  // sum of nt_f
   int device;
+  // geostrophic velocities (vd_bm_uv_tables): -g / f and dy on the u grid
+  // (ny, nx+1), g / f and dx on the v grid (ny+1, nx); null until set
+  const double *au = nullptr, *dyu = nullptr, *bv = nullptr, *dxv = nullptr;
 };
+
+// ------------------------------------------------------ geostrophic velocities
+// Basis_bm with compute_velocities (basis.py:822-860).  The SSH is padded by
+// one edge row and column (np.pad mode 'edge': padded (r, c) = ssh(max(r-1,
+// 0), max(c-1, 0))), and
+//   u (i, c) = (au (i, c) (P(i+1, c) - P(i, c))) / dyu (i, c),  i < ny, c <= nx
+//   v (r, j) = (bv (r, j) (P(r, j+1) - P(r, j))) / dxv (r, j),  r <= ny, j < nx
+// in the reference's operation order (no contraction).  The adjoint gathers,
+// per padded point, the four terms the reference adds in turn (+pu below,
+// -pu at its own row, +pv right, -pv at its own column, pu = (au adu) / dyu,
+// pv = (bv adv) / dxv), folds the padded first row, first column and corner
+// back onto the SSH's edge in the reference's order, and adds the result to
+// adssh.
+__global__ void k_bm_uv(int ny, int nx, int T, const double* __restrict__ au, const double* __restrict__ dyu,
+                        const double* __restrict__ bv, const double* __restrict__ dxv,
+                        const double* __restrict__ ssh, double* __restrict__ u, double* __restrict__ v) {
+#pragma clang fp contract(off)
+  const size_t nu = (size_t)ny * (nx + 1), nv = (size_t)(ny + 1) * nx, n = nu + nv;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n * T; e += (size_t)gridDim.x * blockDim.x) {
+    const size_t t = e / n, k = e % n;
+    const double* s = ssh + t * ny * nx;
+    auto P = [&](int r, int c) { return s[(size_t)max(r - 1, 0) * nx + max(c - 1, 0)]; };
+    if (k < nu) {
+      const int i = (int)(k / (nx + 1)), c = (int)(k % (nx + 1));
+      u[t * nu + k] = (au[k] * (P(i + 1, c) - P(i, c))) / dyu[k];
+    } else {
+      const size_t kv = k - nu;
+      const int r = (int)(kv / nx), j = (int)(kv % nx);
+      v[t * nv + kv] = (bv[kv] * (P(r, j + 1) - P(r, j))) / dxv[kv];
+    }
+  }
+}
+
+__global__ void k_bm_uv_adj(int ny, int nx, int T, const double* __restrict__ au, const double* __restrict__ dyu,
+                            const double* __restrict__ bv, const double* __restrict__ dxv,
+                            const double* __restrict__ adu, const double* __restrict__ adv, double* adssh) {
+#pragma clang fp contract(off)
+  const size_t nu = (size_t)ny * (nx + 1), nv = (size_t)(ny + 1) * nx, np_ = (size_t)ny * nx;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < np_ * T; e += (size_t)gridDim.x * blockDim.x) {
+    const size_t t = e / np_;
+    const int i = (int)((e % np_) / nx), j = (int)(e % nx);
+    const double* du = adu + t * nu;
+    const double* dv = adv + t * nv;
+    // the padded adjoint at (r, c): terms added in the reference's order
+    auto Pad = [&](int r, int c) {
+      double a = 0.0;
+      if (r >= 1) a += (au[(size_t)(r - 1) * (nx + 1) + c] * du[(size_t)(r - 1) * (nx + 1) + c]) / dyu[(size_t)(r - 1) * (nx + 1) + c];
+      if (r <= ny - 1) a += -((au[(size_t)r * (nx + 1) + c] * du[(size_t)r * (nx + 1) + c]) / dyu[(size_t)r * (nx + 1) + c]);
+      if (c >= 1) a += (bv[(size_t)r * nx + c - 1] * dv[(size_t)r * nx + c - 1]) / dxv[(size_t)r * nx + c - 1];
+      if (c <= nx - 1) a += -((bv[(size_t)r * nx + c] * dv[(size_t)r * nx + c]) / dxv[(size_t)r * nx + c]);
+      return a;
+    };
+    double a = Pad(i + 1, j + 1);
+    if (i == 0) a += Pad(0, j + 1);
+    if (j == 0) a += Pad(i + 1, 0);
+    if (i == 0 && j == 0) a += Pad(0, 0);
+    const size_t o = t * np_ + (size_t)i * nx + j;
+    adssh[o] = adssh[o] + a;
+  }
+}
 
 template <typename V>
 static int bm_upload(vd_bm_basis* b, const V* src, size_t n, const V** dst) {
@@ -578,6 +641,42 @@ int vd_bm_operg_adj(vd_bm_basis* b, int T, const double* gt, const double* adssh
   const long long n = (long long)B.nc * 2 * B.ntheta;
   hipLaunchKernelGGL(k_bm_time_adj, dim3((unsigned)std::min<long long>((n + 255) / 256, 65535)),
                      dim3(256), 0, s, B, T, gt, D, adX);
+  VD_HIP_TRY(hipGetLastError());
+  return VD_OK;
+}
+
+int vd_bm_uv_tables(vd_bm_basis* b, const double* au, const double* dyu, const double* bv, const double* dxv) {
+  VD_REQUIRE(b && au && dyu && bv && dxv, VD_ERR_INVALID_ARG, "null argument");
+  const int ny = b->B.ny, nx = b->B.nx;
+  const size_t nu = (size_t)ny * (nx + 1), nv = (size_t)(ny + 1) * nx;
+  VD_REQUIRE(b->au == nullptr, VD_ERR_INVALID_ARG, "velocity tables already set");
+  int rc;
+  if ((rc = bm_upload(b, au, nu, &b->au)) != VD_OK || (rc = bm_upload(b, dyu, nu, &b->dyu)) != VD_OK ||
+      (rc = bm_upload(b, bv, nv, &b->bv)) != VD_OK || (rc = bm_upload(b, dxv, nv, &b->dxv)) != VD_OK)
+    return rc;
+  return VD_OK;
+}
+
+static unsigned bm_blocks(size_t n) { return (unsigned)std::max<size_t>(1, std::min<size_t>((n + 255) / 256, 65535)); }
+
+int vd_bm_ssh2uv(vd_bm_basis* b, int T, const double* ssh, double* u, double* v, void* stream) {
+  VD_REQUIRE(b && ssh && u && v && T > 0, VD_ERR_INVALID_ARG, "null argument or T < 1");
+  VD_REQUIRE(b->au, VD_ERR_INVALID_ARG, "velocity tables not set (vd_bm_uv_tables)");
+  const int ny = b->B.ny, nx = b->B.nx;
+  const size_t n = ((size_t)ny * (nx + 1) + (size_t)(ny + 1) * nx) * T;
+  hipLaunchKernelGGL(k_bm_uv, dim3(bm_blocks(n)), dim3(256), 0, static_cast<hipStream_t>(stream), ny, nx, T, b->au,
+                     b->dyu, b->bv, b->dxv, ssh, u, v);
+  VD_HIP_TRY(hipGetLastError());
+  return VD_OK;
+}
+
+int vd_bm_ssh2uv_adj(vd_bm_basis* b, int T, const double* adu, const double* adv, double* adssh, void* stream) {
+  VD_REQUIRE(b && adu && adv && adssh && T > 0, VD_ERR_INVALID_ARG, "null argument or T < 1");
+  VD_REQUIRE(b->au, VD_ERR_INVALID_ARG, "velocity tables not set (vd_bm_uv_tables)");
+  const int ny = b->B.ny, nx = b->B.nx;
+  const size_t n = (size_t)ny * nx * T;
+  hipLaunchKernelGGL(k_bm_uv_adj, dim3(bm_blocks(n)), dim3(256), 0, static_cast<hipStream_t>(stream), ny, nx, T,
+                     b->au, b->dyu, b->bv, b->dxv, adu, adv, adssh);
   VD_HIP_TRY(hipGetLastError());
   return VD_OK;
 }
