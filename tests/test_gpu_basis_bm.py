@@ -158,3 +158,19 @@ def test_operg_with_velocities():
     # the pair passes the dot-product test
     lhs = np.sum(ssh * adssh) + np.sum(u * adu) + np.sum(v * adv)
     assert abs(lhs - np.dot(X, adX)) <= 1e-12 * abs(lhs)
+
+
+def test_basis_factory_builds_bm():
+    """The reference's factory (basis.py:30-86) with BASIS_BM and
+    BASIS_BM_JAX builds the GPU basis with the config's fields."""
+    import datetime
+
+    from vardyn_amd.basis import Basis
+    lon, lat, mask = grid(20, 24, 0.05, False)
+    for sup in ("BASIS_BM", "BASIS_BM_JAX"):
+        cfg = SimpleNamespace(BASIS=SimpleNamespace(super=sup, lmin=30.0, lmax=110.0, flux=False,
+                                                    name_mod_var="ssh"),
+                              EXP=SimpleNamespace(init_date=datetime.datetime(2012, 10, 1)))
+        b = Basis(cfg, SimpleNamespace(lon=lon, lat=lat, mask=mask), verbose=False)
+        assert isinstance(b, Basis_bm) and b.lmin == 30.0
+        assert b.delta_time_ref == (datetime.datetime(2012, 10, 1) - datetime.datetime(1950, 1, 1)).days

@@ -108,3 +108,28 @@ def test_product_mask_matches_oracle_with_land():
     ssh = np.zeros((40, 33))
     ssh[rng.integers(0, 40, 12), rng.integers(0, 33, 12)] = np.nan
     assert np.array_equal(build_mask(40, 33, ssh), omask(40, 33, ssh))
+
+
+def test_basis_factory_dispatch():
+    """vardyn_amd.basis.Basis follows the reference's factory (basis.py:30-86):
+    None without a basis, the offset basis for BASIS_OFFSET(_JAX), an error
+    for classes not built here and for multi-mode."""
+    from types import SimpleNamespace
+
+    import pytest
+
+    from vardyn_amd._lib import VardynError
+    from vardyn_amd.basis import Basis, Basis_offset
+    State = SimpleNamespace(ny=4, nx=5)
+    assert Basis(SimpleNamespace(BASIS=None), State) is None
+    for sup in ("BASIS_OFFSET", "BASIS_OFFSET_JAX"):
+        cfg = SimpleNamespace(BASIS=SimpleNamespace(super=sup, sigma_B=0.1, name_mod_var="ssh"))
+        b = Basis(cfg, State, verbose=False)
+        assert isinstance(b, Basis_offset) and b.nbasis == 1
+    with pytest.raises(VardynError):
+        Basis(SimpleNamespace(BASIS=SimpleNamespace(super="BASIS_GAUSS3D")), State, verbose=False)
+    with pytest.raises(VardynError):
+        Basis(SimpleNamespace(BASIS=SimpleNamespace(super=None)), State, verbose=False)
+    with pytest.raises(VardynError):
+        Basis(SimpleNamespace(BASIS=SimpleNamespace(super="BASIS_OFFSET", sigma_B=0.1)), State,
+              verbose=False, multi_mode=True)
