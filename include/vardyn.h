@@ -568,6 +568,10 @@ typedef struct vd_sw_params {
   double bottom_drag_coef;
   double h_min, h_min_sharpness; /* smooth clamp of the total thickness (defaults 0.1 m, 10) */
   double visc_coef, diff_coef;   /* must be 0 */
+  const double* wind_u;   /* top-layer wind-stress tendency on the u grid ((nx+1) x ny, scaled units:
+                             tau_x / (rho_water H_wind) dx, zero on land and on the boundary rows),
+                             or null for none (add_wind_forcing, sw.py:567-642) */
+  const double* wind_v;   /* the same on the v grid (nx x (ny+1)), or null */
 } vd_sw_params;
 
 /* Build the masks and upload the grid constants to the current device. */

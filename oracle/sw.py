@@ -28,9 +28,10 @@ Follows ``mapping/models/model_qgsw/`` of datlas-ocean/VarDyn:
 Scope (the options of the reference's default configuration,
 config_default.py:500-560): per-layer constant depths ``H``, scalar ``dx`` /
 ``dy``, any binary ocean mask, any ``f`` on the vorticity grid, slip
-coefficient, bottom drag.  Not restated: viscosity / thickness diffusion,
-wind stress, sponge, open boundaries and the barotropic filter (all zero
-by default there; the GPU model rejects them).
+coefficient, bottom drag, wind stress with a given ``h_wind``
+(``add_wind_forcing``, :567-642).  Not restated: viscosity / thickness
+diffusion, sponge, open boundaries and the barotropic filter (all zero by
+default there; the GPU model rejects them).
 
 The reference quirks are kept: the vorticity flux along y on the u grid
 blends the 6-point reconstruction with ``u_sten_wy_gt4`` (so where the
@@ -214,7 +215,8 @@ class SwOracle:
     grid."""
 
     def __init__(self, nx, ny, dx, dy, H, g_prime, f, dt, mask=None, slip_coef=0.0,
-                 bottom_drag_coef=0.0, h_min=0.1, h_min_sharpness=10.0):
+                 bottom_drag_coef=0.0, h_min=0.1, h_min_sharpness=10.0, taux=0.0, tauy=0.0,
+                 h_wind=None, rho_water=1025.0):
         self.nx, self.ny, self.nl = nx, ny, len(H)
         self.dx, self.dy = float(dx), float(dy)
         self.area = self.dx * self.dy
@@ -234,6 +236,19 @@ class SwOracle:
         f_v = 0.5 * (self.f[1:, :] + self.f[:-1, :])
         self.fstar_u = f_u * self.area
         self.fstar_v = f_v * self.area
+        # add_wind_forcing (sw.py:567-642) with h_wind given: H_ref from the
+        # scalar or the (nx, ny) field averaged onto the interior u / v points
+        self.wind_u = self.wind_v = None
+        if np.any(np.asarray(taux) != 0) or np.any(np.asarray(tauy) != 0):
+            hw = np.asarray(h_wind, dtype=float)
+            if hw.ndim >= 2:
+                Hu = np.maximum(0.5 * (hw[:-1, :] + hw[1:, :]), self.h_min)
+                Hv = np.maximum(0.5 * (hw[:, :-1] + hw[:, 1:]), self.h_min)
+            else:
+                Hu = Hv = np.maximum(hw, self.h_min)
+            mu, mv = self.m["u"][1:-1, :], self.m["v"][:, 1:-1]
+            self.wind_u = np.where(mu > 0.5, taux / (rho_water * Hu) * self.dx, np.zeros_like(mu))
+            self.wind_v = np.where(mv > 0.5, tauy / (rho_water * Hv) * self.dy, np.zeros_like(mv))
 
     # sw.py:653-671
     def omega(self, u, v):
@@ -276,7 +291,10 @@ class SwOracle:
         kep = ke + p
         dt_u = dt_u - (np.diff(kep, axis=-2) + 0.)
         dt_v = dt_v - (np.diff(kep, axis=-1) + 0.)
-        # bottom drag on the last layer (sw.py:644-651)
+        # wind forcing of the top layer (sw.py:567-642), then bottom drag (:644-651)
+        if self.wind_u is not None:
+            dt_u[0] = dt_u[0] + self.wind_u
+            dt_v[0] = dt_v[0] + self.wind_v
         dt_u[-1] = dt_u[-1] - self.drag * u[-1, 1:-1, :]
         dt_v[-1] = dt_v[-1] - self.drag * v[-1, :, 1:-1]
         dt_u = np.pad(dt_u, ((0, 0), (1, 1), (0, 0))) * m["u"]

@@ -9,7 +9,8 @@ tangent: <M dx, y> = <dx, M* y> within 1e-11 relative for several random
 1; the rest state exactly at rest; repeated calls bitwise equal.  Cases: a
 3-layer beta-plane basin, partial slip with bottom drag, a basin with an
 island and land corners (which exercises the 6-point blend at the 4-point
-vorticity faces), one layer, and the 256 x 256 three-layer grid of the
+vorticity faces), one layer, wind stress (a varying x stress, a uniform y
+stress, a mixed-layer depth field), and the 256 x 256 three-layer grid of the
 benchmark (a few steps, and the adjoint identity over 12 steps with the
 GPU's own tangent).
 """
@@ -23,8 +24,13 @@ from vardyn_amd.sw import SW
 pytestmark = pytest.mark.gpu
 
 
-def make(nx=20, ny=18, nl=3, slip=0.0, drag=0.0, island=False, seed=0, amp=1.0):
+def make(nx=20, ny=18, nl=3, slip=0.0, drag=0.0, island=False, seed=0, amp=1.0, wind=False):
     rng = np.random.default_rng(seed)
+    wkw = {}
+    if wind:  # a varying x stress, a uniform y stress, a mixed-layer depth field
+        wr = np.random.default_rng(99)
+        wkw = dict(taux=0.1 * (1 + 0.5 * wr.standard_normal((nx - 1, ny))), tauy=-0.05,
+                   h_wind=50.0 + 20.0 * wr.random((nx, ny)))
     dx = dy = 8e3
     y = (np.arange(ny + 1) - ny / 2) * dy
     f = np.broadcast_to(1e-4 + 1.6e-11 * y, (nx + 1, ny + 1)).copy()
@@ -32,9 +38,10 @@ def make(nx=20, ny=18, nl=3, slip=0.0, drag=0.0, island=False, seed=0, amp=1.0):
     mask = None
     if island:
         mask = np.ones((nx, ny)); mask[6:10, 5:9] = 0; mask[0, 0] = 0; mask[nx - 1, ny - 3:] = 0
-    orc = SwOracle(nx, ny, dx, dy, H, gp, f, 120.0, mask=mask, slip_coef=slip, bottom_drag_coef=drag)
+    orc = SwOracle(nx, ny, dx, dy, H, gp, f, 120.0, mask=mask, slip_coef=slip, bottom_drag_coef=drag, **wkw)
     param = dict(nx=nx, ny=ny, nl=nl, dx=dx, dy=dy, H=H.reshape(nl, 1, 1), g_prime=gp, f=f, dt=120.0,
                  mask=mask, slip_coef=slip, bottom_drag_coef=drag, taux=0., tauy=0.)
+    param.update(wkw)
     model = SW(param)
     m = orc.m
     x = [amp * 0.1 * rng.standard_normal((nl, nx + 1, ny)) * m["u"],
@@ -49,7 +56,7 @@ def rel(a, b):
 
 
 CASES = {"basin3": {}, "slip_drag": dict(slip=0.5, drag=1e-6), "island": dict(island=True),
-         "one_layer": dict(nl=1)}
+         "one_layer": dict(nl=1), "wind": dict(wind=True, island=True, drag=1e-6)}
 
 
 @pytest.mark.parametrize("case", list(CASES))

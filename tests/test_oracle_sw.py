@@ -331,3 +331,29 @@ def test_forcing_tangent_equals_finite_differences():
     for t, a, b in zip(tl, p, q):
         fd = (a - b) / (2 * eps)
         assert np.linalg.norm(t - fd) <= 1e-6 * np.linalg.norm(fd)
+
+
+@pytest.mark.parametrize("hw2d", [False, True])
+def test_wind_tendency_at_rest(hw2d):
+    """At rest the tendency is the wind term alone (add_wind_forcing,
+    sw.py:567-642): tau / (rho_water H_ref) times dx (u) or dy (v) on the top
+    layer's interior ocean points, H_ref the scalar h_wind clamped at h_min or
+    the mean of the two h-cells around the face; nothing below the top layer
+    and nothing in h.  dx != dy, so a swapped metric fails."""
+    nx, ny, nl, dx, dy = 12, 10, 3, 6e3, 9e3
+    f = np.full((nx + 1, ny + 1), 1e-4)
+    mask = np.ones((nx, ny)); mask[4:6, 3:5] = 0
+    rng = np.random.default_rng(5)
+    taux = 0.1 + 0.05 * rng.standard_normal((nx - 1, ny))
+    tauy = 0.07
+    hw = (40.0 + 30.0 * rng.random((nx, ny))) if hw2d else 0.05  # the scalar below h_min = 0.1
+    orc = SwOracle(nx, ny, dx, dy, np.array([500., 1000., 3000.]), np.array([0.02, 0.02, 0.01]), f, 120.0,
+                   mask=mask, taux=taux, tauy=tauy, h_wind=hw, rho_water=1000.0)
+    du, dv, dh = orc.rhs(np.zeros((nl, nx + 1, ny)), np.zeros((nl, nx, ny + 1)), np.zeros((nl, nx, ny)))
+    Hu = 0.5 * (hw[:-1, :] + hw[1:, :]) if hw2d else 0.1
+    Hv = 0.5 * (hw[:, :-1] + hw[:, 1:]) if hw2d else 0.1
+    mu, mv = orc.m["u"], orc.m["v"]
+    np.testing.assert_allclose(du[0, 1:-1, :], taux / (1000.0 * Hu) * dx * mu[1:-1, :], rtol=1e-14)
+    np.testing.assert_allclose(dv[0, :, 1:-1], tauy / (1000.0 * Hv) * dy * mv[:, 1:-1], rtol=1e-14)
+    assert np.all(du[0, [0, -1], :] == 0) and np.all(dv[0, :, [0, -1]] == 0)
+    assert np.all(du[1:] == 0) and np.all(dv[1:] == 0) and np.all(dh == 0)

@@ -53,6 +53,7 @@ class SwParams(C.Structure):
         ("slip_coef", C.c_double), ("bottom_drag_coef", C.c_double),
         ("h_min", C.c_double), ("h_min_sharpness", C.c_double),
         ("visc_coef", C.c_double), ("diff_coef", C.c_double),
+        ("wind_u", C.c_void_p), ("wind_v", C.c_void_p),
     ]
 
 

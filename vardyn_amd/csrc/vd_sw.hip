@@ -196,6 +196,8 @@ __global__ __launch_bounds__(256) void k_sw_tend(SwGrid g, St x, Fld om, Fld ht,
                                    j >= 2 && j + 3 <= g.ny);
       k = fw + g.fstar_u[i * g.ny + j] * vm;
       k = k - ((Ld<S>::at(kep.v, kep.t, ih_(g, l, i, j)) - Ld<S>::at(kep.v, kep.t, ih_(g, l, i - 1, j))) + 0.0);
+      // wind forcing of the top layer, then bottom drag (sw.py:504-505)
+      if (l == 0 && g.wind_u) k = k + Cst<S>::make(g.wind_u[i * g.ny + j]);
       if (l == g.nl - 1) k = k - g.drag * Ld<S>::at(x.u.v, x.u.t, iu_(g, l, i, j));
       k = k * g.mask_u[i * g.ny + j];
     }
@@ -212,6 +214,7 @@ __global__ __launch_bounds__(256) void k_sw_tend(SwGrid g, St x, Fld om, Fld ht,
                                    i >= 2 && i + 3 <= g.nx);
       k = -(fw + g.fstar_v[i * (g.ny + 1) + j] * um);
       k = k - ((Ld<S>::at(kep.v, kep.t, ih_(g, l, i, j)) - Ld<S>::at(kep.v, kep.t, ih_(g, l, i, j - 1))) + 0.0);
+      if (l == 0 && g.wind_v) k = k + Cst<S>::make(g.wind_v[i * (g.ny + 1) + j]);
       if (l == g.nl - 1) k = k - g.drag * Ld<S>::at(x.v.v, x.v.t, iv_(g, l, i, j));
       k = k * g.mask_v[i * (g.ny + 1) + j];
     }
@@ -818,6 +821,17 @@ int vd_sw_create(const vd_sw_params* P, vd_sw_model** out) {
       (rc = sw_alloc(m, m->ns, &m->aF))) {
     delete m;
     return rc;
+  }
+  // the wind tendency is a constant of the plan (no state dependence: nothing
+  // in the tangent or the adjoint)
+  if (P->wind_u || P->wind_v) {
+    std::vector<double> wu((size_t)(nx + 1) * ny, 0.0), wv((size_t)nx * (ny + 1), 0.0);
+    if (P->wind_u) std::copy(P->wind_u, P->wind_u + wu.size(), wu.begin());
+    if (P->wind_v) std::copy(P->wind_v, P->wind_v + wv.size(), wv.begin());
+    if ((rc = sw_upload(m, wu, &g.wind_u)) || (rc = sw_upload(m, wv, &g.wind_v))) {
+      delete m;
+      return rc;
+    }
   }
   *out = m;
   return VD_OK;

@@ -240,6 +240,8 @@ struct SwGrid {
   const uint8_t* code_hy;  // h flux along y, v grid
   const uint8_t* code_wy;  // vorticity flux along y, u grid
   const uint8_t* code_wx;  // vorticity flux along x, v grid
+  const double* wind_u;    // top-layer wind tendency on the u grid, or null
+  const double* wind_v;    // the same on the v grid
 };
 
 VD_HD inline int iu_(const SwGrid& g, int l, int i, int j) { return (l * (g.nx + 1) + i) * g.ny + j; }

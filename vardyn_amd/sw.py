@@ -76,10 +76,6 @@ class SW:
         for key in ("visc_coef", "diff_coef", "sponge_coef"):
             if float(p.get(key, 0.0) or 0.0) != 0.0:
                 raise VardynError(f"sw: {key} != 0 is not implemented")
-        for key in ("taux", "tauy"):
-            t = np.asarray(p.get(key, 0.0), dtype=np.float64)
-            if np.any(t != 0):
-                raise VardynError(f"sw: wind stress ({key}) is not implemented")
         for key in self._UNSUPPORTED_TRUE:
             if p.get(key, False):
                 raise VardynError(f"sw: {key} is not implemented")
@@ -91,11 +87,14 @@ class SW:
         self.bottom_drag_coef = float(p.get("bottom_drag_coef", 0.0))
         self.h_min = float(p.get("h_min", 0.1))
         self.h_min_sharpness = float(p.get("h_min_sharpness", 10.0))
-        self._keep = (self.H, self.g_prime, self.f, self.mask)
+        self.wind_u, self.wind_v = self._wind(p)
+        self._keep = (self.H, self.g_prime, self.f, self.mask, self.wind_u, self.wind_v)
         sp = SwParams(self.nx, self.ny, self.nl, self.dx, self.dy, self.dt,
                       self.H.ctypes.data, self.g_prime.ctypes.data, self.f.ctypes.data,
                       self.mask.ctypes.data if self.mask is not None else None,
-                      self.slip_coef, self.bottom_drag_coef, self.h_min, self.h_min_sharpness, 0.0, 0.0)
+                      self.slip_coef, self.bottom_drag_coef, self.h_min, self.h_min_sharpness, 0.0, 0.0,
+                      self.wind_u.ctypes.data if self.wind_u is not None else None,
+                      self.wind_v.ctypes.data if self.wind_v is not None else None)
         lib = _lib.load()
         h = C.c_void_p()
         with torch.cuda.device(self.device):
@@ -106,6 +105,41 @@ class SW:
         self.h_shape = (self.nl, self.nx, self.ny)
         m = _host_masks(self)
         self.mask_u, self.mask_v, self.mask_h = m
+
+    def _wind(self, p):
+        """add_wind_forcing (sw.py:567-642): the top-layer tendencies
+        tau / (rho_water H_ref) dx on the interior u points ((nx-1) x ny) and
+        tau / (rho_water H_ref) dy on the interior v points, H_ref from
+        ``h_wind`` (a scalar, or an (nx, ny) field averaged onto the u / v
+        points) clamped below at ``h_min``; returned on the full u and v grids
+        (zero boundary rows), or None without wind.  The kernel multiplies the
+        tendency by the u / v masks, so land points get none (the reference's
+        ``where``).  With per-layer constant depths the reference's fallback to
+        its reference thickness (no ``h_wind``) cannot broadcast, so ``h_wind``
+        is required here."""
+        nx, ny = self.nx, self.ny
+        taux = np.asarray(p.get("taux", 0.0), dtype=np.float64)
+        tauy = np.asarray(p.get("tauy", 0.0), dtype=np.float64)
+        if not (np.any(taux != 0) or np.any(tauy != 0)):
+            return None, None
+        if taux.ndim and taux.shape != (nx - 1, ny) or tauy.ndim and tauy.shape != (nx, ny - 1):
+            raise VardynError(f"sw: taux must be a scalar or ({nx - 1}, {ny}), tauy a scalar or ({nx}, {ny - 1})")
+        if p.get("h_wind") is None:
+            raise VardynError("sw: wind stress needs h_wind (the mixed-layer depth, m)")
+        rho = float(p.get("rho_water", 1025.0))
+        hw = np.asarray(p["h_wind"], dtype=np.float64)
+        if hw.ndim >= 2:
+            if hw.shape != (nx, ny):
+                raise VardynError(f"sw: h_wind must be a scalar or ({nx}, {ny})")
+            Hu = np.maximum(0.5 * (hw[:-1, :] + hw[1:, :]), self.h_min)
+            Hv = np.maximum(0.5 * (hw[:, :-1] + hw[:, 1:]), self.h_min)
+        else:
+            Hu = Hv = np.maximum(hw, self.h_min)
+        wu = np.zeros((nx + 1, ny))
+        wv = np.zeros((nx, ny + 1))
+        wu[1:-1, :] = taux / (rho * Hu) * self.dx
+        wv[:, 1:-1] = tauy / (rho * Hv) * self.dy
+        return np.ascontiguousarray(wu), np.ascontiguousarray(wv)
 
     def __del__(self):
         h = getattr(self, "_h", None)
