@@ -550,10 +550,11 @@ int vd_axpby(int precision, int64_t n, double a, const void* x, double b, void* 
  * units (velocities in m/s, h the layer-thickness perturbation in m); f on
  * the (nx+1) x (ny+1) vorticity grid.  Supported: per-layer constant depths,
  * scalar dx / dy, any binary ocean mask, slip coefficient, linear bottom
- * drag on the last layer, the smooth thickness clamp.  Viscosity and
- * thickness diffusion (sw.py:512-573) are rejected with VD_ERR_UNSUPPORTED;
- * wind stress, sponge, open boundaries and the barotropic filter have no
- * parameter (the reference's defaults: none of them).  External forcing
+ * drag on the last layer, the smooth thickness clamp, wind stress on the top
+ * layer, Laplacian viscosity of the velocities and diffusion of the
+ * thickness with zero-flux edges.  The sponge, open boundaries and the
+ * barotropic filter have no parameter (the reference's defaults: none of
+ * them).  External forcing
  * (Fu, Fv, Fh, direct momentum mode) is an argument of the step calls.
  * ------------------------------------------------------------------------- */
 typedef struct vd_sw_model vd_sw_model;
@@ -567,7 +568,8 @@ typedef struct vd_sw_params {
   double slip_coef;       /* 1 free slip, 0 no slip */
   double bottom_drag_coef;
   double h_min, h_min_sharpness; /* smooth clamp of the total thickness (defaults 0.1 m, 10) */
-  double visc_coef, diff_coef;   /* must be 0 */
+  double visc_coef, diff_coef;   /* Laplacian viscosity of u, v and diffusion of h (m^2/s); <= 0: none
+                                    (add_diffusion, add_h_diffusion, sw.py:511-565) */
   const double* wind_u;   /* top-layer wind-stress tendency on the u grid ((nx+1) x ny, scaled units:
                              tau_x / (rho_water H_wind) dx, zero on land and on the boundary rows),
                              or null for none (add_wind_forcing, sw.py:567-642) */

@@ -29,9 +29,12 @@ Scope (the options of the reference's default configuration,
 config_default.py:500-560): per-layer constant depths ``H``, scalar ``dx`` /
 ``dy``, any binary ocean mask, any ``f`` on the vorticity grid, slip
 coefficient, bottom drag, wind stress with a given ``h_wind``
-(``add_wind_forcing``, :567-642).  Not restated: viscosity / thickness
-diffusion, sponge, open boundaries and the barotropic filter (all zero by
-default there; the GPU model rejects them).
+(``add_wind_forcing``, :567-642), Laplacian viscosity of the velocities
+(``add_diffusion``, :511-546, its metrics constant: the u/v-grid dx and dy
+equal the scalar ones) and of the thickness (``add_h_diffusion``,
+:548-565), both with edge-padded (zero-flux) boundaries and only for a
+coefficient > 0 as there.  Not restated: sponge, open boundaries and the
+barotropic filter (all off by default there; the GPU model rejects them).
 
 The reference quirks are kept: the vorticity flux along y on the u grid
 blends the 6-point reconstruction with ``u_sten_wy_gt4`` (so where the
@@ -54,7 +57,10 @@ pressure gradient of a thickness ramp in layer 0 or 1 (layer coupling); the
 advection of a ramp with zero divergence in the first and last cell;
 vorticity of linear shears and at no-slip / partial / free-slip walls; the
 mirror symmetries in x and y (with f -> -f); complex-step tangents against
-central finite differences; the RK3-SSP amplification polynomial.  Parity
+central finite differences; the RK3-SSP amplification polynomial; the
+viscosity and thickness diffusion by closed forms on quadratics (which fix
+the edge padding and the metrics, dx != dy), summation by parts, mass
+conservation and where they enter the tendency.  Parity
 unpinned: the 6-point blend at the 4-point faces of the vorticity flux along
 y (restated from sw.py:285-293; it only acts next to land).
 """
@@ -216,7 +222,7 @@ class SwOracle:
 
     def __init__(self, nx, ny, dx, dy, H, g_prime, f, dt, mask=None, slip_coef=0.0,
                  bottom_drag_coef=0.0, h_min=0.1, h_min_sharpness=10.0, taux=0.0, tauy=0.0,
-                 h_wind=None, rho_water=1025.0):
+                 h_wind=None, rho_water=1025.0, visc_coef=0.0, diff_coef=0.0):
         self.nx, self.ny, self.nl = nx, ny, len(H)
         self.dx, self.dy = float(dx), float(dy)
         self.area = self.dx * self.dy
@@ -227,6 +233,8 @@ class SwOracle:
         self.dt = float(dt)
         self.slip = float(slip_coef)
         self.drag = float(bottom_drag_coef)
+        self.visc = float(visc_coef or 0.0)
+        self.diff = float(diff_coef or 0.0)
         self.h_min, self.sharp = float(h_min), float(h_min_sharpness)
         self.m = sw_masks(np.ones((nx, ny)) if mask is None else mask)
         # reference thicknesses (sw.py:325-349); constant depths give
@@ -262,6 +270,31 @@ class SwOracle:
         return (m["w_valid"] * curl + m["w_cornerout_bound"] * (1 - self.slip) * curl
                 + m["w_vertical_bound"] * alpha * dx_v - m["w_horizontal_bound"] * alpha * dy_u)
 
+    @staticmethod
+    def lap5(q, dx, dy):
+        """Five-point Laplacian of the interior of an array padded by one
+        point on both ends of its last two axes (sw.py:533-540, 561-562):
+        (q[i+1] - 2 q[i] + q[i-1]) / dx^2 + (q[j+1] - 2 q[j] + q[j-1]) / dy^2."""
+        c = q[..., 1:-1, 1:-1]
+        return ((q[..., 2:, 1:-1] - 2 * c + q[..., :-2, 1:-1]) / dx ** 2
+                + (q[..., 1:-1, 2:] - 2 * c + q[..., 1:-1, :-2]) / dy ** 2)
+
+    def viscosity(self, u, v):
+        """add_diffusion (sw.py:511-546): nu lap(u / dx) dx on the interior
+        u points and nu lap(v / dy) dy on the interior v points, u edge-padded
+        along y and v along x (zero flux through the walls); the other
+        neighbours are the boundary rows themselves."""
+        up = np.pad(u, ((0, 0), (0, 0), (1, 1)), mode="edge") / self.dx
+        vp = np.pad(v, ((0, 0), (1, 1), (0, 0)), mode="edge") / self.dy
+        return (self.visc * self.lap5(up, self.dx, self.dy) * self.dx,
+                self.visc * self.lap5(vp, self.dx, self.dy) * self.dy)
+
+    def h_diffusion(self, h):
+        """add_h_diffusion (sw.py:548-565): kappa lap(h / area) area on
+        every cell, h edge-padded on all four sides, masked."""
+        hp = np.pad(h / self.area, ((0, 0), (1, 1), (1, 1)), mode="edge")
+        return self.diff * self.lap5(hp, self.dx, self.dy) * self.area * self.m["h"]
+
     def rhs(self, u, v, h):
         """compute_time_derivatives (sw.py:723-750) on scaled variables."""
         m = self.m
@@ -281,6 +314,8 @@ class SwOracle:
         fx = flux(h_tot, U[..., 1:-1, :], -2, m["u_sten_hx_eq2"][1:-1, :], m["u_sten_hx_eq4"][1:-1, :],
                   m["u_sten_hx_gt6"][1:-1, :])
         dt_h = -div_nofluxbc(fx, fy) * m["h"]
+        if self.diff > 0:
+            dt_h = dt_h + self.h_diffusion(h)
         # advection_momentum (sw.py:481-510)
         om_Vm = flux(om[..., 1:-1, :], V_m, -1, m["u_sten_wy_eq2"][1:-1, :], m["u_sten_wy_eq4"][1:-1, :],
                      m["u_sten_wy_gt4"][1:-1, :])
@@ -297,6 +332,10 @@ class SwOracle:
             dt_v[0] = dt_v[0] + self.wind_v
         dt_u[-1] = dt_u[-1] - self.drag * u[-1, 1:-1, :]
         dt_v[-1] = dt_v[-1] - self.drag * v[-1, :, 1:-1]
+        if self.visc > 0:  # after the drag (sw.py:504-506)
+            vu, vv = self.viscosity(u, v)
+            dt_u = dt_u + vu
+            dt_v = dt_v + vv
         dt_u = np.pad(dt_u, ((0, 0), (1, 1), (0, 0))) * m["u"]
         dt_v = np.pad(dt_v, ((0, 0), (0, 0), (1, 1))) * m["v"]
         return dt_u, dt_v, dt_h

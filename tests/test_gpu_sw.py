@@ -10,7 +10,8 @@ tangent: <M dx, y> = <dx, M* y> within 1e-11 relative for several random
 3-layer beta-plane basin, partial slip with bottom drag, a basin with an
 island and land corners (which exercises the 6-point blend at the 4-point
 vorticity faces), one layer, wind stress (a varying x stress, a uniform y
-stress, a mixed-layer depth field), and the 256 x 256 three-layer grid of the
+stress, a mixed-layer depth field), Laplacian viscosity and thickness
+diffusion (with land and partial slip), and the 256 x 256 three-layer grid of the
 benchmark (a few steps, and the adjoint identity over 12 steps with the
 GPU's own tangent).
 """
@@ -24,14 +25,15 @@ from vardyn_amd.sw import SW
 pytestmark = pytest.mark.gpu
 
 
-def make(nx=20, ny=18, nl=3, slip=0.0, drag=0.0, island=False, seed=0, amp=1.0, wind=False):
+def make(nx=20, ny=18, nl=3, slip=0.0, drag=0.0, island=False, seed=0, amp=1.0, wind=False, visc=0.0, diff=0.0,
+         aspect=1.0):
     rng = np.random.default_rng(seed)
-    wkw = {}
+    wkw = dict(visc_coef=visc, diff_coef=diff) if visc or diff else {}
     if wind:  # a varying x stress, a uniform y stress, a mixed-layer depth field
         wr = np.random.default_rng(99)
         wkw = dict(taux=0.1 * (1 + 0.5 * wr.standard_normal((nx - 1, ny))), tauy=-0.05,
-                   h_wind=50.0 + 20.0 * wr.random((nx, ny)))
-    dx = dy = 8e3
+                   h_wind=50.0 + 20.0 * wr.random((nx, ny)), **wkw)
+    dx, dy = 8e3, 8e3 * aspect
     y = (np.arange(ny + 1) - ny / 2) * dy
     f = np.broadcast_to(1e-4 + 1.6e-11 * y, (nx + 1, ny + 1)).copy()
     H = np.array([500., 1000., 3000.][:nl]); gp = np.array([0.02, 0.02, 0.01][:nl])
@@ -56,7 +58,8 @@ def rel(a, b):
 
 
 CASES = {"basin3": {}, "slip_drag": dict(slip=0.5, drag=1e-6), "island": dict(island=True),
-         "one_layer": dict(nl=1), "wind": dict(wind=True, island=True, drag=1e-6)}
+         "one_layer": dict(nl=1), "wind": dict(wind=True, island=True, drag=1e-6),
+         "diffusion": dict(visc=2e4, diff=1.5e4, island=True, slip=0.5, aspect=1.3)}
 
 
 @pytest.mark.parametrize("case", list(CASES))

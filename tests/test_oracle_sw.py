@@ -357,3 +357,84 @@ def test_wind_tendency_at_rest(hw2d):
     np.testing.assert_allclose(dv[0, :, 1:-1], tauy / (1000.0 * Hv) * dy * mv[:, 1:-1], rtol=1e-14)
     assert np.all(du[0, [0, -1], :] == 0) and np.all(dv[0, :, [0, -1]] == 0)
     assert np.all(du[1:] == 0) and np.all(dv[1:] == 0) and np.all(dh == 0)
+
+
+def _diffusion_oracle(nx=11, ny=9, mask=None, visc=0.0, diff=0.0):
+    f = np.full((nx + 1, ny + 1), 1e-4)
+    return SwOracle(nx, ny, 6e3, 9e3, np.array([500., 1000.]), np.array([0.02, 0.01]), f, 120.0, mask=mask,
+                    visc_coef=visc, diff_coef=diff)
+
+
+def test_thickness_diffusion_closed_forms():
+    """add_h_diffusion (sw.py:548-565) on h / area = a i^2 + b j^2 (index
+    units): 2a / dx^2 + 2b / dy^2 inside; at the first cell of a direction
+    the edge padding leaves a / dx^2 (b / dy^2), at the last one
+    -a (2n - 3) / dx^2; times kappa area.  Summation by parts with the
+    edge padding: sum(q lap q) = -sum(diff_x q^2) / dx^2 - sum(diff_y q^2) / dy^2
+    for any q, so zero padding or a dropped boundary term fails; the total
+    diffusion tendency is zero (mass conserved)."""
+    nx, ny, kap = 11, 9, 250.0
+    o = _diffusion_oracle(nx, ny, diff=kap)
+    dx, dy, area = o.dx, o.dy, o.area
+    a, b = 0.3, -0.7
+    i, j = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    q = a * i ** 2 + b * j ** 2
+    got = o.h_diffusion((q * area)[None])[0] / (kap * area)
+    lx = np.full(nx, 2 * a / dx ** 2); lx[0] = a / dx ** 2; lx[-1] = -a * (2 * nx - 3) / dx ** 2
+    ly = np.full(ny, 2 * b / dy ** 2); ly[0] = b / dy ** 2; ly[-1] = -b * (2 * ny - 3) / dy ** 2
+    np.testing.assert_allclose(got, lx[:, None] + ly[None, :], rtol=1e-12, atol=1e-22)
+    rng = np.random.default_rng(2)
+    q = rng.standard_normal((2, nx, ny))
+    lap = o.h_diffusion(q * area) / (kap * area)
+    lhs = np.sum(q * lap, axis=(1, 2))
+    rhs = -np.sum(np.diff(q, axis=1) ** 2, axis=(1, 2)) / dx ** 2 - np.sum(np.diff(q, axis=2) ** 2, axis=(1, 2)) / dy ** 2
+    np.testing.assert_allclose(lhs, rhs, rtol=1e-12)
+    assert abs(np.sum(lap)) <= 1e-12 * np.sum(np.abs(lap))
+
+
+def test_viscosity_closed_forms():
+    """add_diffusion (sw.py:511-546) on u / dx = c i^2 + d j^2 and
+    v / dy = c i^2 + d j^2 (index units of each grid): on the interior u
+    points the x part is 2c / dx^2 everywhere (the boundary u rows are the
+    neighbours) and the y part is edge-padded (d / dy^2 at j = 0,
+    -d (2 ny - 3) / dy^2 at the last row); on the v points the other way
+    round; times nu dx (u) or nu dy (v)."""
+    nx, ny, nu = 11, 9, 400.0
+    o = _diffusion_oracle(nx, ny, visc=nu)
+    dx, dy = o.dx, o.dy
+    c, d = 0.4, -0.25
+    iu, ju = np.meshgrid(np.arange(nx + 1), np.arange(ny), indexing="ij")
+    iv, jv = np.meshgrid(np.arange(nx), np.arange(ny + 1), indexing="ij")
+    u = ((c * iu ** 2 + d * ju ** 2) * dx)[None]
+    v = ((c * iv ** 2 + d * jv ** 2) * dy)[None]
+    du, dv = o.viscosity(u, v)
+    ly = np.full(ny, 2 * d / dy ** 2); ly[0] = d / dy ** 2; ly[-1] = -d * (2 * ny - 3) / dy ** 2
+    np.testing.assert_allclose(du[0] / (nu * dx), 2 * c / dx ** 2 + ly[None, :] + 0 * du[0], rtol=1e-12)
+    lx = np.full(nx, 2 * c / dx ** 2); lx[0] = c / dx ** 2; lx[-1] = -c * (2 * nx - 3) / dx ** 2
+    np.testing.assert_allclose(dv[0] / (nu * dy), lx[:, None] + 2 * d / dy ** 2 + 0 * dv[0], rtol=1e-12)
+    assert du.shape == (1, nx - 1, ny) and dv.shape == (1, nx, ny - 1)
+
+
+def test_diffusion_terms_enter_the_tendency_masked():
+    """The viscosity is added after the bottom drag on the interior u / v
+    points and the whole tendency is masked (sw.py:504-509); the thickness
+    diffusion is added to the masked advection and masked itself
+    (:743, :564); a coefficient <= 0 adds nothing (:517, :555)."""
+    nx, ny = 12, 10
+    mask = np.ones((nx, ny)); mask[4:6, 3:5] = 0
+    rng = np.random.default_rng(8)
+    base = _diffusion_oracle(nx, ny, mask=mask)
+    m = base.m
+    x = [0.05 * rng.standard_normal((2, nx + 1, ny)) * m["u"], 0.05 * rng.standard_normal((2, nx, ny + 1)) * m["v"],
+         0.3 * rng.standard_normal((2, nx, ny)) * m["h"]]
+    o = _diffusion_oracle(nx, ny, mask=mask, visc=300.0, diff=200.0)
+    r0 = base.rhs(*x)
+    r1 = o.rhs(*x)
+    vu, vv = o.viscosity(x[0], x[1])
+    np.testing.assert_allclose(r1[0] - r0[0], np.pad(vu, ((0, 0), (1, 1), (0, 0))) * m["u"], rtol=1e-9, atol=1e-18)
+    np.testing.assert_allclose(r1[1] - r0[1], np.pad(vv, ((0, 0), (0, 0), (1, 1))) * m["v"], rtol=1e-9, atol=1e-18)
+    np.testing.assert_allclose(r1[2] - r0[2], o.h_diffusion(x[2]), rtol=1e-9, atol=1e-12)
+    assert np.all(o.h_diffusion(x[2])[:, mask == 0] == 0)
+    neg = _diffusion_oracle(nx, ny, mask=mask, visc=-5.0, diff=-5.0)
+    for a, b in zip(neg.rhs(*x), r0):
+        np.testing.assert_array_equal(a, b)

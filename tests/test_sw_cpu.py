@@ -1,7 +1,6 @@
 """Host logic of the shallow-water wrapper without a GPU: parameters the GPU
 model does not implement are rejected before any device call, and the C ABI
-rejects viscosity / diffusion with VD_ERR_UNSUPPORTED before touching a
-device."""
+validates its parameters before touching a device."""
 import ctypes as C
 
 import numpy as np
@@ -20,7 +19,7 @@ def param(**kw):
     return p
 
 
-@pytest.mark.parametrize("kw", [dict(visc_coef=10.0), dict(diff_coef=5.0), dict(taux=0.1),
+@pytest.mark.parametrize("kw", [dict(taux=0.1),
                                 dict(taux=0.1, h_wind=np.ones((3, 3))), dict(tauy=np.ones((12, 10)), h_wind=50.0),
                                 dict(barotropic_filter=True), dict(n_ens=2), dict(sponge_coef=0.1),
                                 dict(dx=np.linspace(8e3, 9e3, 12)), dict(H=np.ones((2, 12, 10))),
@@ -34,14 +33,14 @@ def test_unsupported_parameters_are_rejected(kw):
         SW(param(**kw))
 
 
-def test_capi_rejects_viscosity_without_a_device():
+def test_capi_validates_without_a_device():
     lib = _lib.load()
     H = np.array([500.0]); gp = np.array([0.02]); f = np.full((13, 11), 1e-4)
     p = SwParams(12, 10, 1, 8e3, 8e3, 120.0, H.ctypes.data, gp.ctypes.data, f.ctypes.data, None,
-                 0.0, 0.0, 0.1, 10.0, 1.0, 0.0)
+                 0.0, 0.0, 0.1, 0.0, 1.0, 0.0)
     h = C.c_void_p()
-    assert lib.vd_sw_create(C.byref(p), C.byref(h)) == -4
-    assert "viscosity" in _lib.last_error()
-    p.visc_coef = 0.0
+    assert lib.vd_sw_create(C.byref(p), C.byref(h)) == -1
+    assert "h_min_sharpness" in _lib.last_error()
+    p.h_min_sharpness = 10.0
     p.nl = 9
     assert lib.vd_sw_create(C.byref(p), C.byref(h)) == -1

@@ -11,7 +11,8 @@
 //   k_sw_tend  per (i, j, layer): the thickness tendency (the divergence of
 //              the two upwind-reconstructed fluxes of h_tot), the u and v
 //              tendencies (vortex force from the reconstructed vorticity,
-//              Coriolis, gradient of kinetic energy + pressure, bottom drag)
+//              Coriolis, gradient of kinetic energy + pressure, wind, bottom
+//              drag, viscosity; thickness diffusion)
 //              and the RK3-SSP stage update written to the next state buffer
 // Both are templates over the scalar type: double for the model, a dual
 // number (value, tangent) for the tangent-linear model, whose value part is
@@ -150,6 +151,32 @@ __global__ __launch_bounds__(256) void k_sw_diag(SwGrid g, St x, FldOut om, FldO
   Ld<S>::put(ht.v, ht.t, ih_(g, l, i, j), smooth_clamp(g.href[l] + hh, g.h_min_area, g.sharp));
 }
 
+// Five-point Laplacians of the edge-padded physical fields (add_diffusion,
+// sw.py:511-546; add_h_diffusion, :548-565): u / dx at an interior u point,
+// padded along y (the boundary u rows are its x neighbours); v / dy at an
+// interior v point, padded along x; h / area at any cell, padded both ways.
+template <typename S>
+__device__ __forceinline__ S lap5(const S& c, const S& xm, const S& xp, const S& ym, const S& yp, const SwGrid& g) {
+  return ((xp - 2.0 * c) + xm) / (g.dx * g.dx) + ((yp - 2.0 * c) + ym) / (g.dy * g.dy);
+}
+template <typename S>
+__device__ __forceinline__ S lap_u(const SwGrid& g, Fld u, int l, int i, int j) {
+  auto q = [&](int a, int b) { return Ld<S>::at(u.v, u.t, iu_(g, l, a, min(max(b, 0), g.ny - 1))) / g.dx; };
+  return lap5<S>(q(i, j), q(i - 1, j), q(i + 1, j), q(i, j - 1), q(i, j + 1), g);
+}
+template <typename S>
+__device__ __forceinline__ S lap_v(const SwGrid& g, Fld v, int l, int i, int j) {
+  auto q = [&](int a, int b) { return Ld<S>::at(v.v, v.t, iv_(g, l, min(max(a, 0), g.nx - 1), b)) / g.dy; };
+  return lap5<S>(q(i, j), q(i - 1, j), q(i + 1, j), q(i, j - 1), q(i, j + 1), g);
+}
+template <typename S>
+__device__ __forceinline__ S lap_h(const SwGrid& g, Fld h, int l, int i, int j) {
+  auto q = [&](int a, int b) {
+    return Ld<S>::at(h.v, h.t, ih_(g, l, min(max(a, 0), g.nx - 1), min(max(b, 0), g.ny - 1))) / g.area;
+  };
+  return lap5<S>(q(i, j), q(i - 1, j), q(i + 1, j), q(i, j - 1), q(i, j + 1), g);
+}
+
 // RK3-SSP stage STAGE at state x (sw.py:834-857): k = F(x), then
 //   0: x' = x + dt k            (k stored as k0)
 //   1: x' = x + dt/4 (k - 3 k0)  (k stored as k1)
@@ -182,7 +209,8 @@ __global__ __launch_bounds__(256) void k_sw_tend(SwGrid g, St x, Fld om, Fld ht,
     S divx = zero, divy = zero;
     if (i >= 1 && i <= g.nx - 2) divx = h_flux_x<S>(g, ht, x.u, l, i + 1, j) - h_flux_x<S>(g, ht, x.u, l, i, j);
     if (j >= 1 && j <= g.ny - 2) divy = h_flux_y<S>(g, ht, x.v, l, i, j + 1) - h_flux_y<S>(g, ht, x.v, l, i, j);
-    const S k = -(divy + divx) * g.mask_h[i * g.ny + j];
+    S k = -(divy + divx) * g.mask_h[i * g.ny + j];
+    if (g.diff > 0) k = k + ((g.diff * lap_h<S>(g, x.h, l, i, j)) * g.area) * g.mask_h[i * g.ny + j];
     update(x.h, k0.h, k1.h, kst.h, xn.h, ih_(g, l, i, j), k);
   }
   // u momentum (advection_momentum, sw.py:481-510)
@@ -199,6 +227,7 @@ __global__ __launch_bounds__(256) void k_sw_tend(SwGrid g, St x, Fld om, Fld ht,
       // wind forcing of the top layer, then bottom drag (sw.py:504-505)
       if (l == 0 && g.wind_u) k = k + Cst<S>::make(g.wind_u[i * g.ny + j]);
       if (l == g.nl - 1) k = k - g.drag * Ld<S>::at(x.u.v, x.u.t, iu_(g, l, i, j));
+      if (g.visc > 0) k = k + (g.visc * lap_u<S>(g, x.u, l, i, j)) * g.dx;
       k = k * g.mask_u[i * g.ny + j];
     }
     update(x.u, k0.u, k1.u, kst.u, xn.u, iu_(g, l, i, j), k);
@@ -216,6 +245,7 @@ __global__ __launch_bounds__(256) void k_sw_tend(SwGrid g, St x, Fld om, Fld ht,
       k = k - ((Ld<S>::at(kep.v, kep.t, ih_(g, l, i, j)) - Ld<S>::at(kep.v, kep.t, ih_(g, l, i, j - 1))) + 0.0);
       if (l == 0 && g.wind_v) k = k + Cst<S>::make(g.wind_v[i * (g.ny + 1) + j]);
       if (l == g.nl - 1) k = k - g.drag * Ld<S>::at(x.v.v, x.v.t, iv_(g, l, i, j));
+      if (g.visc > 0) k = k + (g.visc * lap_v<S>(g, x.v, l, i, j)) * g.dy;
       k = k * g.mask_v[i * (g.ny + 1) + j];
     }
     update(x.v, k0.v, k1.v, kst.v, xn.v, iv_(g, l, i, j), k);
@@ -417,10 +447,19 @@ __global__ __launch_bounds__(256) void k_sw_adj_gather1(SwGrid g, const double* 
   ws.pl(kAhp, l, nl)[w] = ahp + cum / g.area;
 }
 
+// Transposed edge-padded Laplacian at point (i, j): the cotangents m0 of the
+// point and of its four neighbours (zero where the forward Laplacian is not
+// evaluated), the padding along a direction folding the outside neighbour of
+// the first / last point back onto it (fold_* = 1 there).
+__device__ __forceinline__ double lap5_adj(double m0, double xm, double xp, double ym, double yp, int fold_x,
+                                           int fold_y, const SwGrid& g) {
+  return ((xm + xp) + (fold_x - 2.0) * m0) / (g.dx * g.dx) + ((ym + yp) + (fold_y - 2.0) * m0) / (g.dy * g.dy);
+}
+
 // lam_out (u, v, h) += F'(x)^T mu, given gather1's planes.
 __global__ __launch_bounds__(256) void k_sw_adj_gather2(SwGrid g, const double* xu, const double* xv,
-                                                        const double* lu, const double* lv, AdjWs ws,
-                                                        double* ou, double* ov, double* oh) {
+                                                        const double* lu, const double* lv, const double* lh,
+                                                        AdjWs ws, double* ou, double* ov, double* oh) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y * blockDim.y + threadIdx.y;
   const int l = blockIdx.z, nl = g.nl, W = g.ny + 1;
   if (i > g.nx || j > g.ny) return;
@@ -434,6 +473,10 @@ __global__ __launch_bounds__(256) void k_sw_adj_gather2(SwGrid g, const double* 
     double a = aU * idx2 + 0.5 * u * idx2 * ake;
     a += -g.wAC[i * W + j] * P(kAom, i, j) + g.wAC[i * W + j + 1] * P(kAom, i, j + 1);
     if (l == nl - 1) a += -g.drag * mu_u(g, lu, l, i, j);
+    if (g.visc > 0)  // (nu lap(u / dx)) dx
+      a += g.visc * lap5_adj(mu_u(g, lu, l, i, j), mu_u(g, lu, l, i - 1, j), mu_u(g, lu, l, i + 1, j),
+                             mu_u(g, lu, l, i, j - 1), mu_u(g, lu, l, i, j + 1), 0,
+                             (j == 0) + (j == g.ny - 1), g);
     ou[iu_(g, l, i, j)] += a;
   }
   if (i < g.nx) {  // v point (i, j)
@@ -444,9 +487,20 @@ __global__ __launch_bounds__(256) void k_sw_adj_gather2(SwGrid g, const double* 
     double a = aV * idy2 + 0.5 * v * idy2 * ake;
     a += g.wAB[i * W + j] * P(kAom, i, j) - g.wAB[(i + 1) * W + j] * P(kAom, i + 1, j);
     if (l == nl - 1) a += -g.drag * mu_v(g, lv, l, i, j);
+    if (g.visc > 0)  // (nu lap(v / dy)) dy
+      a += g.visc * lap5_adj(mu_v(g, lv, l, i, j), mu_v(g, lv, l, i - 1, j), mu_v(g, lv, l, i + 1, j),
+                             mu_v(g, lv, l, i, j - 1), mu_v(g, lv, l, i, j + 1), (i == 0) + (i == g.nx - 1), 0,
+                             g);
     ov[iv_(g, l, i, j)] += a;
   }
-  if (i < g.nx && j < g.ny) oh[ih_(g, l, i, j)] += P(kAhp, i, j);
+  if (i < g.nx && j < g.ny) {  // h cell (i, j); ((kappa lap(h / area)) area) mask_h
+    double a = P(kAhp, i, j);
+    if (g.diff > 0)
+      a += g.diff * lap5_adj(mu_h(g, lh, l, i, j), mu_h(g, lh, l, i - 1, j), mu_h(g, lh, l, i + 1, j),
+                             mu_h(g, lh, l, i, j - 1), mu_h(g, lh, l, i, j + 1), (i == 0) + (i == g.nx - 1),
+                             (j == 0) + (j == g.ny - 1), g);
+    oh[ih_(g, l, i, j)] += a;
+  }
 }
 
 // y_k = a_k y_k + b_k x (a_k = 0: y_k = b_k x) for up to three outputs.
@@ -691,7 +745,7 @@ int sw_rhs_adj(vd_sw_model* m, const double* x, const double* mu, hipStream_t s)
   const double *mu_u = mu, *mu_v = mu + m->nu, *mu_h = mu + m->nu + m->nv;
   hipLaunchKernelGGL(k_sw_adj_face, grd(g, g.nl), blk(), 0, s, g, x, x + m->nu, m->om, m->ht, mu_u, mu_v, mu_h, ws);
   hipLaunchKernelGGL(k_sw_adj_gather1, grd(g, g.nl), blk(), 0, s, g, x + m->nu + m->nv, mu_u, mu_v, ws);
-  hipLaunchKernelGGL(k_sw_adj_gather2, grd(g, g.nl), blk(), 0, s, g, x, x + m->nu, mu_u, mu_v, ws, m->L,
+  hipLaunchKernelGGL(k_sw_adj_gather2, grd(g, g.nl), blk(), 0, s, g, x, x + m->nu, mu_u, mu_v, mu_h, ws, m->L,
                      m->L + m->nu, m->L + m->nu + m->nv);
   VD_HIP_TRY(hipGetLastError());
   return VD_OK;
@@ -767,8 +821,6 @@ int vd_sw_create(const vd_sw_params* P, vd_sw_model** out) {
   VD_REQUIRE(P->dx > 0 && P->dy > 0 && P->dt > 0, VD_ERR_INVALID_ARG, "sw: dx, dy and dt must be positive");
   VD_REQUIRE(P->H && P->g_prime && P->f, VD_ERR_INVALID_ARG, "sw: H, g_prime and f are required");
   VD_REQUIRE(P->slip_coef >= 0 && P->slip_coef <= 1, VD_ERR_INVALID_ARG, "sw: slip coefficient must be in [0, 1]");
-  VD_REQUIRE(P->visc_coef == 0 && P->diff_coef == 0, VD_ERR_UNSUPPORTED,
-             "sw: viscosity and thickness diffusion are not implemented");
   VD_REQUIRE(P->h_min_sharpness > 0, VD_ERR_INVALID_ARG, "sw: h_min_sharpness must be positive");
   vd_sw_model* m = new vd_sw_model();
   (void)hipGetDevice(&m->dev);
@@ -783,6 +835,9 @@ int vd_sw_create(const vd_sw_params* P, vd_sw_model** out) {
   g.h_min_area = P->h_min * g.area;
   g.sharp = P->h_min_sharpness;
   g.drag = P->bottom_drag_coef;
+  // only a positive coefficient diffuses (sw.py:517, 555)
+  g.visc = P->visc_coef > 0 ? P->visc_coef : 0.0;
+  g.diff = P->diff_coef > 0 ? P->diff_coef : 0.0;
   for (int l = 0; l < 8; ++l) {
     g.href[l] = l < P->nl ? P->H[l] * g.area : 0.0;
     g.gp[l] = l < P->nl ? P->g_prime[l] : 0.0;

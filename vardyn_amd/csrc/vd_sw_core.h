@@ -226,6 +226,7 @@ struct SwGrid {
   double dt;
   double h_min_area, sharp;  // smooth clamp of h_ref + h at h_min dx dy
   double drag;               // linear bottom drag on the last layer
+  double visc, diff;         // Laplacian viscosity of u, v and diffusion of h (0: none)
   double href[8];            // H_l dx dy
   double gp[8];              // reduced gravities
   // grid-constant fields (device pointers)

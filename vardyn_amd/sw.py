@@ -12,10 +12,11 @@ ny), v (nl, nx, ny+1), h (nl, nx, ny), optionally with the ensemble axis of
 length 1 in front; f on the (nx+1, ny+1) vorticity grid.
 
 The model runs in libvardyn_hip (``vd_sw_*``, FP64).  Parameters the GPU
-model does not implement are rejected rather than ignored: viscosity and
-thickness diffusion, wind stress, sponge, open boundaries, the barotropic
-filter, spatially varying depths or metrics, and ensembles of more than
-one member.  External forcing (Fu, Fv, Fh; the direct momentum mode) is
+model does not implement are rejected rather than ignored: the sponge, open
+boundaries, the barotropic filter, spatially varying depths or metrics, and
+ensembles of more than one member.  Wind stress (with ``h_wind``), the
+Laplacian viscosity ``visc_coef`` and the thickness diffusion ``diff_coef``
+are built in.  External forcing (Fu, Fv, Fh; the direct momentum mode) is
 supported: ``step`` / ``step_tgl`` take it, ``step_adj_forced`` returns its
 cotangents.
 """
@@ -73,9 +74,12 @@ class SW:
                 raise VardynError("sw: the mask must be binary with at least one ocean cell")
             mask = np.ascontiguousarray(mask, dtype=np.uint8)
         self.mask = mask
-        for key in ("visc_coef", "diff_coef", "sponge_coef"):
-            if float(p.get(key, 0.0) or 0.0) != 0.0:
-                raise VardynError(f"sw: {key} != 0 is not implemented")
+        if float(p.get("sponge_coef", 0.0) or 0.0) != 0.0:
+            raise VardynError("sw: sponge_coef != 0 is not implemented")
+        # add_diffusion / add_h_diffusion (sw.py:511-565): only a positive
+        # coefficient diffuses
+        self.visc_coef = float(p.get("visc_coef", 0.0) or 0.0)
+        self.diff_coef = float(p.get("diff_coef", 0.0) or 0.0)
         for key in self._UNSUPPORTED_TRUE:
             if p.get(key, False):
                 raise VardynError(f"sw: {key} is not implemented")
@@ -92,7 +96,8 @@ class SW:
         sp = SwParams(self.nx, self.ny, self.nl, self.dx, self.dy, self.dt,
                       self.H.ctypes.data, self.g_prime.ctypes.data, self.f.ctypes.data,
                       self.mask.ctypes.data if self.mask is not None else None,
-                      self.slip_coef, self.bottom_drag_coef, self.h_min, self.h_min_sharpness, 0.0, 0.0,
+                      self.slip_coef, self.bottom_drag_coef, self.h_min, self.h_min_sharpness, self.visc_coef,
+                      self.diff_coef,
                       self.wind_u.ctypes.data if self.wind_u is not None else None,
                       self.wind_v.ctypes.data if self.wind_v is not None else None)
         lib = _lib.load()
