@@ -487,6 +487,27 @@ int vd_obs_misfit(const vd_obs_table* tab, int precision, const void* x, double*
 int vd_obs_adjoint(const vd_obs_table* tab, int precision, const double* cot,
                    const uint8_t* valid, void* adx, void* stream);
 
+/* Operator construction (Obsop_interp_l3_jax._sparse_op, obsop.py:417-460,
+ * which queries a scipy KDTree built over the grid's cartesian points,
+ * obsop.py:424-425).  vd_obs_grid_create takes the n_grid x 3 cartesian km of
+ * the grid points (geo2cart, grid.py:61-98; host memory) and dmax (km), and
+ * bins the points into cubes of side >= dmax on the device.
+ * vd_obs_grid_query returns, for each of n_obs observations (device xyz,
+ * n_obs x 3 cartesian km), the npix (1..8) nearest grid points within dmax in
+ * ascending distance (D = sqrt((dx^2 + dy^2) + dz^2), the KDTree's rounding;
+ * exact ties of D^2 go to the lower grid index): col (device int32, n_obs x
+ * npix; -1 past the neighbours found), w (device double, the normalised
+ * Gaussian weights exp(-D^2 / (2 (dmax/2)^2)) of obsop.py:436-447; 0 past
+ * them) and dist (device double or NULL; +inf past them).  Observations with
+ * non-finite coordinates get no neighbours (the Python operator raises on
+ * them, as the reference's KDTree query does).  The reference's Npix == 1 quirk
+ * (an empty operator, obsop.py:449-460) is the caller's. */
+typedef struct vd_obs_grid vd_obs_grid;
+int vd_obs_grid_create(const double* xyz, int64_t n_grid, double dmax, vd_obs_grid** grid);
+int vd_obs_grid_destroy(vd_obs_grid* grid);
+int vd_obs_grid_query(const vd_obs_grid* grid, const double* xyz, int n_obs, int npix, int32_t* col,
+                      double* w, double* dist, void* stream);
+
 /* ---------------------------------------------------------------------------
  * Overlapping tiles of QgmWithTiles (jqgm.py:1033-1258).  Tile t = ty * ntx +
  * tx starts at (ys[ty], xs[tx]) (device int32 arrays of nty and ntx starts,

@@ -95,6 +95,9 @@ SIGNATURES = {
     "vd_group_set_tuning": (_I, [_VP, C.POINTER(_I)]),
     "vd_obs_misfit": (_I, [C.POINTER(ObsTable), _I, _VP, _VP, _VP, _VP, _D, _VP]),
     "vd_obs_adjoint": (_I, [C.POINTER(ObsTable), _I, _VP, _VP, _VP, _VP]),
+    "vd_obs_grid_create": (_I, [_VP, C.c_int64, _D, C.POINTER(_VP)]),
+    "vd_obs_grid_destroy": (_I, [_VP]),
+    "vd_obs_grid_query": (_I, [_VP, _VP, _I, _I, _VP, _VP, _VP, _VP]),
     "vd_tiles_gather": (_I, [_I, _I, _I, _I, _I, _I, _VP, _VP, _VP, _VP, _VP, _VP]),
     "vd_tiles_blend": (_I, [_I, _I, _I, _I, _I, _I, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP]),
     "vd_tiles_blend_adj": (_I, [_I, _I, _I, _I, _I, _I, _VP, _VP, _VP, _VP, _VP, _VP, _VP]),

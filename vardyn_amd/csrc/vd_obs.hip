@@ -5,7 +5,10 @@
 // datlas-ocean/VarDyn.
 #include "vd_internal.h"
 
+#include <algorithm>
+#include <cmath>
 #include <mutex>
+#include <vector>
 
 namespace vd {
 
@@ -169,11 +172,198 @@ static int device_scratch(double** out) {
   return VD_OK;
 }
 
+// ------------------------------------------------------------ operator construction
+// The Npix nearest grid points of each observation that lie within dmax
+// (Obsop_interp_l3_jax._sparse_op, obsop.py:417-460: a scipy KDTree query for
+// the Npix nearest points, then the cut D <= dmax; the result is the Npix
+// nearest among the points within dmax).  The grid's cartesian points are
+// binned into cubes of side L >= dmax and sorted by cube key once, on the
+// host, at creation; a query visits the 3 x 3 x 3 cubes around the
+// observation's cube, which hold every point within dmax, as nine key ranges
+// (the three cubes along z have consecutive keys).  Distances are formed as
+// scipy's KDTree forms them, sqrt((dx^2 + dy^2) + dz^2) without contraction,
+// and ranked by their square with the grid index breaking exact ties.
+constexpr int kKnnMax = 8;
+constexpr int kKeyBits = 21;
+constexpr int64_t kKeyOff = int64_t(1) << (kKeyBits - 1);
+
+__host__ __device__ inline int64_t cube_key(int64_t cx, int64_t cy, int64_t cz) {
+  return ((cx + kKeyOff) << (2 * kKeyBits)) | ((cy + kKeyOff) << kKeyBits) | (cz + kKeyOff);
+}
+
+struct ObsGridDev {
+  int64_t n;
+  double dmax, cube;
+  const int64_t* key;  // sorted cube keys
+  const double* x;     // cartesian km of the points, in key order
+  const double* y;
+  const double* z;
+  const int32_t* idx;  // grid index of each point
+};
+
+__device__ __forceinline__ int64_t lower_bound_key(const int64_t* __restrict__ key, int64_t n, int64_t v) {
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (key[mid] < v) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// One thread per observation; xyz is n_obs x 3 (geo2cart of the
+// observation).  Slots past the neighbours found get col -1, weight 0 and
+// distance +inf.  Weights: exp(-D^2 / (2 (dmax / 2)^2)) normalised over the
+// observation's neighbours (obsop.py:436-447).
+__global__ __launch_bounds__(256) void k_obs_knn(ObsGridDev g, const double* __restrict__ xyz, int n_obs, int npix,
+                                                 int32_t* __restrict__ col, double* __restrict__ w,
+                                                 double* __restrict__ dist) {
+#pragma clang fp contract(off)
+  const int o = blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= n_obs) return;
+  double bd[kKnnMax];
+  int32_t bi[kKnnMax];
+#pragma unroll
+  for (int s = 0; s < kKnnMax; ++s) {
+    bd[s] = __builtin_inf();
+    bi[s] = -1;
+  }
+  const double px = xyz[3 * (size_t)o], py = xyz[3 * (size_t)o + 1], pz = xyz[3 * (size_t)o + 2];
+  if (isfinite(px) && isfinite(py) && isfinite(pz) && fabs(px) < 1e5 && fabs(py) < 1e5 && fabs(pz) < 1e5) {
+    const int64_t cx = (int64_t)floor(px / g.cube), cy = (int64_t)floor(py / g.cube),
+                  cz = (int64_t)floor(pz / g.cube);
+    for (int a = -1; a <= 1; ++a)
+      for (int b = -1; b <= 1; ++b) {
+        const int64_t first = lower_bound_key(g.key, g.n, cube_key(cx + a, cy + b, cz - 1));
+        const int64_t last = lower_bound_key(g.key, g.n, cube_key(cx + a, cy + b, cz + 1) + 1);
+        for (int64_t e = first; e < last; ++e) {
+          const double dx = px - g.x[e], dy = py - g.y[e], dz = pz - g.z[e];
+          double cd = (dx * dx + dy * dy) + dz * dz;
+          if (!(sqrt(cd) <= g.dmax)) continue;
+          int32_t ci = g.idx[e];
+          // insert (cd, ci) into the ascending list (bubble the larger one on)
+#pragma unroll
+          for (int s = 0; s < kKnnMax; ++s) {
+            const bool lt = bi[s] < 0 || cd < bd[s] || (cd == bd[s] && ci < bi[s]);
+            if (lt && ci >= 0) {
+              const double td = bd[s];
+              const int32_t ti = bi[s];
+              bd[s] = cd;
+              bi[s] = ci;
+              cd = td;
+              ci = ti;
+            }
+          }
+        }
+      }
+  }
+  const double h = 0.5 * g.dmax, den = 2.0 * (h * h);
+  double ws[kKnnMax], sum = 0.0;
+#pragma unroll
+  for (int s = 0; s < kKnnMax; ++s) {
+    ws[s] = 0.0;
+    if (s < npix && bi[s] >= 0) {
+      const double D = sqrt(bd[s]);
+      ws[s] = exp(-(D * D) / den);
+    }
+    if (s < npix) sum = sum + ws[s];
+  }
+#pragma unroll
+  for (int s = 0; s < kKnnMax; ++s) {
+    if (s >= npix) break;
+    const bool ok = bi[s] >= 0;
+    col[(size_t)o * npix + s] = ok ? bi[s] : -1;
+    w[(size_t)o * npix + s] = (ok && sum > 0.0) ? ws[s] / sum : 0.0;
+    if (dist) dist[(size_t)o * npix + s] = ok ? sqrt(bd[s]) : __builtin_inf();
+  }
+}
+
 }  // namespace vd
+
+struct vd_obs_grid {
+  int dev = 0;
+  vd::ObsGridDev g{};
+  std::vector<void*> owned;
+};
 
 using namespace vd;
 
 extern "C" {
+
+int vd_obs_grid_create(const double* xyz, int64_t n_grid, double dmax, vd_obs_grid** out) {
+  VD_REQUIRE(xyz && out, VD_ERR_INVALID_ARG, "null argument");
+  *out = nullptr;
+  VD_REQUIRE(n_grid > 0 && n_grid < (int64_t(1) << 31), VD_ERR_INVALID_ARG,
+             "obs grid: the number of grid points must be in [1, 2^31)");
+  VD_REQUIRE(std::isfinite(dmax) && dmax > 0, VD_ERR_INVALID_ARG, "obs grid: dmax must be positive and finite");
+  // cubes of side >= dmax (a point within dmax is at most one cube away along
+  // each axis; the margin covers the rounding of x / L), and large enough for
+  // every key to fit 21 bits per axis within |x| < 1e5 km
+  const double cube = std::max(dmax * (1.0 + 1e-9), 1e5 / double(kKeyOff - 4));
+  std::vector<int64_t> key(n_grid);
+  for (int64_t i = 0; i < n_grid; ++i) {
+    const double* p = xyz + 3 * i;
+    VD_REQUIRE(std::isfinite(p[0]) && std::isfinite(p[1]) && std::isfinite(p[2]) && std::fabs(p[0]) < 1e5 &&
+                   std::fabs(p[1]) < 1e5 && std::fabs(p[2]) < 1e5,
+               VD_ERR_INVALID_ARG, "obs grid: grid coordinates must be finite and within 1e5 km");
+    key[i] = cube_key((int64_t)std::floor(p[0] / cube), (int64_t)std::floor(p[1] / cube),
+                      (int64_t)std::floor(p[2] / cube));
+  }
+  std::vector<int32_t> order(n_grid);
+  for (int64_t i = 0; i < n_grid; ++i) order[i] = (int32_t)i;
+  std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return key[a] < key[b]; });
+  std::vector<int64_t> skey(n_grid);
+  std::vector<double> sx(n_grid), sy(n_grid), sz(n_grid);
+  for (int64_t k = 0; k < n_grid; ++k) {
+    const int32_t i = order[k];
+    skey[k] = key[i];
+    sx[k] = xyz[3 * (size_t)i];
+    sy[k] = xyz[3 * (size_t)i + 1];
+    sz[k] = xyz[3 * (size_t)i + 2];
+  }
+  vd_obs_grid* m = new vd_obs_grid();
+  (void)hipGetDevice(&m->dev);
+  auto up = [&](const void* src, size_t bytes, void** dst) -> hipError_t {
+    hipError_t e = hipMalloc(dst, bytes);
+    if (e != hipSuccess) return e;
+    m->owned.push_back(*dst);
+    return hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice);
+  };
+  void *dk, *dx, *dy, *dz, *di;
+  hipError_t e = up(skey.data(), n_grid * sizeof(int64_t), &dk);
+  if (e == hipSuccess) e = up(sx.data(), n_grid * sizeof(double), &dx);
+  if (e == hipSuccess) e = up(sy.data(), n_grid * sizeof(double), &dy);
+  if (e == hipSuccess) e = up(sz.data(), n_grid * sizeof(double), &dz);
+  if (e == hipSuccess) e = up(order.data(), n_grid * sizeof(int32_t), &di);
+  if (e != hipSuccess) {
+    vd_obs_grid_destroy(m);
+    VD_HIP_TRY(e);
+  }
+  m->g = ObsGridDev{n_grid, dmax, cube, (const int64_t*)dk, (const double*)dx, (const double*)dy,
+                    (const double*)dz, (const int32_t*)di};
+  *out = m;
+  return VD_OK;
+}
+
+int vd_obs_grid_destroy(vd_obs_grid* m) {
+  if (!m) return VD_OK;
+  for (void* p : m->owned) hipFree(p);
+  delete m;
+  return VD_OK;
+}
+
+int vd_obs_grid_query(const vd_obs_grid* m, const double* xyz, int n_obs, int npix, int32_t* col, double* w,
+                      double* dist, void* stream) {
+  VD_REQUIRE(m && col && w && (xyz || n_obs == 0), VD_ERR_INVALID_ARG, "null argument");
+  VD_REQUIRE(n_obs >= 0, VD_ERR_INVALID_ARG, "obs grid: negative observation count");
+  VD_REQUIRE(npix >= 1 && npix <= kKnnMax, VD_ERR_INVALID_ARG, "obs grid: npix must be in [1, 8]");
+  VD_REQUIRE((int64_t)n_obs * npix < (int64_t(1) << 31), VD_ERR_INVALID_ARG, "obs grid: too many observations");
+  if (n_obs == 0) return VD_OK;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(k_obs_knn, dim3((n_obs + 255) / 256), dim3(256), 0, s, m->g, xyz, n_obs, npix, col, w, dist);
+  VD_HIP_TRY(hipGetLastError());
+  return VD_OK;
+}
 
 int vd_obs_misfit(const vd_obs_table* t, int precision, const void* x, double* misfit,
                   uint8_t* valid, double* jo, double inv_sigma_r2, void* stream) {

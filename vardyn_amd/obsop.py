@@ -3,8 +3,11 @@
 Reference: ``Obsop_interp_l3_jax`` (mapping/src/obsop.py:375-678) and
 ``geo2cart`` (mapping/src/grid.py:61-98), datlas-ocean/VarDyn.
 
-Construction happens once on the host, with the same library the reference
-uses (``scipy.spatial.KDTree``), so the selected grid indices are identical:
+Construction runs on the device (``vd_obs_grid_*``): the grid's cartesian
+points are binned into cubes of side >= dmax once, and each observation
+searches the 27 cubes around it for its Npix nearest points within dmax,
+with the distances rounded as the reference's ``scipy.spatial.KDTree``
+rounds them, so the selected grid indices are the KDTree's:
 
 * time matching (obsop.py:387-403): model time t takes the nearest
   observation date when |t - date| <= dt/2 (first one on ties);
@@ -27,8 +30,6 @@ import ctypes as C
 
 import numpy as np
 import torch
-from scipy.spatial import KDTree
-
 from . import _lib
 from ._lib import ObsTable, VardynError
 from .qg import current_stream_ptr, ptr
@@ -65,20 +66,66 @@ def match_times(T, dt, dates):
     return t_obs, date_obs
 
 
-def interp_weights(tree, lon, lat, npix, dmax):
-    """(w, row, col) of ``_sparse_op`` (obsop.py:417-460)."""
-    if len(lon) == 0:
-        return np.zeros(0), np.zeros(0, np.int64), np.zeros(0, np.int64)
+class ObsGrid:
+    """The model grid's cartesian points (n x 3 km) binned on the device for
+    the neighbour queries of the operator construction (``vd_obs_grid_*``;
+    the reference builds a scipy KDTree over the same points,
+    obsop.py:424-425)."""
+
+    def __init__(self, coords_car, dmax, device):
+        xyz = np.ascontiguousarray(coords_car, dtype=np.float64)
+        if xyz.ndim != 2 or xyz.shape[1] != 3:
+            raise VardynError("ObsGrid: coordinates must be n x 3")
+        self.device = torch.device(device)
+        self.dmax = float(dmax)
+        self._lib = _lib.load()
+        h = C.c_void_p()
+        with torch.cuda.device(self.device):
+            _lib.check(self._lib.vd_obs_grid_create(xyz.ctypes.data, xyz.shape[0], self.dmax, C.byref(h)),
+                       "vd_obs_grid_create")
+        self._h = h
+
+    def query(self, xyz, npix):
+        """(col, w, dist) device tensors of n_obs x npix for the device
+        coordinates ``xyz`` (n_obs x 3 km): the npix nearest grid points
+        within dmax, nearest first, col -1 / w 0 / dist inf past them."""
+        xyz = torch.as_tensor(xyz, dtype=torch.float64, device=self.device).contiguous()
+        n = xyz.shape[0]
+        col = torch.empty((n, npix), dtype=torch.int32, device=self.device)
+        w = torch.empty((n, npix), dtype=torch.float64, device=self.device)
+        dist = torch.empty((n, npix), dtype=torch.float64, device=self.device)
+        with torch.cuda.device(self.device):
+            _lib.check(self._lib.vd_obs_grid_query(self._h, ptr(xyz), n, int(npix), ptr(col), ptr(w), ptr(dist),
+                                                   current_stream_ptr()), "vd_obs_grid_query")
+        return col, w, dist
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._lib.vd_obs_grid_destroy(h)
+            self._h = None
+
+
+def interp_weights(grid, lon, lat, npix):
+    """(w, row, col) of ``_sparse_op`` (obsop.py:417-460) from the device
+    neighbour search of ``grid`` (an ``ObsGrid``); entries in row order,
+    nearest first within a row (the reference's ``np.where(valid_mask)``
+    order).  Npix == 1, where the reference's squeezed KDTree output makes
+    its ``try`` fail, and no valid neighbour give an empty operator."""
+    empty = np.zeros(0), np.zeros(0, np.int64), np.zeros(0, np.int64)
+    if len(lon) == 0 or npix == 1:
+        return empty
     pts = geo2cart(np.column_stack((lon, lat)))
-    D, idx = tree.query(pts, k=npix)
-    valid = D <= dmax
-    if D.ndim != 2 or not np.any(valid):
-        return np.zeros(0), np.zeros(0, np.int64), np.zeros(0, np.int64)
-    w = np.exp(-D ** 2 / (2 * (0.5 * dmax) ** 2)) * valid
-    s = np.sum(w, axis=1, keepdims=True)
-    w = np.divide(w, s, out=np.zeros_like(w), where=s > 0)
+    if not np.all(np.isfinite(pts)):
+        # the reference's KDTree query raises on them
+        raise VardynError("observation coordinates must be finite")
+    col, w, _ = grid.query(pts, npix)
+    col, w = col.cpu().numpy(), w.cpu().numpy()
+    valid = col >= 0
+    if not np.any(valid):
+        return empty
     row, k = np.where(valid)
-    return w[row, k], row.astype(np.int64), idx[row, k].astype(np.int64)
+    return w[row, k], row.astype(np.int64), col[row, k].astype(np.int64)
 
 
 class _DevTable:
@@ -154,11 +201,15 @@ class Obsop_interp_l3:
         """Build and upload the per-time tables (obsop.py:486-604)."""
         if var_bc is not None:
             raise NotImplementedError("boundary-field anomalies of observations are not supported")
-        tree = KDTree(self.coords_car)
+        # a non-finite dmax (NaN metrics) keeps no neighbour, as D <= NaN
+        grid = ObsGrid(self.coords_car, self.dmax, self.device) if np.isfinite(self.dmax) and self.dmax > 0 else None
         built = []
         for date, t in zip(self.date_obs, self.t_obs):
             lon, lat, y, err = (np.asarray(a, dtype=np.float64).ravel() for a in self.dict_obs[date])
-            w, row, col = interp_weights(tree, lon, lat, self.Npix, self.dmax)
+            if grid is None:
+                w, row, col = np.zeros(0), np.zeros(0, np.int64), np.zeros(0, np.int64)
+            else:
+                w, row, col = interp_weights(grid, lon, lat, self.Npix)
             built.append((y.size, w, row, col, y, err))
         nd_max = max((b[1].size for b in built), default=0)
         self.tables = []
