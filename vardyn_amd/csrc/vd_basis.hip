@@ -17,12 +17,12 @@
 //   k_bm_time   C[c][q][t] = sum_it g_f(t, it) X[f, it, P, q]   (q: direction, cos/sin)
 //   k_bm_synth  ssh[t][p] = sum over centres c covering p, q of G(p, c, q) C[c][q][t],
 //               per 16 x 16 tile of grid points: the coefficients of the centres
-//               covering the tile are staged in LDS for 8 times at a time, so
+//               covering the tile are staged in LDS for 16 times at a time, so
 //               that every point reads them as LDS broadcasts
 //   k_bm_proj   Dt[e][q][t] = sum over the points of a 16 x 16 tile in the box of
 //               centre c of G(p, c, q) adssh[t][p], for every (tile, centre)
 //               entry e of the tile lists (the tile's adjoint fields in LDS,
-//               8 times per pass)
+//               16 times per pass)
 //   k_bm_proj_sum  D[c][q][t] = sum of c's entries in ascending tile order
 //   k_bm_time_adj  adX[f, it, P, q] += sum_t g_f(t, it) D[c][q][t]
 // The phase is separable on the rectilinear grid: dx depends on the column
@@ -41,9 +41,9 @@ namespace vd {
 
 constexpr double kKm2Deg = 1.0 / 110;
 constexpr int kBmMaxTheta = 16;  // directions per frequency (9 with the reference defaults)
-constexpr int kBmTB = 8;         // times per pass of the synthesis gather
+constexpr int kBmTB = 16;        // times per pass of the synthesis gather and the projection
 constexpr int kBmTile = 16;      // grid-point tile side of the synthesis
-constexpr int kBmCC = 24;        // centres staged in LDS at a time
+constexpr int kBmCC = 12;        // centres staged in LDS at a time (48 KiB of coefficients)
 constexpr int kBmParts = 4;      // lanes per (centre, direction) pair of the projection
 constexpr bool kBmPartCols = true;  // those lanes take interleaved columns (else row bands)
 
@@ -217,12 +217,12 @@ __global__ __launch_bounds__(256) void k_bm_synth(BmTables B, int T, int t0,
   }
 }
 
-// Dt[e][q][t] for times t0 .. t0 + 7 of the tile's entries e: a workgroup per
+// Dt[e][q][t] for times t0 .. t0 + 15 of the tile's entries e: a workgroup per
 // 16 x 16 tile of points holds their adjoint fields in LDS (zero on masked
 // points and outside the grid).  An (entry, direction) pair is taken by PARTS
 // consecutive lanes, each sweeping 16 / PARTS rows of the tile's points inside
 // the centre's box in row-major order (one table lookup per point for the
-// cosine and sine columns of all 8 times); the lanes' partial sums are combined
+// cosine and sine columns of all 16 times); the lanes' partial sums are combined
 // by butterfly shuffles, a fixed tree (the same bits on every run).  With COLS
 // the lanes of a pair take interleaved columns instead of row bands, so that
 // they read adjacent LDS words.
