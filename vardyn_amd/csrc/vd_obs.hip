@@ -232,10 +232,12 @@ __global__ __launch_bounds__(256) void k_obs_knn(ObsGridDev g, const double* __r
   if (isfinite(px) && isfinite(py) && isfinite(pz) && fabs(px) < 1e5 && fabs(py) < 1e5 && fabs(pz) < 1e5) {
     const int64_t cx = (int64_t)floor(px / g.cube), cy = (int64_t)floor(py / g.cube),
                   cz = (int64_t)floor(pz / g.cube);
-    for (int a = -1; a <= 1; ++a)
-      for (int b = -1; b <= 1; ++b) {
-        const int64_t first = lower_bound_key(g.key, g.n, cube_key(cx + a, cy + b, cz - 1));
-        const int64_t last = lower_bound_key(g.key, g.n, cube_key(cx + a, cy + b, cz + 1) + 1);
+    for (int ab = 0; ab < 9; ++ab) {
+      const int a = ab / 3 - 1, b = ab % 3 - 1;
+      // the z-column of three cubes: keys [key(cz - 1), key(cz + 1)]
+      const int64_t first = lower_bound_key(g.key, g.n, cube_key(cx + a, cy + b, cz - 1));
+      const int64_t last = lower_bound_key(g.key, g.n, cube_key(cx + a, cy + b, cz + 1) + 1);
+      {
         for (int64_t e = first; e < last; ++e) {
           const double dx = px - g.x[e], dy = py - g.y[e], dz = pz - g.z[e];
           double cd = (dx * dx + dy * dy) + dz * dz;
@@ -256,6 +258,7 @@ __global__ __launch_bounds__(256) void k_obs_knn(ObsGridDev g, const double* __r
           }
         }
       }
+    }
   }
   const double h = 0.5 * g.dmax, den = 2.0 * (h * h);
   double ws[kKnnMax], sum = 0.0;

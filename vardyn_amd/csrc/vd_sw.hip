@@ -22,8 +22,8 @@
 // every sum has a fixed order (no atomics) and the result is deterministic:
 //   k_sw_adj_face     per face and velocity point: the flux cotangent times
 //                     the partial derivatives of the flux with respect to its
-//                     six stencil values (6-component dual numbers through the
-//                     same reconstruction code) and its velocity
+//                     six stencil values (a reverse sweep through the
+//                     reconstruction) and its velocity
 //   k_sw_adj_gather1  per w point and layer: the vorticity cotangent; per h
 //                     cell and layer: the h_tot, kinetic-energy and pressure
 //                     cotangents (the pressure's layer sums in reverse)
@@ -44,7 +44,6 @@ namespace vd {
 namespace sw {
 
 typedef Dn<1> D1;
-typedef Dn<6> D6;
 
 struct St {
   Fld u, v, h;
@@ -282,20 +281,14 @@ struct AdjWs {
   __device__ __forceinline__ double* pl(int q, int l, int nl) const { return p + ((size_t)q * nl + l) * nw; }
 };
 
-// Partials of vel * recon(s) with respect to s[0..5] (times vel) and vel.
+// Partials of vel * recon(s) with respect to s[0..5] (times vel) and vel,
+// by the reverse sweep of the reconstruction.
 __device__ __forceinline__ void face_partials(const double* s, double vel, uint8_t code, bool ok4, bool ok6,
                                               double* ds, double* dvel) {
-  D6 q6[6];
+  double g[6];
+  *dvel = face_recon_grad(s, vel > 0, code, ok4, ok6, g);
 #pragma unroll
-  for (int k = 0; k < 6; ++k) {
-    q6[k].v = s[k];
-#pragma unroll
-    for (int m = 0; m < 6; ++m) q6[k].d[m] = k == m ? 1.0 : 0.0;
-  }
-  const D6 q = face_recon(q6, vel > 0, code, ok4, ok6);
-#pragma unroll
-  for (int k = 0; k < 6; ++k) ds[k] = vel * q.d[k];
-  *dvel = q.v;
+  for (int k = 0; k < 6; ++k) ds[k] = vel * g[k];
 }
 
 __device__ __forceinline__ double mu_h(const SwGrid& g, const double* lh, int l, int i, int j) {

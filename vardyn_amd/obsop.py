@@ -106,26 +106,40 @@ class ObsGrid:
             self._h = None
 
 
-def interp_weights(grid, lon, lat, npix):
-    """(w, row, col) of ``_sparse_op`` (obsop.py:417-460) from the device
-    neighbour search of ``grid`` (an ``ObsGrid``); entries in row order,
-    nearest first within a row (the reference's ``np.where(valid_mask)``
-    order).  Npix == 1, where the reference's squeezed KDTree output makes
-    its ``try`` fail, and no valid neighbour give an empty operator."""
-    empty = np.zeros(0), np.zeros(0, np.int64), np.zeros(0, np.int64)
-    if len(lon) == 0 or npix == 1:
-        return empty
-    pts = geo2cart(np.column_stack((lon, lat)))
+def interp_weights_many(grid, sets, npix):
+    """(w, row, col) of ``_sparse_op`` (obsop.py:417-460) for each (lon, lat)
+    set of ``sets``, from one device neighbour search of ``grid`` (an
+    ``ObsGrid``) over all of them; entries in row order, nearest first within
+    a row (the reference's ``np.where(valid_mask)`` order).  Npix == 1, where
+    the reference's squeezed KDTree output makes its ``try`` fail, and no
+    valid neighbour give an empty operator; non-finite coordinates raise, as
+    the reference's KDTree query does."""
+    def empty():
+        return np.zeros(0), np.zeros(0, np.int64), np.zeros(0, np.int64)
+    sizes = [len(lon) for lon, _ in sets]
+    if npix == 1 or sum(sizes) == 0:
+        return [empty() for _ in sets]
+    pts = np.concatenate([geo2cart(np.column_stack((lon, lat))) for lon, lat in sets if len(lon)])
     if not np.all(np.isfinite(pts)):
-        # the reference's KDTree query raises on them
         raise VardynError("observation coordinates must be finite")
     col, w, _ = grid.query(pts, npix)
     col, w = col.cpu().numpy(), w.cpu().numpy()
-    valid = col >= 0
-    if not np.any(valid):
-        return empty
-    row, k = np.where(valid)
-    return w[row, k], row.astype(np.int64), col[row, k].astype(np.int64)
+    out, start = [], 0
+    for n in sizes:
+        c, ww = col[start:start + n], w[start:start + n]
+        start += n
+        valid = c >= 0
+        if not np.any(valid):
+            out.append(empty())
+            continue
+        row, k = np.where(valid)
+        out.append((ww[row, k], row.astype(np.int64), c[row, k].astype(np.int64)))
+    return out
+
+
+def interp_weights(grid, lon, lat, npix):
+    """``interp_weights_many`` for one set of observations."""
+    return interp_weights_many(grid, [(lon, lat)], npix)[0]
 
 
 class _DevTable:
@@ -203,14 +217,13 @@ class Obsop_interp_l3:
             raise NotImplementedError("boundary-field anomalies of observations are not supported")
         # a non-finite dmax (NaN metrics) keeps no neighbour, as D <= NaN
         grid = ObsGrid(self.coords_car, self.dmax, self.device) if np.isfinite(self.dmax) and self.dmax > 0 else None
-        built = []
-        for date, t in zip(self.date_obs, self.t_obs):
-            lon, lat, y, err = (np.asarray(a, dtype=np.float64).ravel() for a in self.dict_obs[date])
-            if grid is None:
-                w, row, col = np.zeros(0), np.zeros(0, np.int64), np.zeros(0, np.int64)
-            else:
-                w, row, col = interp_weights(grid, lon, lat, self.Npix)
-            built.append((y.size, w, row, col, y, err))
+        data = [tuple(np.asarray(a, dtype=np.float64).ravel() for a in self.dict_obs[date])
+                for date in self.date_obs]
+        if grid is None:
+            tabs = [(np.zeros(0), np.zeros(0, np.int64), np.zeros(0, np.int64)) for _ in data]
+        else:  # every date in one search
+            tabs = interp_weights_many(grid, [(d[0], d[1]) for d in data], self.Npix)
+        built = [(d[2].size, w, row, col, d[2], d[3]) for d, (w, row, col) in zip(data, tabs)]
         nd_max = max((b[1].size for b in built), default=0)
         self.tables = []
         for n_obs, w, row, col, y, err in built:
