@@ -113,6 +113,21 @@ def test_reference_adjoint_test_and_determinism():
         assert torch.equal(p, q)
 
 
+def test_sponge_coefficient_changes_nothing():
+    """The reference's sponge fields are zero after construction
+    (sw.py:374-377), so a sponge coefficient leaves the steps unchanged."""
+    _, model, x, _ = make(island=True)
+    p = dict(nx=model.nx, ny=model.ny, nl=model.nl, dx=model.dx, dy=model.dy, H=model.H.reshape(-1, 1, 1),
+             g_prime=model.g_prime, f=model.f, dt=model.dt, mask=model.mask, slip_coef=model.slip_coef,
+             sponge_coef=0.7)
+    sp = SW(p)
+    a = model.step(*x, nstep=3)
+    b = sp.step(*x, nstep=3)
+    torch.cuda.synchronize()
+    for p_, q_ in zip(a, b):
+        assert torch.equal(p_, q_)
+
+
 def test_rest_state_and_ensemble_axis():
     _, model, x, _ = make(island=True)
     z = [np.zeros((1,) + a.shape) for a in x]

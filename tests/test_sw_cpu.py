@@ -21,7 +21,7 @@ def param(**kw):
 
 @pytest.mark.parametrize("kw", [dict(taux=0.1),
                                 dict(taux=0.1, h_wind=np.ones((3, 3))), dict(tauy=np.ones((12, 10)), h_wind=50.0),
-                                dict(barotropic_filter=True), dict(n_ens=2), dict(sponge_coef=0.1),
+                                dict(barotropic_filter=True), dict(n_ens=2),
                                 dict(dx=np.linspace(8e3, 9e3, 12)), dict(H=np.ones((2, 12, 10))),
                                 dict(f=np.zeros((12, 10))), dict(mask=np.zeros((12, 10))),
                                 dict(flag_obc=True), dict(forcing_momentum="mass_consistent")])

@@ -180,11 +180,28 @@ __device__ __forceinline__ S lap_h(const SwGrid& g, Fld h, int l, int i, int j) 
 //   0: x' = x + dt k            (k stored as k0)
 //   1: x' = x + dt/4 (k - 3 k0)  (k stored as k1)
 //   2: x' = x + dt/12 (8 k - k1 - k0)
+// The thickness fluxes of the block's faces are computed once each into LDS
+// (the x faces i0 .. i0+4 of its 64 columns, the y faces j0 .. j0+64 of its 4
+// rows) and differenced from there: 10 flux evaluations per thread group of
+// 64 instead of 16 when each cell evaluated its four faces.
+constexpr int kSwBX = 64, kSwBY = 4;
 template <typename S, int STAGE>
 __global__ __launch_bounds__(256) void k_sw_tend(SwGrid g, St x, Fld om, Fld ht, Fld kep, St k0, St k1,
                                                  StOut kst, StOut xn) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y * blockDim.y + threadIdx.y;
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int j0 = blockIdx.x * kSwBX, i0 = blockIdx.y * kSwBY;
+  const int j = j0 + tx, i = i0 + ty;
   const int l = blockIdx.z;
+  const S zero = Cst<S>::make(0.0);
+  __shared__ S fxs[kSwBY + 1][kSwBX];  // x face (u point) i0 + r at column j0 + c
+  __shared__ S fys[kSwBY][kSwBX + 1];  // y face (v point) j0 + c at row i0 + r
+  auto fx = [&](int iu, int jj) { return (iu >= 1 && iu <= g.nx - 1 && jj < g.ny) ? h_flux_x<S>(g, ht, x.u, l, iu, jj) : zero; };
+  auto fy = [&](int ii, int jv) { return (ii < g.nx && jv >= 1 && jv <= g.ny - 1) ? h_flux_y<S>(g, ht, x.v, l, ii, jv) : zero; };
+  fxs[ty][tx] = fx(i, j);
+  fys[ty][tx] = fy(i, j);
+  if (ty == 0) fxs[kSwBY][tx] = fx(i0 + kSwBY, j);
+  if (ty == 1 && tx < kSwBY) fys[tx][kSwBX] = fy(i0 + tx, j0 + kSwBX);
+  __syncthreads();
   if (i > g.nx || j > g.ny) return;
   const double dt = g.dt;
   auto update = [&](const Fld& xf, const Fld& k0f, const Fld& k1f, const FldOut& kf, const FldOut& out, int idx,
@@ -202,12 +219,11 @@ __global__ __launch_bounds__(256) void k_sw_tend(SwGrid g, St x, Fld om, Fld ht,
     }
     Ld<S>::put(out.v, out.t, idx, r);
   };
-  const S zero = Cst<S>::make(0.0);
   // thickness (advection_h, sw.py:465-479; div_nofluxbc, finite_diff.py:42-52)
   if (i < g.nx && j < g.ny) {
     S divx = zero, divy = zero;
-    if (i >= 1 && i <= g.nx - 2) divx = h_flux_x<S>(g, ht, x.u, l, i + 1, j) - h_flux_x<S>(g, ht, x.u, l, i, j);
-    if (j >= 1 && j <= g.ny - 2) divy = h_flux_y<S>(g, ht, x.v, l, i, j + 1) - h_flux_y<S>(g, ht, x.v, l, i, j);
+    if (i >= 1 && i <= g.nx - 2) divx = fxs[ty + 1][tx] - fxs[ty][tx];
+    if (j >= 1 && j <= g.ny - 2) divy = fys[ty][tx + 1] - fys[ty][tx];
     S k = -(divy + divx) * g.mask_h[i * g.ny + j];
     if (g.diff > 0) k = k + ((g.diff * lap_h<S>(g, x.h, l, i, j)) * g.area) * g.mask_h[i * g.ny + j];
     update(x.h, k0.h, k1.h, kst.h, xn.h, ih_(g, l, i, j), k);
@@ -680,8 +696,8 @@ StOut sto_of(const vd_sw_model* m, double* base, double* tan) {
                {base + m->nu + m->nv, tan ? tan + m->nu + m->nv : nullptr}};
 }
 
-dim3 blk() { return dim3(64, 4, 1); }
-dim3 grd(const SwGrid& g, int nz) { return dim3((g.ny + 1 + 63) / 64, (g.nx + 1 + 3) / 4, nz); }
+dim3 blk() { return dim3(kSwBX, kSwBY, 1); }
+dim3 grd(const SwGrid& g, int nz) { return dim3((g.ny + 1 + kSwBX - 1) / kSwBX, (g.nx + 1 + kSwBY - 1) / kSwBY, nz); }
 
 // One RK3-SSP step x <- step(x) on buffer x (value, tangent when TAN),
 // through the stage buffers B, C.

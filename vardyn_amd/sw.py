@@ -12,9 +12,10 @@ ny), v (nl, nx, ny+1), h (nl, nx, ny), optionally with the ensemble axis of
 length 1 in front; f on the (nx+1, ny+1) vorticity grid.
 
 The model runs in libvardyn_hip (``vd_sw_*``, FP64).  Parameters the GPU
-model does not implement are rejected rather than ignored: the sponge, open
-boundaries, the barotropic filter, spatially varying depths or metrics, and
-ensembles of more than one member.  Wind stress (with ``h_wind``), the
+model does not implement are rejected rather than ignored: open boundaries,
+the barotropic filter, spatially varying depths or metrics, and ensembles of
+more than one member.  A ``sponge_coef`` is accepted: the reference's sponge
+fields are zero (sw.py:374-377), so its relaxation adds nothing.  Wind stress (with ``h_wind``), the
 Laplacian viscosity ``visc_coef`` and the thickness diffusion ``diff_coef``
 are built in.  External forcing (Fu, Fv, Fh; the direct momentum mode) is
 supported: ``step`` / ``step_tgl`` take it, ``step_adj_forced`` returns its
@@ -74,8 +75,11 @@ class SW:
                 raise VardynError("sw: the mask must be binary with at least one ocean cell")
             mask = np.ascontiguousarray(mask, dtype=np.uint8)
         self.mask = mask
-        if float(p.get("sponge_coef", 0.0) or 0.0) != 0.0:
-            raise VardynError("sw: sponge_coef != 0 is not implemented")
+        # the sponge (sw.py:825-850) relaxes towards boundary fields with rates
+        # sponge_coef / dt times the sponge fields, which the constructor
+        # sets to zero (sw.py:374-377) and nothing in the model changes: any
+        # sponge_coef leaves the steps unchanged
+        self.sponge_coef = float(p.get("sponge_coef", 0.0) or 0.0)
         # add_diffusion / add_h_diffusion (sw.py:511-565): only a positive
         # coefficient diffuses
         self.visc_coef = float(p.get("visc_coef", 0.0) or 0.0)
