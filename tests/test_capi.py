@@ -63,6 +63,19 @@ def test_error_paths_without_device():
     assert lib.vd_axpby(0, 10, 1.0, None, 1.0, None, None) == -1
     assert lib.vd_convert(0, None, 1, None, 10, 0, None) == -1
     assert lib.vd_last_error()  # a message is set
+    # observation-operator construction: grid and query arguments
+    import numpy as np
+    xyz = np.zeros((4, 3))
+    g = C.c_void_p()
+    assert lib.vd_obs_grid_create(None, 4, 10.0, C.byref(g)) == -1
+    assert lib.vd_obs_grid_create(xyz.ctypes.data, 0, 10.0, C.byref(g)) == -1
+    assert lib.vd_obs_grid_create(xyz.ctypes.data, 4, float("nan"), C.byref(g)) == -1
+    assert b"dmax" in lib.vd_last_error()
+    assert lib.vd_obs_grid_create(xyz.ctypes.data, 4, -1.0, C.byref(g)) == -1
+    bad = np.array([[0.0, 0.0, np.inf]] * 4)
+    assert lib.vd_obs_grid_create(bad.ctypes.data, 4, 10.0, C.byref(g)) == -1
+    assert b"finite" in lib.vd_last_error()
+    assert lib.vd_obs_grid_query(None, null, 1, 4, null, null, None, None) == -1
     # parameter validation of plan creation
     p = _lib.QgParams()
     p.ny, p.nx, p.precision, p.time_scheme = 64, 64, 7, 1
