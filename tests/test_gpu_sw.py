@@ -20,6 +20,7 @@ import pytest
 import torch
 
 from oracle.sw import SwOracle
+from vardyn_amd._lib import VardynError
 from vardyn_amd.sw import SW
 
 pytestmark = pytest.mark.gpu
@@ -126,6 +127,9 @@ def test_sponge_coefficient_changes_nothing():
     torch.cuda.synchronize()
     for p_, q_ in zip(a, b):
         assert torch.equal(p_, q_)
+    sp.sponge_u = np.zeros((1, 1, model.nx + 1, model.ny))
+    with pytest.raises(VardynError):  # a sponge this model does not apply is refused
+        sp.sponge_h = np.ones((1, 1, model.nx, model.ny))
 
 
 def test_rest_state_and_ensemble_axis():

@@ -80,6 +80,9 @@ class SW:
         # sets to zero (sw.py:374-377) and nothing in the model changes: any
         # sponge_coef leaves the steps unchanged
         self.sponge_coef = float(p.get("sponge_coef", 0.0) or 0.0)
+        self.sponge_u = np.zeros((1, 1, self.nx + 1, self.ny))
+        self.sponge_v = np.zeros((1, 1, self.nx, self.ny + 1))
+        self.sponge_h = np.zeros((1, 1, self.nx, self.ny))
         # add_diffusion / add_h_diffusion (sw.py:511-565): only a positive
         # coefficient diffuses
         self.visc_coef = float(p.get("visc_coef", 0.0) or 0.0)
@@ -114,6 +117,25 @@ class SW:
         self.h_shape = (self.nl, self.nx, self.ny)
         m = _host_masks(self)
         self.mask_u, self.mask_v, self.mask_h = m
+
+    # The reference's sponge fields (sw.py:375-377): zero after construction,
+    # which is all this model supports; a caller that sets non-zero fields
+    # with a non-zero sponge_coef (Model_qgsw's boundary sponge,
+    # mod.py:2694-2760) gets an error instead of a silently ignored sponge.
+    def _sponge_get(name):
+        return property(lambda self: getattr(self, "_" + name, None),
+                        lambda self, val: self._sponge_set(name, val))
+
+    sponge_u = _sponge_get("sponge_u")
+    sponge_v = _sponge_get("sponge_v")
+    sponge_h = _sponge_get("sponge_h")
+    del _sponge_get
+
+    def _sponge_set(self, name, val):
+        a = np.asarray(val, dtype=np.float64)
+        if self.sponge_coef != 0.0 and np.any(a != 0):
+            raise VardynError("sw: non-zero sponge fields are not implemented")
+        setattr(self, "_" + name, a)
 
     def _wind(self, p):
         """add_wind_forcing (sw.py:567-642): the top-layer tendencies
