@@ -8,7 +8,8 @@ and without preconditioning; repeated evaluations bitwise equal.  The case:
 the 3-layer beta-plane configuration on a 40 x 36 grid over 240 steps, 6
 nadir tracks, checkpoints and basis times every 60 steps, a 3-scale wavelet
 forcing control, evaluated at half the truth's coefficients (a flowing,
-forced state).
+forced state); and the same with Laplacian viscosity and thickness diffusion
+(add_diffusion / add_h_diffusion, typical coefficients).
 """
 import numpy as np
 import pytest
@@ -26,7 +27,7 @@ pytestmark = pytest.mark.gpu
 def oracle_for(pb, prec=False):
     c = pb.case
     m = pb.model
-    sw = SwOracle(m.nx, m.ny, m.dx, m.dy, m.H, m.g_prime, m.f, m.dt)
+    sw = SwOracle(m.nx, m.ny, m.dx, m.dy, m.H, m.g_prime, m.f, m.dt, visc_coef=m.visc_coef, diff_coef=m.diff_coef)
     lon, lat = pb.grid.lon.T, pb.grid.lat.T
     H = ObsopOracle(lon, lat, np.full(lon.shape, c.dx), np.full(lon.shape, c.dx), pb.T, c.dt, pb.dict_obs, Npix=4)
     kw = basis_kwargs(c)
@@ -35,9 +36,10 @@ def oracle_for(pb, prec=False):
     return SwCostOracle(sw, H, basis, pb.checkpoints, pb.T, c.checkpoint_steps, pb.x0, sigma_b=pb.Q, prec=prec)
 
 
-@pytest.mark.parametrize("prec", [False, True])
-def test_cost_and_gradient_against_oracle(prec):
-    pb = build_sw_problem(small_case(), prec=prec)
+@pytest.mark.parametrize("prec,diffusion", [(False, False), (True, False), (False, True)])
+def test_cost_and_gradient_against_oracle(prec, diffusion):
+    kw = dict(visc_coef=150.0, diff_coef=100.0) if diffusion else {}
+    pb = build_sw_problem(small_case(**kw), prec=prec)
     orc = oracle_for(pb, prec=prec)
     assert orc.basis.nbasis == pb.basis.nbasis
     X0 = 0.5 * pb.X_true / (pb.Q if prec else 1.0)

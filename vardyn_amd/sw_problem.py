@@ -59,6 +59,8 @@ class SwCase:
     eddy_scale_m: float = 50e3
     forcing_rms: float = 0.05       # m/day
     seed: int = 2024
+    visc_coef: float = 0.0          # Laplacian viscosity of u, v (m^2/s; the reference's default 0)
+    diff_coef: float = 0.0          # Laplacian diffusion of h (m^2/s)
 
     @property
     def n_steps(self):
@@ -83,7 +85,8 @@ def sw_params(case: SwCase):
     f = np.broadcast_to(case.f0 + case.beta * y, (nx + 1, ny + 1)).copy()
     nl = len(case.H)
     return dict(nx=nx, ny=ny, nl=nl, dx=case.dx, dy=case.dx, H=np.asarray(case.H).reshape(nl, 1, 1),
-                g_prime=np.asarray(case.g_prime), f=f, dt=case.dt, slip_coef=0.0, bottom_drag_coef=0.0)
+                g_prime=np.asarray(case.g_prime), f=f, dt=case.dt, slip_coef=0.0, bottom_drag_coef=0.0,
+                visc_coef=case.visc_coef, diff_coef=case.diff_coef)
 
 
 def eddy_state(case: SwCase, param):
