@@ -128,3 +128,55 @@ def test_round_trip_of_a_synthetic_observation_set():
     for t in want:
         for a, b in zip(got[t], want[t]):
             assert np.array_equal(a, b)
+
+
+def test_read_netcdf_classic_files(tmp_path):
+    """read_obs_files (the file selection and CF decoding in front of
+    _obs_alti, obs.py:120-147): two netCDF classic files of one nadir track
+    written out of time order, SSH packed as int16 with scale_factor,
+    add_offset and _FillValue, times as "hours since"; the decoded dataset
+    equals the arrays it was written from (fill values NaN, files ordered by
+    their first time), and obs_alti gives the same result from it as from
+    those arrays."""
+    from scipy.io import netcdf_file
+    from vardyn_amd.obs import read_obs_files
+    rng = np.random.default_rng(4)
+    n = 40
+    hours = np.arange(2 * n, dtype=np.float64) * 0.25
+    lon = -60 + 0.05 * np.arange(2 * n)
+    lat = 30 + 0.04 * np.arange(2 * n)
+    packed = rng.integers(-2000, 2000, 2 * n).astype(np.int16)
+    packed[[3, 47]] = -32767
+    scale, offset = 0.001, 0.5
+    for k, sl in enumerate((slice(n, 2 * n), slice(0, n))):  # the later half is file 0
+        with netcdf_file(str(tmp_path / f"alti_{k}.nc"), "w") as nc:
+            nc.createDimension("time", n)
+            t = nc.createVariable("time", "f8", ("time",))
+            t[:] = hours[sl]
+            t.units = b"hours since 2012-10-01 00:00:00"
+            for name, vals in (("longitude", lon[sl]), ("latitude", lat[sl])):
+                v = nc.createVariable(name, "f8", ("time",))
+                v[:] = vals
+            v = nc.createVariable("sla", "i2", ("time",))
+            v[:] = packed[sl]
+            v.scale_factor = scale
+            v.add_offset = offset
+            v._FillValue = np.int16(-32767)
+    ds = read_obs_files(str(tmp_path / "alti_"), "time", "longitude", "latitude", ["sla"])
+    t_ref = np.datetime64("2012-10-01T00:00:00", "ns") + (hours * 3600e9).astype(np.int64).astype("timedelta64[ns]")
+    np.testing.assert_array_equal(ds["time"], t_ref)
+    np.testing.assert_array_equal(ds["longitude"], lon)
+    # scipy stores Python float attributes as float32; they are applied in FP64
+    sla = packed.astype(np.float64) * float(np.float32(scale)) + float(np.float32(offset))
+    sla[[3, 47]] = np.nan
+    np.testing.assert_array_equal(ds["sla"], sla)
+    dates = np.datetime64("2012-10-01T00:00", "ns") + np.arange(0, 20, 3) * np.timedelta64(3600, "s")
+    step = np.timedelta64(3 * 3600, "s")
+    a = obs_alti(ds["time"], ds["longitude"], ds["latitude"], ds["sla"], dates, step)
+    b = obs_alti(t_ref, lon, lat, sla, dates, step)
+    assert a.keys() == b.keys() and len(a) > 3
+    for d in a:
+        for x, y in zip(a[d][:3], b[d][:3]):
+            np.testing.assert_array_equal(x, y)
+    with pytest.raises(FileNotFoundError):
+        read_obs_files(str(tmp_path / "missing_"), "time", "longitude", "latitude", ["sla"])

@@ -6,8 +6,9 @@ of ``Obsop_interp_l3.process_obs`` (mapping/src/obsop.py:486-547),
 datlas-ocean/VarDyn.  The reference opens netCDF files with xarray, writes
 one netCDF file per assimilation date and lists their paths in ``dict_obs``;
 the L3 operator reads them back and merges the datasets of a date.  netCDF
-readers are not available here, so a dataset comes in as arrays (its time
-axis first) and the result is the in-memory dictionary
+readers of netCDF-4 files are not available here: a dataset comes in as arrays
+(its time axis first; ``read_obs_files`` reads netCDF classic files into
+them with scipy) and the result is the in-memory dictionary
 ``{date_seconds: (lon, lat, y, err)}`` that ``vardyn_amd.obsop`` takes --
 the same numbers, without the files in between.
 
@@ -50,6 +51,79 @@ from __future__ import annotations
 
 import numpy as np
 from scipy.interpolate import RegularGridInterpolator
+
+
+_TIME_UNITS = {"seconds": 1, "second": 1, "s": 1, "minutes": 60, "minute": 60, "hours": 3600, "hour": 3600,
+               "h": 3600, "days": 86400, "day": 86400, "d": 86400}
+
+
+def _cf_decode(var):
+    """CF decoding of a netCDF variable as xarray's default
+    ``decode_cf`` does it: ``_FillValue`` / ``missing_value`` to NaN, then
+    ``scale_factor`` and ``add_offset``; a ``units`` of "<unit> since <date>"
+    gives datetime64[ns] (the offsets rounded to whole nanoseconds).  Values
+    are unpacked in FP64 (xarray may unpack small integers with float32
+    packing attributes to float32: a difference at float32 rounding)."""
+    attrs = var._attributes
+    data = np.array(var[:], copy=True)
+    units = attrs.get("units", b"")
+    units = units.decode() if isinstance(units, bytes) else str(units)
+    out = data.astype(np.float64)
+    for key in ("_FillValue", "missing_value"):
+        if key in attrs:
+            fill = np.asarray(attrs[key]).ravel()
+            out[np.isin(data, fill)] = np.nan
+    if "scale_factor" in attrs:
+        out = out * float(np.asarray(attrs["scale_factor"]).ravel()[0])
+    if "add_offset" in attrs:
+        out = out + float(np.asarray(attrs["add_offset"]).ravel()[0])
+    if " since " in units:
+        unit, origin = units.split(" since ", 1)
+        scale = _TIME_UNITS.get(unit.strip().lower())
+        if scale is None:
+            raise ValueError(f"unsupported time unit {unit!r}")
+        base = np.datetime64(origin.strip().replace(" ", "T").rstrip("Z"), "ns")
+        ns = np.round(out * (scale * 1e9))
+        t = base + np.where(np.isfinite(ns), ns, 0).astype(np.int64).astype("timedelta64[ns]")
+        return np.where(np.isfinite(ns), t, np.datetime64("NaT"))
+    return out
+
+
+def read_obs_files(path, name_time, name_lon, name_lat, names):
+    """Read an observation dataset from netCDF classic (version 3) files, the
+    file selection of obs.py:120-147: ``path`` itself when it names one
+    ``.nc`` file, else the files matching ``path`` (a glob pattern) or
+    ``path*.nc``, combined along the time axis in the order of their first
+    time (xarray's ``open_mfdataset`` by coordinates).  Variables are
+    CF-decoded (``_cf_decode``); those without the time dimension come from
+    the first file.  Returns ``{name: array}`` for the time, lon, lat and the
+    ``names`` variables, ready for ``obs_alti``.  (The reference opens the
+    files with xarray, which also reads netCDF-4 / HDF5 files through the
+    netCDF4 library; neither is installed here, and scipy reads only the
+    classic format.)"""
+    import glob
+    from scipy.io import netcdf_file
+    if ".nc" in path and "*" not in path:
+        files = [path]
+    else:
+        files = sorted(glob.glob(path if "*" in path else f"{path}*.nc"))
+    if not files:
+        raise FileNotFoundError(f"no observation file matches {path!r}")
+    wanted = [name_time, name_lon, name_lat] + [n for n in names if n not in (name_time, name_lon, name_lat)]
+    parts = []
+    for f in files:
+        with netcdf_file(f, "r", mmap=False) as nc:
+            tdim = nc.variables[name_time].dimensions[0]
+            parts.append({n: (_cf_decode(nc.variables[n]), nc.variables[n].dimensions[:1] == (tdim,))
+                          for n in wanted})
+    parts.sort(key=lambda p: p[name_time][0][0] if p[name_time][0].size else np.datetime64("NaT"))
+    out = {}
+    for n in wanted:
+        if parts[0][n][1]:
+            out[n] = np.concatenate([p[n][0] for p in parts], axis=0)
+        else:
+            out[n] = parts[0][n][0]
+    return out
 
 
 def convert_lon(lon, lon_unit):
