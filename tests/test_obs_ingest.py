@@ -180,3 +180,24 @@ def test_read_netcdf_classic_files(tmp_path):
             np.testing.assert_array_equal(x, y)
     with pytest.raises(FileNotFoundError):
         read_obs_files(str(tmp_path / "missing_"), "time", "longitude", "latitude", ["sla"])
+
+
+def test_read_auxdata_classic_file(tmp_path):
+    """read_auxdata (tools.py:204-234) from a netCDF classic file: a linear
+    MDT on (lat, lon) axes with longitudes in 0..360 is reproduced exactly at
+    any point by the interpolator it returns, under the -180_180 convention."""
+    from scipy.io import netcdf_file
+    from vardyn_amd.obs import read_auxdata
+    lon = np.arange(290.0, 310.0, 0.5)
+    lat = np.arange(25.0, 40.0, 0.5)
+    mdt = 0.01 * (lon[None, :] - 300.0) + 0.02 * lat[:, None]
+    f = str(tmp_path / "mdt.nc")
+    with netcdf_file(f, "w") as nc:
+        nc.createDimension("lat", lat.size)
+        nc.createDimension("lon", lon.size)
+        nc.createVariable("lon", "f8", ("lon",))[:] = lon
+        nc.createVariable("lat", "f8", ("lat",))[:] = lat
+        nc.createVariable("mdt", "f8", ("lat", "lon"))[:] = mdt
+    fi = read_auxdata(f, {"lon": "lon", "lat": "lat", "mdt": "mdt"}, "-180_180")
+    pts = np.array([[-65.2, 31.3], [-50.75, 26.0], [-69.0, 39.2]])
+    np.testing.assert_allclose(fi(pts), 0.01 * (pts[:, 0] + 360.0 - 300.0) + 0.02 * pts[:, 1], rtol=1e-12)

@@ -155,6 +155,23 @@ def grid_interpolator(lon, lat, field, lon_unit="-180_180"):
                                    fill_value=None)
 
 
+def read_auxdata(file_aux, name_var, lon_unit="-180_180"):
+    """``read_auxdata`` (tools.py:204-234) of a netCDF classic file: the
+    field named by ``name_var['data']``, ``['mdt']`` or ``['var']`` (that
+    order) on the 1-D ``name_var['lon']`` / ``['lat']`` axes, CF-decoded, as
+    the ``grid_interpolator`` of it (the MDT and error fields of
+    ``obs_alti``)."""
+    from scipy.io import netcdf_file
+    with netcdf_file(file_aux, "r", mmap=False) as nc:
+        key = next(k for k in ("data", "mdt", "var") if k in name_var)
+        lon = _cf_decode(nc.variables[name_var["lon"]])
+        lat = _cf_decode(nc.variables[name_var["lat"]])
+        data = _cf_decode(nc.variables[name_var[key]])
+    if lon.ndim != 1 or lat.ndim != 1:
+        raise ValueError("read_auxdata: 2-D coordinates (the reference's LinearNDInterpolator branch) are not read")
+    return grid_interpolator(lon, lat, data, lon_unit)
+
+
 def _coarsen_mean(a, s):
     """xarray ``coarsen({dim: s for dim in dims}, boundary='trim').mean()``."""
     a = np.asarray(a, dtype=np.float64)
