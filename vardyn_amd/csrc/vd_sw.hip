@@ -465,10 +465,27 @@ __device__ __forceinline__ double lap5_adj(double m0, double xm, double xp, doub
   return ((xm + xp) + (fold_x - 2.0) * m0) / (g.dx * g.dx) + ((ym + yp) + (fold_y - 2.0) * m0) / (g.dy * g.dy);
 }
 
-// lam_out (u, v, h) += F'(x)^T mu, given gather1's planes.
+// The stage combination that follows an adjoint RHS evaluation in the
+// reversed RK3 step, y_k = a_k y_k + b_k L (a_k = 0: y_k = b_k L) over the
+// whole state, applied by k_sw_adj_gather2 to each value of L it finishes
+// (the arrays y_k are not read by the evaluation itself).
+struct StageComb {
+  double* y[2];
+  double a[2], b[2];
+  __device__ __forceinline__ void apply(size_t e, double L) const {
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+      if (y[k]) y[k][e] = a[k] == 0.0 ? b[k] * L : a[k] * y[k][e] + b[k] * L;
+  }
+};
+
+// lam_out (u, v, h) += F'(x)^T mu, given gather1's planes; then the stage
+// combination cb of the new values.
 __global__ __launch_bounds__(256) void k_sw_adj_gather2(SwGrid g, const double* xu, const double* xv,
                                                         const double* lu, const double* lv, const double* lh,
-                                                        AdjWs ws, double* ou, double* ov, double* oh) {
+                                                        AdjWs ws, double* ou, double* ov, double* oh,
+                                                        StageComb cb) {
+  const size_t nu = (size_t)g.nl * (g.nx + 1) * g.ny, nv = (size_t)g.nl * g.nx * (g.ny + 1);
   const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y * blockDim.y + threadIdx.y;
   const int l = blockIdx.z, nl = g.nl, W = g.ny + 1;
   if (i > g.nx || j > g.ny) return;
@@ -486,7 +503,9 @@ __global__ __launch_bounds__(256) void k_sw_adj_gather2(SwGrid g, const double* 
       a += g.visc * lap5_adj(mu_u(g, lu, l, i, j), mu_u(g, lu, l, i - 1, j), mu_u(g, lu, l, i + 1, j),
                              mu_u(g, lu, l, i, j - 1), mu_u(g, lu, l, i, j + 1), 0,
                              (j == 0) + (j == g.ny - 1), g);
-    ou[iu_(g, l, i, j)] += a;
+    const double L = ou[iu_(g, l, i, j)] + a;
+    ou[iu_(g, l, i, j)] = L;
+    cb.apply(iu_(g, l, i, j), L);
   }
   if (i < g.nx) {  // v point (i, j)
     double aV = P(kAVf, i, j) + 0.25 * ((j >= 1 ? P(kAVm, i, j - 1) + P(kAVm, i + 1, j - 1) : 0.0) +
@@ -500,7 +519,9 @@ __global__ __launch_bounds__(256) void k_sw_adj_gather2(SwGrid g, const double* 
       a += g.visc * lap5_adj(mu_v(g, lv, l, i, j), mu_v(g, lv, l, i - 1, j), mu_v(g, lv, l, i + 1, j),
                              mu_v(g, lv, l, i, j - 1), mu_v(g, lv, l, i, j + 1), (i == 0) + (i == g.nx - 1), 0,
                              g);
-    ov[iv_(g, l, i, j)] += a;
+    const double L = ov[iv_(g, l, i, j)] + a;
+    ov[iv_(g, l, i, j)] = L;
+    cb.apply(nu + iv_(g, l, i, j), L);
   }
   if (i < g.nx && j < g.ny) {  // h cell (i, j); ((kappa lap(h / area)) area) mask_h
     double a = P(kAhp, i, j);
@@ -508,7 +529,9 @@ __global__ __launch_bounds__(256) void k_sw_adj_gather2(SwGrid g, const double* 
       a += g.diff * lap5_adj(mu_h(g, lh, l, i, j), mu_h(g, lh, l, i - 1, j), mu_h(g, lh, l, i + 1, j),
                              mu_h(g, lh, l, i, j - 1), mu_h(g, lh, l, i, j + 1), (i == 0) + (i == g.nx - 1),
                              (j == 0) + (j == g.ny - 1), g);
-    oh[ih_(g, l, i, j)] += a;
+    const double L = oh[ih_(g, l, i, j)] + a;
+    oh[ih_(g, l, i, j)] = L;
+    cb.apply(nu + nv + ih_(g, l, i, j), L);
   }
 }
 
@@ -745,7 +768,8 @@ void scale3(const vd_sw_model* m, const double* iu, const double* iv, const doub
 }
 
 // F'(x)^T mu added to L, x a scaled state, mu (three fields) at mu_base.
-int sw_rhs_adj(vd_sw_model* m, const double* x, const double* mu, hipStream_t s) {
+int sw_rhs_adj(vd_sw_model* m, const double* x, const double* mu, hipStream_t s,
+               StageComb cb = StageComb{{nullptr, nullptr}, {0, 0}, {0, 0}}) {
   const SwGrid& g = m->g;
   const St xs = st_of(m, x, nullptr);
   hipLaunchKernelGGL(k_sw_diag<double>, grd(g, g.nl), blk(), 0, s, g, xs, FldOut{m->om, nullptr},
@@ -755,7 +779,7 @@ int sw_rhs_adj(vd_sw_model* m, const double* x, const double* mu, hipStream_t s)
   hipLaunchKernelGGL(k_sw_adj_face, grd(g, g.nl), blk(), 0, s, g, x, x + m->nu, m->om, m->ht, mu_u, mu_v, mu_h, ws);
   hipLaunchKernelGGL(k_sw_adj_gather1, grd(g, g.nl), blk(), 0, s, g, x + m->nu + m->nv, mu_u, mu_v, ws);
   hipLaunchKernelGGL(k_sw_adj_gather2, grd(g, g.nl), blk(), 0, s, g, x, x + m->nu, mu_u, mu_v, mu_h, ws, m->L,
-                     m->L + m->nu, m->L + m->nu + m->nv);
+                     m->L + m->nu, m->L + m->nu + m->nv, cb);
   VD_HIP_TRY(hipGetLastError());
   return VD_OK;
 }
@@ -787,12 +811,12 @@ int sw_rk3_adj_states(vd_sw_model* m, const double* x0, const double* x1, const 
   int rc;
   // x3 = x2 + dt/12 (8 k2 - k1 - k0)
   comb(m, m->L, m->LK2, 0.0, (dt / 12.0) * 8.0, m->LK1, 0.0, -(dt / 12.0), m->LK0, 0.0, -(dt / 12.0), s);
-  if ((rc = sw_rhs_adj(m, x2, m->LK2, s))) return rc;
-  // x2 = x1 + dt/4 (k1 - 3 k0)
-  comb(m, m->L, m->LK1, 1.0, dt / 4.0, m->LK0, 1.0, -3.0 * (dt / 4.0), nullptr, 0, 0, s);
-  if ((rc = sw_rhs_adj(m, x1, m->LK1, s))) return rc;
+  // x2 = x1 + dt/4 (k1 - 3 k0): its combination fused into the evaluation's last kernel
+  if ((rc = sw_rhs_adj(m, x2, m->LK2, s,
+                       StageComb{{m->LK1, m->LK0}, {1.0, 1.0}, {dt / 4.0, -3.0 * (dt / 4.0)}})))
+    return rc;
   // x1 = x0 + dt k0
-  comb(m, m->L, m->LK0, 1.0, dt, nullptr, 0, 0, nullptr, 0, 0, s);
+  if ((rc = sw_rhs_adj(m, x1, m->LK1, s, StageComb{{m->LK0, nullptr}, {1.0, 0.0}, {dt, 0.0}}))) return rc;
   return sw_rhs_adj(m, x0, m->LK0, s);
 }
 
