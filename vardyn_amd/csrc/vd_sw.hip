@@ -725,10 +725,13 @@ dim3 grd(const SwGrid& g, int nz) { return dim3((g.ny + 1 + kSwBX - 1) / kSwBX, 
 // One RK3-SSP step x <- step(x) on buffer x (value, tangent when TAN),
 // through the stage buffers B, C.
 template <bool TAN>
-int sw_rk3(vd_sw_model* m, double* xv, double* xt, hipStream_t s) {
+int sw_rk3(vd_sw_model* m, double* xv, double* xt, hipStream_t s, double* x1 = nullptr, double* x2 = nullptr,
+           double* x3 = nullptr) {
   typedef typename std::conditional<TAN, D1, double>::type S;
   const SwGrid& g = m->g;
-  double* bv[3] = {xv, m->SB, m->SC};
+  // value buffers of the stage states: x1, x2 (default SB, SC) and the
+  // step's result x3 (default: over x0); x0 is left intact when x3 is given
+  double* bv[4] = {xv, x1 ? x1 : m->SB, x2 ? x2 : m->SC, x3 ? x3 : xv};
   double* bt[3] = {xt, TAN ? m->TB : nullptr, TAN ? m->TC : nullptr};
   const St k0 = st_of(m, m->K0, TAN ? m->TK0 : nullptr), k1 = st_of(m, m->K1, TAN ? m->TK1 : nullptr);
   const StOut k0o = sto_of(m, m->K0, TAN ? m->TK0 : nullptr), k1o = sto_of(m, m->K1, TAN ? m->TK1 : nullptr);
@@ -738,7 +741,7 @@ int sw_rk3(vd_sw_model* m, double* xv, double* xt, hipStream_t s) {
       kepi{m->kep, TAN ? m->tkep : nullptr};
   for (int stg = 0; stg < 3; ++stg) {
     const St x = st_of(m, bv[stg], bt[stg]);
-    const StOut xn = sto_of(m, bv[(stg + 1) % 3], bt[(stg + 1) % 3]);
+    const StOut xn = sto_of(m, bv[stg + 1], bt[(stg + 1) % 3]);
     hipLaunchKernelGGL(k_sw_diag<S>, grd(g, g.nl), blk(), 0, s, g, x, omo, hto, kepo);
     if (stg == 0) hipLaunchKernelGGL((k_sw_tend<S, 0>), grd(g, g.nl), blk(), 0, s, g, x, omi, hti, kepi, k0, k1, k0o, xn);
     else if (stg == 1)
@@ -1105,6 +1108,21 @@ int vd_sw_step_keep(vd_sw_model* m, const double* u0, const double* v0, const do
   hipStream_t s = static_cast<hipStream_t>(stream);
   const size_t ns = m->ns;
   const bool forced = load_forcing(m, Fu, Fv, Fh, m->F, s);
+  if (stride == 0 && nstep > 0) {
+    // every step's start and stage states: the stages write them straight
+    // into the tape (slots 3k, 3k+1, 3k+2), the last step's result into SA
+    scale3(m, u0, v0, h0, keep, keep + m->nu, keep + m->nu + m->nv, 0, s);
+    for (int k = 0; k < nstep; ++k) {
+      double* x0 = keep + (size_t)3 * k * ns;
+      double* x3 = k + 1 < nstep ? keep + (size_t)3 * (k + 1) * ns : m->SA;
+      int rc = sw_rk3<false>(m, x0, nullptr, s, x0 + ns, x0 + 2 * ns, x3);
+      if (rc) return rc;
+      if (forced) add_forcing(m, x3, m->F, s);
+    }
+    scale3(m, m->SA, m->SA + m->nu, m->SA + m->nu + m->nv, u1, v1, h1, 1, s);
+    VD_HIP_TRY(hipGetLastError());
+    return VD_OK;
+  }
   scale3(m, u0, v0, h0, m->SA, m->SA + m->nu, m->SA + m->nu + m->nv, 0, s);
   for (int k = 0; k < nstep; ++k) {
     if (stride == 0)
