@@ -537,13 +537,14 @@ __global__ __launch_bounds__(256) void k_sw_adj_gather2(SwGrid g, const double* 
 
 // y_k = a_k y_k + b_k x (a_k = 0: y_k = b_k x) for up to three outputs.
 __global__ void k_sw_comb(int n, const double* x, double* y0, double a0, double b0, double* y1, double a1, double b1,
-                          double* y2, double a2, double b2) {
+                          double* y2, double a2, double b2, double* y3, double a3, double b3) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
   const double xv = x[e];
   if (y0) y0[e] = a0 == 0.0 ? b0 * xv : a0 * y0[e] + b0 * xv;
   if (y1) y1[e] = a1 == 0.0 ? b1 * xv : a1 * y1[e] + b1 * xv;
   if (y2) y2[e] = a2 == 0.0 ? b2 * xv : a2 * y2[e] + b2 * xv;
+  if (y3) y3[e] = a3 == 0.0 ? b3 * xv : a3 * y3[e] + b3 * xv;
 }
 
 // ------------------------------------------------------------ host: masks
@@ -788,9 +789,10 @@ int sw_rhs_adj(vd_sw_model* m, const double* x, const double* mu, hipStream_t s,
 }
 
 void comb(const vd_sw_model* m, const double* x, double* y0, double a0, double b0, double* y1, double a1, double b1,
-          double* y2, double a2, double b2, hipStream_t s) {
+          double* y2, double a2, double b2, hipStream_t s, double* y3 = nullptr, double a3 = 0, double b3 = 0) {
   const int n = (int)m->ns;
-  hipLaunchKernelGGL(k_sw_comb, dim3((n + 255) / 256), dim3(256), 0, s, n, x, y0, a0, b0, y1, a1, b1, y2, a2, b2);
+  hipLaunchKernelGGL(k_sw_comb, dim3((n + 255) / 256), dim3(256), 0, s, n, x, y0, a0, b0, y1, a1, b1, y2, a2, b2,
+                     y3, a3, b3);
 }
 
 // Scaled forcing (set_input_uvh of Fu, Fv, Fh; a missing field is zero)
@@ -809,11 +811,15 @@ void add_forcing(const vd_sw_model* m, double* x, const double* F, hipStream_t s
 
 // L <- (one RK3 step at x0)^T L
 // L <- (one RK3 step at x0)^T L with the stage states x1, x2 given.
-int sw_rk3_adj_states(vd_sw_model* m, const double* x0, const double* x1, const double* x2, hipStream_t s) {
+// With aF, the forcing cotangent aF += dt L of the step's result (x3 + dt F)
+// is collected in the same launch as the first stage combination.
+int sw_rk3_adj_states(vd_sw_model* m, const double* x0, const double* x1, const double* x2, hipStream_t s,
+                      double* aF = nullptr) {
   const double dt = m->g.dt;
   int rc;
   // x3 = x2 + dt/12 (8 k2 - k1 - k0)
-  comb(m, m->L, m->LK2, 0.0, (dt / 12.0) * 8.0, m->LK1, 0.0, -(dt / 12.0), m->LK0, 0.0, -(dt / 12.0), s);
+  comb(m, m->L, m->LK2, 0.0, (dt / 12.0) * 8.0, m->LK1, 0.0, -(dt / 12.0), m->LK0, 0.0, -(dt / 12.0), s, aF, 1.0,
+       dt);
   // x2 = x1 + dt/4 (k1 - 3 k0): its combination fused into the evaluation's last kernel
   if ((rc = sw_rhs_adj(m, x2, m->LK2, s,
                        StageComb{{m->LK1, m->LK0}, {1.0, 1.0}, {dt / 4.0, -3.0 * (dt / 4.0)}})))
@@ -1013,9 +1019,8 @@ static int sw_reverse(vd_sw_model* m, bool forced, const double* ck, int seg, in
   if (seg == 0) {
     // every step's start state and its two RK3 stage states were kept
     for (int k = nstep - 1; k >= 0; --k) {
-      if (want_aF) add_forcing(m, m->aF, m->L, s);
       const double* xk = ck + (size_t)3 * k * ns;
-      int rc = sw_rk3_adj_states(m, xk, xk + ns, xk + 2 * ns, s);
+      int rc = sw_rk3_adj_states(m, xk, xk + ns, xk + 2 * ns, s, want_aF ? m->aF : nullptr);
       if (rc) return rc;
     }
     return VD_OK;
